@@ -1,0 +1,178 @@
+// Host-side infrastructure of libgpu_presolver_amd: HIP error handling, the
+// per-thread execution context (stream, flag words, pinned staging) and
+// stream-ordered device buffers.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <cstring>
+#include <initializer_list>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gp_kernels.h"
+#include "gpu_presolver/presolve/presolve_structs.hpp"
+
+namespace gpu_presolver::amd {
+
+// std::runtime_error("<context>: <hip error string>"), the reference's
+// error convention (presolve_helpers.hpp:13-19).
+inline void check(hipError_t status, const char* context) {
+  if (status == hipSuccess) return;
+  throw std::runtime_error(std::string(context) + ": " + hipGetErrorString(status));
+}
+
+// Execution context: one stream, a small block of device flag words read
+// back through pinned memory, and a device-memory pool that keeps freed
+// blocks cached so per-rule scratch allocation stays cheap.
+class Ctx {
+ public:
+  static constexpr int kFlags = 16;
+  explicit Ctx(hipStream_t stream = nullptr, bool own_stream = true);
+  ~Ctx();
+  Ctx(const Ctx&) = delete;
+  Ctx& operator=(const Ctx&) = delete;
+
+  hipStream_t stream() const { return stream_; }
+  // Zeroes the device flag words and returns them.
+  int32_t* flags();
+  // Copies the first `count` flag words to the host (synchronizes the stream).
+  const int32_t* read_flags(int count = kFlags);
+  void sync(const char* context = "stream synchronize");
+
+  template <class T>
+  std::vector<T> to_host(const T* dev, int64_t count, const char* context) {
+    std::vector<T> out(static_cast<size_t>(count > 0 ? count : 0));
+    if (count > 0) {
+      check(hipMemcpyAsync(out.data(), dev, sizeof(T) * static_cast<size_t>(count), hipMemcpyDeviceToHost, stream_),
+            context);
+      sync(context);
+    }
+    return out;
+  }
+  template <class T>
+  T scalar_to_host(const T* dev, const char* context) {
+    check(hipMemcpyAsync(pinned_, dev, sizeof(T), hipMemcpyDeviceToHost, stream_), context);
+    sync(context);
+    T v;
+    std::memcpy(&v, pinned_, sizeof(T));
+    return v;
+  }
+  template <class T>
+  void to_device(T* dev, const T* host, int64_t count, const char* context) {
+    if (count <= 0) return;
+    check(hipMemcpyAsync(dev, host, sizeof(T) * static_cast<size_t>(count), hipMemcpyHostToDevice, stream_), context);
+  }
+
+ private:
+  hipStream_t stream_ = nullptr;
+  bool own_stream_ = false;
+  int32_t* d_flags_ = nullptr;
+  int32_t* h_flags_ = nullptr;
+  void* pinned_ = nullptr;
+};
+
+// Context of the calling thread.  ScopedCtx installs another one (the
+// serving and batched drivers run each job on its own stream).
+Ctx& ctx();
+class ScopedCtx {
+ public:
+  explicit ScopedCtx(Ctx& c);
+  ~ScopedCtx();
+
+ private:
+  Ctx* prev_;
+};
+
+void* dev_alloc(size_t bytes, hipStream_t stream, const char* context);
+void dev_free(void* p, hipStream_t stream) noexcept;
+
+// Stream-ordered device buffer (hipMallocAsync / hipFreeAsync).
+template <class T>
+class DBuf {
+ public:
+  DBuf() = default;
+  explicit DBuf(int64_t count, const char* context = "device buffer") { alloc(count, context); }
+  ~DBuf() { reset(); }
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p_(o.p_), n_(o.n_), s_(o.s_) { o.p_ = nullptr; o.n_ = 0; }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      p_ = o.p_; n_ = o.n_; s_ = o.s_;
+      o.p_ = nullptr; o.n_ = 0;
+    }
+    return *this;
+  }
+  void alloc(int64_t count, const char* context = "device buffer") {
+    reset();
+    s_ = ctx().stream();
+    n_ = count;
+    const size_t bytes = sizeof(T) * static_cast<size_t>(count > 0 ? count : 1);
+    p_ = static_cast<T*>(dev_alloc(bytes, s_, context));
+  }
+  void reset() noexcept {
+    if (p_) dev_free(p_, s_);
+    p_ = nullptr;
+    n_ = 0;
+  }
+  // Gives up ownership (the pointer is then released with dev_free / hipFree).
+  T* release() noexcept {
+    T* p = p_;
+    p_ = nullptr;
+    n_ = 0;
+    return p;
+  }
+  T* get() const { return p_; }
+  operator T*() const { return p_; }
+  int64_t size() const { return n_; }
+
+ private:
+  T* p_ = nullptr;
+  int64_t n_ = 0;
+  hipStream_t s_ = nullptr;
+};
+
+// ---- views -----------------------------------------------------------------
+inline gpk_csr csr_view(const presolve::DeviceCsrMatrix& a) {
+  return gpk_csr{a.rows, a.cols, a.nnz, a.rowPtr, a.colVal, a.nzVal};
+}
+inline gpk_plan plan_view(const presolve::PresolvePlanGpu& p) {
+  return gpk_plan{p.keep_row_mask, p.keep_col_mask, p.new_c, p.new_l, p.new_u, p.new_AL, p.new_AU};
+}
+
+// ---- shared host helpers (host_common.cpp) ---------------------------------
+// Exclusive scan of a uint8 mask into out[0..n]; returns the count.
+int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out);
+// Exclusive scan of int32 counts into out[0..n]; returns the total.
+int32_t scan_counts(const int32_t* in, int32_t n, int32_t* out);
+// Sum, in ascending index order, of vals[i] over mask[i] != 0 (computed on
+// the host from the packed selection, so the rounding order is fixed).
+double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n);
+// Compacts A by row/column keep maps; the result owns fresh device arrays.
+presolve::DeviceCsrMatrix compact_csr(const presolve::DeviceCsrMatrix& A, const int32_t* row_map,
+                                      const int32_t* col_map, int32_t m_new, int32_t n_new);
+// Stable transpose (rows of the result sorted); fresh device arrays.
+presolve::DeviceCsrMatrix transpose_csr(const presolve::DeviceCsrMatrix& A);
+void free_csr(presolve::DeviceCsrMatrix& A) noexcept;
+// Sorts the rows of A in place by column when some row is unsorted.
+void sort_rows_if_needed(const presolve::DeviceCsrMatrix& A);
+// old_to_new map of a keep mask; returns the kept count.
+int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map);
+// FixedCol records and the long-double objective delta
+// (fixed_col_tape.cu:107-267).
+double fixed_col_tape(presolve::PostsolveTape* tape, const uint8_t* fixed_mask, const uint8_t* excluded,
+                      const double* fixed_val, const uint8_t* keep_row, const double* c,
+                      const presolve::DeviceCsrMatrix& AT, int32_t n);
+void append_record(presolve::PostsolveTape& tape, presolve::PostsolveReductionType type,
+                   const int32_t* idx, size_t ni, const double* vals, size_t nv, presolve::PostsolveDualMode mode);
+inline void append_record(presolve::PostsolveTape& tape, presolve::PostsolveReductionType type,
+                          std::initializer_list<int32_t> idx, std::initializer_list<double> vals,
+                          presolve::PostsolveDualMode mode) {
+  append_record(tape, type, idx.begin(), idx.size(), vals.begin(), vals.size(), mode);
+}
+
+}  // namespace gpu_presolver::amd

@@ -1,0 +1,276 @@
+// Host-side infrastructure: context, device memory, scans, compaction,
+// transpose and the FixedCol tape packer.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstring>
+#include <limits>
+#include <mutex>
+
+#include "gp_host.hpp"
+
+namespace gpu_presolver::amd {
+
+using presolve::DeviceCsrMatrix;
+using presolve::PostsolveDualMode;
+using presolve::PostsolveReductionType;
+using presolve::PostsolveTape;
+
+namespace {
+
+// Keep freed blocks of the device's default pool cached for reuse: rules
+// allocate and release their scratch once per call.
+void configure_pool_once() {
+  static std::once_flag once;
+  std::call_once(once, []() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    hipMemPool_t pool = nullptr;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess || pool == nullptr) return;
+    uint64_t threshold = std::numeric_limits<uint64_t>::max();
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+  });
+}
+
+thread_local Ctx* t_current = nullptr;
+
+}  // namespace
+
+Ctx::Ctx(hipStream_t stream, bool own_stream) {
+  configure_pool_once();
+  if (stream == nullptr && own_stream) {
+    check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    own_stream_ = true;
+  } else {
+    stream_ = stream;
+    own_stream_ = false;
+  }
+  check(hipMalloc(&d_flags_, sizeof(int32_t) * kFlags), "hipMalloc flags");
+  check(hipHostMalloc(&h_flags_, sizeof(int32_t) * kFlags, hipHostMallocDefault), "hipHostMalloc flags");
+  check(hipHostMalloc(&pinned_, 64, hipHostMallocDefault), "hipHostMalloc staging");
+}
+
+Ctx::~Ctx() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  (void)hipFree(d_flags_);
+  (void)hipHostFree(h_flags_);
+  (void)hipHostFree(pinned_);
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+int32_t* Ctx::flags() {
+  check(hipMemsetAsync(d_flags_, 0, sizeof(int32_t) * kFlags, stream_), "hipMemset flags");
+  return d_flags_;
+}
+
+const int32_t* Ctx::read_flags(int count) {
+  check(hipMemcpyAsync(h_flags_, d_flags_, sizeof(int32_t) * count, hipMemcpyDeviceToHost, stream_), "read flags");
+  sync("read flags");
+  return h_flags_;
+}
+
+void Ctx::sync(const char* context) { check(hipStreamSynchronize(stream_), context); }
+
+Ctx& ctx() {
+  if (t_current) return *t_current;
+  thread_local Ctx* own = nullptr;
+  if (!own) own = new Ctx();  // lives for the thread
+  return *own;
+}
+
+ScopedCtx::ScopedCtx(Ctx& c) : prev_(t_current) { t_current = &c; }
+ScopedCtx::~ScopedCtx() { t_current = prev_; }
+
+void* dev_alloc(size_t bytes, hipStream_t stream, const char* context) {
+  void* p = nullptr;
+  check(hipMallocAsync(&p, bytes, stream), context);
+  return p;
+}
+
+void dev_free(void* p, hipStream_t stream) noexcept {
+  if (p) (void)hipFreeAsync(p, stream);
+}
+
+int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out) {
+  Ctx& c = ctx();
+  DBuf<int32_t> scratch(gpk_scan_scratch_ints(n), "scan scratch");
+  check(gpk_scan_mask_u8(mask, out, n, scratch, c.stream()), "scan mask");
+  return c.scalar_to_host(out + n, "scan total");
+}
+
+int32_t scan_counts(const int32_t* in, int32_t n, int32_t* out) {
+  Ctx& c = ctx();
+  DBuf<int32_t> scratch(gpk_scan_scratch_ints(n), "scan scratch");
+  check(gpk_scan_exclusive_i32(in, out, n, scratch, c.stream()), "scan counts");
+  return c.scalar_to_host(out + n, "scan total");
+}
+
+double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n) {
+  if (n <= 0) return 0.0;
+  Ctx& c = ctx();
+  DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1);
+  const int32_t k = scan_mask_count(mask, n, prefix);
+  if (k == 0) return 0.0;
+  DBuf<double> packed(k);
+  check(gpk_pack_f64(mask, vals, prefix, n, packed, c.stream()), "pack objective terms");
+  const std::vector<double> h = c.to_host(packed.get(), k, "objective terms");
+  double s = 0.0;
+  for (double v : h) s += v;
+  return s;
+}
+
+int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map) {
+  map.alloc(static_cast<int64_t>(n) + 1, "old_to_new");
+  if (n == 0) return 0;
+  DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1);
+  const int32_t count = scan_mask_count(mask, n, prefix);
+  check(gpk_old_to_new(mask, prefix, map, n, ctx().stream()), "old_to_new");
+  return count;
+}
+
+void free_csr(DeviceCsrMatrix& A) noexcept {
+  hipStream_t s = ctx().stream();
+  dev_free(A.rowPtr, s);
+  dev_free(A.colVal, s);
+  dev_free(A.nzVal, s);
+  A.rowPtr = nullptr;
+  A.colVal = nullptr;
+  A.nzVal = nullptr;
+}
+
+// compact_csr_by_masks_gpu (gpu_presolve.cu:1881-1978) in two passes: per
+// row kept counts, an exclusive scan, then a coalesced stream compaction.
+DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, const int32_t* col_map, int32_t m_new,
+                            int32_t n_new) {
+  Ctx& c = ctx();
+  DeviceCsrMatrix out;
+  out.rows = m_new;
+  out.cols = n_new;
+  DBuf<int32_t> ptr(static_cast<int64_t>(m_new) + 1, "compact rowPtr");
+  if (m_new == 0 || n_new == 0 || A.nnz == 0) {
+    check(hipMemsetAsync(ptr.get(), 0, sizeof(int32_t) * (static_cast<size_t>(m_new) + 1), c.stream()),
+          "compact empty rowPtr");
+    out.rowPtr = ptr.release();
+    return out;
+  }
+  const int32_t m_old = A.rows;
+  DBuf<int32_t> counts(static_cast<int64_t>(m_old) + 1, "compact counts");
+  check(gpk_compact_count(A.rowPtr, A.colVal, m_old, row_map, col_map, counts, c.stream()), "compact count");
+  const int32_t nnz = scan_counts(counts, m_old, counts);
+  DBuf<int32_t> idx(nnz, "compact colVal");
+  DBuf<double> val(nnz, "compact nzVal");
+  check(gpk_compact_copy(A.rowPtr, A.colVal, A.nzVal, m_old, row_map, col_map, counts, ptr, idx, val, m_new,
+                         c.stream()),
+        "compact copy");
+  out.nnz = nnz;
+  out.rowPtr = ptr.release();
+  out.colVal = idx.release();
+  out.nzVal = val.release();
+  return out;
+}
+
+// Stable transpose: LSD radix sort of entries (already in row-major order)
+// by column, which leaves every AT row sorted by row index.
+DeviceCsrMatrix transpose_csr(const DeviceCsrMatrix& A) {
+  Ctx& c = ctx();
+  hipStream_t s = c.stream();
+  DeviceCsrMatrix T;
+  T.rows = A.cols;
+  T.cols = A.rows;
+  T.nnz = A.nnz;
+  DBuf<int32_t> ptr(static_cast<int64_t>(A.cols) + 1, "transpose rowPtr");
+  DBuf<int32_t> counts(static_cast<int64_t>(A.cols) + 1, "transpose counts");
+  check(hipMemsetAsync(counts.get(), 0, sizeof(int32_t) * (static_cast<size_t>(A.cols) + 1), s), "memset counts");
+  if (A.nnz > 0) check(gpk_col_histogram(A.colVal, A.nnz, counts, s), "col histogram");
+  {
+    DBuf<int32_t> scratch(gpk_scan_scratch_ints(A.cols));
+    check(gpk_scan_exclusive_i32(counts, ptr, A.cols, scratch, s), "transpose scan");
+  }
+  if (A.nnz == 0) {
+    T.rowPtr = ptr.release();
+    return T;
+  }
+  const int32_t nnz = A.nnz;
+  int32_t bits = 1;
+  while (bits < 31 && (int64_t{1} << bits) < A.cols) ++bits;
+  DBuf<uint32_t> keys(nnz), keys_alt(nnz);
+  DBuf<int32_t> perm(nnz), perm_alt(nnz), row_of(nnz);
+  check(hipMemcpyAsync(keys.get(), A.colVal, sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, s), "transpose keys");
+  check(gpk_iota_i32(perm, nnz, s), "transpose iota");
+  check(gpk_expand_rows(A.rowPtr, A.rows, row_of, s), "expand rows");
+  DBuf<int32_t> scratch(gpk_radix_scratch_ints(nnz));
+  check(gpk_radix_sort_u32_i32(keys, perm, keys_alt, perm_alt, nnz, bits, scratch, s), "transpose sort");
+  DBuf<int32_t> idx(nnz, "transpose colVal");
+  DBuf<double> val(nnz, "transpose nzVal");
+  check(gpk_transpose_gather(perm, row_of, A.nzVal, nnz, idx, val, s), "transpose gather");
+  T.rowPtr = ptr.release();
+  T.colVal = idx.release();
+  T.nzVal = val.release();
+  return T;
+}
+
+void sort_rows_if_needed(const DeviceCsrMatrix& A) {
+  if (A.rows <= 0 || A.nnz <= 1) return;
+  Ctx& c = ctx();
+  int32_t* f = c.flags();
+  check(gpk_rows_unsorted(A.rowPtr, A.colVal, A.rows, f, c.stream()), "rows unsorted check");
+  if (c.read_flags(1)[0] == 0) return;
+  check(gpk_sort_rows_in_place(A.rowPtr, A.colVal, A.nzVal, A.rows, c.stream()), "sort rows");
+}
+
+void append_record(PostsolveTape& tape, PostsolveReductionType type, const int32_t* idx, size_t ni,
+                   const double* vals, size_t nv, PostsolveDualMode mode) {
+  tape.types.push_back(static_cast<int32_t>(type));
+  tape.indices.insert(tape.indices.end(), idx, idx + ni);
+  tape.vals.insert(tape.vals.end(), vals, vals + nv);
+  tape.index_starts.push_back(static_cast<int32_t>(tape.indices.size()));
+  tape.value_starts.push_back(static_cast<int32_t>(tape.vals.size()));
+  tape.dual_modes.push_back(static_cast<uint8_t>(mode));
+}
+
+double fixed_col_tape(PostsolveTape* tape, const uint8_t* fixed_mask, const uint8_t* excluded,
+                      const double* fixed_val, const uint8_t* keep_row, const double* cvec, const DeviceCsrMatrix& AT,
+                      int32_t n) {
+  if (n <= 0) return 0.0;
+  Ctx& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_csr at = csr_view(AT);
+  DBuf<uint8_t> sel(n);
+  DBuf<int32_t> sel_prefix(static_cast<int64_t>(n) + 1), nnz_prefix(static_cast<int64_t>(n) + 1);
+  check(gpk_fixed_col_counts(at, fixed_mask, excluded, keep_row, sel, nnz_prefix, s), "fixed col counts");
+  const int32_t records = scan_mask_count(sel, n, sel_prefix);
+  if (records == 0) return 0.0;
+  const int32_t active = scan_counts(nnz_prefix, n, nnz_prefix);
+  DBuf<int32_t> cols(records), rows(active), rstart(records);
+  DBuf<double> v(records), cc(records), coef(active);
+  check(gpk_fixed_col_pack(at, sel, sel_prefix, nnz_prefix, fixed_val, cvec, keep_row, cols, v, cc, rstart, rows, coef,
+                           s),
+        "fixed col pack");
+  const std::vector<double> hv = c.to_host(v.get(), records, "fixed col values");
+  const std::vector<double> hc = c.to_host(cc.get(), records, "fixed col costs");
+  long double obj = 0.0L;
+  for (int32_t r = 0; r < records; ++r)
+    obj += static_cast<long double>(hc[r]) * static_cast<long double>(hv[r]);
+  if (tape) {
+    const std::vector<int32_t> hcol = c.to_host(cols.get(), records, "fixed col ids");
+    const std::vector<int32_t> hrow = c.to_host(rows.get(), active, "fixed col rows");
+    const std::vector<double> hcoef = c.to_host(coef.get(), active, "fixed col coefs");
+    std::vector<int32_t> starts = c.to_host(rstart.get(), records, "fixed col starts");
+    starts.push_back(active);
+    std::vector<int32_t> idx;
+    std::vector<double> vals;
+    for (int32_t r = 0; r < records; ++r) {
+      idx.assign(1, hcol[r]);
+      vals.assign({hv[r], hc[r]});
+      for (int32_t q = starts[r]; q < starts[r + 1]; ++q) {
+        idx.push_back(hrow[q]);
+        vals.push_back(hcoef[q]);
+      }
+      append_record(*tape, PostsolveReductionType::FixedCol, idx.data(), idx.size(), vals.data(), vals.size(),
+                    PostsolveDualMode::Minimal);
+    }
+  }
+  return static_cast<double>(obj);
+}
+
+}  // namespace gpu_presolver::amd

@@ -1,0 +1,513 @@
+// Rule operator API (reference cpp/include/gpu_presolver/presolve/rules/*.hpp):
+// host orchestration of each reduction rule over the HIP kernels of
+// k_rules_rows.hip / k_rules_cols.hip.  Each apply_rule_* reads the plan and
+// the phase-entry LP exactly as the reference rule of the same name does and
+// leaves the same plan state, flags and tape records.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <vector>
+
+#include "gp_host.hpp"
+#include "gpu_presolver/presolve/fixed_col_tape.hpp"
+#include "gpu_presolver/presolve/gpu_presolve_kernels.hpp"
+#include "gpu_presolver/presolve/rules/rule_column_singletons.hpp"
+#include "gpu_presolver/presolve/rules/rule_doubleton_equations.hpp"
+#include "gpu_presolver/presolve/rules/rule_dual_fix.hpp"
+#include "gpu_presolver/presolve/rules/rule_duplicate_columns.hpp"
+#include "gpu_presolver/presolve/rules/rule_duplicate_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_empty_cols.hpp"
+#include "gpu_presolver/presolve/rules/rule_empty_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_implied_variable_bounds.hpp"
+#include "gpu_presolver/presolve/rules/rule_infeasible_fixed_variables.hpp"
+#include "gpu_presolver/presolve/rules/rule_infeasible_redundant_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_redundant_bounds.hpp"
+#include "gpu_presolver/presolve/rules/rule_singleton_rows.hpp"
+
+namespace gpu_presolver::presolve {
+
+using amd::check;
+using amd::ctx;
+using amd::DBuf;
+using amd::csr_view;
+using amd::plan_view;
+using amd::append_record;
+using RT = PostsolveReductionType;
+using DM = PostsolveDualMode;
+
+namespace {
+
+template <class T>
+void d2d(T* dst, const T* src, int64_t n, const char* context) {
+  if (n > 0)
+    check(hipMemcpyAsync(dst, src, sizeof(T) * static_cast<size_t>(n), hipMemcpyDeviceToDevice, ctx().stream()),
+          context);
+}
+
+template <class T>
+void zero(T* dst, int64_t n, const char* context) {
+  if (n > 0) check(hipMemsetAsync(dst, 0, sizeof(T) * static_cast<size_t>(n), ctx().stream()), context);
+}
+
+// Packs the indices i with mask[i] != 0 (ascending) and copies them to the host.
+std::vector<int32_t> masked_indices(const uint8_t* mask, int32_t n) {
+  if (n <= 0) return {};
+  DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1);
+  const int32_t k = amd::scan_mask_count(mask, n, prefix);
+  if (k == 0) return {};
+  DBuf<int32_t> out(k);
+  check(gpk_pack_index(mask, prefix, n, out, ctx().stream()), "pack indices");
+  return ctx().to_host(out.get(), k, "indices to host");
+}
+
+}  // namespace
+
+// gpu_presolve_kernels.cu:86-111
+void compute_row_activity_summary(double* row_min_fin, double* row_max_fin, std::int32_t* row_min_neg_inf_count,
+                                  std::int32_t* row_max_pos_inf_count, const DeviceCsrMatrix& A_csr, const double* l,
+                                  const double* u, double zero_tol) {
+  if (A_csr.rows == 0) return;
+  check(gpk_row_activity(A_csr.rowPtr, A_csr.colVal, A_csr.nzVal, A_csr.rows, l, u, zero_tol, row_min_fin,
+                         row_max_fin, row_min_neg_inf_count, row_max_pos_inf_count, ctx().stream()),
+        "row activity");
+}
+
+// gpu_presolve_kernels.cu:113-124
+void compute_col_max_abs(double* col_max_abs, const DeviceCsrMatrix& AT_csr) {
+  if (AT_csr.rows == 0) return;
+  check(gpk_col_max_abs(AT_csr.rowPtr, AT_csr.nzVal, AT_csr.rows, col_max_abs, ctx().stream()), "col max abs");
+}
+
+// fixed_col_tape.cu:107-267
+double append_compact_fixed_col_tape_from_device(PostsolveTape* tape, const std::uint8_t* fixed_mask,
+                                                 const std::uint8_t* excluded_mask, const double* fixed_val,
+                                                 const std::uint8_t* keep_row, const double* c,
+                                                 const DeviceCsrMatrix& AT, std::int32_t n, const char* context) {
+  (void)context;
+  return amd::fixed_col_tape(tape, fixed_mask, excluded_mask, fixed_val, keep_row, c, AT, n);
+}
+
+// ---------------------------------------------------------------------------
+// rule_empty_rows.cu:188-240
+void apply_rule_empty_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                           const PresolveParams& prm) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows;
+  if (m == 0) return;
+  auto& c = ctx();
+  int32_t* f = c.flags();
+  check(gpk_empty_rows(m, plan_view(plan), stats.row_nnz, prm.feasibility_tol, 0, f, c.stream()), "empty rows");
+  const int32_t* h = c.read_flags(2);
+  if (h[0]) {
+    plan.has_infeasible = true;
+    return;
+  }
+  if (h[1]) {
+    check(gpk_empty_rows(m, plan_view(plan), stats.row_nnz, prm.feasibility_tol, 1, f, c.stream()), "empty rows apply");
+    plan.has_row_action = true;
+    plan.has_change = true;
+  }
+}
+
+// rule_singleton_rows.cu:127-248
+void apply_rule_singleton_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                               const PresolveParams& prm) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  DBuf<double> cl(n), cu(n);
+  DBuf<uint8_t> remove(m);
+  d2d(cl.get(), plan.new_l, n, "singleton rows cand_l");
+  d2d(cu.get(), plan.new_u, n, "singleton rows cand_u");
+  const gpk_plan P = plan_view(plan);
+  check(gpk_singleton_rows_bounds(m, n, P, stats.singleton_row_mask, stats.singleton_row_col, stats.singleton_row_val,
+                                  prm.zero_tol, cl, cu, remove, s),
+        "singleton rows bounds");
+  int32_t* f = c.flags();
+  check(gpk_singleton_rows_check(m, n, P, cl, cu, remove, prm.bound_tol, f, s), "singleton rows check");
+  const int32_t* h = c.read_flags(3);
+  if (h[0]) {
+    plan.has_infeasible = true;
+    return;
+  }
+  const bool removed = h[1] != 0, changed = h[2] != 0;
+  if (!(removed || changed)) return;
+  if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu && removed) {
+    const std::vector<uint8_t> hr = c.to_host(remove.get(), m, "singleton tape rows");
+    const std::vector<int32_t> hc = c.to_host(stats.singleton_row_col, m, "singleton tape cols");
+    const std::vector<double> hv = c.to_host(stats.singleton_row_val, m, "singleton tape vals");
+    const std::vector<double> hal = c.to_host(plan.new_AL, m, "singleton tape AL");
+    const std::vector<double> hau = c.to_host(plan.new_AU, m, "singleton tape AU");
+    for (int32_t i = 0; i < m; ++i)
+      if (hr[i]) append_record(plan.tape, RT::DeletedRow, {i, hc[i]}, {hal[i], hau[i], hv[i]}, DM::Minimal);
+  }
+  check(gpk_singleton_rows_commit(m, n, P, cl, cu, remove, s), "singleton rows commit");
+  plan.has_row_action = true;
+  plan.has_change = true;
+}
+
+// rule_infeasible_redundant_rows.cu:119-243
+void apply_rule_infeasible_redundant_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                                          const PresolveParams& prm) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows;
+  if (m == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  DBuf<uint8_t> full(m), dlo(m), dhi(m);
+  const gpk_plan P = plan_view(plan);
+  int32_t* f = c.flags();
+  check(gpk_redundant_rows_classify(csr_view(lp.A), P, stats.row_nnz, prm.bound_tol, full, dlo, dhi, f, s),
+        "redundant rows classify");
+  const int32_t* h = c.read_flags(2);
+  if (h[0]) {
+    plan.has_infeasible = true;
+    return;
+  }
+  if (!h[1]) return;
+  if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
+    const std::vector<uint8_t> hf = c.to_host(full.get(), m, "redundant tape full");
+    const std::vector<uint8_t> hl = c.to_host(dlo.get(), m, "redundant tape lo");
+    const std::vector<uint8_t> hh = c.to_host(dhi.get(), m, "redundant tape hi");
+    const std::vector<double> hal = c.to_host(plan.new_AL, m, "redundant tape AL");
+    const std::vector<double> hau = c.to_host(plan.new_AU, m, "redundant tape AU");
+    const double inf = HUGE_VAL;
+    for (int32_t i = 0; i < m; ++i)
+      if (hf[i]) append_record(plan.tape, RT::DeletedRow, {i}, {hal[i], hau[i]}, DM::Minimal);
+    for (int32_t i = 0; i < m; ++i) {
+      if (hl[i]) append_record(plan.tape, RT::LhsChange, {i}, {hal[i], hau[i], -inf, hau[i]}, DM::Minimal);
+      if (hh[i]) append_record(plan.tape, RT::RhsChange, {i}, {hal[i], hau[i], hal[i], inf}, DM::Minimal);
+    }
+  }
+  check(gpk_redundant_rows_apply(m, P, full, dlo, dhi, s), "redundant rows apply");
+  plan.has_row_action = true;
+  plan.has_change = true;
+}
+
+// rule_implied_variable_bounds.cu:886-1164
+void apply_rule_implied_variable_bounds(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                                        const PresolveParams& prm) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0 || n == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_plan P = plan_view(plan);
+  const gpk_csr A = csr_view(lp.A), AT = csr_view(lp.AT);
+  DBuf<double> rmin(m), rmax(m);
+  DBuf<int32_t> rni(m), rpi(m);
+  check(gpk_row_activity(A.ptr, A.idx, A.val, m, plan.new_l, plan.new_u, prm.zero_tol, rmin, rmax, rni, rpi, s),
+        "implied bounds activity");
+  DBuf<double> cl(n), cu(n), vmax(n);
+  check(gpk_implied_candidates(AT, P, stats.row_nnz, rmin, rmax, rni, rpi, prm.zero_tol, cl, cu, s),
+        "implied bounds candidates");
+  check(gpk_col_max_abs(AT.ptr, AT.val, n, vmax, s), "implied bounds col max");
+  DBuf<double> fl(n), fu(n), fval(n);
+  DBuf<uint8_t> lch(n), uch(n), fixed(n);
+  int32_t* f = c.flags();
+  check(gpk_implied_finalize(n, P, lp.l ? lp.l : plan.new_l, lp.u ? lp.u : plan.new_u, cl, cu, vmax,
+                             prm.implied_variable_bounds_preserve_free_zero_cost ? 1 : 0, prm.zero_tol,
+                             prm.feasibility_tol, fl, fu, lch, uch, fixed, fval, f, s),
+        "implied bounds finalize");
+  const int32_t* h = c.read_flags(3);
+  if (h[0]) {
+    plan.has_infeasible = true;
+    return;
+  }
+  const bool tightened = h[1] != 0, any_fixed = h[2] != 0;
+  if (!(tightened || any_fixed)) return;
+  if (prm.record_postsolve_tape && tightened) {
+    DBuf<int32_t> sl(n), su(n), cnt(static_cast<int64_t>(n) + 1);
+    const double support_tol = std::fmax(10.0 * prm.bound_tol, 1.0e-10);
+    check(gpk_implied_support(AT, P, stats.row_nnz, rmin, rmax, rni, rpi, lch, uch, fl, fu, prm.zero_tol,
+                              prm.bound_tol, support_tol, sl, su, cnt, s),
+          "implied bounds support rows");
+    const int32_t records = amd::scan_counts(cnt, n, cnt);
+    if (records > 0) {
+      DBuf<int32_t> rcol(records), rrow(records);
+      DBuf<double> rval(4 * static_cast<int64_t>(records));
+      check(gpk_implied_pack_tape(n, P, lch, uch, sl, su, fl, fu, cnt, rcol, rrow, rval, s), "implied bounds pack");
+      const std::vector<int32_t> hc = c.to_host(rcol.get(), records, "implied tape cols");
+      const std::vector<int32_t> hr = c.to_host(rrow.get(), records, "implied tape rows");
+      const std::vector<double> hv = c.to_host(rval.get(), 4 * static_cast<int64_t>(records), "implied tape vals");
+      for (int32_t r = 0; r < records; ++r) {
+        const double* v = hv.data() + 4 * static_cast<size_t>(r);
+        if (hr[r] != INT_MAX)
+          append_record(plan.tape, RT::BoundChangeTheRow, {hc[r], hr[r]}, {v[0], v[1], v[2], v[3]}, DM::Minimal);
+        else
+          append_record(plan.tape, RT::BoundChangeNoRow, {hc[r]}, {v[0], v[1], v[2], v[3]}, DM::Minimal);
+      }
+    }
+  }
+  check(gpk_implied_apply(n, P, fl, fu, fixed, s), "implied bounds apply");
+  if (any_fixed) {
+    check(gpk_row_shift(A, plan.keep_row_mask, fixed, fval, -1.0, plan.new_AL, plan.new_AU, s),
+          "implied bounds row shift");
+    f = c.flags();
+    check(gpk_remove_empty_rows_after_fix(A, P, prm.feasibility_tol, f, s), "implied bounds empty rows");
+    if (c.read_flags(4)[3]) {
+      plan.has_infeasible = true;
+      return;
+    }
+  }
+  const double obj = any_fixed ? amd::fixed_col_tape(prm.record_postsolve_tape ? &plan.tape : nullptr, fixed, nullptr,
+                                                     fval, plan.keep_row_mask, plan.new_c, lp.AT, n)
+                               : 0.0;
+  plan.obj_constant_delta += obj;
+  plan.has_row_action = plan.has_row_action || any_fixed;
+  plan.has_col_action = plan.has_col_action || any_fixed;
+  plan.has_change = true;
+}
+
+void release_implied_variable_bounds_workspace() noexcept {}
+
+// rule_duplicate_rows.cu:424-613
+void apply_rule_duplicate_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                               const PresolveParams& prm) {
+  (void)stats;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows;
+  if (m <= 1) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_csr A = csr_view(lp.A);
+  DBuf<uint64_t> hash(m), hash_alt(m);
+  DBuf<int32_t> rows(m), rows_alt(m);
+  check(gpk_dup_row_hash(A, plan.keep_row_mask, prm.zero_tol, hash, s), "duplicate rows hash");
+  check(gpk_iota_i32(rows, m, s), "duplicate rows iota");
+  {
+    DBuf<int32_t> scratch(gpk_radix_scratch_ints(m));
+    check(gpk_radix_sort_u64_i32(hash, rows, hash_alt, rows_alt, m, 64, scratch, s), "duplicate rows sort");
+  }
+  DBuf<uint8_t> del(m);
+  DBuf<int32_t> merge_to(m), step(m);
+  DBuf<double> rec(5 * static_cast<int64_t>(m));
+  zero(del.get(), m, "duplicate rows delete mask");
+  int32_t* f = c.flags();
+  check(gpk_dup_row_groups(A, plan_view(plan), hash, rows, prm.zero_tol, prm.feasibility_tol, del, merge_to, step,
+                           rec, f, s),
+        "duplicate rows groups");
+  if (c.read_flags(1)[0]) {
+    plan.has_infeasible = true;
+    return;
+  }
+  f = c.flags();
+  check(gpk_delete_rows(m, plan.keep_row_mask, del, f, s), "duplicate rows delete");
+  if (c.read_flags(1)[0] == 0) return;
+  if (prm.record_postsolve_tape) {
+    const std::vector<int32_t> deleted = masked_indices(del, m);
+    const std::vector<int32_t> hto = c.to_host(merge_to.get(), m, "duplicate rows merge_to");
+    const std::vector<int32_t> hstep = c.to_host(step.get(), m, "duplicate rows steps");
+    const std::vector<double> hrec = c.to_host(rec.get(), 5 * static_cast<int64_t>(m), "duplicate rows values");
+    std::vector<int32_t> order(deleted);
+    std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+      return hstep[a] != hstep[b] ? hstep[a] < hstep[b] : a < b;
+    });
+    for (int32_t row : order) {
+      const double* v = hrec.data() + 5 * static_cast<size_t>(row);
+      append_record(plan.tape, RT::DuplicateRow, {hto[row], row}, {v[0], v[1], v[2], v[3], v[4]}, DM::Exact);
+    }
+  }
+  plan.has_row_action = true;
+  plan.has_change = true;
+}
+
+// rule_infeasible_fixed_variables.cu:117-266
+void apply_rule_infeasible_fixed_variables(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t n = lp.A.cols, m = lp.A.rows;
+  if (n == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_plan P = plan_view(plan);
+  DBuf<uint8_t> fixed(n);
+  DBuf<double> v(n), contrib(n);
+  int32_t* f = c.flags();
+  check(gpk_fixed_vars_candidates(n, P, prm.bound_tol, fixed, v, contrib, f, s), "fixed vars candidates");
+  if (c.read_flags(1)[0] == 0) return;
+  if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
+    const std::vector<int32_t> cols = masked_indices(fixed, n);
+    const std::vector<double> hv = c.to_host(v.get(), n, "fixed vars tape v");
+    const std::vector<double> hc = c.to_host(plan.new_c, n, "fixed vars tape c");
+    const std::vector<uint8_t> kr = c.to_host(plan.keep_row_mask, m, "fixed vars tape keep_row");
+    const std::vector<int32_t> ap = c.to_host(lp.AT.rowPtr, static_cast<int64_t>(n) + 1, "fixed vars tape AT ptr");
+    const std::vector<int32_t> ai = c.to_host(lp.AT.colVal, lp.AT.nnz, "fixed vars tape AT idx");
+    const std::vector<double> av = c.to_host(lp.AT.nzVal, lp.AT.nnz, "fixed vars tape AT val");
+    for (int32_t j : cols) {
+      std::vector<int32_t> idx{j};
+      std::vector<double> vals{hv[j], hc[j]};
+      for (int32_t q = ap[j]; q < ap[j + 1]; ++q) {
+        if (!kr[ai[q]]) continue;
+        idx.push_back(ai[q]);
+        vals.push_back(av[q]);
+      }
+      append_record(plan.tape, RT::FixedCol, idx.data(), idx.size(), vals.data(), vals.size(), DM::Minimal);
+    }
+  }
+  // Row shift over every row (the reference does not test keep_row here).
+  check(gpk_row_shift(csr_view(lp.A), nullptr, fixed, v, -1.0, plan.new_AL, plan.new_AU, s), "fixed vars row shift");
+  check(gpk_fix_cols_apply(n, P, fixed, v, s), "fixed vars apply");
+  plan.obj_constant_delta += amd::ordered_masked_sum(fixed, contrib, n);
+  plan.has_col_action = true;
+  plan.has_change = true;
+}
+
+// rule_empty_cols.cu:110-198
+void apply_rule_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                           const PresolveParams& prm) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t n = lp.A.cols;
+  if (n == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_plan P = plan_view(plan);
+  DBuf<uint8_t> fixed(n);
+  DBuf<double> v(n), contrib(n);
+  int32_t* f = c.flags();
+  check(gpk_empty_cols(n, P, stats.empty_col_mask, prm.bound_tol, prm.zero_tol, fixed, v, contrib, f, s),
+        "empty cols");
+  const int32_t* h = c.read_flags(3);
+  if (h[0]) {
+    plan.has_infeasible = true;
+    return;
+  }
+  if (h[1]) {
+    plan.has_unbounded = true;
+    return;
+  }
+  if (!h[2]) return;
+  if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
+    const std::vector<int32_t> cols = masked_indices(fixed, n);
+    const std::vector<double> hv = c.to_host(v.get(), n, "empty cols tape v");
+    const std::vector<double> hc = c.to_host(plan.new_c, n, "empty cols tape c");
+    for (int32_t j : cols) append_record(plan.tape, RT::FixedCol, {j}, {hv[j], hc[j]}, DM::Minimal);
+  }
+  check(gpk_fix_cols_apply(n, P, fixed, v, s), "empty cols apply");
+  plan.obj_constant_delta += amd::ordered_masked_sum(fixed, contrib, n);
+  plan.has_col_action = true;
+  plan.has_change = true;
+}
+
+// rule_column_singletons.cu:1631-1928
+static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                                   const PresolveParams& prm, bool eq_mode, bool skip_precheck) {
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0 || n == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_plan P = plan_view(plan);
+  const gpk_csr A = csr_view(lp.A), AT = csr_view(lp.AT);
+  DBuf<uint8_t> needed(m);
+  zero(needed.get(), m, "column singletons needed rows");
+  int32_t* f = c.flags();
+  check(gpk_cs_precheck(n, m, P, stats.column_singleton_mask, stats.column_singleton_row, prm.bound_tol, needed, f, s),
+        "column singletons precheck");
+  if (!skip_precheck) {
+    const int32_t* h = c.read_flags(2);
+    const int32_t count = h[0], eq = h[1];
+    if (count == 0 || (eq_mode && eq == 0) || (!eq_mode && count == eq)) return;
+  }
+  DBuf<double> fmn(m), fmx(m);
+  DBuf<int32_t> mn(m), mp(m), xn(m), xp(m);
+  DBuf<uint8_t> invalid(m);
+  const gpk_row_cache rc{fmn, fmx, mn, mp, xn, xp, invalid};
+  check(gpk_cs_activity_cache(A, P, needed, rc, s), "column singletons activity");
+  DBuf<int32_t> owner(m);
+  check(gpk_fill_i32(owner, INT_MAX, m, s), "column singletons owner fill");
+  f = c.flags();
+  check(gpk_cs_owner(A, P, stats.column_singleton_mask, stats.column_singleton_row, stats.column_singleton_val, rc,
+                     prm.bound_tol, prm.zero_tol, eq_mode ? 1 : 0, owner, f, s),
+        "column singletons owner");
+  if (c.read_flags(1)[0]) {
+    plan.has_unbounded = true;
+    return;
+  }
+  DBuf<double> AL_new(m), AU_new(m), side(n), contrib(n), c_new(n);
+  DBuf<uint8_t> row_del(m), row_changed(m), col_del(n);
+  zero(row_del.get(), m, "cs row_del");
+  zero(row_changed.get(), m, "cs row_changed");
+  zero(col_del.get(), n, "cs col_del");
+  zero(side.get(), n, "cs side");
+  zero(contrib.get(), n, "cs contrib");
+  d2d(AL_new.get(), plan.new_AL, m, "cs AL copy");
+  d2d(AU_new.get(), plan.new_AU, m, "cs AU copy");
+  f = c.flags();
+  check(gpk_cs_process(n, m, P, stats.column_singleton_mask, stats.column_singleton_row, stats.column_singleton_val,
+                       owner, rc, prm.bound_tol, prm.zero_tol, eq_mode ? 1 : 0, AL_new, AU_new, row_del, row_changed,
+                       col_del, side, contrib, f, s),
+        "column singletons process");
+  const int32_t* h = c.read_flags(3);
+  if (h[0]) {
+    plan.has_unbounded = true;
+    return;
+  }
+  const bool st_eq = h[1] != 0, st_dual = h[2] != 0;
+  if (!(st_eq || st_dual)) return;
+  if (st_eq && eq_mode)
+    check(gpk_cs_objective(AT, P, col_del, owner, stats.column_singleton_val, c_new, s), "column singletons objective");
+  else
+    d2d(c_new.get(), plan.new_c, n, "cs c copy");
+  if (st_eq && eq_mode && prm.record_postsolve_tape) {
+    DBuf<uint8_t> sel(m);
+    DBuf<int32_t> sel_prefix(static_cast<int64_t>(m) + 1), sup(static_cast<int64_t>(m) + 1);
+    check(gpk_cs_subcol_counts(A, P, owner, col_del, sel, sup, s), "subcol counts");
+    const int32_t records = amd::scan_mask_count(sel, m, sel_prefix);
+    const int32_t support = amd::scan_counts(sup, m, sup);
+    if (records > 0) {
+      DBuf<int32_t> oi3(3 * static_cast<int64_t>(records)), scol(support);
+      DBuf<double> ov6(6 * static_cast<int64_t>(records)), sval(support);
+      check(gpk_cs_subcol_pack(A, P, owner, row_del, side, sel, sel_prefix, sup, oi3, ov6, scol, sval, s),
+            "subcol pack");
+      // Records are in ascending row order; record r owns the support
+      // entries [start_r, start_{r+1}) of the packed arrays.
+      const std::vector<int32_t> hi3 = c.to_host(oi3.get(), 3 * static_cast<int64_t>(records), "subcol ids");
+      const std::vector<double> hv6 = c.to_host(ov6.get(), 6 * static_cast<int64_t>(records), "subcol vals");
+      const std::vector<int32_t> hsc = c.to_host(scol.get(), support, "subcol support cols");
+      const std::vector<double> hsv = c.to_host(sval.get(), support, "subcol support vals");
+      std::vector<int32_t> idx;
+      std::vector<double> vals;
+      for (int32_t r = 0; r < records; ++r) {
+        const double* v6 = hv6.data() + 6 * static_cast<size_t>(r);
+        if (std::fabs(v6[0]) <= prm.zero_tol) continue;
+        const int32_t b = hi3[3 * r + 2];
+        const int32_t e = r + 1 < records ? hi3[3 * (r + 1) + 2] : support;
+        idx.assign({hi3[3 * r], hi3[3 * r + 1], e - b});
+        vals.assign(v6, v6 + 6);
+        for (int32_t q = b; q < e; ++q) {
+          idx.push_back(hsc[q]);
+          vals.push_back(hsv[q]);
+        }
+        append_record(plan.tape, RT::SubCol, idx.data(), idx.size(), vals.data(), vals.size(), DM::Minimal);
+      }
+    }
+  }
+  if (st_dual && !eq_mode && prm.record_postsolve_tape) {
+    const std::vector<int32_t> rows = masked_indices(row_changed, m);
+    for (int32_t row : rows) append_record(plan.tape, RT::EqToIneq, {row}, {0.0}, DM::Minimal);
+  }
+  check(gpk_cs_apply(m, n, P, row_del, col_del, AL_new, AU_new, c_new, s), "column singletons apply");
+  plan.obj_constant_delta += amd::ordered_masked_sum(col_del, contrib, n);
+  plan.has_row_action = true;
+  plan.has_col_action = plan.has_col_action || st_eq;
+  plan.has_change = true;
+}
+
+void apply_rule_column_singletons_eq(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                                     const PresolveParams& prm, bool skip_precheck) {
+  column_singletons_mode(plan, lp, stats, prm, true, skip_precheck);
+}
+
+void apply_rule_column_singletons_dual_infer(PresolvePlanGpu& plan, const LPInfoGpu& lp,
+                                             const PresolveStatsGpu& stats, const PresolveParams& prm) {
+  column_singletons_mode(plan, lp, stats, prm, false, false);
+}
+
+void release_column_singletons_workspace() noexcept {}
+
+}  // namespace gpu_presolver::presolve

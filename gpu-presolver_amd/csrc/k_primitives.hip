@@ -1,0 +1,649 @@
+// Primitive kernels of the presolve library: fills, scans, row activity,
+// column max-abs, CSR statistics, mask compaction, LSD radix sort and the
+// transpose built on it.  gfx950, wave64, 256-thread blocks.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "gp_device.hpp"
+#include "gp_kernels.h"
+
+using namespace gpdev;
+
+namespace {
+
+// ---------------------------------------------------------------- fills ----
+template <class T>
+__global__ void k_fill(T* dst, T v, int64_t n) {
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < n;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    dst[i] = v;
+}
+
+__global__ void k_iota(int32_t* dst, int32_t n) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = i;
+}
+
+// ---------------------------------------------------------------- scans ----
+// Reduce-then-scan: tile sums, one-block scan of tile sums, tile downsweep.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;  // 2048
+
+template <class In>
+__device__ inline int32_t scan_value(const In* in, int64_t i);
+template <>
+__device__ inline int32_t scan_value<int32_t>(const int32_t* in, int64_t i) { return in[i]; }
+template <>
+__device__ inline int32_t scan_value<uint8_t>(const uint8_t* in, int64_t i) { return in[i] != 0 ? 1 : 0; }
+
+template <class In>
+__global__ __launch_bounds__(kBlock) void k_scan_tiles(const In* in, int64_t n, int32_t* tile_sums) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k)
+    if (base + k < n) s += scan_value(in, base + k);
+  int32_t total;
+  (void)block_exclusive_sum(s, &total);
+  if (threadIdx.x == 0) tile_sums[blockIdx.x] = total;
+}
+
+// One block scans `tiles` tile sums in place (exclusive) and stores the grand total in out[n].
+__global__ __launch_bounds__(kBlock) void k_scan_tile_sums(int32_t* tile_sums, int64_t tiles, int32_t* out_total) {
+  __shared__ int32_t s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int64_t b = 0; b < tiles; b += kBlock) {
+    const int64_t i = b + threadIdx.x;
+    const int32_t v = i < tiles ? tile_sums[i] : 0;
+    int32_t total;
+    const int32_t ex = block_exclusive_sum(v, &total);
+    const int32_t carry = s_carry;
+    if (i < tiles) tile_sums[i] = carry + ex;
+    __syncthreads();
+    if (threadIdx.x == 0) s_carry = carry + total;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out_total = s_carry;
+}
+
+template <class In>
+__global__ __launch_bounds__(kBlock) void k_scan_down(const In* in, int32_t* out, int64_t n, const int32_t* tile_prefix) {
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kScanTile + threadIdx.x * kScanItems;
+  int32_t v[kScanItems];
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? scan_value(in, base + k) : 0;
+    s += v[k];
+  }
+  int32_t total;
+  int32_t run = block_exclusive_sum(s, &total) + tile_prefix[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+
+__global__ void k_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* o2n, int32_t n) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o2n[i] = mask[i] != 0 ? prefix[i] : -1;
+}
+
+// -------------------------------------------------------- row activity ----
+// One thread per row, 256 consecutive rows per block.  The block's CSR
+// segment is streamed through LDS in windows: all threads cooperatively load
+// coalesced (col, a) entries and the gathered l/u, form the min/max terms and
+// stage them; each thread then adds its own row's terms in CSR order, so every
+// row sum has exactly the sequential rounding of the reference kernel
+// (gpu_presolve_kernels.cu:18-66) for any row length.
+constexpr int kActRows = kBlock;
+constexpr int kActCap = 2048;
+
+__global__ __launch_bounds__(kBlock) void k_row_activity(const int32_t* __restrict__ row_ptr,
+                                                         const int32_t* __restrict__ col_idx,
+                                                         const double* __restrict__ val, int32_t m,
+                                                         const double* __restrict__ l,
+                                                         const double* __restrict__ u, double ztol,
+                                                         double* __restrict__ min_fin,
+                                                         double* __restrict__ max_fin,
+                                                         int32_t* __restrict__ neg_inf,
+                                                         int32_t* __restrict__ pos_inf) {
+  __shared__ double s_tmin[kActCap];
+  __shared__ double s_tmax[kActCap];
+  __shared__ int32_t s_ptr[kActRows + 1];
+  const int32_t r0 = blockIdx.x * kActRows;
+  const int32_t rows = min(kActRows, m - r0);
+  const int tid = threadIdx.x;
+  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = row_ptr[r0 + t];
+  __syncthreads();
+  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
+  const int32_t my_b = tid < rows ? s_ptr[tid] : 0;
+  const int32_t my_e = tid < rows ? s_ptr[tid + 1] : 0;
+  double mn = 0.0, mx = 0.0;
+  int32_t ni = 0, pi = 0;
+  for (int32_t w = seg_b; w < seg_e; w += kActCap) {
+    const int32_t we = min(w + kActCap, seg_e);
+    for (int32_t p = w + tid; p < we; p += kBlock) {
+      const double a = val[p];
+      double tmn = 0.0, tmx = 0.0;
+      if (fabs(a) > ztol) {
+        const int32_t j = col_idx[p];
+        const double lj = l[j], uj = u[j];
+        tmn = a >= 0.0 ? a * lj : a * uj;
+        tmx = a >= 0.0 ? a * uj : a * lj;
+      }
+      s_tmin[p - w] = tmn;
+      s_tmax[p - w] = tmx;
+    }
+    __syncthreads();
+    const int32_t b = max(my_b, w), e = min(my_e, we);
+    for (int32_t p = b; p < e; ++p) {
+      const double tmn = s_tmin[p - w], tmx = s_tmax[p - w];
+      if (isfinite(tmn)) mn += tmn; else if (tmn < 0.0) ++ni;
+      if (isfinite(tmx)) mx += tmx; else if (tmx > 0.0) ++pi;
+    }
+    __syncthreads();
+  }
+  if (tid < rows) {
+    min_fin[r0 + tid] = mn;
+    max_fin[r0 + tid] = mx;
+    neg_inf[r0 + tid] = ni;
+    pos_inf[r0 + tid] = pi;
+  }
+}
+
+__global__ void k_col_max_abs(const int32_t* at_ptr, const double* at_val, int32_t n, double* out) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  double v = 0.0;
+  for (int32_t p = at_ptr[j]; p < at_ptr[j + 1]; ++p) v = fmax(v, fabs(at_val[p]));
+  out[j] = v;
+}
+
+__global__ void k_csr_stats(const int32_t* ptr, const int32_t* idx, const double* val, int32_t rows,
+                            int32_t* nnz_out, uint8_t* empty_mask, uint8_t* singleton_mask,
+                            int32_t* singleton_idx, double* singleton_val) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const int32_t b = ptr[i];
+  const int32_t cnt = ptr[i + 1] - b;
+  if (nnz_out) nnz_out[i] = cnt;
+  if (empty_mask) empty_mask[i] = cnt == 0 ? 1 : 0;
+  if (singleton_mask) singleton_mask[i] = cnt == 1 ? 1 : 0;
+  if (singleton_idx) singleton_idx[i] = cnt == 1 ? idx[b] : -1;
+  if (singleton_val) singleton_val[i] = cnt == 1 ? val[b] : 0.0;
+}
+
+// ---------------------------------------------------------- compaction ----
+// 256 old rows per block; their entry segment is processed in chunks of
+// kCompChunk entries, each thread owning kCompItems consecutive entries so a
+// wave touches one contiguous 2 KB span.  The row of an entry is found by a
+// binary search over the block's 257 row pointers in LDS.
+constexpr int kCompItems = 8;
+constexpr int kCompChunk = kBlock * kCompItems;
+
+__device__ inline int find_row(const int32_t* s_ptr, int rows, int32_t p) {
+  int lo = 0, hi = rows;  // s_ptr[lo] <= p < s_ptr[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (s_ptr[mid] <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_count(const int32_t* __restrict__ ptr,
+                                                          const int32_t* __restrict__ idx, int32_t m_old,
+                                                          const int32_t* __restrict__ row_map,
+                                                          const int32_t* __restrict__ col_map,
+                                                          int32_t* __restrict__ counts) {
+  __shared__ int32_t s_ptr[kBlock + 1];
+  __shared__ int32_t s_cnt[kBlock];
+  __shared__ uint8_t s_keep[kBlock];
+  const int32_t r0 = blockIdx.x * kBlock;
+  const int rows = min(kBlock, m_old - r0);
+  const int tid = threadIdx.x;
+  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = ptr[r0 + t];
+  s_cnt[tid] = 0;
+  s_keep[tid] = (tid < rows && row_map[r0 + tid] >= 0) ? 1 : 0;
+  __syncthreads();
+  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
+  for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
+    const int32_t p0 = c + tid * kCompItems;
+    int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
+    int local = 0;
+    for (int k = 0; k < kCompItems; ++k) {
+      const int32_t p = p0 + k;
+      if (p >= seg_e) break;
+      while (s_ptr[row + 1] <= p) {
+        if (local) { atomicAdd(&s_cnt[row], local); local = 0; }
+        ++row;
+      }
+      if (s_keep[row] && col_map[idx[p]] >= 0) ++local;
+    }
+    if (local) atomicAdd(&s_cnt[row], local);
+  }
+  __syncthreads();
+  if (tid < rows) counts[r0 + tid] = s_keep[tid] ? s_cnt[tid] : 0;
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_copy(const int32_t* __restrict__ ptr,
+                                                         const int32_t* __restrict__ idx,
+                                                         const double* __restrict__ val, int32_t m_old,
+                                                         const int32_t* __restrict__ row_map,
+                                                         const int32_t* __restrict__ col_map,
+                                                         const int32_t* __restrict__ new_ptr_by_old,
+                                                         int32_t* __restrict__ out_ptr,
+                                                         int32_t* __restrict__ out_idx,
+                                                         double* __restrict__ out_val) {
+  __shared__ int32_t s_ptr[kBlock + 1];
+  __shared__ uint8_t s_keep[kBlock];
+  const int32_t r0 = blockIdx.x * kBlock;
+  const int rows = min(kBlock, m_old - r0);
+  const int tid = threadIdx.x;
+  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = ptr[r0 + t];
+  int32_t rm = -1;
+  if (tid < rows) {
+    rm = row_map[r0 + tid];
+    if (rm >= 0) out_ptr[rm] = new_ptr_by_old[r0 + tid];
+  }
+  s_keep[tid] = rm >= 0 ? 1 : 0;
+  __syncthreads();
+  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
+  int32_t write_base = new_ptr_by_old[r0];
+  for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
+    const int32_t p0 = c + tid * kCompItems;
+    int32_t jn[kCompItems];
+    int32_t cnt = 0;
+    int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      const int32_t p = p0 + k;
+      jn[k] = -1;
+      if (p < seg_e) {
+        while (s_ptr[row + 1] <= p) ++row;
+        if (s_keep[row]) jn[k] = col_map[idx[p]];
+        cnt += jn[k] >= 0 ? 1 : 0;
+      }
+    }
+    int32_t total;
+    int32_t pos = write_base + block_exclusive_sum(cnt, &total);
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      if (jn[k] >= 0) {
+        out_idx[pos] = jn[k];
+        out_val[pos] = val[p0 + k];
+        ++pos;
+      }
+    }
+    write_base += total;
+  }
+}
+
+__global__ void k_gather5(const int32_t* map, int32_t n_old, const double* s0, double* d0, const double* s1,
+                          double* d1, const double* s2, double* d2, const double* s3, double* d3,
+                          const double* s4, double* d4) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_old) return;
+  const int32_t t = map[i];
+  if (t < 0) return;
+  if (s0) d0[t] = s0[i];
+  if (s1) d1[t] = s1[i];
+  if (s2) d2[t] = s2[i];
+  if (s3) d3[t] = s3[i];
+  if (s4) d4[t] = s4[i];
+}
+
+__global__ void k_gather_i32(const int32_t* map, int32_t n_old, const int32_t* src, int32_t* dst) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_old) return;
+  const int32_t t = map[i];
+  if (t >= 0) dst[t] = src[i];
+}
+
+// ------------------------------------------------------------ radix sort ----
+// Stable LSD radix sort, 8-bit digits.  Tile = 2048 keys; wave w of a block
+// owns the contiguous keys [w*512, w*512+512) of the tile, read 64 at a time.
+// Per 64-key step the lanes holding the same digit are found with eight
+// ballots, ranked by lane order, and appended to a per-wave digit counter in
+// LDS; digit offsets across waves and tiles come from the global histogram
+// scan, so equal digits keep their input order.
+constexpr int kRsTile = 2048;
+constexpr int kRsWaveItems = 512;
+constexpr int kRsWaves = kBlock / 64;
+
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, int32_t n, int shift,
+                                                      int32_t tiles, int32_t* __restrict__ hist) {
+  __shared__ int32_t s_h[256];
+  s_h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile;
+  for (int k = 0; k < kRsTile / kBlock; ++k) {
+    const int64_t e = base + k * kBlock + threadIdx.x;
+    if (e < n) atomicAdd(&s_h[static_cast<int>((keys[e] >> shift) & 255u)], 1);
+  }
+  __syncthreads();
+  hist[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x] = s_h[threadIdx.x];
+}
+
+template <class K>
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ kin, const int32_t* __restrict__ vin,
+                                                         K* __restrict__ kout, int32_t* __restrict__ vout,
+                                                         int32_t n, int shift, int32_t tiles,
+                                                         const int32_t* __restrict__ offs) {
+  __shared__ int32_t s_cnt[kRsWaves][256];
+  __shared__ int32_t s_off[kRsWaves][256];
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kRsWaves * 256; i += kBlock) s_cnt[i >> 8][i & 255] = 0;
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const uint64_t lt = lanemask_lt(lane);
+  const int64_t base = static_cast<int64_t>(blockIdx.x) * kRsTile + wave * kRsWaveItems;
+  constexpr int kSteps = kRsWaveItems / 64;
+  K kk[kSteps];
+  int32_t vv[kSteps];
+  int dig[kSteps];
+  int rank[kSteps];
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    const int64_t e = base + k * 64 + lane;
+    const bool valid = e < n;
+    int d = 0;
+    if (valid) {
+      kk[k] = kin[e];
+      vv[k] = vin[e];
+      d = static_cast<int>((kk[k] >> shift) & 255u);
+    }
+    dig[k] = d;
+    uint64_t match = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1;
+      const uint64_t bal = __ballot(bit);
+      match &= bit ? bal : ~bal;
+    }
+    rank[k] = 0;
+    if (valid) {
+      const int r = __popcll(match & lt);
+      const int prev = s_cnt[wave][d];
+      rank[k] = prev + r;
+      if (r == 0) s_cnt[wave][d] = prev + __popcll(match);
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  {
+    int32_t run = 0;
+    for (int w = 0; w < kRsWaves; ++w) {
+      s_off[w][tid] = run;
+      run += s_cnt[w][tid];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kSteps; ++k) {
+    const int64_t e = base + k * 64 + lane;
+    if (e < n) {
+      const int d = dig[k];
+      const int32_t pos = offs[static_cast<int64_t>(d) * tiles + blockIdx.x] + s_off[wave][d] + rank[k];
+      kout[pos] = kk[k];
+      vout[pos] = vv[k];
+    }
+  }
+}
+
+// ------------------------------------------------------------- transpose ----
+__global__ void k_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) row_of[p] = i;
+}
+
+__global__ void k_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts) {
+  const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < nnz) atomicAdd(&counts[idx[p]], 1);
+}
+
+__global__ void k_transpose_gather(const int32_t* perm, const int32_t* row_of, const double* val, int32_t nnz,
+                                   int32_t* at_idx, double* at_val) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nnz) return;
+  const int32_t p = perm[k];
+  at_idx[k] = row_of[p];
+  at_val[k] = val[p];
+}
+
+__global__ void k_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t* flag) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  for (int32_t p = ptr[i] + 1; p < ptr[i + 1]; ++p) {
+    if (idx[p] < idx[p - 1]) {
+      atomicOr(flag, 1);
+      return;
+    }
+  }
+}
+
+// Stable insertion sort of one row per thread (used only for inputs whose
+// rows arrive unsorted).
+__global__ void k_sort_rows(const int32_t* ptr, int32_t* idx, double* val, int32_t rows) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const int32_t b = ptr[i], e = ptr[i + 1];
+  for (int32_t p = b + 1; p < e; ++p) {
+    const int32_t key = idx[p];
+    const double v = val[p];
+    int32_t q = p - 1;
+    while (q >= b && idx[q] > key) {
+      idx[q + 1] = idx[q];
+      val[q + 1] = val[q];
+      --q;
+    }
+    idx[q + 1] = key;
+    val[q + 1] = v;
+  }
+}
+
+template <class In>
+hipError_t scan_impl(const In* in, int32_t* out, int64_t n, int32_t* scratch, hipStream_t s) {
+  if (n <= 0) {
+    return hipMemsetAsync(out, 0, sizeof(int32_t), s);
+  }
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_scan_tiles<In>, dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, in, n, scratch);
+  hipLaunchKernelGGL(k_scan_tile_sums, dim3(1), dim3(kBlock), 0, s, scratch, tiles, out + n);
+  hipLaunchKernelGGL(k_scan_down<In>, dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, in, out, n,
+                     static_cast<const int32_t*>(scratch));
+  return hipGetLastError();
+}
+
+template <class K>
+hipError_t radix_impl(K* keys, int32_t* vals, K* keys_alt, int32_t* vals_alt, int32_t n, int32_t key_bits,
+                      int32_t* scratch, hipStream_t s) {
+  if (n <= 1) return hipSuccess;
+  const int32_t tiles = (n + kRsTile - 1) / kRsTile;
+  int32_t* hist = scratch;
+  int32_t* scan_scratch = scratch + static_cast<int64_t>(256) * tiles + 1;
+  const int passes = key_bits <= 0 ? 1 : (key_bits + 7) / 8;
+  K* ka = keys;
+  int32_t* va = vals;
+  K* kb = keys_alt;
+  int32_t* vb = vals_alt;
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = pass * 8;
+    hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(kBlock), 0, s, ka, n, shift, tiles, hist);
+    hipError_t e = scan_impl<int32_t>(hist, hist, static_cast<int64_t>(256) * tiles, scan_scratch, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(kBlock), 0, s, ka, va, kb, vb, n, shift, tiles,
+                       static_cast<const int32_t*>(hist));
+    K* tk = ka; ka = kb; kb = tk;
+    int32_t* tv = va; va = vb; vb = tv;
+  }
+  if (ka != keys) {
+    hipError_t e = hipMemcpyAsync(keys, ka, sizeof(K) * n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(vals, va, sizeof(int32_t) * n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipGetLastError();
+}
+
+inline unsigned grid_fill(int64_t n) {
+  const unsigned g = grid_for(n);
+  return g > 65535u ? 65535u : g;
+}
+
+}  // namespace
+
+extern "C" {
+
+hipError_t gpk_fill_u8(uint8_t* dst, uint8_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<uint8_t>, dim3(grid_fill(n)), dim3(kBlock), 0, s, dst, v, n);
+  return hipGetLastError();
+}
+hipError_t gpk_fill_i32(int32_t* dst, int32_t v, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<int32_t>, dim3(grid_fill(n)), dim3(kBlock), 0, s, dst, v, n);
+  return hipGetLastError();
+}
+hipError_t gpk_fill_f64(double* dst, double v, int64_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill<double>, dim3(grid_fill(n)), dim3(kBlock), 0, s, dst, v, n);
+  return hipGetLastError();
+}
+hipError_t gpk_iota_i32(int32_t* dst, int32_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_iota, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, n);
+  return hipGetLastError();
+}
+
+int64_t gpk_scan_scratch_ints(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+hipError_t gpk_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* scratch, hipStream_t s) {
+  return scan_impl<int32_t>(in, out, n, scratch, s);
+}
+
+hipError_t gpk_scan_mask_u8(const uint8_t* mask, int32_t* out, int64_t n, int32_t* scratch, hipStream_t s) {
+  return scan_impl<uint8_t>(mask, out, n, scratch, s);
+}
+
+hipError_t gpk_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* o2n, int32_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_old_to_new, dim3(grid_for(n)), dim3(kBlock), 0, s, mask, prefix, o2n, n);
+  return hipGetLastError();
+}
+
+hipError_t gpk_row_activity(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
+                            const double* l, const double* u, double ztol, double* min_fin, double* max_fin,
+                            int32_t* neg_inf, int32_t* pos_inf, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_activity, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx, val, m, l,
+                     u, ztol, min_fin, max_fin, neg_inf, pos_inf);
+  return hipGetLastError();
+}
+
+hipError_t gpk_col_max_abs(const int32_t* at_ptr, const double* at_val, int32_t n, double* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_col_max_abs, dim3(grid_for(n)), dim3(kBlock), 0, s, at_ptr, at_val, n, out);
+  return hipGetLastError();
+}
+
+hipError_t gpk_csr_stats(const int32_t* ptr, const int32_t* idx, const double* val, int32_t rows, int32_t* nnz_out,
+                         uint8_t* empty_mask, uint8_t* singleton_mask, int32_t* singleton_idx,
+                         double* singleton_val, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_stats, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, val, rows, nnz_out,
+                     empty_mask, singleton_mask, singleton_idx, singleton_val);
+  return hipGetLastError();
+}
+
+hipError_t gpk_compact_count(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
+                             const int32_t* col_map, int32_t* counts, hipStream_t s) {
+  if (m_old <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact_count, dim3(grid_for(m_old)), dim3(kBlock), 0, s, ptr, idx, m_old, row_map, col_map,
+                     counts);
+  return hipGetLastError();
+}
+
+hipError_t gpk_compact_copy(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
+                            const int32_t* row_map, const int32_t* col_map, const int32_t* new_ptr_by_old,
+                            int32_t* out_ptr, int32_t* out_idx, double* out_val, int32_t m_new, hipStream_t s) {
+  if (m_old > 0) {
+    hipLaunchKernelGGL(k_compact_copy, dim3(grid_for(m_old)), dim3(kBlock), 0, s, ptr, idx, val, m_old, row_map,
+                       col_map, new_ptr_by_old, out_ptr, out_idx, out_val);
+  }
+  // out_ptr[m_new] = total kept entries = new_ptr_by_old[m_old]
+  const hipError_t e = hipMemcpyAsync(out_ptr + m_new, new_ptr_by_old + m_old, sizeof(int32_t),
+                                      hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+hipError_t gpk_gather5(const int32_t* map, int32_t n_old, const double* s0, double* d0, const double* s1, double* d1,
+                       const double* s2, double* d2, const double* s3, double* d3, const double* s4, double* d4,
+                       hipStream_t s) {
+  if (n_old <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather5, dim3(grid_for(n_old)), dim3(kBlock), 0, s, map, n_old, s0, d0, s1, d1, s2, d2, s3,
+                     d3, s4, d4);
+  return hipGetLastError();
+}
+
+hipError_t gpk_gather_i32(const int32_t* map, int32_t n_old, const int32_t* src, int32_t* dst, hipStream_t s) {
+  if (n_old <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_gather_i32, dim3(grid_for(n_old)), dim3(kBlock), 0, s, map, n_old, src, dst);
+  return hipGetLastError();
+}
+
+int64_t gpk_radix_scratch_ints(int64_t n) {
+  const int64_t tiles = (n + kRsTile - 1) / kRsTile;
+  return 256 * tiles + 1 + gpk_scan_scratch_ints(256 * tiles) + 1;
+}
+
+hipError_t gpk_radix_sort_u64_i32(uint64_t* keys, int32_t* vals, uint64_t* keys_alt, int32_t* vals_alt, int32_t n,
+                                  int32_t key_bits, int32_t* scratch, hipStream_t s) {
+  return radix_impl<uint64_t>(keys, vals, keys_alt, vals_alt, n, key_bits, scratch, s);
+}
+
+hipError_t gpk_radix_sort_u32_i32(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* vals_alt, int32_t n,
+                                  int32_t key_bits, int32_t* scratch, hipStream_t s) {
+  return radix_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, key_bits, scratch, s);
+}
+
+hipError_t gpk_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_expand_rows, dim3(grid_for(m)), dim3(kBlock), 0, s, ptr, m, row_of);
+  return hipGetLastError();
+}
+
+hipError_t gpk_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts, hipStream_t s) {
+  if (nnz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_col_histogram, dim3(grid_for(nnz)), dim3(kBlock), 0, s, idx, nnz, counts);
+  return hipGetLastError();
+}
+
+hipError_t gpk_transpose_gather(const int32_t* perm, const int32_t* row_of, const double* val, int32_t nnz,
+                                int32_t* at_idx, double* at_val, hipStream_t s) {
+  if (nnz <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_transpose_gather, dim3(grid_for(nnz)), dim3(kBlock), 0, s, perm, row_of, val, nnz, at_idx,
+                     at_val);
+  return hipGetLastError();
+}
+
+hipError_t gpk_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t* flag, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows_unsorted, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, rows, flag);
+  return hipGetLastError();
+}
+
+hipError_t gpk_sort_rows_in_place(const int32_t* ptr, int32_t* idx, double* val, int32_t rows, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_sort_rows, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, val, rows);
+  return hipGetLastError();
+}
+
+}  // extern "C"

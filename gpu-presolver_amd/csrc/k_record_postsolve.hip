@@ -1,0 +1,459 @@
+// Record maintenance (device-resident red2org maps, fixed-column records,
+// tape index globalization) and postsolve kernels: scatter of the reduced
+// solution, fixed-value restore, reverse tape replay and the column-dual
+// projection.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+
+#include "gp_device.hpp"
+#include "gp_kernels.h"
+#include "gp_record_kernels.h"
+
+using namespace gpdev;
+
+namespace {
+
+// ------------------------------------------------------------ record ----
+// Removed columns whose bounds are finite and within 1e-9 (gpu_presolve.cu:1005-1043).
+__global__ void k_removed_fixed_mask(int32_t n, const uint8_t* keep_col, const double* l, const double* u,
+                                     uint8_t* mask) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double lo = l[j], hi = u[j];
+  mask[j] = (!keep_col[j] && isfinite(lo) && isfinite(hi) && fabs(lo - hi) <= 1.0e-9) ? 1 : 0;
+}
+
+__global__ void k_pack_fixed_records(int32_t n, const uint8_t* mask, const int32_t* prefix, const int32_t* red2org,
+                                     const double* l, const double* u, int32_t* out_idx, double* out_val) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n || !mask[j]) return;
+  const int32_t r = prefix[j];
+  out_idx[r] = red2org[j];
+  out_val[r] = 0.5 * (l[j] + u[j]);
+}
+
+// kind: 1 = row index, 2 = column index, 0 = not an index.  Out-of-range
+// values are left unchanged (gpu_presolve.cu:1347-1361).
+__global__ void k_map_tape_indices(int32_t count, int32_t* idx, const uint8_t* kind, const int32_t* row_red2org,
+                                   int32_t m, const int32_t* col_red2org, int32_t n) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= count) return;
+  const int32_t v = idx[t];
+  if (kind[t] == 1) {
+    if (v >= 0 && v < m) idx[t] = row_red2org[v];
+  } else if (kind[t] == 2) {
+    if (v >= 0 && v < n) idx[t] = col_red2org[v];
+  }
+}
+
+// --------------------------------------------------------- postsolve ----
+__global__ void k_scatter(const double* src, const int32_t* red2org, int32_t n_red, double* dst) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_red) dst[red2org[i]] = src[i];
+}
+
+__global__ void k_mark(const int32_t* red2org, int32_t n_red, uint8_t* mask) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_red) mask[red2org[i]] = 1;
+}
+
+__global__ void k_restore_fixed(const int32_t* idx, const double* val, int32_t count, double* x) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) x[idx[i]] = val[i];
+}
+
+__device__ inline bool pfin(double x) { return x > -INFINITY && x < INFINITY; }
+__device__ inline bool at_lower(double x, double l, double tol) { return pfin(l) && x <= l + tol; }
+__device__ inline bool at_upper(double x, double u, double tol) { return pfin(u) && x >= u - tol; }
+__device__ inline double clamp_inf(double x, double lo, double hi) {
+  double y = x;
+  if (pfin(lo)) y = fmax(y, lo);
+  if (pfin(hi)) y = fmin(y, hi);
+  return y;
+}
+
+// gpu_postsolve.cu:189-210
+__device__ double merged_source_value(double lo, double hi, double tol) {
+  if (isfinite(lo) && isfinite(hi)) return 0.5 * (lo + hi);
+  if (isfinite(lo)) {
+    if (0.0 > lo + tol) return 0.0;
+    return lo + fmax(1.0, fabs(lo) * fmax(tol, 1.0e-12));
+  }
+  if (isfinite(hi)) {
+    if (0.0 < hi - tol) return 0.0;
+    return hi - fmax(1.0, fabs(hi) * fmax(tol, 1.0e-12));
+  }
+  return 0.0;
+}
+
+// gpu_postsolve.cu:240-282
+__device__ void recover_dup_col(double y, double z_to, double r, double sl, double su, double tl, double tu,
+                                double tol, double* x_from, double* x_to) {
+  double lo = sl, hi = su;
+  if (isfinite(tu)) { const double b = (y - tu) / r; if (r > 0.0) lo = fmax(lo, b); else hi = fmin(hi, b); }
+  if (isfinite(tl)) { const double b = (y - tl) / r; if (r > 0.0) hi = fmin(hi, b); else lo = fmax(lo, b); }
+  double lower = -INFINITY, upper = INFINITY;
+  if (r > 0.0) {
+    if (isfinite(tl) && isfinite(sl)) lower = tl + r * sl;
+    if (isfinite(tu) && isfinite(su)) upper = tu + r * su;
+  } else {
+    if (isfinite(tl) && isfinite(su)) lower = tl + r * su;
+    if (isfinite(tu) && isfinite(sl)) upper = tu + r * sl;
+  }
+  if (z_to > tol && isfinite(lower) && fabs(y - lower) <= tol) *x_from = r > 0.0 ? sl : su;
+  else if (z_to < -tol && isfinite(upper) && fabs(y - upper) <= tol) *x_from = r > 0.0 ? su : sl;
+  else *x_from = merged_source_value(lo, hi, tol);
+  *x_to = y - r * (*x_from);
+}
+
+enum : uint8_t { ST_ZERO = 0, ST_NONNEG = 1, ST_NONPOS = 2, ST_FREE = 3 };
+__device__ uint8_t row_state(double act, double lo, double hi, double tol) {
+  const bool al = pfin(lo) && act <= lo + tol;
+  const bool au = pfin(hi) && act >= hi - tol;
+  if (al && au) return ST_FREE;
+  if (al) return ST_NONNEG;
+  if (au) return ST_NONPOS;
+  return ST_ZERO;
+}
+__device__ void state_interval(uint8_t s, double* lo, double* hi) {
+  if (s == ST_FREE) { *lo = -INFINITY; *hi = INFINITY; }
+  else if (s == ST_NONNEG) { *lo = 0.0; *hi = INFINITY; }
+  else if (s == ST_NONPOS) { *lo = -INFINITY; *hi = 0.0; }
+  else { *lo = 0.0; *hi = 0.0; }
+}
+__device__ void deleted_dual_interval(double yp, double r, uint8_t ks, double* lo, double* hi) {
+  const double piv = r * yp;
+  if (ks == ST_FREE) { *lo = -INFINITY; *hi = INFINITY; }
+  else if (ks == ST_ZERO) { *lo = piv; *hi = piv; }
+  else if (ks == ST_NONNEG) { if (r > 0.0) { *lo = -INFINITY; *hi = piv; } else { *lo = piv; *hi = INFINITY; } }
+  else { if (r > 0.0) { *lo = piv; *hi = INFINITY; } else { *lo = -INFINITY; *hi = piv; } }
+}
+// gpu_postsolve.cu:409-469
+__device__ bool doc_dual_split(double yp, double r, double kal, double kau, double dsl, double dsu, double tol,
+                               double* op, double* od) {
+  if (fabs(yp) <= tol) { *op = 0.0; *od = 0.0; return true; }
+  const double bl = fmax(kal, dsl), bu = fmin(kau, dsu);
+  if (!(bl <= bu + tol)) { *op = yp; *od = 0.0; return false; }
+  const bool lt = bl > kal + tol, ut = bu < kau - tol;
+  if (lt && ut) { *op = yp; *od = 0.0; return false; }
+  if (!lt && !ut) { *op = yp; *od = 0.0; return true; }
+  if (yp > 0.0) {
+    if (lt) { *op = 0.0; *od = r * yp; } else { *op = yp; *od = 0.0; }
+    return true;
+  }
+  if (lt) { *op = yp; *od = 0.0; } else { *op = 0.0; *od = r * yp; }
+  return true;
+}
+// gpu_postsolve.cu:471-515
+__device__ void dbl_recover_z(double xk, double zk_hat, double al, double okl, double oku, double nkl, double nku,
+                              double ls, double us, double tol, double* zk, double* ze) {
+  if (fabs(al) <= tol) { *zk = zk_hat; *ze = 0.0; return; }
+  const bool atl = pfin(nkl) && xk <= nkl + tol;
+  const bool atu = pfin(nku) && xk >= nku - tol;
+  if (!atl && !atu) { *zk = 0.0; *ze = 0.0; return; }
+  double src = atl ? ls : us;
+  if (src == 0.0) {
+    if (atl) src = (pfin(okl) && fabs(nkl - okl) <= tol) ? 1.0 : 2.0;
+    else src = (pfin(oku) && fabs(nku - oku) <= tol) ? 1.0 : 2.0;
+  }
+  if (src == 2.0) { *zk = 0.0; *ze = zk_hat / al; }
+  else { *zk = zk_hat; *ze = 0.0; }
+}
+
+// Reverse replay of tape records [begin, end) (gpu_postsolve.cu:605-1046).
+// The records form a dependency chain in general, so one lane walks them in
+// order; every other postsolve step is data-parallel.
+__global__ void k_replay(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
+                         int32_t begin, int32_t end, int32_t primal, int32_t dual, double tol) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const bool has_orig = O.present != 0;
+  for (int32_t k = end - 1; k >= begin; --k) {
+    const int32_t type = T.types[k];
+    const int32_t i0 = T.index_starts[k], i1 = T.index_starts[k + 1] - 1;
+    const int32_t v0 = T.value_starts[k], v1 = T.value_starts[k + 1] - 1;
+    const int32_t* I = T.indices;
+    const double* V = T.vals;
+    if (type == 3) {  // DuplicateColumn
+      const int32_t from = I[i0], to = I[i0 + 1];
+      const double r = V[v0];
+      if (primal) {
+        double xf = 0.0, xt = x[to];
+        recover_dup_col(x[to], z ? z[to] : 0.0, r, V[v0 + 1], V[v0 + 2], V[v0 + 3], V[v0 + 4], tol, &xf, &xt);
+        x[from] = xf;
+        x[to] = xt;
+      }
+      if (dual && z) { z[from] = r * z[to]; if (zret) zret[from] = 1; }
+    } else if (type == 13) {  // DoubletonEquation
+      const int32_t ec = I[i0], kc = I[i0 + 1], row = I[i0 + 2];
+      const double ecoef = V[v0], kcoef = V[v0 + 1], rhs = V[v0 + 2], eobj = V[v0 + 9];
+      const double al = -kcoef / ecoef, be = rhs / ecoef;
+      if (primal) x[ec] = be + al * x[kc];
+      if (dual && z && y && has_orig) {
+        const double ls = v1 >= v0 + 10 ? V[v0 + 10] : 0.0;
+        const double us = v1 >= v0 + 11 ? V[v0 + 11] : 0.0;
+        double zk = 0.0, ze = 0.0;
+        dbl_recover_z(x[kc], z[kc], al, V[v0 + 5], V[v0 + 6], V[v0 + 7], V[v0 + 8], ls, us, tol, &zk, &ze);
+        const bool payload = i1 >= i0 + 3 && v1 >= v0 + 11;
+        double so = 0.0;
+        double yrow;
+        if (payload) {
+          for (int32_t t = 0; t < I[i0 + 3]; ++t) so += V[v0 + 12 + t] * y[I[i0 + 4 + t]];
+          yrow = (eobj - so - ze) / ecoef;
+        } else {
+          for (int32_t q = O.at_ptr[ec]; q < O.at_ptr[ec + 1]; ++q)
+            if (O.at_idx[q] != row) so += O.at_val[q] * y[O.at_idx[q]];
+          yrow = (O.c[ec] - so - ze) / ecoef;
+        }
+        y[row] = yrow;
+        z[kc] = zk;
+        z[ec] = ze;
+        if (zret) { zret[kc] = 1; zret[ec] = 1; }
+      } else if (dual) {
+        if (z) { z[ec] = 0.0; if (zret) zret[ec] = 1; }
+        if (y && row >= 0) y[row] = eobj / ecoef;
+      }
+    } else if (type == 2) {  // SubCol
+      const int32_t ec = I[i0], row = I[i0 + 1], sc = I[i0 + 2];
+      const double piv = V[v0], rhs = V[v0 + 1], eobj = V[v0 + 4];
+      const bool row_deleted = V[v0 + 5] != 0.0;
+      double part = 0.0;
+      for (int32_t t = 0; t < sc; ++t) part += V[v0 + 6 + t] * x[I[i0 + 3 + t]];
+      if (primal) x[ec] = (rhs - part) / piv;
+      if (dual && y && row >= 0) {
+        if (row_deleted) y[row] = eobj / piv; else y[row] += eobj / piv;
+      }
+      if (dual && z) {
+        z[ec] = (row_deleted || !y) ? 0.0 : eobj - piv * y[row];
+        if (zret) zret[ec] = 1;
+      }
+    } else if (type == 0) {  // FixedCol
+      const int32_t col = I[i0];
+      if (primal) x[col] = V[v0];
+      if (dual && z) {
+        double zj = V[v0 + 1];
+        for (int32_t pos = i0 + 1; pos <= i1; ++pos) {
+          const double coef = V[v0 + 2 + (pos - (i0 + 1))];
+          if (y) zj -= coef * y[I[pos]];
+        }
+        z[col] = zj;
+        if (zret) zret[col] = 1;
+      }
+    } else if (type == 1) {  // FixedColInf
+      const bool pos_inf = I[i0] > 0;
+      const int32_t col = I[i0 + 1];
+      const int32_t nrows = static_cast<int32_t>(llround(V[v0]));
+      double ext = V[v0 + 1];
+      int32_t ip = i0 + 2, vp = v0 + 2;
+      if (primal) {
+        for (int32_t r = 0; r < nrows; ++r) {
+          const int32_t len = I[ip];
+          double side = V[vp], coef = 0.0;
+          for (int32_t t = 0; t < len; ++t) {
+            const int32_t rc = I[ip + 1 + t];
+            const double rv = V[vp + 1 + t];
+            if (rc == col) coef = rv; else side -= rv * x[rc];
+          }
+          const double cand = side / coef;
+          ext = pos_inf ? fmax(ext, cand) : fmin(ext, cand);
+          ip += len + 1;
+          vp += len + 1;
+        }
+        x[col] = ext;
+      }
+      if (dual && z) { z[col] = 0.0; if (zret) zret[col] = 1; }
+    } else if (type == 14) {  // FmeCol
+      const int32_t K = I[i0], ec = I[i0 + 1];
+      const int32_t L = static_cast<int32_t>(llround(V[v0]));
+      int32_t cc = i0 + 3 + K, vp = v0 + 1;
+      double lo = -INFINITY, hi = INFINITY;
+      bool hl = false, hh = false;
+      if (primal) {
+        for (int32_t s = 0; s < L; ++s) {
+          const double orient = V[vp];
+          const int32_t len = static_cast<int32_t>(llround(V[vp + 1]));
+          double cand = V[vp + 2];
+          for (int32_t t = 0; t < len; ++t) cand += V[vp + 3 + t] * x[I[cc++]];
+          if (orient > 0.0) { lo = fmax(lo, cand); hl = true; }
+          else { hi = fmin(hi, cand); hh = true; }
+          vp += 3 + len;
+        }
+        x[ec] = hl ? lo : (hh ? hi : 0.0);
+      }
+      if (dual && z) { z[ec] = 0.0; if (zret) zret[ec] = 1; }
+      if (dual && y) for (int32_t t = 0; t < K; ++t) y[I[i0 + 2 + t]] = 0.0;
+    } else if (dual && type == 10) {  // EqToIneq
+      if (y) y[I[i0]] += V[v0];
+    } else if (dual && type == 4 && i1 >= i0 + 1 && v1 >= v0 + 4) {  // DuplicateRow
+      const int32_t kr = I[i0], dr = I[i0 + 1];
+      const double r = V[v0], kal = V[v0 + 1], kau = V[v0 + 2], dal = V[v0 + 3], dau = V[v0 + 4];
+      if (fabs(r) <= tol || !y) continue;
+      const double a = dal * r, b = dau * r;
+      const double dsl = r >= 0.0 ? a : b, dsu = r >= 0.0 ? b : a;
+      if (dsl <= kal + tol && kau <= dsu + tol) { y[dr] = 0.0; continue; }
+      if (has_orig) {
+        double ap = 0.0, ad = 0.0;
+        for (int32_t q = O.a_ptr[kr]; q < O.a_ptr[kr + 1]; ++q) ap += O.a_val[q] * x[O.a_idx[q]];
+        for (int32_t q = O.a_ptr[dr]; q < O.a_ptr[dr + 1]; ++q) ad += O.a_val[q] * x[O.a_idx[q]];
+        const uint8_t ks = row_state(ap, kal, kau, tol), ds = row_state(ad, dal, dau, tol);
+        double ydl, ydh, ypl, yph;
+        state_interval(ds, &ydl, &ydh);
+        deleted_dual_interval(y[kr], r, ks, &ypl, &yph);
+        const double sl = fmax(ydl, ypl), sh = fmin(ydh, yph);
+        if (sl <= sh + tol) {
+          const double ydn = clamp_inf(0.0, sl, sh);
+          y[kr] = y[kr] - ydn / r;
+          y[dr] = ydn;
+          continue;
+        }
+        double ypd = y[kr], ydd = 0.0;
+        if (doc_dual_split(y[kr], r, kal, kau, dsl, dsu, tol, &ypd, &ydd)) { y[kr] = ypd; y[dr] = ydd; }
+      }
+    } else if (dual && type == 5 && has_orig && i1 >= i0 + 1 && v1 >= v0 + 2) {  // DeletedRow
+      const int32_t row = I[i0], col = I[i0 + 1];
+      const double oal = V[v0], oau = V[v0 + 1], aij = V[v0 + 2];
+      if (fabs(aij) <= tol) continue;
+      double gamma = O.c[col];
+      for (int32_t q = O.at_ptr[col]; q < O.at_ptr[col + 1]; ++q)
+        if (O.at_idx[q] != row && y) gamma -= O.at_val[q] * y[O.at_idx[q]];
+      const double xj = x[col], zbar = z[col];
+      const double act = aij * xj;
+      const bool ral = pfin(oal) && act <= oal + tol;
+      const bool rau = pfin(oau) && act >= oau - tol;
+      double izl = -INFINITY, izh = INFINITY;
+      if (!(ral && rau)) {
+        if (ral) { if (aij > 0.0) izh = gamma; else izl = gamma; }
+        else if (rau) { if (aij > 0.0) izl = gamma; else izh = gamma; }
+        else { izl = gamma; izh = gamma; }
+      }
+      const bool atl = at_lower(xj, O.l[col], tol), atu = at_upper(xj, O.u[col], tol);
+      double zl = izl, zh = izh;
+      if (!atl && !atu) {
+        if ((pfin(izl) && izl > tol) || (pfin(izh) && izh < -tol)) { zl = gamma; zh = gamma; }
+        else { zl = 0.0; zh = 0.0; }
+      } else if (atl && !atu) {
+        zl = fmax(zl, 0.0);
+      } else if (!atl && atu) {
+        zh = fmin(zh, 0.0);
+      }
+      if (pfin(zl) && pfin(zh) && zl > zh + tol) continue;
+      const double zn = clamp_inf(zbar, zl, zh);
+      z[col] = zn;
+      if (y) y[row] = (gamma - zn) / aij;
+      if (zret) zret[col] = 1;
+    } else if (dual && type == 12 && has_orig && i1 >= i0 + 1) {  // BoundChangeTheRow
+      const int32_t col = I[i0], row = I[i0 + 1];
+      if (zret && !zret[col]) continue;
+      const double ol = V[v0], ou = V[v0 + 1], nl = V[v0 + 2], nu = V[v0 + 3];
+      const bool lc = nl > ol + tol, uc = nu < ou - tol;
+      if (lc == uc) continue;
+      const double implied = lc ? nl : nu;
+      const double other = lc ? ou : ol;
+      const bool other_is_lower = !lc;
+      const double xj = x[col], zj = z[col];
+      if (pfin(other) && fabs(xj - other) <= tol) {
+        const bool ok = other_is_lower ? (zj >= -tol) : (zj <= tol);
+        if (ok) continue;
+      }
+      if (fabs(xj - implied) > tol) continue;
+      double aij = 0.0;
+      for (int32_t q = O.a_ptr[row]; q < O.a_ptr[row + 1]; ++q)
+        if (O.a_idx[q] == col) { aij = O.a_val[q]; break; }
+      if (fabs(aij) <= tol) continue;
+      if (y) y[row] += zj / aij;
+      for (int32_t q = O.a_ptr[row]; q < O.a_ptr[row + 1]; ++q) {
+        const int32_t kc = O.a_idx[q];
+        if (kc == col || (zret && !zret[kc])) continue;
+        z[kc] -= (O.a_val[q] / aij) * zj;
+      }
+      z[col] = 0.0;
+      if (zret) zret[col] = 1;
+    }
+  }
+}
+
+// gpu_postsolve.cu:162-180, 568-603
+__global__ void k_project_duals(gp_orig_dev O, const uint8_t* prot, const double* x, const double* y, double* z,
+                                int32_t n, double tol) {
+  const int32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n || (prot && prot[col])) return;
+  double aty = 0.0;
+  for (int32_t q = O.at_ptr[col]; q < O.at_ptr[col + 1]; ++q) aty += O.at_val[q] * y[O.at_idx[q]];
+  const double r = O.c[col] - aty;
+  const double xj = x[col], lj = O.l[col], uj = O.u[col];
+  const bool atl = at_lower(xj, lj, tol), atu = at_upper(xj, uj, tol);
+  double zj;
+  if (atl && atu) zj = r;
+  else if (!atl && !atu) zj = 0.0;
+  else if (atl) zj = fmax(r, 0.0);
+  else zj = fmin(r, 0.0);
+  const double eps = fmin(tol, 1.0e-12);
+  if (zj > eps && !isfinite(lj)) zj = 0.0;
+  else if (zj < -eps && !isfinite(uj)) zj = 0.0;
+  z[col] = zj;
+}
+
+// Columns whose dual the replay determines exactly (gpu_postsolve.cu:1370-1402).
+__global__ void k_protect(gp_tape_dev T, int32_t n, uint8_t* prot) {
+  const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= T.count) return;
+  const int32_t i0 = T.index_starts[k];
+  const int32_t t = T.types[k];
+  int32_t a = -1, b = -1;
+  if (t == 2) a = T.indices[i0];
+  else if (t == 13) { a = T.indices[i0]; b = T.indices[i0 + 1]; }
+  else if (t == 1) a = T.indices[i0 + 1];
+  else if (t == 3) a = T.indices[i0];
+  if (a >= 0 && a < n) prot[a] = 1;
+  if (b >= 0 && b < n) prot[b] = 1;
+}
+
+}  // namespace
+
+#define GPK_LAUNCH(kernel, n, ...)                                                      \
+  do {                                                                                  \
+    if ((n) > 0) hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kBlock), 0, stream, __VA_ARGS__); \
+    return hipGetLastError();                                                           \
+  } while (0)
+
+extern "C" {
+
+hipError_t gpk_removed_fixed_mask(int32_t n, const uint8_t* keep_col, const double* l, const double* u,
+                                  uint8_t* mask, hipStream_t stream) {
+  GPK_LAUNCH(k_removed_fixed_mask, n, n, keep_col, l, u, mask);
+}
+hipError_t gpk_pack_fixed_records(int32_t n, const uint8_t* mask, const int32_t* prefix, const int32_t* red2org,
+                                  const double* l, const double* u, int32_t* out_idx, double* out_val,
+                                  hipStream_t stream) {
+  GPK_LAUNCH(k_pack_fixed_records, n, n, mask, prefix, red2org, l, u, out_idx, out_val);
+}
+hipError_t gpk_map_tape_indices(int32_t count, int32_t* idx, const uint8_t* kind, const int32_t* row_red2org,
+                                int32_t m, const int32_t* col_red2org, int32_t n, hipStream_t stream) {
+  GPK_LAUNCH(k_map_tape_indices, count, count, idx, kind, row_red2org, m, col_red2org, n);
+}
+hipError_t gpk_scatter(const double* src, const int32_t* red2org, int32_t n_red, double* dst, hipStream_t stream) {
+  GPK_LAUNCH(k_scatter, n_red, src, red2org, n_red, dst);
+}
+hipError_t gpk_mark(const int32_t* red2org, int32_t n_red, uint8_t* mask, hipStream_t stream) {
+  GPK_LAUNCH(k_mark, n_red, red2org, n_red, mask);
+}
+hipError_t gpk_restore_fixed(const int32_t* idx, const double* val, int32_t count, double* x, hipStream_t stream) {
+  GPK_LAUNCH(k_restore_fixed, count, idx, val, count, x);
+}
+hipError_t gpk_replay(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, double* z, uint8_t* zret,
+                      int32_t begin, int32_t end, int32_t primal, int32_t dual, double tol, hipStream_t stream) {
+  if (end <= begin) return hipSuccess;
+  hipLaunchKernelGGL(k_replay, dim3(1), dim3(64), 0, stream, tape, orig, x, y, z, zret, begin, end, primal, dual, tol);
+  return hipGetLastError();
+}
+hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipStream_t stream) {
+  GPK_LAUNCH(k_protect, tape.count, tape, n, prot);
+}
+hipError_t gpk_project_duals(gp_orig_dev orig, const uint8_t* prot, const double* x, const double* y, double* z,
+                             int32_t n, double tol, hipStream_t stream) {
+  GPK_LAUNCH(k_project_duals, n, orig, prot, x, y, z, n, tol);
+}
+
+}  // extern "C"

@@ -1,0 +1,667 @@
+// Row-side reduction kernels: empty rows, singleton rows, activity-redundant
+// rows, implied (dominated) bound tightening, FixedCol tape packing and
+// duplicate rows.  Every floating-point reduction runs in one fixed order
+// (CSR order along a row, AT order along a column) so results do not depend
+// on scheduling.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+
+#include "gp_device.hpp"
+#include "gp_kernels.h"
+
+using namespace gpdev;
+
+namespace {
+
+// ------------------------------------------------------------ generic ----
+__global__ void k_row_shift(gpk_csr A, const uint8_t* keep_row, const uint8_t* col_flag, const double* col_val,
+                            double abs_min, double* AL, double* AU) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.rows) return;
+  double shift = 0.0;
+  if (keep_row == nullptr || keep_row[i] != 0) {
+    for (int32_t p = A.ptr[i]; p < A.ptr[i + 1]; ++p) {
+      const int32_t j = A.idx[p];
+      if (!col_flag[j]) continue;
+      const double a = A.val[p];
+      if (abs_min >= 0.0 && !(fabs(a) > abs_min)) continue;
+      shift += a * col_val[j];
+    }
+  }
+  AL[i] -= shift;
+  AU[i] -= shift;
+}
+
+__global__ void k_pack_f64(const uint8_t* mask, const double* vals, const int32_t* prefix, int32_t n, double* out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && mask[i]) out[prefix[i]] = vals[i];
+}
+
+__global__ void k_pack_index(const uint8_t* mask, const int32_t* prefix, int32_t n, int32_t* out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && mask[i]) out[prefix[i]] = i;
+}
+
+// ------------------------------------------------------- empty rows ----
+// flags[0]: live empty row infeasible at 0; flags[1]: removable empty row.
+__global__ void k_empty_rows(int32_t m, gpk_plan p, const int32_t* row_nnz, double tol, int32_t apply,
+                             int32_t* flags) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !p.keep_row[i] || row_nnz[i] != 0) return;
+  if (p.AL[i] <= tol && p.AU[i] >= -tol) {
+    if (apply) p.keep_row[i] = 0; else atomicOr(&flags[1], 1);
+  } else if (!apply) {
+    atomicOr(&flags[0], 1);
+  }
+}
+
+// --------------------------------------------------- singleton rows ----
+__global__ void k_singleton_rows_bounds(int32_t m, int32_t n, gpk_plan p, const uint8_t* mask, const int32_t* col,
+                                        const double* val, double ztol, double* cl, double* cu, uint8_t* remove) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  remove[i] = 0;
+  if (!p.keep_row[i] || !mask[i]) return;
+  const int32_t j = col[i];
+  const double a = val[i];
+  if (j < 0 || j >= n || !(fabs(a) > ztol)) return;
+  const double lo = p.AL[i], hi = p.AU[i];
+  atomic_max_f64(&cl[j], a > 0.0 ? lo / a : hi / a);
+  atomic_min_f64(&cu[j], a > 0.0 ? hi / a : lo / a);
+  remove[i] = 1;
+}
+
+// flags[0] infeasible, flags[1] some row removed, flags[2] some bound changed
+__global__ void k_singleton_rows_check(int32_t m, int32_t n, gpk_plan p, const double* cl, const double* cu,
+                                       const uint8_t* remove, double tol, int32_t* flags) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    if (cl[t] > cu[t] + tol) atomicOr(&flags[0], 1);
+    if (cl[t] != p.l[t] || cu[t] != p.u[t]) atomicOr(&flags[2], 1);
+  }
+  if (t < m && remove[t]) atomicOr(&flags[1], 1);
+}
+
+__global__ void k_singleton_rows_commit(int32_t m, int32_t n, gpk_plan p, const double* cl, const double* cu,
+                                        const uint8_t* remove) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < m && remove[t]) p.keep_row[t] = 0;
+  if (t < n) {
+    p.l[t] = cl[t];
+    p.u[t] = cu[t];
+  }
+}
+
+// -------------------------------------------------- redundant rows ----
+// flags[0] infeasible, flags[1] any action.
+__global__ void k_redundant_rows_classify(gpk_csr A, gpk_plan p, const int32_t* row_nnz, double tol, uint8_t* full,
+                                          uint8_t* dlo, uint8_t* dhi, int32_t* flags) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.rows) return;
+  full[i] = 0;
+  dlo[i] = 0;
+  dhi[i] = 0;
+  if (!p.keep_row[i] || row_nnz[i] <= 1) return;
+  const double lo = p.AL[i], hi = p.AU[i];
+  const bool lf = isfinite(lo), hf = isfinite(hi);
+  if (lf && hf && fabs(hi - lo) <= tol) return;
+  double rmin = 0.0, rmax = 0.0;
+  for (int32_t q = A.ptr[i]; q < A.ptr[i + 1]; ++q) {
+    const int32_t j = A.idx[q];
+    const double a = A.val[q];
+    if (a >= 0.0) {
+      rmin += a * p.l[j];
+      rmax += a * p.u[j];
+    } else {
+      rmin += a * p.u[j];
+      rmax += a * p.l[j];
+    }
+  }
+  if ((lf && rmax < lo - tol) || (hf && rmin > hi + tol)) {
+    atomicOr(&flags[0], 1);
+    return;
+  }
+  const bool lo_imp = !lf || rmin >= lo - tol;
+  const bool hi_imp = !hf || rmax <= hi + tol;
+  if (lo_imp && hi_imp) {
+    full[i] = 1;
+    atomicOr(&flags[1], 1);
+  } else if (lf && lo_imp) {
+    dlo[i] = 1;
+    atomicOr(&flags[1], 1);
+  } else if (hf && hi_imp) {
+    dhi[i] = 1;
+    atomicOr(&flags[1], 1);
+  }
+}
+
+__global__ void k_redundant_rows_apply(int32_t m, gpk_plan p, const uint8_t* full, const uint8_t* dlo,
+                                       const uint8_t* dhi) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  if (full[i]) p.keep_row[i] = 0;
+  if (dlo[i]) p.AL[i] = -INFINITY;
+  if (dhi[i]) p.AU[i] = INFINITY;
+}
+
+// ------------------------------------------------- implied bounds ----
+__device__ inline void term_interval(double a, double lj, double uj, double* tmin, double* tmax) {
+  if (a >= 0.0) {
+    *tmin = a * lj;
+    *tmax = a * uj;
+  } else {
+    *tmin = a * uj;
+    *tmax = a * lj;
+  }
+}
+__device__ inline double resid_min(double s, int32_t ninf, double tmin) {
+  if (isfinite(tmin)) return ninf == 0 ? s - tmin : -INFINITY;
+  if (tmin < 0.0) return ninf > 1 ? -INFINITY : s;
+  return s;
+}
+__device__ inline double resid_max(double s, int32_t pinf, double tmax) {
+  if (isfinite(tmax)) return pinf == 0 ? s - tmax : INFINITY;
+  if (tmax > 0.0) return pinf > 1 ? INFINITY : s;
+  return s;
+}
+__device__ inline void implied_from_row(double a, double lo, double hi, double rs_min, double rs_max, double* il,
+                                        double* iu) {
+  *il = -INFINITY;
+  *iu = INFINITY;
+  if (a > 0.0) {
+    if (isfinite(lo) && isfinite(rs_max)) *il = (lo - rs_max) / a;
+    if (isfinite(hi) && isfinite(rs_min)) *iu = (hi - rs_min) / a;
+  } else {
+    if (isfinite(hi) && isfinite(rs_min)) *il = (hi - rs_min) / a;
+    if (isfinite(lo) && isfinite(rs_max)) *iu = (lo - rs_max) / a;
+  }
+}
+
+// Candidate bounds per column, pulled along AT (no atomics): the max (min)
+// over the column's live rows of the implied lower (upper) bound, seeded with
+// the current bound.  Same values as the reference's per-entry CAS push.
+__global__ void k_implied_candidates(gpk_csr AT, gpk_plan p, const int32_t* row_nnz, const double* rmin,
+                                     const double* rmax, const int32_t* rni, const int32_t* rpi, double ztol,
+                                     double* cand_l, double* cand_u) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= AT.rows) return;
+  const double ol = p.l[j], ou = p.u[j];
+  double cl = ol, cu = ou;
+  for (int32_t q = AT.ptr[j]; q < AT.ptr[j + 1]; ++q) {
+    const int32_t i = AT.idx[q];
+    if (!p.keep_row[i] || row_nnz[i] <= 1) continue;
+    const double a = AT.val[q];
+    if (fabs(a) <= ztol) continue;
+    double tmin, tmax;
+    term_interval(a, ol, ou, &tmin, &tmax);
+    const double rs_min = resid_min(rmin[i], rni[i], tmin);
+    const double rs_max = resid_max(rmax[i], rpi[i], tmax);
+    double il, iu;
+    implied_from_row(a, p.AL[i], p.AU[i], rs_min, rs_max, &il, &iu);
+    double nl = ol, nu = ou;
+    if (isfinite(il)) nl = fmax(nl, il);
+    if (isfinite(iu)) nu = fmin(nu, iu);
+    if (nl > ol && !(cl >= nl)) cl = nl;
+    if (nu < ou && !(cu <= nu)) cu = nu;
+  }
+  cand_l[j] = cl;
+  cand_u[j] = cu;
+}
+
+// rule_implied_variable_bounds.cu:513-649; flags[0] infeasible, [1] tightened, [2] fixed.
+__global__ void k_implied_finalize(int32_t n, gpk_plan p, const double* ref_l, const double* ref_u,
+                                   const double* cand_l, const double* cand_u, const double* vmax, int32_t preserve,
+                                   double ztol, double ftol, double* fl, double* fu, uint8_t* lch, uint8_t* uch,
+                                   uint8_t* fixed, double* fixed_val, int32_t* flags) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const double lj = p.l[j], uj = p.u[j];
+  double cl = cand_l[j], cu = cand_u[j];
+  const double vm = vmax[j];
+  double nl = lj, nu = uj;
+  uint8_t lc = 0, uc = 0, fx = 0;
+  double fat = 0.0;
+  double el = lj, eu = uj;
+  if (isfinite(cl)) el = fmax(el, cl);
+  if (isfinite(cu)) eu = fmin(eu, cu);
+  if (el > eu + ftol) {
+    atomicOr(&flags[0], 1);
+    return;
+  }
+  const bool was_free = !isfinite(ref_l[j]) && !isfinite(ref_u[j]);
+  const bool box = isfinite(cl) && isfinite(cu);
+  const bool zero_cost = p.c != nullptr && fabs(p.c[j]) <= ztol;
+  const bool nearly_fixed = box && (cu <= cl || (cu - cl) * vm <= ftol);
+  if (preserve && was_free && box && zero_cost && !nearly_fixed) {
+    cl = lj;
+    cu = uj;
+  }
+  if (isfinite(cl) && cl > lj && isfinite(uj)) {
+    if (cl >= uj + ftol) {
+      atomicOr(&flags[0], 1);
+      return;
+    }
+    if (cl >= uj || (uj - cl) * vm <= ftol) {
+      fx = 1;
+      fat = uj;
+      nl = uj;
+      nu = uj;
+    }
+  }
+  if (!fx && isfinite(cl) && cl > lj) {
+    const bool tighten = !isfinite(lj) || ((cl - lj > ftol * 1.0e4) && (cl - lj > 1.0e-2 * fabs(lj)));
+    if (tighten) {
+      if (cl != nearbyint(cl)) cl -= 0.5 * ftol * fabs(cl);
+      nl = cl;
+      lc = 1;
+    }
+  }
+  if (!fx && isfinite(cu) && cu < uj && isfinite(nl)) {
+    if (cu <= nl - ftol) {
+      atomicOr(&flags[0], 1);
+      return;
+    }
+    if (cu <= nl || (cu - nl) * vm <= ftol) {
+      fx = 1;
+      fat = nl;
+      nu = nl;
+    }
+  }
+  if (!fx && isfinite(cu) && cu < uj) {
+    const bool tighten = !isfinite(uj) || ((uj - cu > ftol * 1.0e4) && (uj - cu > 1.0e-2 * fabs(uj)));
+    if (tighten) {
+      if (cu != nearbyint(cu)) cu += 0.5 * ftol * fabs(cu);
+      nu = cu;
+      uc = 1;
+    }
+  }
+  if (fx) {
+    lc = 0;
+    uc = 0;
+    atomicOr(&flags[2], 1);
+  }
+  if (lc || uc) atomicOr(&flags[1], 1);
+  fl[j] = nl;
+  fu[j] = nu;
+  lch[j] = lc;
+  uch[j] = uc;
+  fixed[j] = fx;
+  fixed_val[j] = fat;
+}
+
+// rule_implied_variable_bounds.cu:668-750 plus the per-column tape record count.
+__global__ void k_implied_support(gpk_csr AT, gpk_plan p, const int32_t* row_nnz, const double* rmin,
+                                  const double* rmax, const int32_t* rni, const int32_t* rpi, const uint8_t* lch,
+                                  const uint8_t* uch, const double* fl, const double* fu, double ztol, double tol,
+                                  double stol, int32_t* sup_l, int32_t* sup_u, int32_t* rec_count) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= AT.rows) return;
+  int32_t sl = INT_MAX, su = INT_MAX;
+  const bool need_l = lch[j] != 0, need_u = uch[j] != 0;
+  if (need_l || need_u) {
+    const double ol = p.l[j], ou = p.u[j];
+    const double cl = fl[j], cu = fu[j];
+    for (int32_t q = AT.ptr[j]; q < AT.ptr[j + 1]; ++q) {
+      const int32_t i = AT.idx[q];
+      if (!p.keep_row[i] || row_nnz[i] <= 1) continue;
+      const double a = AT.val[q];
+      if (fabs(a) <= ztol) continue;
+      double tmin, tmax;
+      term_interval(a, ol, ou, &tmin, &tmax);
+      const double rs_min = resid_min(rmin[i], rni[i], tmin);
+      const double rs_max = resid_max(rmax[i], rpi[i], tmax);
+      double il, iu;
+      implied_from_row(a, p.AL[i], p.AU[i], rs_min, rs_max, &il, &iu);
+      if (need_l && cl > ol + tol && isfinite(il) && fabs(il - cl) <= stol) sl = min(sl, i);
+      if (need_u && cu < ou - tol && isfinite(iu) && fabs(iu - cu) <= stol) su = min(su, i);
+    }
+  }
+  sup_l[j] = sl;
+  sup_u[j] = su;
+  rec_count[j] = (need_l ? 1 : 0) + (need_u ? 1 : 0);
+}
+
+// rule_implied_variable_bounds.cu:771-804: records {col, row, old_l, old_u, new_l, new_u}.
+__global__ void k_implied_pack_tape(int32_t n, gpk_plan p, const uint8_t* lch, const uint8_t* uch,
+                                    const int32_t* sup_l, const int32_t* sup_u, const double* fl, const double* fu,
+                                    const int32_t* prefix, int32_t* rcol, int32_t* rrow, double* rvals) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  int32_t out = prefix[j];
+  double cur_l = p.l[j];
+  const double cur_u = p.u[j];
+  if (lch[j]) {
+    rcol[out] = j;
+    rrow[out] = sup_l[j];
+    rvals[4 * out + 0] = cur_l;
+    rvals[4 * out + 1] = cur_u;
+    rvals[4 * out + 2] = fl[j];
+    rvals[4 * out + 3] = cur_u;
+    cur_l = fl[j];
+    ++out;
+  }
+  if (uch[j]) {
+    rcol[out] = j;
+    rrow[out] = sup_u[j];
+    rvals[4 * out + 0] = cur_l;
+    rvals[4 * out + 1] = cur_u;
+    rvals[4 * out + 2] = cur_l;
+    rvals[4 * out + 3] = fu[j];
+  }
+}
+
+__global__ void k_implied_apply(int32_t n, gpk_plan p, const double* fl, const double* fu, const uint8_t* fixed) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  p.l[j] = fl[j];
+  p.u[j] = fu[j];
+  if (fixed[j]) p.keep_col[j] = 0;
+}
+
+// flags[3] infeasible empty row, flags[4] removed empty row.
+__global__ void k_remove_empty_rows_after_fix(gpk_csr A, gpk_plan p, double ftol, int32_t* flags) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.rows || !p.keep_row[i]) return;
+  int32_t cnt = 0;
+  for (int32_t q = A.ptr[i]; q < A.ptr[i + 1]; ++q) cnt += p.keep_col[A.idx[q]] ? 1 : 0;
+  if (cnt != 0) return;
+  if (p.AL[i] <= ftol && p.AU[i] >= -ftol) {
+    p.keep_row[i] = 0;
+    atomicOr(&flags[4], 1);
+  } else {
+    atomicOr(&flags[3], 1);
+  }
+}
+
+// ------------------------------------------------ FixedCol packing ----
+__global__ void k_fixed_col_counts(gpk_csr AT, const uint8_t* fixed, const uint8_t* excluded,
+                                   const uint8_t* keep_row, uint8_t* selected, int32_t* active) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= AT.rows) return;
+  const bool sel = fixed[j] && (excluded == nullptr || !excluded[j]);
+  selected[j] = sel ? 1 : 0;
+  int32_t cnt = 0;
+  if (sel)
+    for (int32_t q = AT.ptr[j]; q < AT.ptr[j + 1]; ++q) cnt += keep_row[AT.idx[q]] ? 1 : 0;
+  active[j] = cnt;
+}
+
+__global__ void k_fixed_col_pack(gpk_csr AT, const uint8_t* selected, const int32_t* sel_prefix,
+                                 const int32_t* nnz_prefix, const double* fixed_val, const double* c,
+                                 const uint8_t* keep_row, int32_t* out_col, double* out_v, double* out_c,
+                                 int32_t* out_start, int32_t* out_rows, double* out_coef) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= AT.rows || !selected[j]) return;
+  const int32_t r = sel_prefix[j];
+  out_col[r] = j;
+  out_start[r] = nnz_prefix[j];
+  out_v[r] = fixed_val[j];
+  out_c[r] = c ? c[j] : 0.0;
+  int32_t o = nnz_prefix[j];
+  for (int32_t q = AT.ptr[j]; q < AT.ptr[j + 1]; ++q) {
+    if (!keep_row[AT.idx[q]]) continue;
+    out_rows[o] = AT.idx[q];
+    out_coef[o] = AT.val[q];
+    ++o;
+  }
+}
+
+// --------------------------------------------------- duplicate rows ----
+constexpr uint64_t kFnvOffset = 0xcbf29ce484222325ULL;
+constexpr uint64_t kFnvPrime = 0x100000001b3ULL;
+__device__ inline uint64_t fnv_mix(uint64_t h, uint64_t x) { return (h ^ x) * kFnvPrime; }
+__device__ inline uint64_t dbits(double v) { return static_cast<uint64_t>(__double_as_longlong(v)); }
+
+// rule_duplicate_rows.cu:78-114 (sequential FNV over len, col, a/a_first).
+__global__ void k_dup_row_hash(gpk_csr A, const uint8_t* keep_row, double ctol, uint64_t* hash) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.rows) return;
+  if (!keep_row[i]) {
+    hash[i] = static_cast<uint64_t>(i);
+    return;
+  }
+  const int32_t b = A.ptr[i], e = A.ptr[i + 1];
+  if (e - b <= 0 || fabs(A.val[b]) <= ctol) {
+    hash[i] = fnv_mix(kFnvOffset, static_cast<uint64_t>(i));
+    return;
+  }
+  const double piv = A.val[b];
+  uint64_t h = fnv_mix(kFnvOffset, static_cast<uint64_t>(e - b));
+  for (int32_t q = b; q < e; ++q) {
+    h = fnv_mix(h, static_cast<uint64_t>(static_cast<int64_t>(A.idx[q])));
+    h = fnv_mix(h, dbits(A.val[q] / piv));
+  }
+  hash[i] = h;
+}
+
+__device__ bool dup_row_ratio(const gpk_csr& A, int32_t i, int32_t k, double ctol, double* r_out) {
+  const int32_t bi = A.ptr[i], li = A.ptr[i + 1] - bi;
+  const int32_t bk = A.ptr[k], lk = A.ptr[k + 1] - bk;
+  if (li != lk) return false;
+  if (li <= 0) {
+    *r_out = 1.0;
+    return true;
+  }
+  double r = 0.0;
+  bool set = false;
+  for (int32_t o = 0; o < li; ++o) {
+    if (A.idx[bi + o] != A.idx[bk + o]) return false;
+    const double ai = A.val[bi + o], ak = A.val[bk + o];
+    if (!set) {
+      if (fabs(ak) <= ctol) return false;
+      r = ai / ak;
+      set = true;
+    }
+    if (fabs(ai - r * ak) > ctol) return false;
+  }
+  *r_out = r;
+  return set;
+}
+
+__device__ inline void scaled_interval(double lo, double hi, double r, double* sl, double* su) {
+  const double a = lo * r, b = hi * r;
+  if (r >= 0.0) {
+    *sl = a;
+    *su = b;
+  } else {
+    *sl = b;
+    *su = a;
+  }
+}
+
+// rule_duplicate_rows.cu:243-381, one thread per hash bucket (its first slot).
+// rec_vals[5*row..]: ratio, kept_old_AL, kept_old_AU, deleted_old_AL, deleted_old_AU.
+__global__ void k_dup_row_groups(gpk_csr A, gpk_plan p, const uint64_t* sh, const int32_t* srow, double ctol,
+                                 double itol, uint8_t* row_delete, int32_t* merge_to, int32_t* merge_step,
+                                 double* rec, int32_t* flags) {
+  const int32_t m = A.rows;
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m) return;
+  const uint64_t hs = sh[s];
+  if (s > 0 && sh[s - 1] == hs) return;
+  int32_t e = s;
+  while (e + 1 < m && sh[e + 1] == hs) ++e;
+  if (e <= s) return;
+  int32_t seed = -1;
+  bool seed_eq = false;
+  for (int32_t a = s; a <= e; ++a) {
+    const int32_t row = srow[a];
+    if (!p.keep_row[row]) continue;
+    const double lo = p.AL[row], hi = p.AU[row];
+    const bool eq = isfinite(lo) && isfinite(hi) && fabs(lo - hi) <= itol;
+    if (seed < 0 || (eq && !seed_eq)) {
+      seed = row;
+      seed_eq = eq;
+    }
+  }
+  if (seed < 0) return;
+  int32_t rep = seed;
+  double rsl = p.AL[seed], rsu = p.AU[seed];
+  for (int32_t a = s; a <= e; ++a) {
+    const int32_t row = srow[a];
+    if (row == seed || !p.keep_row[row]) continue;
+    double r;
+    if (!dup_row_ratio(A, seed, row, ctol, &r)) continue;
+    double sl, su;
+    scaled_interval(p.AL[row], p.AU[row], r, &sl, &su);
+    if (rsl <= sl + itol && su <= rsu + itol) {
+      rep = row;
+      rsl = sl;
+      rsu = su;
+    }
+  }
+  double rl = p.AL[rep], ru = p.AU[rep];
+  bool peer = false;
+  int32_t step = 0;
+  for (int32_t a = s; a <= e; ++a) {
+    const int32_t row = srow[a];
+    if (row == rep || !p.keep_row[row]) continue;
+    double r;
+    if (!dup_row_ratio(A, rep, row, ctol, &r)) continue;
+    peer = true;
+    double sl, su;
+    scaled_interval(p.AL[row], p.AU[row], r, &sl, &su);
+    if (fmax(rl, sl) > fmin(ru, su) + itol) {
+      atomicOr(&flags[0], 1);
+      return;
+    }
+    ++step;
+    merge_to[row] = rep;
+    merge_step[row] = step;
+    rec[5 * static_cast<int64_t>(row) + 0] = r;
+    rec[5 * static_cast<int64_t>(row) + 1] = rl;
+    rec[5 * static_cast<int64_t>(row) + 2] = ru;
+    rec[5 * static_cast<int64_t>(row) + 3] = p.AL[row];
+    rec[5 * static_cast<int64_t>(row) + 4] = p.AU[row];
+    rl = fmax(rl, sl);
+    ru = fmin(ru, su);
+    row_delete[row] = 1;
+  }
+  if (peer) {
+    p.AL[rep] = rl;
+    p.AU[rep] = ru;
+  }
+}
+
+__global__ void k_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m && row_delete[i]) {
+    keep_row[i] = 0;
+    atomicAdd(count, 1);
+  }
+}
+
+}  // namespace
+
+#define GPK_LAUNCH(kernel, n, ...)                                                      \
+  do {                                                                                  \
+    if ((n) > 0) hipLaunchKernelGGL(kernel, dim3(grid_for(n)), dim3(kBlock), 0, stream, __VA_ARGS__); \
+    return hipGetLastError();                                                           \
+  } while (0)
+
+extern "C" {
+
+hipError_t gpk_row_shift(gpk_csr A, const uint8_t* keep_row, const uint8_t* col_flag, const double* col_val,
+                         double abs_min, double* AL, double* AU, hipStream_t stream) {
+  GPK_LAUNCH(k_row_shift, A.rows, A, keep_row, col_flag, col_val, abs_min, AL, AU);
+}
+hipError_t gpk_pack_f64(const uint8_t* mask, const double* vals, const int32_t* prefix, int32_t n, double* out,
+                        hipStream_t stream) {
+  GPK_LAUNCH(k_pack_f64, n, mask, vals, prefix, n, out);
+}
+hipError_t gpk_pack_index(const uint8_t* mask, const int32_t* prefix, int32_t n, int32_t* out, hipStream_t stream) {
+  GPK_LAUNCH(k_pack_index, n, mask, prefix, n, out);
+}
+hipError_t gpk_empty_rows(int32_t m, gpk_plan plan, const int32_t* row_nnz, double tol, int32_t apply, int32_t* flags,
+                          hipStream_t stream) {
+  GPK_LAUNCH(k_empty_rows, m, m, plan, row_nnz, tol, apply, flags);
+}
+hipError_t gpk_singleton_rows_bounds(int32_t m, int32_t n, gpk_plan plan, const uint8_t* mask, const int32_t* col,
+                                     const double* val, double zero_tol, double* cand_l, double* cand_u,
+                                     uint8_t* row_remove, hipStream_t stream) {
+  GPK_LAUNCH(k_singleton_rows_bounds, m, m, n, plan, mask, col, val, zero_tol, cand_l, cand_u, row_remove);
+}
+hipError_t gpk_singleton_rows_check(int32_t m, int32_t n, gpk_plan plan, const double* cand_l, const double* cand_u,
+                                    const uint8_t* row_remove, double bound_tol, int32_t* flags, hipStream_t stream) {
+  const int32_t mn = m > n ? m : n;
+  GPK_LAUNCH(k_singleton_rows_check, mn, m, n, plan, cand_l, cand_u, row_remove, bound_tol, flags);
+}
+hipError_t gpk_singleton_rows_commit(int32_t m, int32_t n, gpk_plan plan, const double* cand_l, const double* cand_u,
+                                     const uint8_t* row_remove, hipStream_t stream) {
+  const int32_t mn = m > n ? m : n;
+  GPK_LAUNCH(k_singleton_rows_commit, mn, m, n, plan, cand_l, cand_u, row_remove);
+}
+hipError_t gpk_redundant_rows_classify(gpk_csr A, gpk_plan plan, const int32_t* row_nnz, double tol, uint8_t* full,
+                                       uint8_t* dlo, uint8_t* dhi, int32_t* flags, hipStream_t stream) {
+  GPK_LAUNCH(k_redundant_rows_classify, A.rows, A, plan, row_nnz, tol, full, dlo, dhi, flags);
+}
+hipError_t gpk_redundant_rows_apply(int32_t m, gpk_plan plan, const uint8_t* full, const uint8_t* dlo,
+                                    const uint8_t* dhi, hipStream_t stream) {
+  GPK_LAUNCH(k_redundant_rows_apply, m, m, plan, full, dlo, dhi);
+}
+hipError_t gpk_implied_candidates(gpk_csr AT, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
+                                  const double* rmax, const int32_t* rni, const int32_t* rpi, double zero_tol,
+                                  double* cand_l, double* cand_u, hipStream_t stream) {
+  GPK_LAUNCH(k_implied_candidates, AT.rows, AT, plan, row_nnz, rmin, rmax, rni, rpi, zero_tol, cand_l, cand_u);
+}
+hipError_t gpk_implied_finalize(int32_t n, gpk_plan plan, const double* ref_l, const double* ref_u,
+                                const double* cand_l, const double* cand_u, const double* vmax,
+                                int32_t preserve_free_zero_cost, double zero_tol, double feas_tol, double* fin_l,
+                                double* fin_u, uint8_t* lch, uint8_t* uch, uint8_t* fixed, double* fixed_val,
+                                int32_t* flags, hipStream_t stream) {
+  GPK_LAUNCH(k_implied_finalize, n, n, plan, ref_l, ref_u, cand_l, cand_u, vmax, preserve_free_zero_cost, zero_tol,
+             feas_tol, fin_l, fin_u, lch, uch, fixed, fixed_val, flags);
+}
+hipError_t gpk_implied_support(gpk_csr AT, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
+                               const double* rmax, const int32_t* rni, const int32_t* rpi, const uint8_t* lch,
+                               const uint8_t* uch, const double* fin_l, const double* fin_u, double zero_tol,
+                               double tol, double support_tol, int32_t* support_l, int32_t* support_u,
+                               int32_t* rec_count, hipStream_t stream) {
+  GPK_LAUNCH(k_implied_support, AT.rows, AT, plan, row_nnz, rmin, rmax, rni, rpi, lch, uch, fin_l, fin_u, zero_tol,
+             tol, support_tol, support_l, support_u, rec_count);
+}
+hipError_t gpk_implied_pack_tape(int32_t n, gpk_plan plan, const uint8_t* lch, const uint8_t* uch,
+                                 const int32_t* support_l, const int32_t* support_u, const double* fin_l,
+                                 const double* fin_u, const int32_t* prefix, int32_t* rec_col, int32_t* rec_row,
+                                 double* rec_vals, hipStream_t stream) {
+  GPK_LAUNCH(k_implied_pack_tape, n, n, plan, lch, uch, support_l, support_u, fin_l, fin_u, prefix, rec_col, rec_row,
+             rec_vals);
+}
+hipError_t gpk_implied_apply(int32_t n, gpk_plan plan, const double* fin_l, const double* fin_u, const uint8_t* fixed,
+                             hipStream_t stream) {
+  GPK_LAUNCH(k_implied_apply, n, n, plan, fin_l, fin_u, fixed);
+}
+hipError_t gpk_remove_empty_rows_after_fix(gpk_csr A, gpk_plan plan, double feas_tol, int32_t* flags,
+                                           hipStream_t stream) {
+  GPK_LAUNCH(k_remove_empty_rows_after_fix, A.rows, A, plan, feas_tol, flags);
+}
+hipError_t gpk_fixed_col_counts(gpk_csr AT, const uint8_t* fixed, const uint8_t* excluded, const uint8_t* keep_row,
+                                uint8_t* selected, int32_t* active_nnz, hipStream_t stream) {
+  GPK_LAUNCH(k_fixed_col_counts, AT.rows, AT, fixed, excluded, keep_row, selected, active_nnz);
+}
+hipError_t gpk_fixed_col_pack(gpk_csr AT, const uint8_t* selected, const int32_t* sel_prefix,
+                              const int32_t* nnz_prefix, const double* fixed_val, const double* c,
+                              const uint8_t* keep_row, int32_t* out_col, double* out_v, double* out_c,
+                              int32_t* out_start, int32_t* out_rows, double* out_coef, hipStream_t stream) {
+  GPK_LAUNCH(k_fixed_col_pack, AT.rows, AT, selected, sel_prefix, nnz_prefix, fixed_val, c, keep_row, out_col, out_v,
+             out_c, out_start, out_rows, out_coef);
+}
+hipError_t gpk_dup_row_hash(gpk_csr A, const uint8_t* keep_row, double coeff_tol, uint64_t* hash,
+                            hipStream_t stream) {
+  GPK_LAUNCH(k_dup_row_hash, A.rows, A, keep_row, coeff_tol, hash);
+}
+hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_hash, const int32_t* sorted_rows,
+                              double coeff_tol, double interval_tol, uint8_t* row_delete, int32_t* merge_to,
+                              int32_t* merge_step, double* rec_vals, int32_t* flags, hipStream_t stream) {
+  GPK_LAUNCH(k_dup_row_groups, A.rows, A, plan, sorted_hash, sorted_rows, coeff_tol, interval_tol, row_delete,
+             merge_to, merge_step, rec_vals, flags);
+}
+hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count,
+                           hipStream_t stream) {
+  GPK_LAUNCH(k_delete_rows, m, m, keep_row, row_delete, count);
+}
+
+}  // extern "C"

@@ -1,0 +1,316 @@
+/*
+ * Internal C-ABI between the host C++ of libgpu_presolver_amd.so and its
+ * hand-written HIP kernels (gfx950).  The reference launches its CUDA
+ * kernels directly with <<<>>> from the rule files; this library keeps all
+ * device code behind these extern "C" launchers, each taking plain pointers,
+ * sizes and the stream to run on.  Every launcher is asynchronous on
+ * `stream` and returns hipSuccess or the launch error.
+ *
+ * Kernel-to-reference map (reference paths under cpp/src/presolve/):
+ *   gpk_row_activity        gpu_presolve_kernels.cu:18-66
+ *   gpk_col_max_abs         gpu_presolve_kernels.cu:68-82
+ *   gpk_scan_*              cub::DeviceScan::InclusiveSum call sites (gpu_presolve.cu:67-81)
+ *   gpk_compact_*           gpu_presolve.cu:297-531 (count/copy compacted rows)
+ *   gpk_radix_sort_*        thrust::sort / sort_by_key call sites
+ *   gpk_transpose_*         cusparseCsr2cscEx2 (gpu_presolve.cu:1804-1879)
+ *   gpk_csr_stats           gpu_presolve.cu:249-276
+ */
+#ifndef GP_KERNELS_H
+#define GP_KERNELS_H
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- fills and small helpers ------------------------------------------- */
+hipError_t gpk_fill_u8(uint8_t* dst, uint8_t v, int64_t n, hipStream_t stream);
+hipError_t gpk_fill_i32(int32_t* dst, int32_t v, int64_t n, hipStream_t stream);
+hipError_t gpk_fill_f64(double* dst, double v, int64_t n, hipStream_t stream);
+hipError_t gpk_iota_i32(int32_t* dst, int32_t n, hipStream_t stream);
+
+/* ---- scans --------------------------------------------------------------
+ * Exclusive prefix sum of n int32 values into out[0..n]; out[n] is the total.
+ * `scratch` holds gpk_scan_scratch_ints(n) ints.  In-place (out == in) is allowed. */
+int64_t gpk_scan_scratch_ints(int64_t n);
+hipError_t gpk_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, int32_t* scratch,
+                                  hipStream_t stream);
+/* Exclusive scan of (mask[i] != 0): out[0..n], out[n] = popcount. */
+hipError_t gpk_scan_mask_u8(const uint8_t* mask, int32_t* out, int64_t n, int32_t* scratch,
+                            hipStream_t stream);
+/* old_to_new[i] = mask[i] ? prefix[i] : -1. */
+hipError_t gpk_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* old_to_new, int32_t n,
+                          hipStream_t stream);
+
+/* ---- activity primitives ----------------------------------------------- */
+hipError_t gpk_row_activity(const int32_t* row_ptr, const int32_t* col_idx, const double* val,
+                            int32_t m, const double* l, const double* u, double zero_tol,
+                            double* min_fin, double* max_fin, int32_t* neg_inf, int32_t* pos_inf,
+                            hipStream_t stream);
+hipError_t gpk_col_max_abs(const int32_t* at_ptr, const double* at_val, int32_t n, double* out,
+                           hipStream_t stream);
+/* Row stats of a CSR (counts include explicit zeros; singleton = first entry). */
+hipError_t gpk_csr_stats(const int32_t* ptr, const int32_t* idx, const double* val, int32_t rows,
+                         int32_t* nnz_out, uint8_t* empty_mask, uint8_t* singleton_mask,
+                         int32_t* singleton_idx, double* singleton_val, hipStream_t stream);
+
+/* ---- compaction ----------------------------------------------------------
+ * Stable compaction of a CSR by row/column maps (old_to_new, -1 = dropped).
+ * Step 1 writes counts[i] = kept entries of kept old row i (0 if dropped).
+ * Step 2 (after an exclusive scan of counts into new_ptr_by_old[0..m_old]) copies
+ * the kept entries; new row r's pointer is new_ptr_by_old[old row of r]. */
+hipError_t gpk_compact_count(const int32_t* ptr, const int32_t* idx, int32_t m_old,
+                             const int32_t* row_map, const int32_t* col_map, int32_t* counts,
+                             hipStream_t stream);
+hipError_t gpk_compact_copy(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
+                            const int32_t* row_map, const int32_t* col_map,
+                            const int32_t* new_ptr_by_old, int32_t* out_ptr, int32_t* out_idx,
+                            double* out_val, int32_t m_new, hipStream_t stream);
+/* dst[map[i]] = src[i] for map[i] >= 0, for up to 5 vectors of the same length. */
+hipError_t gpk_gather5(const int32_t* map, int32_t n_old, const double* s0, double* d0,
+                       const double* s1, double* d1, const double* s2, double* d2,
+                       const double* s3, double* d3, const double* s4, double* d4,
+                       hipStream_t stream);
+hipError_t gpk_gather_i32(const int32_t* map, int32_t n_old, const int32_t* src, int32_t* dst,
+                          hipStream_t stream);
+
+/* ---- radix sort -----------------------------------------------------------
+ * Stable LSD radix sort of (key, value) pairs, 8-bit digits over the low
+ * `key_bits` bits.  Buffers *_alt have the same size; the sorted result is
+ * left in keys/vals.  scratch holds gpk_radix_scratch_ints(n) ints. */
+int64_t gpk_radix_scratch_ints(int64_t n);
+hipError_t gpk_radix_sort_u64_i32(uint64_t* keys, int32_t* vals, uint64_t* keys_alt,
+                                  int32_t* vals_alt, int32_t n, int32_t key_bits, int32_t* scratch,
+                                  hipStream_t stream);
+hipError_t gpk_radix_sort_u32_i32(uint32_t* keys, int32_t* vals, uint32_t* keys_alt,
+                                  int32_t* vals_alt, int32_t n, int32_t key_bits, int32_t* scratch,
+                                  hipStream_t stream);
+
+/* ---- transpose -------------------------------------------------------------
+ * Expands the row index of every entry (row_of[p] = i), for use as the sort
+ * payload of a transpose, and the final gather of the transpose. */
+hipError_t gpk_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of, hipStream_t stream);
+hipError_t gpk_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts, hipStream_t stream);
+hipError_t gpk_transpose_gather(const int32_t* perm, const int32_t* row_of, const double* val,
+                                int32_t nnz, int32_t* at_idx, double* at_val, hipStream_t stream);
+/* flag[0] |= 1 if some row of the CSR is not sorted by column. */
+hipError_t gpk_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t* flag,
+                             hipStream_t stream);
+/* In-place sort of every row of a CSR by column (values follow). */
+hipError_t gpk_sort_rows_in_place(const int32_t* ptr, int32_t* idx, double* val, int32_t rows,
+                                  hipStream_t stream);
+
+
+/* ======================================================================
+ * Rule kernels.  Views of the CSR and of the plan vectors are passed by
+ * value.  Reference kernels restated are cited per launcher group.
+ * ====================================================================== */
+typedef struct gpk_csr {
+  int32_t rows, cols, nnz;
+  const int32_t* ptr;
+  const int32_t* idx;
+  const double* val;
+} gpk_csr;
+
+typedef struct gpk_plan {
+  uint8_t* keep_row;
+  uint8_t* keep_col;
+  double* c;
+  double* l;
+  double* u;
+  double* AL;
+  double* AU;
+} gpk_plan;
+
+/* Generic row shift: for each row i (all rows, or only keep_row[i] != 0 when
+ * keep_row is given) shift = sum over entries with col_flag[j] of a*col_val[j]
+ * in CSR order, then AL[i] -= shift, AU[i] -= shift for every row.  `abs_min`
+ * skips entries with |a| <= abs_min when >= 0. */
+hipError_t gpk_row_shift(gpk_csr A, const uint8_t* keep_row, const uint8_t* col_flag, const double* col_val,
+                         double abs_min, double* AL, double* AU, hipStream_t stream);
+/* out[k] = vals[i] for the k-th selected i (mask != 0), using prefix = exclusive scan of mask. */
+hipError_t gpk_pack_f64(const uint8_t* mask, const double* vals, const int32_t* prefix, int32_t n, double* out,
+                        hipStream_t stream);
+hipError_t gpk_pack_index(const uint8_t* mask, const int32_t* prefix, int32_t n, int32_t* out, hipStream_t stream);
+
+/* rule_empty_rows.cu:143-184 */
+hipError_t gpk_empty_rows(int32_t m, gpk_plan plan, const int32_t* row_nnz, double tol, int32_t apply,
+                          int32_t* flags, hipStream_t stream);
+/* rule_singleton_rows.cu:54-125 */
+hipError_t gpk_singleton_rows_bounds(int32_t m, int32_t n, gpk_plan plan, const uint8_t* mask, const int32_t* col,
+                                     const double* val, double zero_tol, double* cand_l, double* cand_u,
+                                     uint8_t* row_remove, hipStream_t stream);
+hipError_t gpk_singleton_rows_check(int32_t m, int32_t n, gpk_plan plan, const double* cand_l, const double* cand_u,
+                                    const uint8_t* row_remove, double bound_tol, int32_t* flags, hipStream_t stream);
+hipError_t gpk_singleton_rows_commit(int32_t m, int32_t n, gpk_plan plan, const double* cand_l,
+                                     const double* cand_u, const uint8_t* row_remove, hipStream_t stream);
+/* rule_infeasible_redundant_rows.cu:20-117 */
+hipError_t gpk_redundant_rows_classify(gpk_csr A, gpk_plan plan, const int32_t* row_nnz, double tol,
+                                       uint8_t* full, uint8_t* dlo, uint8_t* dhi, int32_t* flags,
+                                       hipStream_t stream);
+hipError_t gpk_redundant_rows_apply(int32_t m, gpk_plan plan, const uint8_t* full, const uint8_t* dlo,
+                                    const uint8_t* dhi, hipStream_t stream);
+/* rule_implied_variable_bounds.cu:339-866 */
+hipError_t gpk_implied_candidates(gpk_csr AT, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
+                                  const double* rmax, const int32_t* rni, const int32_t* rpi, double zero_tol,
+                                  double* cand_l, double* cand_u, hipStream_t stream);
+hipError_t gpk_implied_finalize(int32_t n, gpk_plan plan, const double* ref_l, const double* ref_u,
+                                const double* cand_l, const double* cand_u, const double* vmax,
+                                int32_t preserve_free_zero_cost, double zero_tol, double feas_tol, double* fin_l,
+                                double* fin_u, uint8_t* lch, uint8_t* uch, uint8_t* fixed, double* fixed_val,
+                                int32_t* flags, hipStream_t stream);
+hipError_t gpk_implied_support(gpk_csr AT, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
+                               const double* rmax, const int32_t* rni, const int32_t* rpi, const uint8_t* lch,
+                               const uint8_t* uch, const double* fin_l, const double* fin_u, double zero_tol,
+                               double tol, double support_tol, int32_t* support_l, int32_t* support_u,
+                               int32_t* rec_count, hipStream_t stream);
+hipError_t gpk_implied_pack_tape(int32_t n, gpk_plan plan, const uint8_t* lch, const uint8_t* uch,
+                                 const int32_t* support_l, const int32_t* support_u, const double* fin_l,
+                                 const double* fin_u, const int32_t* prefix, int32_t* rec_col, int32_t* rec_row,
+                                 double* rec_vals, hipStream_t stream);
+hipError_t gpk_implied_apply(int32_t n, gpk_plan plan, const double* fin_l, const double* fin_u,
+                             const uint8_t* fixed, hipStream_t stream);
+hipError_t gpk_remove_empty_rows_after_fix(gpk_csr A, gpk_plan plan, double feas_tol, int32_t* flags,
+                                           hipStream_t stream);
+/* fixed_col_tape.cu:41-103 */
+hipError_t gpk_fixed_col_counts(gpk_csr AT, const uint8_t* fixed, const uint8_t* excluded, const uint8_t* keep_row,
+                                uint8_t* selected, int32_t* active_nnz, hipStream_t stream);
+hipError_t gpk_fixed_col_pack(gpk_csr AT, const uint8_t* selected, const int32_t* sel_prefix,
+                              const int32_t* nnz_prefix, const double* fixed_val, const double* c,
+                              const uint8_t* keep_row, int32_t* out_col, double* out_v, double* out_c,
+                              int32_t* out_start, int32_t* out_rows, double* out_coef, hipStream_t stream);
+/* rule_duplicate_rows.cu:78-418 */
+hipError_t gpk_dup_row_hash(gpk_csr A, const uint8_t* keep_row, double coeff_tol, uint64_t* hash,
+                            hipStream_t stream);
+hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_hash, const int32_t* sorted_rows,
+                              double coeff_tol, double interval_tol, uint8_t* row_delete, int32_t* merge_to,
+                              int32_t* merge_step, double* rec_vals, int32_t* flags, hipStream_t stream);
+hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count,
+                           hipStream_t stream);
+
+/* rule_infeasible_fixed_variables.cu:42-104, rule_empty_cols.cu:42-108 */
+hipError_t gpk_fixed_vars_candidates(int32_t n, gpk_plan plan, double bound_tol, uint8_t* fixed, double* v,
+                                     double* contrib, int32_t* count, hipStream_t stream);
+hipError_t gpk_empty_cols(int32_t n, gpk_plan plan, const uint8_t* empty_mask, double bound_tol, double zero_tol,
+                          uint8_t* fixed, double* v, double* contrib, int32_t* flags, hipStream_t stream);
+/* l = u = v and keep_col = 0 for fixed columns. */
+hipError_t gpk_fix_cols_apply(int32_t n, gpk_plan plan, const uint8_t* fixed, const double* v, hipStream_t stream);
+
+/* rule_column_singletons.cu:411-1215 */
+typedef struct gpk_row_cache {
+  double* fmin;
+  double* fmax;
+  int32_t* mn;
+  int32_t* mp;
+  int32_t* xn;
+  int32_t* xp;
+  uint8_t* invalid;
+} gpk_row_cache;
+hipError_t gpk_cs_precheck(int32_t n, int32_t m, gpk_plan plan, const uint8_t* mask, const int32_t* srow,
+                           double bound_tol, uint8_t* needed, int32_t* counts, hipStream_t stream);
+hipError_t gpk_cs_activity_cache(gpk_csr A, gpk_plan plan, const uint8_t* needed, gpk_row_cache cache,
+                                 hipStream_t stream);
+hipError_t gpk_cs_owner(gpk_csr A, gpk_plan plan, const uint8_t* mask, const int32_t* srow, const double* sval,
+                        gpk_row_cache cache, double tol, double zero_tol, int32_t eq_mode, int32_t* owner,
+                        int32_t* flags, hipStream_t stream);
+hipError_t gpk_cs_process(int32_t n, int32_t m, gpk_plan plan, const uint8_t* mask, const int32_t* srow,
+                          const double* sval, const int32_t* owner, gpk_row_cache cache, double tol,
+                          double zero_tol, int32_t eq_mode, double* AL_new, double* AU_new, uint8_t* row_del,
+                          uint8_t* row_changed, uint8_t* col_del, double* side, double* contrib, int32_t* flags,
+                          hipStream_t stream);
+hipError_t gpk_cs_objective(gpk_csr AT, gpk_plan plan, const uint8_t* col_del, const int32_t* owner,
+                            const double* sval, double* c_new, hipStream_t stream);
+hipError_t gpk_cs_subcol_counts(gpk_csr A, gpk_plan plan, const int32_t* owner, const uint8_t* col_del,
+                                uint8_t* selected, int32_t* support_count, hipStream_t stream);
+hipError_t gpk_cs_subcol_pack(gpk_csr A, gpk_plan plan, const int32_t* owner, const uint8_t* row_del,
+                              const double* side, const uint8_t* selected, const int32_t* sel_prefix,
+                              const int32_t* sup_prefix, int32_t* out_i2, double* out_v6, int32_t* out_scol,
+                              double* out_sval, hipStream_t stream);
+hipError_t gpk_cs_apply(int32_t m, int32_t n, gpk_plan plan, const uint8_t* row_del, const uint8_t* col_del,
+                        const double* AL_new, const double* AU_new, const double* c_new, hipStream_t stream);
+
+/* rule_dual_fix.cu:174-384 */
+hipError_t gpk_dual_fix_candidates(gpk_csr AT, gpk_plan plan, const uint8_t* candidate, double zero_tol,
+                                   double bound_tol, int32_t max_degree, uint8_t* fixed, uint8_t* inf_fix,
+                                   uint8_t* finite_fix, double* v, double* contrib, int32_t* flags,
+                                   hipStream_t stream);
+hipError_t gpk_dual_fix_rows(gpk_csr A, gpk_plan plan, const uint8_t* finite_fix, const uint8_t* inf_fix,
+                             const double* v, double* shift, uint8_t* row_remove, int32_t* flags,
+                             hipStream_t stream);
+hipError_t gpk_dual_fix_apply(int32_t m, int32_t n, gpk_plan plan, const uint8_t* fixed, const double* v,
+                              const double* shift, const uint8_t* row_remove, hipStream_t stream);
+
+/* rule_duplicate_columns.cu:95-556 */
+hipError_t gpk_dup_col_hash(gpk_csr AT, gpk_plan plan, double zero_tol, uint64_t* hash, hipStream_t stream);
+hipError_t gpk_dup_col_guard(gpk_csr AT, gpk_plan plan, const uint64_t* sorted_hash, const int32_t* sorted_cols,
+                             double zero_tol, double coeff_tol, double obj_tol, double max_ratio,
+                             int32_t max_group, int32_t* count, hipStream_t stream);
+hipError_t gpk_dup_col_groups(gpk_csr AT, gpk_plan plan, const uint64_t* sorted_hash, const int32_t* sorted_cols,
+                              double zero_tol, double coeff_tol, double obj_tol, double max_ratio,
+                              const int32_t* guard_count, int32_t guard_min_groups, uint8_t* col_del,
+                              uint8_t* fixed, double* fixed_val, int32_t* merge_to, double* rec5, int32_t* flags,
+                              hipStream_t stream);
+hipError_t gpk_dup_col_apply(int32_t n, gpk_plan plan, const uint8_t* col_del, const uint8_t* fixed,
+                             const double* fixed_val, double* contrib, hipStream_t stream);
+
+/* rule_doubleton_equations.cu:143-1314 (batch path) */
+typedef struct gpk_dbl {
+  uint8_t* acceptable;
+  uint8_t* active;
+  uint8_t* selected;
+  uint8_t* blocked;
+  int32_t* cec;
+  int32_t* ckc;
+  double* cev;
+  double* ckv;
+  int32_t* owner;
+  int32_t* live_nnz;
+} gpk_dbl;
+typedef struct gpk_dbl_cols {
+  uint8_t* is_elim;
+  uint8_t* is_fixed_keep;
+  double* alpha_of;
+  double* beta_of;
+  double* fixed_val_of;
+  int32_t* row_of;
+  int32_t* keep_of;
+} gpk_dbl_cols;
+hipError_t gpk_dbl_live_nnz(gpk_csr AT, gpk_plan plan, double zero_tol, int32_t* live_nnz, hipStream_t stream);
+hipError_t gpk_dbl_candidates(gpk_csr A, gpk_csr AT, gpk_plan plan, gpk_dbl d, double zero_tol, double tol,
+                              double ratio_tol, int32_t max_fill, int32_t live_fill, int32_t* counts,
+                              hipStream_t stream);
+hipError_t gpk_dbl_claim(int32_t m, int32_t n, gpk_dbl d, hipStream_t stream);
+hipError_t gpk_dbl_select(int32_t m, gpk_dbl d, int32_t* counts, hipStream_t stream);
+hipError_t gpk_dbl_apply(int32_t k, const int32_t* sel, gpk_plan plan, gpk_dbl d, gpk_dbl_cols cols, double tol,
+                         uint8_t* subst, uint8_t* fixedp, double* osub, double* ofix, int32_t* rec_i3,
+                         double* rec_v12, int32_t* flags, hipStream_t stream);
+hipError_t gpk_dbl_delta_count(gpk_csr A, gpk_plan plan, gpk_dbl_cols cols, double zero_tol, int32_t* counts,
+                               hipStream_t stream);
+hipError_t gpk_dbl_shift(gpk_csr A, gpk_plan plan, gpk_dbl_cols cols, double zero_tol, int32_t include_subst,
+                         hipStream_t stream);
+hipError_t gpk_dbl_delta_fill(gpk_csr A, gpk_plan plan, gpk_dbl_cols cols, double zero_tol,
+                              const int32_t* delta_ptr, int32_t* dcol, double* dval, hipStream_t stream);
+hipError_t gpk_dbl_merge(gpk_csr A, const int32_t* delta_ptr, const int32_t* dcol, const double* dval,
+                         double zero_tol, const int32_t* out_ptr, int32_t* counts, int32_t* out_idx, double* out_val,
+                         hipStream_t stream);
+
+/* rule_doubleton_equations.cu:1779-1909 (single-doubleton path) */
+hipError_t gpk_dbl_scalar_rows(gpk_csr A, int32_t target, int32_t ec, int32_t kc, double alpha, double zero_tol,
+                               const int32_t* out_ptr, int32_t* counts, int32_t* out_idx, double* out_val,
+                               hipStream_t stream);
+
+/* rule_redundant_bounds.cu:31-262 */
+hipError_t gpk_rb_candidates(gpk_csr A, gpk_csr AT, gpk_plan plan, const double* c_lp, int32_t one_sided,
+                             double zero_tol, double tol, uint8_t* dlo, uint8_t* dhi, hipStream_t stream);
+hipError_t gpk_rb_select(gpk_csr A, gpk_csr AT, const uint8_t* dlo, const uint8_t* dhi, uint64_t* row_key,
+                         uint8_t* sel_lo, uint8_t* sel_hi, int32_t* flags, hipStream_t stream);
+hipError_t gpk_rb_apply(int32_t n, gpk_plan plan, const uint8_t* sel_lo, const uint8_t* sel_hi,
+                        hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GP_KERNELS_H */

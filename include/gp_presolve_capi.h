@@ -1,0 +1,254 @@
+/*
+ * C-ABI of the MI355X LP presolve/postsolve library (libgpu_presolver_amd.so).
+ *
+ * The reference exposes only a C++ API (cpp/include/gpu_presolver/presolve/
+ * gpu_presolve.hpp:34-40, gpu_postsolve.hpp:16-27, rules/rule_NAME.hpp,
+ * gpu_presolve_kernels.hpp:7-16).  Its std::vector members make that API a
+ * C++ ABI, so this header is the flat, plain-pointer form of the same entry
+ * points that an FFI (ctypes, cgo, JNI) binds.  The C++ API in
+ * include/gpu_presolver/presolve/ is implemented on top of the same code.
+ *
+ * Conventions
+ *  - Every array argument of a gp_* compute call is HIP device memory unless
+ *    the comment says "host".
+ *  - Calls are synchronous with respect to the host (like the reference,
+ *    which returns after cudaDeviceSynchronize) and run on the library's
+ *    internal stream.
+ *  - Errors: a nonzero gp_status; gp_last_error() gives the message of the
+ *    last failure on the calling thread.  LP outcomes (infeasible, unbounded)
+ *    are not errors; they are flags in the results, as in the reference.
+ */
+#ifndef GP_PRESOLVE_CAPI_H
+#define GP_PRESOLVE_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum gp_status {
+  GP_OK = 0,
+  GP_ERR_INVALID_ARGUMENT = 1,
+  GP_ERR_DEVICE = 2,      /* a HIP runtime call failed */
+  GP_ERR_INTERNAL = 3,    /* std::exception inside the library */
+  GP_ERR_NO_DEVICE = 4,   /* no usable gfx950 device */
+} gp_status;
+
+/* int32 zero-based CSR (reference DeviceCsrMatrix, presolve_structs.hpp:137-144). */
+typedef struct gp_csr {
+  int32_t rows;
+  int32_t cols;
+  int32_t nnz;
+  int32_t* row_ptr;
+  int32_t* col_idx;
+  double* val;
+} gp_csr;
+
+/* The LP (reference LPInfoGpu, presolve_structs.hpp:146-155). */
+typedef struct gp_lp {
+  gp_csr A;
+  gp_csr AT;
+  double* c;
+  double* AL;
+  double* AU;
+  double* l;
+  double* u;
+  double obj_constant;
+} gp_lp;
+
+/* PresolveParams (reference presolve_structs.hpp:21-134), every field in
+ * declaration order, booleans as int32.  gp_params_default() fills the
+ * reference defaults. */
+typedef struct gp_params {
+  int32_t max_iters;
+  double max_time;
+  int32_t verbose;
+  int32_t record_postsolve_tape;
+  int32_t record_postsolve_tape_cpu;
+  double feasibility_tol;
+  double bound_tol;
+  double zero_tol;
+  int32_t implied_variable_bounds_max_rounds;
+  int32_t implied_variable_bounds_max_bound_only_rounds;
+  int32_t implied_variable_bounds_preserve_free_zero_cost;
+  int32_t doubleton_equations_single_batch_per_iter;
+  int32_t doubleton_equations_max_fill_in_proxy;
+  int32_t doubleton_equations_scan;
+  int32_t doubleton_equations_min_selected_per_batch;
+  double doubleton_equations_min_selected_ratio;
+  int32_t doubleton_equations_max_batch_rounds;
+  double doubleton_equations_max_time;
+  int32_t enable_infeasible_fixed_variables;
+  int32_t enable_empty_rows;
+  int32_t enable_singleton_rows;
+  int32_t enable_infeasible_redundant_rows;
+  int32_t enable_implied_variable_bounds;
+  int32_t enable_duplicate_rows;
+  int32_t enable_empty_cols;
+  int32_t enable_column_singletons_eq;
+  int32_t enable_column_singletons_dual_infer;
+  int32_t enable_doubleton_equations;
+  int32_t enable_dual_fix;
+  int32_t dual_fix_max_nonzero_cost_col_degree;
+  int32_t enable_duplicate_columns;
+  double duplicate_columns_max_abs_ratio;
+  int32_t duplicate_columns_scale_guard_max_group_size;
+  int32_t duplicate_columns_scale_guard_min_groups;
+  int32_t enable_structure_specific_rules;
+  int32_t enable_zero_cost_redundant_box_bounds;
+  int32_t enable_redundant_bounds;
+  int32_t use_tiered_scheduler;
+  int32_t enable_tiered_bootstrap;
+  int64_t implied_variable_bounds_bound_only_nnz_round_budget;
+} gp_params;
+
+/* Summary scalars of one presolve (reference GpuPresolveSummary, gpu_presolve.hpp:9-29). */
+typedef struct gp_summary {
+  int32_t original_rows;
+  int32_t original_cols;
+  int32_t reduced_rows;
+  int32_t reduced_cols;
+  int32_t reduced_nnz;
+  int32_t iterations;
+  double elapsed_seconds;
+  double obj_constant_delta;
+  int32_t has_infeasible;
+  int32_t has_unbounded;
+} gp_summary;
+
+/* Flat read-only view of a PresolveRecordGpu (reference presolve_structs.hpp:
+ * 255-319), all pointers host memory.  A view returned by the library stays
+ * valid until its owner is freed. */
+typedef struct gp_record_view {
+  int32_t m0, n0, m1, n1;
+  const int32_t* row_red2org;   /* m1 */
+  const int32_t* col_red2org;   /* n1 */
+  int32_t fixed_count;
+  const int32_t* fixed_idx;
+  const double* fixed_val;
+  double obj_constant_old;
+  double obj_constant_new;
+  int32_t tape_count;           /* number of records */
+  const int32_t* tape_types;    /* tape_count */
+  const int32_t* tape_index_starts; /* tape_count + 1 */
+  const int32_t* tape_value_starts; /* tape_count + 1 */
+  const uint8_t* tape_dual_modes;   /* tape_count */
+  int32_t tape_index_count;
+  const int32_t* tape_indices;
+  int32_t tape_value_count;
+  const double* tape_vals;
+} gp_record_view;
+
+/* Reduction plan of one phase (reference PresolvePlanGpu, presolve_structs.hpp:
+ * 329-359) in plain form, for calling one rule directly as the reference's
+ * unit tests do.  Any vector a rule does not read may be null. */
+typedef struct gp_plan_view {
+  uint8_t* keep_row;
+  uint8_t* keep_col;
+  double* c;
+  double* l;
+  double* u;
+  double* AL;
+  double* AU;
+  double obj_constant_delta;
+  int32_t has_change;
+  int32_t has_row_action;
+  int32_t has_col_action;
+  int32_t has_infeasible;
+  int32_t has_unbounded;
+  int32_t has_new_A;   /* out: set when the rule rebuilt A (doubleton) */
+  gp_csr new_A;        /* out: device CSR owned by the caller (gp_device_free) */
+} gp_plan_view;
+
+/* PresolveStatsGpu (reference presolve_structs.hpp:158-171). */
+typedef struct gp_stats_view {
+  int32_t* row_nnz;
+  uint8_t* singleton_row_mask;
+  int32_t* singleton_row_col;
+  double* singleton_row_val;
+  int32_t* col_nnz;
+  uint8_t* empty_col_mask;
+  uint8_t* column_singleton_mask;
+  int32_t* column_singleton_row;
+  double* column_singleton_val;
+} gp_stats_view;
+
+/* Rule identifiers for gp_rule_apply (one per reference rules/rule_*.hpp). */
+typedef enum gp_rule_id {
+  GP_RULE_EMPTY_ROWS = 0,
+  GP_RULE_SINGLETON_ROWS = 1,
+  GP_RULE_INFEASIBLE_REDUNDANT_ROWS = 2,
+  GP_RULE_IMPLIED_VARIABLE_BOUNDS = 3,
+  GP_RULE_DUPLICATE_ROWS = 4,
+  GP_RULE_INFEASIBLE_FIXED_VARIABLES = 5,
+  GP_RULE_EMPTY_COLS = 6,
+  GP_RULE_COLUMN_SINGLETONS_DUAL_INFER = 7,
+  GP_RULE_COLUMN_SINGLETONS_EQ = 8,
+  GP_RULE_DOUBLETON_EQUATIONS = 9,
+  GP_RULE_DUAL_FIX = 10,
+  GP_RULE_DUPLICATE_COLUMNS = 11,
+  GP_RULE_REDUNDANT_BOUNDS = 12,
+} gp_rule_id;
+
+typedef struct gp_presolve_handle gp_presolve_handle;
+typedef struct gp_tape_handle gp_tape_handle;
+
+/* ---- library ---------------------------------------------------------- */
+const char* gp_last_error(void);
+const char* gp_version(void);
+void gp_params_default(gp_params* params);
+/* hipFree of a pointer the library handed to the caller. */
+gp_status gp_device_free(void* device_ptr);
+
+/* ---- primitives (reference gpu_presolve_kernels.hpp:7-16) -------------- */
+/* Replaces compute_row_activity_summary (reference gpu_presolve_kernels.hpp:7-14). */
+gp_status gp_row_activity(const gp_csr* A, const double* l, const double* u, double zero_tol,
+                          double* row_min_fin, double* row_max_fin,
+                          int32_t* row_min_neg_inf_count, int32_t* row_max_pos_inf_count);
+/* Replaces compute_col_max_abs (reference gpu_presolve_kernels.hpp:16). */
+gp_status gp_col_max_abs(const gp_csr* AT, double* col_max_abs);
+/* Mask compaction of a CSR matrix and its re-transpose, the apply step of the
+ * reference (gpu_presolve.cu:1881-1978 and :1804-1879).  keep masks are
+ * device uint8; outputs are allocated by the library (gp_device_free). */
+gp_status gp_compact_csr(const gp_csr* A, const uint8_t* keep_row, const uint8_t* keep_col,
+                         gp_csr* out);
+gp_status gp_transpose_csr(const gp_csr* A, gp_csr* out);
+
+/* ---- rule operator API (reference rules/rule_*.hpp) -------------------- */
+/* Runs one rule on `plan` exactly as apply_rule_<name> does.  `lp` and
+ * `stats` are device views; `candidate_col_mask` is used by the dual-fix
+ * rule only (may be null).  Tape records produced by the rule are appended
+ * to *tape (created on first use; free with gp_tape_free). */
+gp_status gp_rule_apply(int32_t rule_id, gp_plan_view* plan, const gp_lp* lp,
+                        const gp_stats_view* stats, const gp_params* params,
+                        const uint8_t* candidate_col_mask, gp_tape_handle** tape);
+gp_status gp_tape_view(const gp_tape_handle* tape, gp_record_view* out);
+void gp_tape_free(gp_tape_handle* tape);
+
+/* ---- presolve (reference gpu_presolve.hpp:34-40) ----------------------- */
+/* run_gpu_presolve_with_reduced_lp (keep_reduced_lp=1) or
+ * run_gpu_presolve_with_record (keep_reduced_lp=0).  `lp` is device memory
+ * owned by the caller and must outlive presolve and postsolve. */
+gp_status gp_presolve(const gp_lp* lp, const gp_params* params, int32_t keep_reduced_lp,
+                      gp_presolve_handle** out);
+gp_status gp_presolve_summary(const gp_presolve_handle* h, gp_summary* out);
+gp_status gp_presolve_record(const gp_presolve_handle* h, gp_record_view* out);
+/* Device view of the reduced LP, owned by the handle (valid until gp_presolve_free). */
+gp_status gp_presolve_reduced_lp(const gp_presolve_handle* h, gp_lp* out);
+/* free_gpu_presolve_reduced_lp + record release (reference gpu_presolve.hpp:40). */
+void gp_presolve_free(gp_presolve_handle* h);
+
+/* ---- postsolve (reference gpu_postsolve.hpp:16-27) --------------------- */
+/* postsolve_gpu with caller-provided outputs: x_org, z_org hold n0 doubles,
+ * y_org m0 doubles (device).  x_red/y_red/z_red may be null.  `original`
+ * (device view, may be null) enables the dual recoveries that need A. */
+gp_status gp_postsolve(const gp_record_view* record, const double* x_red, const double* y_red,
+                       const double* z_red, const gp_lp* original, double* x_org, double* y_org,
+                       double* z_org);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GP_PRESOLVE_CAPI_H */
