@@ -17,11 +17,21 @@
 
 namespace gpu_presolver::amd {
 
-// std::runtime_error("<context>: <hip error string>"), the reference's
-// error convention (presolve_helpers.hpp:13-19).
+// A failed HIP runtime call: std::runtime_error("<context>: <hip error
+// string>"), the reference's error convention (presolve_helpers.hpp:13-19).
+class HipError : public std::runtime_error {
+ public:
+  HipError(const char* context, hipError_t status)
+      : std::runtime_error(std::string(context) + ": " + hipGetErrorString(status)), status_(status) {}
+  hipError_t status() const { return status_; }
+
+ private:
+  hipError_t status_;
+};
+
 inline void check(hipError_t status, const char* context) {
   if (status == hipSuccess) return;
-  throw std::runtime_error(std::string(context) + ": " + hipGetErrorString(status));
+  throw HipError(context, status);
 }
 
 // Execution context: one stream, a small block of device flag words read
@@ -86,6 +96,9 @@ class ScopedCtx {
   Ctx* prev_;
 };
 
+// Stream-ordered pool allocation.  Every block is registered so that
+// dev_free can tell pool blocks (hipFreeAsync) from memory the caller
+// allocated with hipMalloc (hipFree); a plan's new_A may be either.
 void* dev_alloc(size_t bytes, hipStream_t stream, const char* context);
 void dev_free(void* p, hipStream_t stream) noexcept;
 

@@ -1,0 +1,533 @@
+// C-ABI of libgpu_presolver_amd.so (include/gp_presolve_capi.h): flat
+// wrappers over the C++ API, exceptions turned into gp_status codes with the
+// message kept per thread for gp_last_error().
+#include "gp_presolve_capi.h"
+
+#include <hip/hip_runtime_api.h>
+
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "gp_host.hpp"
+#include "gpu_presolver/presolve/gpu_postsolve.hpp"
+#include "gpu_presolver/presolve/gpu_presolve.hpp"
+#include "gpu_presolver/presolve/gpu_presolve_kernels.hpp"
+#include "gpu_presolver/presolve/rules/rule_column_singletons.hpp"
+#include "gpu_presolver/presolve/rules/rule_doubleton_equations.hpp"
+#include "gpu_presolver/presolve/rules/rule_dual_fix.hpp"
+#include "gpu_presolver/presolve/rules/rule_duplicate_columns.hpp"
+#include "gpu_presolver/presolve/rules/rule_duplicate_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_empty_cols.hpp"
+#include "gpu_presolver/presolve/rules/rule_empty_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_implied_variable_bounds.hpp"
+#include "gpu_presolver/presolve/rules/rule_infeasible_fixed_variables.hpp"
+#include "gpu_presolver/presolve/rules/rule_infeasible_redundant_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_redundant_bounds.hpp"
+#include "gpu_presolver/presolve/rules/rule_singleton_rows.hpp"
+
+using namespace gpu_presolver::presolve;
+namespace amd = gpu_presolver::amd;
+
+struct gp_presolve_handle {
+  GpuPresolveSummary summary;
+  bool keep_reduced = false;
+};
+
+struct gp_tape_handle {
+  PostsolveTape tape;
+};
+
+namespace {
+
+thread_local std::string t_error;
+
+struct InvalidArgument : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+
+template <class F>
+gp_status guarded(F&& f) {
+  try {
+    f();
+    t_error.clear();
+    return GP_OK;
+  } catch (const InvalidArgument& e) {
+    t_error = e.what();
+    return GP_ERR_INVALID_ARGUMENT;
+  } catch (const amd::HipError& e) {
+    t_error = e.what();
+    return e.status() == hipErrorNoDevice || e.status() == hipErrorInvalidDevice ? GP_ERR_NO_DEVICE : GP_ERR_DEVICE;
+  } catch (const std::bad_alloc&) {
+    t_error = "host allocation failed";
+    return GP_ERR_INTERNAL;
+  } catch (const std::exception& e) {
+    t_error = e.what();
+    return GP_ERR_INTERNAL;
+  } catch (...) {
+    t_error = "unknown exception";
+    return GP_ERR_INTERNAL;
+  }
+}
+
+void require(bool cond, const char* what) {
+  if (!cond) throw InvalidArgument(what);
+}
+
+void require_device() {
+  int count = 0;
+  const hipError_t st = hipGetDeviceCount(&count);
+  if (st != hipSuccess || count <= 0) throw amd::HipError("no HIP device", hipErrorNoDevice);
+}
+
+DeviceCsrMatrix to_csr(const gp_csr& a) { return DeviceCsrMatrix{a.rows, a.cols, a.nnz, a.row_ptr, a.col_idx, a.val}; }
+gp_csr from_csr(const DeviceCsrMatrix& a) { return gp_csr{a.rows, a.cols, a.nnz, a.rowPtr, a.colVal, a.nzVal}; }
+
+void check_csr(const gp_csr& a, const char* name) {
+  if (a.rows < 0 || a.cols < 0 || a.nnz < 0) throw InvalidArgument(std::string(name) + ": negative dimension");
+  if (a.row_ptr == nullptr) throw InvalidArgument(std::string(name) + ": null row_ptr");
+  if (a.nnz > 0 && (a.col_idx == nullptr || a.val == nullptr))
+    throw InvalidArgument(std::string(name) + ": null col_idx/val");
+}
+
+LPInfoGpu to_lp(const gp_lp& p) {
+  check_csr(p.A, "lp.A");
+  check_csr(p.AT, "lp.AT");
+  require(p.AT.rows == p.A.cols && p.AT.cols == p.A.rows && p.AT.nnz == p.A.nnz, "lp.AT is not shaped as A^T");
+  LPInfoGpu lp;
+  lp.A = to_csr(p.A);
+  lp.AT = to_csr(p.AT);
+  lp.c = p.c;
+  lp.AL = p.AL;
+  lp.AU = p.AU;
+  lp.l = p.l;
+  lp.u = p.u;
+  lp.obj_constant = p.obj_constant;
+  return lp;
+}
+
+gp_lp from_lp(const LPInfoGpu& lp) {
+  gp_lp p;
+  p.A = from_csr(lp.A);
+  p.AT = from_csr(lp.AT);
+  p.c = lp.c;
+  p.AL = lp.AL;
+  p.AU = lp.AU;
+  p.l = lp.l;
+  p.u = lp.u;
+  p.obj_constant = lp.obj_constant;
+  return p;
+}
+
+PresolveParams to_params(const gp_params* q) {
+  PresolveParams p;
+  if (q == nullptr) return p;
+  p.max_iters = q->max_iters;
+  p.max_time = q->max_time;
+  p.verbose = q->verbose != 0;
+  p.record_postsolve_tape = q->record_postsolve_tape != 0;
+  p.record_postsolve_tape_cpu = q->record_postsolve_tape_cpu != 0;
+  p.feasibility_tol = q->feasibility_tol;
+  p.bound_tol = q->bound_tol;
+  p.zero_tol = q->zero_tol;
+  p.implied_variable_bounds_max_rounds = q->implied_variable_bounds_max_rounds;
+  p.implied_variable_bounds_max_bound_only_rounds = q->implied_variable_bounds_max_bound_only_rounds;
+  p.implied_variable_bounds_preserve_free_zero_cost = q->implied_variable_bounds_preserve_free_zero_cost != 0;
+  p.doubleton_equations_single_batch_per_iter = q->doubleton_equations_single_batch_per_iter != 0;
+  p.doubleton_equations_max_fill_in_proxy = q->doubleton_equations_max_fill_in_proxy;
+  p.doubleton_equations_scan = q->doubleton_equations_scan != 0;
+  p.doubleton_equations_min_selected_per_batch = q->doubleton_equations_min_selected_per_batch;
+  p.doubleton_equations_min_selected_ratio = q->doubleton_equations_min_selected_ratio;
+  p.doubleton_equations_max_batch_rounds = q->doubleton_equations_max_batch_rounds;
+  p.doubleton_equations_max_time = q->doubleton_equations_max_time;
+  p.enable_infeasible_fixed_variables = q->enable_infeasible_fixed_variables != 0;
+  p.enable_empty_rows = q->enable_empty_rows != 0;
+  p.enable_singleton_rows = q->enable_singleton_rows != 0;
+  p.enable_infeasible_redundant_rows = q->enable_infeasible_redundant_rows != 0;
+  p.enable_implied_variable_bounds = q->enable_implied_variable_bounds != 0;
+  p.enable_duplicate_rows = q->enable_duplicate_rows != 0;
+  p.enable_empty_cols = q->enable_empty_cols != 0;
+  p.enable_column_singletons_eq = q->enable_column_singletons_eq != 0;
+  p.enable_column_singletons_dual_infer = q->enable_column_singletons_dual_infer != 0;
+  p.enable_doubleton_equations = q->enable_doubleton_equations != 0;
+  p.enable_dual_fix = q->enable_dual_fix != 0;
+  p.dual_fix_max_nonzero_cost_col_degree = q->dual_fix_max_nonzero_cost_col_degree;
+  p.enable_duplicate_columns = q->enable_duplicate_columns != 0;
+  p.duplicate_columns_max_abs_ratio = q->duplicate_columns_max_abs_ratio;
+  p.duplicate_columns_scale_guard_max_group_size = q->duplicate_columns_scale_guard_max_group_size;
+  p.duplicate_columns_scale_guard_min_groups = q->duplicate_columns_scale_guard_min_groups;
+  p.enable_structure_specific_rules = q->enable_structure_specific_rules != 0;
+  p.enable_zero_cost_redundant_box_bounds = q->enable_zero_cost_redundant_box_bounds != 0;
+  p.enable_redundant_bounds = q->enable_redundant_bounds != 0;
+  p.use_tiered_scheduler = q->use_tiered_scheduler != 0;
+  p.enable_tiered_bootstrap = q->enable_tiered_bootstrap != 0;
+  p.implied_variable_bounds_bound_only_nnz_round_budget = q->implied_variable_bounds_bound_only_nnz_round_budget;
+  return p;
+}
+
+void fill_tape_view(const PostsolveTape& t, gp_record_view* out) {
+  out->tape_count = static_cast<int32_t>(t.types.size());
+  out->tape_types = t.types.data();
+  out->tape_index_starts = t.index_starts.data();
+  out->tape_value_starts = t.value_starts.data();
+  out->tape_dual_modes = t.dual_modes.data();
+  out->tape_index_count = static_cast<int32_t>(t.indices.size());
+  out->tape_indices = t.indices.data();
+  out->tape_value_count = static_cast<int32_t>(t.vals.size());
+  out->tape_vals = t.vals.data();
+}
+
+template <class T>
+std::vector<T> host_vec(const T* p, int32_t n, const char* what) {
+  if (n < 0) throw InvalidArgument(std::string(what) + ": negative length");
+  if (n > 0 && p == nullptr) throw InvalidArgument(std::string(what) + ": null pointer");
+  return n > 0 ? std::vector<T>(p, p + n) : std::vector<T>{};
+}
+
+PresolveRecordGpu to_record(const gp_record_view& v) {
+  require(v.m0 >= 0 && v.n0 >= 0 && v.m1 >= 0 && v.n1 >= 0 && v.m1 <= v.m0 && v.n1 <= v.n0,
+          "record: invalid dimensions");
+  PresolveRecordGpu r;
+  r.m0 = v.m0;
+  r.n0 = v.n0;
+  r.m1 = v.m1;
+  r.n1 = v.n1;
+  r.row_red2org = host_vec(v.row_red2org, v.m1, "record.row_red2org");
+  r.col_red2org = host_vec(v.col_red2org, v.n1, "record.col_red2org");
+  r.fixed_idx = host_vec(v.fixed_idx, v.fixed_count, "record.fixed_idx");
+  r.fixed_val = host_vec(v.fixed_val, v.fixed_count, "record.fixed_val");
+  r.obj_constant_old = v.obj_constant_old;
+  r.obj_constant_new = v.obj_constant_new;
+  r.tape.types = host_vec(v.tape_types, v.tape_count, "record.tape_types");
+  r.tape.index_starts = host_vec(v.tape_index_starts, v.tape_count + 1, "record.tape_index_starts");
+  r.tape.value_starts = host_vec(v.tape_value_starts, v.tape_count + 1, "record.tape_value_starts");
+  r.tape.dual_modes = v.tape_dual_modes ? host_vec(v.tape_dual_modes, v.tape_count, "record.tape_dual_modes")
+                                        : std::vector<uint8_t>(static_cast<size_t>(v.tape_count), 0);
+  r.tape.indices = host_vec(v.tape_indices, v.tape_index_count, "record.tape_indices");
+  r.tape.vals = host_vec(v.tape_vals, v.tape_value_count, "record.tape_vals");
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gp_last_error(void) { return t_error.c_str(); }
+
+const char* gp_version(void) { return "gpu_presolver_amd 0.1.0 (gfx950)"; }
+
+void gp_params_default(gp_params* q) {
+  if (q == nullptr) return;
+  const PresolveParams p;
+  q->max_iters = p.max_iters;
+  q->max_time = p.max_time;
+  q->verbose = p.verbose;
+  q->record_postsolve_tape = p.record_postsolve_tape;
+  q->record_postsolve_tape_cpu = p.record_postsolve_tape_cpu;
+  q->feasibility_tol = p.feasibility_tol;
+  q->bound_tol = p.bound_tol;
+  q->zero_tol = p.zero_tol;
+  q->implied_variable_bounds_max_rounds = p.implied_variable_bounds_max_rounds;
+  q->implied_variable_bounds_max_bound_only_rounds = p.implied_variable_bounds_max_bound_only_rounds;
+  q->implied_variable_bounds_preserve_free_zero_cost = p.implied_variable_bounds_preserve_free_zero_cost;
+  q->doubleton_equations_single_batch_per_iter = p.doubleton_equations_single_batch_per_iter;
+  q->doubleton_equations_max_fill_in_proxy = p.doubleton_equations_max_fill_in_proxy;
+  q->doubleton_equations_scan = p.doubleton_equations_scan;
+  q->doubleton_equations_min_selected_per_batch = p.doubleton_equations_min_selected_per_batch;
+  q->doubleton_equations_min_selected_ratio = p.doubleton_equations_min_selected_ratio;
+  q->doubleton_equations_max_batch_rounds = p.doubleton_equations_max_batch_rounds;
+  q->doubleton_equations_max_time = p.doubleton_equations_max_time;
+  q->enable_infeasible_fixed_variables = p.enable_infeasible_fixed_variables;
+  q->enable_empty_rows = p.enable_empty_rows;
+  q->enable_singleton_rows = p.enable_singleton_rows;
+  q->enable_infeasible_redundant_rows = p.enable_infeasible_redundant_rows;
+  q->enable_implied_variable_bounds = p.enable_implied_variable_bounds;
+  q->enable_duplicate_rows = p.enable_duplicate_rows;
+  q->enable_empty_cols = p.enable_empty_cols;
+  q->enable_column_singletons_eq = p.enable_column_singletons_eq;
+  q->enable_column_singletons_dual_infer = p.enable_column_singletons_dual_infer;
+  q->enable_doubleton_equations = p.enable_doubleton_equations;
+  q->enable_dual_fix = p.enable_dual_fix;
+  q->dual_fix_max_nonzero_cost_col_degree = p.dual_fix_max_nonzero_cost_col_degree;
+  q->enable_duplicate_columns = p.enable_duplicate_columns;
+  q->duplicate_columns_max_abs_ratio = p.duplicate_columns_max_abs_ratio;
+  q->duplicate_columns_scale_guard_max_group_size = p.duplicate_columns_scale_guard_max_group_size;
+  q->duplicate_columns_scale_guard_min_groups = p.duplicate_columns_scale_guard_min_groups;
+  q->enable_structure_specific_rules = p.enable_structure_specific_rules;
+  q->enable_zero_cost_redundant_box_bounds = p.enable_zero_cost_redundant_box_bounds;
+  q->enable_redundant_bounds = p.enable_redundant_bounds;
+  q->use_tiered_scheduler = p.use_tiered_scheduler;
+  q->enable_tiered_bootstrap = p.enable_tiered_bootstrap;
+  q->implied_variable_bounds_bound_only_nnz_round_budget = p.implied_variable_bounds_bound_only_nnz_round_budget;
+}
+
+gp_status gp_device_free(void* device_ptr) {
+  return guarded([&] {
+    if (device_ptr == nullptr) return;
+    amd::dev_free(device_ptr, amd::ctx().stream());
+    amd::ctx().sync("gp_device_free");
+  });
+}
+
+gp_status gp_row_activity(const gp_csr* A, const double* l, const double* u, double zero_tol, double* row_min_fin,
+                          double* row_max_fin, int32_t* row_min_neg_inf_count, int32_t* row_max_pos_inf_count,
+                          void* stream) {
+  return guarded([&] {
+    require(A != nullptr, "A is null");
+    check_csr(*A, "A");
+    require(A->rows == 0 || (row_min_fin && row_max_fin && row_min_neg_inf_count && row_max_pos_inf_count),
+            "null output");
+    require(A->cols == 0 || (l && u), "null bounds");
+    if (stream != nullptr) {
+      if (A->rows == 0) return;
+      amd::check(gpk_row_activity(A->row_ptr, A->col_idx, A->val, A->rows, l, u, zero_tol, row_min_fin, row_max_fin,
+                                  row_min_neg_inf_count, row_max_pos_inf_count, static_cast<hipStream_t>(stream)),
+                 "row activity");
+      return;
+    }
+    require_device();
+    amd::check(hipDeviceSynchronize(), "entry synchronize");
+    compute_row_activity_summary(row_min_fin, row_max_fin, row_min_neg_inf_count, row_max_pos_inf_count, to_csr(*A),
+                                 l, u, zero_tol);
+    amd::ctx().sync("gp_row_activity");
+  });
+}
+
+gp_status gp_col_max_abs(const gp_csr* AT, double* col_max_abs, void* stream) {
+  return guarded([&] {
+    require(AT != nullptr, "AT is null");
+    check_csr(*AT, "AT");
+    require(AT->rows == 0 || col_max_abs, "null output");
+    if (stream != nullptr) {
+      if (AT->rows == 0) return;
+      amd::check(gpk_col_max_abs(AT->row_ptr, AT->val, AT->rows, col_max_abs, static_cast<hipStream_t>(stream)),
+                 "col max abs");
+      return;
+    }
+    require_device();
+    amd::check(hipDeviceSynchronize(), "entry synchronize");
+    compute_col_max_abs(col_max_abs, to_csr(*AT));
+    amd::ctx().sync("gp_col_max_abs");
+  });
+}
+
+gp_status gp_compact_csr(const gp_csr* A, const uint8_t* keep_row, const uint8_t* keep_col, gp_csr* out) {
+  return guarded([&] {
+    require(A != nullptr && out != nullptr, "null argument");
+    check_csr(*A, "A");
+    require((A->rows == 0 || keep_row) && (A->cols == 0 || keep_col), "null keep mask");
+    require_device();
+    amd::check(hipDeviceSynchronize(), "entry synchronize");
+    amd::DBuf<int32_t> row_map, col_map;
+    const int32_t m_new = amd::build_old_to_new(keep_row, A->rows, row_map);
+    const int32_t n_new = amd::build_old_to_new(keep_col, A->cols, col_map);
+    const DeviceCsrMatrix r = amd::compact_csr(to_csr(*A), row_map, col_map, m_new, n_new);
+    amd::ctx().sync("gp_compact_csr");
+    *out = from_csr(r);
+  });
+}
+
+gp_status gp_transpose_csr(const gp_csr* A, gp_csr* out) {
+  return guarded([&] {
+    require(A != nullptr && out != nullptr, "null argument");
+    check_csr(*A, "A");
+    require_device();
+    amd::check(hipDeviceSynchronize(), "entry synchronize");
+    const DeviceCsrMatrix r = amd::transpose_csr(to_csr(*A));
+    amd::ctx().sync("gp_transpose_csr");
+    *out = from_csr(r);
+  });
+}
+
+gp_status gp_rule_apply(int32_t rule_id, gp_plan_view* plan, const gp_lp* lp, const gp_stats_view* stats,
+                        const gp_params* params, const uint8_t* candidate_col_mask, gp_tape_handle** tape) {
+  return guarded([&] {
+    require(plan != nullptr && lp != nullptr, "null plan or lp");
+    require(rule_id >= GP_RULE_EMPTY_ROWS && rule_id <= GP_RULE_REDUNDANT_BOUNDS, "unknown rule id");
+    const LPInfoGpu L = to_lp(*lp);
+    require_device();
+    amd::check(hipDeviceSynchronize(), "entry synchronize");
+    PresolvePlanGpu P;
+    P.keep_row_mask = plan->keep_row;
+    P.keep_col_mask = plan->keep_col;
+    P.new_c = plan->c;
+    P.new_l = plan->l;
+    P.new_u = plan->u;
+    P.new_AL = plan->AL;
+    P.new_AU = plan->AU;
+    P.obj_constant_delta = plan->obj_constant_delta;
+    P.has_change = plan->has_change != 0;
+    P.has_row_action = plan->has_row_action != 0;
+    P.has_col_action = plan->has_col_action != 0;
+    P.has_infeasible = plan->has_infeasible != 0;
+    P.has_unbounded = plan->has_unbounded != 0;
+    P.has_new_A = plan->has_new_A != 0;
+    if (P.has_new_A) {
+      check_csr(plan->new_A, "plan.new_A");
+      P.new_A = to_csr(plan->new_A);
+    }
+    PresolveStatsGpu S;
+    if (stats != nullptr) {
+      S.row_nnz = stats->row_nnz;
+      S.singleton_row_mask = stats->singleton_row_mask;
+      S.singleton_row_col = stats->singleton_row_col;
+      S.singleton_row_val = stats->singleton_row_val;
+      S.col_nnz = stats->col_nnz;
+      S.empty_col_mask = stats->empty_col_mask;
+      S.column_singleton_mask = stats->column_singleton_mask;
+      S.column_singleton_row = stats->column_singleton_row;
+      S.column_singleton_val = stats->column_singleton_val;
+    }
+    const PresolveParams prm = to_params(params);
+    switch (rule_id) {
+      case GP_RULE_EMPTY_ROWS: apply_rule_empty_rows(P, L, S, prm); break;
+      case GP_RULE_SINGLETON_ROWS: apply_rule_singleton_rows(P, L, S, prm); break;
+      case GP_RULE_INFEASIBLE_REDUNDANT_ROWS: apply_rule_infeasible_redundant_rows(P, L, S, prm); break;
+      case GP_RULE_IMPLIED_VARIABLE_BOUNDS: apply_rule_implied_variable_bounds(P, L, S, prm); break;
+      case GP_RULE_DUPLICATE_ROWS: apply_rule_duplicate_rows(P, L, S, prm); break;
+      case GP_RULE_INFEASIBLE_FIXED_VARIABLES: apply_rule_infeasible_fixed_variables(P, L, prm); break;
+      case GP_RULE_EMPTY_COLS: apply_rule_empty_cols(P, L, S, prm); break;
+      case GP_RULE_COLUMN_SINGLETONS_DUAL_INFER: apply_rule_column_singletons_dual_infer(P, L, S, prm); break;
+      case GP_RULE_COLUMN_SINGLETONS_EQ: apply_rule_column_singletons_eq(P, L, S, prm, false); break;
+      case GP_RULE_DOUBLETON_EQUATIONS: apply_rule_doubleton_equations(P, L, S, prm); break;
+      case GP_RULE_DUAL_FIX: apply_rule_dual_fix(P, L, S, prm, candidate_col_mask); break;
+      case GP_RULE_DUPLICATE_COLUMNS: apply_rule_duplicate_columns(P, L, S, prm); break;
+      case GP_RULE_REDUNDANT_BOUNDS: apply_rule_redundant_bounds(P, L, S, prm); break;
+      default: break;
+    }
+    amd::ctx().sync("gp_rule_apply");
+    plan->obj_constant_delta = P.obj_constant_delta;
+    plan->has_change = P.has_change;
+    plan->has_row_action = P.has_row_action;
+    plan->has_col_action = P.has_col_action;
+    plan->has_infeasible = P.has_infeasible;
+    plan->has_unbounded = P.has_unbounded;
+    plan->has_new_A = P.has_new_A;
+    if (P.has_new_A) plan->new_A = from_csr(P.new_A);
+    if (tape != nullptr && !P.tape.types.empty()) {
+      if (*tape == nullptr) *tape = new gp_tape_handle();
+      PostsolveTape& dst = (*tape)->tape;
+      const int32_t bi = static_cast<int32_t>(dst.indices.size()), bv = static_cast<int32_t>(dst.vals.size());
+      dst.types.insert(dst.types.end(), P.tape.types.begin(), P.tape.types.end());
+      dst.dual_modes.insert(dst.dual_modes.end(), P.tape.dual_modes.begin(), P.tape.dual_modes.end());
+      dst.indices.insert(dst.indices.end(), P.tape.indices.begin(), P.tape.indices.end());
+      dst.vals.insert(dst.vals.end(), P.tape.vals.begin(), P.tape.vals.end());
+      for (size_t k = 1; k < P.tape.index_starts.size(); ++k) {
+        dst.index_starts.push_back(bi + P.tape.index_starts[k]);
+        dst.value_starts.push_back(bv + P.tape.value_starts[k]);
+      }
+    }
+  });
+}
+
+gp_status gp_tape_view(const gp_tape_handle* tape, gp_record_view* out) {
+  return guarded([&] {
+    require(out != nullptr, "null output");
+    *out = gp_record_view{};
+    static const PostsolveTape kEmpty;
+    fill_tape_view(tape ? tape->tape : kEmpty, out);
+  });
+}
+
+void gp_tape_free(gp_tape_handle* tape) { delete tape; }
+
+gp_status gp_presolve(const gp_lp* lp, const gp_params* params, int32_t keep_reduced_lp, gp_presolve_handle** out) {
+  return guarded([&] {
+    require(lp != nullptr && out != nullptr, "null argument");
+    *out = nullptr;
+    const LPInfoGpu L = to_lp(*lp);
+    require_device();
+    const PresolveParams prm = to_params(params);
+    auto* h = new gp_presolve_handle();
+    try {
+      h->keep_reduced = keep_reduced_lp != 0;
+      h->summary = h->keep_reduced ? run_gpu_presolve_with_reduced_lp(L, prm) : run_gpu_presolve_with_record(L, prm);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+gp_status gp_presolve_summary(const gp_presolve_handle* h, gp_summary* out) {
+  return guarded([&] {
+    require(h != nullptr && out != nullptr, "null argument");
+    const GpuPresolveSummary& s = h->summary;
+    out->original_rows = s.original_rows;
+    out->original_cols = s.original_cols;
+    out->reduced_rows = s.reduced_rows;
+    out->reduced_cols = s.reduced_cols;
+    out->reduced_nnz = s.reduced_nnz;
+    out->iterations = s.iterations;
+    out->elapsed_seconds = s.elapsed_seconds;
+    out->obj_constant_delta = s.obj_constant_delta;
+    out->has_infeasible = s.has_infeasible;
+    out->has_unbounded = s.has_unbounded;
+  });
+}
+
+gp_status gp_presolve_record(const gp_presolve_handle* h, gp_record_view* out) {
+  return guarded([&] {
+    require(h != nullptr && out != nullptr, "null argument");
+    const PresolveRecordGpu& r = h->summary.record;
+    *out = gp_record_view{};
+    out->m0 = r.m0;
+    out->n0 = r.n0;
+    out->m1 = r.m1;
+    out->n1 = r.n1;
+    out->row_red2org = r.row_red2org.data();
+    out->col_red2org = r.col_red2org.data();
+    out->fixed_count = static_cast<int32_t>(r.fixed_idx.size());
+    out->fixed_idx = r.fixed_idx.data();
+    out->fixed_val = r.fixed_val.data();
+    out->obj_constant_old = r.obj_constant_old;
+    out->obj_constant_new = r.obj_constant_new;
+    fill_tape_view(r.tape, out);
+  });
+}
+
+gp_status gp_presolve_reduced_lp(const gp_presolve_handle* h, gp_lp* out) {
+  return guarded([&] {
+    require(h != nullptr && out != nullptr, "null argument");
+    require(h->keep_reduced, "presolve ran without keep_reduced_lp");
+    *out = from_lp(h->summary.reduced_lp);
+  });
+}
+
+void gp_presolve_free(gp_presolve_handle* h) {
+  if (h == nullptr) return;
+  free_gpu_presolve_reduced_lp(h->summary);
+  delete h;
+}
+
+gp_status gp_postsolve(const gp_record_view* record, const double* x_red, const double* y_red, const double* z_red,
+                       const gp_lp* original, double* x_org, double* y_org, double* z_org) {
+  return guarded([&] {
+    require(record != nullptr, "null record");
+    const PresolveRecordGpu rec = to_record(*record);
+    require(rec.n0 == 0 || (x_org && z_org), "null x_org/z_org");
+    require(rec.m0 == 0 || y_org, "null y_org");
+    LPInfoGpu orig;
+    if (original != nullptr) orig = to_lp(*original);
+    require_device();
+    GpuPostsolveResult r = postsolve_gpu(const_cast<double*>(x_red), const_cast<double*>(y_red),
+                                         const_cast<double*>(z_red), rec, original ? &orig : nullptr, nullptr);
+    hipStream_t s = amd::ctx().stream();
+    hipError_t st = hipSuccess;
+    if (rec.n0 > 0) {
+      st = hipMemcpyAsync(x_org, r.x_org, sizeof(double) * rec.n0, hipMemcpyDeviceToDevice, s);
+      if (st == hipSuccess) st = hipMemcpyAsync(z_org, r.z_org, sizeof(double) * rec.n0, hipMemcpyDeviceToDevice, s);
+    }
+    if (st == hipSuccess && rec.m0 > 0)
+      st = hipMemcpyAsync(y_org, r.y_org, sizeof(double) * rec.m0, hipMemcpyDeviceToDevice, s);
+    if (st == hipSuccess) st = hipStreamSynchronize(s);
+    (void)hipFree(r.x_org);
+    (void)hipFree(r.y_org);
+    (void)hipFree(r.z_org);
+    amd::check(st, "gp_postsolve copy-out");
+  });
+}
+
+}  // extern "C"

@@ -6,6 +6,7 @@
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <unordered_set>
 
 #include "gp_host.hpp"
 
@@ -81,14 +82,35 @@ Ctx& ctx() {
 ScopedCtx::ScopedCtx(Ctx& c) : prev_(t_current) { t_current = &c; }
 ScopedCtx::~ScopedCtx() { t_current = prev_; }
 
+namespace {
+std::mutex g_pool_mu;
+std::unordered_set<void*>& pool_blocks() {
+  static auto* s = new std::unordered_set<void*>();
+  return *s;
+}
+}  // namespace
+
 void* dev_alloc(size_t bytes, hipStream_t stream, const char* context) {
   void* p = nullptr;
   check(hipMallocAsync(&p, bytes, stream), context);
+  std::lock_guard<std::mutex> lock(g_pool_mu);
+  pool_blocks().insert(p);
   return p;
 }
 
 void dev_free(void* p, hipStream_t stream) noexcept {
-  if (p) (void)hipFreeAsync(p, stream);
+  if (!p) return;
+  bool pooled = false;
+  {
+    std::lock_guard<std::mutex> lock(g_pool_mu);
+    pooled = pool_blocks().erase(p) != 0;
+  }
+  if (pooled) {
+    (void)hipFreeAsync(p, stream);
+  } else {
+    (void)hipStreamSynchronize(stream);
+    (void)hipFree(p);
+  }
 }
 
 int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out) {

@@ -1,0 +1,880 @@
+// Presolve driver (reference cpp/src/presolve/gpu_presolve.cu): the working
+// LP in HBM, per-phase plans and statistics, plan application (compaction,
+// re-transpose, record update) and the tiered and fixed schedulers.
+//
+// Device-side layout: the working LP owns one CSR pair (A, AT) and five
+// vectors from the stream-ordered pool; each phase owns one contiguous plan
+// block (masks + five vectors) and one contiguous statistics block.  The
+// reduced-to-original maps of the record stay on the device for the whole
+// run and are copied to the host once at the end.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <utility>
+#include <vector>
+
+#include "gp_host.hpp"
+#include "gp_record_kernels.h"
+#include "gpu_presolver/presolve/gpu_presolve.hpp"
+#include "gpu_presolver/presolve/rules/rule_column_singletons.hpp"
+#include "gpu_presolver/presolve/rules/rule_doubleton_equations.hpp"
+#include "gpu_presolver/presolve/rules/rule_dual_fix.hpp"
+#include "gpu_presolver/presolve/rules/rule_duplicate_columns.hpp"
+#include "gpu_presolver/presolve/rules/rule_duplicate_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_empty_cols.hpp"
+#include "gpu_presolver/presolve/rules/rule_empty_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_implied_variable_bounds.hpp"
+#include "gpu_presolver/presolve/rules/rule_infeasible_fixed_variables.hpp"
+#include "gpu_presolver/presolve/rules/rule_infeasible_redundant_rows.hpp"
+#include "gpu_presolver/presolve/rules/rule_redundant_bounds.hpp"
+#include "gpu_presolver/presolve/rules/rule_singleton_rows.hpp"
+
+namespace gpu_presolver::presolve {
+
+using amd::check;
+using amd::ctx;
+using amd::DBuf;
+using RT = PostsolveReductionType;
+
+namespace {
+
+size_t align256(size_t b) { return (b + 255) & ~size_t{255}; }
+
+// ---- working LP ------------------------------------------------------------
+// Borrows the caller's LP until the first structural change, then owns
+// pool-allocated arrays (gpu_presolve.cu:2021-2147, WorkingLp).
+struct WorkLp {
+  LPInfoGpu lp;
+  bool owns = false;
+  WorkLp() = default;
+  WorkLp(const WorkLp&) = delete;
+  WorkLp& operator=(const WorkLp&) = delete;
+  ~WorkLp() { release(); }
+  void release() noexcept {
+    if (!owns) return;
+    hipStream_t s = ctx().stream();
+    amd::free_csr(lp.A);
+    amd::free_csr(lp.AT);
+    amd::dev_free(lp.c, s);
+    amd::dev_free(lp.l, s);
+    amd::dev_free(lp.u, s);
+    amd::dev_free(lp.AL, s);
+    amd::dev_free(lp.AU, s);
+    lp = LPInfoGpu{};
+    owns = false;
+  }
+};
+
+double* pool_f64(int64_t n, const char* what) {
+  return static_cast<double*>(amd::dev_alloc(sizeof(double) * static_cast<size_t>(n > 0 ? n : 1), ctx().stream(), what));
+}
+
+template <class T>
+void d2d(T* dst, const T* src, int64_t n, const char* what) {
+  if (n > 0)
+    check(hipMemcpyAsync(dst, src, sizeof(T) * static_cast<size_t>(n), hipMemcpyDeviceToDevice, ctx().stream()), what);
+}
+
+// ---- phase plan ------------------------------------------------------------
+// Fresh plan over the working LP: masks all ones, copies of c/l/u/AL/AU
+// (gpu_presolve.cu:197-247), in one contiguous block.
+struct PhasePlan {
+  PresolvePlanGpu p;
+  explicit PhasePlan(const LPInfoGpu& lp) {
+    const size_t m = static_cast<size_t>(lp.A.rows), n = static_cast<size_t>(lp.A.cols);
+    const size_t bytes = align256(8 * n) * 3 + align256(8 * m) * 2 + align256(m) + align256(n);
+    char* b = static_cast<char*>(amd::dev_alloc(bytes, ctx().stream(), "plan storage"));
+    p.contiguous_storage = b;
+    p.new_c = reinterpret_cast<double*>(b);
+    b += align256(8 * n);
+    p.new_l = reinterpret_cast<double*>(b);
+    b += align256(8 * n);
+    p.new_u = reinterpret_cast<double*>(b);
+    b += align256(8 * n);
+    p.new_AL = reinterpret_cast<double*>(b);
+    b += align256(8 * m);
+    p.new_AU = reinterpret_cast<double*>(b);
+    b += align256(8 * m);
+    p.keep_row_mask = reinterpret_cast<uint8_t*>(b);
+    b += align256(m);
+    p.keep_col_mask = reinterpret_cast<uint8_t*>(b);
+    hipStream_t s = ctx().stream();
+    if (m > 0) check(gpk_fill_u8(p.keep_row_mask, 1, static_cast<int64_t>(m), s), "plan keep_row");
+    if (n > 0) check(gpk_fill_u8(p.keep_col_mask, 1, static_cast<int64_t>(n), s), "plan keep_col");
+    d2d(p.new_c, lp.c, static_cast<int64_t>(n), "plan c");
+    d2d(p.new_l, lp.l, static_cast<int64_t>(n), "plan l");
+    d2d(p.new_u, lp.u, static_cast<int64_t>(n), "plan u");
+    d2d(p.new_AL, lp.AL, static_cast<int64_t>(m), "plan AL");
+    d2d(p.new_AU, lp.AU, static_cast<int64_t>(m), "plan AU");
+  }
+  ~PhasePlan() {
+    if (p.has_new_A) amd::free_csr(p.new_A);
+    amd::dev_free(p.contiguous_storage, ctx().stream());
+  }
+  PhasePlan(const PhasePlan&) = delete;
+  PhasePlan& operator=(const PhasePlan&) = delete;
+};
+
+// ---- phase statistics ------------------------------------------------------
+// recompute_row_stats_from_csr / recompute_col_stats_from_csr
+// (gpu_presolve.cu:249-276, 696-732).
+struct PhaseStats {
+  PresolveStatsGpu s;
+  explicit PhaseStats(const LPInfoGpu& lp) {
+    const size_t m = static_cast<size_t>(lp.A.rows), n = static_cast<size_t>(lp.A.cols);
+    const size_t bytes = align256(4 * m) * 2 + align256(8 * m) + align256(m) + align256(4 * n) * 2 +
+                         align256(8 * n) + align256(n) * 2;
+    char* b = static_cast<char*>(amd::dev_alloc(bytes, ctx().stream(), "stats storage"));
+    s.contiguous_storage = b;
+    s.row_nnz = reinterpret_cast<int32_t*>(b);
+    b += align256(4 * m);
+    s.singleton_row_col = reinterpret_cast<int32_t*>(b);
+    b += align256(4 * m);
+    s.singleton_row_val = reinterpret_cast<double*>(b);
+    b += align256(8 * m);
+    s.singleton_row_mask = reinterpret_cast<uint8_t*>(b);
+    b += align256(m);
+    s.col_nnz = reinterpret_cast<int32_t*>(b);
+    b += align256(4 * n);
+    s.column_singleton_row = reinterpret_cast<int32_t*>(b);
+    b += align256(4 * n);
+    s.column_singleton_val = reinterpret_cast<double*>(b);
+    b += align256(8 * n);
+    s.empty_col_mask = reinterpret_cast<uint8_t*>(b);
+    b += align256(n);
+    s.column_singleton_mask = reinterpret_cast<uint8_t*>(b);
+    hipStream_t st = ctx().stream();
+    if (m > 0)
+      check(gpk_csr_stats(lp.A.rowPtr, lp.A.colVal, lp.A.nzVal, lp.A.rows, s.row_nnz, nullptr, s.singleton_row_mask,
+                          s.singleton_row_col, s.singleton_row_val, st),
+            "row stats");
+    if (n > 0)
+      check(gpk_csr_stats(lp.AT.rowPtr, lp.AT.colVal, lp.AT.nzVal, lp.AT.rows, s.col_nnz, s.empty_col_mask,
+                          s.column_singleton_mask, s.column_singleton_row, s.column_singleton_val, st),
+            "col stats");
+  }
+  ~PhaseStats() { amd::dev_free(s.contiguous_storage, ctx().stream()); }
+  PhaseStats(const PhaseStats&) = delete;
+  PhaseStats& operator=(const PhaseStats&) = delete;
+};
+
+// ---- record state ----------------------------------------------------------
+struct RecordState {
+  PresolveRecordGpu* rec = nullptr;
+  DBuf<int32_t> row_r2o, col_r2o;  // device reduced-to-original maps
+  int32_t m = 0, n = 0;            // their current lengths
+};
+
+// Index kinds of one phase tape for globalization: 1 row, 2 column, 0 none
+// (append_globalized_postsolve_tape, gpu_presolve.cu:1363-1533).
+void tape_index_kinds(const PostsolveTape& t, std::vector<uint8_t>& kind) {
+  kind.assign(t.indices.size(), 0);
+  const int32_t* I = t.indices.data();
+  for (size_t k = 0; k < t.types.size(); ++k) {
+    const int32_t b = t.index_starts[k];
+    const int32_t cnt = t.index_starts[k + 1] - b;
+    uint8_t* kd = kind.data() + b;
+    const int32_t* ix = I + b;
+    switch (static_cast<RT>(t.types[k])) {
+      case RT::FixedCol:
+        if (cnt > 0) {
+          kd[0] = 2;
+          for (int32_t q = 1; q < cnt; ++q) kd[q] = 1;
+        }
+        break;
+      case RT::FixedColInf:
+        if (cnt >= 2) {
+          kd[1] = 2;
+          int32_t q = 2;
+          while (q < cnt) {
+            const int32_t len = ix[q++];
+            for (int32_t e = 0; e < len && q < cnt; ++e, ++q) kd[q] = 2;
+          }
+        }
+        break;
+      case RT::SubCol:
+        if (cnt >= 3) {
+          kd[0] = 2;
+          kd[1] = 1;
+          for (int32_t e = 0; e < ix[2] && 3 + e < cnt; ++e) kd[3 + e] = 2;
+        }
+        break;
+      case RT::DuplicateColumn:
+        if (cnt >= 2) kd[0] = kd[1] = 2;
+        break;
+      case RT::DuplicateRow:
+        if (cnt >= 2) kd[0] = kd[1] = 1;
+        break;
+      case RT::DeletedRow:
+        if (cnt >= 1) {
+          kd[0] = 1;
+          if (cnt >= 2) kd[1] = 2;
+        }
+        break;
+      case RT::LhsChange:
+      case RT::RhsChange:
+      case RT::EqToIneq:
+        if (cnt >= 1) kd[0] = 1;
+        break;
+      case RT::BoundChangeNoRow:
+        if (cnt >= 1) kd[0] = 2;
+        break;
+      case RT::BoundChangeTheRow:
+        if (cnt >= 2) {
+          kd[0] = 2;
+          kd[1] = 1;
+        }
+        break;
+      case RT::DoubletonEquation:
+        if (cnt >= 4) {
+          kd[0] = kd[1] = 2;
+          kd[2] = 1;
+          for (int32_t e = 0; e < ix[3] && 4 + e < cnt; ++e) kd[4 + e] = 1;
+        }
+        break;
+      case RT::FmeCol:
+        if (cnt >= 3) {
+          const int32_t inv = ix[0];
+          kd[1] = 2;
+          for (int32_t e = 0; e < inv && 2 + e < cnt; ++e) kd[2 + e] = 1;
+          const int32_t sp = 2 + inv;
+          if (sp < cnt)
+            for (int32_t e = 0; e < ix[sp] && sp + 1 + e < cnt; ++e) kd[sp + 1 + e] = 2;
+        }
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+void append_global_tape(RecordState& rs, const PostsolveTape& src) {
+  if (src.types.empty()) return;
+  PostsolveTape& dst = rs.rec->tape;
+  std::vector<int32_t> idx = src.indices;
+  if (!idx.empty()) {
+    std::vector<uint8_t> kind;
+    tape_index_kinds(src, kind);
+    auto& c = ctx();
+    const int64_t cnt = static_cast<int64_t>(idx.size());
+    DBuf<int32_t> d_idx(cnt);
+    DBuf<uint8_t> d_kind(cnt);
+    c.to_device(d_idx.get(), idx.data(), cnt, "tape indices upload");
+    c.to_device(d_kind.get(), kind.data(), cnt, "tape kinds upload");
+    check(gpk_map_tape_indices(static_cast<int32_t>(cnt), d_idx, d_kind, rs.row_r2o, rs.m, rs.col_r2o, rs.n,
+                               c.stream()),
+          "tape globalize");
+    idx = c.to_host(d_idx.get(), cnt, "tape indices download");
+  }
+  const int32_t base_idx = static_cast<int32_t>(dst.indices.size());
+  const int32_t base_val = static_cast<int32_t>(dst.vals.size());
+  dst.types.insert(dst.types.end(), src.types.begin(), src.types.end());
+  dst.dual_modes.insert(dst.dual_modes.end(), src.dual_modes.begin(), src.dual_modes.end());
+  dst.indices.insert(dst.indices.end(), idx.begin(), idx.end());
+  dst.vals.insert(dst.vals.end(), src.vals.begin(), src.vals.end());
+  for (size_t k = 1; k < src.index_starts.size(); ++k) {
+    dst.index_starts.push_back(base_idx + src.index_starts[k]);
+    dst.value_starts.push_back(base_val + src.value_starts[k]);
+  }
+}
+
+// update_record_from_plan (gpu_presolve.cu:1535-1802).
+void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t n, const int32_t* row_map,
+                   const int32_t* col_map, int32_t m_new, int32_t n_new) {
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  PresolveRecordGpu& rec = *rs.rec;
+  if (n_new < n) {
+    // Removed columns with a finite fixed interval become (global index, midpoint).
+    DBuf<uint8_t> mask(n);
+    DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1);
+    check(gpk_removed_fixed_mask(n, p.keep_col_mask, p.new_l, p.new_u, mask, s), "removed fixed mask");
+    const int32_t k = amd::scan_mask_count(mask, n, prefix);
+    if (k > 0) {
+      DBuf<int32_t> oi(k);
+      DBuf<double> ov(k);
+      check(gpk_pack_fixed_records(n, mask, prefix, rs.col_r2o, p.new_l, p.new_u, oi, ov, s), "fixed records");
+      const std::vector<int32_t> hi = c.to_host(oi.get(), k, "fixed records idx");
+      const std::vector<double> hv = c.to_host(ov.get(), k, "fixed records val");
+      rec.fixed_idx.insert(rec.fixed_idx.end(), hi.begin(), hi.end());
+      rec.fixed_val.insert(rec.fixed_val.end(), hv.begin(), hv.end());
+    }
+  }
+  append_global_tape(rs, p.tape);
+  if (m_new < m) {
+    DBuf<int32_t> next(m_new);
+    check(gpk_gather_i32(row_map, m, rs.row_r2o, next, s), "row red2org");
+    rs.row_r2o = std::move(next);
+    rs.m = m_new;
+  }
+  if (n_new < n) {
+    DBuf<int32_t> next(n_new);
+    check(gpk_gather_i32(col_map, n, rs.col_r2o, next, s), "col red2org");
+    rs.col_r2o = std::move(next);
+    rs.n = n_new;
+  }
+}
+
+// apply_plan_to_working_lp (gpu_presolve.cu:2021-2147).
+void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const int32_t m = cur.lp.A.rows, n = cur.lp.A.cols;
+  DBuf<int32_t> row_map, col_map;
+  const int32_t m_new = amd::build_old_to_new(p.keep_row_mask, m, row_map);
+  const int32_t n_new = amd::build_old_to_new(p.keep_col_mask, n, col_map);
+  const double obj_new = cur.lp.obj_constant + p.obj_constant_delta;
+  if (cur.owns && !p.has_new_A && m_new == m && n_new == n) {
+    update_record(rs, p, m, n, row_map, col_map, m_new, n_new);
+    d2d(cur.lp.c, p.new_c, n, "apply c");
+    d2d(cur.lp.l, p.new_l, n, "apply l");
+    d2d(cur.lp.u, p.new_u, n, "apply u");
+    d2d(cur.lp.AL, p.new_AL, m, "apply AL");
+    d2d(cur.lp.AU, p.new_AU, m, "apply AU");
+    cur.lp.obj_constant = obj_new;
+    return;
+  }
+  const DeviceCsrMatrix& src = p.has_new_A ? p.new_A : cur.lp.A;
+  LPInfoGpu next;
+  next.A = amd::compact_csr(src, row_map, col_map, m_new, n_new);
+  // A kept AT is compacted with the maps swapped; a rewritten A, or the
+  // caller's AT on the first change, is re-transposed.
+  if (p.has_new_A || !cur.owns) {
+    next.AT = amd::transpose_csr(next.A);
+  } else {
+    next.AT = amd::compact_csr(cur.lp.AT, col_map, row_map, n_new, m_new);
+  }
+  next.c = pool_f64(n_new, "working c");
+  next.l = pool_f64(n_new, "working l");
+  next.u = pool_f64(n_new, "working u");
+  next.AL = pool_f64(m_new, "working AL");
+  next.AU = pool_f64(m_new, "working AU");
+  if (n > 0)
+    check(gpk_gather5(col_map, n, p.new_c, next.c, p.new_l, next.l, p.new_u, next.u, nullptr, nullptr, nullptr,
+                      nullptr, s),
+          "gather column vectors");
+  if (m > 0)
+    check(gpk_gather5(row_map, m, p.new_AL, next.AL, p.new_AU, next.AU, nullptr, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, s),
+          "gather row vectors");
+  update_record(rs, p, m, n, row_map, col_map, m_new, n_new);
+  next.obj_constant = obj_new;
+  cur.release();
+  cur.lp = next;
+  cur.owns = true;
+}
+
+// Copies the working LP into hipMalloc'ed arrays the caller releases with
+// hipFree (free_gpu_presolve_reduced_lp).
+LPInfoGpu export_lp(const LPInfoGpu& w) {
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  LPInfoGpu out = w;
+  auto take = [&](auto*& dst, const auto* src, int64_t count, const char* what) {
+    using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
+    void* p = nullptr;
+    check(hipMalloc(&p, sizeof(T) * static_cast<size_t>(count > 0 ? count : 1)), what);
+    dst = static_cast<T*>(p);
+    if (count > 0)
+      check(hipMemcpyAsync(dst, src, sizeof(T) * static_cast<size_t>(count), hipMemcpyDeviceToDevice, s), what);
+  };
+  take(out.A.rowPtr, w.A.rowPtr, static_cast<int64_t>(w.A.rows) + 1, "reduced A rowPtr");
+  take(out.A.colVal, w.A.colVal, w.A.nnz, "reduced A colVal");
+  take(out.A.nzVal, w.A.nzVal, w.A.nnz, "reduced A nzVal");
+  take(out.AT.rowPtr, w.AT.rowPtr, static_cast<int64_t>(w.AT.rows) + 1, "reduced AT rowPtr");
+  take(out.AT.colVal, w.AT.colVal, w.AT.nnz, "reduced AT colVal");
+  take(out.AT.nzVal, w.AT.nzVal, w.AT.nnz, "reduced AT nzVal");
+  take(out.c, w.c, w.A.cols, "reduced c");
+  take(out.l, w.l, w.A.cols, "reduced l");
+  take(out.u, w.u, w.A.cols, "reduced u");
+  take(out.AL, w.AL, w.A.rows, "reduced AL");
+  take(out.AU, w.AU, w.A.rows, "reduced AU");
+  c.sync("reduced LP export");
+  return out;
+}
+
+void rebuild_org2red(std::vector<int32_t>& org2red, const std::vector<int32_t>& red2org, int32_t count) {
+  org2red.assign(static_cast<size_t>(count > 0 ? count : 0), -1);
+  for (size_t r = 0; r < red2org.size(); ++r) {
+    const int32_t o = red2org[r];
+    if (o >= 0 && o < count) org2red[static_cast<size_t>(o)] = static_cast<int32_t>(r);
+  }
+}
+
+void validate_lp(const LPInfoGpu& lp) {
+  if (lp.A.rows < 0 || lp.A.cols < 0 || lp.A.nnz < 0) throw std::runtime_error("LPInfoGpu: negative dimension");
+  if (lp.AT.rows != lp.A.cols || lp.AT.cols != lp.A.rows || lp.AT.nnz != lp.A.nnz)
+    throw std::runtime_error("LPInfoGpu: AT is not the transpose shape of A");
+  if ((lp.A.cols > 0 && (!lp.c || !lp.l || !lp.u)) || (lp.A.rows > 0 && (!lp.AL || !lp.AU)))
+    throw std::runtime_error("LPInfoGpu: missing vector");
+  if (!lp.A.rowPtr || !lp.AT.rowPtr || (lp.A.nnz > 0 && (!lp.A.colVal || !lp.A.nzVal || !lp.AT.colVal || !lp.AT.nzVal)))
+    throw std::runtime_error("LPInfoGpu: missing CSR array");
+}
+
+// run_gpu_presolve_impl (gpu_presolve.cu:2162-3400) without the six
+// structure-specific rules (they find nothing on LPs without their block
+// patterns, so both values of enable_structure_specific_rules schedule the
+// same phases here).
+GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool keep_reduced) {
+  validate_lp(lp);
+  check(hipDeviceSynchronize(), "presolve entry synchronize");
+  GpuPresolveSummary res;
+  res.original_rows = lp.A.rows;
+  res.original_cols = lp.A.cols;
+  PresolveRecordGpu& rec = res.record;
+  rec.m0 = rec.m1 = lp.A.rows;
+  rec.n0 = rec.n1 = lp.A.cols;
+  rec.obj_constant_old = rec.obj_constant_new = lp.obj_constant;
+  RecordState rs;
+  rs.rec = &rec;
+  rs.m = lp.A.rows;
+  rs.n = lp.A.cols;
+  rs.row_r2o.alloc(rs.m);
+  rs.col_r2o.alloc(rs.n);
+  if (rs.m > 0) check(gpk_iota_i32(rs.row_r2o, rs.m, ctx().stream()), "row red2org init");
+  if (rs.n > 0) check(gpk_iota_i32(rs.col_r2o, rs.n, ctx().stream()), "col red2org init");
+
+  WorkLp cur;
+  cur.lp = lp;
+  cur.owns = false;
+
+  bool terminal = false;
+  const auto t0 = std::chrono::steady_clock::now();
+  const bool has_limit = prm.max_time > 0.0 && std::isfinite(prm.max_time);
+  bool cs_dual_clean = false, cs_eq_clean = false;
+  auto invalidate_clean = [&]() {
+    cs_dual_clean = false;
+    cs_eq_clean = false;
+  };
+  auto time_exceeded = [&]() {
+    if (!has_limit) return false;
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= prm.max_time;
+  };
+  auto finish_plan = [&](PresolvePlanGpu& p) -> bool {
+    res.has_infeasible = p.has_infeasible;
+    res.has_unbounded = p.has_unbounded;
+    terminal = p.has_infeasible || p.has_unbounded;
+    const bool changed = !terminal && p.has_change;
+    if (changed) {
+      apply_plan(cur, p, rs);
+      invalidate_clean();
+    }
+    return changed;
+  };
+
+  // run_row_phase (gpu_presolve.cu:2219-2301)
+  auto row_phase = [&](bool empty, bool singleton, bool redundant, bool implied, bool dup) -> bool {
+    if (terminal || time_exceeded()) return false;
+    PhasePlan pp(cur.lp);
+    PhaseStats st(cur.lp);
+    PresolvePlanGpu& p = pp.p;
+    const LPInfoGpu& v = cur.lp;
+    if (!time_exceeded() && empty && prm.enable_empty_rows) apply_rule_empty_rows(p, v, st.s, prm);
+    if (!time_exceeded() && singleton && prm.enable_singleton_rows) apply_rule_singleton_rows(p, v, st.s, prm);
+    if (!time_exceeded() && redundant && prm.enable_infeasible_redundant_rows)
+      apply_rule_infeasible_redundant_rows(p, v, st.s, prm);
+    if (!time_exceeded() && implied && prm.enable_implied_variable_bounds)
+      apply_rule_implied_variable_bounds(p, v, st.s, prm);
+    if (!time_exceeded() && dup && prm.enable_duplicate_rows) apply_rule_duplicate_rows(p, v, st.s, prm);
+    return finish_plan(p);
+  };
+
+  // run_col_phase (gpu_presolve.cu:2303-2452)
+  auto col_phase = [&](bool fixed, bool empty_cols, bool cs_dual, bool cs_eq, bool doubleton, bool dual_fix,
+                       bool dup_cols, bool redundant_bounds) -> bool {
+    if (terminal || time_exceeded()) return false;
+    PhasePlan pp(cur.lp);
+    PhaseStats st(cur.lp);
+    PresolvePlanGpu& p = pp.p;
+    const LPInfoGpu& v = cur.lp;
+    const bool run_redundant =
+        redundant_bounds && (prm.enable_redundant_bounds || prm.enable_zero_cost_redundant_box_bounds);
+    if (!time_exceeded() && fixed && prm.enable_infeasible_fixed_variables)
+      apply_rule_infeasible_fixed_variables(p, v, prm);
+    if (!time_exceeded() && empty_cols && prm.enable_empty_cols) apply_rule_empty_cols(p, v, st.s, prm);
+    if (!time_exceeded() && cs_dual && prm.enable_column_singletons_dual_infer)
+      apply_rule_column_singletons_dual_infer(p, v, st.s, prm);
+    if (!time_exceeded() && cs_eq && prm.enable_column_singletons_eq)
+      apply_rule_column_singletons_eq(p, v, st.s, prm, false);
+    if (!time_exceeded() && doubleton && prm.enable_doubleton_equations)
+      apply_rule_doubleton_equations(p, v, st.s, prm);
+    if (!time_exceeded() && dual_fix && prm.enable_dual_fix) apply_rule_dual_fix(p, v, st.s, prm, nullptr);
+    if (!time_exceeded() && dup_cols && prm.enable_duplicate_columns) apply_rule_duplicate_columns(p, v, st.s, prm);
+    if (!time_exceeded() && run_redundant) apply_rule_redundant_bounds(p, v, st.s, prm);
+    const bool changed = finish_plan(p);
+    if (!changed) {
+      const bool only_dual = !fixed && !empty_cols && cs_dual && !cs_eq && !doubleton && !dual_fix && !dup_cols &&
+                             !redundant_bounds && prm.enable_column_singletons_dual_infer;
+      const bool only_eq = !fixed && !empty_cols && !cs_dual && cs_eq && !doubleton && !dual_fix && !dup_cols &&
+                           !redundant_bounds && prm.enable_column_singletons_eq;
+      if (only_dual) cs_dual_clean = true;
+      if (only_eq) cs_eq_clean = true;
+    }
+    return changed;
+  };
+
+  auto singleton_rows_to_exhaustion = [&]() {
+    bool any = false;
+    while (!terminal && !time_exceeded()) {
+      const bool ch = row_phase(false, true, false, false, false);
+      any = any || ch;
+      if (!ch) break;
+    }
+    return any;
+  };
+  auto trivial_cleanup = [&]() {
+    bool any = false;
+    while (!terminal && !time_exceeded()) {
+      bool pass = false;
+      pass = col_phase(true, true, false, false, false, false, false, false) || pass;
+      pass = row_phase(true, true, false, false, false) || pass;
+      pass = col_phase(false, true, false, false, false, false, false, false) || pass;
+      any = pass || any;
+      if (!pass) break;
+    }
+    return any;
+  };
+  auto cleanup = [&]() {
+    bool any = false;
+    any = col_phase(true, true, false, false, false, false, false, false) || any;
+    any = col_phase(false, false, false, false, false, true, false, false) || any;
+    any = singleton_rows_to_exhaustion() || any;
+    any = row_phase(true, false, false, false, false) || any;
+    any = col_phase(false, true, false, false, false, false, false, false) || any;
+    return any;
+  };
+  auto repeated_doubleton = [&]() {
+    bool any = false;
+    while (!terminal && !time_exceeded()) {
+      const bool ch = col_phase(false, false, false, false, true, false, false, false);
+      any = any || ch;
+      if (!ch || prm.doubleton_equations_single_batch_per_iter) break;
+    }
+    return any;
+  };
+  // run_fast_phase (gpu_presolve.cu:2833-2863).  The batched singleton
+  // closure the reference switches to at nnz >= 1e7 (:2507-2512) is not
+  // part of this driver.
+  auto fast_phase = [&]() {
+    bool any = false, singleton = false;
+    while (!terminal && !time_exceeded()) {
+      const bool ch_dual = cs_dual_clean ? false : col_phase(false, false, true, false, false, false, false, false);
+      const bool ch_eq = cs_eq_clean ? false : col_phase(false, false, false, true, false, false, false, false);
+      const bool ch = ch_dual || ch_eq;
+      any = any || ch;
+      singleton = singleton || ch;
+      if (ch) any = cleanup() || any;
+      if (!ch) break;
+    }
+    const bool dbl = repeated_doubleton();
+    any = dbl || any;
+    if (dbl || singleton) any = cleanup() || any;
+    return any;
+  };
+  // run_implied_variable_bounds_with_activity_closure (gpu_presolve.cu:2865-2987)
+  auto implied_closure = [&]() {
+    if (!prm.enable_implied_variable_bounds) return false;
+    bool any = false;
+    const int max_rounds = std::max(1, prm.implied_variable_bounds_max_rounds);
+    const int cfg_bound_only = std::max(1, prm.implied_variable_bounds_max_bound_only_rounds);
+    const int64_t pr = cur.lp.A.rows, pc = cur.lp.A.cols, pn = cur.lp.A.nnz;
+    const bool wide_dense =
+        pr >= 512 && pr <= 2048 && pc >= 128 * pr && pc <= 256 * pr && pn >= 2048 * pr && pn <= 8192 * pr;
+    auto effective_limit = [&]() {
+      int eff = cfg_bound_only;
+      const int64_t budget = prm.implied_variable_bounds_bound_only_nnz_round_budget;
+      if (budget <= 0 || cur.lp.A.nnz <= 0) return eff;
+      const int64_t br = std::max<int64_t>(1, budget / cur.lp.A.nnz);
+      if (br < eff) eff = static_cast<int>(br);
+      if (wide_dense) eff = std::max(eff, std::min(2, cfg_bound_only));
+      return eff;
+    };
+    int rounds = 0, bound_only = 0, limit = cfg_bound_only;
+    while (rounds < max_rounds && !terminal && !time_exceeded()) {
+      limit = effective_limit();
+      PhasePlan pp(cur.lp);
+      PhaseStats st(cur.lp);
+      PresolvePlanGpu& p = pp.p;
+      bool group_changed = false, structural = false;
+      while (rounds < max_rounds && !terminal && !time_exceeded()) {
+        p.has_change = false;
+        apply_rule_implied_variable_bounds(p, cur.lp, st.s, prm);
+        ++rounds;
+        res.has_infeasible = p.has_infeasible;
+        res.has_unbounded = p.has_unbounded;
+        terminal = p.has_infeasible || p.has_unbounded;
+        if (terminal || !p.has_change) break;
+        group_changed = true;
+        any = true;
+        structural = p.has_new_A || p.has_row_action || p.has_col_action;
+        if (structural) {
+          bound_only = 0;
+          break;
+        }
+        ++bound_only;
+        if (bound_only >= limit) break;
+      }
+      if (group_changed && !terminal) {
+        apply_plan(cur, p, rs);
+        invalidate_clean();
+      }
+      if (terminal || !group_changed || !structural || bound_only >= limit) break;
+    }
+    if (any && !terminal && !time_exceeded()) any = row_phase(false, false, true, false, false) || any;
+    return any;
+  };
+  auto medium_phase = [&]() {
+    bool any = false, prop = false;
+    prop = row_phase(false, false, true, false, false) || prop;
+    prop = implied_closure() || prop;
+    any = prop || any;
+    if (prop) any = cleanup() || any;
+    const bool dr = row_phase(false, false, false, false, true);
+    any = dr || any;
+    const bool dc = col_phase(false, false, false, false, false, false, true, false);
+    any = dc || any;
+    if (dr || dc) any = cleanup() || any;
+    return any;
+  };
+  auto fixed_row_sequence = [&]() {
+    bool any = false;
+    any = row_phase(true, false, false, false, false) || any;
+    any = row_phase(false, true, false, false, false) || any;
+    any = row_phase(false, false, true, false, false) || any;
+    const bool primal = implied_closure();
+    any = primal || any;
+    if (primal) any = trivial_cleanup() || any;
+    any = row_phase(false, false, false, false, true) || any;
+    return any;
+  };
+  auto trigger_col = [&](bool fixed, bool cs_dual, bool cs_eq, bool doubleton, bool dual_fix, bool dup) {
+    const bool ch = col_phase(fixed, false, cs_dual, cs_eq, doubleton, dual_fix, dup, false);
+    if (ch) (void)trivial_cleanup();
+    return ch;
+  };
+  auto fixed_col_sequence = [&]() {
+    bool any = false;
+    any = trigger_col(true, false, false, false, false, false) || any;
+    any = col_phase(false, true, false, false, false, false, false, false) || any;
+    bool ch = false;
+    do {
+      ch = trigger_col(false, true, false, false, false, false);
+      any = ch || any;
+    } while (ch && !terminal && !time_exceeded());
+    do {
+      ch = trigger_col(false, false, true, false, false, false);
+      any = ch || any;
+    } while (ch && !terminal && !time_exceeded());
+    do {
+      ch = trigger_col(false, false, false, true, false, false);
+      any = ch || any;
+    } while (ch && !prm.doubleton_equations_single_batch_per_iter && !terminal && !time_exceeded());
+    any = trigger_col(false, false, false, false, true, false) || any;
+    any = trigger_col(false, false, false, false, false, true) || any;
+    return any;
+  };
+
+  if (!prm.use_tiered_scheduler) {
+    for (int iter = 0; iter < prm.max_iters && !terminal && !time_exceeded(); ++iter) {
+      bool ch = false;
+      ch = fixed_row_sequence() || ch;
+      ch = fixed_col_sequence() || ch;
+      ++res.iterations;
+      if (!ch) break;
+    }
+  } else {
+    if (prm.enable_tiered_bootstrap && prm.max_iters > 0 && !time_exceeded()) {
+      // Bootstrap (gpu_presolve.cu:3082-3332) without the structure detectors.
+      row_phase(true, false, false, false, false);
+      row_phase(false, true, false, false, false);
+      row_phase(false, false, true, false, false);
+      row_phase(false, false, false, false, true);
+      bool ch = col_phase(true, true, false, false, false, false, false, false);
+      if (ch) trivial_cleanup();
+      col_phase(false, true, false, false, false, false, false, false);
+      do {
+        ch = col_phase(false, false, true, false, false, false, false, false);
+        if (ch) trivial_cleanup();
+      } while (ch && !terminal && !time_exceeded());
+      do {
+        ch = col_phase(false, false, false, true, false, false, false, false);
+        if (ch) trivial_cleanup();
+      } while (ch && !terminal && !time_exceeded());
+      ch = col_phase(false, false, false, false, false, true, false, false);
+      if (ch) trivial_cleanup();
+      ch = col_phase(false, false, false, false, false, false, true, false);
+      if (ch) trivial_cleanup();
+      ++res.iterations;
+    }
+    bool fast = true;
+    int32_t cycle_nnz_before = cur.lp.A.nnz;
+    constexpr double kProgressRatio = 0.95;
+    for (int iter = res.iterations; iter < prm.max_iters && !terminal && !time_exceeded(); ++iter) {
+      bool ch = false;
+      ch = cleanup() || ch;
+      const int32_t nnz_before = cur.lp.A.nnz;
+      if (fast) {
+        ch = fast_phase() || ch;
+        fast = _has_good_nnz_progress(nnz_before, cur.lp.A.nnz, kProgressRatio);
+      } else {
+        ch = medium_phase() || ch;
+        const bool productive = _has_good_nnz_progress(cycle_nnz_before, cur.lp.A.nnz, kProgressRatio);
+        if (!ch || !productive) {
+          ++res.iterations;
+          break;
+        }
+        cycle_nnz_before = cur.lp.A.nnz;
+        fast = true;
+      }
+      ++res.iterations;
+      if (!ch && fast) break;
+    }
+  }
+  if (!terminal && !time_exceeded() && (prm.enable_redundant_bounds || prm.enable_zero_cost_redundant_box_bounds))
+    (void)col_phase(false, false, false, false, false, false, false, true);
+
+  auto& c = ctx();
+  rec.row_red2org = c.to_host(rs.row_r2o.get(), rs.m, "record row red2org");
+  rec.col_red2org = c.to_host(rs.col_r2o.get(), rs.n, "record col red2org");
+  res.reduced_rows = cur.lp.A.rows;
+  res.reduced_cols = cur.lp.A.cols;
+  res.reduced_nnz = cur.lp.A.nnz;
+  res.obj_constant_delta = cur.lp.obj_constant - lp.obj_constant;
+  rec.m1 = res.reduced_rows;
+  rec.n1 = res.reduced_cols;
+  rec.obj_constant_new = cur.lp.obj_constant;
+  rebuild_org2red(rec.row_org2red, rec.row_red2org, rec.m0);
+  rebuild_org2red(rec.col_org2red, rec.col_red2org, rec.n0);
+  if (keep_reduced) {
+    if (cur.owns) {
+      res.reduced_lp = export_lp(cur.lp);
+      res.owns_reduced_lp = true;
+    } else {
+      res.reduced_lp = cur.lp;  // unchanged: the caller's own LP (owns = false)
+      res.owns_reduced_lp = false;
+    }
+  }
+  cur.release();
+  c.sync("presolve exit");
+  res.elapsed_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return res;
+}
+
+}  // namespace
+
+bool _has_good_nnz_progress(std::int32_t nnz_before, std::int32_t nnz_after, double ratio) {
+  if (nnz_before <= 0) return false;
+  return static_cast<double>(nnz_after) < ratio * static_cast<double>(nnz_before);
+}
+
+GpuPresolveSummary run_gpu_presolve_with_record(const LPInfoGpu& lp, const PresolveParams& params) {
+  return run_impl(lp, params, false);
+}
+
+GpuPresolveSummary run_gpu_presolve_with_reduced_lp(const LPInfoGpu& lp, const PresolveParams& params) {
+  return run_impl(lp, params, true);
+}
+
+GpuPresolveSummary::GpuPresolveSummary(GpuPresolveSummary&& o) noexcept
+    : original_rows(std::exchange(o.original_rows, 0)),
+      original_cols(std::exchange(o.original_cols, 0)),
+      reduced_rows(std::exchange(o.reduced_rows, 0)),
+      reduced_cols(std::exchange(o.reduced_cols, 0)),
+      reduced_nnz(std::exchange(o.reduced_nnz, 0)),
+      iterations(std::exchange(o.iterations, 0)),
+      elapsed_seconds(std::exchange(o.elapsed_seconds, 0.0)),
+      obj_constant_delta(std::exchange(o.obj_constant_delta, 0.0)),
+      has_infeasible(std::exchange(o.has_infeasible, false)),
+      has_unbounded(std::exchange(o.has_unbounded, false)),
+      record(std::move(o.record)),
+      reduced_lp(std::exchange(o.reduced_lp, LPInfoGpu{})),
+      owns_reduced_lp(std::exchange(o.owns_reduced_lp, false)) {}
+
+GpuPresolveSummary& GpuPresolveSummary::operator=(GpuPresolveSummary&& o) noexcept {
+  if (this != &o) {
+    free_gpu_presolve_reduced_lp(*this);
+    original_rows = std::exchange(o.original_rows, 0);
+    original_cols = std::exchange(o.original_cols, 0);
+    reduced_rows = std::exchange(o.reduced_rows, 0);
+    reduced_cols = std::exchange(o.reduced_cols, 0);
+    reduced_nnz = std::exchange(o.reduced_nnz, 0);
+    iterations = std::exchange(o.iterations, 0);
+    elapsed_seconds = std::exchange(o.elapsed_seconds, 0.0);
+    obj_constant_delta = std::exchange(o.obj_constant_delta, 0.0);
+    has_infeasible = std::exchange(o.has_infeasible, false);
+    has_unbounded = std::exchange(o.has_unbounded, false);
+    record = std::move(o.record);
+    reduced_lp = std::exchange(o.reduced_lp, LPInfoGpu{});
+    owns_reduced_lp = std::exchange(o.owns_reduced_lp, false);
+  }
+  return *this;
+}
+
+void free_gpu_presolve_reduced_lp(GpuPresolveSummary& summary) {
+  if (!summary.owns_reduced_lp) {
+    summary.reduced_lp = LPInfoGpu{};
+    return;
+  }
+  LPInfoGpu& r = summary.reduced_lp;
+  for (void* p : {static_cast<void*>(r.A.rowPtr), static_cast<void*>(r.A.colVal), static_cast<void*>(r.A.nzVal),
+                  static_cast<void*>(r.AT.rowPtr), static_cast<void*>(r.AT.colVal), static_cast<void*>(r.AT.nzVal),
+                  static_cast<void*>(r.c), static_cast<void*>(r.AL), static_cast<void*>(r.AU),
+                  static_cast<void*>(r.l), static_cast<void*>(r.u)})
+    if (p) (void)hipFree(p);
+  summary.reduced_lp = LPInfoGpu{};
+  summary.owns_reduced_lp = false;
+}
+
+// ---- PostsolveTapeGpu (presolve_structs.cu:9-60) ---------------------------
+PostsolveTapeGpu::~PostsolveTapeGpu() { reset(); }
+
+PostsolveTapeGpu::PostsolveTapeGpu(PostsolveTapeGpu&& o) noexcept
+    : types(std::exchange(o.types, nullptr)),
+      index_starts(std::exchange(o.index_starts, nullptr)),
+      value_starts(std::exchange(o.value_starts, nullptr)),
+      dual_modes(std::exchange(o.dual_modes, nullptr)),
+      indices(std::exchange(o.indices, nullptr)),
+      vals(std::exchange(o.vals, nullptr)),
+      record_count(std::exchange(o.record_count, 0)),
+      index_count(std::exchange(o.index_count, 0)),
+      value_count(std::exchange(o.value_count, 0)),
+      record_capacity(std::exchange(o.record_capacity, 0)),
+      index_capacity(std::exchange(o.index_capacity, 0)),
+      value_capacity(std::exchange(o.value_capacity, 0)) {}
+
+PostsolveTapeGpu& PostsolveTapeGpu::operator=(PostsolveTapeGpu&& o) noexcept {
+  if (this != &o) {
+    reset();
+    types = std::exchange(o.types, nullptr);
+    index_starts = std::exchange(o.index_starts, nullptr);
+    value_starts = std::exchange(o.value_starts, nullptr);
+    dual_modes = std::exchange(o.dual_modes, nullptr);
+    indices = std::exchange(o.indices, nullptr);
+    vals = std::exchange(o.vals, nullptr);
+    record_count = std::exchange(o.record_count, 0);
+    index_count = std::exchange(o.index_count, 0);
+    value_count = std::exchange(o.value_count, 0);
+    record_capacity = std::exchange(o.record_capacity, 0);
+    index_capacity = std::exchange(o.index_capacity, 0);
+    value_capacity = std::exchange(o.value_capacity, 0);
+  }
+  return *this;
+}
+
+void PostsolveTapeGpu::reset() noexcept {
+  for (void* p : {static_cast<void*>(types), static_cast<void*>(index_starts), static_cast<void*>(value_starts),
+                  static_cast<void*>(dual_modes), static_cast<void*>(indices), static_cast<void*>(vals)})
+    if (p) (void)hipFree(p);
+  types = nullptr;
+  index_starts = nullptr;
+  value_starts = nullptr;
+  dual_modes = nullptr;
+  indices = nullptr;
+  vals = nullptr;
+  record_count = index_count = value_count = 0;
+  record_capacity = index_capacity = value_capacity = 0;
+}
+
+}  // namespace gpu_presolver::presolve

@@ -510,4 +510,471 @@ void apply_rule_column_singletons_dual_infer(PresolvePlanGpu& plan, const LPInfo
 
 void release_column_singletons_workspace() noexcept {}
 
+
+// ---------------------------------------------------------------------------
+// Doubleton equations (rule_doubleton_equations.cu)
+namespace {
+
+constexpr int kDblMatchRounds = 8;
+constexpr int kDblInnerRounds = 1024;
+
+struct DblWork {
+  DBuf<uint8_t> acceptable, active, selected, blocked;
+  DBuf<int32_t> cec, ckc, owner, live_nnz;
+  DBuf<double> cev, ckv;
+  DBuf<uint8_t> is_elim, is_fixed_keep;
+  DBuf<double> alpha_of, beta_of, fixed_val_of;
+  DBuf<int32_t> row_of, keep_of;
+  DblWork(int32_t m, int32_t n)
+      : acceptable(m), active(m), selected(m), blocked(n), cec(m), ckc(m), owner(n), live_nnz(n), cev(m), ckv(m),
+        is_elim(n), is_fixed_keep(n), alpha_of(n), beta_of(n), fixed_val_of(n), row_of(n), keep_of(n) {}
+  gpk_dbl view() {
+    return gpk_dbl{acceptable, active, selected, blocked, cec, ckc, cev, ckv, owner, live_nnz};
+  }
+  gpk_dbl_cols cols() { return gpk_dbl_cols{is_elim, is_fixed_keep, alpha_of, beta_of, fixed_val_of, row_of, keep_of}; }
+};
+
+struct DblTapeRec {
+  int32_t i3[3];
+  double v12[12];
+};
+
+// One batch (rule_doubleton_equations.cu:1432-1677).  Returns true when the
+// plan changed; os/of receive the substitution and fixed-pair objective
+// deltas of the batch.
+bool dbl_batch_once(PresolvePlanGpu& plan, const LPInfoGpu& lp, const DeviceCsrMatrix& AT_src,
+                    const PresolveParams& prm, DblWork& w, std::vector<DblTapeRec>* recs, double* os, double* of) {
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0 || n == 0) return false;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_plan P = plan_view(plan);
+  const gpk_csr A = csr_view(plan.has_new_A ? plan.new_A : lp.A);
+  const gpk_csr AT = csr_view(AT_src);
+  const double zt = prm.zero_tol, tol = prm.bound_tol;
+  const double rtol = std::fmax(1.0e-9, prm.bound_tol);
+  gpk_dbl d = w.view();
+  check(gpk_dbl_live_nnz(AT, P, zt, d.live_nnz, s), "doubleton live nnz");
+  int32_t* f = c.flags();
+  check(gpk_dbl_candidates(A, AT, P, d, zt, tol, rtol, prm.doubleton_equations_max_fill_in_proxy, 1, f, s),
+        "doubleton candidates");
+  zero(d.blocked, n, "doubleton blocked");
+  for (int r = 0; r < kDblMatchRounds; ++r) {
+    check(gpk_dbl_claim(m, n, d, s), "doubleton claim");
+    check(gpk_dbl_select(m, d, f, s), "doubleton select");
+  }
+  const int32_t* h = c.read_flags(2);
+  const int32_t acceptable = h[0], k = h[1];
+  const double ratio = acceptable == 0 ? 0.0 : static_cast<double>(k) / acceptable;
+  if (k == 0 || (prm.doubleton_equations_scan && (k < prm.doubleton_equations_min_selected_per_batch ||
+                                                  ratio < prm.doubleton_equations_min_selected_ratio)))
+    return false;
+  DBuf<int32_t> prefix(static_cast<int64_t>(m) + 1), sel(k);
+  amd::scan_mask_count(d.selected, m, prefix);
+  check(gpk_pack_index(d.selected, prefix, m, sel, s), "doubleton pack selected");
+  gpk_dbl_cols cols = w.cols();
+  zero(cols.is_elim, n, "doubleton is_elim");
+  zero(cols.is_fixed_keep, n, "doubleton is_fixed_keep");
+  DBuf<uint8_t> subst(k), fixedp(k);
+  DBuf<double> osub(k), ofix(k), rv(12 * static_cast<int64_t>(k));
+  DBuf<int32_t> ri(3 * static_cast<int64_t>(k));
+  f = c.flags();
+  check(gpk_dbl_apply(k, sel, P, d, cols, tol, subst, fixedp, osub, ofix, ri, rv, f, s), "doubleton apply");
+  h = c.read_flags(4);
+  const bool infeasible = h[0] != 0;
+  const int32_t subst_count = h[2], fixed_count = h[3];
+  *os = amd::ordered_masked_sum(subst, osub, k);
+  *of = amd::ordered_masked_sum(fixedp, ofix, k);
+  if (infeasible) {
+    plan.has_infeasible = true;
+    return false;
+  }
+  if (recs != nullptr && subst_count > 0) {
+    const std::vector<uint8_t> hs = c.to_host(subst.get(), k, "doubleton tape mask");
+    const std::vector<int32_t> hi = c.to_host(ri.get(), 3 * static_cast<int64_t>(k), "doubleton tape ids");
+    const std::vector<double> hv = c.to_host(rv.get(), 12 * static_cast<int64_t>(k), "doubleton tape vals");
+    for (int32_t t = 0; t < k; ++t) {
+      if (!hs[t]) continue;
+      DblTapeRec r;
+      std::copy(hi.begin() + 3 * t, hi.begin() + 3 * t + 3, r.i3);
+      std::copy(hv.begin() + 12 * static_cast<size_t>(t), hv.begin() + 12 * static_cast<size_t>(t) + 12, r.v12);
+      recs->push_back(r);
+    }
+  }
+  int32_t total = 0;
+  DBuf<int32_t> dptr;
+  if (subst_count > 0) {
+    dptr.alloc(static_cast<int64_t>(m) + 1);
+    check(gpk_dbl_delta_count(A, P, cols, zt, dptr, s), "doubleton delta count");
+    total = amd::scan_counts(dptr, m, dptr);
+  }
+  // Row shift: fixed keep columns, then substituted columns when A is rebuilt.
+  if (fixed_count > 0 || total > 0)
+    check(gpk_dbl_shift(A, P, cols, zt, total > 0 ? 1 : 0, s), "doubleton row shift");
+  if (total > 0) {
+    DBuf<int32_t> dcol(total);
+    DBuf<double> dval(total);
+    check(gpk_dbl_delta_fill(A, P, cols, zt, dptr, dcol, dval, s), "doubleton delta fill");
+    DeviceCsrMatrix out;
+    out.rows = A.rows;
+    out.cols = A.cols;
+    DBuf<int32_t> optr(static_cast<int64_t>(m) + 1);
+    check(gpk_dbl_merge(A, dptr, dcol, dval, zt, nullptr, optr, nullptr, nullptr, s), "doubleton merge count");
+    out.nnz = amd::scan_counts(optr, m, optr);
+    DBuf<int32_t> oidx(out.nnz);
+    DBuf<double> oval(out.nnz);
+    check(gpk_dbl_merge(A, dptr, dcol, dval, zt, optr, nullptr, oidx, oval, s), "doubleton merge");
+    out.rowPtr = optr.release();
+    out.colVal = oidx.release();
+    out.nzVal = oval.release();
+    if (plan.has_new_A) amd::free_csr(plan.new_A);
+    plan.new_A = out;
+    plan.has_new_A = true;
+  }
+  plan.has_row_action = subst_count > 0;
+  plan.has_col_action = true;
+  plan.has_change = true;
+  return true;
+}
+
+// Single-doubleton path, doubleton_equations_scan = false
+// (rule_doubleton_equations.cu:1913-2084, 266-445, 1779-1909).
+void dbl_scalar(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm) {
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0 || n == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_plan P = plan_view(plan);
+  const gpk_csr A = csr_view(plan.has_new_A ? plan.new_A : lp.A);
+  const gpk_csr AT = csr_view(lp.AT);
+  const double zt = prm.zero_tol, tol = prm.bound_tol;
+  const double rtol = std::fmax(1.0e-9, prm.bound_tol);
+  DblWork w(m, n);
+  gpk_dbl d = w.view();
+  check(gpk_dbl_live_nnz(AT, P, zt, d.live_nnz, s), "doubleton live nnz");
+  int32_t* f = c.flags();
+  check(gpk_dbl_candidates(A, AT, P, d, zt, tol, rtol, prm.doubleton_equations_max_fill_in_proxy, 0, f, s),
+        "doubleton candidates");
+  if (c.read_flags(1)[0] == 0) return;
+  const std::vector<int32_t> rows = masked_indices(d.acceptable, m);
+  const int32_t target = rows.front();  // smallest row index (atomicMin in the reference)
+  const int32_t ec = c.scalar_to_host(d.cec + target, "doubleton elim col");
+  const int32_t kc = c.scalar_to_host(d.ckc + target, "doubleton keep col");
+  const double ev = c.scalar_to_host(d.cev + target, "doubleton elim val");
+  const double kv = c.scalar_to_host(d.ckv + target, "doubleton keep val");
+  const double rhs = c.scalar_to_host(plan.new_AU + target, "doubleton rhs");
+  const double ol_e = c.scalar_to_host(plan.new_l + ec, "doubleton l_e");
+  const double ou_e = c.scalar_to_host(plan.new_u + ec, "doubleton u_e");
+  const double oc_e = c.scalar_to_host(plan.new_c + ec, "doubleton c_e");
+  const double ol_k = c.scalar_to_host(plan.new_l + kc, "doubleton l_k");
+  const double ou_k = c.scalar_to_host(plan.new_u + kc, "doubleton u_k");
+  const double oc_k = c.scalar_to_host(plan.new_c + kc, "doubleton c_k");
+  const double al = -kv / ev, be = rhs / ev;
+  double ml = -HUGE_VAL, mu = HUGE_VAL;
+  if (std::isfinite(ol_e)) {
+    const double b = (ol_e - be) / al;
+    if (al > 0.0) ml = b; else mu = b;
+  }
+  if (std::isfinite(ou_e)) {
+    const double b = (ou_e - be) / al;
+    if (al > 0.0) mu = b; else ml = b;
+  }
+  if (ml > mu) std::swap(ml, mu);
+  const double nkl = std::isfinite(ml) ? std::fmax(ol_k, ml) : ol_k;
+  const double nku = std::isfinite(mu) ? std::fmin(ou_k, mu) : ou_k;
+  if (nkl > nku + tol) {
+    plan.has_infeasible = true;
+    return;
+  }
+  const int32_t ec_b = c.scalar_to_host(lp.AT.rowPtr + ec, "doubleton AT ptr");
+  const int32_t ec_e = c.scalar_to_host(lp.AT.rowPtr + ec + 1, "doubleton AT ptr");
+  check(gpk_dbl_scalar_shift(AT, ec, ec_e - ec_b, be, plan.new_AL, plan.new_AU, s), "doubleton scalar shift");
+  check(gpk_fill_f64(plan.new_l + kc, nkl, 1, s), "doubleton set l");
+  check(gpk_fill_f64(plan.new_u + kc, nku, 1, s), "doubleton set u");
+  check(gpk_fill_f64(plan.new_c + kc, oc_k + oc_e * al, 1, s), "doubleton set c");
+  check(gpk_fill_u8(plan.keep_row_mask + target, 0, 1, s), "doubleton drop row");
+  check(gpk_fill_u8(plan.keep_col_mask + ec, 0, 1, s), "doubleton drop col");
+  if (prm.record_postsolve_tape)
+    append_record(plan.tape, RT::DoubletonEquation, {ec, kc, target, 0},
+                  {ev, kv, rhs, ol_e, ou_e, ol_k, ou_k, nkl, nku, oc_e, 0.0, 0.0}, DM::Minimal);
+  DeviceCsrMatrix out;
+  out.rows = A.rows;
+  out.cols = A.cols;
+  DBuf<int32_t> optr(static_cast<int64_t>(m) + 1);
+  check(gpk_dbl_scalar_rows(A, target, ec, kc, al, zt, nullptr, optr, nullptr, nullptr, s), "doubleton rows count");
+  out.nnz = amd::scan_counts(optr, m, optr);
+  DBuf<int32_t> oidx(out.nnz);
+  DBuf<double> oval(out.nnz);
+  check(gpk_dbl_scalar_rows(A, target, ec, kc, al, zt, optr, nullptr, oidx, oval, s), "doubleton rows");
+  out.rowPtr = optr.release();
+  out.colVal = oidx.release();
+  out.nzVal = oval.release();
+  if (plan.has_new_A) amd::free_csr(plan.new_A);
+  plan.new_A = out;
+  plan.has_new_A = true;
+  plan.obj_constant_delta += oc_e * be;
+  plan.has_row_action = true;
+  plan.has_col_action = true;
+  plan.has_change = true;
+}
+
+}  // namespace
+
+// rule_doubleton_equations.cu:1679-1777 (batch driver) and 1913-2084
+void apply_rule_doubleton_equations(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                                    const PresolveParams& prm) {
+  (void)stats;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  if (!prm.doubleton_equations_scan) {
+    dbl_scalar(plan, lp, prm);
+    return;
+  }
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0 || n == 0) return;
+  amd::sort_rows_if_needed(lp.AT);
+  DeviceCsrMatrix at_own;
+  bool owns_at = false;
+  const DeviceCsrMatrix* at = &lp.AT;
+  const int limit = prm.doubleton_equations_max_batch_rounds > 0 &&
+                            prm.doubleton_equations_max_batch_rounds < kDblInnerRounds
+                        ? prm.doubleton_equations_max_batch_rounds
+                        : kDblInnerRounds;
+  const bool time_limit = prm.doubleton_equations_max_time > 0.0 && std::isfinite(prm.doubleton_equations_max_time);
+  const auto start = std::chrono::steady_clock::now();
+  std::vector<DblTapeRec> recs;
+  std::vector<double> objs;
+  DblWork w(m, n);
+  try {
+    for (int round = 0; round < limit; ++round) {
+      if (time_limit) {
+        const std::chrono::duration<double> el = std::chrono::steady_clock::now() - start;
+        if (el.count() >= prm.doubleton_equations_max_time) break;
+      }
+      double os = 0.0, of = 0.0;
+      const bool changed = dbl_batch_once(plan, lp, *at, prm, w, prm.record_postsolve_tape ? &recs : nullptr, &os, &of);
+      if (plan.has_infeasible || plan.has_unbounded || !changed) break;
+      objs.push_back(os);
+      objs.push_back(of);
+      if (plan.has_new_A) {
+        DeviceCsrMatrix t = amd::transpose_csr(plan.new_A);
+        if (owns_at) amd::free_csr(at_own);
+        at_own = t;
+        owns_at = true;
+        at = &at_own;
+      }
+    }
+  } catch (...) {
+    if (owns_at) amd::free_csr(at_own);
+    throw;
+  }
+  if (owns_at) amd::free_csr(at_own);
+  for (const DblTapeRec& r : recs)
+    append_record(plan.tape, RT::DoubletonEquation, {r.i3[0], r.i3[1], r.i3[2], 0}, {r.v12[0], r.v12[1], r.v12[2],
+                  r.v12[3], r.v12[4], r.v12[5], r.v12[6], r.v12[7], r.v12[8], r.v12[9], r.v12[10], r.v12[11]},
+                  DM::Minimal);
+  for (double o : objs) plan.obj_constant_delta += o;
+}
+
+// ---------------------------------------------------------------------------
+// rule_dual_fix.cu:174-554
+void apply_rule_dual_fix(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                         const PresolveParams& prm, const std::uint8_t* candidate_col_mask) {
+  (void)stats;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t n = lp.A.cols, m = lp.A.rows;
+  if (n == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_plan P = plan_view(plan);
+  DBuf<uint8_t> fixed(n), inf_fix(n), finite_fix(n);
+  DBuf<double> v(n), contrib(n);
+  int32_t* f = c.flags();
+  check(gpk_dual_fix_candidates(csr_view(lp.AT), P, candidate_col_mask, prm.zero_tol, prm.bound_tol,
+                                prm.dual_fix_max_nonzero_cost_col_degree, fixed, inf_fix, finite_fix, v, contrib, f, s),
+        "dual fix candidates");
+  const int32_t* h = c.read_flags(3);
+  if (h[0] == 1) {
+    plan.has_infeasible = true;
+    return;
+  }
+  if (h[0] == 2) {
+    plan.has_unbounded = true;
+    return;
+  }
+  if (!h[1]) return;
+  const bool any_inf = h[2] != 0;
+  DBuf<double> shift(m);
+  DBuf<uint8_t> row_remove(m);
+  f = c.flags();
+  if (m > 0)
+    check(gpk_dual_fix_rows(csr_view(lp.A), P, finite_fix, inf_fix, v, shift, row_remove, f, s), "dual fix rows");
+  const bool row_changed = m > 0 && c.read_flags(4)[3] != 0;
+  if (prm.record_postsolve_tape) {
+    if (any_inf) {
+      // FixedColInf: the column, its infinite side and every live row it
+      // touches with that row's entries (rule_dual_fix.cu:386-470).
+      const std::vector<int32_t> cols = masked_indices(inf_fix, n);
+      const std::vector<double> hv = c.to_host(v.get(), n, "dual fix tape v");
+      const std::vector<double> hl = c.to_host(plan.new_l, n, "dual fix tape l");
+      const std::vector<double> hu = c.to_host(plan.new_u, n, "dual fix tape u");
+      const std::vector<uint8_t> kr = c.to_host(plan.keep_row_mask, m, "dual fix tape keep_row");
+      const std::vector<double> hal = c.to_host(plan.new_AL, m, "dual fix tape AL");
+      const std::vector<double> hau = c.to_host(plan.new_AU, m, "dual fix tape AU");
+      const std::vector<int32_t> tp = c.to_host(lp.AT.rowPtr, static_cast<int64_t>(n) + 1, "dual fix tape AT ptr");
+      const std::vector<int32_t> ti = c.to_host(lp.AT.colVal, lp.AT.nnz, "dual fix tape AT idx");
+      const std::vector<int32_t> ap = c.to_host(lp.A.rowPtr, static_cast<int64_t>(m) + 1, "dual fix tape A ptr");
+      const std::vector<int32_t> ai = c.to_host(lp.A.colVal, lp.A.nnz, "dual fix tape A idx");
+      const std::vector<double> av = c.to_host(lp.A.nzVal, lp.A.nnz, "dual fix tape A val");
+      std::vector<int32_t> idx;
+      std::vector<double> vals;
+      for (int32_t j : cols) {
+        const bool pos = hv[j] > 0.0;
+        idx.assign({pos ? 1 : -1, j});
+        vals.assign({0.0, pos ? hl[j] : hu[j]});
+        int32_t active = 0;
+        for (int32_t q = tp[j]; q < tp[j + 1]; ++q) {
+          const int32_t row = ti[q];
+          if (!kr[row]) continue;
+          ++active;
+          idx.push_back(ap[row + 1] - ap[row]);
+          vals.push_back(std::isfinite(hal[row]) ? hal[row] : hau[row]);
+          for (int32_t t = ap[row]; t < ap[row + 1]; ++t) {
+            idx.push_back(ai[t]);
+            vals.push_back(av[t]);
+          }
+        }
+        vals[0] = static_cast<double>(active);
+        append_record(plan.tape, RT::FixedColInf, idx.data(), idx.size(), vals.data(), vals.size(), DM::Exact);
+      }
+    }
+    if (prm.record_postsolve_tape_cpu) {
+      DBuf<uint8_t> keep_after(m);
+      d2d(keep_after.get(), plan.keep_row_mask, m, "dual fix keep_after");
+      f = c.flags();
+      if (m > 0) check(gpk_delete_rows(m, keep_after, row_remove, f, s), "dual fix keep_after");
+      amd::fixed_col_tape(&plan.tape, fixed, inf_fix, v, keep_after, plan.new_c, lp.AT, n);
+    }
+  }
+  check(gpk_dual_fix_apply(m, n, P, fixed, v, shift, row_remove, s), "dual fix apply");
+  plan.obj_constant_delta += amd::ordered_masked_sum(fixed, contrib, n);
+  plan.has_row_action = plan.has_row_action || row_changed;
+  plan.has_col_action = true;
+  plan.has_change = true;
+}
+
+// ---------------------------------------------------------------------------
+// rule_duplicate_columns.cu:95-842
+void apply_rule_duplicate_columns(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                                  const PresolveParams& prm) {
+  (void)stats;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t n = lp.A.cols, m = lp.A.rows;
+  if (n <= 1) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  if (lp.AT.nnz > 0) amd::sort_rows_if_needed(lp.AT);
+  const gpk_plan P = plan_view(plan);
+  const gpk_csr AT = csr_view(lp.AT);
+  const double zt = prm.zero_tol;
+  const double ctol = std::fmax(prm.zero_tol, prm.bound_tol);
+  const double otol = ctol;
+  DBuf<uint64_t> hash(n), hash_alt(n);
+  DBuf<int32_t> cols(n), cols_alt(n);
+  check(gpk_dup_col_hash(AT, P, zt, hash, s), "duplicate cols hash");
+  check(gpk_iota_i32(cols, n, s), "duplicate cols iota");
+  {
+    DBuf<int32_t> scratch(gpk_radix_scratch_ints(n));
+    check(gpk_radix_sort_u64_i32(hash, cols, hash_alt, cols_alt, n, 64, scratch, s), "duplicate cols sort");
+  }
+  const double max_ratio = prm.duplicate_columns_max_abs_ratio;
+  const bool use_guard = std::isfinite(max_ratio) && prm.duplicate_columns_scale_guard_min_groups > 1;
+  DBuf<int32_t> guard(1);
+  if (use_guard) {
+    zero(guard.get(), 1, "duplicate cols guard count");
+    check(gpk_dup_col_guard(AT, P, hash, cols, zt, ctol, otol, max_ratio,
+                            prm.duplicate_columns_scale_guard_max_group_size, guard, s),
+          "duplicate cols guard");
+  }
+  DBuf<uint8_t> del(n), fixed(n);
+  DBuf<double> fixed_val(n), rec5(5 * static_cast<int64_t>(n)), contrib(n);
+  DBuf<int32_t> merge_to(n);
+  zero(del.get(), n, "duplicate cols delete mask");
+  zero(fixed.get(), n, "duplicate cols fixed mask");
+  int32_t* f = c.flags();
+  check(gpk_dup_col_groups(AT, P, hash, cols, zt, ctol, otol, max_ratio, use_guard ? guard.get() : nullptr,
+                           prm.duplicate_columns_scale_guard_min_groups, del, fixed, fixed_val, merge_to, rec5, f, s),
+        "duplicate cols groups");
+  const int32_t* h = c.read_flags(3);
+  if (h[0]) {
+    plan.has_unbounded = true;
+    return;
+  }
+  const bool merged = h[1] != 0, any_fixed = h[2] != 0;
+  if (!(merged || any_fixed)) return;
+  if (merged && prm.record_postsolve_tape) {
+    const std::vector<int32_t> deleted = masked_indices(del, n);
+    const std::vector<int32_t> hto = c.to_host(merge_to.get(), n, "duplicate cols merge_to");
+    const std::vector<double> hr = c.to_host(rec5.get(), 5 * static_cast<int64_t>(n), "duplicate cols values");
+    for (int32_t k : deleted) {
+      const double* r = hr.data() + 5 * static_cast<size_t>(k);
+      append_record(plan.tape, RT::DuplicateColumn, {k, hto[k]}, {r[0], r[1], r[2], r[3], r[4]}, DM::Minimal);
+    }
+  }
+  if (any_fixed) {
+    if (prm.record_postsolve_tape)
+      amd::fixed_col_tape(&plan.tape, fixed, nullptr, fixed_val, plan.keep_row_mask, plan.new_c, lp.AT, n);
+    if (m > 0)
+      check(gpk_row_shift(csr_view(lp.A), plan.keep_row_mask, fixed, fixed_val, -1.0, plan.new_AL, plan.new_AU, s),
+            "duplicate cols row shift");
+    plan.has_row_action = true;
+  }
+  check(gpk_dup_col_apply(n, P, del, fixed, fixed_val, contrib, s), "duplicate cols apply");
+  plan.obj_constant_delta += amd::ordered_masked_sum(fixed, contrib, n);
+  plan.has_col_action = true;
+  plan.has_change = true;
+}
+
+// ---------------------------------------------------------------------------
+// rule_redundant_bounds.cu:31-411 (opt-in)
+void apply_rule_redundant_bounds(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                                 const PresolveParams& prm) {
+  (void)stats;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t m = lp.A.rows, n = lp.A.cols;
+  if (m == 0 || n == 0) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const gpk_plan P = plan_view(plan);
+  const gpk_csr A = csr_view(lp.A), AT = csr_view(lp.AT);
+  DBuf<uint8_t> dlo(n), dhi(n), sl(n), su(n);
+  DBuf<uint64_t> rkey(m);
+  bool changed = false;
+  for (;;) {
+    check(gpk_rb_candidates(A, AT, P, lp.c, prm.enable_redundant_bounds ? 1 : 0, prm.zero_tol, prm.bound_tol, dlo,
+                            dhi, s),
+          "redundant bounds candidates");
+    int32_t* f = c.flags();
+    check(gpk_rb_select(A, AT, dlo, dhi, rkey, sl, su, f, s), "redundant bounds select");
+    if (c.read_flags(1)[0] == 0) break;
+    if (prm.record_postsolve_tape) {
+      const std::vector<uint8_t> hsl = c.to_host(sl.get(), n, "redundant bounds lo");
+      const std::vector<uint8_t> hsu = c.to_host(su.get(), n, "redundant bounds hi");
+      const std::vector<double> hl = c.to_host(plan.new_l, n, "redundant bounds l");
+      const std::vector<double> hu = c.to_host(plan.new_u, n, "redundant bounds u");
+      for (int32_t j = 0; j < n; ++j) {
+        if (!hsl[j] && !hsu[j]) continue;
+        const double nl = hsl[j] ? -HUGE_VAL : hl[j];
+        const double nu = hsu[j] ? HUGE_VAL : hu[j];
+        append_record(plan.tape, RT::BoundChangeNoRow, {j}, {hl[j], hu[j], nl, nu}, DM::Minimal);
+      }
+    }
+    check(gpk_rb_apply(n, P, sl, su, s), "redundant bounds apply");
+    changed = true;
+  }
+  if (changed) {
+    plan.has_col_action = true;
+    plan.has_change = true;
+  }
+}
+
 }  // namespace gpu_presolver::presolve

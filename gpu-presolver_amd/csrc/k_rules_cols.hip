@@ -1029,6 +1029,18 @@ __global__ void k_dbl_scalar_rows(gpk_csr A, int32_t target, int32_t ec, int32_t
   if (!out_idx) counts[r] = cnt;
 }
 
+
+// Row shift of the single-doubleton path: every row t holding the eliminated
+// column in the phase-entry AT gets AL/AU -= a_te * beta.
+__global__ void k_dbl_scalar_shift(gpk_csr AT, int32_t ec, double beta, double* AL, double* AU) {
+  const int32_t q = AT.ptr[ec] + blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= AT.ptr[ec + 1]) return;
+  const int32_t t = AT.idx[q];
+  const double sh = AT.val[q] * beta;
+  AL[t] -= sh;
+  AU[t] -= sh;
+}
+
 // ------------------------------------------------ redundant bounds ----
 __device__ bool rb_rest(const gpk_csr& A, const gpk_plan& p, int32_t i, int32_t j, bool want_min, double* out) {
   double acc = 0.0;
@@ -1261,6 +1273,10 @@ hipError_t gpk_dbl_scalar_rows(gpk_csr A, int32_t target, int32_t ec, int32_t kc
                                const int32_t* out_ptr, int32_t* counts, int32_t* out_idx, double* out_val,
                                hipStream_t stream) {
   GPK_LAUNCH(k_dbl_scalar_rows, A.rows, A, target, ec, kc, alpha, zero_tol, out_ptr, counts, out_idx, out_val);
+}
+hipError_t gpk_dbl_scalar_shift(gpk_csr AT, int32_t ec, int32_t ec_len, double beta, double* AL, double* AU,
+                                hipStream_t stream) {
+  GPK_LAUNCH(k_dbl_scalar_shift, ec_len, AT, ec, beta, AL, AU);
 }
 hipError_t gpk_rb_candidates(gpk_csr A, gpk_csr AT, gpk_plan plan, const double* c_lp, int32_t one_sided,
                              double zero_tol, double tol, uint8_t* dlo, uint8_t* dhi, hipStream_t stream) {

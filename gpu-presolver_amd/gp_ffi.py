@@ -1,0 +1,367 @@
+"""ctypes binding of libgpu_presolver_amd.so (the HIP library, C-ABI in
+include/gp_presolve_capi.h).
+
+Device memory is managed through the HIP runtime directly (libamdhip64), so
+the binding needs neither torch nor a CUDA-compat layer.  The functions take
+and return host numpy arrays with the same layout as the CPU oracle's
+binding, which lets a test compare the two field by field.  There is no CPU
+fallback: if the library is not built or no GPU is present, every call
+raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+import gp_types as T
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgpu_presolver_amd.so")
+_lib = None
+_hip = None
+
+H2D, D2H, D2D = 1, 2, 3
+
+
+class GpError(RuntimeError):
+    pass
+
+
+def hip():
+    """The HIP runtime (device memory, copies, events, streams)."""
+    global _hip
+    if _hip is None:
+        H = C.CDLL("libamdhip64.so", mode=C.RTLD_GLOBAL)
+        vp = C.c_void_p
+        H.hipMalloc.argtypes = [C.POINTER(vp), C.c_size_t]
+        H.hipFree.argtypes = [vp]
+        H.hipMemcpy.argtypes = [vp, vp, C.c_size_t, C.c_int]
+        H.hipMemset.argtypes = [vp, C.c_int, C.c_size_t]
+        H.hipGetErrorString.restype = C.c_char_p
+        H.hipGetErrorString.argtypes = [C.c_int]
+        H.hipGetDeviceCount.argtypes = [C.POINTER(C.c_int)]
+        H.hipEventCreate.argtypes = [C.POINTER(vp)]
+        H.hipEventRecord.argtypes = [vp, vp]
+        H.hipEventSynchronize.argtypes = [vp]
+        H.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), vp, vp]
+        H.hipEventDestroy.argtypes = [vp]
+        H.hipStreamCreate.argtypes = [C.POINTER(vp)]
+        H.hipStreamDestroy.argtypes = [vp]
+        H.hipStreamSynchronize.argtypes = [vp]
+        _hip = H
+    return _hip
+
+
+def hip_check(rc, what):
+    if rc != 0:
+        raise GpError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+
+
+def device_count():
+    n = C.c_int(0)
+    rc = hip().hipGetDeviceCount(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GpError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        hip()
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.gp_last_error.restype = C.c_char_p
+        L.gp_version.restype = C.c_char_p
+        L.gp_params_default.argtypes = [C.POINTER(T.GpParams)]
+        L.gp_device_free.argtypes = [vp]
+        L.gp_row_activity.argtypes = [C.POINTER(T.GpCsr), vp, vp, C.c_double, vp, vp, vp, vp, vp]
+        L.gp_col_max_abs.argtypes = [C.POINTER(T.GpCsr), vp, vp]
+        L.gp_compact_csr.argtypes = [C.POINTER(T.GpCsr), vp, vp, C.POINTER(T.GpCsr)]
+        L.gp_transpose_csr.argtypes = [C.POINTER(T.GpCsr), C.POINTER(T.GpCsr)]
+        L.gp_rule_apply.argtypes = [C.c_int32, C.POINTER(T.GpPlanView), C.POINTER(T.GpLp),
+                                    C.POINTER(T.GpStatsView), C.POINTER(T.GpParams), vp, C.POINTER(vp)]
+        L.gp_tape_view.argtypes = [vp, C.POINTER(T.GpRecordView)]
+        L.gp_tape_free.argtypes = [vp]
+        L.gp_presolve.argtypes = [C.POINTER(T.GpLp), C.POINTER(T.GpParams), C.c_int32, C.POINTER(vp)]
+        L.gp_presolve_summary.argtypes = [vp, C.POINTER(T.GpSummary)]
+        L.gp_presolve_record.argtypes = [vp, C.POINTER(T.GpRecordView)]
+        L.gp_presolve_reduced_lp.argtypes = [vp, C.POINTER(T.GpLp)]
+        L.gp_presolve_free.argtypes = [vp]
+        L.gp_postsolve.argtypes = [C.POINTER(T.GpRecordView), vp, vp, vp, C.POINTER(T.GpLp), vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise GpError(f"gp status {rc}: " + lib().gp_last_error().decode())
+
+
+def params(**kw):
+    p = T.GpParams()
+    lib().gp_params_default(C.byref(p))
+    return p.set(**kw)
+
+
+# ---- device memory ---------------------------------------------------------
+class DeviceArray:
+    """hipMalloc'ed copy of a numpy array (freed with the object)."""
+
+    def __init__(self, host=None, *, size=0, dtype=np.float64):
+        if host is not None:
+            host = np.ascontiguousarray(host)
+            self.dtype, self.size = host.dtype, host.size
+        else:
+            self.dtype, self.size = np.dtype(dtype), int(size)
+        self.nbytes = self.size * self.dtype.itemsize
+        p = C.c_void_p()
+        hip_check(hip().hipMalloc(C.byref(p), max(self.nbytes, 8)), "hipMalloc")
+        self.ptr = p.value
+        if host is not None and self.nbytes:
+            hip_check(hip().hipMemcpy(self.ptr, host.ctypes.data, self.nbytes, H2D), "hipMemcpy H2D")
+
+    def numpy(self):
+        out = np.empty(self.size, dtype=self.dtype)
+        if self.nbytes:
+            hip_check(hip().hipMemcpy(out.ctypes.data, self.ptr, self.nbytes, D2H), "hipMemcpy D2H")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _hip is not None:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+
+def download(ptr, n, dtype):
+    """Copies n elements from a device pointer to a new numpy array."""
+    out = np.empty(n, dtype=dtype)
+    if n > 0:
+        hip_check(hip().hipMemcpy(out.ctypes.data, ptr, out.nbytes, D2H), "hipMemcpy D2H")
+    return out
+
+
+def _take_csr(g, free=True):
+    ptr = download(g.row_ptr, g.rows + 1, np.int32)
+    idx = download(g.col_idx, g.nnz, np.int32)
+    val = download(g.val, g.nnz, np.float64)
+    if free:
+        for p in (g.row_ptr, g.col_idx, g.val):
+            if p:
+                _check(lib().gp_device_free(p))
+    return ptr, idx, val
+
+
+class DeviceCsr:
+    def __init__(self, rows, cols, ptr, idx, val):
+        self.keep = [DeviceArray(np.asarray(ptr, np.int32)), DeviceArray(np.asarray(idx, np.int32)),
+                     DeviceArray(np.asarray(val, np.float64))]
+        self.g = T.GpCsr(rows, cols, len(idx), self.keep[0].ptr, self.keep[1].ptr, self.keep[2].ptr)
+
+
+class DeviceLp:
+    """An LP dict (see lp_synth) uploaded to HBM behind a gp_lp of device pointers."""
+
+    def __init__(self, lp):
+        m, n = lp["m"], lp["n"]
+        self.m, self.n = m, n
+        self.A = DeviceCsr(m, n, lp["A_ptr"], lp["A_idx"], lp["A_val"])
+        self.AT = DeviceCsr(n, m, lp["AT_ptr"], lp["AT_idx"], lp["AT_val"])
+        self.vec = {k: DeviceArray(np.asarray(lp[k], np.float64)) for k in ("c", "AL", "AU", "l", "u")}
+        g = T.GpLp()
+        g.A, g.AT = self.A.g, self.AT.g
+        for k in ("c", "AL", "AU", "l", "u"):
+            setattr(g, k, self.vec[k].ptr)
+        g.obj_constant = float(lp.get("obj", 0.0))
+        self.g = g
+
+    def at_host(self):
+        """The (possibly row-sorted in place) AT back on the host."""
+        return (self.AT.keep[0].numpy(), self.AT.keep[1].numpy(), self.AT.keep[2].numpy())
+
+
+# ---- primitives ------------------------------------------------------------
+def row_activity(m, n, ptr, idx, val, l, u, ztol, stream=None):
+    A = DeviceCsr(m, n, ptr, idx, val)
+    dl, du = DeviceArray(np.asarray(l, np.float64)), DeviceArray(np.asarray(u, np.float64))
+    mn, mx = DeviceArray(size=m), DeviceArray(size=m)
+    ni, pi = DeviceArray(size=m, dtype=np.int32), DeviceArray(size=m, dtype=np.int32)
+    _check(lib().gp_row_activity(C.byref(A.g), dl.ptr, du.ptr, ztol, mn.ptr, mx.ptr, ni.ptr, pi.ptr, stream))
+    if stream is not None:
+        hip_check(hip().hipStreamSynchronize(stream), "hipStreamSynchronize")
+    return mn.numpy(), mx.numpy(), ni.numpy(), pi.numpy()
+
+
+def col_max_abs(n, m, ptr, idx, val):
+    AT = DeviceCsr(n, m, ptr, idx, val)
+    out = DeviceArray(size=n)
+    _check(lib().gp_col_max_abs(C.byref(AT.g), out.ptr, None))
+    return out.numpy()
+
+
+def compact_csr(m, n, ptr, idx, val, keep_row, keep_col):
+    A = DeviceCsr(m, n, ptr, idx, val)
+    kr, kc = DeviceArray(np.asarray(keep_row, np.uint8)), DeviceArray(np.asarray(keep_col, np.uint8))
+    out = T.GpCsr()
+    _check(lib().gp_compact_csr(C.byref(A.g), kr.ptr, kc.ptr, C.byref(out)))
+    return (out.rows, out.cols) + _take_csr(out)
+
+
+def transpose_csr(m, n, ptr, idx, val):
+    A = DeviceCsr(m, n, ptr, idx, val)
+    out = T.GpCsr()
+    _check(lib().gp_transpose_csr(C.byref(A.g), C.byref(out)))
+    return (out.rows, out.cols) + _take_csr(out)
+
+
+# ---- rule operator API -----------------------------------------------------
+_PLAN_VEC = ("keep_row", "keep_col", "c", "l", "u", "AL", "AU")
+
+
+def rule_apply(rule, plan, lp, stats, prm, candidate=None):
+    """Runs one rule on the GPU.  plan: dict with keep_row, keep_col, c, l, u,
+    AL, AU (numpy, updated in place from the device) and flags; stats: dict of
+    numpy arrays.  Returns (plan_flags dict, tape dict, new_A or None)."""
+    dl = DeviceLp(lp)
+    dev = {}
+    pv = T.GpPlanView()
+    for k in _PLAN_VEC:
+        a = plan.get(k)
+        if a is not None:
+            dev[k] = DeviceArray(a)
+            setattr(pv, k, dev[k].ptr)
+    pv.obj_constant_delta = plan.get("obj_constant_delta", 0.0)
+    for k in ("has_change", "has_row_action", "has_col_action", "has_infeasible", "has_unbounded"):
+        setattr(pv, k, int(plan.get(k, 0)))
+    sv = T.GpStatsView()
+    sdev = {}
+    for k, _ in T.GpStatsView._fields_:
+        a = stats.get(k) if stats else None
+        if a is not None:
+            sdev[k] = DeviceArray(a)
+            setattr(sv, k, sdev[k].ptr)
+    cand = DeviceArray(np.asarray(candidate, np.uint8)) if candidate is not None else None
+    tape = C.c_void_p()
+    _check(lib().gp_rule_apply(T.GP_RULE[rule] if isinstance(rule, str) else rule, C.byref(pv), C.byref(dl.g),
+                               C.byref(sv), C.byref(prm), cand.ptr if cand is not None else None, C.byref(tape)))
+    for k, d in dev.items():
+        plan[k][...] = d.numpy()
+    flags = {k: getattr(pv, k) for k in ("obj_constant_delta", "has_change", "has_row_action",
+                                          "has_col_action", "has_infeasible", "has_unbounded", "has_new_A")}
+    rv = T.GpRecordView()
+    _check(lib().gp_tape_view(tape, C.byref(rv)))
+    t = T.record_to_dict(rv)
+    lib().gp_tape_free(tape)
+    new_A = _take_csr(pv.new_A) if pv.has_new_A else None
+    # The rules sort the caller's AT rows in place (doubleton batch,
+    # duplicate columns); report it so a test can compare with the oracle.
+    t["_AT_after"] = dl.at_host()
+    return flags, t, new_A
+
+
+# ---- presolve / postsolve --------------------------------------------------
+def lp_from_struct(g):
+    m, n, nnz = g.A.rows, g.A.cols, g.A.nnz
+    return {"m": m, "n": n,
+            "A_ptr": download(g.A.row_ptr, m + 1, np.int32), "A_idx": download(g.A.col_idx, nnz, np.int32),
+            "A_val": download(g.A.val, nnz, np.float64),
+            "AT_ptr": download(g.AT.row_ptr, n + 1, np.int32), "AT_idx": download(g.AT.col_idx, nnz, np.int32),
+            "AT_val": download(g.AT.val, nnz, np.float64),
+            "c": download(g.c, n, np.float64), "l": download(g.l, n, np.float64),
+            "u": download(g.u, n, np.float64), "AL": download(g.AL, m, np.float64),
+            "AU": download(g.AU, m, np.float64), "obj": g.obj_constant}
+
+
+class Presolved:
+    """A presolve handle (device-resident reduced LP and host record)."""
+
+    def __init__(self, dlp, prm, keep_reduced=True):
+        self.dlp = dlp
+        self.h = C.c_void_p()
+        _check(lib().gp_presolve(C.byref(dlp.g), C.byref(prm), int(keep_reduced), C.byref(self.h)))
+        self.keep_reduced = keep_reduced
+
+    def summary(self):
+        s = T.GpSummary()
+        _check(lib().gp_presolve_summary(self.h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in T.GpSummary._fields_}
+
+    def record(self):
+        rv = T.GpRecordView()
+        _check(lib().gp_presolve_record(self.h, C.byref(rv)))
+        return T.record_to_dict(rv)
+
+    def reduced(self):
+        g = T.GpLp()
+        _check(lib().gp_presolve_reduced_lp(self.h, C.byref(g)))
+        return lp_from_struct(g)
+
+    def free(self):
+        if self.h:
+            lib().gp_presolve_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        if _lib is not None:
+            self.free()
+
+
+def presolve(lp, prm, keep_reduced=True):
+    """Returns (summary dict, record dict, reduced LP dict or None)."""
+    dlp = DeviceLp(lp)
+    h = Presolved(dlp, prm, keep_reduced)
+    try:
+        return h.summary(), h.record(), (h.reduced() if keep_reduced else None)
+    finally:
+        h.free()
+
+
+def postsolve(rec, x_red, y_red, z_red, original=None):
+    rk = T.RecordKeepAlive(rec)
+    n0, m0 = rec["n0"], rec["m0"]
+    dx, dy, dz = DeviceArray(size=n0), DeviceArray(size=m0), DeviceArray(size=n0)
+    ins = [DeviceArray(np.asarray(v, np.float64)) if v is not None else None for v in (x_red, y_red, z_red)]
+    orig = DeviceLp(original) if original is not None else None
+    _check(lib().gp_postsolve(C.byref(rk.view), *[d.ptr if d is not None else None for d in ins],
+                              C.byref(orig.g) if orig is not None else None, dx.ptr, dy.ptr, dz.ptr))
+    return dx.numpy(), dy.numpy(), dz.numpy()
+
+
+# ---- timing ----------------------------------------------------------------
+class Stream:
+    """A HIP stream at the default priority."""
+
+    def __init__(self):
+        s = C.c_void_p()
+        hip_check(hip().hipStreamCreate(C.byref(s)), "hipStreamCreate")
+        self.ptr = s.value
+
+    def sync(self):
+        hip_check(hip().hipStreamSynchronize(self.ptr), "hipStreamSynchronize")
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _hip is not None:
+            hip().hipStreamDestroy(self.ptr)
+            self.ptr = None
+
+
+class Event:
+    def __init__(self):
+        e = C.c_void_p()
+        hip_check(hip().hipEventCreate(C.byref(e)), "hipEventCreate")
+        self.ptr = e.value
+
+    def record(self, stream):
+        hip_check(hip().hipEventRecord(self.ptr, stream.ptr if isinstance(stream, Stream) else stream),
+                  "hipEventRecord")
+
+    def elapsed_ms(self, later):
+        hip_check(hip().hipEventSynchronize(later.ptr), "hipEventSynchronize")
+        ms = C.c_float()
+        hip_check(hip().hipEventElapsedTime(C.byref(ms), self.ptr, later.ptr), "hipEventElapsedTime")
+        return ms.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and _hip is not None:
+            hip().hipEventDestroy(self.ptr)
+            self.ptr = None

@@ -300,6 +300,9 @@ hipError_t gpk_dbl_merge(gpk_csr A, const int32_t* delta_ptr, const int32_t* dco
 hipError_t gpk_dbl_scalar_rows(gpk_csr A, int32_t target, int32_t ec, int32_t kc, double alpha, double zero_tol,
                                const int32_t* out_ptr, int32_t* counts, int32_t* out_idx, double* out_val,
                                hipStream_t stream);
+/* ec_len: entries of column ec in AT (the launch covers exactly those). */
+hipError_t gpk_dbl_scalar_shift(gpk_csr AT, int32_t ec, int32_t ec_len, double beta, double* AL, double* AU,
+                                hipStream_t stream);
 
 /* rule_redundant_bounds.cu:31-262 */
 hipError_t gpk_rb_candidates(gpk_csr A, gpk_csr AT, gpk_plan plan, const double* c_lp, int32_t one_sided,

@@ -202,12 +202,17 @@ void gp_params_default(gp_params* params);
 gp_status gp_device_free(void* device_ptr);
 
 /* ---- primitives (reference gpu_presolve_kernels.hpp:7-16) -------------- */
-/* Replaces compute_row_activity_summary (reference gpu_presolve_kernels.hpp:7-14). */
+/* Replaces compute_row_activity_summary (reference gpu_presolve_kernels.hpp:7-14).
+ * `stream` (a hipStream_t) may be NULL: the call then runs on the library's
+ * stream and returns when the results are written.  With a stream the kernel
+ * is only enqueued on it (no synchronization), for callers that order their
+ * own work, time it with events on that stream, or capture it in a graph. */
 gp_status gp_row_activity(const gp_csr* A, const double* l, const double* u, double zero_tol,
                           double* row_min_fin, double* row_max_fin,
-                          int32_t* row_min_neg_inf_count, int32_t* row_max_pos_inf_count);
-/* Replaces compute_col_max_abs (reference gpu_presolve_kernels.hpp:16). */
-gp_status gp_col_max_abs(const gp_csr* AT, double* col_max_abs);
+                          int32_t* row_min_neg_inf_count, int32_t* row_max_pos_inf_count,
+                          void* stream);
+/* Replaces compute_col_max_abs (reference gpu_presolve_kernels.hpp:16); `stream` as above. */
+gp_status gp_col_max_abs(const gp_csr* AT, double* col_max_abs, void* stream);
 /* Mask compaction of a CSR matrix and its re-transpose, the apply step of the
  * reference (gpu_presolve.cu:1881-1978 and :1804-1879).  keep masks are
  * device uint8; outputs are allocated by the library (gp_device_free). */
