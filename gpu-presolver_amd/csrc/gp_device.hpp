@@ -22,6 +22,28 @@ __device__ inline uint64_t lanemask_lt(int lane) {
   return lane == 0 ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Flag and counter updates aggregated per wave: the calling lanes (those for
+// which the event happened) elect their lowest lane, which issues a single
+// atomic for the whole wave.  Safe in divergent code: only the lanes that
+// reach the call take part.
+__device__ inline bool wave_leader(uint64_t* active) {
+  const uint64_t m = __ballot(1);
+  *active = m;
+  return lane_id() == __ffsll(static_cast<unsigned long long>(m)) - 1;
+}
+__device__ inline void wave_or(int32_t* addr, int32_t v) {
+  uint64_t m;
+  if (wave_leader(&m)) atomicOr(addr, v);
+}
+__device__ inline void wave_max(int32_t* addr, int32_t v) {
+  uint64_t m;
+  if (wave_leader(&m)) atomicMax(addr, v);
+}
+__device__ inline void wave_add1(int32_t* addr) {
+  uint64_t m;
+  if (wave_leader(&m)) atomicAdd(addr, static_cast<int32_t>(__popcll(m)));
+}
+
 // Inclusive wave64 prefix sum of an int32.
 __device__ inline int32_t wave_inclusive_sum(int32_t v) {
   const int lane = lane_id();

@@ -496,6 +496,10 @@ gp_status gp_presolve_reduced_lp(const gp_presolve_handle* h, gp_lp* out) {
   });
 }
 
+int32_t gp_has_good_nnz_progress(int32_t nnz_before, int32_t nnz_after, double ratio) {
+  return _has_good_nnz_progress(nnz_before, nnz_after, ratio) ? 1 : 0;
+}
+
 void gp_presolve_free(gp_presolve_handle* h) {
   if (h == nullptr) return;
   free_gpu_presolve_reduced_lp(h->summary);

@@ -30,7 +30,7 @@ __global__ void k_fixed_vars_candidates(int32_t n, gpk_plan p, double tol, uint8
     fixed[j] = 1;
     v[j] = vj;
     contrib[j] = p.c[j] * vj;
-    atomicAdd(count, 1);
+    wave_add1(count);
   }
 }
 
@@ -45,11 +45,11 @@ __global__ void k_empty_cols(int32_t n, gpk_plan p, const uint8_t* empty, double
   if (!(p.keep_col[j] && empty[j])) return;
   const double cj = p.c[j], lj = p.l[j], uj = p.u[j];
   if (lj > uj + btol) {
-    atomicOr(&flags[0], 1);
+    wave_or(&flags[0], 1);
     return;
   }
   if ((cj > ztol && !isfinite(lj)) || (cj < -ztol && !isfinite(uj))) {
-    atomicOr(&flags[1], 1);
+    wave_or(&flags[1], 1);
     return;
   }
   double val = 0.0;
@@ -60,7 +60,7 @@ __global__ void k_empty_cols(int32_t n, gpk_plan p, const uint8_t* empty, double
   fixed[j] = 1;
   v[j] = val;
   contrib[j] = cj * val;
-  atomicOr(&flags[2], 1);
+  wave_or(&flags[2], 1);
 }
 
 __global__ void k_fix_cols_apply(int32_t n, gpk_plan p, const uint8_t* fixed, const double* v) {
@@ -153,9 +153,9 @@ __global__ void k_cs_precheck(int32_t n, int32_t m, gpk_plan p, const uint8_t* m
   const int32_t row = srow[j];
   if (row < 0 || row >= m || !p.keep_row[row]) return;
   needed[row] = 1;
-  atomicAdd(&counts[0], 1);
+  wave_add1(&counts[0]);
   const double lo = p.AL[row], hi = p.AU[row];
-  if (isfinite(lo) && isfinite(hi) && fabs(hi - lo) <= tol) atomicAdd(&counts[1], 1);
+  if (isfinite(lo) && isfinite(hi) && fabs(hi - lo) <= tol) wave_add1(&counts[1]);
 }
 
 // rule_column_singletons.cu:455-535 (sequential along the row).
@@ -235,7 +235,7 @@ __global__ void k_cs_owner(gpk_csr A, gpk_plan p, const uint8_t* mask, const int
   if (live <= 1) return;
   const double lhs = p.AL[row], rhs = p.AU[row], cj = p.c[j];
   if (!eq_mode && cs_direct_unbounded(cj, a, lhs, rhs, p.l[j], p.u[j], zt)) {
-    atomicOr(&flags[0], 1);
+    wave_or(&flags[0], 1);
     return;
   }
   bool is_eq, above, below;
@@ -258,7 +258,7 @@ __global__ void k_cs_process(int32_t n, int32_t m, gpk_plan p, const uint8_t* ma
   if (fabs(a) <= zt) return;
   const double lhs = AL_new[row], rhs = AU_new[row], cj = p.c[j];
   if (!eq_mode && cs_direct_unbounded(cj, a, lhs, rhs, p.l[j], p.u[j], zt)) {
-    atomicOr(&flags[0], 1);
+    wave_or(&flags[0], 1);
     return;
   }
   bool is_eq, above, below;
@@ -282,7 +282,7 @@ __global__ void k_cs_process(int32_t n, int32_t m, gpk_plan p, const uint8_t* ma
       }
     }
     col_del[j] = 1;
-    atomicOr(&flags[1], 1);
+    wave_or(&flags[1], 1);
     return;
   }
   double side = 0.0;
@@ -299,7 +299,7 @@ __global__ void k_cs_process(int32_t n, int32_t m, gpk_plan p, const uint8_t* ma
     row_changed[row] = 1;
     AU_new[row] = lhs;
   }
-  atomicOr(&flags[2], 1);
+  wave_or(&flags[2], 1);
 }
 
 // rule_column_singletons.cu:1063-1091 (AT order, no atomics).
@@ -396,7 +396,7 @@ __global__ void k_dual_fix_candidates(gpk_csr AT, gpk_plan p, const uint8_t* can
   if (!p.keep_col[j] || (cand != nullptr && !cand[j])) return;
   const double lj = p.l[j], uj = p.u[j];
   if (lj > uj + btol) {
-    atomicMax(&flags[0], 1);
+    wave_max(&flags[0], 1);
     return;
   }
   const int32_t b = AT.ptr[j], e = AT.ptr[j + 1];
@@ -421,12 +421,12 @@ __global__ void k_dual_fix_candidates(gpk_csr AT, gpk_plan p, const uint8_t* can
   if (cj > zt) {
     if (!down) {
       if (isfinite(lj)) { vj = lj; fx = true; }
-      else { atomicMax(&flags[0], 2); return; }
+      else { wave_max(&flags[0], 2); return; }
     }
   } else if (cj < -zt) {
     if (!up) {
       if (isfinite(uj)) { vj = uj; fx = true; }
-      else { atomicMax(&flags[0], 2); return; }
+      else { wave_max(&flags[0], 2); return; }
     }
   } else if (fabs(cj) <= zt) {
     if (!down) {
@@ -443,8 +443,8 @@ __global__ void k_dual_fix_candidates(gpk_csr AT, gpk_plan p, const uint8_t* can
     finite_fix[j] = infx ? 0 : 1;
     v[j] = vj;
     contrib[j] = infx ? 0.0 : cj * vj;
-    atomicOr(&flags[1], 1);
-    if (infx) atomicOr(&flags[2], 1);
+    wave_or(&flags[1], 1);
+    if (infx) wave_or(&flags[2], 1);
   }
 }
 
@@ -465,7 +465,7 @@ __global__ void k_dual_fix_rows(gpk_csr A, gpk_plan p, const uint8_t* finite_fix
   }
   shift[i] = s;
   remove[i] = rm;
-  if (rm) atomicOr(&flags[3], 1);
+  if (rm) wave_or(&flags[3], 1);
 }
 
 __global__ void k_dual_fix_apply(int32_t m, int32_t n, gpk_plan p, const uint8_t* fixed, const double* v,
@@ -585,7 +585,7 @@ __global__ void k_dup_col_guard(gpk_csr AT, gpk_plan p, const uint64_t* sh, cons
       if (!dc_ratio(AT, p, j, k, zt, ctol, &r)) continue;
       if (fabs(p.c[k] - r * p.c[j]) > otol) continue;
       if (ratio_violates(max_ratio, r)) {
-        atomicAdd(count, 1);
+        wave_add1(count);
         return;
       }
     }
@@ -637,7 +637,7 @@ __global__ void k_dup_col_groups(gpk_csr AT, gpk_plan p, const uint64_t* sh, con
         p.l[j] = nl;
         p.u[j] = nu;
         col_del[k] = 1;
-        atomicOr(&flags[1], 1);
+        wave_or(&flags[1], 1);
         continue;
       }
       bool k_lo = false, k_hi = false, j_lo = false, j_hi = false;
@@ -649,18 +649,18 @@ __global__ void k_dup_col_groups(gpk_csr AT, gpk_plan p, const uint64_t* sh, con
         else { k_hi = !isfinite(tu); j_hi = !isfinite(su); }
       }
       if (k_lo) {
-        if (!isfinite(sl)) { atomicOr(&flags[0], 1); return; }
-        fixed[k] = 1; fixed_val[k] = sl; atomicOr(&flags[2], 1); continue;
+        if (!isfinite(sl)) { wave_or(&flags[0], 1); return; }
+        fixed[k] = 1; fixed_val[k] = sl; wave_or(&flags[2], 1); continue;
       } else if (k_hi) {
-        if (!isfinite(su)) { atomicOr(&flags[0], 1); return; }
-        fixed[k] = 1; fixed_val[k] = su; atomicOr(&flags[2], 1); continue;
+        if (!isfinite(su)) { wave_or(&flags[0], 1); return; }
+        fixed[k] = 1; fixed_val[k] = su; wave_or(&flags[2], 1); continue;
       }
       if (j_lo) {
-        if (!isfinite(tl)) { atomicOr(&flags[0], 1); return; }
-        fixed[j] = 1; fixed_val[j] = tl; atomicOr(&flags[2], 1); break;
+        if (!isfinite(tl)) { wave_or(&flags[0], 1); return; }
+        fixed[j] = 1; fixed_val[j] = tl; wave_or(&flags[2], 1); break;
       } else if (j_hi) {
-        if (!isfinite(tu)) { atomicOr(&flags[0], 1); return; }
-        fixed[j] = 1; fixed_val[j] = tu; atomicOr(&flags[2], 1); break;
+        if (!isfinite(tu)) { wave_or(&flags[0], 1); return; }
+        fixed[j] = 1; fixed_val[j] = tu; wave_or(&flags[2], 1); break;
       }
     }
   }
@@ -800,7 +800,7 @@ __global__ void k_dbl_candidates(gpk_csr A, gpk_csr AT, gpk_plan p, gpk_dbl d, d
   if (fill > max_fill) return;
   d.acceptable[i] = 1;
   d.active[i] = 1;
-  atomicAdd(&counts[0], 1);
+  wave_add1(&counts[0]);
 }
 
 __global__ void k_dbl_reset_owner(int32_t n, gpk_dbl d) {
@@ -828,7 +828,7 @@ __global__ void k_dbl_select(int32_t m, gpk_dbl d, int32_t* counts) {
     d.active[i] = 0;
     d.blocked[k] = 1;
     d.blocked[e] = 1;
-    atomicAdd(&counts[1], 1);
+    wave_add1(&counts[1]);
   }
 }
 
@@ -861,7 +861,7 @@ __global__ void k_dbl_apply(int32_t k, const int32_t* sel, gpk_plan p, gpk_dbl d
   const double nkl = isfinite(ml) ? fmax(ol_k, ml) : ol_k;
   const double nku = isfinite(mu) ? fmin(ou_k, mu) : ou_k;
   if (nkl > nku + tol) {
-    atomicOr(&flags[0], 1);
+    wave_or(&flags[0], 1);
     return;
   }
   const bool keep_fixed = isfinite(nkl) && isfinite(nku) && nku <= nkl + tol;
@@ -874,7 +874,7 @@ __global__ void k_dbl_apply(int32_t k, const int32_t* sel, gpk_plan p, gpk_dbl d
     ofix[t] = oc_k * fat;
     cols.is_fixed_keep[kc] = 1;
     cols.fixed_val_of[kc] = fat;
-    atomicAdd(&flags[3], 1);
+    wave_add1(&flags[3]);
   } else {
     p.l[kc] = nkl;
     p.u[kc] = nku;
@@ -890,7 +890,7 @@ __global__ void k_dbl_apply(int32_t k, const int32_t* sel, gpk_plan p, gpk_dbl d
     cols.beta_of[ec] = be;
     cols.row_of[ec] = row;
     cols.keep_of[ec] = kc;
-    atomicAdd(&flags[2], 1);
+    wave_add1(&flags[2]);
   }
 }
 
@@ -1122,7 +1122,7 @@ __global__ void k_rb_select(gpk_csr AT, const uint8_t* dlo, const uint8_t* dhi, 
     if (rkey[AT.idx[q]] != key) return;
   sl[j] = dlo[j];
   su[j] = dhi[j];
-  atomicOr(&flags[0], 1);
+  wave_or(&flags[0], 1);
 }
 
 __global__ void k_rb_apply(int32_t n, gpk_plan p, const uint8_t* sl, const uint8_t* su) {

@@ -89,6 +89,8 @@ def lib():
         L.gp_presolve_reduced_lp.argtypes = [vp, C.POINTER(T.GpLp)]
         L.gp_presolve_free.argtypes = [vp]
         L.gp_postsolve.argtypes = [C.POINTER(T.GpRecordView), vp, vp, vp, C.POINTER(T.GpLp), vp, vp, vp]
+        L.gp_has_good_nnz_progress.argtypes = [C.c_int32, C.c_int32, C.c_double]
+        L.gp_has_good_nnz_progress.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -102,6 +104,10 @@ def params(**kw):
     p = T.GpParams()
     lib().gp_params_default(C.byref(p))
     return p.set(**kw)
+
+
+def has_good_nnz_progress(before, after, ratio):
+    return bool(lib().gp_has_good_nnz_progress(before, after, ratio))
 
 
 # ---- device memory ---------------------------------------------------------
@@ -167,11 +173,13 @@ class DeviceLp:
         self.m, self.n = m, n
         self.A = DeviceCsr(m, n, lp["A_ptr"], lp["A_idx"], lp["A_val"])
         self.AT = DeviceCsr(n, m, lp["AT_ptr"], lp["AT_idx"], lp["AT_val"])
-        self.vec = {k: DeviceArray(np.asarray(lp[k], np.float64)) for k in ("c", "AL", "AU", "l", "u")}
+        # Absent vectors stay null pointers, as a caller may leave them.
+        self.vec = {k: DeviceArray(np.asarray(lp[k], np.float64)) for k in ("c", "AL", "AU", "l", "u")
+                    if lp.get(k) is not None}
         g = T.GpLp()
         g.A, g.AT = self.A.g, self.AT.g
         for k in ("c", "AL", "AU", "l", "u"):
-            setattr(g, k, self.vec[k].ptr)
+            setattr(g, k, self.vec[k].ptr if k in self.vec else None)
         g.obj_constant = float(lp.get("obj", 0.0))
         self.g = g
 

@@ -244,6 +244,10 @@ gp_status gp_presolve_reduced_lp(const gp_presolve_handle* h, gp_lp* out);
 /* free_gpu_presolve_reduced_lp + record release (reference gpu_presolve.hpp:40). */
 void gp_presolve_free(gp_presolve_handle* h);
 
+/* Replaces _has_good_nnz_progress (reference gpu_presolve.hpp:32): 1 when
+ * nnz_after < ratio * nnz_before (and nnz_before > 0), else 0.  Host only. */
+int32_t gp_has_good_nnz_progress(int32_t nnz_before, int32_t nnz_after, double ratio);
+
 /* ---- postsolve (reference gpu_postsolve.hpp:16-27) --------------------- */
 /* postsolve_gpu with caller-provided outputs: x_org, z_org hold n0 doubles,
  * y_org m0 doubles (device).  x_red/y_red/z_red may be null.  `original`

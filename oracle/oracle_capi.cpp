@@ -67,6 +67,11 @@ void orc_params_default(gp_params* p) { orc::default_params(p); }
 
 void orc_free(void* p) { std::free(p); }
 
+// gpu_presolve.cu:2151-2158
+int32_t orc_has_good_nnz_progress(int32_t before, int32_t after, double ratio) {
+  return orc::has_good_nnz_progress(before, after, ratio) ? 1 : 0;
+}
+
 int orc_row_activity(const gp_csr* A, const double* l, const double* u, double ztol, double* mn, double* mx,
                      int32_t* ni, int32_t* pi) {
   orc::row_activity(csr_of(*A), l, u, ztol, mn, mx, ni, pi);

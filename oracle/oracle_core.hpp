@@ -170,6 +170,7 @@ struct Result {
 };
 
 Result presolve(const LpV& lp, const Params& params, bool keep_reduced);
+bool has_good_nnz_progress(int32_t before, int32_t after, double ratio);
 
 struct RecordV {
   int32_t m0, n0, m1, n1;

@@ -43,6 +43,8 @@ def lib():
         L.orc_result_free.argtypes = [vp]
         L.orc_postsolve.argtypes = [C.POINTER(T.GpRecordView), vp, vp, vp, C.POINTER(T.GpLp), vp, vp, vp]
         L.orc_free.argtypes = [vp]
+        L.orc_has_good_nnz_progress.argtypes = [C.c_int32, C.c_int32, C.c_double]
+        L.orc_has_good_nnz_progress.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -58,6 +60,10 @@ def params(**kw):
     return p.set(**kw)
 
 
+def has_good_nnz_progress(before, after, ratio):
+    return bool(lib().orc_has_good_nnz_progress(before, after, ratio))
+
+
 def _ptr(a):
     return a.ctypes.data if a is not None and a.size else None
 
@@ -68,6 +74,8 @@ class HostLp:
     def __init__(self, lp):
         self.keep = {}
         def arr(k, dt):
+            if lp.get(k) is None:
+                return None
             a = np.ascontiguousarray(lp[k], dtype=dt)
             if a.size == 0:
                 a = np.zeros(1, dtype=dt)
