@@ -8,11 +8,14 @@ planted reductions of the workload description).  `value` is LPs presolved
 per second.
 
 Extra objects on the JSON line:
-  roofline      the row-activity kernel (gp_row_activity on a caller stream,
-                timed with HIP events around back-to-back launches) against
-                8 TB/s, with its algorithmic bytes 4(m+1) + 12 nnz + 16 n + 24 m;
-  cpu_baseline  the serial CPU oracle presolving the same LP once on the
-                host (single thread).
+  roofline           the compaction kernels (count + copy, the largest share of
+                     presolve kernel time) on the full A with 15% of rows and
+                     columns dropped, timed with HIP events around back-to-back
+                     launches on one stream, against 8 TB/s;
+  roofline_activity  the row-activity kernel (gp_row_activity on a caller
+                     stream), algorithmic bytes 4(m+1) + 12 nnz + 16 n + 24 m;
+  cpu_baseline       the serial CPU oracle presolving the same LP once on the
+                     host (single thread).
 """
 import argparse
 import json
@@ -34,6 +37,122 @@ def activity_bytes(m, n, nnz):
     return 4 * (m + 1) + 12 * nnz + 16 * n + 24 * m
 
 
+def kernel_rooflines(gp_ffi, dlp, lp, launches):
+    """Row activity (gp_row_activity on a caller stream) and compaction
+    kernels, each timed with HIP events around back-to-back launches."""
+    import ctypes as C
+
+    import numpy as np
+
+    m, n, nnz = lp["m"], lp["n"], len(lp["A_idx"])
+    stream = gp_ffi.Stream()
+    mn, mx = gp_ffi.DeviceArray(size=m), gp_ffi.DeviceArray(size=m)
+    ni, pi = gp_ffi.DeviceArray(size=m, dtype=np.int32), gp_ffi.DeviceArray(size=m, dtype=np.int32)
+    L = gp_ffi.lib()
+
+    def launch():
+        gp_ffi._check(L.gp_row_activity(C.byref(dlp.A.g), dlp.vec["l"].ptr, dlp.vec["u"].ptr, 1e-10, mn.ptr, mx.ptr,
+                                        ni.ptr, pi.ptr, stream.ptr))
+
+    for _ in range(5):
+        launch()
+    stream.sync()
+    e0, e1 = gp_ffi.Event(), gp_ffi.Event()
+    e0.record(stream)
+    for _ in range(launches):
+        launch()
+    e1.record(stream)
+    act_ms = e0.elapsed_ms(e1) / launches
+    act_bytes = activity_bytes(m, n, nnz)
+    act = {"us": 1e3 * act_ms, "bytes": act_bytes, "achieved": act_bytes / (act_ms * 1e-3) / 1e9}
+    log(f"row activity: {act['us']:.1f} us/launch, {act['achieved']:.0f} GB/s")
+    comp = measure_compaction(gp_ffi, dlp, lp, stream, launches)
+    log(f"compaction: count {comp['count_us']:.1f} us + copy {comp['copy_us']:.1f} us, "
+        f"{comp['achieved']:.0f} GB/s")
+    return act, comp
+
+
+CALIBRATION_BYTES = 256 << 20
+
+
+def calibration_copy(gp_ffi):
+    """One device-to-device copy of 256 MiB: the reference point that scales
+    the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    a = gp_ffi.DeviceArray(size=CALIBRATION_BYTES // 8)
+    b = gp_ffi.DeviceArray(size=CALIBRATION_BYTES // 8)
+    gp_ffi.hip_check(gp_ffi.hip().hipMemcpy(b.ptr, a.ptr, CALIBRATION_BYTES, gp_ffi.D2D), "calibration copy")
+    gp_ffi.hip_check(gp_ffi.hip().hipDeviceSynchronize(), "sync")
+
+
+def compaction_bytes(m, n, nnz, m_new, nnz_new):
+    """Compulsory traffic of one mask compaction: the count pass reads the row
+    pointers, the row map, the column indices and the column map; the copy
+    pass reads the row pointers, both maps, the new row offsets and the
+    entries, and writes the kept entries and the new row pointers."""
+    count = 4 * (m + 1) + 4 * m + 4 * nnz + 4 * n + 4 * m
+    copy = 4 * (m + 1) + 4 * m + 4 * n + 4 * m + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1)
+    return count + copy
+
+
+def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
+    """Times the two compaction kernels on the full A with 15% of rows and
+    columns dropped (a typical early apply), back to back on one stream."""
+    import ctypes as C
+
+    import numpy as np
+
+    m, n = lp["m"], lp["n"]
+    rng = np.random.default_rng(seed)
+    keep_r = rng.random(m) >= frac
+    keep_c = rng.random(n) >= frac
+    row_map = np.where(keep_r, np.cumsum(keep_r) - 1, -1).astype(np.int32)
+    col_map = np.where(keep_c, np.cumsum(keep_c) - 1, -1).astype(np.int32)
+    rows = np.repeat(np.arange(m), np.diff(lp["A_ptr"]))
+    kept = keep_r[rows] & keep_c[lp["A_idx"]]
+    counts = np.bincount(rows[kept], minlength=m).astype(np.int64)
+    by_old = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
+    m_new, nnz_new = int(keep_r.sum()), int(by_old[-1])
+    L = gp_ffi.lib()
+    vp = C.c_void_p
+    L.gpk_compact_count.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp]
+    L.gpk_compact_copy.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, C.c_int32, vp]
+    d_rm, d_cm, d_by = gp_ffi.DeviceArray(row_map), gp_ffi.DeviceArray(col_map), gp_ffi.DeviceArray(by_old)
+    d_cnt = gp_ffi.DeviceArray(size=m + 1, dtype=np.int32)
+    o_ptr = gp_ffi.DeviceArray(size=m_new + 1, dtype=np.int32)
+    o_idx = gp_ffi.DeviceArray(size=nnz_new, dtype=np.int32)
+    o_val = gp_ffi.DeviceArray(size=nnz_new)
+    A = dlp.A.g
+
+    def count():
+        gp_ffi.hip_check(L.gpk_compact_count(A.row_ptr, A.col_idx, m, d_rm.ptr, d_cm.ptr, d_cnt.ptr, stream.ptr),
+                         "compact count")
+
+    def copy():
+        gp_ffi.hip_check(L.gpk_compact_copy(A.row_ptr, A.col_idx, A.val, m, d_rm.ptr, d_cm.ptr, d_by.ptr, o_ptr.ptr,
+                                            o_idx.ptr, o_val.ptr, m_new, stream.ptr), "compact copy")
+
+    out = {}
+    for name, fn in (("count", count), ("copy", copy)):
+        for _ in range(3):
+            fn()
+        stream.sync()
+        e0, e1 = gp_ffi.Event(), gp_ffi.Event()
+        e0.record(stream)
+        for _ in range(launches):
+            fn()
+        e1.record(stream)
+        out[name + "_us"] = 1e3 * e0.elapsed_ms(e1) / launches
+    # the timed kernels produce the compacted matrix: check it once
+    got_cnt = d_cnt.numpy()[:m]
+    if not np.array_equal(got_cnt, counts.astype(np.int32)):
+        raise SystemExit("bench.py: compaction counts differ from the host")
+    nbytes = compaction_bytes(m, n, len(lp["A_idx"]), m_new, nnz_new)
+    out.update(bytes=nbytes, achieved=nbytes / ((out["count_us"] + out["copy_us"]) * 1e-6) / 1e9,
+               workload=f"A of the LP ({m}x{n}, {len(lp['A_idx'])} nnz) with 15% of rows and columns dropped "
+                        f"-> {m_new} rows, {nnz_new} nnz")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -42,6 +161,9 @@ def main():
     ap.add_argument("--activity-launches", type=int, default=50)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="only the row-activity and compaction kernel timings plus a 256 MiB device copy "
+                         "(for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this library runs one LP presolve per GPU and "
@@ -62,6 +184,12 @@ def main():
     dlp = gp_ffi.DeviceLp(lp)
     prm = gp_ffi.params(enable_structure_specific_rules=0)
     H = gp_ffi.hip()
+
+    if args.kernels_only:
+        act, comp = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
+        calibration_copy(gp_ffi)
+        print(json.dumps({"kernels_only": True, "row_activity": act, "compaction": comp}), flush=True)
+        return
 
     def one_presolve():
         h = gp_ffi.Presolved(dlp, prm, keep_reduced=True)
@@ -99,30 +227,7 @@ def main():
     last.free()
     log(f"postsolve (incl. host<->device copies of x/y/z and the record): {post_ms:.1f} ms")
 
-    # Row-activity kernel roofline: back-to-back launches on one stream.
-    stream = gp_ffi.Stream()
-    mn, mx = gp_ffi.DeviceArray(size=m), gp_ffi.DeviceArray(size=m)
-    ni, pi = gp_ffi.DeviceArray(size=m, dtype=np.int32), gp_ffi.DeviceArray(size=m, dtype=np.int32)
-    L = gp_ffi.lib()
-    import ctypes as C
-
-    def launch():
-        gp_ffi._check(L.gp_row_activity(C.byref(dlp.A.g), dlp.vec["l"].ptr, dlp.vec["u"].ptr, 1e-10, mn.ptr, mx.ptr,
-                                        ni.ptr, pi.ptr, stream.ptr))
-
-    for _ in range(5):
-        launch()
-    stream.sync()
-    e0, e1 = gp_ffi.Event(), gp_ffi.Event()
-    e0.record(stream)
-    for _ in range(args.activity_launches):
-        launch()
-    e1.record(stream)
-    act_ms = e0.elapsed_ms(e1) / args.activity_launches
-    act_bytes = activity_bytes(m, n, nnz)
-    achieved = act_bytes / (act_ms * 1e-3) / 1e9
-    log(f"row activity: {1e3 * act_ms:.1f} us/launch, {achieved:.0f} GB/s")
-
+    act, comp = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
     cpu = None
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -156,9 +261,14 @@ def main():
                    "row_len": "U[2,14]", "col_popularity": "Zipf(1.5) cap 5000", "seed": args.seed,
                    "reduced": [summary["reduced_rows"], summary["reduced_cols"], summary["reduced_nnz"]],
                    "iterations": summary["iterations"], "postsolve_ms": post_ms},
-        "roofline": {"bound": "hbm", "kernel": "row_activity", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "us_per_launch": 1e3 * act_ms, "algorithmic_bytes": act_bytes},
+        "roofline": {"bound": "hbm", "kernel": "compaction (k_compact_count + k_compact_copy)",
+                     "achieved": comp["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": comp["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                     "us_per_launch": comp["count_us"] + comp["copy_us"], "algorithmic_bytes": comp["bytes"],
+                     "workload": comp["workload"]},
+        "roofline_activity": {"bound": "hbm", "kernel": "k_row_activity", "achieved": act["achieved"],
+                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": act["achieved"] / HBM_PEAK_GBS,
+                              "traffic": None, "us_per_launch": act["us"], "algorithmic_bytes": act["bytes"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

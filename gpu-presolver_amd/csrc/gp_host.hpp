@@ -40,15 +40,17 @@ inline void check(hipError_t status, const char* context) {
 class Ctx {
  public:
   static constexpr int kFlags = 16;
+  static constexpr int kFlagSlots = 256;
   explicit Ctx(hipStream_t stream = nullptr, bool own_stream = true);
   ~Ctx();
   Ctx(const Ctx&) = delete;
   Ctx& operator=(const Ctx&) = delete;
 
   hipStream_t stream() const { return stream_; }
-  // Zeroes the device flag words and returns them.
+  // Returns kFlags zeroed device flag words.
   int32_t* flags();
-  // Copies the first `count` flag words to the host (synchronizes the stream).
+  // Copies the first `count` words of the flags last returned by flags() to
+  // the host (synchronizes the stream).
   const int32_t* read_flags(int count = kFlags);
   void sync(const char* context = "stream synchronize");
 
@@ -80,6 +82,8 @@ class Ctx {
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
   int32_t* d_flags_ = nullptr;
+  int32_t* cur_ = nullptr;
+  int slot_ = 0;
   int32_t* h_flags_ = nullptr;
   void* pinned_ = nullptr;
 };
@@ -96,11 +100,13 @@ class ScopedCtx {
   Ctx* prev_;
 };
 
-// Stream-ordered pool allocation.  Every block is registered so that
-// dev_free can tell pool blocks (hipFreeAsync) from memory the caller
-// allocated with hipMalloc (hipFree); a plan's new_A may be either.
+// Caching stream-ordered allocation (host_common.cpp).  dev_free returns the
+// library's own blocks to the cache of their stream and hipFree()s memory the
+// caller allocated with hipMalloc (a plan's new_A may be either).
 void* dev_alloc(size_t bytes, hipStream_t stream, const char* context);
 void dev_free(void* p, hipStream_t stream) noexcept;
+// Returns the cached free blocks of `stream` to the device pool.
+void release_cached_blocks(hipStream_t stream) noexcept;
 
 // Stream-ordered device buffer (hipMallocAsync / hipFreeAsync).
 template <class T>

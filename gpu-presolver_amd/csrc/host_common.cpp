@@ -6,7 +6,8 @@
 #include <cstring>
 #include <limits>
 #include <mutex>
-#include <unordered_set>
+#include <unordered_map>
+#include <vector>
 
 #include "gp_host.hpp"
 
@@ -46,12 +47,15 @@ Ctx::Ctx(hipStream_t stream, bool own_stream) {
     stream_ = stream;
     own_stream_ = false;
   }
-  check(hipMalloc(&d_flags_, sizeof(int32_t) * kFlags), "hipMalloc flags");
+  check(hipMalloc(&d_flags_, sizeof(int32_t) * kFlags * kFlagSlots), "hipMalloc flags");
+  check(hipMemset(d_flags_, 0, sizeof(int32_t) * kFlags * kFlagSlots), "hipMemset flags");
   check(hipHostMalloc(&h_flags_, sizeof(int32_t) * kFlags, hipHostMallocDefault), "hipHostMalloc flags");
   check(hipHostMalloc(&pinned_, 64, hipHostMallocDefault), "hipHostMalloc staging");
 }
 
 Ctx::~Ctx() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  release_cached_blocks(stream_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   (void)hipFree(d_flags_);
   (void)hipHostFree(h_flags_);
@@ -59,13 +63,20 @@ Ctx::~Ctx() {
   if (own_stream_) (void)hipStreamDestroy(stream_);
 }
 
+// Flag words come from a ring of zeroed slots; the whole ring is cleared
+// (stream-ordered, after every earlier use) once per kFlagSlots handouts.
 int32_t* Ctx::flags() {
-  check(hipMemsetAsync(d_flags_, 0, sizeof(int32_t) * kFlags, stream_), "hipMemset flags");
-  return d_flags_;
+  if (slot_ == kFlagSlots) {
+    check(hipMemsetAsync(d_flags_, 0, sizeof(int32_t) * kFlags * kFlagSlots, stream_), "hipMemset flags");
+    slot_ = 0;
+  }
+  cur_ = d_flags_ + static_cast<size_t>(slot_++) * kFlags;
+  return cur_;
 }
 
 const int32_t* Ctx::read_flags(int count) {
-  check(hipMemcpyAsync(h_flags_, d_flags_, sizeof(int32_t) * count, hipMemcpyDeviceToHost, stream_), "read flags");
+  check(hipMemcpyAsync(h_flags_, cur_ ? cur_ : d_flags_, sizeof(int32_t) * count, hipMemcpyDeviceToHost, stream_),
+        "read flags");
   sync("read flags");
   return h_flags_;
 }
@@ -82,35 +93,91 @@ Ctx& ctx() {
 ScopedCtx::ScopedCtx(Ctx& c) : prev_(t_current) { t_current = &c; }
 ScopedCtx::~ScopedCtx() { t_current = prev_; }
 
+// ---- caching device allocator ----------------------------------------------
+// Blocks are rounded up to a power of two (>= 512 B) and, when released, kept
+// on a free list of the stream they were allocated on.  Work on one stream
+// runs in order, so a block released after its last use can be handed to the
+// next allocation on that stream without any synchronization; the device is
+// only asked for memory (hipMallocAsync) on a miss.  This replaces one
+// hipMallocAsync/hipFreeAsync pair per temporary buffer of every rule call.
 namespace {
-std::mutex g_pool_mu;
-std::unordered_set<void*>& pool_blocks() {
-  static auto* s = new std::unordered_set<void*>();
-  return *s;
+struct BlockInfo {
+  hipStream_t stream;
+  size_t bytes;
+};
+struct Arena {
+  std::mutex mu;
+  std::unordered_map<void*, BlockInfo> live;
+  std::unordered_map<hipStream_t, std::unordered_map<size_t, std::vector<void*>>> free_lists;
+};
+Arena& arena() {
+  static auto* a = new Arena();  // never destroyed: blocks may be released at exit
+  return *a;
+}
+size_t size_class(size_t bytes) {
+  size_t c = 512;
+  while (c < bytes) c <<= 1;
+  return c;
 }
 }  // namespace
 
+void release_cached_blocks(hipStream_t stream) noexcept {
+  Arena& a = arena();
+  std::vector<void*> blocks;
+  {
+    std::lock_guard<std::mutex> lock(a.mu);
+    auto it = a.free_lists.find(stream);
+    if (it == a.free_lists.end()) return;
+    for (auto& kv : it->second) blocks.insert(blocks.end(), kv.second.begin(), kv.second.end());
+    a.free_lists.erase(it);
+  }
+  for (void* p : blocks) (void)hipFreeAsync(p, stream);
+}
+
 void* dev_alloc(size_t bytes, hipStream_t stream, const char* context) {
+  Arena& a = arena();
+  const size_t cls = size_class(bytes);
+  {
+    std::lock_guard<std::mutex> lock(a.mu);
+    auto s = a.free_lists.find(stream);
+    if (s != a.free_lists.end()) {
+      auto f = s->second.find(cls);
+      if (f != s->second.end() && !f->second.empty()) {
+        void* p = f->second.back();
+        f->second.pop_back();
+        a.live.emplace(p, BlockInfo{stream, cls});
+        return p;
+      }
+    }
+  }
   void* p = nullptr;
-  check(hipMallocAsync(&p, bytes, stream), context);
-  std::lock_guard<std::mutex> lock(g_pool_mu);
-  pool_blocks().insert(p);
+  hipError_t st = hipMallocAsync(&p, cls, stream);
+  if (st == hipErrorOutOfMemory) {
+    (void)hipGetLastError();
+    release_cached_blocks(stream);
+    st = hipMallocAsync(&p, cls, stream);
+  }
+  check(st, context);
+  std::lock_guard<std::mutex> lock(a.mu);
+  a.live.emplace(p, BlockInfo{stream, cls});
   return p;
 }
 
 void dev_free(void* p, hipStream_t stream) noexcept {
   if (!p) return;
-  bool pooled = false;
+  Arena& a = arena();
   {
-    std::lock_guard<std::mutex> lock(g_pool_mu);
-    pooled = pool_blocks().erase(p) != 0;
+    std::lock_guard<std::mutex> lock(a.mu);
+    auto it = a.live.find(p);
+    if (it != a.live.end()) {
+      a.free_lists[it->second.stream][it->second.bytes].push_back(p);
+      a.live.erase(it);
+      return;
+    }
   }
-  if (pooled) {
-    (void)hipFreeAsync(p, stream);
-  } else {
-    (void)hipStreamSynchronize(stream);
-    (void)hipFree(p);
-  }
+  // Memory the caller allocated with hipMalloc (e.g. a plan's new_A).
+  (void)hipStreamSynchronize(stream);
+  (void)hipFree(p);
 }
 
 int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out) {

@@ -100,14 +100,8 @@ struct PhasePlan {
     p.keep_row_mask = reinterpret_cast<uint8_t*>(b);
     b += align256(m);
     p.keep_col_mask = reinterpret_cast<uint8_t*>(b);
-    hipStream_t s = ctx().stream();
-    if (m > 0) check(gpk_fill_u8(p.keep_row_mask, 1, static_cast<int64_t>(m), s), "plan keep_row");
-    if (n > 0) check(gpk_fill_u8(p.keep_col_mask, 1, static_cast<int64_t>(n), s), "plan keep_col");
-    d2d(p.new_c, lp.c, static_cast<int64_t>(n), "plan c");
-    d2d(p.new_l, lp.l, static_cast<int64_t>(n), "plan l");
-    d2d(p.new_u, lp.u, static_cast<int64_t>(n), "plan u");
-    d2d(p.new_AL, lp.AL, static_cast<int64_t>(m), "plan AL");
-    d2d(p.new_AU, lp.AU, static_cast<int64_t>(m), "plan AU");
+    check(gpk_plan_init(lp.A.rows, lp.A.cols, amd::plan_view(p), lp.c, lp.l, lp.u, lp.AL, lp.AU, ctx().stream()),
+          "plan init");
   }
   ~PhasePlan() {
     if (p.has_new_A) amd::free_csr(p.new_A);

@@ -47,6 +47,22 @@ __global__ void k_pack_index(const uint8_t* mask, const int32_t* prefix, int32_t
 
 // ------------------------------------------------------- empty rows ----
 // flags[0]: live empty row infeasible at 0; flags[1]: removable empty row.
+__global__ void k_plan_init(int32_t m, int32_t n, gpk_plan p, const double* c, const double* l, const double* u,
+                            const double* AL, const double* AU) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p.keep_col[i] = 1;
+    p.c[i] = c[i];
+    p.l[i] = l[i];
+    p.u[i] = u[i];
+  }
+  if (i < m) {
+    p.keep_row[i] = 1;
+    p.AL[i] = AL[i];
+    p.AU[i] = AU[i];
+  }
+}
+
 __global__ void k_empty_rows(int32_t m, gpk_plan p, const int32_t* row_nnz, double tol, int32_t apply,
                              int32_t* flags) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -574,6 +590,10 @@ hipError_t gpk_pack_f64(const uint8_t* mask, const double* vals, const int32_t* 
 }
 hipError_t gpk_pack_index(const uint8_t* mask, const int32_t* prefix, int32_t n, int32_t* out, hipStream_t stream) {
   GPK_LAUNCH(k_pack_index, n, mask, prefix, n, out);
+}
+hipError_t gpk_plan_init(int32_t m, int32_t n, gpk_plan plan, const double* c, const double* l, const double* u,
+                         const double* AL, const double* AU, hipStream_t stream) {
+  GPK_LAUNCH(k_plan_init, m > n ? m : n, m, n, plan, c, l, u, AL, AU);
 }
 hipError_t gpk_empty_rows(int32_t m, gpk_plan plan, const int32_t* row_nnz, double tol, int32_t apply, int32_t* flags,
                           hipStream_t stream) {

@@ -124,6 +124,11 @@ typedef struct gpk_plan {
   double* AU;
 } gpk_plan;
 
+/* Fresh phase plan (gpu_presolve.cu:197-247): keep masks set to 1 and
+ * copies of the working LP's c, l, u (n) and AL, AU (m), in one launch. */
+hipError_t gpk_plan_init(int32_t m, int32_t n, gpk_plan plan, const double* c, const double* l, const double* u,
+                         const double* AL, const double* AU, hipStream_t stream);
+
 /* Generic row shift: for each row i (all rows, or only keep_row[i] != 0 when
  * keep_row is given) shift = sum over entries with col_flag[j] of a*col_val[j]
  * in CSR order, then AL[i] -= shift, AU[i] -= shift for every row.  `abs_min`

@@ -12,6 +12,18 @@
 //   * per-row shifts: row entries in CSR (ascending column) order,
 //   * objective deltas: ascending column (or selected-row) order.
 // On dyadic data every such sum is exact and any order gives the same bits.
+//
+// Pinned by: the known-answer cases of the reference's own unit tests
+// (tests/golden/reference_kats.json: activity, every rule, scheduler cases
+// and postsolve replays), a numpy closed form of row activity, scipy
+// transposes/submatrices for the transpose and compaction, and LP
+// equivalence checked with HiGHS (optimum preserved by presolve, postsolve
+// returning a feasible optimal point).  Parity unpinned beyond this
+// restatement (no reference test fixes them): the tiered schedule's record
+// on large random LPs, the doubleton batch matching with many overlapping
+// candidates, the duplicate-row representative with three or more parallel
+// rows, the implied-bound relaxation on non-integral values, and objective
+// bits on non-dyadic data.
 #pragma once
 
 #include <cstdint>
