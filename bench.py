@@ -8,7 +8,7 @@ planted reductions of the workload description).  `value` is LPs presolved
 per second.
 
 Extra objects on the JSON line:
-  roofline           the compaction kernels (count + copy, the largest share of
+  roofline           the single-pass compaction kernel (the largest share of
                      presolve kernel time) on the full A with 15% of rows and
                      columns dropped, timed with HIP events around back-to-back
                      launches on one stream, against 8 TB/s;
@@ -67,8 +67,7 @@ def kernel_rooflines(gp_ffi, dlp, lp, launches):
     act = {"us": 1e3 * act_ms, "bytes": act_bytes, "achieved": act_bytes / (act_ms * 1e-3) / 1e9}
     log(f"row activity: {act['us']:.1f} us/launch, {act['achieved']:.0f} GB/s")
     comp = measure_compaction(gp_ffi, dlp, lp, stream, launches)
-    log(f"compaction: count {comp['count_us']:.1f} us + copy {comp['copy_us']:.1f} us, "
-        f"{comp['achieved']:.0f} GB/s")
+    log(f"compaction: {comp['us']:.1f} us/launch, {comp['achieved']:.0f} GB/s")
     return act, comp
 
 
@@ -85,21 +84,22 @@ def calibration_copy(gp_ffi):
 
 
 def compaction_bytes(m, n, nnz, m_new, nnz_new):
-    """Compulsory traffic of one mask compaction: the count pass reads the row
-    pointers, the row map, the column indices and the column map; the copy
-    pass reads the row pointers, both maps, the new row offsets and the
-    entries, and writes the kept entries and the new row pointers."""
-    count = 4 * (m + 1) + 4 * m + 4 * nnz + 4 * n + 4 * m
-    copy = 4 * (m + 1) + 4 * m + 4 * n + 4 * m + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1)
-    return count + copy
+    """Compulsory traffic of one mask compaction: read the row pointers, the
+    row map, the column map and every entry once; write the kept entries and
+    the new row pointers."""
+    return 4 * (m + 1) + 4 * m + 4 * n + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1)
 
 
 def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
-    """Times the two compaction kernels on the full A with 15% of rows and
-    columns dropped (a typical early apply), back to back on one stream."""
+    """Times the single-pass compaction kernel (gpk_compact_lb, the one the
+    presolve uses) on the full A with 15% of rows and columns dropped (a
+    typical early apply), back to back on one stream, and checks its output."""
     import ctypes as C
 
     import numpy as np
+
+    class Lookback(C.Structure):
+        _fields_ = [("status", C.c_void_p), ("ticket", C.c_void_p), ("error", C.c_void_p), ("epoch", C.c_uint32)]
 
     m, n = lp["m"], lp["n"]
     rng = np.random.default_rng(seed)
@@ -109,48 +109,49 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
     col_map = np.where(keep_c, np.cumsum(keep_c) - 1, -1).astype(np.int32)
     rows = np.repeat(np.arange(m), np.diff(lp["A_ptr"]))
     kept = keep_r[rows] & keep_c[lp["A_idx"]]
-    counts = np.bincount(rows[kept], minlength=m).astype(np.int64)
-    by_old = np.concatenate([[0], np.cumsum(counts)]).astype(np.int32)
-    m_new, nnz_new = int(keep_r.sum()), int(by_old[-1])
+    m_new, nnz_new = int(keep_r.sum()), int(kept.sum())
     L = gp_ffi.lib()
     vp = C.c_void_p
-    L.gpk_compact_count.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp]
-    L.gpk_compact_copy.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp, C.c_int32, vp]
-    d_rm, d_cm, d_by = gp_ffi.DeviceArray(row_map), gp_ffi.DeviceArray(col_map), gp_ffi.DeviceArray(by_old)
-    d_cnt = gp_ffi.DeviceArray(size=m + 1, dtype=np.int32)
+    L.gpk_compact_tiles.restype = C.c_int64
+    L.gpk_compact_tiles.argtypes = [C.c_int32]
+    L.gpk_compact_lb.argtypes = [vp, vp, vp, C.c_int32, vp, vp, C.c_int32, vp, vp, vp, vp, Lookback, vp]
+    tiles = L.gpk_compact_tiles(m)
+    d_rm, d_cm = gp_ffi.DeviceArray(row_map), gp_ffi.DeviceArray(col_map)
+    status = gp_ffi.DeviceArray(np.zeros(tiles, np.uint64))
+    tickets = gp_ffi.DeviceArray(np.zeros(launches + 8, np.int32))
+    err = gp_ffi.DeviceArray(np.zeros(1, np.int32))
+    total = gp_ffi.DeviceArray(size=1, dtype=np.int32)
     o_ptr = gp_ffi.DeviceArray(size=m_new + 1, dtype=np.int32)
-    o_idx = gp_ffi.DeviceArray(size=nnz_new, dtype=np.int32)
-    o_val = gp_ffi.DeviceArray(size=nnz_new)
+    o_idx = gp_ffi.DeviceArray(size=len(lp["A_idx"]), dtype=np.int32)
+    o_val = gp_ffi.DeviceArray(size=len(lp["A_idx"]))
     A = dlp.A.g
+    launch_no = [0]
 
-    def count():
-        gp_ffi.hip_check(L.gpk_compact_count(A.row_ptr, A.col_idx, m, d_rm.ptr, d_cm.ptr, d_cnt.ptr, stream.ptr),
-                         "compact count")
+    def compact():
+        k = launch_no[0]
+        launch_no[0] += 1
+        lb = Lookback(status.ptr, tickets.ptr + 4 * k, err.ptr, k + 1)
+        gp_ffi.hip_check(L.gpk_compact_lb(A.row_ptr, A.col_idx, A.val, m, d_rm.ptr, d_cm.ptr, m_new, o_ptr.ptr,
+                                          o_idx.ptr, o_val.ptr, total.ptr, lb, stream.ptr), "compact")
 
-    def copy():
-        gp_ffi.hip_check(L.gpk_compact_copy(A.row_ptr, A.col_idx, A.val, m, d_rm.ptr, d_cm.ptr, d_by.ptr, o_ptr.ptr,
-                                            o_idx.ptr, o_val.ptr, m_new, stream.ptr), "compact copy")
-
-    out = {}
-    for name, fn in (("count", count), ("copy", copy)):
-        for _ in range(3):
-            fn()
-        stream.sync()
-        e0, e1 = gp_ffi.Event(), gp_ffi.Event()
-        e0.record(stream)
-        for _ in range(launches):
-            fn()
-        e1.record(stream)
-        out[name + "_us"] = 1e3 * e0.elapsed_ms(e1) / launches
-    # the timed kernels produce the compacted matrix: check it once
-    got_cnt = d_cnt.numpy()[:m]
-    if not np.array_equal(got_cnt, counts.astype(np.int32)):
-        raise SystemExit("bench.py: compaction counts differ from the host")
+    for _ in range(3):
+        compact()
+    stream.sync()
+    e0, e1 = gp_ffi.Event(), gp_ffi.Event()
+    e0.record(stream)
+    for _ in range(launches):
+        compact()
+    e1.record(stream)
+    us = 1e3 * e0.elapsed_ms(e1) / launches
+    if err.numpy()[0] != 0 or total.numpy()[0] != nnz_new:
+        raise SystemExit("bench.py: the compaction kernel did not produce the host's kept count")
+    want_ptr = np.concatenate([[0], np.cumsum(np.bincount(rows[kept], minlength=m)[keep_r])]).astype(np.int32)
+    if not np.array_equal(o_ptr.numpy(), want_ptr):
+        raise SystemExit("bench.py: compacted row pointers differ from the host")
     nbytes = compaction_bytes(m, n, len(lp["A_idx"]), m_new, nnz_new)
-    out.update(bytes=nbytes, achieved=nbytes / ((out["count_us"] + out["copy_us"]) * 1e-6) / 1e9,
-               workload=f"A of the LP ({m}x{n}, {len(lp['A_idx'])} nnz) with 15% of rows and columns dropped "
-                        f"-> {m_new} rows, {nnz_new} nnz")
-    return out
+    return {"us": us, "bytes": nbytes, "achieved": nbytes / (us * 1e-6) / 1e9,
+            "workload": f"A of the LP ({m}x{n}, {len(lp['A_idx'])} nnz) with 15% of rows and columns dropped "
+                        f"-> {m_new} rows, {nnz_new} nnz"}
 
 
 def main():
@@ -261,10 +262,9 @@ def main():
                    "row_len": "U[2,14]", "col_popularity": "Zipf(1.5) cap 5000", "seed": args.seed,
                    "reduced": [summary["reduced_rows"], summary["reduced_cols"], summary["reduced_nnz"]],
                    "iterations": summary["iterations"], "postsolve_ms": post_ms},
-        "roofline": {"bound": "hbm", "kernel": "compaction (k_compact_count + k_compact_copy)",
-                     "achieved": comp["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": comp["achieved"] / HBM_PEAK_GBS, "traffic": None,
-                     "us_per_launch": comp["count_us"] + comp["copy_us"], "algorithmic_bytes": comp["bytes"],
+        "roofline": {"bound": "hbm", "kernel": "k_compact_lb", "achieved": comp["achieved"],
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": comp["achieved"] / HBM_PEAK_GBS,
+                     "traffic": None, "us_per_launch": comp["us"], "algorithmic_bytes": comp["bytes"],
                      "workload": comp["workload"]},
         "roofline_activity": {"bound": "hbm", "kernel": "k_row_activity", "achieved": act["achieved"],
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": act["achieved"] / HBM_PEAK_GBS,

@@ -47,6 +47,10 @@ class Ctx {
   Ctx& operator=(const Ctx&) = delete;
 
   hipStream_t stream() const { return stream_; }
+  // Workspace of one single-pass (look-back) launch over `tiles` tiles: the
+  // persistent status array (grown on demand), a fresh zeroed ticket word and
+  // a new epoch.  The error flag is checked by sync().
+  gpk_lookback lookback(int64_t tiles);
   // Returns kFlags zeroed device flag words.
   int32_t* flags();
   // Copies the first `count` words of the flags last returned by flags() to
@@ -84,6 +88,14 @@ class Ctx {
   int32_t* d_flags_ = nullptr;
   int32_t* cur_ = nullptr;
   int slot_ = 0;
+  static constexpr int kTickets = 1024;
+  int32_t* d_tickets_ = nullptr;
+  int ticket_ = 0;
+  uint64_t* d_status_ = nullptr;
+  int64_t status_cap_ = 0;
+  uint32_t epoch_ = 0;
+  int32_t* h_error_ = nullptr;  // host-mapped
+  int32_t* d_error_ = nullptr;
   int32_t* h_flags_ = nullptr;
   void* pinned_ = nullptr;
 };

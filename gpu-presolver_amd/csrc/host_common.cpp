@@ -48,9 +48,34 @@ Ctx::Ctx(hipStream_t stream, bool own_stream) {
     own_stream_ = false;
   }
   check(hipMalloc(&d_flags_, sizeof(int32_t) * kFlags * kFlagSlots), "hipMalloc flags");
-  check(hipMemset(d_flags_, 0, sizeof(int32_t) * kFlags * kFlagSlots), "hipMemset flags");
+  check(hipMemsetAsync(d_flags_, 0, sizeof(int32_t) * kFlags * kFlagSlots, stream_), "hipMemset flags");
   check(hipHostMalloc(&h_flags_, sizeof(int32_t) * kFlags, hipHostMallocDefault), "hipHostMalloc flags");
   check(hipHostMalloc(&pinned_, 64, hipHostMallocDefault), "hipHostMalloc staging");
+  check(hipMalloc(&d_tickets_, sizeof(int32_t) * kTickets), "hipMalloc tickets");
+  check(hipMemsetAsync(d_tickets_, 0, sizeof(int32_t) * kTickets, stream_), "hipMemset tickets");
+  check(hipHostMalloc(&h_error_, sizeof(int32_t), hipHostMallocMapped), "hipHostMalloc error flag");
+  *h_error_ = 0;
+  check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_error_), h_error_, 0), "hipHostGetDevicePointer");
+}
+
+gpk_lookback Ctx::lookback(int64_t tiles) {
+  if (tiles > status_cap_) {
+    int64_t cap = 1024;
+    while (cap < tiles) cap <<= 1;
+    sync("look-back status growth");
+    if (d_status_) check(hipFree(d_status_), "hipFree look-back status");
+    d_status_ = nullptr;
+    check(hipMalloc(&d_status_, sizeof(uint64_t) * static_cast<size_t>(cap)), "hipMalloc look-back status");
+    check(hipMemsetAsync(d_status_, 0, sizeof(uint64_t) * static_cast<size_t>(cap), stream_),
+          "hipMemset look-back status");
+    status_cap_ = cap;
+  }
+  if (ticket_ == kTickets) {
+    check(hipMemsetAsync(d_tickets_, 0, sizeof(int32_t) * kTickets, stream_), "hipMemset tickets");
+    ticket_ = 0;
+  }
+  epoch_ = epoch_ + 1 >= (1u << 30) ? 1u : epoch_ + 1;
+  return gpk_lookback{d_status_, d_tickets_ + ticket_++, d_error_, epoch_};
 }
 
 Ctx::~Ctx() {
@@ -58,6 +83,9 @@ Ctx::~Ctx() {
   release_cached_blocks(stream_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   (void)hipFree(d_flags_);
+  (void)hipFree(d_tickets_);
+  if (d_status_) (void)hipFree(d_status_);
+  (void)hipHostFree(h_error_);
   (void)hipHostFree(h_flags_);
   (void)hipHostFree(pinned_);
   if (own_stream_) (void)hipStreamDestroy(stream_);
@@ -81,7 +109,13 @@ const int32_t* Ctx::read_flags(int count) {
   return h_flags_;
 }
 
-void Ctx::sync(const char* context) { check(hipStreamSynchronize(stream_), context); }
+void Ctx::sync(const char* context) {
+  check(hipStreamSynchronize(stream_), context);
+  if (*h_error_ != 0) {
+    *h_error_ = 0;
+    throw std::runtime_error(std::string(context) + ": a single-pass scan exceeded its spin limit");
+  }
+}
 
 Ctx& ctx() {
   if (t_current) return *t_current;
@@ -182,15 +216,21 @@ void dev_free(void* p, hipStream_t stream) noexcept {
 
 int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out) {
   Ctx& c = ctx();
-  DBuf<int32_t> scratch(gpk_scan_scratch_ints(n), "scan scratch");
-  check(gpk_scan_mask_u8(mask, out, n, scratch, c.stream()), "scan mask");
+  if (n <= 0) {
+    check(gpk_fill_i32(out, 0, 1, c.stream()), "scan empty");
+    return 0;
+  }
+  check(gpk_scan_lb_mask(mask, out, n, c.lookback(gpk_lookback_tiles(n)), c.stream()), "scan mask");
   return c.scalar_to_host(out + n, "scan total");
 }
 
 int32_t scan_counts(const int32_t* in, int32_t n, int32_t* out) {
   Ctx& c = ctx();
-  DBuf<int32_t> scratch(gpk_scan_scratch_ints(n), "scan scratch");
-  check(gpk_scan_exclusive_i32(in, out, n, scratch, c.stream()), "scan counts");
+  if (n <= 0) {
+    check(gpk_fill_i32(out, 0, 1, c.stream()), "scan empty");
+    return 0;
+  }
+  check(gpk_scan_lb_i32(in, out, n, c.lookback(gpk_lookback_tiles(n)), c.stream()), "scan counts");
   return c.scalar_to_host(out + n, "scan total");
 }
 
@@ -211,10 +251,10 @@ double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n) {
 int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map) {
   map.alloc(static_cast<int64_t>(n) + 1, "old_to_new");
   if (n == 0) return 0;
-  DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1);
-  const int32_t count = scan_mask_count(mask, n, prefix);
-  check(gpk_old_to_new(mask, prefix, map, n, ctx().stream()), "old_to_new");
-  return count;
+  Ctx& c = ctx();
+  int32_t* count = map.get() + n;  // spare word after the map
+  check(gpk_mask_to_map(mask, map, n, count, c.lookback(gpk_lookback_tiles(n)), c.stream()), "old_to_new");
+  return c.scalar_to_host(count, "old_to_new count");
 }
 
 void free_csr(DeviceCsrMatrix& A) noexcept {
@@ -242,16 +282,16 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
     out.rowPtr = ptr.release();
     return out;
   }
+  // One launch: kept entries are at most A.nnz, so the outputs are sized
+  // for that and the kept count is read back afterwards.
   const int32_t m_old = A.rows;
-  DBuf<int32_t> counts(static_cast<int64_t>(m_old) + 1, "compact counts");
-  check(gpk_compact_count(A.rowPtr, A.colVal, m_old, row_map, col_map, counts, c.stream()), "compact count");
-  const int32_t nnz = scan_counts(counts, m_old, counts);
-  DBuf<int32_t> idx(nnz, "compact colVal");
-  DBuf<double> val(nnz, "compact nzVal");
-  check(gpk_compact_copy(A.rowPtr, A.colVal, A.nzVal, m_old, row_map, col_map, counts, ptr, idx, val, m_new,
-                         c.stream()),
-        "compact copy");
-  out.nnz = nnz;
+  DBuf<int32_t> idx(A.nnz, "compact colVal");
+  DBuf<double> val(A.nnz, "compact nzVal");
+  int32_t* total = c.flags();
+  check(gpk_compact_lb(A.rowPtr, A.colVal, A.nzVal, m_old, row_map, col_map, m_new, ptr, idx, val, total,
+                       c.lookback(gpk_compact_tiles(m_old)), c.stream()),
+        "compact");
+  out.nnz = c.read_flags(1)[0];
   out.rowPtr = ptr.release();
   out.colVal = idx.release();
   out.nzVal = val.release();

@@ -3,7 +3,10 @@
 // (primal pass, then dual pass) and the final column-dual projection.
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
+#include <cmath>
 #include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "gp_host.hpp"
@@ -67,6 +70,277 @@ gp_orig_dev orig_view(const LPInfoGpu* o) {
   v.l = o->l;
   v.u = o->u;
   return v;
+}
+
+// ---- dependency-level replay schedule ---------------------------------------
+// The reference replays the tape in reverse with one lane.  Here each pass is
+// scheduled in levels: record k gets level 1 + the highest level among the
+// later records (replayed before it) that write a variable it reads or
+// writes, or read a variable it writes.  Records of one level touch disjoint
+// variables and run in parallel; every variable sees its reads and writes in
+// the serial order, so the results are the serial replay's, bit for bit.
+// Variables: column c (x, z and z_retrieved) -> c, row r (y) -> n0 + r.
+
+// Pattern of the original A and AT on the host, for the dual recoveries that
+// walk original rows or columns.
+struct OrigPattern {
+  std::vector<int32_t> a_ptr, a_idx, at_ptr, at_idx;
+};
+
+struct Segment {
+  int32_t begin, count;
+  bool serial;
+};
+
+struct Schedule {
+  std::vector<int32_t> order;
+  std::vector<Segment> segs;
+};
+
+constexpr int32_t kParallelMin = 64;
+
+[[noreturn]] void bad_tape(int32_t k, const char* what) {
+  throw std::runtime_error("postsolve_gpu: tape record " + std::to_string(k) + ": " + what);
+}
+
+class TapeScanner {
+ public:
+  TapeScanner(const PostsolveTape& t, int32_t n0, int32_t m0) : t_(t), n0_(n0), m0_(m0) {}
+
+  // Read and write sets of record k in the primal (primal = true) or dual
+  // pass; returns false when the record does nothing in that pass.
+  bool sets(int32_t k, bool primal, bool has_orig, bool has_y, const OrigPattern* op, std::vector<int32_t>& rd,
+            std::vector<int32_t>& wr) const {
+    rd.clear();
+    wr.clear();
+    const int32_t type = t_.types[k];
+    const int32_t i0 = t_.index_starts[k], i1 = t_.index_starts[k + 1] - 1;
+    const int32_t v0 = t_.value_starts[k], v1 = t_.value_starts[k + 1] - 1;
+    const int32_t* I = t_.indices.data();
+    const double* V = t_.vals.data();
+    const int32_t ni = i1 - i0 + 1, nv = v1 - v0 + 1;
+    auto need = [&](bool ok) {
+      if (!ok) bad_tape(k, "too few indices or values for its type");
+    };
+    switch (type) {
+      case 3: {  // DuplicateColumn
+        need(ni >= 2 && nv >= 5);
+        const int32_t from = col(k, I[i0]), to = col(k, I[i0 + 1]);
+        rd.push_back(to);
+        wr.push_back(from);
+        if (primal) wr.push_back(to);
+        return true;
+      }
+      case 13: {  // DoubletonEquation
+        need(ni >= 3 && nv >= 10);
+        const int32_t ec = col(k, I[i0]), kc = col(k, I[i0 + 1]), row = I[i0 + 2];
+        if (primal) {
+          rd.push_back(kc);
+          wr.push_back(ec);
+          return true;
+        }
+        if (has_orig && has_y) {
+          const int32_t rv = row_var(k, row);
+          rd.push_back(kc);
+          const bool payload = ni >= 4 && nv >= 12;
+          if (payload) {
+            need(ni >= 4 + I[i0 + 3] && nv >= 12 + I[i0 + 3]);
+            for (int32_t t = 0; t < I[i0 + 3]; ++t) rd.push_back(row_var(k, I[i0 + 4 + t]));
+          } else {
+            for (int32_t q = op->at_ptr[ec]; q < op->at_ptr[ec + 1]; ++q)
+              if (op->at_idx[q] != row) rd.push_back(row_var(k, op->at_idx[q]));
+          }
+          wr.push_back(rv);
+          wr.push_back(kc);
+          wr.push_back(ec);
+        } else {
+          wr.push_back(ec);
+          if (has_y && row >= 0) wr.push_back(row_var(k, row));
+        }
+        return true;
+      }
+      case 2: {  // SubCol
+        need(ni >= 3 && nv >= 6);
+        const int32_t ec = col(k, I[i0]), row = I[i0 + 1], sc = I[i0 + 2];
+        need(sc >= 0 && ni >= 3 + sc && nv >= 6 + sc);
+        for (int32_t t = 0; t < sc; ++t) rd.push_back(col(k, I[i0 + 3 + t]));
+        wr.push_back(ec);
+        if (!primal && has_y && row >= 0) {
+          rd.push_back(row_var(k, row));
+          wr.push_back(row_var(k, row));
+        }
+        return true;
+      }
+      case 0: {  // FixedCol
+        need(ni >= 1 && nv >= 1 + (primal ? 0 : 1) + (primal ? 0 : ni - 1));
+        wr.push_back(col(k, I[i0]));
+        if (!primal && has_y)
+          for (int32_t pos = i0 + 1; pos <= i1; ++pos) rd.push_back(row_var(k, I[pos]));
+        return true;
+      }
+      case 1: {  // FixedColInf
+        need(ni >= 2 && nv >= 2);
+        const int32_t c = col(k, I[i0 + 1]);
+        wr.push_back(c);
+        if (primal) {
+          const int64_t nrows = std::llround(V[v0]);
+          int32_t ip = i0 + 2, vp = v0 + 2;
+          for (int64_t r = 0; r < nrows; ++r) {
+            need(ip <= i1);
+            const int32_t len = I[ip];
+            need(len >= 0 && ip + len <= i1 && vp + len <= v1);
+            for (int32_t q = 0; q < len; ++q) rd.push_back(col(k, I[ip + 1 + q]));
+            ip += len + 1;
+            vp += len + 1;
+          }
+        }
+        return true;
+      }
+      case 14: {  // FmeCol
+        need(ni >= 3 && nv >= 1);
+        const int32_t K = I[i0];
+        need(K >= 0 && ni >= 3 + K);
+        const int32_t ec = col(k, I[i0 + 1]);
+        wr.push_back(ec);
+        if (primal) {
+          const int64_t L = std::llround(V[v0]);
+          int32_t cc = i0 + 3 + K, vp = v0 + 1;
+          for (int64_t s = 0; s < L; ++s) {
+            need(vp + 2 <= v1);
+            const int64_t len = std::llround(V[vp + 1]);
+            need(len >= 0 && cc + len - 1 <= i1 && vp + 2 + len <= v1);
+            for (int64_t q = 0; q < len; ++q) rd.push_back(col(k, I[cc++]));
+            vp += 3 + static_cast<int32_t>(len);
+          }
+        } else if (has_y) {
+          for (int32_t q = 0; q < K; ++q) wr.push_back(row_var(k, I[i0 + 2 + q]));
+        }
+        return true;
+      }
+      case 10:  // EqToIneq
+        if (primal || !has_y) return false;
+        need(ni >= 1 && nv >= 1);
+        rd.push_back(row_var(k, I[i0]));
+        wr.push_back(row_var(k, I[i0]));
+        return true;
+      case 4:  // DuplicateRow
+        if (primal || !has_y || !(ni >= 2 && nv >= 5)) return false;
+        for (int32_t q = 0; q < 2; ++q) {
+          rd.push_back(row_var(k, I[i0 + q]));
+          wr.push_back(row_var(k, I[i0 + q]));
+        }
+        return true;
+      case 5: {  // DeletedRow
+        if (primal || !has_orig || !(ni >= 2 && nv >= 3)) return false;
+        const int32_t row = I[i0], c = col(k, I[i0 + 1]);
+        rd.push_back(c);
+        if (has_y) {
+          for (int32_t q = op->at_ptr[c]; q < op->at_ptr[c + 1]; ++q)
+            if (op->at_idx[q] != row) rd.push_back(row_var(k, op->at_idx[q]));
+          wr.push_back(row_var(k, row));
+        }
+        wr.push_back(c);
+        return true;
+      }
+      case 12: {  // BoundChangeTheRow
+        if (primal || !has_orig || ni < 2) return false;
+        need(nv >= 4);
+        const int32_t c = col(k, I[i0]), row = I[i0 + 1];
+        const int32_t rv = row_var(k, row);
+        rd.push_back(c);
+        wr.push_back(c);
+        if (has_y) {
+          rd.push_back(rv);
+          wr.push_back(rv);
+        }
+        const int32_t r = rv - n0_;
+        for (int32_t q = op->a_ptr[r]; q < op->a_ptr[r + 1]; ++q) {
+          rd.push_back(col(k, op->a_idx[q]));
+          wr.push_back(op->a_idx[q]);
+        }
+        return true;
+      }
+      default:
+        return false;
+    }
+  }
+
+  // True when the dual pass needs the original pattern on the host.
+  bool dual_needs_pattern() const {
+    for (size_t k = 0; k < t_.types.size(); ++k) {
+      const int32_t type = t_.types[k];
+      if (type == 5 || type == 12) return true;
+      if (type == 13) {
+        const int32_t ni = t_.index_starts[k + 1] - t_.index_starts[k];
+        const int32_t nv = t_.value_starts[k + 1] - t_.value_starts[k];
+        if (!(ni >= 4 && nv >= 12)) return true;
+      }
+    }
+    return false;
+  }
+
+ private:
+  int32_t col(int32_t k, int32_t c) const {
+    if (c < 0 || c >= n0_) bad_tape(k, "column index out of range");
+    return c;
+  }
+  int32_t row_var(int32_t k, int32_t r) const {
+    if (r < 0 || r >= m0_) bad_tape(k, "row index out of range");
+    return n0_ + r;
+  }
+  const PostsolveTape& t_;
+  int32_t n0_, m0_;
+};
+
+Schedule level_schedule(const TapeScanner& scan, int32_t count, int32_t n0, int32_t m0, bool primal, bool has_orig,
+                        bool has_y, const OrigPattern* op) {
+  std::vector<int32_t> lw(static_cast<size_t>(n0) + m0, -1), lr(static_cast<size_t>(n0) + m0, -1);
+  std::vector<int32_t> level(count, -1);
+  std::vector<int32_t> rd, wr;
+  int32_t levels = 0;
+  for (int32_t k = count - 1; k >= 0; --k) {
+    if (!scan.sets(k, primal, has_orig, has_y, op, rd, wr)) continue;
+    int32_t lv = 0;
+    for (int32_t v : rd) lv = std::max(lv, lw[v] + 1);
+    for (int32_t v : wr) lv = std::max(lv, std::max(lw[v], lr[v]) + 1);
+    for (int32_t v : wr) lw[v] = lv;
+    for (int32_t v : rd) lr[v] = std::max(lr[v], lv);
+    level[k] = lv;
+    levels = std::max(levels, lv + 1);
+  }
+  std::vector<int32_t> start(static_cast<size_t>(levels) + 1, 0);
+  for (int32_t k = 0; k < count; ++k)
+    if (level[k] >= 0) ++start[level[k] + 1];
+  for (int32_t l = 0; l < levels; ++l) start[l + 1] += start[l];
+  Schedule sch;
+  sch.order.resize(start[levels]);
+  std::vector<int32_t> fill(start.begin(), start.end() - 1);
+  for (int32_t k = count - 1; k >= 0; --k)
+    if (level[k] >= 0) sch.order[fill[level[k]]++] = k;  // descending k within a level
+  for (int32_t l = 0; l < levels; ++l) {
+    const int32_t b = start[l], c = start[l + 1] - start[l];
+    if (c == 0) continue;
+    const bool small = c < kParallelMin;
+    if (small && !sch.segs.empty() && sch.segs.back().serial) {
+      sch.segs.back().count += c;  // consecutive small levels: one lane, level order
+    } else {
+      sch.segs.push_back(Segment{b, c, small});
+    }
+  }
+  return sch;
+}
+
+void run_schedule(const Schedule& sch, const gp_tape_dev& tape, const gp_orig_dev& orig, double* x, double* y,
+                  double* z, uint8_t* zret, bool primal, double tol) {
+  if (sch.order.empty()) return;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  DBuf<int32_t> order(static_cast<int64_t>(sch.order.size()));
+  c.to_device(order.get(), sch.order.data(), static_cast<int64_t>(sch.order.size()), "replay schedule upload");
+  for (const Segment& g : sch.segs)
+    check(gpk_replay_list(tape, orig, x, y, z, zret, order.get() + g.begin, g.count, g.serial ? 1 : 0, primal ? 1 : 0,
+                          primal ? 0 : 1, tol, s),
+          primal ? "replay primal" : "replay dual");
 }
 
 void upload_map(DBuf<int32_t>& dst, const std::vector<int32_t>& src, int32_t bound, const char* what) {
@@ -143,10 +417,22 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
     DeviceTape tape(record.tape);
     const gp_orig_dev orig = orig_view(original_model);
     double* y = m0 > 0 ? res.y_org : nullptr;
-    check(gpk_replay(tape.view, orig, res.x_org, y, res.z_org, zret, 0, tape.view.count, 1, 0, kPostsolveTol, s),
-          "replay primal");
-    check(gpk_replay(tape.view, orig, res.x_org, y, res.z_org, zret, 0, tape.view.count, 0, 1, kPostsolveTol, s),
-          "replay dual");
+    const TapeScanner scanner(record.tape, n0, m0);
+    const bool has_orig = original_model != nullptr;
+    const bool has_y = y != nullptr;
+    OrigPattern pattern;
+    if (has_orig && scanner.dual_needs_pattern()) {
+      const LPInfoGpu& o = *original_model;
+      pattern.a_ptr = c.to_host(o.A.rowPtr, static_cast<int64_t>(m0) + 1, "original A pattern");
+      pattern.a_idx = c.to_host(o.A.colVal, o.A.nnz, "original A pattern");
+      pattern.at_ptr = c.to_host(o.AT.rowPtr, static_cast<int64_t>(n0) + 1, "original AT pattern");
+      pattern.at_idx = c.to_host(o.AT.colVal, o.AT.nnz, "original AT pattern");
+    }
+    const int32_t count = tape.view.count;
+    run_schedule(level_schedule(scanner, count, n0, m0, true, has_orig, has_y, &pattern), tape.view, orig, res.x_org,
+                 y, res.z_org, zret, true, kPostsolveTol);
+    run_schedule(level_schedule(scanner, count, n0, m0, false, has_orig, has_y, &pattern), tape.view, orig, res.x_org,
+                 y, res.z_org, zret, false, kPostsolveTol);
     // project_column_duals_from_original (gpu_postsolve.cu:1370-1435)
     if (original_model != nullptr && y != nullptr && original_model->AT.rows > 0 && original_model->c &&
         original_model->l && original_model->u) {

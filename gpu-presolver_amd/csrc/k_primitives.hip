@@ -283,6 +283,228 @@ __global__ __launch_bounds__(kBlock) void k_compact_copy(const int32_t* __restri
   }
 }
 
+// ------------------------------------------------ decoupled look-back ----
+constexpr uint64_t kLbSpinLimit = 1ull << 20;
+
+__device__ inline uint64_t lb_word(uint32_t epoch, uint32_t flag, int32_t value) {
+  return (static_cast<uint64_t>(epoch) << 34) | (static_cast<uint64_t>(flag) << 32) |
+         static_cast<uint64_t>(static_cast<uint32_t>(value));
+}
+
+// Called by the first wave of the tile (all 64 lanes).  Publishes the tile's
+// aggregate, then looks back over 64 predecessors at a time: every lane waits
+// for its predecessor's word, the nearest inclusive prefix in the window ends
+// the walk, otherwise the window's aggregates are summed and the walk moves
+// 64 tiles further back.  Publishes the tile's inclusive prefix and returns
+// its exclusive prefix (in every lane).  Tile ids are tickets taken in
+// dispatch order, so every predecessor is already running and each wait
+// ends; the spin limit only guards a broken invariant (*error is set and the
+// result is meaningless).  The status words carry all the data a reader
+// needs, so relaxed device-scope atomics suffice: acquire/release would add
+// cache invalidation / write-back on every poll.
+__device__ int32_t lb_exclusive(const gpk_lookback& lb, int32_t tile, int32_t aggregate) {
+  const int lane = lane_id();
+  if (lane == 0)
+    __hip_atomic_store(lb.status + tile, lb_word(lb.epoch, tile == 0 ? 2u : 1u, aggregate), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  if (tile == 0) return 0;
+  int32_t excl = 0;
+  for (int32_t j0 = tile - 1;; j0 -= 64) {
+    const int32_t j = j0 - lane;
+    uint64_t w = lb_word(lb.epoch, 2u, 0);  // before tile 0: an inclusive prefix of 0
+    if (j >= 0) {
+      uint64_t spins = 0;
+      while (true) {
+        w = __hip_atomic_load(lb.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if ((w >> 34) == lb.epoch && ((w >> 32) & 3u) != 0) break;
+        if (++spins > kLbSpinLimit) {
+          __hip_atomic_store(lb.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+          w = lb_word(lb.epoch, 2u, 0);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+    const bool inclusive = ((w >> 32) & 3u) == 2u;
+    const uint64_t incl = __ballot(inclusive);
+    const int first = incl ? __ffsll(static_cast<unsigned long long>(incl)) - 1 : 64;
+    const int32_t v = lane <= first ? static_cast<int32_t>(static_cast<uint32_t>(w)) : 0;
+    excl += __shfl(wave_inclusive_sum(v), 63, 64);
+    if (incl) break;
+  }
+  if (lane == 0)
+    __hip_atomic_store(lb.status + tile, lb_word(lb.epoch, 2u, excl + aggregate), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+  return excl;
+}
+
+__device__ inline int32_t lb_ticket(const gpk_lookback& lb) {
+  __shared__ int32_t s_tile;
+  if (threadIdx.x == 0) s_tile = atomicAdd(lb.ticket, 1);
+  __syncthreads();
+  return s_tile;
+}
+
+// Exclusive scan, 2048 items per tile (8 consecutive per thread).  mode 0:
+// out = prefix (out[n] = total); mode 1 (mask input): out = prefix or -1 for
+// unset entries, *count = total.
+template <class In, int kMode>
+__global__ __launch_bounds__(kBlock) void k_scan_lb(const In* in, int32_t* out, int64_t n, int32_t* count,
+                                                    gpk_lookback lb) {
+  const int32_t tile = lb_ticket(lb);
+  if (tile >= static_cast<int32_t>(gridDim.x)) {  // the ticket word was not zeroed
+    if (threadIdx.x == 0) __hip_atomic_store(lb.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
+  const int64_t base = static_cast<int64_t>(tile) * kScanTile + threadIdx.x * kScanItems;
+  int32_t v[kScanItems];
+  int32_t s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    v[k] = base + k < n ? scan_value(in, base + k) : 0;
+    s += v[k];
+  }
+  int32_t total;
+  const int32_t ex = block_exclusive_sum(s, &total);
+  __shared__ int32_t s_excl;
+  if (threadIdx.x < 64) {
+    const int32_t e = lb_exclusive(lb, tile, total);
+    if (threadIdx.x == 0) s_excl = e;
+  }
+  __syncthreads();
+  int32_t run = s_excl + ex;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    if (base + k < n) out[base + k] = kMode == 1 ? (v[k] ? run : -1) : run;
+    run += v[k];
+  }
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (threadIdx.x == 0 && tile == tiles - 1) {
+    if (kMode == 1) *count = s_excl + total;
+    else out[n] = s_excl + total;
+  }
+}
+
+// Fused stable compaction.  Tile = 1024 old rows (4 per thread), so a large
+// matrix has few enough tiles that the look-back stays short.  Pass 1 maps
+// every entry of the tile's segment (row kept, column map) and counts kept
+// entries per row in LDS; the tile's kept total goes through the look-back;
+// pass 2 maps the entries again, stages each 2048-entry chunk of kept
+// (column, value) pairs in LDS and writes it with coalesced stores.
+constexpr int kCompRows = 1024;
+constexpr int kCompRowsPerThread = kCompRows / kBlock;
+
+__global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict__ ptr,
+                                                       const int32_t* __restrict__ idx,
+                                                       const double* __restrict__ val, int32_t m_old,
+                                                       const int32_t* __restrict__ row_map,
+                                                       const int32_t* __restrict__ col_map, int32_t m_new,
+                                                       int32_t* __restrict__ out_ptr, int32_t* __restrict__ out_idx,
+                                                       double* __restrict__ out_val, int32_t* __restrict__ total_out,
+                                                       gpk_lookback lb) {
+  __shared__ int32_t s_ptr[kCompRows + 1];
+  __shared__ int32_t s_cnt[kCompRows];
+  __shared__ uint8_t s_keep[kCompRows];
+  __shared__ int32_t s_oidx[kCompChunk];
+  __shared__ double s_oval[kCompChunk];
+  __shared__ int32_t s_excl;
+  const int32_t tile = lb_ticket(lb);
+  if (tile >= static_cast<int32_t>(gridDim.x)) {  // the ticket word was not zeroed
+    if (threadIdx.x == 0) __hip_atomic_store(lb.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
+  const int32_t r0 = tile * kCompRows;
+  const int rows = min(kCompRows, m_old - r0);
+  const int tid = threadIdx.x;
+  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = ptr[r0 + t];
+  for (int t = tid; t < kCompRows; t += kBlock) {
+    s_cnt[t] = 0;
+    s_keep[t] = (t < rows && row_map[r0 + t] >= 0) ? 1 : 0;
+  }
+  __syncthreads();
+  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
+  // pass 1: per-row kept counts
+  for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
+    const int32_t p0 = c + tid * kCompItems;
+    int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
+    int local = 0;
+    for (int k = 0; k < kCompItems; ++k) {
+      const int32_t p = p0 + k;
+      if (p >= seg_e) break;
+      while (s_ptr[row + 1] <= p) {
+        if (local) { atomicAdd(&s_cnt[row], local); local = 0; }
+        ++row;
+      }
+      if (s_keep[row] && col_map[idx[p]] >= 0) ++local;
+    }
+    if (local) atomicAdd(&s_cnt[row], local);
+  }
+  __syncthreads();
+  // row offsets: thread t owns rows 4t .. 4t+3
+  int32_t mine[kCompRowsPerThread];
+  int32_t msum = 0;
+#pragma unroll
+  for (int k = 0; k < kCompRowsPerThread; ++k) {
+    const int r = tid * kCompRowsPerThread + k;
+    mine[k] = s_keep[r] ? s_cnt[r] : 0;
+    msum += mine[k];
+  }
+  int32_t agg;
+  int32_t off = block_exclusive_sum(msum, &agg);
+  if (tid < 64) {
+    const int32_t e = lb_exclusive(lb, tile, agg);
+    if (tid == 0) s_excl = e;
+  }
+  __syncthreads();
+  const int32_t excl = s_excl;
+#pragma unroll
+  for (int k = 0; k < kCompRowsPerThread; ++k) {
+    const int r = tid * kCompRowsPerThread + k;
+    if (r < rows && s_keep[r]) out_ptr[row_map[r0 + r]] = excl + off;
+    off += mine[k];
+  }
+  const int32_t tiles = (m_old + kCompRows - 1) / kCompRows;
+  if (tid == 0 && tile == tiles - 1) {
+    out_ptr[m_new] = excl + agg;
+    *total_out = excl + agg;
+  }
+  // pass 2: stage kept entries chunk by chunk, then coalesced stores
+  int32_t write_base = excl;
+  for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
+    const int32_t p0 = c + tid * kCompItems;
+    int32_t jn[kCompItems];
+    int32_t cnt = 0;
+    int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      const int32_t p = p0 + k;
+      jn[k] = -1;
+      if (p < seg_e) {
+        while (s_ptr[row + 1] <= p) ++row;
+        if (s_keep[row]) jn[k] = col_map[idx[p]];
+        cnt += jn[k] >= 0 ? 1 : 0;
+      }
+    }
+    int32_t chunk_total;
+    int32_t pos = block_exclusive_sum(cnt, &chunk_total);
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      if (jn[k] >= 0) {
+        s_oidx[pos] = jn[k];
+        s_oval[pos] = val[p0 + k];
+        ++pos;
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < chunk_total; t += kBlock) {
+      out_idx[write_base + t] = s_oidx[t];
+      out_val[write_base + t] = s_oval[t];
+    }
+    write_base += chunk_total;
+    __syncthreads();
+  }
+}
+
 __global__ void k_gather5(const int32_t* map, int32_t n_old, const double* s0, double* d0, const double* s1,
                           double* d1, const double* s2, double* d2, const double* s3, double* d3,
                           const double* s4, double* d4) {
@@ -530,6 +752,44 @@ hipError_t gpk_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, in
 
 hipError_t gpk_scan_mask_u8(const uint8_t* mask, int32_t* out, int64_t n, int32_t* scratch, hipStream_t s) {
   return scan_impl<uint8_t>(mask, out, n, scratch, s);
+}
+
+int64_t gpk_lookback_tiles(int64_t n) { return n > 0 ? (n + kScanTile - 1) / kScanTile : 1; }
+
+hipError_t gpk_scan_lb_i32(const int32_t* in, int32_t* out, int64_t n, gpk_lookback lb, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL((k_scan_lb<int32_t, 0>), dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, in, out, n,
+                     nullptr, lb);
+  return hipGetLastError();
+}
+
+hipError_t gpk_scan_lb_mask(const uint8_t* mask, int32_t* out, int64_t n, gpk_lookback lb, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL((k_scan_lb<uint8_t, 0>), dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, mask, out, n,
+                     nullptr, lb);
+  return hipGetLastError();
+}
+
+hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t* count, gpk_lookback lb,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL((k_scan_lb<uint8_t, 1>), dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, mask, map, n,
+                     count, lb);
+  return hipGetLastError();
+}
+
+int64_t gpk_compact_tiles(int32_t m_old) { return m_old > 0 ? (m_old + kCompRows - 1) / kCompRows : 1; }
+
+hipError_t gpk_compact_lb(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
+                          const int32_t* row_map, const int32_t* col_map, int32_t m_new, int32_t* out_ptr,
+                          int32_t* out_idx, double* out_val, int32_t* total, gpk_lookback lb, hipStream_t s) {
+  if (m_old <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact_lb, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, idx, val, m_old, row_map,
+                     col_map, m_new, out_ptr, out_idx, out_val, total, lb);
+  return hipGetLastError();
 }
 
 hipError_t gpk_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* o2n, int32_t n, hipStream_t s) {

@@ -166,212 +166,233 @@ __device__ void dbl_recover_z(double xk, double zk_hat, double al, double okl, d
 // Reverse replay of tape records [begin, end) (gpu_postsolve.cu:605-1046).
 // The records form a dependency chain in general, so one lane walks them in
 // order; every other postsolve step is data-parallel.
+// One record of the reverse replay (_kernel_replay_postsolve_tape,
+// gpu_postsolve.cu:605-1046).
+__device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x, double* y, double* z,
+                           uint8_t* zret, int32_t k, bool primal, bool dual, double tol) {
+  const bool has_orig = O.present != 0;
+  const int32_t type = T.types[k];
+  const int32_t i0 = T.index_starts[k], i1 = T.index_starts[k + 1] - 1;
+  const int32_t v0 = T.value_starts[k], v1 = T.value_starts[k + 1] - 1;
+  const int32_t* I = T.indices;
+  const double* V = T.vals;
+  if (type == 3) {  // DuplicateColumn
+    const int32_t from = I[i0], to = I[i0 + 1];
+    const double r = V[v0];
+    if (primal) {
+      double xf = 0.0, xt = x[to];
+      recover_dup_col(x[to], z ? z[to] : 0.0, r, V[v0 + 1], V[v0 + 2], V[v0 + 3], V[v0 + 4], tol, &xf, &xt);
+      x[from] = xf;
+      x[to] = xt;
+    }
+    if (dual && z) { z[from] = r * z[to]; if (zret) zret[from] = 1; }
+  } else if (type == 13) {  // DoubletonEquation
+    const int32_t ec = I[i0], kc = I[i0 + 1], row = I[i0 + 2];
+    const double ecoef = V[v0], kcoef = V[v0 + 1], rhs = V[v0 + 2], eobj = V[v0 + 9];
+    const double al = -kcoef / ecoef, be = rhs / ecoef;
+    if (primal) x[ec] = be + al * x[kc];
+    if (dual && z && y && has_orig) {
+      const double ls = v1 >= v0 + 10 ? V[v0 + 10] : 0.0;
+      const double us = v1 >= v0 + 11 ? V[v0 + 11] : 0.0;
+      double zk = 0.0, ze = 0.0;
+      dbl_recover_z(x[kc], z[kc], al, V[v0 + 5], V[v0 + 6], V[v0 + 7], V[v0 + 8], ls, us, tol, &zk, &ze);
+      const bool payload = i1 >= i0 + 3 && v1 >= v0 + 11;
+      double so = 0.0;
+      double yrow;
+      if (payload) {
+        for (int32_t t = 0; t < I[i0 + 3]; ++t) so += V[v0 + 12 + t] * y[I[i0 + 4 + t]];
+        yrow = (eobj - so - ze) / ecoef;
+      } else {
+        for (int32_t q = O.at_ptr[ec]; q < O.at_ptr[ec + 1]; ++q)
+          if (O.at_idx[q] != row) so += O.at_val[q] * y[O.at_idx[q]];
+        yrow = (O.c[ec] - so - ze) / ecoef;
+      }
+      y[row] = yrow;
+      z[kc] = zk;
+      z[ec] = ze;
+      if (zret) { zret[kc] = 1; zret[ec] = 1; }
+    } else if (dual) {
+      if (z) { z[ec] = 0.0; if (zret) zret[ec] = 1; }
+      if (y && row >= 0) y[row] = eobj / ecoef;
+    }
+  } else if (type == 2) {  // SubCol
+    const int32_t ec = I[i0], row = I[i0 + 1], sc = I[i0 + 2];
+    const double piv = V[v0], rhs = V[v0 + 1], eobj = V[v0 + 4];
+    const bool row_deleted = V[v0 + 5] != 0.0;
+    double part = 0.0;
+    for (int32_t t = 0; t < sc; ++t) part += V[v0 + 6 + t] * x[I[i0 + 3 + t]];
+    if (primal) x[ec] = (rhs - part) / piv;
+    if (dual && y && row >= 0) {
+      if (row_deleted) y[row] = eobj / piv; else y[row] += eobj / piv;
+    }
+    if (dual && z) {
+      z[ec] = (row_deleted || !y) ? 0.0 : eobj - piv * y[row];
+      if (zret) zret[ec] = 1;
+    }
+  } else if (type == 0) {  // FixedCol
+    const int32_t col = I[i0];
+    if (primal) x[col] = V[v0];
+    if (dual && z) {
+      double zj = V[v0 + 1];
+      for (int32_t pos = i0 + 1; pos <= i1; ++pos) {
+        const double coef = V[v0 + 2 + (pos - (i0 + 1))];
+        if (y) zj -= coef * y[I[pos]];
+      }
+      z[col] = zj;
+      if (zret) zret[col] = 1;
+    }
+  } else if (type == 1) {  // FixedColInf
+    const bool pos_inf = I[i0] > 0;
+    const int32_t col = I[i0 + 1];
+    const int32_t nrows = static_cast<int32_t>(llround(V[v0]));
+    double ext = V[v0 + 1];
+    int32_t ip = i0 + 2, vp = v0 + 2;
+    if (primal) {
+      for (int32_t r = 0; r < nrows; ++r) {
+        const int32_t len = I[ip];
+        double side = V[vp], coef = 0.0;
+        for (int32_t t = 0; t < len; ++t) {
+          const int32_t rc = I[ip + 1 + t];
+          const double rv = V[vp + 1 + t];
+          if (rc == col) coef = rv; else side -= rv * x[rc];
+        }
+        const double cand = side / coef;
+        ext = pos_inf ? fmax(ext, cand) : fmin(ext, cand);
+        ip += len + 1;
+        vp += len + 1;
+      }
+      x[col] = ext;
+    }
+    if (dual && z) { z[col] = 0.0; if (zret) zret[col] = 1; }
+  } else if (type == 14) {  // FmeCol
+    const int32_t K = I[i0], ec = I[i0 + 1];
+    const int32_t L = static_cast<int32_t>(llround(V[v0]));
+    int32_t cc = i0 + 3 + K, vp = v0 + 1;
+    double lo = -INFINITY, hi = INFINITY;
+    bool hl = false, hh = false;
+    if (primal) {
+      for (int32_t s = 0; s < L; ++s) {
+        const double orient = V[vp];
+        const int32_t len = static_cast<int32_t>(llround(V[vp + 1]));
+        double cand = V[vp + 2];
+        for (int32_t t = 0; t < len; ++t) cand += V[vp + 3 + t] * x[I[cc++]];
+        if (orient > 0.0) { lo = fmax(lo, cand); hl = true; }
+        else { hi = fmin(hi, cand); hh = true; }
+        vp += 3 + len;
+      }
+      x[ec] = hl ? lo : (hh ? hi : 0.0);
+    }
+    if (dual && z) { z[ec] = 0.0; if (zret) zret[ec] = 1; }
+    if (dual && y) for (int32_t t = 0; t < K; ++t) y[I[i0 + 2 + t]] = 0.0;
+  } else if (dual && type == 10) {  // EqToIneq
+    if (y) y[I[i0]] += V[v0];
+  } else if (dual && type == 4 && i1 >= i0 + 1 && v1 >= v0 + 4) {  // DuplicateRow
+    const int32_t kr = I[i0], dr = I[i0 + 1];
+    const double r = V[v0], kal = V[v0 + 1], kau = V[v0 + 2], dal = V[v0 + 3], dau = V[v0 + 4];
+    if (fabs(r) <= tol || !y) return;
+    const double a = dal * r, b = dau * r;
+    const double dsl = r >= 0.0 ? a : b, dsu = r >= 0.0 ? b : a;
+    if (dsl <= kal + tol && kau <= dsu + tol) { y[dr] = 0.0; return; }
+    if (has_orig) {
+      double ap = 0.0, ad = 0.0;
+      for (int32_t q = O.a_ptr[kr]; q < O.a_ptr[kr + 1]; ++q) ap += O.a_val[q] * x[O.a_idx[q]];
+      for (int32_t q = O.a_ptr[dr]; q < O.a_ptr[dr + 1]; ++q) ad += O.a_val[q] * x[O.a_idx[q]];
+      const uint8_t ks = row_state(ap, kal, kau, tol), ds = row_state(ad, dal, dau, tol);
+      double ydl, ydh, ypl, yph;
+      state_interval(ds, &ydl, &ydh);
+      deleted_dual_interval(y[kr], r, ks, &ypl, &yph);
+      const double sl = fmax(ydl, ypl), sh = fmin(ydh, yph);
+      if (sl <= sh + tol) {
+        const double ydn = clamp_inf(0.0, sl, sh);
+        y[kr] = y[kr] - ydn / r;
+        y[dr] = ydn;
+        return;
+      }
+      double ypd = y[kr], ydd = 0.0;
+      if (doc_dual_split(y[kr], r, kal, kau, dsl, dsu, tol, &ypd, &ydd)) { y[kr] = ypd; y[dr] = ydd; }
+    }
+  } else if (dual && type == 5 && has_orig && i1 >= i0 + 1 && v1 >= v0 + 2) {  // DeletedRow
+    const int32_t row = I[i0], col = I[i0 + 1];
+    const double oal = V[v0], oau = V[v0 + 1], aij = V[v0 + 2];
+    if (fabs(aij) <= tol) return;
+    double gamma = O.c[col];
+    for (int32_t q = O.at_ptr[col]; q < O.at_ptr[col + 1]; ++q)
+      if (O.at_idx[q] != row && y) gamma -= O.at_val[q] * y[O.at_idx[q]];
+    const double xj = x[col], zbar = z[col];
+    const double act = aij * xj;
+    const bool ral = pfin(oal) && act <= oal + tol;
+    const bool rau = pfin(oau) && act >= oau - tol;
+    double izl = -INFINITY, izh = INFINITY;
+    if (!(ral && rau)) {
+      if (ral) { if (aij > 0.0) izh = gamma; else izl = gamma; }
+      else if (rau) { if (aij > 0.0) izl = gamma; else izh = gamma; }
+      else { izl = gamma; izh = gamma; }
+    }
+    const bool atl = at_lower(xj, O.l[col], tol), atu = at_upper(xj, O.u[col], tol);
+    double zl = izl, zh = izh;
+    if (!atl && !atu) {
+      if ((pfin(izl) && izl > tol) || (pfin(izh) && izh < -tol)) { zl = gamma; zh = gamma; }
+      else { zl = 0.0; zh = 0.0; }
+    } else if (atl && !atu) {
+      zl = fmax(zl, 0.0);
+    } else if (!atl && atu) {
+      zh = fmin(zh, 0.0);
+    }
+    if (pfin(zl) && pfin(zh) && zl > zh + tol) return;
+    const double zn = clamp_inf(zbar, zl, zh);
+    z[col] = zn;
+    if (y) y[row] = (gamma - zn) / aij;
+    if (zret) zret[col] = 1;
+  } else if (dual && type == 12 && has_orig && i1 >= i0 + 1) {  // BoundChangeTheRow
+    const int32_t col = I[i0], row = I[i0 + 1];
+    if (zret && !zret[col]) return;
+    const double ol = V[v0], ou = V[v0 + 1], nl = V[v0 + 2], nu = V[v0 + 3];
+    const bool lc = nl > ol + tol, uc = nu < ou - tol;
+    if (lc == uc) return;
+    const double implied = lc ? nl : nu;
+    const double other = lc ? ou : ol;
+    const bool other_is_lower = !lc;
+    const double xj = x[col], zj = z[col];
+    if (pfin(other) && fabs(xj - other) <= tol) {
+      const bool ok = other_is_lower ? (zj >= -tol) : (zj <= tol);
+      if (ok) return;
+    }
+    if (fabs(xj - implied) > tol) return;
+    double aij = 0.0;
+    for (int32_t q = O.a_ptr[row]; q < O.a_ptr[row + 1]; ++q)
+      if (O.a_idx[q] == col) { aij = O.a_val[q]; break; }
+    if (fabs(aij) <= tol) return;
+    if (y) y[row] += zj / aij;
+    for (int32_t q = O.a_ptr[row]; q < O.a_ptr[row + 1]; ++q) {
+      const int32_t kc = O.a_idx[q];
+      if (kc == col || (zret && !zret[kc])) continue;
+      z[kc] -= (O.a_val[q] / aij) * zj;
+    }
+    z[col] = 0.0;
+    if (zret) zret[col] = 1;
+  }
+}
+
+// Records begin..end-1 in reverse, one lane (the reference's order).
 __global__ void k_replay(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
                          int32_t begin, int32_t end, int32_t primal, int32_t dual, double tol) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  const bool has_orig = O.present != 0;
-  for (int32_t k = end - 1; k >= begin; --k) {
-    const int32_t type = T.types[k];
-    const int32_t i0 = T.index_starts[k], i1 = T.index_starts[k + 1] - 1;
-    const int32_t v0 = T.value_starts[k], v1 = T.value_starts[k + 1] - 1;
-    const int32_t* I = T.indices;
-    const double* V = T.vals;
-    if (type == 3) {  // DuplicateColumn
-      const int32_t from = I[i0], to = I[i0 + 1];
-      const double r = V[v0];
-      if (primal) {
-        double xf = 0.0, xt = x[to];
-        recover_dup_col(x[to], z ? z[to] : 0.0, r, V[v0 + 1], V[v0 + 2], V[v0 + 3], V[v0 + 4], tol, &xf, &xt);
-        x[from] = xf;
-        x[to] = xt;
-      }
-      if (dual && z) { z[from] = r * z[to]; if (zret) zret[from] = 1; }
-    } else if (type == 13) {  // DoubletonEquation
-      const int32_t ec = I[i0], kc = I[i0 + 1], row = I[i0 + 2];
-      const double ecoef = V[v0], kcoef = V[v0 + 1], rhs = V[v0 + 2], eobj = V[v0 + 9];
-      const double al = -kcoef / ecoef, be = rhs / ecoef;
-      if (primal) x[ec] = be + al * x[kc];
-      if (dual && z && y && has_orig) {
-        const double ls = v1 >= v0 + 10 ? V[v0 + 10] : 0.0;
-        const double us = v1 >= v0 + 11 ? V[v0 + 11] : 0.0;
-        double zk = 0.0, ze = 0.0;
-        dbl_recover_z(x[kc], z[kc], al, V[v0 + 5], V[v0 + 6], V[v0 + 7], V[v0 + 8], ls, us, tol, &zk, &ze);
-        const bool payload = i1 >= i0 + 3 && v1 >= v0 + 11;
-        double so = 0.0;
-        double yrow;
-        if (payload) {
-          for (int32_t t = 0; t < I[i0 + 3]; ++t) so += V[v0 + 12 + t] * y[I[i0 + 4 + t]];
-          yrow = (eobj - so - ze) / ecoef;
-        } else {
-          for (int32_t q = O.at_ptr[ec]; q < O.at_ptr[ec + 1]; ++q)
-            if (O.at_idx[q] != row) so += O.at_val[q] * y[O.at_idx[q]];
-          yrow = (O.c[ec] - so - ze) / ecoef;
-        }
-        y[row] = yrow;
-        z[kc] = zk;
-        z[ec] = ze;
-        if (zret) { zret[kc] = 1; zret[ec] = 1; }
-      } else if (dual) {
-        if (z) { z[ec] = 0.0; if (zret) zret[ec] = 1; }
-        if (y && row >= 0) y[row] = eobj / ecoef;
-      }
-    } else if (type == 2) {  // SubCol
-      const int32_t ec = I[i0], row = I[i0 + 1], sc = I[i0 + 2];
-      const double piv = V[v0], rhs = V[v0 + 1], eobj = V[v0 + 4];
-      const bool row_deleted = V[v0 + 5] != 0.0;
-      double part = 0.0;
-      for (int32_t t = 0; t < sc; ++t) part += V[v0 + 6 + t] * x[I[i0 + 3 + t]];
-      if (primal) x[ec] = (rhs - part) / piv;
-      if (dual && y && row >= 0) {
-        if (row_deleted) y[row] = eobj / piv; else y[row] += eobj / piv;
-      }
-      if (dual && z) {
-        z[ec] = (row_deleted || !y) ? 0.0 : eobj - piv * y[row];
-        if (zret) zret[ec] = 1;
-      }
-    } else if (type == 0) {  // FixedCol
-      const int32_t col = I[i0];
-      if (primal) x[col] = V[v0];
-      if (dual && z) {
-        double zj = V[v0 + 1];
-        for (int32_t pos = i0 + 1; pos <= i1; ++pos) {
-          const double coef = V[v0 + 2 + (pos - (i0 + 1))];
-          if (y) zj -= coef * y[I[pos]];
-        }
-        z[col] = zj;
-        if (zret) zret[col] = 1;
-      }
-    } else if (type == 1) {  // FixedColInf
-      const bool pos_inf = I[i0] > 0;
-      const int32_t col = I[i0 + 1];
-      const int32_t nrows = static_cast<int32_t>(llround(V[v0]));
-      double ext = V[v0 + 1];
-      int32_t ip = i0 + 2, vp = v0 + 2;
-      if (primal) {
-        for (int32_t r = 0; r < nrows; ++r) {
-          const int32_t len = I[ip];
-          double side = V[vp], coef = 0.0;
-          for (int32_t t = 0; t < len; ++t) {
-            const int32_t rc = I[ip + 1 + t];
-            const double rv = V[vp + 1 + t];
-            if (rc == col) coef = rv; else side -= rv * x[rc];
-          }
-          const double cand = side / coef;
-          ext = pos_inf ? fmax(ext, cand) : fmin(ext, cand);
-          ip += len + 1;
-          vp += len + 1;
-        }
-        x[col] = ext;
-      }
-      if (dual && z) { z[col] = 0.0; if (zret) zret[col] = 1; }
-    } else if (type == 14) {  // FmeCol
-      const int32_t K = I[i0], ec = I[i0 + 1];
-      const int32_t L = static_cast<int32_t>(llround(V[v0]));
-      int32_t cc = i0 + 3 + K, vp = v0 + 1;
-      double lo = -INFINITY, hi = INFINITY;
-      bool hl = false, hh = false;
-      if (primal) {
-        for (int32_t s = 0; s < L; ++s) {
-          const double orient = V[vp];
-          const int32_t len = static_cast<int32_t>(llround(V[vp + 1]));
-          double cand = V[vp + 2];
-          for (int32_t t = 0; t < len; ++t) cand += V[vp + 3 + t] * x[I[cc++]];
-          if (orient > 0.0) { lo = fmax(lo, cand); hl = true; }
-          else { hi = fmin(hi, cand); hh = true; }
-          vp += 3 + len;
-        }
-        x[ec] = hl ? lo : (hh ? hi : 0.0);
-      }
-      if (dual && z) { z[ec] = 0.0; if (zret) zret[ec] = 1; }
-      if (dual && y) for (int32_t t = 0; t < K; ++t) y[I[i0 + 2 + t]] = 0.0;
-    } else if (dual && type == 10) {  // EqToIneq
-      if (y) y[I[i0]] += V[v0];
-    } else if (dual && type == 4 && i1 >= i0 + 1 && v1 >= v0 + 4) {  // DuplicateRow
-      const int32_t kr = I[i0], dr = I[i0 + 1];
-      const double r = V[v0], kal = V[v0 + 1], kau = V[v0 + 2], dal = V[v0 + 3], dau = V[v0 + 4];
-      if (fabs(r) <= tol || !y) continue;
-      const double a = dal * r, b = dau * r;
-      const double dsl = r >= 0.0 ? a : b, dsu = r >= 0.0 ? b : a;
-      if (dsl <= kal + tol && kau <= dsu + tol) { y[dr] = 0.0; continue; }
-      if (has_orig) {
-        double ap = 0.0, ad = 0.0;
-        for (int32_t q = O.a_ptr[kr]; q < O.a_ptr[kr + 1]; ++q) ap += O.a_val[q] * x[O.a_idx[q]];
-        for (int32_t q = O.a_ptr[dr]; q < O.a_ptr[dr + 1]; ++q) ad += O.a_val[q] * x[O.a_idx[q]];
-        const uint8_t ks = row_state(ap, kal, kau, tol), ds = row_state(ad, dal, dau, tol);
-        double ydl, ydh, ypl, yph;
-        state_interval(ds, &ydl, &ydh);
-        deleted_dual_interval(y[kr], r, ks, &ypl, &yph);
-        const double sl = fmax(ydl, ypl), sh = fmin(ydh, yph);
-        if (sl <= sh + tol) {
-          const double ydn = clamp_inf(0.0, sl, sh);
-          y[kr] = y[kr] - ydn / r;
-          y[dr] = ydn;
-          continue;
-        }
-        double ypd = y[kr], ydd = 0.0;
-        if (doc_dual_split(y[kr], r, kal, kau, dsl, dsu, tol, &ypd, &ydd)) { y[kr] = ypd; y[dr] = ydd; }
-      }
-    } else if (dual && type == 5 && has_orig && i1 >= i0 + 1 && v1 >= v0 + 2) {  // DeletedRow
-      const int32_t row = I[i0], col = I[i0 + 1];
-      const double oal = V[v0], oau = V[v0 + 1], aij = V[v0 + 2];
-      if (fabs(aij) <= tol) continue;
-      double gamma = O.c[col];
-      for (int32_t q = O.at_ptr[col]; q < O.at_ptr[col + 1]; ++q)
-        if (O.at_idx[q] != row && y) gamma -= O.at_val[q] * y[O.at_idx[q]];
-      const double xj = x[col], zbar = z[col];
-      const double act = aij * xj;
-      const bool ral = pfin(oal) && act <= oal + tol;
-      const bool rau = pfin(oau) && act >= oau - tol;
-      double izl = -INFINITY, izh = INFINITY;
-      if (!(ral && rau)) {
-        if (ral) { if (aij > 0.0) izh = gamma; else izl = gamma; }
-        else if (rau) { if (aij > 0.0) izl = gamma; else izh = gamma; }
-        else { izl = gamma; izh = gamma; }
-      }
-      const bool atl = at_lower(xj, O.l[col], tol), atu = at_upper(xj, O.u[col], tol);
-      double zl = izl, zh = izh;
-      if (!atl && !atu) {
-        if ((pfin(izl) && izl > tol) || (pfin(izh) && izh < -tol)) { zl = gamma; zh = gamma; }
-        else { zl = 0.0; zh = 0.0; }
-      } else if (atl && !atu) {
-        zl = fmax(zl, 0.0);
-      } else if (!atl && atu) {
-        zh = fmin(zh, 0.0);
-      }
-      if (pfin(zl) && pfin(zh) && zl > zh + tol) continue;
-      const double zn = clamp_inf(zbar, zl, zh);
-      z[col] = zn;
-      if (y) y[row] = (gamma - zn) / aij;
-      if (zret) zret[col] = 1;
-    } else if (dual && type == 12 && has_orig && i1 >= i0 + 1) {  // BoundChangeTheRow
-      const int32_t col = I[i0], row = I[i0 + 1];
-      if (zret && !zret[col]) continue;
-      const double ol = V[v0], ou = V[v0 + 1], nl = V[v0 + 2], nu = V[v0 + 3];
-      const bool lc = nl > ol + tol, uc = nu < ou - tol;
-      if (lc == uc) continue;
-      const double implied = lc ? nl : nu;
-      const double other = lc ? ou : ol;
-      const bool other_is_lower = !lc;
-      const double xj = x[col], zj = z[col];
-      if (pfin(other) && fabs(xj - other) <= tol) {
-        const bool ok = other_is_lower ? (zj >= -tol) : (zj <= tol);
-        if (ok) continue;
-      }
-      if (fabs(xj - implied) > tol) continue;
-      double aij = 0.0;
-      for (int32_t q = O.a_ptr[row]; q < O.a_ptr[row + 1]; ++q)
-        if (O.a_idx[q] == col) { aij = O.a_val[q]; break; }
-      if (fabs(aij) <= tol) continue;
-      if (y) y[row] += zj / aij;
-      for (int32_t q = O.a_ptr[row]; q < O.a_ptr[row + 1]; ++q) {
-        const int32_t kc = O.a_idx[q];
-        if (kc == col || (zret && !zret[kc])) continue;
-        z[kc] -= (O.a_val[q] / aij) * zj;
-      }
-      z[col] = 0.0;
-      if (zret) zret[col] = 1;
-    }
+  for (int32_t k = end - 1; k >= begin; --k) replay_one(T, O, x, y, z, zret, k, primal != 0, dual != 0, tol);
+}
+
+// One segment of a dependency-level schedule: either the records of one
+// level, mutually independent, one per thread; or (serial != 0) a run of
+// small levels replayed in list order by a single lane.
+__global__ void k_replay_list(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
+                              const int32_t* records, int32_t count, int32_t serial, int32_t primal, int32_t dual,
+                              double tol) {
+  if (serial) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    for (int32_t i = 0; i < count; ++i) replay_one(T, O, x, y, z, zret, records[i], primal != 0, dual != 0, tol);
+    return;
   }
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) replay_one(T, O, x, y, z, zret, records[i], primal != 0, dual != 0, tol);
 }
 
 // gpu_postsolve.cu:162-180, 568-603
@@ -446,6 +467,14 @@ hipError_t gpk_replay(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, 
                       int32_t begin, int32_t end, int32_t primal, int32_t dual, double tol, hipStream_t stream) {
   if (end <= begin) return hipSuccess;
   hipLaunchKernelGGL(k_replay, dim3(1), dim3(64), 0, stream, tape, orig, x, y, z, zret, begin, end, primal, dual, tol);
+  return hipGetLastError();
+}
+hipError_t gpk_replay_list(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, double* z, uint8_t* zret,
+                           const int32_t* records, int32_t count, int32_t serial, int32_t primal, int32_t dual,
+                           double tol, hipStream_t stream) {
+  if (count <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_replay_list, dim3(serial ? 1 : grid_for(count)), dim3(serial ? 64 : kBlock), 0, stream, tape,
+                     orig, x, y, z, zret, records, count, serial, primal, dual, tol);
   return hipGetLastError();
 }
 hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipStream_t stream) {

@@ -40,6 +40,36 @@ hipError_t gpk_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, in
 /* Exclusive scan of (mask[i] != 0): out[0..n], out[n] = popcount. */
 hipError_t gpk_scan_mask_u8(const uint8_t* mask, int32_t* out, int64_t n, int32_t* scratch,
                             hipStream_t stream);
+/* ---- single-pass scans (decoupled look-back) ----------------------------
+ * One launch per scan: each 2048-item tile takes a ticket (dynamic tile id, in
+ * dispatch order), publishes its aggregate, walks back over its predecessors'
+ * published aggregates / inclusive prefixes, and writes its outputs.
+ * Status words: [63:34] epoch, [33:32] 1 = aggregate, 2 = inclusive prefix,
+ * [31:0] value; words of older epochs read as "not yet published", so the
+ * status array never needs clearing.  A spin that exceeds its limit sets
+ * *error (host-mapped memory) and the kernel finishes without waiting. */
+typedef struct gpk_lookback {
+  uint64_t* status; /* >= tiles words, used only by look-back kernels */
+  int32_t* ticket;  /* one zeroed word per launch */
+  int32_t* error;   /* host-mapped flag, set on a spin-limit exit */
+  uint32_t epoch;   /* 1 .. 2^30 - 1, unique per launch */
+} gpk_lookback;
+int64_t gpk_lookback_tiles(int64_t n);
+/* out[0..n] = exclusive prefix of in (out[n] = total); out == in allowed. */
+hipError_t gpk_scan_lb_i32(const int32_t* in, int32_t* out, int64_t n, gpk_lookback lb, hipStream_t stream);
+/* Same over (mask[i] != 0). */
+hipError_t gpk_scan_lb_mask(const uint8_t* mask, int32_t* out, int64_t n, gpk_lookback lb, hipStream_t stream);
+/* map[i] = mask[i] ? (number of set mask entries before i) : -1; *count = total. */
+hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t* count, gpk_lookback lb,
+                           hipStream_t stream);
+/* Stable compaction of a CSR by old-to-new row/column maps in one launch
+ * (256 old rows per tile): out_ptr (m_new + 1), out_idx / out_val (capacity
+ * >= kept entries) and *total = kept entries. */
+int64_t gpk_compact_tiles(int32_t m_old);
+hipError_t gpk_compact_lb(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
+                          const int32_t* row_map, const int32_t* col_map, int32_t m_new, int32_t* out_ptr,
+                          int32_t* out_idx, double* out_val, int32_t* total, gpk_lookback lb, hipStream_t stream);
+
 /* old_to_new[i] = mask[i] ? prefix[i] : -1. */
 hipError_t gpk_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* old_to_new, int32_t n,
                           hipStream_t stream);
