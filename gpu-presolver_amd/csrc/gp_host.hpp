@@ -175,6 +175,11 @@ inline gpk_plan plan_view(const presolve::PresolvePlanGpu& p) {
   return gpk_plan{p.keep_row_mask, p.keep_col_mask, p.new_c, p.new_l, p.new_u, p.new_AL, p.new_AU};
 }
 
+// Working-LP nnz from which each fast phase opens with the batched singleton
+// closure (process-wide; 10,000,000 as in the reference).
+void set_batched_closure_min_nnz(int32_t nnz);
+int32_t batched_closure_min_nnz();
+
 // ---- shared host helpers (host_common.cpp) ---------------------------------
 // Exclusive scan of a uint8 mask into out[0..n]; returns the count.
 int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out);

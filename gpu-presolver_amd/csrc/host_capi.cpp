@@ -500,6 +500,9 @@ int32_t gp_has_good_nnz_progress(int32_t nnz_before, int32_t nnz_after, double r
   return _has_good_nnz_progress(nnz_before, nnz_after, ratio) ? 1 : 0;
 }
 
+void gp_set_batched_closure_min_nnz(int32_t nnz) { amd::set_batched_closure_min_nnz(nnz); }
+int32_t gp_batched_closure_min_nnz(void) { return amd::batched_closure_min_nnz(); }
+
 void gp_presolve_free(gp_presolve_handle* h) {
   if (h == nullptr) return;
   free_gpu_presolve_reduced_lp(h->summary);

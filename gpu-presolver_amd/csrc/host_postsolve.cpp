@@ -87,17 +87,20 @@ struct OrigPattern {
   std::vector<int32_t> a_ptr, a_idx, at_ptr, at_idx;
 };
 
+// Levels [first, first + count): one grid launch (a single large level) or
+// one workgroup that steps through a run of smaller levels.
 struct Segment {
-  int32_t begin, count;
-  bool serial;
+  int32_t first, count;
+  bool grid;
 };
 
 struct Schedule {
-  std::vector<int32_t> order;
+  std::vector<int32_t> order;        // record indices, level by level
+  std::vector<int32_t> level_start;  // levels + 1 offsets into order
   std::vector<Segment> segs;
 };
 
-constexpr int32_t kParallelMin = 64;
+constexpr int32_t kGridLevelMin = 2048;
 
 [[noreturn]] void bad_tape(int32_t k, const char* what) {
   throw std::runtime_error("postsolve_gpu: tape record " + std::to_string(k) + ": " + what);
@@ -318,15 +321,14 @@ Schedule level_schedule(const TapeScanner& scan, int32_t count, int32_t n0, int3
   for (int32_t k = count - 1; k >= 0; --k)
     if (level[k] >= 0) sch.order[fill[level[k]]++] = k;  // descending k within a level
   for (int32_t l = 0; l < levels; ++l) {
-    const int32_t b = start[l], c = start[l + 1] - start[l];
-    if (c == 0) continue;
-    const bool small = c < kParallelMin;
-    if (small && !sch.segs.empty() && sch.segs.back().serial) {
-      sch.segs.back().count += c;  // consecutive small levels: one lane, level order
+    const bool big = start[l + 1] - start[l] >= kGridLevelMin;
+    if (!big && !sch.segs.empty() && !sch.segs.back().grid) {
+      ++sch.segs.back().count;
     } else {
-      sch.segs.push_back(Segment{b, c, small});
+      sch.segs.push_back(Segment{l, 1, big});
     }
   }
+  sch.level_start = std::move(start);
   return sch;
 }
 
@@ -335,12 +337,23 @@ void run_schedule(const Schedule& sch, const gp_tape_dev& tape, const gp_orig_de
   if (sch.order.empty()) return;
   auto& c = ctx();
   hipStream_t s = c.stream();
-  DBuf<int32_t> order(static_cast<int64_t>(sch.order.size()));
-  c.to_device(order.get(), sch.order.data(), static_cast<int64_t>(sch.order.size()), "replay schedule upload");
-  for (const Segment& g : sch.segs)
-    check(gpk_replay_list(tape, orig, x, y, z, zret, order.get() + g.begin, g.count, g.serial ? 1 : 0, primal ? 1 : 0,
-                          primal ? 0 : 1, tol, s),
-          primal ? "replay primal" : "replay dual");
+  const int64_t nr = static_cast<int64_t>(sch.order.size()), nl = static_cast<int64_t>(sch.level_start.size());
+  DBuf<int32_t> buf(nr + nl);
+  c.to_device(buf.get(), sch.order.data(), nr, "replay schedule upload");
+  c.to_device(buf.get() + nr, sch.level_start.data(), nl, "replay schedule upload");
+  const int32_t* order = buf.get();
+  const int32_t* starts = buf.get() + nr;
+  const int32_t p = primal ? 1 : 0;
+  const char* what = primal ? "replay primal" : "replay dual";
+  for (const Segment& g : sch.segs) {
+    if (g.grid) {
+      const int32_t b = sch.level_start[g.first];
+      check(gpk_replay_list(tape, orig, x, y, z, zret, order + b, sch.level_start[g.first + 1] - b, p, 1 - p, tol, s),
+            what);
+    } else {
+      check(gpk_replay_levels(tape, orig, x, y, z, zret, order, starts + g.first, g.count, p, 1 - p, tol, s), what);
+    }
+  }
 }
 
 void upload_map(DBuf<int32_t>& dst, const std::vector<int32_t>& src, int32_t bound, const char* what) {

@@ -10,6 +10,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <utility>
@@ -30,6 +31,15 @@
 #include "gpu_presolver/presolve/rules/rule_infeasible_redundant_rows.hpp"
 #include "gpu_presolver/presolve/rules/rule_redundant_bounds.hpp"
 #include "gpu_presolver/presolve/rules/rule_singleton_rows.hpp"
+
+namespace gpu_presolver::amd {
+namespace {
+// kMinNnzForBatchedClosure (gpu_presolve.cu:2506).
+std::atomic<int32_t> g_closure_min_nnz{10000000};
+}  // namespace
+void set_batched_closure_min_nnz(int32_t nnz) { g_closure_min_nnz.store(nnz); }
+int32_t batched_closure_min_nnz() { return g_closure_min_nnz.load(); }
+}  // namespace gpu_presolver::amd
 
 namespace gpu_presolver::presolve {
 
@@ -548,11 +558,85 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     }
     return any;
   };
-  // run_fast_phase (gpu_presolve.cu:2833-2863).  The batched singleton
-  // closure the reference switches to at nnz >= 1e7 (:2507-2512) is not
-  // part of this driver.
+  // run_batched_singleton_cleanup_closure (gpu_presolve.cu:2503-2831):
+  // column-singleton EQ, dual fix over incremental candidates and singleton
+  // rows, repeated on one plan whose live degrees and singleton statistics
+  // are refreshed from the masks on the device; one compaction at the end.
+  auto singleton_closure = [&]() -> bool {
+    if (terminal || time_exceeded() || cur.lp.A.nnz < amd::batched_closure_min_nnz() ||
+        !prm.enable_column_singletons_eq || !prm.enable_dual_fix || !prm.enable_singleton_rows)
+      return false;
+    const LPInfoGpu& v = cur.lp;
+    const int32_t m = v.A.rows, n = v.A.cols;
+    hipStream_t s = ctx().stream();
+    PhasePlan pp(v);
+    PhaseStats st(v);
+    PresolvePlanGpu& p = pp.p;
+    const gpk_plan pv = amd::plan_view(p);
+    const gpk_csr av = amd::csr_view(v.A), atv = amd::csr_view(v.AT);
+    DBuf<uint8_t> row_before(m), col_before(n), cand(n), row_seen(m);
+    DBuf<double> c_seen(n), l_seen(n), u_seen(n), AL_seen(m), AU_seen(m);
+    if (m > 0) check(hipMemcpyAsync(row_before, p.keep_row_mask, m, hipMemcpyDeviceToDevice, s), "closure masks");
+    if (n > 0) check(hipMemcpyAsync(col_before, p.keep_col_mask, n, hipMemcpyDeviceToDevice, s), "closure masks");
+    const gpk_plan seen{row_seen, nullptr, c_seen, l_seen, u_seen, AL_seen, AU_seen};
+    auto snapshot = [&]() { check(gpk_closure_snapshot(m, n, pv, seen, s), "closure snapshot"); };
+    auto refresh = [&]() {
+      check(gpk_closure_degrees(av, atv, p.keep_row_mask, p.keep_col_mask, row_before, col_before, st.s.row_nnz,
+                                st.s.col_nnz, s),
+            "closure degree refresh");
+      check(gpk_closure_singletons(av, atv, p.keep_row_mask, p.keep_col_mask, st.s.row_nnz, st.s.col_nnz,
+                                   st.s.singleton_row_mask, st.s.singleton_row_col, st.s.singleton_row_val,
+                                   st.s.column_singleton_mask, st.s.column_singleton_row,
+                                   st.s.column_singleton_val, s),
+            "closure singleton refresh");
+    };
+    snapshot();
+    bool any = false, first_dual = true;
+    while (!terminal && !time_exceeded()) {
+      p.has_change = false;
+      apply_rule_column_singletons_eq(p, v, st.s, prm, true);
+      const bool ch_eq = p.has_change;
+      terminal = p.has_infeasible || p.has_unbounded;
+      bool ch_dual = false;
+      if (!terminal && !time_exceeded()) {
+        if (first_dual) {
+          if (n > 0) check(hipMemsetAsync(cand, 1, n, s), "closure first dual sweep");
+          first_dual = false;
+        } else {
+          check(gpk_closure_candidates(av, pv, seen, cand, s), "closure dual candidates");
+        }
+        snapshot();
+        p.has_change = false;
+        apply_rule_dual_fix(p, v, st.s, prm, cand);
+        ch_dual = p.has_change;
+        terminal = p.has_infeasible || p.has_unbounded;
+      }
+      bool ch_rows = false;
+      if (!terminal && !time_exceeded()) {
+        refresh();
+        p.has_change = false;
+        apply_rule_singleton_rows(p, v, st.s, prm);
+        ch_rows = p.has_change;
+        terminal = p.has_infeasible || p.has_unbounded;
+        if (!terminal && ch_rows) refresh();
+      }
+      const bool ch = ch_eq || ch_dual || ch_rows;
+      any = any || ch;
+      if (!ch) break;
+    }
+    res.has_infeasible = p.has_infeasible;
+    res.has_unbounded = p.has_unbounded;
+    p.has_change = any;
+    if (any && !terminal) {
+      apply_plan(cur, p, rs);
+      invalidate_clean();
+    }
+    return any;
+  };
+  // run_fast_phase (gpu_presolve.cu:2833-2863)
   auto fast_phase = [&]() {
-    bool any = false, singleton = false;
+    bool any = singleton_closure();
+    bool singleton = false;
     while (!terminal && !time_exceeded()) {
       const bool ch_dual = cs_dual_clean ? false : col_phase(false, false, true, false, false, false, false, false);
       const bool ch_eq = cs_eq_clean ? false : col_phase(false, false, false, true, false, false, false, false);

@@ -102,6 +102,7 @@ __global__ void k_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t
 // (gpu_presolve_kernels.cu:18-66) for any row length.
 constexpr int kActRows = kBlock;
 constexpr int kActCap = 2048;
+constexpr int kActItems = kActCap / kBlock;
 
 __global__ __launch_bounds__(kBlock) void k_row_activity(const int32_t* __restrict__ row_ptr,
                                                          const int32_t* __restrict__ col_idx,
@@ -127,14 +128,30 @@ __global__ __launch_bounds__(kBlock) void k_row_activity(const int32_t* __restri
   int32_t ni = 0, pi = 0;
   for (int32_t w = seg_b; w < seg_e; w += kActCap) {
     const int32_t we = min(w + kActCap, seg_e);
-    for (int32_t p = w + tid; p < we; p += kBlock) {
-      const double a = val[p];
+    // All loads of the window first (coalesced val/col_idx, then the l/u
+    // gathers), so each lane keeps kActItems * 2 gathers in flight.
+    double a[kActItems], lj[kActItems], uj[kActItems];
+    int32_t j[kActItems];
+#pragma unroll
+    for (int k = 0; k < kActItems; ++k) {
+      const int32_t p = w + tid + k * kBlock;
+      a[k] = p < we ? val[p] : 0.0;
+      j[k] = p < we ? col_idx[p] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kActItems; ++k) {
+      const bool use = fabs(a[k]) > ztol;
+      lj[k] = use ? l[j[k]] : 0.0;
+      uj[k] = use ? u[j[k]] : 0.0;
+    }
+#pragma unroll
+    for (int k = 0; k < kActItems; ++k) {
+      const int32_t p = w + tid + k * kBlock;
+      if (p >= we) break;
       double tmn = 0.0, tmx = 0.0;
-      if (fabs(a) > ztol) {
-        const int32_t j = col_idx[p];
-        const double lj = l[j], uj = u[j];
-        tmn = a >= 0.0 ? a * lj : a * uj;
-        tmx = a >= 0.0 ? a * uj : a * lj;
+      if (fabs(a[k]) > ztol) {
+        tmn = a[k] >= 0.0 ? a[k] * lj[k] : a[k] * uj[k];
+        tmx = a[k] >= 0.0 ? a[k] * uj[k] : a[k] * lj[k];
       }
       s_tmin[p - w] = tmn;
       s_tmax[p - w] = tmx;
@@ -394,6 +411,30 @@ __global__ __launch_bounds__(kBlock) void k_scan_lb(const In* in, int32_t* out, 
 constexpr int kCompRows = 1024;
 constexpr int kCompRowsPerThread = kCompRows / kBlock;
 
+// Maps the thread's kCompItems consecutive entries p0.. of the tile segment:
+// rr[k] = tile-local row (-1 past the segment), jn[k] = new column index or
+// -1 when the row or the column is dropped.  The column indices are loaded
+// first and the map gathers issued together, so they overlap.
+__device__ inline void comp_map_items(const int32_t* s_ptr, const uint8_t* s_keep, int rows, int32_t seg_e,
+                                      int32_t p0, const int32_t* __restrict__ idx,
+                                      const int32_t* __restrict__ col_map, int32_t* jn, int* rr) {
+  int32_t cj[kCompItems];
+#pragma unroll
+  for (int k = 0; k < kCompItems; ++k) cj[k] = p0 + k < seg_e ? idx[p0 + k] : 0;
+  int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
+#pragma unroll
+  for (int k = 0; k < kCompItems; ++k) {
+    const int32_t p = p0 + k;
+    rr[k] = -1;
+    if (p < seg_e) {
+      while (s_ptr[row + 1] <= p) ++row;
+      rr[k] = row;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kCompItems; ++k) jn[k] = (rr[k] >= 0 && s_keep[rr[k]]) ? col_map[cj[k]] : -1;
+}
+
 __global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict__ ptr,
                                                        const int32_t* __restrict__ idx,
                                                        const double* __restrict__ val, int32_t m_old,
@@ -425,19 +466,21 @@ __global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict
   const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
   // pass 1: per-row kept counts
   for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
-    const int32_t p0 = c + tid * kCompItems;
-    int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
-    int local = 0;
+    int32_t jn[kCompItems];
+    int rr[kCompItems];
+    comp_map_items(s_ptr, s_keep, rows, seg_e, c + tid * kCompItems, idx, col_map, jn, rr);
+    int local = 0, cur = rr[0];
+#pragma unroll
     for (int k = 0; k < kCompItems; ++k) {
-      const int32_t p = p0 + k;
-      if (p >= seg_e) break;
-      while (s_ptr[row + 1] <= p) {
-        if (local) { atomicAdd(&s_cnt[row], local); local = 0; }
-        ++row;
+      if (rr[k] < 0) break;
+      if (rr[k] != cur) {
+        if (local) atomicAdd(&s_cnt[cur], local);
+        local = 0;
+        cur = rr[k];
       }
-      if (s_keep[row] && col_map[idx[p]] >= 0) ++local;
+      local += jn[k] >= 0 ? 1 : 0;
     }
-    if (local) atomicAdd(&s_cnt[row], local);
+    if (local) atomicAdd(&s_cnt[cur], local);
   }
   __syncthreads();
   // row offsets: thread t owns rows 4t .. 4t+3
@@ -473,17 +516,14 @@ __global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict
   for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
     const int32_t p0 = c + tid * kCompItems;
     int32_t jn[kCompItems];
+    int rr[kCompItems];
+    comp_map_items(s_ptr, s_keep, rows, seg_e, p0, idx, col_map, jn, rr);
+    double v[kCompItems];
     int32_t cnt = 0;
-    int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
 #pragma unroll
     for (int k = 0; k < kCompItems; ++k) {
-      const int32_t p = p0 + k;
-      jn[k] = -1;
-      if (p < seg_e) {
-        while (s_ptr[row + 1] <= p) ++row;
-        if (s_keep[row]) jn[k] = col_map[idx[p]];
-        cnt += jn[k] >= 0 ? 1 : 0;
-      }
+      v[k] = jn[k] >= 0 ? val[p0 + k] : 0.0;
+      cnt += jn[k] >= 0 ? 1 : 0;
     }
     int32_t chunk_total;
     int32_t pos = block_exclusive_sum(cnt, &chunk_total);
@@ -491,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict
     for (int k = 0; k < kCompItems; ++k) {
       if (jn[k] >= 0) {
         s_oidx[pos] = jn[k];
-        s_oval[pos] = val[p0 + k];
+        s_oval[pos] = v[k];
         ++pos;
       }
     }

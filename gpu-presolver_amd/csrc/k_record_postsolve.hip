@@ -380,19 +380,29 @@ __global__ void k_replay(gp_tape_dev T, gp_orig_dev O, double* x, double* y, dou
   for (int32_t k = end - 1; k >= begin; --k) replay_one(T, O, x, y, z, zret, k, primal != 0, dual != 0, tol);
 }
 
-// One segment of a dependency-level schedule: either the records of one
-// level, mutually independent, one per thread; or (serial != 0) a run of
-// small levels replayed in list order by a single lane.
+// One level of a dependency-level schedule: its records touch disjoint
+// variables and are replayed one per thread.
 __global__ void k_replay_list(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
-                              const int32_t* records, int32_t count, int32_t serial, int32_t primal, int32_t dual,
-                              double tol) {
-  if (serial) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int32_t i = 0; i < count; ++i) replay_one(T, O, x, y, z, zret, records[i], primal != 0, dual != 0, tol);
-    return;
-  }
+                              const int32_t* records, int32_t count, int32_t primal, int32_t dual, double tol) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) replay_one(T, O, x, y, z, zret, records[i], primal != 0, dual != 0, tol);
+}
+
+// A run of small levels in one workgroup: the threads share each level's
+// records and meet at a barrier before the next level.  All waves of the
+// workgroup run on one CU, so the barrier's workgroup-scope fence makes one
+// level's global stores visible to the next level's loads.
+constexpr int kLevelBlock = 512;
+__global__ void __launch_bounds__(kLevelBlock)
+    k_replay_levels(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
+                    const int32_t* records, const int32_t* level_start, int32_t nlevels, int32_t primal,
+                    int32_t dual, double tol) {
+  for (int32_t lv = 0; lv < nlevels; ++lv) {
+    const int32_t e = level_start[lv + 1];
+    for (int32_t i = level_start[lv] + static_cast<int32_t>(threadIdx.x); i < e; i += kLevelBlock)
+      replay_one(T, O, x, y, z, zret, records[i], primal != 0, dual != 0, tol);
+    __syncthreads();
+  }
 }
 
 // gpu_postsolve.cu:162-180, 568-603
@@ -470,11 +480,19 @@ hipError_t gpk_replay(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, 
   return hipGetLastError();
 }
 hipError_t gpk_replay_list(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, double* z, uint8_t* zret,
-                           const int32_t* records, int32_t count, int32_t serial, int32_t primal, int32_t dual,
-                           double tol, hipStream_t stream) {
+                           const int32_t* records, int32_t count, int32_t primal, int32_t dual, double tol,
+                           hipStream_t stream) {
   if (count <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_replay_list, dim3(serial ? 1 : grid_for(count)), dim3(serial ? 64 : kBlock), 0, stream, tape,
-                     orig, x, y, z, zret, records, count, serial, primal, dual, tol);
+  hipLaunchKernelGGL(k_replay_list, dim3(grid_for(count)), dim3(kBlock), 0, stream, tape, orig, x, y, z, zret, records,
+                     count, primal, dual, tol);
+  return hipGetLastError();
+}
+hipError_t gpk_replay_levels(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, double* z, uint8_t* zret,
+                             const int32_t* records, const int32_t* level_start, int32_t nlevels, int32_t primal,
+                             int32_t dual, double tol, hipStream_t stream) {
+  if (nlevels <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_replay_levels, dim3(1), dim3(kLevelBlock), 0, stream, tape, orig, x, y, z, zret, records,
+                     level_start, nlevels, primal, dual, tol);
   return hipGetLastError();
 }
 hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipStream_t stream) {

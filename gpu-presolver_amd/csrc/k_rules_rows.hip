@@ -570,6 +570,113 @@ __global__ void k_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_d
   }
 }
 
+// ---- batched singleton closure (gpu_presolve.cu:98-247) ---------------------
+// One launch over max(m, n) threads: thread t takes column t and row t.
+// Live degrees lose the entries of columns (rows) removed since the previous
+// refresh; integer atomics, so the result does not depend on their order.
+// Each thread then copies its own entry of the current masks into the
+// "before" masks (no other thread reads that entry).
+__global__ void k_closure_degrees(gpk_csr A, gpk_csr AT, const uint8_t* keep_row, const uint8_t* keep_col,
+                                  uint8_t* row_before, uint8_t* col_before, int32_t* row_nnz, int32_t* col_nnz) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < A.cols) {
+    const uint8_t kc = keep_col[t];
+    if (col_before[t] != 0 && kc == 0) {
+      for (int32_t q = AT.ptr[t]; q < AT.ptr[t + 1]; ++q) {
+        const int32_t row = AT.idx[q];
+        if (keep_row[row] != 0) atomicSub(&row_nnz[row], 1);
+      }
+    }
+    col_before[t] = kc;
+  }
+  if (t < A.rows) {
+    const uint8_t kr = keep_row[t];
+    if (row_before[t] != 0 && kr == 0) {
+      for (int32_t q = A.ptr[t]; q < A.ptr[t + 1]; ++q) {
+        const int32_t col = A.idx[q];
+        if (keep_col[col] != 0) atomicSub(&col_nnz[col], 1);
+      }
+    }
+    row_before[t] = kr;
+  }
+}
+
+// Singleton rows and columns from the live degrees; the singleton entry is
+// the first live one of the row (column).
+__global__ void k_closure_singletons(gpk_csr A, gpk_csr AT, const uint8_t* keep_row, const uint8_t* keep_col,
+                                     const int32_t* row_nnz, const int32_t* col_nnz, uint8_t* srow_mask,
+                                     int32_t* srow_col, double* srow_val, uint8_t* scol_mask, int32_t* scol_row,
+                                     double* scol_val) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < A.rows) {
+    const bool single = keep_row[t] != 0 && row_nnz[t] == 1;
+    int32_t col = -1;
+    double val = 0.0;
+    if (single) {
+      for (int32_t q = A.ptr[t]; q < A.ptr[t + 1]; ++q) {
+        if (keep_col[A.idx[q]] == 0) continue;
+        col = A.idx[q];
+        val = A.val[q];
+        break;
+      }
+    }
+    srow_mask[t] = single ? 1 : 0;
+    srow_col[t] = col;
+    srow_val[t] = val;
+  }
+  if (t < A.cols) {
+    const bool single = keep_col[t] != 0 && col_nnz[t] == 1;
+    int32_t row = -1;
+    double val = 0.0;
+    if (single) {
+      for (int32_t q = AT.ptr[t]; q < AT.ptr[t + 1]; ++q) {
+        if (keep_row[AT.idx[q]] == 0) continue;
+        row = AT.idx[q];
+        val = AT.val[q];
+        break;
+      }
+    }
+    scol_mask[t] = single ? 1 : 0;
+    scol_row[t] = row;
+    scol_val[t] = val;
+  }
+}
+
+// Snapshot of the plan state the dual-fix candidates are measured against.
+__global__ void k_closure_snapshot(int32_t m, int32_t n, gpk_plan plan, gpk_plan seen) {
+  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    seen.c[t] = plan.c[t];
+    seen.l[t] = plan.l[t];
+    seen.u[t] = plan.u[t];
+  }
+  if (t < m) {
+    seen.keep_row[t] = plan.keep_row[t];
+    seen.AL[t] = plan.AL[t];
+    seen.AU[t] = plan.AU[t];
+  }
+}
+
+// Live columns whose c, l or u changed since the snapshot.
+__global__ void k_closure_mark_cols(int32_t n, gpk_plan plan, gpk_plan seen, uint8_t* cand) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  cand[j] = plan.keep_col[j] != 0 && (plan.c[j] != seen.c[j] || plan.l[j] != seen.l[j] || plan.u[j] != seen.u[j])
+                ? 1
+                : 0;
+}
+
+// Live columns of rows removed, or with AL/AU changed, since the snapshot.
+__global__ void k_closure_mark_rows(gpk_csr A, gpk_plan plan, gpk_plan seen, uint8_t* cand) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.rows) return;
+  if (plan.keep_row[i] == seen.keep_row[i] && plan.AL[i] == seen.AL[i] && plan.AU[i] == seen.AU[i]) return;
+  for (int32_t q = A.ptr[i]; q < A.ptr[i + 1]; ++q) {
+    const int32_t col = A.idx[q];
+    if (plan.keep_col[col] != 0) cand[col] = 1;
+  }
+}
+
 }  // namespace
 
 #define GPK_LAUNCH(kernel, n, ...)                                                      \
@@ -682,6 +789,30 @@ hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_h
 hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count,
                            hipStream_t stream) {
   GPK_LAUNCH(k_delete_rows, m, m, keep_row, row_delete, count);
+}
+
+hipError_t gpk_closure_degrees(gpk_csr A, gpk_csr AT, const uint8_t* keep_row, const uint8_t* keep_col,
+                               uint8_t* row_before, uint8_t* col_before, int32_t* row_nnz, int32_t* col_nnz,
+                               hipStream_t stream) {
+  GPK_LAUNCH(k_closure_degrees, A.rows > A.cols ? A.rows : A.cols, A, AT, keep_row, keep_col, row_before, col_before,
+             row_nnz, col_nnz);
+}
+hipError_t gpk_closure_singletons(gpk_csr A, gpk_csr AT, const uint8_t* keep_row, const uint8_t* keep_col,
+                                  const int32_t* row_nnz, const int32_t* col_nnz, uint8_t* srow_mask,
+                                  int32_t* srow_col, double* srow_val, uint8_t* scol_mask, int32_t* scol_row,
+                                  double* scol_val, hipStream_t stream) {
+  GPK_LAUNCH(k_closure_singletons, A.rows > A.cols ? A.rows : A.cols, A, AT, keep_row, keep_col, row_nnz, col_nnz,
+             srow_mask, srow_col, srow_val, scol_mask, scol_row, scol_val);
+}
+hipError_t gpk_closure_snapshot(int32_t m, int32_t n, gpk_plan plan, gpk_plan seen, hipStream_t stream) {
+  GPK_LAUNCH(k_closure_snapshot, m > n ? m : n, m, n, plan, seen);
+}
+hipError_t gpk_closure_candidates(gpk_csr A, gpk_plan plan, gpk_plan seen, uint8_t* cand, hipStream_t stream) {
+  if (A.cols > 0) hipLaunchKernelGGL(k_closure_mark_cols, dim3(grid_for(A.cols)), dim3(kBlock), 0, stream, A.cols,
+                                     plan, seen, cand);
+  if (A.rows > 0) hipLaunchKernelGGL(k_closure_mark_rows, dim3(grid_for(A.rows)), dim3(kBlock), 0, stream, A, plan,
+                                     seen, cand);
+  return hipGetLastError();
 }
 
 }  // extern "C"

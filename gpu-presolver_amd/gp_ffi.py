@@ -91,6 +91,10 @@ def lib():
         L.gp_postsolve.argtypes = [C.POINTER(T.GpRecordView), vp, vp, vp, C.POINTER(T.GpLp), vp, vp, vp]
         L.gp_has_good_nnz_progress.argtypes = [C.c_int32, C.c_int32, C.c_double]
         L.gp_has_good_nnz_progress.restype = C.c_int32
+        L.gp_set_batched_closure_min_nnz.argtypes = [C.c_int32]
+        L.gp_set_batched_closure_min_nnz.restype = None
+        L.gp_batched_closure_min_nnz.argtypes = []
+        L.gp_batched_closure_min_nnz.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -108,6 +112,16 @@ def params(**kw):
 
 def has_good_nnz_progress(before, after, ratio):
     return bool(lib().gp_has_good_nnz_progress(before, after, ratio))
+
+
+def set_batched_closure_min_nnz(nnz):
+    """Working-LP nnz from which the fast phase opens with the batched
+    singleton closure (10,000,000 by default, as in the reference)."""
+    lib().gp_set_batched_closure_min_nnz(int(nnz))
+
+
+def batched_closure_min_nnz():
+    return int(lib().gp_batched_closure_min_nnz())
 
 
 # ---- device memory ---------------------------------------------------------

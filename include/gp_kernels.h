@@ -159,6 +159,25 @@ typedef struct gpk_plan {
 hipError_t gpk_plan_init(int32_t m, int32_t n, gpk_plan plan, const double* c, const double* l, const double* u,
                          const double* AL, const double* AU, hipStream_t stream);
 
+/* Batched singleton closure (reference gpu_presolve.cu:98-247, 2506-2831).
+ * gpk_closure_degrees: row_nnz (col_nnz) lose the live entries of columns
+ *   (rows) that were kept in col_before (row_before) and are removed now;
+ *   the before masks are then set to the current ones.
+ * gpk_closure_singletons: singleton rows / columns from the live degrees,
+ *   with the first live entry as the singleton.
+ * gpk_closure_snapshot: seen.{keep_row, c, l, u, AL, AU} := plan's.
+ * gpk_closure_candidates: cand[j] = live column j whose c/l/u differ from
+ *   seen, or that lies in a row removed or re-bounded since seen. */
+hipError_t gpk_closure_degrees(gpk_csr A, gpk_csr AT, const uint8_t* keep_row, const uint8_t* keep_col,
+                               uint8_t* row_before, uint8_t* col_before, int32_t* row_nnz, int32_t* col_nnz,
+                               hipStream_t stream);
+hipError_t gpk_closure_singletons(gpk_csr A, gpk_csr AT, const uint8_t* keep_row, const uint8_t* keep_col,
+                                  const int32_t* row_nnz, const int32_t* col_nnz, uint8_t* srow_mask,
+                                  int32_t* srow_col, double* srow_val, uint8_t* scol_mask, int32_t* scol_row,
+                                  double* scol_val, hipStream_t stream);
+hipError_t gpk_closure_snapshot(int32_t m, int32_t n, gpk_plan plan, gpk_plan seen, hipStream_t stream);
+hipError_t gpk_closure_candidates(gpk_csr A, gpk_plan plan, gpk_plan seen, uint8_t* cand, hipStream_t stream);
+
 /* Generic row shift: for each row i (all rows, or only keep_row[i] != 0 when
  * keep_row is given) shift = sum over entries with col_flag[j] of a*col_val[j]
  * in CSR order, then AL[i] -= shift, AU[i] -= shift for every row.  `abs_min`

@@ -248,6 +248,14 @@ void gp_presolve_free(gp_presolve_handle* h);
  * nnz_after < ratio * nnz_before (and nnz_before > 0), else 0.  Host only. */
 int32_t gp_has_good_nnz_progress(int32_t nnz_before, int32_t nnz_after, double ratio);
 
+/* Working-LP nnz from which each fast phase opens with the batched singleton
+ * closure (column-singleton EQ, dual fix and singleton rows repeated on one
+ * plan, then one compaction).  The reference fixes it at 10,000,000
+ * (kMinNnzForBatchedClosure, gpu_presolve.cu:2506), which is the default;
+ * lowering it exercises the closure on small LPs.  Process-wide, host only. */
+void gp_set_batched_closure_min_nnz(int32_t nnz);
+int32_t gp_batched_closure_min_nnz(void);
+
 /* ---- postsolve (reference gpu_postsolve.hpp:16-27) --------------------- */
 /* postsolve_gpu with caller-provided outputs: x_org, z_org hold n0 doubles,
  * y_org m0 doubles (device).  x_red/y_red/z_red may be null.  `original`

@@ -59,12 +59,16 @@ hipError_t gpk_mark(const int32_t* red2org, int32_t n_red, uint8_t* mask, hipStr
 hipError_t gpk_restore_fixed(const int32_t* idx, const double* val, int32_t count, double* x, hipStream_t stream);
 hipError_t gpk_replay(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, double* z, uint8_t* zret,
                       int32_t begin, int32_t end, int32_t primal, int32_t dual, double tol, hipStream_t stream);
-/* One segment of a dependency-level replay schedule: records[0..count) are
- * replayed one per thread (serial = 0: they touch disjoint variables) or in
- * list order by one lane (serial = 1). */
+/* Dependency-level replay (the reverse tape order split into levels whose
+ * records touch disjoint variables).  gpk_replay_list: the records of one
+ * level, one per thread.  gpk_replay_levels: nlevels consecutive levels in one
+ * workgroup, level lv being records[level_start[lv] .. level_start[lv+1]). */
 hipError_t gpk_replay_list(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, double* z, uint8_t* zret,
-                           const int32_t* records, int32_t count, int32_t serial, int32_t primal, int32_t dual,
-                           double tol, hipStream_t stream);
+                           const int32_t* records, int32_t count, int32_t primal, int32_t dual, double tol,
+                           hipStream_t stream);
+hipError_t gpk_replay_levels(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, double* z, uint8_t* zret,
+                             const int32_t* records, const int32_t* level_start, int32_t nlevels, int32_t primal,
+                             int32_t dual, double tol, hipStream_t stream);
 hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipStream_t stream);
 hipError_t gpk_project_duals(gp_orig_dev orig, const uint8_t* prot, const double* x, const double* y, double* z,
                              int32_t n, double tol, hipStream_t stream);

@@ -183,6 +183,10 @@ struct Result {
 
 Result presolve(const LpV& lp, const Params& params, bool keep_reduced);
 bool has_good_nnz_progress(int32_t before, int32_t after, double ratio);
+// Working-LP nnz from which the fast phase starts with the batched singleton
+// closure (gpu_presolve.cu:2506: 10,000,000).
+void set_batched_closure_min_nnz(int32_t nnz);
+int32_t batched_closure_min_nnz();
 
 struct RecordV {
   int32_t m0, n0, m1, n1;

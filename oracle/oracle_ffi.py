@@ -45,6 +45,10 @@ def lib():
         L.orc_free.argtypes = [vp]
         L.orc_has_good_nnz_progress.argtypes = [C.c_int32, C.c_int32, C.c_double]
         L.orc_has_good_nnz_progress.restype = C.c_int32
+        L.orc_set_batched_closure_min_nnz.argtypes = [C.c_int32]
+        L.orc_set_batched_closure_min_nnz.restype = None
+        L.orc_batched_closure_min_nnz.argtypes = []
+        L.orc_batched_closure_min_nnz.restype = C.c_int32
         _lib = L
     return _lib
 
@@ -62,6 +66,16 @@ def params(**kw):
 
 def has_good_nnz_progress(before, after, ratio):
     return bool(lib().orc_has_good_nnz_progress(before, after, ratio))
+
+
+def set_batched_closure_min_nnz(nnz):
+    """Working-LP nnz from which the fast phase opens with the batched
+    singleton closure (10,000,000 by default, as in the reference)."""
+    lib().orc_set_batched_closure_min_nnz(int(nnz))
+
+
+def batched_closure_min_nnz():
+    return int(lib().orc_batched_closure_min_nnz())
 
 
 def _ptr(a):

@@ -254,6 +254,15 @@ void apply_plan(Working& cur, const Plan& p, Record& rec) {
 
 }  // namespace
 
+namespace {
+// kMinNnzForBatchedClosure (gpu_presolve.cu:2506); adjustable so the closure
+// can be exercised on small LPs.
+int32_t g_closure_min_nnz = 10000000;
+}  // namespace
+
+void set_batched_closure_min_nnz(int32_t nnz) { g_closure_min_nnz = nnz; }
+int32_t batched_closure_min_nnz() { return g_closure_min_nnz; }
+
 bool has_good_nnz_progress(int32_t before, int32_t after, double ratio) {
   if (before <= 0) return false;
   return static_cast<double>(after) < ratio * static_cast<double>(before);
@@ -393,10 +402,128 @@ Result presolve(const LpV& lp, const Params& prm, bool keep_reduced) {
     }
     return any;
   };
-  // run_fast_phase (:2833-2863); the batched singleton closure runs only for
-  // nnz >= 1e7 (:2507-2512) and is not restated here.
+  // run_batched_singleton_cleanup_closure (:2503-2831): column-singleton EQ,
+  // dual fix over incremental candidates and singleton rows, repeated on one
+  // plan whose degree statistics are refreshed from the masks; a single
+  // compaction at the end.
+  auto singleton_closure = [&]() -> bool {
+    if (terminal || time_exceeded() || cur.lp.A.nnz() < g_closure_min_nnz || !prm.enable_column_singletons_eq ||
+        !prm.enable_dual_fix || !prm.enable_singleton_rows)
+      return false;
+    OwnedPlan op(cur.lp);
+    OwnedStats os(cur.lp);
+    const LpV v = cur.lp.view();
+    Plan& p = op.p;
+    const int32_t m = cur.lp.A.rows, n = cur.lp.A.cols;
+    std::vector<uint8_t> row_before(op.keep_row), col_before(op.keep_col);
+    std::vector<uint8_t> cand(n, 0), row_seen;
+    std::vector<double> c_seen, l_seen, u_seen, AL_seen, AU_seen;
+    auto snapshot = [&]() {
+      row_seen = op.keep_row;
+      c_seen = op.c;
+      l_seen = op.l;
+      u_seen = op.u;
+      AL_seen = op.AL;
+      AU_seen = op.AU;
+    };
+    snapshot();
+    // _kernel_mark_changed_dual_cols, _kernel_mark_dual_cols_from_changed_rows (:200-247)
+    auto mark_candidates = [&]() {
+      for (int32_t j = 0; j < n; ++j)
+        cand[j] = op.keep_col[j] && (op.c[j] != c_seen[j] || op.l[j] != l_seen[j] || op.u[j] != u_seen[j]);
+      for (int32_t i = 0; i < m; ++i) {
+        if (op.keep_row[i] == row_seen[i] && op.AL[i] == AL_seen[i] && op.AU[i] == AU_seen[i]) continue;
+        for (int32_t q = v.A.ptr[i]; q < v.A.ptr[i + 1]; ++q)
+          if (op.keep_col[v.A.idx[q]]) cand[v.A.idx[q]] = 1;
+      }
+    };
+    // _kernel_update_{row,col}_degrees_from_removed_*, _kernel_refresh_* (:98-198)
+    auto refresh = [&]() {
+      for (int32_t j = 0; j < n; ++j) {
+        if (!col_before[j] || op.keep_col[j]) continue;
+        for (int32_t q = v.AT.ptr[j]; q < v.AT.ptr[j + 1]; ++q)
+          if (op.keep_row[v.AT.idx[q]]) --os.row_nnz[v.AT.idx[q]];
+      }
+      for (int32_t i = 0; i < m; ++i) {
+        if (!row_before[i] || op.keep_row[i]) continue;
+        for (int32_t q = v.A.ptr[i]; q < v.A.ptr[i + 1]; ++q)
+          if (op.keep_col[v.A.idx[q]]) --os.col_nnz[v.A.idx[q]];
+      }
+      row_before = op.keep_row;
+      col_before = op.keep_col;
+      for (int32_t i = 0; i < m; ++i) {
+        const bool single = op.keep_row[i] && os.row_nnz[i] == 1;
+        os.srow_mask[i] = single;
+        os.srow_col[i] = -1;
+        os.srow_val[i] = 0.0;
+        if (!single) continue;
+        for (int32_t q = v.A.ptr[i]; q < v.A.ptr[i + 1]; ++q) {
+          if (!op.keep_col[v.A.idx[q]]) continue;
+          os.srow_col[i] = v.A.idx[q];
+          os.srow_val[i] = v.A.val[q];
+          break;
+        }
+      }
+      for (int32_t j = 0; j < n; ++j) {
+        const bool single = op.keep_col[j] && os.col_nnz[j] == 1;
+        os.scol_mask[j] = single;
+        os.scol_row[j] = -1;
+        os.scol_val[j] = 0.0;
+        if (!single) continue;
+        for (int32_t q = v.AT.ptr[j]; q < v.AT.ptr[j + 1]; ++q) {
+          if (!op.keep_row[v.AT.idx[q]]) continue;
+          os.scol_row[j] = v.AT.idx[q];
+          os.scol_val[j] = v.AT.val[q];
+          break;
+        }
+      }
+    };
+    bool any = false, first_dual = true;
+    while (!terminal && !time_exceeded()) {
+      p.has_change = false;
+      rule_column_singletons(p, v, os.s, prm, true, true);
+      const bool ch_eq = p.has_change;
+      terminal = p.has_infeasible || p.has_unbounded;
+      bool ch_dual = false;
+      if (!terminal && !time_exceeded()) {
+        if (first_dual) {
+          std::fill(cand.begin(), cand.end(), uint8_t{1});
+          first_dual = false;
+        } else {
+          mark_candidates();
+        }
+        snapshot();
+        p.has_change = false;
+        rule_dual_fix(p, v, os.s, prm, cand.data());
+        ch_dual = p.has_change;
+        terminal = p.has_infeasible || p.has_unbounded;
+      }
+      bool ch_rows = false;
+      if (!terminal && !time_exceeded()) {
+        refresh();
+        p.has_change = false;
+        rule_singleton_rows(p, v, os.s, prm);
+        ch_rows = p.has_change;
+        terminal = p.has_infeasible || p.has_unbounded;
+        if (!terminal && ch_rows) refresh();
+      }
+      const bool ch = ch_eq || ch_dual || ch_rows;
+      any = any || ch;
+      if (!ch) break;
+    }
+    res.has_infeasible = p.has_infeasible;
+    res.has_unbounded = p.has_unbounded;
+    p.has_change = any;
+    if (any && !terminal) {
+      apply_plan(cur, p, rec);
+      invalidate_clean();
+    }
+    return any;
+  };
+  // run_fast_phase (:2833-2863)
   auto fast_phase_fn = [&]() {
-    bool any = false, singleton = false;
+    bool any = singleton_closure();
+    bool singleton = false;
     while (!terminal && !time_exceeded()) {
       const bool ch_dual = cs_dual_clean ? false : col_phase(false, false, true, false, false, false, false, false);
       const bool ch_eq = cs_eq_clean ? false : col_phase(false, false, false, true, false, false, false, false);

@@ -100,3 +100,15 @@ def test_library_exports_cxx_api(built_lib):
                          text=True).stdout
     missing = [sig for sig in CXX_API if sig not in out]
     assert not missing, f"C++ API symbols not exported: {missing}"
+
+
+def test_batched_closure_threshold_default(built_lib):
+    """kMinNnzForBatchedClosure (reference gpu_presolve.cu:2506); settable
+    without a GPU."""
+    lib = C.CDLL(built_lib)
+    lib.gp_batched_closure_min_nnz.restype = C.c_int32
+    lib.gp_set_batched_closure_min_nnz.argtypes = [C.c_int32]
+    assert lib.gp_batched_closure_min_nnz() == 10_000_000
+    lib.gp_set_batched_closure_min_nnz(5)
+    assert lib.gp_batched_closure_min_nnz() == 5
+    lib.gp_set_batched_closure_min_nnz(10_000_000)

@@ -63,6 +63,40 @@ def test_presolve_postsolve(case):
     _postsolve_both(lp, ro, lo, seed=3)
 
 
+@pytest.fixture
+def closure_always():
+    """Batched singleton closure on every fast phase, in both libraries."""
+    before = (gp_ffi.batched_closure_min_nnz(), O.batched_closure_min_nnz())
+    gp_ffi.set_batched_closure_min_nnz(0)
+    O.set_batched_closure_min_nnz(0)
+    yield
+    gp_ffi.set_batched_closure_min_nnz(before[0])
+    O.set_batched_closure_min_nnz(before[1])
+
+
+CLOSURE_CASES = {
+    "c0": (lambda: lp_synth.config0(seed=0), {}),
+    "c0_no_bootstrap": (lambda: lp_synth.config0(seed=0), dict(enable_tiered_bootstrap=0)),
+    "c0_no_bootstrap_tape_cpu": (lambda: lp_synth.config0(seed=0),
+                                 dict(enable_tiered_bootstrap=0, record_postsolve_tape_cpu=1)),
+    "r_zipf_no_bootstrap": (lambda: lp_synth.generate(20000, 30000, seed=5, row_len=(2, 14), zipf=1.5),
+                            dict(enable_tiered_bootstrap=0)),
+    "r_dense_eq_no_bootstrap": (lambda: lp_synth.generate(3000, 4000, seed=6, mean_row_len=3.0, frac_doubleton=0.2,
+                                                          frac_eq=0.6), dict(enable_tiered_bootstrap=0)),
+    "r_long_rows": (lambda: lp_synth.generate(20000, 30000, seed=9, row_len=(2, 64), zipf=1.5), {}),
+}
+
+
+@pytest.mark.parametrize("case", list(CLOSURE_CASES))
+def test_batched_singleton_closure(case, closure_always):
+    """The fast phase's batched singleton closure (the reference runs it from
+    10M nnz; here forced on at every size) against the oracle's."""
+    make, kw = CLOSURE_CASES[case]
+    lp = make()
+    so, ro, lo = _run_both(lp, enable_structure_specific_rules=0, **kw)
+    _postsolve_both(lp, ro, lo, seed=5)
+
+
 def test_structure_switch_is_identical():
     lp = lp_synth.config0(seed=0)
     a = gp_ffi.presolve(lp, gp_ffi.params(enable_structure_specific_rules=1), True)

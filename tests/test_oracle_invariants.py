@@ -130,6 +130,60 @@ def test_presolve_preserves_optimum_and_postsolve_recovers_it(lname, pname):
     assert abs(K.objective(lp, x) - f0) <= 1e-5 * scale
 
 
+@pytest.fixture
+def closure_always():
+    """Batched singleton closure on every fast phase (threshold 0 nnz)."""
+    before = O.batched_closure_min_nnz()
+    O.set_batched_closure_min_nnz(0)
+    yield
+    O.set_batched_closure_min_nnz(before)
+
+
+def test_closure_threshold_default():
+    assert O.batched_closure_min_nnz() == 10_000_000
+
+
+CLOSURE_PARAMS = {
+    "default": {},
+    "no_bootstrap": dict(enable_tiered_bootstrap=0),
+    "no_bootstrap_tape_cpu": dict(enable_tiered_bootstrap=0, record_postsolve_tape_cpu=1),
+}
+
+
+@pytest.mark.parametrize("pname", list(CLOSURE_PARAMS))
+@pytest.mark.parametrize("lname", list(EQUIV_LPS))
+def test_closure_preserves_optimum_and_postsolve_recovers_it(lname, pname, closure_always):
+    """The batched singleton closure (column-singleton EQ, dual fix and
+    singleton rows on one plan with degree-refreshed statistics) keeps the
+    LP's optimum, and postsolve of the reduced optimum recovers a feasible
+    optimal point; same tolerances as above.  Without the bootstrap the
+    first fast phase's closure does the whole singleton cascade, so the
+    record must differ from the closure-free schedule's (it really ran)."""
+    lp = _lp(lname)
+    prm = dict(enable_structure_specific_rules=0, **CLOSURE_PARAMS[pname])
+    s, rec, red = O.presolve(lp, O.params(**prm), True)
+    assert not s["has_infeasible"] and not s["has_unbounded"]
+    if pname != "default":
+        O.set_batched_closure_min_nnz(10_000_000)
+        _, rec_plain, _ = O.presolve(lp, O.params(**prm), True)
+        O.set_batched_closure_min_nnz(0)
+        assert (rec["types"].tolist() != rec_plain["types"].tolist()
+                or rec["col_red2org"].tolist() != rec_plain["col_red2org"].tolist()
+                or rec["indices"].tolist() != rec_plain["indices"].tolist())
+    st0, f0, *_ = K.solve(lp)
+    st1, f1, x1, y1, z1 = K.solve(red)
+    if st1 == 2:
+        st1, f1, x1, y1, z1 = K.solve(red, feasibility_tol=1e-5)
+    assert st0 == 0 and st1 == 0, (st0, st1)
+    scale = 1.0 + abs(f0)
+    assert abs(f1 - f0) <= 1e-5 * scale, (f0, f1)
+    x, y, z = O.postsolve(rec, x1, y1, z1, lp)
+    assert K.primal_violation(lp, x) <= 1e-4
+    assert abs(K.objective(lp, x) - f0) <= 1e-5 * scale
+    if pname == "no_bootstrap_tape_cpu" and lname in ("c0", "r1"):
+        assert K.dual_residual(lp, y, z) <= 1e-9
+
+
 @pytest.mark.parametrize("lname", ["c0", "r1"])
 def test_dual_recovery_exact_with_cpu_tape(lname):
     """With every rule taping (record_postsolve_tape_cpu), the recovered
