@@ -154,6 +154,123 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
                         f"-> {m_new} rows, {nnz_new} nnz"}
 
 
+def _upload_pool(gp_ffi, lps):
+    t = time.time()
+    d = [gp_ffi.DeviceLp(lp) for lp in lps]
+    log(f"uploaded {len(d)} LPs in {time.time() - t:.1f}s")
+    return d
+
+
+def bench_batched(args):
+    """4,096 independent LPs presolved together (every job arrives at t = 0)
+    by the serving runtime, against the same LPs presolved one at a time
+    through the plain entry point (gp_presolve, synchronous, hipMalloc'ed
+    reduced LP).  The worker count comes from the autotuner on a 256-LP
+    sample."""
+    import numpy as np
+
+    import gp_ffi
+    import workloads
+
+    t = time.time()
+    lps = workloads.batched_pool(args.batch_lps)
+    ms = np.array([lp["m"] for lp in lps])
+    nnz = np.array([len(lp["A_idx"]) for lp in lps])
+    log(f"generated {len(lps)} LPs (sum m {ms.sum()}, sum nnz {nnz.sum()}) in {time.time() - t:.1f}s")
+    dlps = _upload_pool(gp_ffi, lps)
+    prm = gp_ffi.params(enable_structure_specific_rules=0)
+    sample = np.random.default_rng(11).choice(len(dlps), size=min(256, len(dlps)), replace=False).astype(np.int32)
+
+    def run(w, jobs=sample):
+        return gp_ffi.serve_trace(dlps, jobs, prm, gp_ffi.serve_config(workers=w, postsolve=False))
+
+    run(4, sample[:32])  # warm-up: contexts, pools, code objects
+    best, table = workloads.autotune_workers(run)
+    log(f"autotune (LP/s on a {len(sample)}-LP sample): {table} -> {best} workers")
+    allj = np.arange(len(dlps), dtype=np.int32)
+    res = run(best, allj)
+    bad = [r for r in res if r["status"] != 0]
+    if bad:
+        raise SystemExit(f"bench.py: {len(bad)} batched jobs failed")
+    st = workloads.latency_stats(res)
+    log(f"batched: {st}")
+    t = time.perf_counter()
+    for d in dlps:
+        h = gp_ffi.Presolved(d, prm, keep_reduced=True)
+        h.free()
+    plain_s = time.perf_counter() - t
+    plain = len(dlps) / plain_s
+    log(f"plain one at a time: {plain:.1f} LP/s ({plain_s:.1f} s)")
+    out = {
+        "metric": "batched presolve LPs/s vs one-at-a-time plain presolve",
+        "value": st["lp_per_s"], "unit": "LP/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (seeded, dyadic values, planted reductions)",
+        "config": {"workload": "batched presolve", "lps": len(dlps), "m": "logU[100,20000]", "n": "1.5 m",
+                   "row_len": "Poisson(6), >= 2", "sum_m": int(ms.sum()), "sum_nnz": int(nnz.sum()),
+                   "workers": best},
+        "makespan_s": st["makespan_s"],
+        "plain_one_at_a_time": {"value": plain, "unit": "LP/s", "seconds": plain_s},
+        "speedup_vs_plain": st["lp_per_s"] / plain,
+        "autotune_lp_per_s": {str(k): v for k, v in table.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_serving(args):
+    """Poisson arrivals of presolve + postsolve requests (seeded reduced
+    point in place of a solver) served with continuous admission under a
+    200 GB budget; the requests reference a pool of pre-generated LPs drawn
+    from the request size distribution, resident in HBM.  The same trace's
+    first requests are also served one at a time (one worker) for the serial
+    baseline."""
+    import numpy as np
+
+    import gp_ffi
+    import workloads
+
+    t = time.time()
+    lps = workloads.serving_pool(args.serve_pool)
+    ms = np.array([lp["m"] for lp in lps])
+    nnz = np.array([len(lp["A_idx"]) for lp in lps])
+    log(f"generated a pool of {len(lps)} LPs (mean m {ms.mean():.0f}, mean nnz {nnz.mean():.0f}) "
+        f"in {time.time() - t:.1f}s")
+    dlps = _upload_pool(gp_ffi, lps)
+    prm = gp_ffi.params(enable_structure_specific_rules=0)
+    arrival, lp_of_job = workloads.poisson_trace(args.serve_rate, args.serve_seconds, len(dlps))
+    budget = 200e9
+
+    def run(w, jobs, arr=None):
+        cfg = gp_ffi.serve_config(workers=w, postsolve=True, hbm_budget_bytes=budget, seed=99)
+        return gp_ffi.serve_trace(dlps, jobs, prm, cfg, arr)
+
+    sample = lp_of_job[:64]
+    run(4, sample[:16])  # warm-up
+    best, table = workloads.autotune_workers(lambda w: run(w, sample))
+    log(f"autotune (LP/s, 64 jobs at once): {table} -> {best} workers")
+    res = run(best, lp_of_job, arrival)
+    bad = [r for r in res if r["status"] != 0]
+    if bad:
+        raise SystemExit(f"bench.py: {len(bad)} served jobs failed")
+    st = workloads.latency_stats(res)
+    log(f"served: {st}")
+    k = int(np.searchsorted(arrival, min(10.0, args.serve_seconds)))
+    ser = workloads.latency_stats(run(1, lp_of_job[:k], arrival[:k]))
+    log(f"serial (first {k} requests, one worker): {ser}")
+    out = {
+        "metric": "served presolve+postsolve requests: LPs/s and p50/p99 latency",
+        "value": st["lp_per_s"], "unit": "LP/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f64", "data": "synthetic (seeded, dyadic values, planted reductions)",
+        "config": {"workload": "serving", "rate_per_s": args.serve_rate, "seconds": args.serve_seconds,
+                   "requests": st["jobs"], "pool_lps": len(dlps), "m": "logU[1000,500000]", "n": "1.5 m",
+                   "row_len": "U[4,10]", "pool_mean_m": float(ms.mean()), "pool_mean_nnz": float(nnz.mean()),
+                   "hbm_budget_bytes": budget, "workers": best},
+        "p50_ms": st["p50_ms"], "p99_ms": st["p99_ms"], "max_ms": st["max_ms"],
+        "serial_one_worker": dict(ser, requests=k, sample=f"first {k} requests of the same trace"),
+        "capacity_lp_per_s_by_workers": {str(k2): v for k2, v in table.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,11 +282,22 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the row-activity and compaction kernel timings plus a 256 MiB device copy "
                          "(for hardware-counter runs)")
+    ap.add_argument("--workload", choices=("plain", "batched", "serving"), default="plain",
+                    help="plain: the single large LP (default); batched: 4,096 small LPs at once against "
+                         "one-at-a-time plain presolve; serving: Poisson presolve+postsolve requests")
+    ap.add_argument("--batch-lps", type=int, default=4096)
+    ap.add_argument("--serve-seconds", type=float, default=60.0)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-pool", type=int, default=48)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this library runs one LP presolve per GPU and "
               "the benchmark measures one GPU only (N=1).", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batched":
+        return bench_batched(args)
+    if args.workload == "serving":
+        return bench_serving(args)
 
     import numpy as np
 

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "gp_kernels.h"
+#include "gp_presolve_capi.h"
 #include "gpu_presolver/presolve/presolve_structs.hpp"
 
 namespace gpu_presolver::amd {
@@ -174,6 +175,28 @@ inline gpk_csr csr_view(const presolve::DeviceCsrMatrix& a) {
 inline gpk_plan plan_view(const presolve::PresolvePlanGpu& p) {
   return gpk_plan{p.keep_row_mask, p.keep_col_mask, p.new_c, p.new_l, p.new_u, p.new_AL, p.new_AU};
 }
+
+// Options of the calling thread's presolve and postsolve calls.  The public
+// entry points synchronize the whole device on entry (their inputs may have
+// been written on the legacy stream) and hand out hipMalloc'ed outputs that
+// the caller releases with hipFree.  A job of the serving runtime has its
+// inputs ready, runs on its own stream and returns its outputs to the pool
+// (dev_free), so it does neither: a device-wide synchronization or a hipFree
+// would stall every other job in flight.
+struct JobMode {
+  bool device_sync_on_entry = true;
+  bool pooled_outputs = false;
+};
+JobMode& job_mode();
+
+// Serving runtime (host_serve.cpp).
+class Server;
+Server* server_create(const presolve::PresolveParams& prm, const ::gp_serve_config& cfg);
+void server_destroy(Server* s);
+int64_t server_submit(Server* s, const presolve::LPInfoGpu& lp, uint64_t tag, double t_arrival);
+int server_wait(Server* s, ::gp_job_result* out, int max, double timeout_s);
+double server_now(const Server* s);
+uint64_t checksum_xyz(const double* x, int32_t n, const double* y, int32_t m, const double* z);
 
 // Working-LP nnz from which each fast phase opens with the batched singleton
 // closure (process-wide; 10,000,000 as in the reference).

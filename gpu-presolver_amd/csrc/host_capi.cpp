@@ -5,8 +5,11 @@
 
 #include <hip/hip_runtime_api.h>
 
+#include <chrono>
+#include <memory>
 #include <new>
 #include <stdexcept>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -37,6 +40,10 @@ struct gp_presolve_handle {
 
 struct gp_tape_handle {
   PostsolveTape tape;
+};
+
+struct gp_server {
+  amd::Server* impl = nullptr;
 };
 
 namespace {
@@ -498,6 +505,96 @@ gp_status gp_presolve_reduced_lp(const gp_presolve_handle* h, gp_lp* out) {
 
 int32_t gp_has_good_nnz_progress(int32_t nnz_before, int32_t nnz_after, double ratio) {
   return _has_good_nnz_progress(nnz_before, nnz_after, ratio) ? 1 : 0;
+}
+
+gp_status gp_server_create(const gp_params* params, const gp_serve_config* config, gp_server** out) {
+  return guarded([&] {
+    require(config != nullptr && out != nullptr, "null argument");
+    require(config->workers >= 1, "gp_serve_config.workers must be >= 1");
+    *out = nullptr;
+    require_device();
+    auto* h = new gp_server();
+    try {
+      h->impl = amd::server_create(to_params(params), *config);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *out = h;
+  });
+}
+
+gp_status gp_server_submit(gp_server* server, const gp_lp* lp, uint64_t tag, double t_arrival, int64_t* job) {
+  return guarded([&] {
+    require(server != nullptr && lp != nullptr, "null argument");
+    const LPInfoGpu L = to_lp(*lp);
+    const int64_t id = amd::server_submit(server->impl, L, tag, t_arrival);
+    if (job) *job = id;
+  });
+}
+
+int32_t gp_server_wait(gp_server* server, gp_job_result* out, int32_t max, double timeout_s) {
+  if (server == nullptr || out == nullptr || max <= 0) return 0;
+  return amd::server_wait(server->impl, out, max, timeout_s);
+}
+
+double gp_server_now(const gp_server* server) { return server ? amd::server_now(server->impl) : 0.0; }
+
+void gp_server_destroy(gp_server* server) {
+  if (server == nullptr) return;
+  amd::server_destroy(server->impl);
+  delete server;
+}
+
+gp_status gp_serve_trace(const gp_lp* lps, int32_t lp_count, const int32_t* lp_of_job, const double* arrival_s,
+                         int32_t jobs, const gp_params* params, const gp_serve_config* config,
+                         gp_job_result* results) {
+  return guarded([&] {
+    require(lps != nullptr && lp_of_job != nullptr && config != nullptr && results != nullptr, "null argument");
+    require(lp_count > 0 && jobs >= 0 && config->workers >= 1, "invalid trace size or worker count");
+    std::vector<LPInfoGpu> L;
+    for (int32_t k = 0; k < lp_count; ++k) L.push_back(to_lp(lps[k]));
+    for (int32_t k = 0; k < jobs; ++k) {
+      require(lp_of_job[k] >= 0 && lp_of_job[k] < lp_count, "lp_of_job out of range");
+      if (arrival_s) require(k == 0 || arrival_s[k] >= arrival_s[k - 1], "arrival times must be non-decreasing");
+    }
+    require_device();
+    std::unique_ptr<amd::Server, void (*)(amd::Server*)> srv(amd::server_create(to_params(params), *config),
+                                                             amd::server_destroy);
+    for (int32_t k = 0; k < jobs; ++k) {
+      const double t = arrival_s ? arrival_s[k] : 0.0;
+      const double wait = t - amd::server_now(srv.get());
+      if (wait > 0.0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+      amd::server_submit(srv.get(), L[lp_of_job[k]], static_cast<uint64_t>(k), t);
+    }
+    std::vector<gp_job_result> buf(64);
+    for (int32_t got = 0; got < jobs;) {
+      const int n = amd::server_wait(srv.get(), buf.data(), static_cast<int>(buf.size()), 1.0);
+      for (int i = 0; i < n; ++i) results[buf[i].tag] = buf[i];
+      got += n;
+    }
+  });
+}
+
+gp_status gp_synthetic_point(uint64_t seed, const gp_lp* lp, double* x, double* y, double* z) {
+  return guarded([&] {
+    require(lp != nullptr, "null argument");
+    const LPInfoGpu L = to_lp(*lp);
+    require((L.A.cols == 0 || (x && z)) && (L.A.rows == 0 || y), "null output vector");
+    require_device();
+    amd::Ctx& c = amd::ctx();
+    amd::check(gpk_synthetic_point(seed, L.A.cols, L.A.rows, L.l, L.u, x, y, z, c.stream()), "synthetic point");
+    c.sync("synthetic point");
+  });
+}
+
+gp_status gp_checksum(const double* x, int32_t n, const double* y, int32_t m, const double* z, uint64_t* out) {
+  return guarded([&] {
+    require(out != nullptr && n >= 0 && m >= 0, "invalid argument");
+    require((n == 0 || (x && z)) && (m == 0 || y), "null vector");
+    require_device();
+    *out = amd::checksum_xyz(x, n, y, m, z);
+  });
 }
 
 void gp_set_batched_closure_min_nnz(int32_t nnz) { amd::set_batched_closure_min_nnz(nnz); }

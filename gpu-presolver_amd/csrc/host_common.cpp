@@ -124,6 +124,11 @@ Ctx& ctx() {
   return *own;
 }
 
+JobMode& job_mode() {
+  thread_local JobMode mode;
+  return mode;
+}
+
 ScopedCtx::ScopedCtx(Ctx& c) : prev_(t_current) { t_current = &c; }
 ScopedCtx::~ScopedCtx() { t_current = prev_; }
 

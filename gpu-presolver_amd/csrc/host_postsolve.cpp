@@ -25,8 +25,10 @@ constexpr double kPostsolveTol = 1.0e-7;
 
 double* malloc_f64(int32_t n, const char* what) {
   if (n <= 0) return nullptr;
+  const size_t bytes = sizeof(double) * static_cast<size_t>(n);
+  if (amd::job_mode().pooled_outputs) return static_cast<double*>(amd::dev_alloc(bytes, ctx().stream(), what));
   void* p = nullptr;
-  check(hipMalloc(&p, sizeof(double) * static_cast<size_t>(n)), what);
+  check(hipMalloc(&p, bytes), what);
   return static_cast<double*>(p);
 }
 
@@ -396,7 +398,7 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
   if (original_model != nullptr &&
       (original_model->A.rows != record.m0 || original_model->A.cols != record.n0))
     throw std::runtime_error("postsolve_gpu: original model does not match the record's dimensions");
-  check(hipDeviceSynchronize(), "postsolve entry synchronize");
+  if (amd::job_mode().device_sync_on_entry) check(hipDeviceSynchronize(), "postsolve entry synchronize");
   auto& c = ctx();
   hipStream_t s = c.stream();
   GpuPostsolveResult res;
@@ -457,9 +459,9 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
     c.sync("postsolve_gpu synchronize");
   } catch (...) {
     (void)hipStreamSynchronize(s);
-    if (res.x_org) (void)hipFree(res.x_org);
-    if (res.y_org) (void)hipFree(res.y_org);
-    if (res.z_org) (void)hipFree(res.z_org);
+    amd::dev_free(res.x_org, s);
+    amd::dev_free(res.y_org, s);
+    amd::dev_free(res.z_org, s);
     throw;
   }
   return res;

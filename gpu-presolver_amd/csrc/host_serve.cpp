@@ -1,0 +1,229 @@
+// Serving runtime (no counterpart in the reference, which presolves one LP
+// per call on the legacy stream): a FIFO job queue, admission by a device
+// memory budget, and a pool of worker threads.  Each worker owns an
+// execution context (its own non-blocking stream, flag ring and pool free
+// lists, host_common.cpp), so the jobs in flight overlap on the device while
+// each one runs exactly the kernels it would run alone: every result is
+// bit-identical to the same job processed by itself.
+//
+// A job: presolve of a caller-owned device LP; then, when postsolve is on, a
+// seeded reduced-space point in place of an LP solver and postsolve against
+// the original LP; the postsolved x, y, z are reduced to an order-free
+// checksum of their bit patterns and returned to the pool.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "gp_host.hpp"
+#include "gp_presolve_capi.h"
+#include "gpu_presolver/presolve/gpu_postsolve.hpp"
+#include "gpu_presolver/presolve/gpu_presolve.hpp"
+
+namespace gpu_presolver::amd {
+
+using presolve::GpuPostsolveResult;
+using presolve::GpuPresolveSummary;
+using presolve::LPInfoGpu;
+using presolve::PresolveParams;
+
+namespace {
+
+// Device bytes of an LP (A, AT, five vectors).
+double lp_bytes(const LPInfoGpu& lp) {
+  const double m = lp.A.rows, n = lp.A.cols, nnz = lp.A.nnz;
+  return 4.0 * (m + 1) + 4.0 * (n + 1) + 24.0 * nnz + 8.0 * (3.0 * n + 2.0 * m);
+}
+
+}  // namespace
+
+// Salts of the three vectors in gp_checksum.
+constexpr uint64_t kSaltX = 0, kSaltY = 1ull << 40, kSaltZ = 2ull << 40;
+
+uint64_t checksum_xyz(const double* x, int32_t n, const double* y, int32_t m, const double* z) {
+  Ctx& c = ctx();
+  auto* acc = reinterpret_cast<unsigned long long*>(c.flags());  // two zeroed words, 64-byte aligned
+  check(gpk_checksum_f64(x, n, kSaltX, acc, c.stream()), "checksum x");
+  check(gpk_checksum_f64(y, m, kSaltY, acc, c.stream()), "checksum y");
+  check(gpk_checksum_f64(z, n, kSaltZ, acc, c.stream()), "checksum z");
+  const int32_t* w = c.read_flags(2);
+  return static_cast<uint64_t>(static_cast<uint32_t>(w[0])) | (static_cast<uint64_t>(static_cast<uint32_t>(w[1])) << 32);
+}
+
+class Server {
+ public:
+  Server(const PresolveParams& prm, const gp_serve_config& cfg) : prm_(prm), cfg_(cfg) {
+    if (cfg_.workers < 1) throw std::invalid_argument("gp_serve_config.workers must be >= 1");
+    check(hipGetDevice(&device_), "hipGetDevice");
+    t0_ = std::chrono::steady_clock::now();
+    for (int w = 0; w < cfg_.workers; ++w) ctxs_.push_back(std::make_unique<Ctx>());
+    for (int w = 0; w < cfg_.workers; ++w) workers_.emplace_back([this, w] { worker_main(w); });
+  }
+
+  ~Server() {
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto& t : workers_) t.join();
+    ctxs_.clear();
+  }
+
+  double now() const {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+  }
+
+  int64_t submit(const LPInfoGpu& lp, uint64_t tag, double t_arrival) {
+    Job j{0, lp, tag, t_arrival < 0.0 ? now() : t_arrival, 4.0 * lp_bytes(lp)};
+    {
+      std::lock_guard<std::mutex> lock(mu_);
+      j.id = next_id_++;
+      queue_.push_back(j);
+    }
+    cv_work_.notify_one();
+    return j.id;
+  }
+
+  int wait(gp_job_result* out, int max, double timeout_s) {
+    std::unique_lock<std::mutex> lock(mu_);
+    const auto until = std::chrono::steady_clock::now() + std::chrono::duration<double>(std::max(0.0, timeout_s));
+    cv_done_.wait_until(lock, until, [&] { return !done_.empty(); });
+    int k = 0;
+    while (k < max && !done_.empty()) {
+      out[k++] = done_.front();
+      done_.pop_front();
+    }
+    return k;
+  }
+
+ private:
+  struct Job {
+    int64_t id;
+    LPInfoGpu lp;
+    uint64_t tag;
+    double t_arrival;
+    double bytes;
+  };
+
+  // FIFO admission: the head job starts when a worker is free and it fits
+  // the budget next to the jobs in flight (or nothing else is in flight).
+  bool admissible() const {
+    if (queue_.empty()) return false;
+    if (inflight_jobs_ == 0 || cfg_.hbm_budget_bytes <= 0.0) return true;
+    return inflight_bytes_ + queue_.front().bytes <= cfg_.hbm_budget_bytes;
+  }
+
+  void worker_main(int w) {
+    (void)hipSetDevice(device_);
+    ScopedCtx scope(*ctxs_[w]);
+    job_mode() = JobMode{false, true};
+    while (true) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lock(mu_);
+        cv_work_.wait(lock, [&] { return stop_ && queue_.empty() ? true : admissible(); });
+        if (queue_.empty()) return;  // stopping and drained
+        j = queue_.front();
+        queue_.pop_front();
+        inflight_bytes_ += j.bytes;
+        ++inflight_jobs_;
+      }
+      gp_job_result r = run(j, w);
+      {
+        std::lock_guard<std::mutex> lock(mu_);
+        inflight_bytes_ -= j.bytes;
+        --inflight_jobs_;
+        done_.push_back(r);
+      }
+      cv_done_.notify_all();
+      cv_work_.notify_all();
+    }
+  }
+
+  gp_job_result run(const Job& j, int w) {
+    gp_job_result r{};
+    r.job = j.id;
+    r.tag = j.tag;
+    r.worker = w;
+    r.t_arrival = j.t_arrival;
+    r.t_start = now();
+    Ctx& c = ctx();
+    try {
+      GpuPresolveSummary sum = presolve::run_gpu_presolve_with_reduced_lp(j.lp, prm_);
+      r.reduced_rows = sum.reduced_rows;
+      r.reduced_cols = sum.reduced_cols;
+      r.reduced_nnz = sum.reduced_nnz;
+      r.iterations = sum.iterations;
+      r.has_infeasible = sum.has_infeasible;
+      r.has_unbounded = sum.has_unbounded;
+      try {
+        if (cfg_.postsolve && !sum.has_infeasible && !sum.has_unbounded) {
+          const LPInfoGpu& red = sum.reduced_lp;
+          const int32_t n1 = red.A.cols, m1 = red.A.rows;
+          DBuf<double> x(n1), y(m1), z(n1);
+          check(gpk_synthetic_point(cfg_.seed ^ j.tag, n1, m1, red.l, red.u, x, y, z, c.stream()),
+                "synthetic reduced point");
+          GpuPostsolveResult ps = presolve::postsolve_gpu(x, y, z, sum.record, &j.lp, nullptr);
+          try {
+            r.checksum = checksum_xyz(ps.x_org, ps.n0, ps.y_org, ps.m0, ps.z_org);
+          } catch (...) {
+            dev_free(ps.x_org, c.stream());
+            dev_free(ps.y_org, c.stream());
+            dev_free(ps.z_org, c.stream());
+            throw;
+          }
+          dev_free(ps.x_org, c.stream());
+          dev_free(ps.y_org, c.stream());
+          dev_free(ps.z_org, c.stream());
+        }
+      } catch (...) {
+        presolve::free_gpu_presolve_reduced_lp(sum);
+        throw;
+      }
+      presolve::free_gpu_presolve_reduced_lp(sum);
+      c.sync("serving job");
+      r.status = GP_OK;
+    } catch (const HipError&) {
+      r.status = GP_ERR_DEVICE;
+    } catch (const std::bad_alloc&) {
+      r.status = GP_ERR_INTERNAL;
+    } catch (const std::exception&) {
+      r.status = GP_ERR_INTERNAL;
+    }
+    r.t_end = now();
+    return r;
+  }
+
+  PresolveParams prm_;
+  gp_serve_config cfg_;
+  int device_ = 0;
+  std::chrono::steady_clock::time_point t0_;
+  std::vector<std::unique_ptr<Ctx>> ctxs_;
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<Job> queue_;
+  std::deque<gp_job_result> done_;
+  double inflight_bytes_ = 0.0;
+  int inflight_jobs_ = 0;
+  int64_t next_id_ = 0;
+  bool stop_ = false;
+};
+
+Server* server_create(const PresolveParams& prm, const gp_serve_config& cfg) { return new Server(prm, cfg); }
+void server_destroy(Server* s) { delete s; }
+int64_t server_submit(Server* s, const LPInfoGpu& lp, uint64_t tag, double t_arrival) {
+  return s->submit(lp, tag, t_arrival);
+}
+int server_wait(Server* s, gp_job_result* out, int max, double timeout_s) { return s->wait(out, max, timeout_s); }
+double server_now(const Server* s) { return s->now(); }
+
+}  // namespace gpu_presolver::amd

@@ -95,6 +95,18 @@ def lib():
         L.gp_set_batched_closure_min_nnz.restype = None
         L.gp_batched_closure_min_nnz.argtypes = []
         L.gp_batched_closure_min_nnz.restype = C.c_int32
+        L.gp_server_create.argtypes = [vp, C.POINTER(T.GpServeConfig), C.POINTER(vp)]
+        L.gp_server_submit.argtypes = [vp, C.POINTER(T.GpLp), C.c_uint64, C.c_double, C.POINTER(C.c_int64)]
+        L.gp_server_wait.argtypes = [vp, C.POINTER(T.GpJobResult), C.c_int32, C.c_double]
+        L.gp_server_wait.restype = C.c_int32
+        L.gp_server_now.argtypes = [vp]
+        L.gp_server_now.restype = C.c_double
+        L.gp_server_destroy.argtypes = [vp]
+        L.gp_server_destroy.restype = None
+        L.gp_serve_trace.argtypes = [C.POINTER(T.GpLp), C.c_int32, vp, vp, C.c_int32, vp,
+                                     C.POINTER(T.GpServeConfig), C.POINTER(T.GpJobResult)]
+        L.gp_synthetic_point.argtypes = [C.c_uint64, C.POINTER(T.GpLp), vp, vp, vp]
+        L.gp_checksum.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
 
@@ -313,10 +325,14 @@ class Presolved:
         _check(lib().gp_presolve_record(self.h, C.byref(rv)))
         return T.record_to_dict(rv)
 
-    def reduced(self):
+    def reduced_view(self):
+        """Device view (gp_lp) of the reduced LP, valid until free()."""
         g = T.GpLp()
         _check(lib().gp_presolve_reduced_lp(self.h, C.byref(g)))
-        return lp_from_struct(g)
+        return g
+
+    def reduced(self):
+        return lp_from_struct(self.reduced_view())
 
     def free(self):
         if self.h:
@@ -387,3 +403,77 @@ class Event:
         if getattr(self, "ptr", None) and _hip is not None:
             hip().hipEventDestroy(self.ptr)
             self.ptr = None
+
+
+# ---- serving and batching ----------------------------------------------------
+JOB_FIELDS = [k for k, _ in T.GpJobResult._fields_]
+
+
+def serve_config(workers, postsolve=True, hbm_budget_bytes=0.0, seed=0):
+    return T.GpServeConfig(int(workers), int(bool(postsolve)), float(hbm_budget_bytes), int(seed))
+
+
+def _results_dict(res):
+    return [{k: getattr(r, k) for k in JOB_FIELDS} for r in res]
+
+
+def serve_trace(dlps, lp_of_job, prm, cfg, arrival_s=None):
+    """Runs job k = dlps[lp_of_job[k]] arriving at arrival_s[k] (None: all at
+    once) through the serving runtime; returns a list of result dicts."""
+    jobs = len(lp_of_job)
+    lps = (T.GpLp * len(dlps))(*[d.g for d in dlps])
+    loj = np.ascontiguousarray(lp_of_job, dtype=np.int32)
+    arr = None if arrival_s is None else np.ascontiguousarray(arrival_s, dtype=np.float64)
+    res = (T.GpJobResult * max(jobs, 1))()
+    _check(lib().gp_serve_trace(lps, len(dlps), loj.ctypes.data, None if arr is None else arr.ctypes.data, jobs,
+                                C.byref(prm), C.byref(cfg), res))
+    return _results_dict(res[:jobs])
+
+
+class Server:
+    """The streaming form: submit jobs, wait for results."""
+
+    def __init__(self, prm, cfg):
+        self.h = C.c_void_p()
+        _check(lib().gp_server_create(C.byref(prm), C.byref(cfg), C.byref(self.h)))
+        self._keep = []
+
+    def submit(self, dlp, tag, t_arrival=-1.0):
+        self._keep.append(dlp)
+        job = C.c_int64()
+        _check(lib().gp_server_submit(self.h, C.byref(dlp.g), int(tag), float(t_arrival), C.byref(job)))
+        return job.value
+
+    def wait(self, max_results=64, timeout_s=1.0):
+        buf = (T.GpJobResult * max_results)()
+        n = lib().gp_server_wait(self.h, buf, max_results, timeout_s)
+        return _results_dict(buf[:n])
+
+    def now(self):
+        return lib().gp_server_now(self.h)
+
+    def close(self):
+        if self.h:
+            lib().gp_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        if _lib is not None:
+            self.close()
+
+
+def synthetic_point(seed, reduced_g):
+    """The serving runtime's seeded reduced point for a reduced LP (gp_lp of
+    device pointers); returns host copies of x, y, z."""
+    n, m = reduced_g.A.cols, reduced_g.A.rows
+    x, y, z = DeviceArray(size=n), DeviceArray(size=m), DeviceArray(size=n)
+    _check(lib().gp_synthetic_point(int(seed), C.byref(reduced_g), x.ptr, y.ptr, z.ptr))
+    return x.numpy(), y.numpy(), z.numpy()
+
+
+def checksum(x, y, z):
+    """gp_checksum of host vectors (uploaded first)."""
+    dx, dy, dz = (DeviceArray(np.asarray(v, np.float64)) for v in (x, y, z))
+    out = C.c_uint64()
+    _check(lib().gp_checksum(dx.ptr, dx.size, dy.ptr, dy.size, dz.ptr, C.byref(out)))
+    return out.value

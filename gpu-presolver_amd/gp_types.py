@@ -97,6 +97,19 @@ class GpSummary(C.Structure):
                 ("has_infeasible", C.c_int32), ("has_unbounded", C.c_int32)]
 
 
+class GpServeConfig(C.Structure):
+    _fields_ = [("workers", C.c_int32), ("postsolve", C.c_int32), ("hbm_budget_bytes", C.c_double),
+                ("seed", C.c_uint64)]
+
+
+class GpJobResult(C.Structure):
+    _fields_ = [("job", C.c_int64), ("tag", C.c_uint64), ("status", C.c_int32), ("worker", C.c_int32),
+                ("reduced_rows", C.c_int32), ("reduced_cols", C.c_int32), ("reduced_nnz", C.c_int32),
+                ("iterations", C.c_int32), ("has_infeasible", C.c_int32), ("has_unbounded", C.c_int32),
+                ("checksum", C.c_uint64), ("t_arrival", C.c_double), ("t_start", C.c_double),
+                ("t_end", C.c_double)]
+
+
 class GpRecordView(C.Structure):
     _fields_ = [("m0", C.c_int32), ("n0", C.c_int32), ("m1", C.c_int32), ("n1", C.c_int32),
                 ("row_red2org", i32p), ("col_red2org", i32p),

@@ -159,6 +159,15 @@ typedef struct gpk_plan {
 hipError_t gpk_plan_init(int32_t m, int32_t n, gpk_plan plan, const double* c, const double* l, const double* u,
                          const double* AL, const double* AU, hipStream_t stream);
 
+/* Serving runtime (no reference counterpart).  gpk_synthetic_point: seeded
+ * reduced-space point standing in for a solver (x in [l, u] on the 2^-10
+ * grid, y and z uniform in [-1, 1)); element draws depend only on (seed,
+ * index).  gpk_checksum_f64: *acc += sum_i mix(bits(v[i]), salt + i) mod
+ * 2^64, independent of the summation order. */
+hipError_t gpk_synthetic_point(uint64_t seed, int32_t n, int32_t m, const double* l, const double* u, double* x,
+                               double* y, double* z, hipStream_t stream);
+hipError_t gpk_checksum_f64(const double* v, int32_t n, uint64_t salt, unsigned long long* acc, hipStream_t stream);
+
 /* Batched singleton closure (reference gpu_presolve.cu:98-247, 2506-2831).
  * gpk_closure_degrees: row_nnz (col_nnz) lose the live entries of columns
  *   (rows) that were kept in col_before (row_before) and are removed now;

@@ -264,6 +264,60 @@ gp_status gp_postsolve(const gp_record_view* record, const double* x_red, const 
                        const double* z_red, const gp_lp* original, double* x_org, double* y_org,
                        double* z_org);
 
+/* ---- serving and batching (no counterpart in the reference) ------------
+ * The reference presolves one LP per call on the legacy stream.  The serving
+ * runtime keeps several jobs in flight, each on its own stream and execution
+ * context, so their kernels overlap on the device.  A job presolves a device
+ * LP the caller owns (it must stay valid until the job's result is
+ * returned); with postsolve on, it then draws the seeded reduced-space point
+ * gp_synthetic_point(seed ^ tag, reduced LP) in place of an LP solver,
+ * postsolves it against the original LP and reports gp_checksum of the
+ * postsolved x, y, z.  Jobs start in submission order when a worker is free
+ * and 4x the LP's device bytes fit hbm_budget_bytes next to the jobs in
+ * flight (a job alone is always admitted).  Every kernel is deterministic,
+ * so a job's result is bit-identical alone and in any mix. */
+typedef struct gp_serve_config {
+  int32_t workers;         /* jobs in flight, >= 1 */
+  int32_t postsolve;       /* 1: presolve + seeded point + postsolve; 0: presolve only */
+  double hbm_budget_bytes; /* admission budget; <= 0: unlimited */
+  uint64_t seed;           /* job seeds are seed ^ tag */
+} gp_serve_config;
+
+typedef struct gp_job_result {
+  int64_t job; /* submission index */
+  uint64_t tag;
+  int32_t status; /* gp_status of the job */
+  int32_t worker;
+  int32_t reduced_rows, reduced_cols, reduced_nnz, iterations;
+  int32_t has_infeasible, has_unbounded;
+  uint64_t checksum;                 /* 0 without postsolve */
+  double t_arrival, t_start, t_end;  /* seconds since the server was created */
+} gp_job_result;
+
+typedef struct gp_server gp_server;
+gp_status gp_server_create(const gp_params* params, const gp_serve_config* config, gp_server** out);
+/* Queues a job (t_arrival < 0: now, on the server's clock); *job gets its index. */
+gp_status gp_server_submit(gp_server* server, const gp_lp* lp, uint64_t tag, double t_arrival, int64_t* job);
+/* Moves up to max finished results to out, waiting up to timeout_s for the
+ * first one; returns how many (host only). */
+int32_t gp_server_wait(gp_server* server, gp_job_result* out, int32_t max, double timeout_s);
+double gp_server_now(const gp_server* server);
+/* Finishes every queued job, then stops the workers. */
+void gp_server_destroy(gp_server* server);
+
+/* A whole arrival trace on a fresh server: job k runs lps[lp_of_job[k]] and
+ * arrives arrival_s[k] seconds after the start (non-decreasing; NULL: all at
+ * once, the batched mode); its tag is k and results[k] receives its result. */
+gp_status gp_serve_trace(const gp_lp* lps, int32_t lp_count, const int32_t* lp_of_job, const double* arrival_s,
+                         int32_t jobs, const gp_params* params, const gp_serve_config* config,
+                         gp_job_result* results);
+
+/* The seeded reduced-space point of a serving job: x in [l, u] on the 2^-10
+ * grid, y and z uniform in [-1, 1); lp is the reduced LP (device). */
+gp_status gp_synthetic_point(uint64_t seed, const gp_lp* lp, double* x, double* y, double* z);
+/* Order-free checksum of the bit patterns of x (n), y (m) and z (n). */
+gp_status gp_checksum(const double* x, int32_t n, const double* y, int32_t m, const double* z, uint64_t* out);
+
 #ifdef __cplusplus
 }
 #endif
