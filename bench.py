@@ -66,9 +66,54 @@ def kernel_rooflines(gp_ffi, dlp, lp, launches):
     act_bytes = activity_bytes(m, n, nnz)
     act = {"us": 1e3 * act_ms, "bytes": act_bytes, "achieved": act_bytes / (act_ms * 1e-3) / 1e9}
     log(f"row activity: {act['us']:.1f} us/launch, {act['achieved']:.0f} GB/s")
+    act["packed"] = measure_activity_packed(gp_ffi, dlp, lp, stream, launches, mn.numpy(), mx.numpy())
     comp = measure_compaction(gp_ffi, dlp, lp, stream, launches)
     log(f"compaction: {comp['us']:.1f} us/launch, {comp['achieved']:.0f} GB/s")
     return act, comp
+
+
+def measure_activity_packed(gp_ffi, dlp, lp, stream, launches, want_mn, want_mx):
+    """The row-activity variant over interleaved (l, u): gpk_pack_lu, then
+    gpk_row_activity_lu; both timed, results checked against the plain one."""
+    import ctypes as C
+
+    import numpy as np
+
+    m, n, nnz = lp["m"], lp["n"], len(lp["A_idx"])
+    L = gp_ffi.lib()
+    vp = C.c_void_p
+    L.gpk_pack_lu.argtypes = [vp, vp, C.c_int32, vp, vp]
+    L.gpk_row_activity_lu.argtypes = [vp, vp, vp, C.c_int32, vp, C.c_double, vp, vp, vp, vp, vp]
+    lu = gp_ffi.DeviceArray(size=2 * n)
+    mn, mx = gp_ffi.DeviceArray(size=m), gp_ffi.DeviceArray(size=m)
+    ni, pi = gp_ffi.DeviceArray(size=m, dtype=np.int32), gp_ffi.DeviceArray(size=m, dtype=np.int32)
+    A = dlp.A.g
+
+    def pack():
+        gp_ffi.hip_check(L.gpk_pack_lu(dlp.vec["l"].ptr, dlp.vec["u"].ptr, n, lu.ptr, stream.ptr), "pack lu")
+
+    def act():
+        gp_ffi.hip_check(L.gpk_row_activity_lu(A.row_ptr, A.col_idx, A.val, m, lu.ptr, 1e-10, mn.ptr, mx.ptr,
+                                               ni.ptr, pi.ptr, stream.ptr), "activity lu")
+
+    pack()
+    for _ in range(3):
+        act()
+    stream.sync()
+    e0, e1, e2 = gp_ffi.Event(), gp_ffi.Event(), gp_ffi.Event()
+    e0.record(stream)
+    for _ in range(launches):
+        act()
+    e1.record(stream)
+    for _ in range(launches):
+        pack()
+    e2.record(stream)
+    us = 1e3 * e0.elapsed_ms(e1) / launches
+    pack_us = 1e3 * e1.elapsed_ms(e2) / launches
+    same = bool(np.array_equal(mn.numpy(), want_mn) and np.array_equal(mx.numpy(), want_mx))
+    b = activity_bytes(m, n, nnz)
+    log(f"row activity (packed l/u): {us:.1f} us/launch + pack {pack_us:.1f} us, same={same}")
+    return {"us": us, "pack_us": pack_us, "achieved_incl_pack": b / ((us + pack_us) * 1e-6) / 1e9, "same": same}
 
 
 CALIBRATION_BYTES = 256 << 20
@@ -84,10 +129,10 @@ def calibration_copy(gp_ffi):
 
 
 def compaction_bytes(m, n, nnz, m_new, nnz_new):
-    """Compulsory traffic of one mask compaction: read the row pointers, the
-    row map, the column map and every entry once; write the kept entries and
-    the new row pointers."""
-    return 4 * (m + 1) + 4 * m + 4 * n + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1)
+    """Compulsory traffic of one k_compact_lb launch: read the row pointers,
+    the row map, the column rank words (16 bytes per 64 columns) and every
+    entry once; write the kept entries and the new row pointers."""
+    return 4 * (m + 1) + 4 * m + 16 * ((n + 63) // 64) + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1)
 
 
 def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
@@ -115,8 +160,10 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
     L.gpk_compact_tiles.restype = C.c_int64
     L.gpk_compact_tiles.argtypes = [C.c_int32]
     L.gpk_compact_lb.argtypes = [vp, vp, vp, C.c_int32, vp, vp, C.c_int32, vp, vp, vp, vp, Lookback, vp]
+    L.gpk_build_rank.argtypes = [vp, C.c_int32, vp, vp]
     tiles = L.gpk_compact_tiles(m)
     d_rm, d_cm = gp_ffi.DeviceArray(row_map), gp_ffi.DeviceArray(col_map)
+    d_rank = gp_ffi.DeviceArray(size=2 * ((n + 63) // 64), dtype=np.uint64)
     status = gp_ffi.DeviceArray(np.zeros(tiles, np.uint64))
     tickets = gp_ffi.DeviceArray(np.zeros(launches + 8, np.int32))
     err = gp_ffi.DeviceArray(np.zeros(1, np.int32))
@@ -131,27 +178,54 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
         k = launch_no[0]
         launch_no[0] += 1
         lb = Lookback(status.ptr, tickets.ptr + 4 * k, err.ptr, k + 1)
-        gp_ffi.hip_check(L.gpk_compact_lb(A.row_ptr, A.col_idx, A.val, m, d_rm.ptr, d_cm.ptr, m_new, o_ptr.ptr,
+        gp_ffi.hip_check(L.gpk_compact_lb(A.row_ptr, A.col_idx, A.val, m, d_rm.ptr, d_rank.ptr, m_new, o_ptr.ptr,
                                           o_idx.ptr, o_val.ptr, total.ptr, lb, stream.ptr), "compact")
 
+    def build_rank():
+        gp_ffi.hip_check(L.gpk_build_rank(d_cm.ptr, n, d_rank.ptr, stream.ptr), "build rank")
+
+    build_rank()
     for _ in range(3):
         compact()
     stream.sync()
-    e0, e1 = gp_ffi.Event(), gp_ffi.Event()
+    e0, e1, e2 = gp_ffi.Event(), gp_ffi.Event(), gp_ffi.Event()
     e0.record(stream)
     for _ in range(launches):
         compact()
     e1.record(stream)
+    for _ in range(launches):
+        build_rank()
+    e2.record(stream)
     us = 1e3 * e0.elapsed_ms(e1) / launches
+    rank_us = 1e3 * e1.elapsed_ms(e2) / launches
     if err.numpy()[0] != 0 or total.numpy()[0] != nnz_new:
         raise SystemExit("bench.py: the compaction kernel did not produce the host's kept count")
     want_ptr = np.concatenate([[0], np.cumsum(np.bincount(rows[kept], minlength=m)[keep_r])]).astype(np.int32)
     if not np.array_equal(o_ptr.numpy(), want_ptr):
         raise SystemExit("bench.py: compacted row pointers differ from the host")
     nbytes = compaction_bytes(m, n, len(lp["A_idx"]), m_new, nnz_new)
-    return {"us": us, "bytes": nbytes, "achieved": nbytes / (us * 1e-6) / 1e9,
+    return {"us": us, "bytes": nbytes, "achieved": nbytes / (us * 1e-6) / 1e9, "rank_build_us": rank_us,
             "workload": f"A of the LP ({m}x{n}, {len(lp['A_idx'])} nnz) with 15% of rows and columns dropped "
                         f"-> {m_new} rows, {nnz_new} nnz"}
+
+
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_compact_lb_k_row_activity_hbm_traffic.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE and WRITE_SIZE runs, each scaled by a device copy of
+    known size), or None when the summary has no row for it."""
+    if not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 7 and cells[0] == f"`{kernel}`":
+            try:
+                return float(cells[6]) * 1e6
+            except ValueError:
+                return None
+    return None
 
 
 def _upload_pool(gp_ffi, lps):
@@ -351,10 +425,10 @@ def main():
     red = last.reduced()
     x, y, z = lp_synth.reduced_solution(3, red["m"], red["n"], red["l"], red["u"])
     t = time.perf_counter()
-    gp_ffi.postsolve(rec, x, y, z, lp)
+    gp_ffi.postsolve(rec, x, y, z, original_dev=dlp)
     post_ms = 1e3 * (time.perf_counter() - t)
     last.free()
-    log(f"postsolve (incl. host<->device copies of x/y/z and the record): {post_ms:.1f} ms")
+    log(f"postsolve (original LP resident; incl. copies of x/y/z in and out and the record): {post_ms:.1f} ms")
 
     act, comp = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
     cpu = None
@@ -392,11 +466,13 @@ def main():
                    "iterations": summary["iterations"], "postsolve_ms": post_ms},
         "roofline": {"bound": "hbm", "kernel": "k_compact_lb", "achieved": comp["achieved"],
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": comp["achieved"] / HBM_PEAK_GBS,
-                     "traffic": None, "us_per_launch": comp["us"], "algorithmic_bytes": comp["bytes"],
-                     "workload": comp["workload"]},
+                     "traffic": profiled_traffic("k_compact_lb"), "us_per_launch": comp["us"],
+                     "algorithmic_bytes": comp["bytes"],
+                     "workload": comp["workload"], "k_build_rank_us": comp["rank_build_us"]},
         "roofline_activity": {"bound": "hbm", "kernel": "k_row_activity", "achieved": act["achieved"],
                               "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": act["achieved"] / HBM_PEAK_GBS,
-                              "traffic": None, "us_per_launch": act["us"], "algorithmic_bytes": act["bytes"]},
+                              "traffic": profiled_traffic("k_row_activity"), "us_per_launch": act["us"],
+                              "algorithmic_bytes": act["bytes"], "packed_bounds_variant": act["packed"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -292,8 +292,10 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
   const int32_t m_old = A.rows;
   DBuf<int32_t> idx(A.nnz, "compact colVal");
   DBuf<double> val(A.nnz, "compact nzVal");
+  DBuf<gpk_rank_word> rank((static_cast<int64_t>(A.cols) + 63) / 64, "compact column rank");
+  check(gpk_build_rank(col_map, A.cols, rank, c.stream()), "compact column rank");
   int32_t* total = c.flags();
-  check(gpk_compact_lb(A.rowPtr, A.colVal, A.nzVal, m_old, row_map, col_map, m_new, ptr, idx, val, total,
+  check(gpk_compact_lb(A.rowPtr, A.colVal, A.nzVal, m_old, row_map, rank, m_new, ptr, idx, val, total,
                        c.lookback(gpk_compact_tiles(m_old)), c.stream()),
         "compact");
   out.nnz = c.read_flags(1)[0];

@@ -454,7 +454,9 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
       DBuf<uint8_t> prot(n0);
       check(hipMemsetAsync(prot.get(), 0, n0, s), "protected columns clear");
       check(gpk_protect_columns(tape.view, n0, prot, s), "protected columns");
-      check(gpk_project_duals(orig, prot, res.x_org, y, res.z_org, n0, kPostsolveTol, s), "project duals");
+      DBuf<int32_t> long_cols(n0);
+      check(gpk_project_duals(orig, prot, res.x_org, y, res.z_org, n0, kPostsolveTol, c.flags(), long_cols, s),
+            "project duals");
     }
     c.sync("postsolve_gpu synchronize");
   } catch (...) {

@@ -173,6 +173,77 @@ __global__ __launch_bounds__(kBlock) void k_row_activity(const int32_t* __restri
   }
 }
 
+// (l, u) interleaved, so an activity term needs one 16-byte gather.
+__global__ void k_pack_lu(const double* __restrict__ l, const double* __restrict__ u, int32_t n,
+                          double2* __restrict__ lu) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < n) lu[j] = make_double2(l[j], u[j]);
+}
+
+// k_row_activity over packed bounds.
+__global__ __launch_bounds__(kBlock) void k_row_activity_lu(const int32_t* __restrict__ row_ptr,
+                                                            const int32_t* __restrict__ col_idx,
+                                                            const double* __restrict__ val, int32_t m,
+                                                            const double2* __restrict__ lu, double ztol,
+                                                            double* __restrict__ min_fin,
+                                                            double* __restrict__ max_fin,
+                                                            int32_t* __restrict__ neg_inf,
+                                                            int32_t* __restrict__ pos_inf) {
+  __shared__ double s_tmin[kActCap];
+  __shared__ double s_tmax[kActCap];
+  __shared__ int32_t s_ptr[kActRows + 1];
+  const int32_t r0 = blockIdx.x * kActRows;
+  const int32_t rows = min(kActRows, m - r0);
+  const int tid = threadIdx.x;
+  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = row_ptr[r0 + t];
+  __syncthreads();
+  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
+  const int32_t my_b = tid < rows ? s_ptr[tid] : 0;
+  const int32_t my_e = tid < rows ? s_ptr[tid + 1] : 0;
+  double mn = 0.0, mx = 0.0;
+  int32_t ni = 0, pi = 0;
+  for (int32_t w = seg_b; w < seg_e; w += kActCap) {
+    const int32_t we = min(w + kActCap, seg_e);
+    double a[kActItems];
+    double2 b[kActItems];
+    int32_t j[kActItems];
+#pragma unroll
+    for (int k = 0; k < kActItems; ++k) {
+      const int32_t p = w + tid + k * kBlock;
+      a[k] = p < we ? val[p] : 0.0;
+      j[k] = p < we ? col_idx[p] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kActItems; ++k) b[k] = fabs(a[k]) > ztol ? lu[j[k]] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < kActItems; ++k) {
+      const int32_t p = w + tid + k * kBlock;
+      if (p >= we) break;
+      double tmn = 0.0, tmx = 0.0;
+      if (fabs(a[k]) > ztol) {
+        tmn = a[k] >= 0.0 ? a[k] * b[k].x : a[k] * b[k].y;
+        tmx = a[k] >= 0.0 ? a[k] * b[k].y : a[k] * b[k].x;
+      }
+      s_tmin[p - w] = tmn;
+      s_tmax[p - w] = tmx;
+    }
+    __syncthreads();
+    const int32_t rb = max(my_b, w), re = min(my_e, we);
+    for (int32_t p = rb; p < re; ++p) {
+      const double tmn = s_tmin[p - w], tmx = s_tmax[p - w];
+      if (isfinite(tmn)) mn += tmn; else if (tmn < 0.0) ++ni;
+      if (isfinite(tmx)) mx += tmx; else if (tmx > 0.0) ++pi;
+    }
+    __syncthreads();
+  }
+  if (tid < rows) {
+    min_fin[r0 + tid] = mn;
+    max_fin[r0 + tid] = mx;
+    neg_inf[r0 + tid] = ni;
+    pos_inf[r0 + tid] = pi;
+  }
+}
+
 __global__ void k_col_max_abs(const int32_t* at_ptr, const double* at_val, int32_t n, double* out) {
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
@@ -411,13 +482,38 @@ __global__ __launch_bounds__(kBlock) void k_scan_lb(const In* in, int32_t* out, 
 constexpr int kCompRows = 1024;
 constexpr int kCompRowsPerThread = kCompRows / kBlock;
 
+// Rank form of a keep map (map[j] = new index or -1), one 16-byte word per 64
+// entries: bit b of word w is set when entry 64w + b is kept, and base is the
+// new index of the word's first kept entry.  A lookup then reads one small,
+// cache-resident word instead of gathering from the n-entry map.
+__global__ void k_build_rank(const int32_t* __restrict__ map, int32_t n, gpk_rank_word* __restrict__ out) {
+  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int32_t v = j < n ? map[j] : -1;
+  const uint64_t bits = __ballot(v >= 0);
+  const int first = bits ? __ffsll(static_cast<unsigned long long>(bits)) - 1 : 0;
+  const int32_t base = __shfl(v, first, 64);
+  if (lane_id() == 0 && (j >> 6) <= ((n - 1) >> 6)) {
+    gpk_rank_word w;
+    w.bits = bits;
+    w.base = bits ? base : 0;
+    w.pad = 0;
+    out[j >> 6] = w;
+  }
+}
+
+__device__ inline int32_t rank_lookup(const gpk_rank_word* __restrict__ rank, int32_t j) {
+  const gpk_rank_word w = rank[j >> 6];
+  const uint64_t bit = 1ull << (j & 63);
+  return (w.bits & bit) ? w.base + static_cast<int32_t>(__popcll(w.bits & (bit - 1))) : -1;
+}
+
 // Maps the thread's kCompItems consecutive entries p0.. of the tile segment:
 // rr[k] = tile-local row (-1 past the segment), jn[k] = new column index or
 // -1 when the row or the column is dropped.  The column indices are loaded
 // first and the map gathers issued together, so they overlap.
 __device__ inline void comp_map_items(const int32_t* s_ptr, const uint8_t* s_keep, int rows, int32_t seg_e,
                                       int32_t p0, const int32_t* __restrict__ idx,
-                                      const int32_t* __restrict__ col_map, int32_t* jn, int* rr) {
+                                      const gpk_rank_word* __restrict__ col_rank, int32_t* jn, int* rr) {
   int32_t cj[kCompItems];
 #pragma unroll
   for (int k = 0; k < kCompItems; ++k) cj[k] = p0 + k < seg_e ? idx[p0 + k] : 0;
@@ -432,14 +528,14 @@ __device__ inline void comp_map_items(const int32_t* s_ptr, const uint8_t* s_kee
     }
   }
 #pragma unroll
-  for (int k = 0; k < kCompItems; ++k) jn[k] = (rr[k] >= 0 && s_keep[rr[k]]) ? col_map[cj[k]] : -1;
+  for (int k = 0; k < kCompItems; ++k) jn[k] = (rr[k] >= 0 && s_keep[rr[k]]) ? rank_lookup(col_rank, cj[k]) : -1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict__ ptr,
                                                        const int32_t* __restrict__ idx,
                                                        const double* __restrict__ val, int32_t m_old,
                                                        const int32_t* __restrict__ row_map,
-                                                       const int32_t* __restrict__ col_map, int32_t m_new,
+                                                       const gpk_rank_word* __restrict__ col_rank, int32_t m_new,
                                                        int32_t* __restrict__ out_ptr, int32_t* __restrict__ out_idx,
                                                        double* __restrict__ out_val, int32_t* __restrict__ total_out,
                                                        gpk_lookback lb) {
@@ -468,7 +564,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict
   for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
     int32_t jn[kCompItems];
     int rr[kCompItems];
-    comp_map_items(s_ptr, s_keep, rows, seg_e, c + tid * kCompItems, idx, col_map, jn, rr);
+    comp_map_items(s_ptr, s_keep, rows, seg_e, c + tid * kCompItems, idx, col_rank, jn, rr);
     int local = 0, cur = rr[0];
 #pragma unroll
     for (int k = 0; k < kCompItems; ++k) {
@@ -517,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict
     const int32_t p0 = c + tid * kCompItems;
     int32_t jn[kCompItems];
     int rr[kCompItems];
-    comp_map_items(s_ptr, s_keep, rows, seg_e, p0, idx, col_map, jn, rr);
+    comp_map_items(s_ptr, s_keep, rows, seg_e, p0, idx, col_rank, jn, rr);
     double v[kCompItems];
     int32_t cnt = 0;
 #pragma unroll
@@ -823,12 +919,18 @@ hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t
 
 int64_t gpk_compact_tiles(int32_t m_old) { return m_old > 0 ? (m_old + kCompRows - 1) / kCompRows : 1; }
 
+hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_build_rank, dim3(grid_for(n)), dim3(kBlock), 0, s, map, n, out);
+  return hipGetLastError();
+}
+
 hipError_t gpk_compact_lb(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
-                          const int32_t* row_map, const int32_t* col_map, int32_t m_new, int32_t* out_ptr,
+                          const int32_t* row_map, const gpk_rank_word* col_rank, int32_t m_new, int32_t* out_ptr,
                           int32_t* out_idx, double* out_val, int32_t* total, gpk_lookback lb, hipStream_t s) {
   if (m_old <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_compact_lb, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, idx, val, m_old, row_map,
-                     col_map, m_new, out_ptr, out_idx, out_val, total, lb);
+                     col_rank, m_new, out_ptr, out_idx, out_val, total, lb);
   return hipGetLastError();
 }
 
@@ -844,6 +946,21 @@ hipError_t gpk_row_activity(const int32_t* row_ptr, const int32_t* col_idx, cons
   if (m <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_row_activity, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx, val, m, l,
                      u, ztol, min_fin, max_fin, neg_inf, pos_inf);
+  return hipGetLastError();
+}
+
+hipError_t gpk_pack_lu(const double* l, const double* u, int32_t n, double* lu, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_lu, dim3(grid_for(n)), dim3(kBlock), 0, s, l, u, n, reinterpret_cast<double2*>(lu));
+  return hipGetLastError();
+}
+
+hipError_t gpk_row_activity_lu(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
+                               const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
+                               int32_t* pos_inf, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_activity_lu, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx, val, m,
+                     reinterpret_cast<const double2*>(lu), ztol, min_fin, max_fin, neg_inf, pos_inf);
   return hipGetLastError();
 }
 

@@ -405,13 +405,13 @@ __global__ void __launch_bounds__(kLevelBlock)
   }
 }
 
-// gpu_postsolve.cu:162-180, 568-603
-__global__ void k_project_duals(gp_orig_dev O, const uint8_t* prot, const double* x, const double* y, double* z,
-                                int32_t n, double tol) {
-  const int32_t col = blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= n || (prot && prot[col])) return;
-  double aty = 0.0;
-  for (int32_t q = O.at_ptr[col]; q < O.at_ptr[col + 1]; ++q) aty += O.at_val[q] * y[O.at_idx[q]];
+// gpu_postsolve.cu:162-180, 568-603: z_j from r = c_j - (A^T y)_j and the
+// bound state of x_j.  (A^T y)_j is summed in AT order; columns longer than
+// kLongCol are left to k_project_duals_long, which loads a column 64 entries
+// at a time across a wave and adds the products in the same order.
+constexpr int32_t kLongCol = 64;
+
+__device__ inline double project_dual(const gp_orig_dev& O, int32_t col, double aty, const double* x, double tol) {
   const double r = O.c[col] - aty;
   const double xj = x[col], lj = O.l[col], uj = O.u[col];
   const bool atl = at_lower(xj, lj, tol), atu = at_upper(xj, uj, tol);
@@ -423,7 +423,41 @@ __global__ void k_project_duals(gp_orig_dev O, const uint8_t* prot, const double
   const double eps = fmin(tol, 1.0e-12);
   if (zj > eps && !isfinite(lj)) zj = 0.0;
   else if (zj < -eps && !isfinite(uj)) zj = 0.0;
-  z[col] = zj;
+  return zj;
+}
+
+__global__ void k_project_duals(gp_orig_dev O, const uint8_t* prot, const double* x, const double* y, double* z,
+                                int32_t n, double tol, int32_t* long_count, int32_t* long_cols) {
+  const int32_t col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= n || (prot && prot[col])) return;
+  const int32_t b = O.at_ptr[col], e = O.at_ptr[col + 1];
+  if (e - b > kLongCol) {
+    long_cols[atomicAdd(long_count, 1)] = col;
+    return;
+  }
+  double aty = 0.0;
+  for (int32_t q = b; q < e; ++q) aty += O.at_val[q] * y[O.at_idx[q]];
+  z[col] = project_dual(O, col, aty, x, tol);
+}
+
+// One wave per listed column; a fixed grid whose waves stride over the list.
+__global__ void k_project_duals_long(gp_orig_dev O, const double* x, const double* y, double* z, double tol,
+                                     const int32_t* long_count, const int32_t* long_cols) {
+  const int lane = lane_id();
+  const int32_t waves = gridDim.x * (blockDim.x / 64);
+  const int32_t count = *long_count;
+  for (int32_t w = blockIdx.x * (blockDim.x / 64) + static_cast<int32_t>(threadIdx.x / 64); w < count; w += waves) {
+    const int32_t col = long_cols[w];
+    const int32_t b = O.at_ptr[col], e = O.at_ptr[col + 1];
+    double aty = 0.0;
+    for (int32_t base = b; base < e; base += 64) {
+      const int32_t q = base + lane;
+      const double prod = q < e ? O.at_val[q] * y[O.at_idx[q]] : 0.0;
+      const int32_t k = min(64, e - base);
+      for (int32_t t = 0; t < k; ++t) aty += __shfl(prod, t, 64);
+    }
+    if (lane == 0) z[col] = project_dual(O, col, aty, x, tol);
+  }
 }
 
 // Columns whose dual the replay determines exactly (gpu_postsolve.cu:1370-1402).
@@ -499,8 +533,13 @@ hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipSt
   GPK_LAUNCH(k_protect, tape.count, tape, n, prot);
 }
 hipError_t gpk_project_duals(gp_orig_dev orig, const uint8_t* prot, const double* x, const double* y, double* z,
-                             int32_t n, double tol, hipStream_t stream) {
-  GPK_LAUNCH(k_project_duals, n, orig, prot, x, y, z, n, tol);
+                             int32_t n, double tol, int32_t* long_count, int32_t* long_cols, hipStream_t stream) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_project_duals, dim3(grid_for(n)), dim3(kBlock), 0, stream, orig, prot, x, y, z, n, tol,
+                     long_count, long_cols);
+  hipLaunchKernelGGL(k_project_duals_long, dim3(256), dim3(kBlock), 0, stream, orig, x, y, z, tol, long_count,
+                     long_cols);
+  return hipGetLastError();
 }
 
 }  // extern "C"

@@ -354,12 +354,13 @@ def presolve(lp, prm, keep_reduced=True):
         h.free()
 
 
-def postsolve(rec, x_red, y_red, z_red, original=None):
+def postsolve(rec, x_red, y_red, z_red, original=None, original_dev=None):
+    """original: LP dict (uploaded here) or original_dev: a resident DeviceLp."""
     rk = T.RecordKeepAlive(rec)
     n0, m0 = rec["n0"], rec["m0"]
     dx, dy, dz = DeviceArray(size=n0), DeviceArray(size=m0), DeviceArray(size=n0)
     ins = [DeviceArray(np.asarray(v, np.float64)) if v is not None else None for v in (x_red, y_red, z_red)]
-    orig = DeviceLp(original) if original is not None else None
+    orig = original_dev if original_dev is not None else (DeviceLp(original) if original is not None else None)
     _check(lib().gp_postsolve(C.byref(rk.view), *[d.ptr if d is not None else None for d in ins],
                               C.byref(orig.g) if orig is not None else None, dx.ptr, dy.ptr, dz.ptr))
     return dx.numpy(), dy.numpy(), dz.numpy()

@@ -62,12 +62,22 @@ hipError_t gpk_scan_lb_mask(const uint8_t* mask, int32_t* out, int64_t n, gpk_lo
 /* map[i] = mask[i] ? (number of set mask entries before i) : -1; *count = total. */
 hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t* count, gpk_lookback lb,
                            hipStream_t stream);
-/* Stable compaction of a CSR by old-to-new row/column maps in one launch
- * (256 old rows per tile): out_ptr (m_new + 1), out_idx / out_val (capacity
- * >= kept entries) and *total = kept entries. */
+/* Rank form of an old-to-new map, one word per 64 entries: bit b of word w
+ * is set when entry 64w + b is kept; base is the new index of the word's
+ * first kept entry.  gpk_build_rank writes ceil(n / 64) words. */
+typedef struct __attribute__((aligned(16))) gpk_rank_word {
+  uint64_t bits;
+  int32_t base;
+  int32_t pad;
+} gpk_rank_word;
+hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hipStream_t stream);
+/* Stable compaction of a CSR by an old-to-new row map and the rank form of
+ * the column map, in one launch (1024 old rows per tile): out_ptr (m_new +
+ * 1), out_idx / out_val (capacity >= kept entries) and *total = kept
+ * entries. */
 int64_t gpk_compact_tiles(int32_t m_old);
 hipError_t gpk_compact_lb(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
-                          const int32_t* row_map, const int32_t* col_map, int32_t m_new, int32_t* out_ptr,
+                          const int32_t* row_map, const gpk_rank_word* col_rank, int32_t m_new, int32_t* out_ptr,
                           int32_t* out_idx, double* out_val, int32_t* total, gpk_lookback lb, hipStream_t stream);
 
 /* old_to_new[i] = mask[i] ? prefix[i] : -1. */
@@ -79,6 +89,12 @@ hipError_t gpk_row_activity(const int32_t* row_ptr, const int32_t* col_idx, cons
                             int32_t m, const double* l, const double* u, double zero_tol,
                             double* min_fin, double* max_fin, int32_t* neg_inf, int32_t* pos_inf,
                             hipStream_t stream);
+/* The same over bounds packed as lu[2j] = l[j], lu[2j+1] = u[j] (16-byte
+ * aligned, written by gpk_pack_lu): one gather per entry instead of two. */
+hipError_t gpk_pack_lu(const double* l, const double* u, int32_t n, double* lu, hipStream_t stream);
+hipError_t gpk_row_activity_lu(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
+                               const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
+                               int32_t* pos_inf, hipStream_t stream);
 hipError_t gpk_col_max_abs(const int32_t* at_ptr, const double* at_val, int32_t n, double* out,
                            hipStream_t stream);
 /* Row stats of a CSR (counts include explicit zeros; singleton = first entry). */

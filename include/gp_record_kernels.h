@@ -70,8 +70,10 @@ hipError_t gpk_replay_levels(gp_tape_dev tape, gp_orig_dev orig, double* x, doub
                              const int32_t* records, const int32_t* level_start, int32_t nlevels, int32_t primal,
                              int32_t dual, double tol, hipStream_t stream);
 hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipStream_t stream);
+/* long_count: one zeroed device word; long_cols: n words of scratch (columns
+ * with more than 64 entries are projected one wave per column). */
 hipError_t gpk_project_duals(gp_orig_dev orig, const uint8_t* prot, const double* x, const double* y, double* z,
-                             int32_t n, double tol, hipStream_t stream);
+                             int32_t n, double tol, int32_t* long_count, int32_t* long_cols, hipStream_t stream);
 
 #ifdef __cplusplus
 }
