@@ -8,12 +8,15 @@ planted reductions of the workload description).  `value` is LPs presolved
 per second.
 
 Extra objects on the JSON line:
-  roofline           the single-pass compaction kernel (the largest share of
-                     presolve kernel time) on the full A with 15% of rows and
-                     columns dropped, timed with HIP events around back-to-back
-                     launches on one stream, against 8 TB/s;
-  roofline_activity  the row-activity kernel (gp_row_activity on a caller
-                     stream), algorithmic bytes 4(m+1) + 12 nnz + 16 n + 24 m;
+  roofline           the single-pass compaction kernel k_compact_lb (the
+                     largest share of presolve kernel time) on the full A with
+                     15% of rows and columns dropped, timed with HIP events
+                     around back-to-back launches on one stream, against 8 TB/s;
+                     traffic comes from the committed rocprofv3 PMC summary;
+  roofline_activity  one row-activity call (gp_row_activity on a caller
+                     stream: k_pack_lu then k_row_activity_lu), algorithmic
+                     bytes 4(m+1) + 12 nnz + 16 n + 24 m, with both kernels
+                     also timed apart;
   cpu_baseline       the serial CPU oracle presolving the same LP once on the
                      host (single thread).
 """
@@ -65,16 +68,17 @@ def kernel_rooflines(gp_ffi, dlp, lp, launches):
     act_ms = e0.elapsed_ms(e1) / launches
     act_bytes = activity_bytes(m, n, nnz)
     act = {"us": 1e3 * act_ms, "bytes": act_bytes, "achieved": act_bytes / (act_ms * 1e-3) / 1e9}
-    log(f"row activity: {act['us']:.1f} us/launch, {act['achieved']:.0f} GB/s")
-    act["packed"] = measure_activity_packed(gp_ffi, dlp, lp, stream, launches, mn.numpy(), mx.numpy())
+    log(f"row activity (k_pack_lu + k_row_activity_lu): {act['us']:.1f} us/call, {act['achieved']:.0f} GB/s")
+    act["kernels"] = measure_activity_kernels(gp_ffi, dlp, lp, stream, launches, mn.numpy(), mx.numpy())
     comp = measure_compaction(gp_ffi, dlp, lp, stream, launches)
-    log(f"compaction: {comp['us']:.1f} us/launch, {comp['achieved']:.0f} GB/s")
+    log(f"compaction: {comp['us']:.1f} us/call, {comp['achieved']:.0f} GB/s")
     return act, comp
 
 
-def measure_activity_packed(gp_ffi, dlp, lp, stream, launches, want_mn, want_mx):
-    """The row-activity variant over interleaved (l, u): gpk_pack_lu, then
-    gpk_row_activity_lu; both timed, results checked against the plain one."""
+def measure_activity_kernels(gp_ffi, dlp, lp, stream, launches, want_mn, want_mx):
+    """The two kernels of one row-activity call timed apart: gpk_pack_lu
+    (interleaves l and u) and gpk_row_activity_lu; the results must equal
+    those of the gp_row_activity call."""
     import ctypes as C
 
     import numpy as np
@@ -111,9 +115,28 @@ def measure_activity_packed(gp_ffi, dlp, lp, stream, launches, want_mn, want_mx)
     us = 1e3 * e0.elapsed_ms(e1) / launches
     pack_us = 1e3 * e1.elapsed_ms(e2) / launches
     same = bool(np.array_equal(mn.numpy(), want_mn) and np.array_equal(mx.numpy(), want_mx))
+    L.gpk_row_activity_lu_nt.argtypes = L.gpk_row_activity_lu.argtypes
+
+    def act_nt():
+        gp_ffi.hip_check(L.gpk_row_activity_lu_nt(A.row_ptr, A.col_idx, A.val, m, lu.ptr, 1e-10, mn.ptr, mx.ptr,
+                                                  ni.ptr, pi.ptr, stream.ptr), "activity lu nt")
+
+    act_nt()
+    stream.sync()
+    g0, g1 = gp_ffi.Event(), gp_ffi.Event()
+    g0.record(stream)
+    for _ in range(launches):
+        act_nt()
+    g1.record(stream)
+    nt_us = 1e3 * g0.elapsed_ms(g1) / launches
+    same = same and bool(np.array_equal(mn.numpy(), want_mn) and np.array_equal(mx.numpy(), want_mx))
+    log(f"  k_row_activity_lu with non-temporal CSR loads: {nt_us:.1f} us/launch")
     b = activity_bytes(m, n, nnz)
-    log(f"row activity (packed l/u): {us:.1f} us/launch + pack {pack_us:.1f} us, same={same}")
-    return {"us": us, "pack_us": pack_us, "achieved_incl_pack": b / ((us + pack_us) * 1e-6) / 1e9, "same": same}
+    if not same:
+        raise SystemExit("bench.py: gpk_row_activity_lu differs from gp_row_activity")
+    log(f"  k_row_activity_lu {us:.1f} us/launch ({b / (us * 1e-6) / 1e9:.0f} GB/s), k_pack_lu {pack_us:.1f} us")
+    return {"k_row_activity_lu_us": us, "k_pack_lu_us": pack_us, "nontemporal_variant_us": nt_us,
+            "k_row_activity_lu_achieved": b / (us * 1e-6) / 1e9}
 
 
 CALIBRATION_BYTES = 256 << 20
@@ -129,16 +152,18 @@ def calibration_copy(gp_ffi):
 
 
 def compaction_bytes(m, n, nnz, m_new, nnz_new):
-    """Compulsory traffic of one k_compact_lb launch: read the row pointers,
-    the row map, the column rank words (16 bytes per 64 columns) and every
-    entry once; write the kept entries and the new row pointers."""
-    return 4 * (m + 1) + 4 * m + 16 * ((n + 63) // 64) + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1)
+    """Compulsory traffic of one compaction: read the row pointers, the row
+    map, the column map and every entry once; write the kept entries and the
+    new row pointers."""
+    return 4 * (m + 1) + 4 * m + 4 * n + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1)
 
 
 def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
-    """Times the single-pass compaction kernel (gpk_compact_lb, the one the
-    presolve uses) on the full A with 15% of rows and columns dropped (a
-    typical early apply), back to back on one stream, and checks its output."""
+    """Times the compaction the presolve uses (gpk_build_rank, gpk_compact_map,
+    the row-pointer scan and gpk_compact_write, back to back on one stream)
+    on the full A with 15% of rows and columns dropped (a typical early
+    apply), each kernel also timed apart, and checks the output against the
+    host."""
     import ctypes as C
 
     import numpy as np
@@ -147,6 +172,7 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
         _fields_ = [("status", C.c_void_p), ("ticket", C.c_void_p), ("error", C.c_void_p), ("epoch", C.c_uint32)]
 
     m, n = lp["m"], lp["n"]
+    nnz = len(lp["A_idx"])
     rng = np.random.default_rng(seed)
     keep_r = rng.random(m) >= frac
     keep_c = rng.random(n) >= frac
@@ -157,59 +183,87 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
     m_new, nnz_new = int(keep_r.sum()), int(kept.sum())
     L = gp_ffi.lib()
     vp = C.c_void_p
-    L.gpk_compact_tiles.restype = C.c_int64
-    L.gpk_compact_tiles.argtypes = [C.c_int32]
-    L.gpk_compact_lb.argtypes = [vp, vp, vp, C.c_int32, vp, vp, C.c_int32, vp, vp, vp, vp, Lookback, vp]
     L.gpk_build_rank.argtypes = [vp, C.c_int32, vp, vp]
-    tiles = L.gpk_compact_tiles(m)
+    L.gpk_compact_map.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp, vp]
+    L.gpk_compact_write.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp]
+    L.gpk_scan_lb_i32.argtypes = [vp, vp, C.c_int64, Lookback, vp]
+    L.gpk_lookback_tiles.restype = C.c_int64
+    L.gpk_lookback_tiles.argtypes = [C.c_int64]
     d_rm, d_cm = gp_ffi.DeviceArray(row_map), gp_ffi.DeviceArray(col_map)
     d_rank = gp_ffi.DeviceArray(size=2 * ((n + 63) // 64), dtype=np.uint64)
-    status = gp_ffi.DeviceArray(np.zeros(tiles, np.uint64))
-    tickets = gp_ffi.DeviceArray(np.zeros(launches + 8, np.int32))
-    err = gp_ffi.DeviceArray(np.zeros(1, np.int32))
-    total = gp_ffi.DeviceArray(size=1, dtype=np.int32)
+    jn = gp_ffi.DeviceArray(size=nnz, dtype=np.int32)
     o_ptr = gp_ffi.DeviceArray(size=m_new + 1, dtype=np.int32)
-    o_idx = gp_ffi.DeviceArray(size=len(lp["A_idx"]), dtype=np.int32)
-    o_val = gp_ffi.DeviceArray(size=len(lp["A_idx"]))
+    o_idx = gp_ffi.DeviceArray(size=nnz, dtype=np.int32)
+    o_val = gp_ffi.DeviceArray(size=nnz)
+    status = gp_ffi.DeviceArray(np.zeros(max(1, L.gpk_lookback_tiles(m_new)), np.uint64))
+    tickets = gp_ffi.DeviceArray(np.zeros(4 * launches + 16, np.int32))
+    err = gp_ffi.DeviceArray(np.zeros(1, np.int32))
     A = dlp.A.g
     launch_no = [0]
+    ck = gp_ffi.hip_check
 
-    def compact():
+    def rank():
+        ck(L.gpk_build_rank(d_cm.ptr, n, d_rank.ptr, stream.ptr), "build rank")
+
+    def cmap():
+        ck(L.gpk_compact_map(A.row_ptr, A.col_idx, m, d_rm.ptr, d_rank.ptr, jn.ptr, o_ptr.ptr, stream.ptr), "map")
+
+    def scan():
         k = launch_no[0]
         launch_no[0] += 1
         lb = Lookback(status.ptr, tickets.ptr + 4 * k, err.ptr, k + 1)
-        gp_ffi.hip_check(L.gpk_compact_lb(A.row_ptr, A.col_idx, A.val, m, d_rm.ptr, d_rank.ptr, m_new, o_ptr.ptr,
-                                          o_idx.ptr, o_val.ptr, total.ptr, lb, stream.ptr), "compact")
+        ck(L.gpk_scan_lb_i32(o_ptr.ptr, o_ptr.ptr, m_new, lb, stream.ptr), "scan")
 
-    def build_rank():
-        gp_ffi.hip_check(L.gpk_build_rank(d_cm.ptr, n, d_rank.ptr, stream.ptr), "build rank")
+    def write():
+        ck(L.gpk_compact_write(A.row_ptr, A.val, m, d_rm.ptr, jn.ptr, o_ptr.ptr, o_idx.ptr, o_val.ptr, stream.ptr),
+           "write")
 
-    build_rank()
+    def whole():
+        rank()
+        cmap()
+        scan()
+        write()
+
     for _ in range(3):
-        compact()
+        whole()
     stream.sync()
-    e0, e1, e2 = gp_ffi.Event(), gp_ffi.Event(), gp_ffi.Event()
-    e0.record(stream)
+    ev = [gp_ffi.Event() for _ in range(6)]
+    ev[0].record(stream)
     for _ in range(launches):
-        compact()
-    e1.record(stream)
-    for _ in range(launches):
-        build_rank()
-    e2.record(stream)
-    us = 1e3 * e0.elapsed_ms(e1) / launches
-    rank_us = 1e3 * e1.elapsed_ms(e2) / launches
-    if err.numpy()[0] != 0 or total.numpy()[0] != nnz_new:
-        raise SystemExit("bench.py: the compaction kernel did not produce the host's kept count")
+        whole()
+    ev[1].record(stream)
+    us = 1e3 * ev[0].elapsed_ms(ev[1]) / launches
     want_ptr = np.concatenate([[0], np.cumsum(np.bincount(rows[kept], minlength=m)[keep_r])]).astype(np.int32)
-    if not np.array_equal(o_ptr.numpy(), want_ptr):
-        raise SystemExit("bench.py: compacted row pointers differ from the host")
-    nbytes = compaction_bytes(m, n, len(lp["A_idx"]), m_new, nnz_new)
-    return {"us": us, "bytes": nbytes, "achieved": nbytes / (us * 1e-6) / 1e9, "rank_build_us": rank_us,
-            "workload": f"A of the LP ({m}x{n}, {len(lp['A_idx'])} nnz) with 15% of rows and columns dropped "
+    if (err.numpy()[0] != 0 or not np.array_equal(o_ptr.numpy(), want_ptr)
+            or not np.array_equal(o_idx.numpy()[:nnz_new], col_map[lp["A_idx"][kept]])
+            or not np.array_equal(o_val.numpy()[:nnz_new], lp["A_val"][kept])):
+        raise SystemExit("bench.py: the compaction differs from the host")
+    # Each kernel apart (the map needs the rank, the write the scanned pointers).
+    parts = {}
+    for name, f in (("k_build_rank", rank), ("k_compact_map", cmap), ("k_scan_lb", None), ("k_compact_write", write)):
+        if f is None:
+            e0, e1 = gp_ffi.Event(), gp_ffi.Event()
+            e0.record(stream)
+            for _ in range(launches):
+                cmap()
+                scan()
+            e1.record(stream)
+            parts[name] = 1e3 * e0.elapsed_ms(e1) / launches - parts["k_compact_map"]
+            continue
+        e0, e1 = gp_ffi.Event(), gp_ffi.Event()
+        e0.record(stream)
+        for _ in range(launches):
+            f()
+        e1.record(stream)
+        parts[name] = 1e3 * e0.elapsed_ms(e1) / launches
+    nbytes = compaction_bytes(m, n, nnz, m_new, nnz_new)
+    log("compaction kernels (us): " + ", ".join(f"{k} {v:.1f}" for k, v in parts.items()))
+    return {"us": us, "bytes": nbytes, "achieved": nbytes / (us * 1e-6) / 1e9, "kernels_us": parts,
+            "workload": f"A of the LP ({m}x{n}, {nnz} nnz) with 15% of rows and columns dropped "
                         f"-> {m_new} rows, {nnz_new} nnz"}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "k_compact_lb_k_row_activity_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "compaction_row_activity_hbm_traffic.md")
 
 
 def profiled_traffic(kernel):
@@ -226,6 +280,13 @@ def profiled_traffic(kernel):
             except ValueError:
                 return None
     return None
+
+
+def compaction_traffic():
+    """HBM bytes of one compaction call: the sum over its kernels in the
+    committed PMC summary, or None when one of them is missing."""
+    parts = [profiled_traffic(k) for k in ("k_build_rank", "k_compact_map", "k_scan_lb<int, 0>", "k_compact_write")]
+    return None if any(p is None for p in parts) else sum(parts)
 
 
 def _upload_pool(gp_ffi, lps):
@@ -424,11 +485,15 @@ def main():
     rec = last.record()
     red = last.reduced()
     x, y, z = lp_synth.reduced_solution(3, red["m"], red["n"], red["l"], red["u"])
-    t = time.perf_counter()
-    gp_ffi.postsolve(rec, x, y, z, original_dev=dlp)
-    post_ms = 1e3 * (time.perf_counter() - t)
+    ps = gp_ffi.PreparedPostsolve(rec, x, y, z, dlp)
+    post = []
+    for _ in range(3):
+        t = time.perf_counter()
+        ps.run()
+        post.append(1e3 * (time.perf_counter() - t))
+    post_ms = min(post)
     last.free()
-    log(f"postsolve (original LP resident; incl. copies of x/y/z in and out and the record): {post_ms:.1f} ms")
+    log(f"postsolve (inputs resident; record tape uploaded by the call): {post_ms:.1f} ms (runs {post})")
 
     act, comp = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
     cpu = None
@@ -464,15 +529,18 @@ def main():
                    "row_len": "U[2,14]", "col_popularity": "Zipf(1.5) cap 5000", "seed": args.seed,
                    "reduced": [summary["reduced_rows"], summary["reduced_cols"], summary["reduced_nnz"]],
                    "iterations": summary["iterations"], "postsolve_ms": post_ms},
-        "roofline": {"bound": "hbm", "kernel": "k_compact_lb", "achieved": comp["achieved"],
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": comp["achieved"] / HBM_PEAK_GBS,
-                     "traffic": profiled_traffic("k_compact_lb"), "us_per_launch": comp["us"],
-                     "algorithmic_bytes": comp["bytes"],
-                     "workload": comp["workload"], "k_build_rank_us": comp["rank_build_us"]},
-        "roofline_activity": {"bound": "hbm", "kernel": "k_row_activity", "achieved": act["achieved"],
-                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": act["achieved"] / HBM_PEAK_GBS,
-                              "traffic": profiled_traffic("k_row_activity"), "us_per_launch": act["us"],
-                              "algorithmic_bytes": act["bytes"], "packed_bounds_variant": act["packed"]},
+        "roofline": {"bound": "hbm",
+                     "kernel": "compaction: k_build_rank + k_compact_map + k_scan_lb + k_compact_write",
+                     "achieved": comp["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": comp["achieved"] / HBM_PEAK_GBS, "traffic": compaction_traffic(),
+                     "us_per_call": comp["us"], "algorithmic_bytes": comp["bytes"],
+                     "workload": comp["workload"], "kernels_us": comp["kernels_us"]},
+        "roofline_activity": {"bound": "hbm", "kernel": "k_pack_lu + k_row_activity_lu (one gp_row_activity call)",
+                              "achieved": act["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                              "frac": act["achieved"] / HBM_PEAK_GBS,
+                              "traffic": (None if profiled_traffic("k_row_activity_lu") is None else
+                                          profiled_traffic("k_row_activity_lu") + (profiled_traffic("k_pack_lu") or 0)),
+                              "us_per_call": act["us"], "algorithmic_bytes": act["bytes"], **act["kernels"]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -204,6 +204,12 @@ void set_batched_closure_min_nnz(int32_t nnz);
 int32_t batched_closure_min_nnz();
 
 // ---- shared host helpers (host_common.cpp) ---------------------------------
+// Row activity (gpu_presolve_kernels.cu:18-111) on `stream`: the bounds are
+// first interleaved into a pool block (one 16-byte gather per entry), then
+// k_row_activity_lu runs; enqueue only.
+void row_activity(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m, int32_t n, const double* l,
+                  const double* u, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf, int32_t* pos_inf,
+                  hipStream_t stream);
 // Exclusive scan of a uint8 mask into out[0..n]; returns the count.
 int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out);
 // Exclusive scan of int32 counts into out[0..n]; returns the total.

@@ -288,9 +288,8 @@ gp_status gp_row_activity(const gp_csr* A, const double* l, const double* u, dou
     require(A->cols == 0 || (l && u), "null bounds");
     if (stream != nullptr) {
       if (A->rows == 0) return;
-      amd::check(gpk_row_activity(A->row_ptr, A->col_idx, A->val, A->rows, l, u, zero_tol, row_min_fin, row_max_fin,
-                                  row_min_neg_inf_count, row_max_pos_inf_count, static_cast<hipStream_t>(stream)),
-                 "row activity");
+      amd::row_activity(A->row_ptr, A->col_idx, A->val, A->rows, A->cols, l, u, zero_tol, row_min_fin, row_max_fin,
+                        row_min_neg_inf_count, row_max_pos_inf_count, static_cast<hipStream_t>(stream));
       return;
     }
     require_device();

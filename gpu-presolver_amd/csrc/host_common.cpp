@@ -219,6 +219,19 @@ void dev_free(void* p, hipStream_t stream) noexcept {
   (void)hipFree(p);
 }
 
+void row_activity(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m, int32_t n, const double* l,
+                  const double* u, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf, int32_t* pos_inf,
+                  hipStream_t stream) {
+  if (m <= 0) return;
+  void* lu = dev_alloc(sizeof(double) * 2 * static_cast<size_t>(n > 0 ? n : 1), stream, "packed bounds");
+  hipError_t st = gpk_pack_lu(l, u, n, static_cast<double*>(lu), stream);
+  if (st == hipSuccess)
+    st = gpk_row_activity_lu(ptr, idx, val, m, static_cast<const double*>(lu), ztol, min_fin, max_fin, neg_inf,
+                             pos_inf, stream);
+  dev_free(lu, stream);  // stream-ordered reuse: later work on `stream` runs after the kernels
+  check(st, "row activity");
+}
+
 int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out) {
   Ctx& c = ctx();
   if (n <= 0) {
@@ -272,8 +285,10 @@ void free_csr(DeviceCsrMatrix& A) noexcept {
   A.nzVal = nullptr;
 }
 
-// compact_csr_by_masks_gpu (gpu_presolve.cu:1881-1978) in two passes: per
-// row kept counts, an exclusive scan, then a coalesced stream compaction.
+// compact_csr_by_masks_gpu (gpu_presolve.cu:1881-1978): the column map in
+// rank form, one mapping pass over the entries (per-entry new column and
+// per-row kept counts), an exclusive scan into the row pointers, then the
+// coalesced write of the kept entries.
 DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, const int32_t* col_map, int32_t m_new,
                             int32_t n_new) {
   Ctx& c = ctx();
@@ -287,18 +302,19 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
     out.rowPtr = ptr.release();
     return out;
   }
-  // One launch: kept entries are at most A.nnz, so the outputs are sized
-  // for that and the kept count is read back afterwards.
   const int32_t m_old = A.rows;
+  DBuf<gpk_rank_word> rank((static_cast<int64_t>(A.cols) + 63) / 64, "compact column rank");
+  DBuf<int32_t> jn(A.nnz, "compact entry map");
+  hipStream_t s = c.stream();
+  check(gpk_build_rank(col_map, A.cols, rank, s), "compact column rank");
+  check(gpk_compact_map(A.rowPtr, A.colVal, m_old, row_map, rank, jn, ptr, s), "compact map");
+  check(gpk_scan_lb_i32(ptr, ptr, m_new, c.lookback(gpk_lookback_tiles(m_new)), s), "compact row pointers");
+  // Kept entries are at most A.nnz: the outputs are sized for that and the
+  // kept count is read back once the entries are written.
   DBuf<int32_t> idx(A.nnz, "compact colVal");
   DBuf<double> val(A.nnz, "compact nzVal");
-  DBuf<gpk_rank_word> rank((static_cast<int64_t>(A.cols) + 63) / 64, "compact column rank");
-  check(gpk_build_rank(col_map, A.cols, rank, c.stream()), "compact column rank");
-  int32_t* total = c.flags();
-  check(gpk_compact_lb(A.rowPtr, A.colVal, A.nzVal, m_old, row_map, rank, m_new, ptr, idx, val, total,
-                       c.lookback(gpk_compact_tiles(m_old)), c.stream()),
-        "compact");
-  out.nnz = c.read_flags(1)[0];
+  check(gpk_compact_write(A.rowPtr, A.nzVal, m_old, row_map, jn, ptr, idx, val, s), "compact write");
+  out.nnz = c.scalar_to_host(ptr.get() + m_new, "compact kept count");
   out.rowPtr = ptr.release();
   out.colVal = idx.release();
   out.nzVal = val.release();

@@ -83,11 +83,42 @@ gp_orig_dev orig_view(const LPInfoGpu* o) {
 // the serial order, so the results are the serial replay's, bit for bit.
 // Variables: column c (x, z and z_retrieved) -> c, row r (y) -> n0 + r.
 
-// Pattern of the original A and AT on the host, for the dual recoveries that
-// walk original rows or columns.
-struct OrigPattern {
-  std::vector<int32_t> a_ptr, a_idx, at_ptr, at_idx;
+// Index lists of the original rows of A (columns of AT) that the dual
+// recoveries walk, gathered on the device and copied to the host: line l is
+// idx[off[slot[l]] .. off[slot[l] + 1]).
+struct Lines {
+  std::vector<int32_t> slot, off, idx;
+  const int32_t* begin(int32_t l) const { return idx.data() + off[slot[l]]; }
+  const int32_t* end(int32_t l) const { return idx.data() + off[slot[l] + 1]; }
 };
+
+struct OrigPattern {
+  Lines a_rows, at_cols;
+};
+
+Lines gather_lines(const int32_t* ptr, const int32_t* idx, int32_t lines_total, const std::vector<int32_t>& wanted) {
+  Lines L;
+  L.slot.assign(static_cast<size_t>(lines_total), -1);
+  std::vector<int32_t> list;
+  for (int32_t l : wanted)
+    if (L.slot[l] < 0) {
+      L.slot[l] = static_cast<int32_t>(list.size());
+      list.push_back(l);
+    }
+  const int32_t k = static_cast<int32_t>(list.size());
+  L.off.assign(1, 0);
+  if (k == 0) return L;
+  auto& c = ctx();
+  DBuf<int32_t> d_list(k), len(static_cast<int64_t>(k) + 1), off(static_cast<int64_t>(k) + 1);
+  c.to_device(d_list.get(), list.data(), k, "pattern line list");
+  check(gpk_seg_lengths(ptr, d_list, k, len, c.stream()), "pattern line lengths");
+  const int32_t total = amd::scan_counts(len, k, off);
+  DBuf<int32_t> out(total);
+  check(gpk_seg_copy(ptr, idx, d_list, k, off, out, c.stream()), "pattern line copy");
+  L.off = c.to_host(off.get(), static_cast<int64_t>(k) + 1, "pattern line offsets");
+  L.idx = c.to_host(out.get(), total, "pattern lines");
+  return L;
+}
 
 // Levels [first, first + count): one grid launch (a single large level) or
 // one workgroup that steps through a run of smaller levels.
@@ -152,8 +183,8 @@ class TapeScanner {
             need(ni >= 4 + I[i0 + 3] && nv >= 12 + I[i0 + 3]);
             for (int32_t t = 0; t < I[i0 + 3]; ++t) rd.push_back(row_var(k, I[i0 + 4 + t]));
           } else {
-            for (int32_t q = op->at_ptr[ec]; q < op->at_ptr[ec + 1]; ++q)
-              if (op->at_idx[q] != row) rd.push_back(row_var(k, op->at_idx[q]));
+            for (const int32_t* q = op->at_cols.begin(ec); q != op->at_cols.end(ec); ++q)
+              if (*q != row) rd.push_back(row_var(k, *q));
           }
           wr.push_back(rv);
           wr.push_back(kc);
@@ -240,8 +271,8 @@ class TapeScanner {
         const int32_t row = I[i0], c = col(k, I[i0 + 1]);
         rd.push_back(c);
         if (has_y) {
-          for (int32_t q = op->at_ptr[c]; q < op->at_ptr[c + 1]; ++q)
-            if (op->at_idx[q] != row) rd.push_back(row_var(k, op->at_idx[q]));
+          for (const int32_t* q = op->at_cols.begin(c); q != op->at_cols.end(c); ++q)
+            if (*q != row) rd.push_back(row_var(k, *q));
           wr.push_back(row_var(k, row));
         }
         wr.push_back(c);
@@ -259,9 +290,9 @@ class TapeScanner {
           wr.push_back(rv);
         }
         const int32_t r = rv - n0_;
-        for (int32_t q = op->a_ptr[r]; q < op->a_ptr[r + 1]; ++q) {
-          rd.push_back(col(k, op->a_idx[q]));
-          wr.push_back(op->a_idx[q]);
+        for (const int32_t* q = op->a_rows.begin(r); q != op->a_rows.end(r); ++q) {
+          rd.push_back(col(k, *q));
+          wr.push_back(*q);
         }
         return true;
       }
@@ -270,18 +301,23 @@ class TapeScanner {
     }
   }
 
-  // True when the dual pass needs the original pattern on the host.
-  bool dual_needs_pattern() const {
-    for (size_t k = 0; k < t_.types.size(); ++k) {
+  // Original rows (type 12) and columns (types 5 and 13 without a row
+  // payload) whose index lists the dual pass walks.
+  void pattern_lines(std::vector<int32_t>& rows, std::vector<int32_t>& cols) const {
+    for (int32_t k = 0; k < static_cast<int32_t>(t_.types.size()); ++k) {
       const int32_t type = t_.types[k];
-      if (type == 5 || type == 12) return true;
-      if (type == 13) {
-        const int32_t ni = t_.index_starts[k + 1] - t_.index_starts[k];
-        const int32_t nv = t_.value_starts[k + 1] - t_.value_starts[k];
-        if (!(ni >= 4 && nv >= 12)) return true;
+      const int32_t i0 = t_.index_starts[k];
+      const int32_t ni = t_.index_starts[k + 1] - i0;
+      const int32_t nv = t_.value_starts[k + 1] - t_.value_starts[k];
+      if (type == 12 && ni >= 2) {
+        row_var(k, t_.indices[i0 + 1]);
+        rows.push_back(t_.indices[i0 + 1]);
+      } else if (type == 5 && ni >= 2) {
+        cols.push_back(col(k, t_.indices[i0 + 1]));
+      } else if (type == 13 && ni >= 3 && !(ni >= 4 && nv >= 12)) {
+        cols.push_back(col(k, t_.indices[i0]));
       }
     }
-    return false;
   }
 
  private:
@@ -436,12 +472,12 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
     const bool has_orig = original_model != nullptr;
     const bool has_y = y != nullptr;
     OrigPattern pattern;
-    if (has_orig && scanner.dual_needs_pattern()) {
+    if (has_orig) {
       const LPInfoGpu& o = *original_model;
-      pattern.a_ptr = c.to_host(o.A.rowPtr, static_cast<int64_t>(m0) + 1, "original A pattern");
-      pattern.a_idx = c.to_host(o.A.colVal, o.A.nnz, "original A pattern");
-      pattern.at_ptr = c.to_host(o.AT.rowPtr, static_cast<int64_t>(n0) + 1, "original AT pattern");
-      pattern.at_idx = c.to_host(o.AT.colVal, o.AT.nnz, "original AT pattern");
+      std::vector<int32_t> rows, cols;
+      scanner.pattern_lines(rows, cols);
+      pattern.a_rows = gather_lines(o.A.rowPtr, o.A.colVal, m0, rows);
+      pattern.at_cols = gather_lines(o.AT.rowPtr, o.AT.colVal, n0, cols);
     }
     const int32_t count = tape.view.count;
     run_schedule(level_schedule(scanner, count, n0, m0, true, has_orig, has_y, &pattern), tape.view, orig, res.x_org,

@@ -34,6 +34,7 @@ using amd::ctx;
 using amd::DBuf;
 using amd::csr_view;
 using amd::plan_view;
+using amd::row_activity;
 using amd::append_record;
 using RT = PostsolveReductionType;
 using DM = PostsolveDualMode;
@@ -70,9 +71,8 @@ void compute_row_activity_summary(double* row_min_fin, double* row_max_fin, std:
                                   std::int32_t* row_max_pos_inf_count, const DeviceCsrMatrix& A_csr, const double* l,
                                   const double* u, double zero_tol) {
   if (A_csr.rows == 0) return;
-  check(gpk_row_activity(A_csr.rowPtr, A_csr.colVal, A_csr.nzVal, A_csr.rows, l, u, zero_tol, row_min_fin,
-                         row_max_fin, row_min_neg_inf_count, row_max_pos_inf_count, ctx().stream()),
-        "row activity");
+  row_activity(A_csr.rowPtr, A_csr.colVal, A_csr.nzVal, A_csr.rows, A_csr.cols, l, u, zero_tol, row_min_fin,
+               row_max_fin, row_min_neg_inf_count, row_max_pos_inf_count, ctx().stream());
 }
 
 // gpu_presolve_kernels.cu:113-124
@@ -201,8 +201,7 @@ void apply_rule_implied_variable_bounds(PresolvePlanGpu& plan, const LPInfoGpu& 
   const gpk_csr A = csr_view(lp.A), AT = csr_view(lp.AT);
   DBuf<double> rmin(m), rmax(m);
   DBuf<int32_t> rni(m), rpi(m);
-  check(gpk_row_activity(A.ptr, A.idx, A.val, m, plan.new_l, plan.new_u, prm.zero_tol, rmin, rmax, rni, rpi, s),
-        "implied bounds activity");
+  row_activity(A.ptr, A.idx, A.val, m, n, plan.new_l, plan.new_u, prm.zero_tol, rmin, rmax, rni, rpi, s);
   DBuf<double> cl(n), cu(n), vmax(n);
   check(gpk_implied_candidates(AT, P, stats.row_nnz, rmin, rmax, rni, rpi, prm.zero_tol, cl, cu, s),
         "implied bounds candidates");

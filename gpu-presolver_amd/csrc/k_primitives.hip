@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <climits>
 #include <cstdint>
 
 #include "gp_device.hpp"
@@ -96,82 +97,14 @@ __global__ void k_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t
 // -------------------------------------------------------- row activity ----
 // One thread per row, 256 consecutive rows per block.  The block's CSR
 // segment is streamed through LDS in windows: all threads cooperatively load
-// coalesced (col, a) entries and the gathered l/u, form the min/max terms and
-// stage them; each thread then adds its own row's terms in CSR order, so every
-// row sum has exactly the sequential rounding of the reference kernel
+// coalesced (col, a) entries, then gather the packed (l, u) pairs (all loads
+// of a window issued before any is used), form the min/max terms and stage
+// them; each thread then adds its own row's terms in CSR order, so every row
+// sum has exactly the sequential rounding of the reference kernel
 // (gpu_presolve_kernels.cu:18-66) for any row length.
 constexpr int kActRows = kBlock;
 constexpr int kActCap = 2048;
 constexpr int kActItems = kActCap / kBlock;
-
-__global__ __launch_bounds__(kBlock) void k_row_activity(const int32_t* __restrict__ row_ptr,
-                                                         const int32_t* __restrict__ col_idx,
-                                                         const double* __restrict__ val, int32_t m,
-                                                         const double* __restrict__ l,
-                                                         const double* __restrict__ u, double ztol,
-                                                         double* __restrict__ min_fin,
-                                                         double* __restrict__ max_fin,
-                                                         int32_t* __restrict__ neg_inf,
-                                                         int32_t* __restrict__ pos_inf) {
-  __shared__ double s_tmin[kActCap];
-  __shared__ double s_tmax[kActCap];
-  __shared__ int32_t s_ptr[kActRows + 1];
-  const int32_t r0 = blockIdx.x * kActRows;
-  const int32_t rows = min(kActRows, m - r0);
-  const int tid = threadIdx.x;
-  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = row_ptr[r0 + t];
-  __syncthreads();
-  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
-  const int32_t my_b = tid < rows ? s_ptr[tid] : 0;
-  const int32_t my_e = tid < rows ? s_ptr[tid + 1] : 0;
-  double mn = 0.0, mx = 0.0;
-  int32_t ni = 0, pi = 0;
-  for (int32_t w = seg_b; w < seg_e; w += kActCap) {
-    const int32_t we = min(w + kActCap, seg_e);
-    // All loads of the window first (coalesced val/col_idx, then the l/u
-    // gathers), so each lane keeps kActItems * 2 gathers in flight.
-    double a[kActItems], lj[kActItems], uj[kActItems];
-    int32_t j[kActItems];
-#pragma unroll
-    for (int k = 0; k < kActItems; ++k) {
-      const int32_t p = w + tid + k * kBlock;
-      a[k] = p < we ? val[p] : 0.0;
-      j[k] = p < we ? col_idx[p] : 0;
-    }
-#pragma unroll
-    for (int k = 0; k < kActItems; ++k) {
-      const bool use = fabs(a[k]) > ztol;
-      lj[k] = use ? l[j[k]] : 0.0;
-      uj[k] = use ? u[j[k]] : 0.0;
-    }
-#pragma unroll
-    for (int k = 0; k < kActItems; ++k) {
-      const int32_t p = w + tid + k * kBlock;
-      if (p >= we) break;
-      double tmn = 0.0, tmx = 0.0;
-      if (fabs(a[k]) > ztol) {
-        tmn = a[k] >= 0.0 ? a[k] * lj[k] : a[k] * uj[k];
-        tmx = a[k] >= 0.0 ? a[k] * uj[k] : a[k] * lj[k];
-      }
-      s_tmin[p - w] = tmn;
-      s_tmax[p - w] = tmx;
-    }
-    __syncthreads();
-    const int32_t b = max(my_b, w), e = min(my_e, we);
-    for (int32_t p = b; p < e; ++p) {
-      const double tmn = s_tmin[p - w], tmx = s_tmax[p - w];
-      if (isfinite(tmn)) mn += tmn; else if (tmn < 0.0) ++ni;
-      if (isfinite(tmx)) mx += tmx; else if (tmx > 0.0) ++pi;
-    }
-    __syncthreads();
-  }
-  if (tid < rows) {
-    min_fin[r0 + tid] = mn;
-    max_fin[r0 + tid] = mx;
-    neg_inf[r0 + tid] = ni;
-    pos_inf[r0 + tid] = pi;
-  }
-}
 
 // (l, u) interleaved, so an activity term needs one 16-byte gather.
 __global__ void k_pack_lu(const double* __restrict__ l, const double* __restrict__ u, int32_t n,
@@ -180,7 +113,20 @@ __global__ void k_pack_lu(const double* __restrict__ l, const double* __restrict
   if (j < n) lu[j] = make_double2(l[j], u[j]);
 }
 
-// k_row_activity over packed bounds.
+template <bool kStream>
+__device__ inline double act_load(const double* p) {
+  if constexpr (kStream) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool kStream>
+__device__ inline int32_t act_load(const int32_t* p) {
+  if constexpr (kStream) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+
+// kStream: the once-read CSR entries are loaded with the non-temporal hint,
+// so they do not displace the reused (l, u) lines from the caches.
+template <bool kStream>
 __global__ __launch_bounds__(kBlock) void k_row_activity_lu(const int32_t* __restrict__ row_ptr,
                                                             const int32_t* __restrict__ col_idx,
                                                             const double* __restrict__ val, int32_t m,
@@ -207,14 +153,20 @@ __global__ __launch_bounds__(kBlock) void k_row_activity_lu(const int32_t* __res
     double a[kActItems];
     double2 b[kActItems];
     int32_t j[kActItems];
+    // Every load is issued unconditionally (positions past the window read
+    // its last entry and are discarded), so the compiler emits the 16 loads
+    // back to back and then the 8 gathers, with no wait in between.
 #pragma unroll
     for (int k = 0; k < kActItems; ++k) {
-      const int32_t p = w + tid + k * kBlock;
-      a[k] = p < we ? val[p] : 0.0;
-      j[k] = p < we ? col_idx[p] : 0;
+      const int32_t pc = min(w + tid + k * kBlock, we - 1);
+      a[k] = act_load<kStream>(val + pc);
+      j[k] = act_load<kStream>(col_idx + pc);
     }
 #pragma unroll
-    for (int k = 0; k < kActItems; ++k) b[k] = fabs(a[k]) > ztol ? lu[j[k]] : make_double2(0.0, 0.0);
+    for (int k = 0; k < kActItems; ++k) {
+      if (w + tid + k * kBlock >= we) a[k] = 0.0;
+      b[k] = lu[fabs(a[k]) > ztol ? j[k] : 0];
+    }
 #pragma unroll
     for (int k = 0; k < kActItems; ++k) {
       const int32_t p = w + tid + k * kBlock;
@@ -267,10 +219,9 @@ __global__ void k_csr_stats(const int32_t* ptr, const int32_t* idx, const double
 }
 
 // ---------------------------------------------------------- compaction ----
-// 256 old rows per block; their entry segment is processed in chunks of
-// kCompChunk entries, each thread owning kCompItems consecutive entries so a
-// wave touches one contiguous 2 KB span.  The row of an entry is found by a
-// binary search over the block's 257 row pointers in LDS.
+// A tile's entry segment is processed in chunks of kCompChunk entries,
+// striped: entry c + 256k + t goes to thread t, so every load instruction of
+// a wave reads 64 consecutive entries.
 constexpr int kCompItems = 8;
 constexpr int kCompChunk = kBlock * kCompItems;
 
@@ -283,94 +234,6 @@ __device__ inline int find_row(const int32_t* s_ptr, int rows, int32_t p) {
   return lo;
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact_count(const int32_t* __restrict__ ptr,
-                                                          const int32_t* __restrict__ idx, int32_t m_old,
-                                                          const int32_t* __restrict__ row_map,
-                                                          const int32_t* __restrict__ col_map,
-                                                          int32_t* __restrict__ counts) {
-  __shared__ int32_t s_ptr[kBlock + 1];
-  __shared__ int32_t s_cnt[kBlock];
-  __shared__ uint8_t s_keep[kBlock];
-  const int32_t r0 = blockIdx.x * kBlock;
-  const int rows = min(kBlock, m_old - r0);
-  const int tid = threadIdx.x;
-  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = ptr[r0 + t];
-  s_cnt[tid] = 0;
-  s_keep[tid] = (tid < rows && row_map[r0 + tid] >= 0) ? 1 : 0;
-  __syncthreads();
-  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
-  for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
-    const int32_t p0 = c + tid * kCompItems;
-    int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
-    int local = 0;
-    for (int k = 0; k < kCompItems; ++k) {
-      const int32_t p = p0 + k;
-      if (p >= seg_e) break;
-      while (s_ptr[row + 1] <= p) {
-        if (local) { atomicAdd(&s_cnt[row], local); local = 0; }
-        ++row;
-      }
-      if (s_keep[row] && col_map[idx[p]] >= 0) ++local;
-    }
-    if (local) atomicAdd(&s_cnt[row], local);
-  }
-  __syncthreads();
-  if (tid < rows) counts[r0 + tid] = s_keep[tid] ? s_cnt[tid] : 0;
-}
-
-__global__ __launch_bounds__(kBlock) void k_compact_copy(const int32_t* __restrict__ ptr,
-                                                         const int32_t* __restrict__ idx,
-                                                         const double* __restrict__ val, int32_t m_old,
-                                                         const int32_t* __restrict__ row_map,
-                                                         const int32_t* __restrict__ col_map,
-                                                         const int32_t* __restrict__ new_ptr_by_old,
-                                                         int32_t* __restrict__ out_ptr,
-                                                         int32_t* __restrict__ out_idx,
-                                                         double* __restrict__ out_val) {
-  __shared__ int32_t s_ptr[kBlock + 1];
-  __shared__ uint8_t s_keep[kBlock];
-  const int32_t r0 = blockIdx.x * kBlock;
-  const int rows = min(kBlock, m_old - r0);
-  const int tid = threadIdx.x;
-  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = ptr[r0 + t];
-  int32_t rm = -1;
-  if (tid < rows) {
-    rm = row_map[r0 + tid];
-    if (rm >= 0) out_ptr[rm] = new_ptr_by_old[r0 + tid];
-  }
-  s_keep[tid] = rm >= 0 ? 1 : 0;
-  __syncthreads();
-  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
-  int32_t write_base = new_ptr_by_old[r0];
-  for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
-    const int32_t p0 = c + tid * kCompItems;
-    int32_t jn[kCompItems];
-    int32_t cnt = 0;
-    int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
-#pragma unroll
-    for (int k = 0; k < kCompItems; ++k) {
-      const int32_t p = p0 + k;
-      jn[k] = -1;
-      if (p < seg_e) {
-        while (s_ptr[row + 1] <= p) ++row;
-        if (s_keep[row]) jn[k] = col_map[idx[p]];
-        cnt += jn[k] >= 0 ? 1 : 0;
-      }
-    }
-    int32_t total;
-    int32_t pos = write_base + block_exclusive_sum(cnt, &total);
-#pragma unroll
-    for (int k = 0; k < kCompItems; ++k) {
-      if (jn[k] >= 0) {
-        out_idx[pos] = jn[k];
-        out_val[pos] = val[p0 + k];
-        ++pos;
-      }
-    }
-    write_base += total;
-  }
-}
-
 // ------------------------------------------------ decoupled look-back ----
 constexpr uint64_t kLbSpinLimit = 1ull << 20;
 
@@ -380,16 +243,37 @@ __device__ inline uint64_t lb_word(uint32_t epoch, uint32_t flag, int32_t value)
 }
 
 // Called by the first wave of the tile (all 64 lanes).  Publishes the tile's
-// aggregate, then looks back over 64 predecessors at a time: every lane waits
-// for its predecessor's word, the nearest inclusive prefix in the window ends
-// the walk, otherwise the window's aggregates are summed and the walk moves
-// 64 tiles further back.  Publishes the tile's inclusive prefix and returns
-// its exclusive prefix (in every lane).  Tile ids are tickets taken in
-// dispatch order, so every predecessor is already running and each wait
-// ends; the spin limit only guards a broken invariant (*error is set and the
-// result is meaningless).  The status words carry all the data a reader
-// needs, so relaxed device-scope atomics suffice: acquire/release would add
-// cache invalidation / write-back on every poll.
+// aggregate, then looks back over kLbWindow predecessors at a time, lane l
+// watching kLbPerLane consecutive ones (nearest first): every lane waits for
+// its predecessors' words, the nearest inclusive prefix in the window ends the
+// walk, otherwise the window's aggregates are summed and the walk moves a
+// window further back.  A wide window matters on this GPU: a status word
+// written on one XCD is read through memory by the others, so each step
+// costs a memory round trip, and the walk's depth is tiles / window.
+// Publishes the tile's inclusive prefix and returns its exclusive prefix (in
+// every lane).  Tile ids are tickets taken in dispatch order, so every
+// predecessor is already running and each wait ends; the spin limit only
+// guards a broken invariant (*error is set and the result is meaningless).
+// The status words carry all the data a reader needs, so relaxed
+// device-scope atomics suffice: acquire/release would add cache
+// invalidation / write-back on every poll.
+constexpr int kLbPerLane = 4;
+constexpr int kLbWindow = 64 * kLbPerLane;
+
+__device__ inline uint64_t lb_wait(const gpk_lookback& lb, int32_t j) {
+  if (j < 0) return lb_word(lb.epoch, 2u, 0);  // before tile 0: an inclusive prefix of 0
+  uint64_t spins = 0;
+  while (true) {
+    const uint64_t w = __hip_atomic_load(lb.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if ((w >> 34) == lb.epoch && ((w >> 32) & 3u) != 0) return w;
+    if (++spins > kLbSpinLimit) {
+      __hip_atomic_store(lb.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+      return lb_word(lb.epoch, 2u, 0);
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 __device__ int32_t lb_exclusive(const gpk_lookback& lb, int32_t tile, int32_t aggregate) {
   const int lane = lane_id();
   if (lane == 0)
@@ -397,26 +281,23 @@ __device__ int32_t lb_exclusive(const gpk_lookback& lb, int32_t tile, int32_t ag
                        __HIP_MEMORY_SCOPE_AGENT);
   if (tile == 0) return 0;
   int32_t excl = 0;
-  for (int32_t j0 = tile - 1;; j0 -= 64) {
-    const int32_t j = j0 - lane;
-    uint64_t w = lb_word(lb.epoch, 2u, 0);  // before tile 0: an inclusive prefix of 0
-    if (j >= 0) {
-      uint64_t spins = 0;
-      while (true) {
-        w = __hip_atomic_load(lb.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if ((w >> 34) == lb.epoch && ((w >> 32) & 3u) != 0) break;
-        if (++spins > kLbSpinLimit) {
-          __hip_atomic_store(lb.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-          w = lb_word(lb.epoch, 2u, 0);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
-      }
-    }
-    const bool inclusive = ((w >> 32) & 3u) == 2u;
-    const uint64_t incl = __ballot(inclusive);
+  for (int32_t j0 = tile - 1;; j0 -= kLbWindow) {
+    uint64_t w[kLbPerLane];
+#pragma unroll
+    for (int q = 0; q < kLbPerLane; ++q) w[q] = lb_wait(lb, j0 - lane * kLbPerLane - q);
+    int q_incl = kLbPerLane;  // this lane's nearest inclusive prefix
+#pragma unroll
+    for (int q = kLbPerLane - 1; q >= 0; --q)
+      if (((w[q] >> 32) & 3u) == 2u) q_incl = q;
+    const uint64_t incl = __ballot(q_incl < kLbPerLane);
     const int first = incl ? __ffsll(static_cast<unsigned long long>(incl)) - 1 : 64;
-    const int32_t v = lane <= first ? static_cast<int32_t>(static_cast<uint32_t>(w)) : 0;
+    int32_t v = 0;
+    if (lane <= first) {
+      const int last_q = lane < first ? kLbPerLane - 1 : q_incl;
+#pragma unroll
+      for (int q = 0; q < kLbPerLane; ++q)
+        if (q <= last_q) v += static_cast<int32_t>(static_cast<uint32_t>(w[q]));
+    }
     excl += __shfl(wave_inclusive_sum(v), 63, 64);
     if (incl) break;
   }
@@ -439,48 +320,63 @@ __device__ inline int32_t lb_ticket(const gpk_lookback& lb) {
 template <class In, int kMode>
 __global__ __launch_bounds__(kBlock) void k_scan_lb(const In* in, int32_t* out, int64_t n, int32_t* count,
                                                     gpk_lookback lb) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int32_t s_wbase[kScanItems * kWaves];
+  __shared__ int32_t s_excl;
   const int32_t tile = lb_ticket(lb);
   if (tile >= static_cast<int32_t>(gridDim.x)) {  // the ticket word was not zeroed
     if (threadIdx.x == 0) __hip_atomic_store(lb.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
-  const int64_t base = static_cast<int64_t>(tile) * kScanTile + threadIdx.x * kScanItems;
-  int32_t v[kScanItems];
-  int32_t s = 0;
+  // Striped items: element base + 256k + t belongs to thread t, so every load
+  // and store instruction of a wave covers 64 consecutive elements.  Each
+  // slice k is scanned across the wave with shuffles; the (slice, wave)
+  // totals, in element order, give each wave-slice its offset in the tile.
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  const int64_t base = static_cast<int64_t>(tile) * kScanTile + tid;
+  int32_t v[kScanItems], inc[kScanItems];
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    v[k] = base + k < n ? scan_value(in, base + k) : 0;
-    s += v[k];
+    const int64_t i = base + k * kBlock;
+    v[k] = i < n ? scan_value(in, i) : 0;
   }
-  int32_t total;
-  const int32_t ex = block_exclusive_sum(s, &total);
-  __shared__ int32_t s_excl;
-  if (threadIdx.x < 64) {
-    const int32_t e = lb_exclusive(lb, tile, total);
-    if (threadIdx.x == 0) s_excl = e;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    inc[k] = wave_inclusive_sum(v[k]);
+    if (lane == 63) s_wbase[k * kWaves + wave] = inc[k];
   }
   __syncthreads();
-  int32_t run = s_excl + ex;
+  if (tid < 64) {
+    const int32_t c = tid < kScanItems * kWaves ? s_wbase[tid] : 0;
+    const int32_t ci = wave_inclusive_sum(c);
+    if (tid < kScanItems * kWaves) s_wbase[tid] = ci - c;
+    const int32_t total = __shfl(ci, 63, 64);
+    const int32_t e = lb_exclusive(lb, tile, total);
+    if (tid == 0) s_excl = e;
+    const int64_t tiles = (n + kScanTile - 1) / kScanTile;
+    if (tid == 0 && tile == tiles - 1) {
+      if (kMode == 1) *count = e + total;
+      else out[n] = e + total;
+    }
+  }
+  __syncthreads();
+  const int32_t excl = s_excl;
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
-    if (base + k < n) out[base + k] = kMode == 1 ? (v[k] ? run : -1) : run;
-    run += v[k];
-  }
-  const int64_t tiles = (n + kScanTile - 1) / kScanTile;
-  if (threadIdx.x == 0 && tile == tiles - 1) {
-    if (kMode == 1) *count = s_excl + total;
-    else out[n] = s_excl + total;
+    const int64_t i = base + k * kBlock;
+    const int32_t prefix = excl + s_wbase[k * kWaves + wave] + inc[k] - v[k];
+    if (i < n) out[i] = kMode == 1 ? (v[k] ? prefix : -1) : prefix;
   }
 }
 
-// Fused stable compaction.  Tile = 1024 old rows (4 per thread), so a large
-// matrix has few enough tiles that the look-back stays short.  Pass 1 maps
-// every entry of the tile's segment (row kept, column map) and counts kept
-// entries per row in LDS; the tile's kept total goes through the look-back;
-// pass 2 maps the entries again, stages each 2048-entry chunk of kept
-// (column, value) pairs in LDS and writes it with coalesced stores.
-constexpr int kCompRows = 1024;
-constexpr int kCompRowsPerThread = kCompRows / kBlock;
+// Compaction in three launches (gpu_presolve.cu:1881-1978): k_compact_map
+// maps every entry once (row kept and column rank, one cache-resident rank
+// gather per entry), stores the new column index or -1 per entry and each
+// kept row's kept count at out_ptr[new row]; an exclusive scan makes out_ptr
+// the new row pointers; k_compact_write streams the mapped indices and the
+// values and writes each tile's kept entries, positioned by wave ballots and
+// staged in LDS for coalesced stores.  Tiles are 256 old rows.
+constexpr int kCompRows = kBlock;
 
 // Rank form of a keep map (map[j] = new index or -1), one 16-byte word per 64
 // entries: bit b of word w is set when entry 64w + b is kept, and base is the
@@ -501,142 +397,151 @@ __global__ void k_build_rank(const int32_t* __restrict__ map, int32_t n, gpk_ran
   }
 }
 
-__device__ inline int32_t rank_lookup(const gpk_rank_word* __restrict__ rank, int32_t j) {
-  const gpk_rank_word w = rank[j >> 6];
+// One 16-byte load of the word, then the rank; -1 when entry j is dropped.
+__device__ inline uint4 rank_word(const gpk_rank_word* __restrict__ rank, int32_t j) {
+  return *reinterpret_cast<const uint4*>(rank + (j >> 6));
+}
+__device__ inline int32_t rank_of(uint4 w, int32_t j) {
+  const uint64_t bits = static_cast<uint64_t>(w.x) | (static_cast<uint64_t>(w.y) << 32);
   const uint64_t bit = 1ull << (j & 63);
-  return (w.bits & bit) ? w.base + static_cast<int32_t>(__popcll(w.bits & (bit - 1))) : -1;
+  const int32_t r = static_cast<int32_t>(w.z) + static_cast<int32_t>(__popcll(bits & (bit - 1)));
+  return (bits & bit) ? r : -1;
 }
 
-// Maps the thread's kCompItems consecutive entries p0.. of the tile segment:
-// rr[k] = tile-local row (-1 past the segment), jn[k] = new column index or
-// -1 when the row or the column is dropped.  The column indices are loaded
-// first and the map gathers issued together, so they overlap.
-__device__ inline void comp_map_items(const int32_t* s_ptr, const uint8_t* s_keep, int rows, int32_t seg_e,
-                                      int32_t p0, const int32_t* __restrict__ idx,
-                                      const gpk_rank_word* __restrict__ col_rank, int32_t* jn, int* rr) {
-  int32_t cj[kCompItems];
-#pragma unroll
-  for (int k = 0; k < kCompItems; ++k) cj[k] = p0 + k < seg_e ? idx[p0 + k] : 0;
-  int row = p0 < seg_e ? find_row(s_ptr, rows, p0) : 0;
-#pragma unroll
-  for (int k = 0; k < kCompItems; ++k) {
-    const int32_t p = p0 + k;
-    rr[k] = -1;
-    if (p < seg_e) {
-      while (s_ptr[row + 1] <= p) ++row;
-      rr[k] = row;
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < kCompItems; ++k) jn[k] = (rr[k] >= 0 && s_keep[rr[k]]) ? rank_lookup(col_rank, cj[k]) : -1;
+// Per-row kept counts of one striped item per lane: the lanes of a wave that
+// hold entries of the same row are consecutive, so the first lane of each
+// run adds the run's kept count with one LDS atomic.
+__device__ inline void comp_count_run(int row, bool valid, bool kept, int32_t* s_cnt) {
+  const int lane = lane_id();
+  const uint64_t kb = __ballot(kept);
+  const int prev = __shfl_up(row, 1, 64);
+  const bool head = valid && (lane == 0 || prev != row);
+  const uint64_t hb = __ballot(head);
+  if (!head) return;
+  const uint64_t after = lane == 63 ? 0ull : (hb & (~0ull << (lane + 1)));
+  const int next = after ? __ffsll(static_cast<unsigned long long>(after)) - 1 : 64;
+  const uint64_t run = (next == 64 ? ~0ull : ((1ull << next) - 1)) & ~lanemask_lt(lane);
+  const int c = static_cast<int>(__popcll(kb & run));
+  if (c) atomicAdd(&s_cnt[row], c);
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact_lb(const int32_t* __restrict__ ptr,
-                                                       const int32_t* __restrict__ idx,
-                                                       const double* __restrict__ val, int32_t m_old,
-                                                       const int32_t* __restrict__ row_map,
-                                                       const gpk_rank_word* __restrict__ col_rank, int32_t m_new,
-                                                       int32_t* __restrict__ out_ptr, int32_t* __restrict__ out_idx,
-                                                       double* __restrict__ out_val, int32_t* __restrict__ total_out,
-                                                       gpk_lookback lb) {
-  __shared__ int32_t s_ptr[kCompRows + 1];
+__global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ idx, int32_t m_old,
+                                                        const int32_t* __restrict__ row_map,
+                                                        const gpk_rank_word* __restrict__ col_rank,
+                                                        int32_t* __restrict__ jn_out, int32_t* __restrict__ out_ptr) {
   __shared__ int32_t s_cnt[kCompRows];
   __shared__ uint8_t s_keep[kCompRows];
-  __shared__ int32_t s_oidx[kCompChunk];
-  __shared__ double s_oval[kCompChunk];
-  __shared__ int32_t s_excl;
-  const int32_t tile = lb_ticket(lb);
-  if (tile >= static_cast<int32_t>(gridDim.x)) {  // the ticket word was not zeroed
-    if (threadIdx.x == 0) __hip_atomic_store(lb.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    return;
-  }
-  const int32_t r0 = tile * kCompRows;
+  __shared__ uint8_t s_row[kCompChunk];  // tile-local row of each entry of the chunk
+  static_assert(kCompRows <= 256, "tile-local rows are stored in one byte");
+  const int32_t r0 = blockIdx.x * kCompRows;
   const int rows = min(kCompRows, m_old - r0);
   const int tid = threadIdx.x;
-  for (int t = tid; t <= rows; t += kBlock) s_ptr[t] = ptr[r0 + t];
-  for (int t = tid; t < kCompRows; t += kBlock) {
-    s_cnt[t] = 0;
-    s_keep[t] = (t < rows && row_map[r0 + t] >= 0) ? 1 : 0;
-  }
-  __syncthreads();
-  const int32_t seg_b = s_ptr[0], seg_e = s_ptr[rows];
-  // pass 1: per-row kept counts
+  const int32_t my_new_row = tid < rows ? row_map[r0 + tid] : -1;
+  const int32_t my_b = tid < rows ? ptr[r0 + tid] : 0;
+  const int32_t my_e = tid < rows ? ptr[r0 + tid + 1] : 0;
+  s_cnt[tid] = 0;
+  s_keep[tid] = my_new_row >= 0 ? 1 : 0;
+  const int32_t seg_b = ptr[r0], seg_e = ptr[r0 + rows];
   for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
-    int32_t jn[kCompItems];
+    // Each thread labels its row's entries of the chunk; then rows and
+    // liveness come from LDS, and every load goes out at once (a register a
+    // load is still writing is never assigned).
+    for (int32_t q = max(my_b, c); q < min(my_e, c + kCompChunk); ++q) s_row[q - c] = static_cast<uint8_t>(tid);
+    __syncthreads();
     int rr[kCompItems];
-    comp_map_items(s_ptr, s_keep, rows, seg_e, c + tid * kCompItems, idx, col_rank, jn, rr);
-    int local = 0, cur = rr[0];
+    bool live[kCompItems];
 #pragma unroll
     for (int k = 0; k < kCompItems; ++k) {
-      if (rr[k] < 0) break;
-      if (rr[k] != cur) {
-        if (local) atomicAdd(&s_cnt[cur], local);
-        local = 0;
-        cur = rr[k];
-      }
-      local += jn[k] >= 0 ? 1 : 0;
+      const int32_t p = c + k * kBlock + tid;
+      rr[k] = p < seg_e ? static_cast<int>(s_row[p - c]) : -1;
+      live[k] = rr[k] >= 0 && s_keep[rr[k]] != 0;
     }
-    if (local) atomicAdd(&s_cnt[cur], local);
-  }
-  __syncthreads();
-  // row offsets: thread t owns rows 4t .. 4t+3
-  int32_t mine[kCompRowsPerThread];
-  int32_t msum = 0;
+    int32_t cj[kCompItems];
 #pragma unroll
-  for (int k = 0; k < kCompRowsPerThread; ++k) {
-    const int r = tid * kCompRowsPerThread + k;
-    mine[k] = s_keep[r] ? s_cnt[r] : 0;
-    msum += mine[k];
-  }
-  int32_t agg;
-  int32_t off = block_exclusive_sum(msum, &agg);
-  if (tid < 64) {
-    const int32_t e = lb_exclusive(lb, tile, agg);
-    if (tid == 0) s_excl = e;
-  }
-  __syncthreads();
-  const int32_t excl = s_excl;
+    for (int k = 0; k < kCompItems; ++k) cj[k] = idx[min(c + k * kBlock + tid, seg_e - 1)];
+    uint4 w[kCompItems];
 #pragma unroll
-  for (int k = 0; k < kCompRowsPerThread; ++k) {
-    const int r = tid * kCompRowsPerThread + k;
-    if (r < rows && s_keep[r]) out_ptr[row_map[r0 + r]] = excl + off;
-    off += mine[k];
+    for (int k = 0; k < kCompItems; ++k) w[k] = rank_word(col_rank, live[k] ? cj[k] : 0);
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      const int32_t p = c + k * kBlock + tid;
+      const int32_t jn = live[k] ? rank_of(w[k], cj[k]) : -1;
+      if (p < seg_e) jn_out[p] = jn;
+      comp_count_run(rr[k], rr[k] >= 0, jn >= 0, s_cnt);
+    }
+    __syncthreads();  // s_row is relabelled for the next chunk
   }
-  const int32_t tiles = (m_old + kCompRows - 1) / kCompRows;
-  if (tid == 0 && tile == tiles - 1) {
-    out_ptr[m_new] = excl + agg;
-    *total_out = excl + agg;
-  }
-  // pass 2: stage kept entries chunk by chunk, then coalesced stores
-  int32_t write_base = excl;
+  if (my_new_row >= 0) out_ptr[my_new_row] = s_cnt[tid];
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restrict__ ptr,
+                                                          const double* __restrict__ val, int32_t m_old,
+                                                          const int32_t* __restrict__ row_map,
+                                                          const int32_t* __restrict__ jn_in,
+                                                          const int32_t* __restrict__ out_ptr,
+                                                          int32_t* __restrict__ out_idx,
+                                                          double* __restrict__ out_val) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int32_t s_oidx[kCompChunk];
+  __shared__ double s_oval[kCompChunk];
+  __shared__ int32_t s_wcnt[kCompItems * kWaves];
+  __shared__ int32_t s_first, s_total;
+  const int32_t r0 = blockIdx.x * kCompRows;
+  const int rows = min(kCompRows, m_old - r0);
+  const int tid = threadIdx.x;
+  const int lane = lane_id(), wave = tid >> 6;
+  const int32_t my_new_row = tid < rows ? row_map[r0 + tid] : -1;
+  if (tid == 0) s_first = INT_MAX;
+  __syncthreads();
+  if (my_new_row >= 0) atomicMin(&s_first, my_new_row);
+  __syncthreads();
+  if (s_first == INT_MAX) return;  // no kept row in the tile
+  int32_t base = out_ptr[s_first];
+  const int32_t seg_b = ptr[r0], seg_e = ptr[r0 + rows];
+  const uint64_t below = lanemask_lt(lane);
   for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
-    const int32_t p0 = c + tid * kCompItems;
     int32_t jn[kCompItems];
-    int rr[kCompItems];
-    comp_map_items(s_ptr, s_keep, rows, seg_e, p0, idx, col_rank, jn, rr);
     double v[kCompItems];
-    int32_t cnt = 0;
 #pragma unroll
     for (int k = 0; k < kCompItems; ++k) {
-      v[k] = jn[k] >= 0 ? val[p0 + k] : 0.0;
-      cnt += jn[k] >= 0 ? 1 : 0;
+      const int32_t pc = min(c + k * kBlock + tid, seg_e - 1);
+      jn[k] = jn_in[pc];
+      v[k] = val[pc];
     }
-    int32_t chunk_total;
-    int32_t pos = block_exclusive_sum(cnt, &chunk_total);
+    uint64_t ball[kCompItems];
 #pragma unroll
     for (int k = 0; k < kCompItems; ++k) {
-      if (jn[k] >= 0) {
+      const bool keep = c + k * kBlock + tid < seg_e && jn[k] >= 0;
+      ball[k] = __ballot(keep);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < kCompItems; ++k) s_wcnt[k * kWaves + wave] = static_cast<int32_t>(__popcll(ball[k]));
+    }
+    __syncthreads();
+    if (tid < 64) {  // exclusive prefix over (slice, wave) in entry order
+      const int32_t cnt = tid < kCompItems * kWaves ? s_wcnt[tid] : 0;
+      const int32_t inc = wave_inclusive_sum(cnt);
+      if (tid < kCompItems * kWaves) s_wcnt[tid] = inc - cnt;
+      if (tid == 63) s_total = inc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kCompItems; ++k) {
+      if ((ball[k] >> lane) & 1ull) {
+        const int32_t pos = s_wcnt[k * kWaves + wave] + static_cast<int32_t>(__popcll(ball[k] & below));
         s_oidx[pos] = jn[k];
         s_oval[pos] = v[k];
-        ++pos;
       }
     }
     __syncthreads();
-    for (int t = tid; t < chunk_total; t += kBlock) {
-      out_idx[write_base + t] = s_oidx[t];
-      out_val[write_base + t] = s_oval[t];
+    const int32_t total = s_total;
+    for (int t = tid; t < total; t += kBlock) {
+      out_idx[base + t] = s_oidx[t];
+      out_val[base + t] = s_oval[t];
     }
-    write_base += chunk_total;
+    base += total;
     __syncthreads();
   }
 }
@@ -917,7 +822,7 @@ hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t
   return hipGetLastError();
 }
 
-int64_t gpk_compact_tiles(int32_t m_old) { return m_old > 0 ? (m_old + kCompRows - 1) / kCompRows : 1; }
+
 
 hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hipStream_t s) {
   if (n <= 0) return hipSuccess;
@@ -925,27 +830,26 @@ hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hip
   return hipGetLastError();
 }
 
-hipError_t gpk_compact_lb(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
-                          const int32_t* row_map, const gpk_rank_word* col_rank, int32_t m_new, int32_t* out_ptr,
-                          int32_t* out_idx, double* out_val, int32_t* total, gpk_lookback lb, hipStream_t s) {
+hipError_t gpk_compact_map(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
+                           const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, hipStream_t s) {
   if (m_old <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_compact_lb, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, idx, val, m_old, row_map,
-                     col_rank, m_new, out_ptr, out_idx, out_val, total, lb);
+  hipLaunchKernelGGL(k_compact_map, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, idx, m_old, row_map,
+                     col_rank, jn, out_ptr);
+  return hipGetLastError();
+}
+
+hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
+                             const int32_t* jn, const int32_t* out_ptr, int32_t* out_idx, double* out_val,
+                             hipStream_t s) {
+  if (m_old <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact_write, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, val, m_old, row_map,
+                     jn, out_ptr, out_idx, out_val);
   return hipGetLastError();
 }
 
 hipError_t gpk_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* o2n, int32_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_old_to_new, dim3(grid_for(n)), dim3(kBlock), 0, s, mask, prefix, o2n, n);
-  return hipGetLastError();
-}
-
-hipError_t gpk_row_activity(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
-                            const double* l, const double* u, double ztol, double* min_fin, double* max_fin,
-                            int32_t* neg_inf, int32_t* pos_inf, hipStream_t s) {
-  if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_activity, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx, val, m, l,
-                     u, ztol, min_fin, max_fin, neg_inf, pos_inf);
   return hipGetLastError();
 }
 
@@ -959,8 +863,17 @@ hipError_t gpk_row_activity_lu(const int32_t* row_ptr, const int32_t* col_idx, c
                                const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
                                int32_t* pos_inf, hipStream_t s) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_activity_lu, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx, val, m,
-                     reinterpret_cast<const double2*>(lu), ztol, min_fin, max_fin, neg_inf, pos_inf);
+  hipLaunchKernelGGL(k_row_activity_lu<false>, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx,
+                     val, m, reinterpret_cast<const double2*>(lu), ztol, min_fin, max_fin, neg_inf, pos_inf);
+  return hipGetLastError();
+}
+
+hipError_t gpk_row_activity_lu_nt(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
+                                  const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
+                                  int32_t* pos_inf, hipStream_t s) {
+  if (m <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_row_activity_lu<true>, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx,
+                     val, m, reinterpret_cast<const double2*>(lu), ztol, min_fin, max_fin, neg_inf, pos_inf);
   return hipGetLastError();
 }
 
@@ -976,28 +889,6 @@ hipError_t gpk_csr_stats(const int32_t* ptr, const int32_t* idx, const double* v
   if (rows <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_csr_stats, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, val, rows, nnz_out,
                      empty_mask, singleton_mask, singleton_idx, singleton_val);
-  return hipGetLastError();
-}
-
-hipError_t gpk_compact_count(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
-                             const int32_t* col_map, int32_t* counts, hipStream_t s) {
-  if (m_old <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_compact_count, dim3(grid_for(m_old)), dim3(kBlock), 0, s, ptr, idx, m_old, row_map, col_map,
-                     counts);
-  return hipGetLastError();
-}
-
-hipError_t gpk_compact_copy(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
-                            const int32_t* row_map, const int32_t* col_map, const int32_t* new_ptr_by_old,
-                            int32_t* out_ptr, int32_t* out_idx, double* out_val, int32_t m_new, hipStream_t s) {
-  if (m_old > 0) {
-    hipLaunchKernelGGL(k_compact_copy, dim3(grid_for(m_old)), dim3(kBlock), 0, s, ptr, idx, val, m_old, row_map,
-                       col_map, new_ptr_by_old, out_ptr, out_idx, out_val);
-  }
-  // out_ptr[m_new] = total kept entries = new_ptr_by_old[m_old]
-  const hipError_t e = hipMemcpyAsync(out_ptr + m_new, new_ptr_by_old + m_old, sizeof(int32_t),
-                                      hipMemcpyDeviceToDevice, s);
-  if (e != hipSuccess) return e;
   return hipGetLastError();
 }
 

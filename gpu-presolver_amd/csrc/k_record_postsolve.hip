@@ -460,6 +460,22 @@ __global__ void k_project_duals_long(gp_orig_dev O, const double* x, const doubl
   }
 }
 
+// Lengths and packed copies of selected CSR lines (the original rows and
+// columns that the host schedules the dual replay by).
+__global__ void k_seg_lengths(const int32_t* ptr, const int32_t* list, int32_t count, int32_t* len) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) len[i] = ptr[list[i] + 1] - ptr[list[i]];
+}
+
+__global__ void k_seg_copy(const int32_t* ptr, const int32_t* idx, const int32_t* list, int32_t count,
+                           const int32_t* off, int32_t* out) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int32_t b = ptr[list[i]], e = ptr[list[i] + 1];
+  int32_t o = off[i];
+  for (int32_t q = b; q < e; ++q) out[o++] = idx[q];
+}
+
 // Columns whose dual the replay determines exactly (gpu_postsolve.cu:1370-1402).
 __global__ void k_protect(gp_tape_dev T, int32_t n, uint8_t* prot) {
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -528,6 +544,13 @@ hipError_t gpk_replay_levels(gp_tape_dev tape, gp_orig_dev orig, double* x, doub
   hipLaunchKernelGGL(k_replay_levels, dim3(1), dim3(kLevelBlock), 0, stream, tape, orig, x, y, z, zret, records,
                      level_start, nlevels, primal, dual, tol);
   return hipGetLastError();
+}
+hipError_t gpk_seg_lengths(const int32_t* ptr, const int32_t* list, int32_t count, int32_t* len, hipStream_t stream) {
+  GPK_LAUNCH(k_seg_lengths, count, ptr, list, count, len);
+}
+hipError_t gpk_seg_copy(const int32_t* ptr, const int32_t* idx, const int32_t* list, int32_t count,
+                        const int32_t* off, int32_t* out, hipStream_t stream) {
+  GPK_LAUNCH(k_seg_copy, count, ptr, idx, list, count, off, out);
 }
 hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipStream_t stream) {
   GPK_LAUNCH(k_protect, tape.count, tape, n, prot);

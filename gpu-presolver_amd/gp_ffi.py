@@ -354,6 +354,23 @@ def presolve(lp, prm, keep_reduced=True):
         h.free()
 
 
+class PreparedPostsolve:
+    """A postsolve whose inputs (reduced x/y/z, record view, original LP) are
+    already in place, so run() times gp_postsolve alone."""
+
+    def __init__(self, rec, x_red, y_red, z_red, original_dev):
+        self.rk = T.RecordKeepAlive(rec)
+        n0, m0 = rec["n0"], rec["m0"]
+        self.out = (DeviceArray(size=n0), DeviceArray(size=m0), DeviceArray(size=n0))
+        self.ins = [DeviceArray(np.asarray(v, np.float64)) for v in (x_red, y_red, z_red)]
+        self.orig = original_dev
+
+    def run(self):
+        dx, dy, dz = self.out
+        _check(lib().gp_postsolve(C.byref(self.rk.view), *[d.ptr for d in self.ins], C.byref(self.orig.g),
+                                  dx.ptr, dy.ptr, dz.ptr))
+
+
 def postsolve(rec, x_red, y_red, z_red, original=None, original_dev=None):
     """original: LP dict (uploaded here) or original_dev: a resident DeviceLp."""
     rk = T.RecordKeepAlive(rec)

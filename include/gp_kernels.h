@@ -7,10 +7,10 @@
  * `stream` and returns hipSuccess or the launch error.
  *
  * Kernel-to-reference map (reference paths under cpp/src/presolve/):
- *   gpk_row_activity        gpu_presolve_kernels.cu:18-66
+ *   gpk_row_activity_lu     gpu_presolve_kernels.cu:18-66
  *   gpk_col_max_abs         gpu_presolve_kernels.cu:68-82
  *   gpk_scan_*              cub::DeviceScan::InclusiveSum call sites (gpu_presolve.cu:67-81)
- *   gpk_compact_*           gpu_presolve.cu:297-531 (count/copy compacted rows)
+ *   gpk_compact_*           gpu_presolve.cu:297-531, 1881-1978 (compact_csr_by_masks_gpu)
  *   gpk_radix_sort_*        thrust::sort / sort_by_key call sites
  *   gpk_transpose_*         cusparseCsr2cscEx2 (gpu_presolve.cu:1804-1879)
  *   gpk_csr_stats           gpu_presolve.cu:249-276
@@ -72,29 +72,36 @@ typedef struct __attribute__((aligned(16))) gpk_rank_word {
 } gpk_rank_word;
 hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hipStream_t stream);
 /* Stable compaction of a CSR by an old-to-new row map and the rank form of
- * the column map, in one launch (1024 old rows per tile): out_ptr (m_new +
- * 1), out_idx / out_val (capacity >= kept entries) and *total = kept
- * entries. */
-int64_t gpk_compact_tiles(int32_t m_old);
-hipError_t gpk_compact_lb(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
-                          const int32_t* row_map, const gpk_rank_word* col_rank, int32_t m_new, int32_t* out_ptr,
-                          int32_t* out_idx, double* out_val, int32_t* total, gpk_lookback lb, hipStream_t stream);
+ * the column map, in three launches over tiles of 256 old rows:
+ *   gpk_compact_map    jn[p] = new column of entry p, or -1 when its row or
+ *                      column is dropped; out_ptr[new row] = kept count;
+ *   gpk_scan_lb_i32(out_ptr, out_ptr, m_new, ...) turns the counts into row
+ *                      pointers (out_ptr[m_new] = kept entries);
+ *   gpk_compact_write  writes the kept (column, value) pairs.
+ * jn holds one int32 per entry of A. */
+hipError_t gpk_compact_map(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
+                           const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, hipStream_t stream);
+hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
+                             const int32_t* jn, const int32_t* out_ptr, int32_t* out_idx, double* out_val,
+                             hipStream_t stream);
 
 /* old_to_new[i] = mask[i] ? prefix[i] : -1. */
 hipError_t gpk_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* old_to_new, int32_t n,
                           hipStream_t stream);
 
 /* ---- activity primitives ----------------------------------------------- */
-hipError_t gpk_row_activity(const int32_t* row_ptr, const int32_t* col_idx, const double* val,
-                            int32_t m, const double* l, const double* u, double zero_tol,
-                            double* min_fin, double* max_fin, int32_t* neg_inf, int32_t* pos_inf,
-                            hipStream_t stream);
-/* The same over bounds packed as lu[2j] = l[j], lu[2j+1] = u[j] (16-byte
- * aligned, written by gpk_pack_lu): one gather per entry instead of two. */
+/* Row activity (reference gpu_presolve_kernels.cu:18-111) over bounds packed
+ * as lu[2j] = l[j], lu[2j+1] = u[j] (16-byte aligned, written by
+ * gpk_pack_lu), so each entry needs one 16-byte gather; 256 rows per block,
+ * each row summed in CSR order. */
 hipError_t gpk_pack_lu(const double* l, const double* u, int32_t n, double* lu, hipStream_t stream);
 hipError_t gpk_row_activity_lu(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
                                const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
                                int32_t* pos_inf, hipStream_t stream);
+/* The same with the CSR entries loaded non-temporally. */
+hipError_t gpk_row_activity_lu_nt(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
+                                  const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
+                                  int32_t* pos_inf, hipStream_t stream);
 hipError_t gpk_col_max_abs(const int32_t* at_ptr, const double* at_val, int32_t n, double* out,
                            hipStream_t stream);
 /* Row stats of a CSR (counts include explicit zeros; singleton = first entry). */
@@ -102,18 +109,7 @@ hipError_t gpk_csr_stats(const int32_t* ptr, const int32_t* idx, const double* v
                          int32_t* nnz_out, uint8_t* empty_mask, uint8_t* singleton_mask,
                          int32_t* singleton_idx, double* singleton_val, hipStream_t stream);
 
-/* ---- compaction ----------------------------------------------------------
- * Stable compaction of a CSR by row/column maps (old_to_new, -1 = dropped).
- * Step 1 writes counts[i] = kept entries of kept old row i (0 if dropped).
- * Step 2 (after an exclusive scan of counts into new_ptr_by_old[0..m_old]) copies
- * the kept entries; new row r's pointer is new_ptr_by_old[old row of r]. */
-hipError_t gpk_compact_count(const int32_t* ptr, const int32_t* idx, int32_t m_old,
-                             const int32_t* row_map, const int32_t* col_map, int32_t* counts,
-                             hipStream_t stream);
-hipError_t gpk_compact_copy(const int32_t* ptr, const int32_t* idx, const double* val, int32_t m_old,
-                            const int32_t* row_map, const int32_t* col_map,
-                            const int32_t* new_ptr_by_old, int32_t* out_ptr, int32_t* out_idx,
-                            double* out_val, int32_t m_new, hipStream_t stream);
+/* ---- compaction helpers ---------------------------------------------- */
 /* dst[map[i]] = src[i] for map[i] >= 0, for up to 5 vectors of the same length. */
 hipError_t gpk_gather5(const int32_t* map, int32_t n_old, const double* s0, double* d0,
                        const double* s1, double* d1, const double* s2, double* d2,

@@ -69,6 +69,11 @@ hipError_t gpk_replay_list(gp_tape_dev tape, gp_orig_dev orig, double* x, double
 hipError_t gpk_replay_levels(gp_tape_dev tape, gp_orig_dev orig, double* x, double* y, double* z, uint8_t* zret,
                              const int32_t* records, const int32_t* level_start, int32_t nlevels, int32_t primal,
                              int32_t dual, double tol, hipStream_t stream);
+/* len[i] = ptr[list[i] + 1] - ptr[list[i]]; gpk_seg_copy packs line
+ * list[i]'s idx entries at out + off[i]. */
+hipError_t gpk_seg_lengths(const int32_t* ptr, const int32_t* list, int32_t count, int32_t* len, hipStream_t stream);
+hipError_t gpk_seg_copy(const int32_t* ptr, const int32_t* idx, const int32_t* list, int32_t count,
+                        const int32_t* off, int32_t* out, hipStream_t stream);
 hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipStream_t stream);
 /* long_count: one zeroed device word; long_cols: n words of scratch (columns
  * with more than 64 entries are projected one wave per column). */
