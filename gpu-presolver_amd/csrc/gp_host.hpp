@@ -69,6 +69,15 @@ class Ctx {
     }
     return out;
   }
+  // Enqueues a copy into `out` (resized to count); the data is valid after
+  // the next sync(), so several copies can share one synchronization.
+  template <class T>
+  void to_host_async(std::vector<T>& out, const T* dev, int64_t count, const char* context) {
+    out.resize(static_cast<size_t>(count > 0 ? count : 0));
+    if (count > 0)
+      check(hipMemcpyAsync(out.data(), dev, sizeof(T) * static_cast<size_t>(count), hipMemcpyDeviceToHost, stream_),
+            context);
+  }
   template <class T>
   T scalar_to_host(const T* dev, const char* context) {
     check(hipMemcpyAsync(pinned_, dev, sizeof(T), hipMemcpyDeviceToHost, stream_), context);
@@ -218,8 +227,10 @@ int32_t scan_counts(const int32_t* in, int32_t n, int32_t* out);
 // the host from the packed selection, so the rounding order is fixed).
 double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n);
 // Compacts A by row/column keep maps; the result owns fresh device arrays.
+// known_nnz >= 0 is the kept count when the caller knows it (the transpose
+// of a matrix compacted with the same maps): the read-back is skipped.
 presolve::DeviceCsrMatrix compact_csr(const presolve::DeviceCsrMatrix& A, const int32_t* row_map,
-                                      const int32_t* col_map, int32_t m_new, int32_t n_new);
+                                      const int32_t* col_map, int32_t m_new, int32_t n_new, int32_t known_nnz = -1);
 // Stable transpose (rows of the result sorted); fresh device arrays.
 presolve::DeviceCsrMatrix transpose_csr(const presolve::DeviceCsrMatrix& A);
 void free_csr(presolve::DeviceCsrMatrix& A) noexcept;

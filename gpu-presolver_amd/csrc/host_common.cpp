@@ -290,7 +290,7 @@ void free_csr(DeviceCsrMatrix& A) noexcept {
 // per-row kept counts), an exclusive scan into the row pointers, then the
 // coalesced write of the kept entries.
 DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, const int32_t* col_map, int32_t m_new,
-                            int32_t n_new) {
+                            int32_t n_new, int32_t known_nnz) {
   Ctx& c = ctx();
   DeviceCsrMatrix out;
   out.rows = m_new;
@@ -314,7 +314,7 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
   DBuf<int32_t> idx(A.nnz, "compact colVal");
   DBuf<double> val(A.nnz, "compact nzVal");
   check(gpk_compact_write(A.rowPtr, A.nzVal, m_old, row_map, jn, ptr, idx, val, s), "compact write");
-  out.nnz = c.scalar_to_host(ptr.get() + m_new, "compact kept count");
+  out.nnz = known_nnz >= 0 ? known_nnz : c.scalar_to_host(ptr.get() + m_new, "compact kept count");
   out.rowPtr = ptr.release();
   out.colVal = idx.release();
   out.nzVal = val.release();

@@ -12,6 +12,11 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <optional>
+#include <string>
 #include <cmath>
 #include <utility>
 #include <vector>
@@ -285,26 +290,34 @@ void append_global_tape(RecordState& rs, const PostsolveTape& src) {
 }
 
 // update_record_from_plan (gpu_presolve.cu:1535-1802).
+// fixed: the removed columns with a finite fixed interval (mask and its
+// old-to-new map, k of them), computed by apply_plan with the keep maps.
+struct FixedCols {
+  DBuf<uint8_t> mask;
+  DBuf<int32_t> map;
+  int32_t k = 0;
+};
+
 void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t n, const int32_t* row_map,
-                   const int32_t* col_map, int32_t m_new, int32_t n_new) {
+                   const int32_t* col_map, int32_t m_new, int32_t n_new, const FixedCols& fixed) {
   auto& c = ctx();
   hipStream_t s = c.stream();
   PresolveRecordGpu& rec = *rs.rec;
-  if (n_new < n) {
+  if (fixed.k > 0) {
     // Removed columns with a finite fixed interval become (global index, midpoint).
-    DBuf<uint8_t> mask(n);
-    DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1);
-    check(gpk_removed_fixed_mask(n, p.keep_col_mask, p.new_l, p.new_u, mask, s), "removed fixed mask");
-    const int32_t k = amd::scan_mask_count(mask, n, prefix);
-    if (k > 0) {
-      DBuf<int32_t> oi(k);
-      DBuf<double> ov(k);
-      check(gpk_pack_fixed_records(n, mask, prefix, rs.col_r2o, p.new_l, p.new_u, oi, ov, s), "fixed records");
-      const std::vector<int32_t> hi = c.to_host(oi.get(), k, "fixed records idx");
-      const std::vector<double> hv = c.to_host(ov.get(), k, "fixed records val");
-      rec.fixed_idx.insert(rec.fixed_idx.end(), hi.begin(), hi.end());
-      rec.fixed_val.insert(rec.fixed_val.end(), hv.begin(), hv.end());
-    }
+    const int32_t k = fixed.k;
+    DBuf<int32_t> oi(k);
+    DBuf<double> ov(k);
+    check(gpk_pack_fixed_records(n, fixed.mask, fixed.map, rs.col_r2o, p.new_l, p.new_u, oi, ov, s),
+          "fixed records");
+    const size_t base = rec.fixed_idx.size();
+    rec.fixed_idx.resize(base + static_cast<size_t>(k));
+    rec.fixed_val.resize(base + static_cast<size_t>(k));
+    check(hipMemcpyAsync(rec.fixed_idx.data() + base, oi.get(), sizeof(int32_t) * k, hipMemcpyDeviceToHost, s),
+          "fixed records idx");
+    check(hipMemcpyAsync(rec.fixed_val.data() + base, ov.get(), sizeof(double) * k, hipMemcpyDeviceToHost, s),
+          "fixed records val");
+    c.sync("fixed records");
   }
   append_global_tape(rs, p.tape);
   if (m_new < m) {
@@ -326,12 +339,27 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   auto& c = ctx();
   hipStream_t s = c.stream();
   const int32_t m = cur.lp.A.rows, n = cur.lp.A.cols;
-  DBuf<int32_t> row_map, col_map;
-  const int32_t m_new = amd::build_old_to_new(p.keep_row_mask, m, row_map);
-  const int32_t n_new = amd::build_old_to_new(p.keep_col_mask, n, col_map);
+  // The row and column keep maps and the removed-fixed columns, with their
+  // three counts read back in one synchronization.
+  DBuf<int32_t> row_map(static_cast<int64_t>(m) + 1), col_map(static_cast<int64_t>(n) + 1);
+  FixedCols fixed;
+  fixed.mask.alloc(n);
+  fixed.map.alloc(static_cast<int64_t>(n) + 1);
+  int32_t* counts = c.flags();
+  if (m > 0)
+    check(gpk_mask_to_map(p.keep_row_mask, row_map, m, counts, c.lookback(gpk_lookback_tiles(m)), s), "row map");
+  if (n > 0) {
+    check(gpk_mask_to_map(p.keep_col_mask, col_map, n, counts + 1, c.lookback(gpk_lookback_tiles(n)), s), "col map");
+    check(gpk_removed_fixed_mask(n, p.keep_col_mask, p.new_l, p.new_u, fixed.mask, s), "removed fixed mask");
+    check(gpk_mask_to_map(fixed.mask, fixed.map, n, counts + 2, c.lookback(gpk_lookback_tiles(n)), s),
+          "removed fixed map");
+  }
+  const int32_t* h = c.read_flags(3);
+  const int32_t m_new = h[0], n_new = h[1];
+  fixed.k = h[2];
   const double obj_new = cur.lp.obj_constant + p.obj_constant_delta;
   if (cur.owns && !p.has_new_A && m_new == m && n_new == n) {
-    update_record(rs, p, m, n, row_map, col_map, m_new, n_new);
+    update_record(rs, p, m, n, row_map, col_map, m_new, n_new, fixed);
     d2d(cur.lp.c, p.new_c, n, "apply c");
     d2d(cur.lp.l, p.new_l, n, "apply l");
     d2d(cur.lp.u, p.new_u, n, "apply u");
@@ -348,7 +376,7 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   if (p.has_new_A || !cur.owns) {
     next.AT = amd::transpose_csr(next.A);
   } else {
-    next.AT = amd::compact_csr(cur.lp.AT, col_map, row_map, n_new, m_new);
+    next.AT = amd::compact_csr(cur.lp.AT, col_map, row_map, n_new, m_new, next.A.nnz);
   }
   next.c = pool_f64(n_new, "working c");
   next.l = pool_f64(n_new, "working l");
@@ -363,7 +391,7 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
     check(gpk_gather5(row_map, m, p.new_AL, next.AL, p.new_AU, next.AU, nullptr, nullptr, nullptr, nullptr, nullptr,
                       nullptr, s),
           "gather row vectors");
-  update_record(rs, p, m, n, row_map, col_map, m_new, n_new);
+  update_record(rs, p, m, n, row_map, col_map, m_new, n_new, fixed);
   next.obj_constant = obj_new;
   cur.release();
   cur.lp = next;
@@ -420,6 +448,40 @@ void validate_lp(const LPInfoGpu& lp) {
     throw std::runtime_error("LPInfoGpu: missing CSR array");
 }
 
+// Per-step wall time of one presolve, printed to stderr when the environment
+// sets GP_PRESOLVE_PROFILE (the reference's rule profiling prints, e.g.
+// gpu_presolve.cu:2797-2812, play the same role).  Each timed step
+// synchronizes the stream first and last, so the figures are exact but the
+// run is slower; without the variable nothing is synchronized or timed.
+class StepProfile {
+ public:
+  StepProfile() : on_(std::getenv("GP_PRESOLVE_PROFILE") != nullptr) {}
+  ~StepProfile() {
+    if (!on_) return;
+    std::fprintf(stderr, "presolve profile (ms, calls):\n");
+    for (const auto& kv : t_)
+      std::fprintf(stderr, "  %-34s %9.3f %6d\n", kv.first.c_str(), 1e3 * kv.second.second, kv.second.first);
+  }
+  template <class F>
+  void run(const char* name, F&& f) {
+    if (!on_) {
+      f();
+      return;
+    }
+    ctx().sync("profile");
+    const auto t0 = std::chrono::steady_clock::now();
+    f();
+    ctx().sync("profile");
+    auto& e = t_[name];
+    ++e.first;
+    e.second += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  }
+
+ private:
+  bool on_;
+  std::map<std::string, std::pair<int, double>> t_;
+};
+
 // run_gpu_presolve_impl (gpu_presolve.cu:2162-3400) without the six
 // structure-specific rules (they find nothing on LPs without their block
 // patterns, so both values of enable_structure_specific_rules schedule the
@@ -448,6 +510,7 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
   cur.owns = false;
 
   bool terminal = false;
+  StepProfile prof;
   const auto t0 = std::chrono::steady_clock::now();
   const bool has_limit = prm.max_time > 0.0 && std::isfinite(prm.max_time);
   bool cs_dual_clean = false, cs_eq_clean = false;
@@ -465,7 +528,7 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     terminal = p.has_infeasible || p.has_unbounded;
     const bool changed = !terminal && p.has_change;
     if (changed) {
-      apply_plan(cur, p, rs);
+      prof.run("apply plan", [&] { apply_plan(cur, p, rs); });
       invalidate_clean();
     }
     return changed;
@@ -474,17 +537,26 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
   // run_row_phase (gpu_presolve.cu:2219-2301)
   auto row_phase = [&](bool empty, bool singleton, bool redundant, bool implied, bool dup) -> bool {
     if (terminal || time_exceeded()) return false;
-    PhasePlan pp(cur.lp);
-    PhaseStats st(cur.lp);
+    std::optional<PhasePlan> opp;
+    std::optional<PhaseStats> ost;
+    prof.run("plan + stats", [&] {
+      opp.emplace(cur.lp);
+      ost.emplace(cur.lp);
+    });
+    PhasePlan& pp = *opp;
+    PhaseStats& st = *ost;
     PresolvePlanGpu& p = pp.p;
     const LPInfoGpu& v = cur.lp;
-    if (!time_exceeded() && empty && prm.enable_empty_rows) apply_rule_empty_rows(p, v, st.s, prm);
-    if (!time_exceeded() && singleton && prm.enable_singleton_rows) apply_rule_singleton_rows(p, v, st.s, prm);
+    if (!time_exceeded() && empty && prm.enable_empty_rows)
+      prof.run("rule empty_rows", [&] { apply_rule_empty_rows(p, v, st.s, prm); });
+    if (!time_exceeded() && singleton && prm.enable_singleton_rows)
+      prof.run("rule singleton_rows", [&] { apply_rule_singleton_rows(p, v, st.s, prm); });
     if (!time_exceeded() && redundant && prm.enable_infeasible_redundant_rows)
-      apply_rule_infeasible_redundant_rows(p, v, st.s, prm);
+      prof.run("rule infeasible_redundant_rows", [&] { apply_rule_infeasible_redundant_rows(p, v, st.s, prm); });
     if (!time_exceeded() && implied && prm.enable_implied_variable_bounds)
-      apply_rule_implied_variable_bounds(p, v, st.s, prm);
-    if (!time_exceeded() && dup && prm.enable_duplicate_rows) apply_rule_duplicate_rows(p, v, st.s, prm);
+      prof.run("rule implied_variable_bounds", [&] { apply_rule_implied_variable_bounds(p, v, st.s, prm); });
+    if (!time_exceeded() && dup && prm.enable_duplicate_rows)
+      prof.run("rule duplicate_rows", [&] { apply_rule_duplicate_rows(p, v, st.s, prm); });
     return finish_plan(p);
   };
 
@@ -492,24 +564,34 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
   auto col_phase = [&](bool fixed, bool empty_cols, bool cs_dual, bool cs_eq, bool doubleton, bool dual_fix,
                        bool dup_cols, bool redundant_bounds) -> bool {
     if (terminal || time_exceeded()) return false;
-    PhasePlan pp(cur.lp);
-    PhaseStats st(cur.lp);
+    std::optional<PhasePlan> opp;
+    std::optional<PhaseStats> ost;
+    prof.run("plan + stats", [&] {
+      opp.emplace(cur.lp);
+      ost.emplace(cur.lp);
+    });
+    PhasePlan& pp = *opp;
+    PhaseStats& st = *ost;
     PresolvePlanGpu& p = pp.p;
     const LPInfoGpu& v = cur.lp;
     const bool run_redundant =
         redundant_bounds && (prm.enable_redundant_bounds || prm.enable_zero_cost_redundant_box_bounds);
     if (!time_exceeded() && fixed && prm.enable_infeasible_fixed_variables)
-      apply_rule_infeasible_fixed_variables(p, v, prm);
-    if (!time_exceeded() && empty_cols && prm.enable_empty_cols) apply_rule_empty_cols(p, v, st.s, prm);
+      prof.run("rule infeasible_fixed_variables", [&] { apply_rule_infeasible_fixed_variables(p, v, prm); });
+    if (!time_exceeded() && empty_cols && prm.enable_empty_cols)
+      prof.run("rule empty_cols", [&] { apply_rule_empty_cols(p, v, st.s, prm); });
     if (!time_exceeded() && cs_dual && prm.enable_column_singletons_dual_infer)
-      apply_rule_column_singletons_dual_infer(p, v, st.s, prm);
+      prof.run("rule column_singletons_dual_infer", [&] { apply_rule_column_singletons_dual_infer(p, v, st.s, prm); });
     if (!time_exceeded() && cs_eq && prm.enable_column_singletons_eq)
-      apply_rule_column_singletons_eq(p, v, st.s, prm, false);
+      prof.run("rule column_singletons_eq", [&] { apply_rule_column_singletons_eq(p, v, st.s, prm, false); });
     if (!time_exceeded() && doubleton && prm.enable_doubleton_equations)
-      apply_rule_doubleton_equations(p, v, st.s, prm);
-    if (!time_exceeded() && dual_fix && prm.enable_dual_fix) apply_rule_dual_fix(p, v, st.s, prm, nullptr);
-    if (!time_exceeded() && dup_cols && prm.enable_duplicate_columns) apply_rule_duplicate_columns(p, v, st.s, prm);
-    if (!time_exceeded() && run_redundant) apply_rule_redundant_bounds(p, v, st.s, prm);
+      prof.run("rule doubleton_equations", [&] { apply_rule_doubleton_equations(p, v, st.s, prm); });
+    if (!time_exceeded() && dual_fix && prm.enable_dual_fix)
+      prof.run("rule dual_fix", [&] { apply_rule_dual_fix(p, v, st.s, prm, nullptr); });
+    if (!time_exceeded() && dup_cols && prm.enable_duplicate_columns)
+      prof.run("rule duplicate_columns", [&] { apply_rule_duplicate_columns(p, v, st.s, prm); });
+    if (!time_exceeded() && run_redundant)
+      prof.run("rule redundant_bounds", [&] { apply_rule_redundant_bounds(p, v, st.s, prm); });
     const bool changed = finish_plan(p);
     if (!changed) {
       const bool only_dual = !fixed && !empty_cols && cs_dual && !cs_eq && !doubleton && !dual_fix && !dup_cols &&

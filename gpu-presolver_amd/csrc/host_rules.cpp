@@ -231,9 +231,12 @@ void apply_rule_implied_variable_bounds(PresolvePlanGpu& plan, const LPInfoGpu& 
       DBuf<int32_t> rcol(records), rrow(records);
       DBuf<double> rval(4 * static_cast<int64_t>(records));
       check(gpk_implied_pack_tape(n, P, lch, uch, sl, su, fl, fu, cnt, rcol, rrow, rval, s), "implied bounds pack");
-      const std::vector<int32_t> hc = c.to_host(rcol.get(), records, "implied tape cols");
-      const std::vector<int32_t> hr = c.to_host(rrow.get(), records, "implied tape rows");
-      const std::vector<double> hv = c.to_host(rval.get(), 4 * static_cast<int64_t>(records), "implied tape vals");
+      std::vector<int32_t> hc, hr;
+      std::vector<double> hv;
+      c.to_host_async(hc, rcol.get(), records, "implied tape cols");
+      c.to_host_async(hr, rrow.get(), records, "implied tape rows");
+      c.to_host_async(hv, rval.get(), 4 * static_cast<int64_t>(records), "implied tape vals");
+      c.sync("implied tape");
       for (int32_t r = 0; r < records; ++r) {
         const double* v = hv.data() + 4 * static_cast<size_t>(r);
         if (hr[r] != INT_MAX)
@@ -299,17 +302,31 @@ void apply_rule_duplicate_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
   check(gpk_delete_rows(m, plan.keep_row_mask, del, f, s), "duplicate rows delete");
   if (c.read_flags(1)[0] == 0) return;
   if (prm.record_postsolve_tape) {
-    const std::vector<int32_t> deleted = masked_indices(del, m);
-    const std::vector<int32_t> hto = c.to_host(merge_to.get(), m, "duplicate rows merge_to");
-    const std::vector<int32_t> hstep = c.to_host(step.get(), m, "duplicate rows steps");
-    const std::vector<double> hrec = c.to_host(rec.get(), 5 * static_cast<int64_t>(m), "duplicate rows values");
-    std::vector<int32_t> order(deleted);
+    // Only the deleted rows' data is packed on the device and copied back.
+    DBuf<int32_t> prefix(static_cast<int64_t>(m) + 1);
+    const int32_t k = amd::scan_mask_count(del, m, prefix);
+    DBuf<int32_t> d_row(k), d_to(k), d_step(k);
+    DBuf<double> d_rec(5 * static_cast<int64_t>(k));
+    check(gpk_dup_row_pack(m, del, prefix, merge_to, step, rec, d_row, d_to, d_step, d_rec, s),
+          "duplicate rows pack");
+    std::vector<int32_t> hrow(k), hto(k), hstep(k);
+    std::vector<double> hrec(5 * static_cast<size_t>(k));
+    if (k > 0) {
+      check(hipMemcpyAsync(hrow.data(), d_row, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s), "duplicate rows");
+      check(hipMemcpyAsync(hto.data(), d_to, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s), "duplicate rows");
+      check(hipMemcpyAsync(hstep.data(), d_step, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s), "duplicate rows");
+      check(hipMemcpyAsync(hrec.data(), d_rec, sizeof(double) * 5 * k, hipMemcpyDeviceToHost, s), "duplicate rows");
+      c.sync("duplicate rows tape");
+    }
+    // Tape order: by merge step, then by deleted row.
+    std::vector<int32_t> order(k);
+    for (int32_t t = 0; t < k; ++t) order[t] = t;
     std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-      return hstep[a] != hstep[b] ? hstep[a] < hstep[b] : a < b;
+      return hstep[a] != hstep[b] ? hstep[a] < hstep[b] : hrow[a] < hrow[b];
     });
-    for (int32_t row : order) {
-      const double* v = hrec.data() + 5 * static_cast<size_t>(row);
-      append_record(plan.tape, RT::DuplicateRow, {hto[row], row}, {v[0], v[1], v[2], v[3], v[4]}, DM::Exact);
+    for (int32_t t : order) {
+      const double* v = hrec.data() + 5 * static_cast<size_t>(t);
+      append_record(plan.tape, RT::DuplicateRow, {hto[t], hrow[t]}, {v[0], v[1], v[2], v[3], v[4]}, DM::Exact);
     }
   }
   plan.has_row_action = true;
@@ -465,10 +482,13 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
             "subcol pack");
       // Records are in ascending row order; record r owns the support
       // entries [start_r, start_{r+1}) of the packed arrays.
-      const std::vector<int32_t> hi3 = c.to_host(oi3.get(), 3 * static_cast<int64_t>(records), "subcol ids");
-      const std::vector<double> hv6 = c.to_host(ov6.get(), 6 * static_cast<int64_t>(records), "subcol vals");
-      const std::vector<int32_t> hsc = c.to_host(scol.get(), support, "subcol support cols");
-      const std::vector<double> hsv = c.to_host(sval.get(), support, "subcol support vals");
+      std::vector<int32_t> hi3, hsc;
+      std::vector<double> hv6, hsv;
+      c.to_host_async(hi3, oi3.get(), 3 * static_cast<int64_t>(records), "subcol ids");
+      c.to_host_async(hv6, ov6.get(), 6 * static_cast<int64_t>(records), "subcol vals");
+      c.to_host_async(hsc, scol.get(), support, "subcol support cols");
+      c.to_host_async(hsv, sval.get(), support, "subcol support vals");
+      c.sync("subcol tape");
       std::vector<int32_t> idx;
       std::vector<double> vals;
       for (int32_t r = 0; r < records; ++r) {
@@ -589,9 +609,13 @@ bool dbl_batch_once(PresolvePlanGpu& plan, const LPInfoGpu& lp, const DeviceCsrM
     return false;
   }
   if (recs != nullptr && subst_count > 0) {
-    const std::vector<uint8_t> hs = c.to_host(subst.get(), k, "doubleton tape mask");
-    const std::vector<int32_t> hi = c.to_host(ri.get(), 3 * static_cast<int64_t>(k), "doubleton tape ids");
-    const std::vector<double> hv = c.to_host(rv.get(), 12 * static_cast<int64_t>(k), "doubleton tape vals");
+    std::vector<uint8_t> hs;
+    std::vector<int32_t> hi;
+    std::vector<double> hv;
+    c.to_host_async(hs, subst.get(), k, "doubleton tape mask");
+    c.to_host_async(hi, ri.get(), 3 * static_cast<int64_t>(k), "doubleton tape ids");
+    c.to_host_async(hv, rv.get(), 12 * static_cast<int64_t>(k), "doubleton tape vals");
+    c.sync("doubleton tape");
     for (int32_t t = 0; t < k; ++t) {
       if (!hs[t]) continue;
       DblTapeRec r;

@@ -369,6 +369,56 @@ __global__ __launch_bounds__(kBlock) void k_scan_lb(const In* in, int32_t* out, 
   }
 }
 
+// Small scans in one workgroup (no tickets, no look-back): 1024 threads walk
+// the array in chunks of 8192 with a running carry, each chunk striped and
+// scanned like a look-back tile.  Same outputs as k_scan_lb.
+constexpr int kSmallBlock = 1024;
+constexpr int64_t kSmallScanMax = 16384;
+
+template <class In, int kMode>
+__global__ __launch_bounds__(kSmallBlock) void k_scan_small(const In* in, int32_t* out, int64_t n, int32_t* count) {
+  constexpr int kWaves = kSmallBlock / 64;
+  constexpr int kSlots = kScanItems * kWaves;  // 128
+  __shared__ int32_t s_w[kSlots];
+  __shared__ int32_t s_total;
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  int32_t carry = 0;
+  for (int64_t c = 0; c < n; c += static_cast<int64_t>(kScanItems) * kSmallBlock) {
+    int32_t v[kScanItems], inc[kScanItems];
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const int64_t i = c + k * kSmallBlock + tid;
+      v[k] = i < n ? scan_value(in, i) : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      inc[k] = wave_inclusive_sum(v[k]);
+      if (lane == 63) s_w[k * kWaves + wave] = inc[k];
+    }
+    __syncthreads();
+    if (tid < 64) {  // exclusive prefix of the 128 (slice, wave) totals, two per lane
+      const int32_t a = s_w[2 * tid], b = s_w[2 * tid + 1];
+      const int32_t pi = wave_inclusive_sum(a + b);
+      s_w[2 * tid] = pi - a - b;
+      s_w[2 * tid + 1] = pi - b;
+      if (tid == 63) s_total = pi;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kScanItems; ++k) {
+      const int64_t i = c + k * kSmallBlock + tid;
+      const int32_t prefix = carry + s_w[k * kWaves + wave] + inc[k] - v[k];
+      if (i < n) out[i] = kMode == 1 ? (v[k] ? prefix : -1) : prefix;
+    }
+    carry += s_total;
+    __syncthreads();  // s_w and s_total are rewritten by the next chunk
+  }
+  if (tid == 0) {
+    if (kMode == 1) *count = carry;
+    else out[n] = carry;
+  }
+}
+
 // Compaction in three launches (gpu_presolve.cu:1881-1978): k_compact_map
 // maps every entry once (row kept and column rank, one cache-resident rank
 // gather per entry), stores the new column index or -1 per entry and each
@@ -799,6 +849,10 @@ int64_t gpk_lookback_tiles(int64_t n) { return n > 0 ? (n + kScanTile - 1) / kSc
 
 hipError_t gpk_scan_lb_i32(const int32_t* in, int32_t* out, int64_t n, gpk_lookback lb, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (n <= kSmallScanMax) {
+    hipLaunchKernelGGL((k_scan_small<int32_t, 0>), dim3(1), dim3(kSmallBlock), 0, s, in, out, n, nullptr);
+    return hipGetLastError();
+  }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL((k_scan_lb<int32_t, 0>), dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, in, out, n,
                      nullptr, lb);
@@ -807,6 +861,10 @@ hipError_t gpk_scan_lb_i32(const int32_t* in, int32_t* out, int64_t n, gpk_lookb
 
 hipError_t gpk_scan_lb_mask(const uint8_t* mask, int32_t* out, int64_t n, gpk_lookback lb, hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (n <= kSmallScanMax) {
+    hipLaunchKernelGGL((k_scan_small<uint8_t, 0>), dim3(1), dim3(kSmallBlock), 0, s, mask, out, n, nullptr);
+    return hipGetLastError();
+  }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL((k_scan_lb<uint8_t, 0>), dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, mask, out, n,
                      nullptr, lb);
@@ -816,6 +874,10 @@ hipError_t gpk_scan_lb_mask(const uint8_t* mask, int32_t* out, int64_t n, gpk_lo
 hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t* count, gpk_lookback lb,
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
+  if (n <= kSmallScanMax) {
+    hipLaunchKernelGGL((k_scan_small<uint8_t, 1>), dim3(1), dim3(kSmallBlock), 0, s, mask, map, n, count);
+    return hipGetLastError();
+  }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL((k_scan_lb<uint8_t, 1>), dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, mask, map, n,
                      count, lb);

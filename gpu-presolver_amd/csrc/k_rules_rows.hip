@@ -570,6 +570,21 @@ __global__ void k_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_d
   }
 }
 
+// Deleted duplicate rows packed in ascending row order (prefix = exclusive
+// scan of del): row index, kept partner, merge step and the five record
+// values, so only the deleted rows' data crosses to the host.
+__global__ void k_dup_row_pack(int32_t m, const uint8_t* del, const int32_t* prefix, const int32_t* merge_to,
+                               const int32_t* step, const double* rec, int32_t* out_row, int32_t* out_to,
+                               int32_t* out_step, double* out_rec) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !del[i]) return;
+  const int32_t r = prefix[i];
+  out_row[r] = i;
+  out_to[r] = merge_to[i];
+  out_step[r] = step[i];
+  for (int t = 0; t < 5; ++t) out_rec[5 * static_cast<int64_t>(r) + t] = rec[5 * static_cast<int64_t>(i) + t];
+}
+
 // ---- batched singleton closure (gpu_presolve.cu:98-247) ---------------------
 // One launch over max(m, n) threads: thread t takes column t and row t.
 // Live degrees lose the entries of columns (rows) removed since the previous
@@ -789,6 +804,11 @@ hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_h
 hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count,
                            hipStream_t stream) {
   GPK_LAUNCH(k_delete_rows, m, m, keep_row, row_delete, count);
+}
+hipError_t gpk_dup_row_pack(int32_t m, const uint8_t* del, const int32_t* prefix, const int32_t* merge_to,
+                            const int32_t* step, const double* rec, int32_t* out_row, int32_t* out_to,
+                            int32_t* out_step, double* out_rec, hipStream_t stream) {
+  GPK_LAUNCH(k_dup_row_pack, m, m, del, prefix, merge_to, step, rec, out_row, out_to, out_step, out_rec);
 }
 
 hipError_t gpk_closure_degrees(gpk_csr A, gpk_csr AT, const uint8_t* keep_row, const uint8_t* keep_col,

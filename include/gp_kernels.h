@@ -264,6 +264,11 @@ hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_h
                               int32_t* merge_step, double* rec_vals, int32_t* flags, hipStream_t stream);
 hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count,
                            hipStream_t stream);
+/* Deleted duplicate rows (del[i] != 0, prefix = exclusive scan of del) packed
+ * in ascending row order: row, merge_to, step and the 5 record values. */
+hipError_t gpk_dup_row_pack(int32_t m, const uint8_t* del, const int32_t* prefix, const int32_t* merge_to,
+                            const int32_t* step, const double* rec, int32_t* out_row, int32_t* out_to,
+                            int32_t* out_step, double* out_rec, hipStream_t stream);
 
 /* rule_infeasible_fixed_variables.cu:42-104, rule_empty_cols.cu:42-108 */
 hipError_t gpk_fixed_vars_candidates(int32_t n, gpk_plan plan, double bound_tol, uint8_t* fixed, double* v,
