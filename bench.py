@@ -115,27 +115,11 @@ def measure_activity_kernels(gp_ffi, dlp, lp, stream, launches, want_mn, want_mx
     us = 1e3 * e0.elapsed_ms(e1) / launches
     pack_us = 1e3 * e1.elapsed_ms(e2) / launches
     same = bool(np.array_equal(mn.numpy(), want_mn) and np.array_equal(mx.numpy(), want_mx))
-    L.gpk_row_activity_lu_nt.argtypes = L.gpk_row_activity_lu.argtypes
-
-    def act_nt():
-        gp_ffi.hip_check(L.gpk_row_activity_lu_nt(A.row_ptr, A.col_idx, A.val, m, lu.ptr, 1e-10, mn.ptr, mx.ptr,
-                                                  ni.ptr, pi.ptr, stream.ptr), "activity lu nt")
-
-    act_nt()
-    stream.sync()
-    g0, g1 = gp_ffi.Event(), gp_ffi.Event()
-    g0.record(stream)
-    for _ in range(launches):
-        act_nt()
-    g1.record(stream)
-    nt_us = 1e3 * g0.elapsed_ms(g1) / launches
-    same = same and bool(np.array_equal(mn.numpy(), want_mn) and np.array_equal(mx.numpy(), want_mx))
-    log(f"  k_row_activity_lu with non-temporal CSR loads: {nt_us:.1f} us/launch")
     b = activity_bytes(m, n, nnz)
     if not same:
         raise SystemExit("bench.py: gpk_row_activity_lu differs from gp_row_activity")
     log(f"  k_row_activity_lu {us:.1f} us/launch ({b / (us * 1e-6) / 1e9:.0f} GB/s), k_pack_lu {pack_us:.1f} us")
-    return {"k_row_activity_lu_us": us, "k_pack_lu_us": pack_us, "nontemporal_variant_us": nt_us,
+    return {"k_row_activity_lu_us": us, "k_pack_lu_us": pack_us,
             "k_row_activity_lu_achieved": b / (us * 1e-6) / 1e9}
 
 
@@ -280,6 +264,13 @@ def profiled_traffic(kernel):
             except ValueError:
                 return None
     return None
+
+
+def activity_traffic():
+    """HBM bytes of one row-activity call (k_pack_lu + k_row_activity_lu) from
+    the committed PMC summary, or None."""
+    parts = [profiled_traffic(k) for k in ("k_pack_lu", "k_row_activity_lu")]
+    return None if any(p is None for p in parts) else sum(parts)
 
 
 def compaction_traffic():
@@ -538,8 +529,7 @@ def main():
         "roofline_activity": {"bound": "hbm", "kernel": "k_pack_lu + k_row_activity_lu (one gp_row_activity call)",
                               "achieved": act["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": act["achieved"] / HBM_PEAK_GBS,
-                              "traffic": (None if profiled_traffic("k_row_activity_lu") is None else
-                                          profiled_traffic("k_row_activity_lu") + (profiled_traffic("k_pack_lu") or 0)),
+                              "traffic": activity_traffic(),
                               "us_per_call": act["us"], "algorithmic_bytes": act["bytes"], **act["kernels"]},
         "cpu_baseline": cpu,
     }

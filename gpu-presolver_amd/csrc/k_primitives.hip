@@ -113,20 +113,6 @@ __global__ void k_pack_lu(const double* __restrict__ l, const double* __restrict
   if (j < n) lu[j] = make_double2(l[j], u[j]);
 }
 
-template <bool kStream>
-__device__ inline double act_load(const double* p) {
-  if constexpr (kStream) return __builtin_nontemporal_load(p);
-  else return *p;
-}
-template <bool kStream>
-__device__ inline int32_t act_load(const int32_t* p) {
-  if constexpr (kStream) return __builtin_nontemporal_load(p);
-  else return *p;
-}
-
-// kStream: the once-read CSR entries are loaded with the non-temporal hint,
-// so they do not displace the reused (l, u) lines from the caches.
-template <bool kStream>
 __global__ __launch_bounds__(kBlock) void k_row_activity_lu(const int32_t* __restrict__ row_ptr,
                                                             const int32_t* __restrict__ col_idx,
                                                             const double* __restrict__ val, int32_t m,
@@ -159,8 +145,8 @@ __global__ __launch_bounds__(kBlock) void k_row_activity_lu(const int32_t* __res
 #pragma unroll
     for (int k = 0; k < kActItems; ++k) {
       const int32_t pc = min(w + tid + k * kBlock, we - 1);
-      a[k] = act_load<kStream>(val + pc);
-      j[k] = act_load<kStream>(col_idx + pc);
+      a[k] = val[pc];
+      j[k] = col_idx[pc];
     }
 #pragma unroll
     for (int k = 0; k < kActItems; ++k) {
@@ -925,16 +911,7 @@ hipError_t gpk_row_activity_lu(const int32_t* row_ptr, const int32_t* col_idx, c
                                const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
                                int32_t* pos_inf, hipStream_t s) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_activity_lu<false>, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx,
-                     val, m, reinterpret_cast<const double2*>(lu), ztol, min_fin, max_fin, neg_inf, pos_inf);
-  return hipGetLastError();
-}
-
-hipError_t gpk_row_activity_lu_nt(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
-                                  const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
-                                  int32_t* pos_inf, hipStream_t s) {
-  if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_row_activity_lu<true>, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx,
+  hipLaunchKernelGGL(k_row_activity_lu, dim3(grid_for(m, kActRows)), dim3(kBlock), 0, s, row_ptr, col_idx,
                      val, m, reinterpret_cast<const double2*>(lu), ztol, min_fin, max_fin, neg_inf, pos_inf);
   return hipGetLastError();
 }

@@ -98,10 +98,6 @@ hipError_t gpk_pack_lu(const double* l, const double* u, int32_t n, double* lu, 
 hipError_t gpk_row_activity_lu(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
                                const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
                                int32_t* pos_inf, hipStream_t stream);
-/* The same with the CSR entries loaded non-temporally. */
-hipError_t gpk_row_activity_lu_nt(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
-                                  const double* lu, double ztol, double* min_fin, double* max_fin, int32_t* neg_inf,
-                                  int32_t* pos_inf, hipStream_t stream);
 hipError_t gpk_col_max_abs(const int32_t* at_ptr, const double* at_val, int32_t n, double* out,
                            hipStream_t stream);
 /* Row stats of a CSR (counts include explicit zeros; singleton = first entry). */
