@@ -136,15 +136,23 @@ def calibration_copy(gp_ffi):
 
 
 def compaction_bytes(m, n, nnz, m_new, nnz_new):
-    """Compulsory traffic of one compaction: read the row pointers, the row
-    map, the column map and every entry once; write the kept entries and the
-    new row pointers."""
+    """Algorithmic bytes of one compaction in the count / scan / copy model of
+    the reference's compaction (gpu_presolve.cu:1881-1978): a counting pass
+    over the row pointers and column indices, the row and column maps, one
+    read of every entry and one write of every kept entry, the new row
+    pointers, and the row-pointer scan (read + write)."""
+    return (4 * (m + 1) + 4 * nnz) + 4 * m + 4 * n + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1) + 2 * 4 * m_new
+
+
+def compaction_compulsory_bytes(m, n, nnz, m_new, nnz_new):
+    """The floor under any compaction: read the row pointers, the maps and
+    every entry once; write the kept entries and the new row pointers."""
     return 4 * (m + 1) + 4 * m + 4 * n + 12 * nnz + 12 * nnz_new + 4 * (m_new + 1)
 
 
 def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
     """Times the compaction the presolve uses (gpk_build_rank, gpk_compact_map,
-    the row-pointer scan and gpk_compact_write, back to back on one stream)
+    the tile-offset scan and gpk_compact_write, back to back on one stream)
     on the full A with 15% of rows and columns dropped (a typical early
     apply), each kernel also timed apart, and checks the output against the
     host."""
@@ -168,8 +176,10 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
     L = gp_ffi.lib()
     vp = C.c_void_p
     L.gpk_build_rank.argtypes = [vp, C.c_int32, vp, vp]
-    L.gpk_compact_map.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp, vp]
-    L.gpk_compact_write.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp]
+    L.gpk_compact_map.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp, vp, vp]
+    L.gpk_compact_write.argtypes = [vp, vp, C.c_int32, vp, vp, vp, vp, C.c_int32, vp, vp, vp]
+    L.gpk_compact_tiles.restype = C.c_int32
+    L.gpk_compact_tiles.argtypes = [C.c_int32]
     L.gpk_scan_lb_i32.argtypes = [vp, vp, C.c_int64, Lookback, vp]
     L.gpk_lookback_tiles.restype = C.c_int64
     L.gpk_lookback_tiles.argtypes = [C.c_int64]
@@ -179,7 +189,9 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
     o_ptr = gp_ffi.DeviceArray(size=m_new + 1, dtype=np.int32)
     o_idx = gp_ffi.DeviceArray(size=nnz, dtype=np.int32)
     o_val = gp_ffi.DeviceArray(size=nnz)
-    status = gp_ffi.DeviceArray(np.zeros(max(1, L.gpk_lookback_tiles(m_new)), np.uint64))
+    tiles = L.gpk_compact_tiles(m)
+    t_off = gp_ffi.DeviceArray(size=tiles + 1, dtype=np.int32)
+    status = gp_ffi.DeviceArray(np.zeros(max(1, L.gpk_lookback_tiles(tiles)), np.uint64))
     tickets = gp_ffi.DeviceArray(np.zeros(4 * launches + 16, np.int32))
     err = gp_ffi.DeviceArray(np.zeros(1, np.int32))
     A = dlp.A.g
@@ -190,17 +202,18 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
         ck(L.gpk_build_rank(d_cm.ptr, n, d_rank.ptr, stream.ptr), "build rank")
 
     def cmap():
-        ck(L.gpk_compact_map(A.row_ptr, A.col_idx, m, d_rm.ptr, d_rank.ptr, jn.ptr, o_ptr.ptr, stream.ptr), "map")
+        ck(L.gpk_compact_map(A.row_ptr, A.col_idx, m, d_rm.ptr, d_rank.ptr, jn.ptr, o_ptr.ptr, t_off.ptr, stream.ptr),
+           "map")
 
     def scan():
         k = launch_no[0]
         launch_no[0] += 1
         lb = Lookback(status.ptr, tickets.ptr + 4 * k, err.ptr, k + 1)
-        ck(L.gpk_scan_lb_i32(o_ptr.ptr, o_ptr.ptr, m_new, lb, stream.ptr), "scan")
+        ck(L.gpk_scan_lb_i32(t_off.ptr, t_off.ptr, tiles, lb, stream.ptr), "scan")
 
     def write():
-        ck(L.gpk_compact_write(A.row_ptr, A.val, m, d_rm.ptr, jn.ptr, o_ptr.ptr, o_idx.ptr, o_val.ptr, stream.ptr),
-           "write")
+        ck(L.gpk_compact_write(A.row_ptr, A.val, m, d_rm.ptr, jn.ptr, t_off.ptr, o_ptr.ptr, m_new, o_idx.ptr,
+                               o_val.ptr, stream.ptr), "write")
 
     def whole():
         rank()
@@ -222,10 +235,11 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
             or not np.array_equal(o_idx.numpy()[:nnz_new], col_map[lp["A_idx"][kept]])
             or not np.array_equal(o_val.numpy()[:nnz_new], lp["A_val"][kept])):
         raise SystemExit("bench.py: the compaction differs from the host")
-    # Each kernel apart (the map needs the rank, the write the scanned pointers).
+    # Each kernel apart (the map needs the rank, the write the scanned offsets
+    # and the map's counts, so the write is timed after map + scan).
     parts = {}
-    for name, f in (("k_build_rank", rank), ("k_compact_map", cmap), ("k_scan_lb", None), ("k_compact_write", write)):
-        if f is None:
+    for name, f in (("k_build_rank", rank), ("k_compact_map", cmap), ("tile scan", None), ("k_compact_write", None)):
+        if name == "tile scan":
             e0, e1 = gp_ffi.Event(), gp_ffi.Event()
             e0.record(stream)
             for _ in range(launches):
@@ -234,6 +248,16 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
             e1.record(stream)
             parts[name] = 1e3 * e0.elapsed_ms(e1) / launches - parts["k_compact_map"]
             continue
+        if name == "k_compact_write":
+            e0, e1 = gp_ffi.Event(), gp_ffi.Event()
+            e0.record(stream)
+            for _ in range(launches):
+                cmap()
+                scan()
+                write()
+            e1.record(stream)
+            parts[name] = 1e3 * e0.elapsed_ms(e1) / launches - parts["k_compact_map"] - parts["tile scan"]
+            continue
         e0, e1 = gp_ffi.Event(), gp_ffi.Event()
         e0.record(stream)
         for _ in range(launches):
@@ -241,8 +265,10 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
         e1.record(stream)
         parts[name] = 1e3 * e0.elapsed_ms(e1) / launches
     nbytes = compaction_bytes(m, n, nnz, m_new, nnz_new)
+    floor = compaction_compulsory_bytes(m, n, nnz, m_new, nnz_new)
     log("compaction kernels (us): " + ", ".join(f"{k} {v:.1f}" for k, v in parts.items()))
     return {"us": us, "bytes": nbytes, "achieved": nbytes / (us * 1e-6) / 1e9, "kernels_us": parts,
+            "compulsory_bytes": floor, "compulsory_achieved": floor / (us * 1e-6) / 1e9,
             "workload": f"A of the LP ({m}x{n}, {nnz} nnz) with 15% of rows and columns dropped "
                         f"-> {m_new} rows, {nnz_new} nnz"}
 
@@ -276,7 +302,7 @@ def activity_traffic():
 def compaction_traffic():
     """HBM bytes of one compaction call: the sum over its kernels in the
     committed PMC summary, or None when one of them is missing."""
-    parts = [profiled_traffic(k) for k in ("k_build_rank", "k_compact_map", "k_scan_lb<int, 0>", "k_compact_write")]
+    parts = [profiled_traffic(k) for k in ("k_build_rank", "k_compact_map", "k_scan_small<int, 0>", "k_compact_write")]
     return None if any(p is None for p in parts) else sum(parts)
 
 
@@ -285,6 +311,10 @@ def _upload_pool(gp_ffi, lps):
     d = [gp_ffi.DeviceLp(lp) for lp in lps]
     log(f"uploaded {len(d)} LPs in {time.time() - t:.1f}s")
     return d
+
+
+def _worker_candidates(args):
+    return tuple(int(w) for w in args.workers.split(",") if w.strip())
 
 
 def bench_batched(args):
@@ -311,7 +341,7 @@ def bench_batched(args):
         return gp_ffi.serve_trace(dlps, jobs, prm, gp_ffi.serve_config(workers=w, postsolve=False))
 
     run(4, sample[:32])  # warm-up: contexts, pools, code objects
-    best, table = workloads.autotune_workers(run)
+    best, table = workloads.autotune_workers(run, _worker_candidates(args))
     log(f"autotune (LP/s on a {len(sample)}-LP sample): {table} -> {best} workers")
     allj = np.arange(len(dlps), dtype=np.int32)
     res = run(best, allj)
@@ -371,7 +401,7 @@ def bench_serving(args):
 
     sample = lp_of_job[:64]
     run(4, sample[:16])  # warm-up
-    best, table = workloads.autotune_workers(lambda w: run(w, sample))
+    best, table = workloads.autotune_workers(lambda w: run(w, sample), _worker_candidates(args))
     log(f"autotune (LP/s, 64 jobs at once): {table} -> {best} workers")
     res = run(best, lp_of_job, arrival)
     bad = [r for r in res if r["status"] != 0]
@@ -412,6 +442,8 @@ def main():
                     help="plain: the single large LP (default); batched: 4,096 small LPs at once against "
                          "one-at-a-time plain presolve; serving: Poisson presolve+postsolve requests")
     ap.add_argument("--batch-lps", type=int, default=4096)
+    ap.add_argument("--workers", type=str, default="1,2,4,8,12,16",
+                    help="worker counts the autotuner tries (batched and serving)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool", type=int, default=48)
@@ -521,10 +553,12 @@ def main():
                    "reduced": [summary["reduced_rows"], summary["reduced_cols"], summary["reduced_nnz"]],
                    "iterations": summary["iterations"], "postsolve_ms": post_ms},
         "roofline": {"bound": "hbm",
-                     "kernel": "compaction: k_build_rank + k_compact_map + k_scan_lb + k_compact_write",
+                     "kernel": "compaction: k_build_rank + k_compact_map + tile-offset scan + k_compact_write",
                      "achieved": comp["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": comp["achieved"] / HBM_PEAK_GBS, "traffic": compaction_traffic(),
                      "us_per_call": comp["us"], "algorithmic_bytes": comp["bytes"],
+                     "compulsory_bytes": comp["compulsory_bytes"],
+                     "compulsory_frac": comp["compulsory_achieved"] / HBM_PEAK_GBS,
                      "workload": comp["workload"], "kernels_us": comp["kernels_us"]},
         "roofline_activity": {"bound": "hbm", "kernel": "k_pack_lu + k_row_activity_lu (one gp_row_activity call)",
                               "achieved": act["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -58,6 +58,9 @@ class Ctx {
   // the host (synchronizes the stream).
   const int32_t* read_flags(int count = kFlags);
   void sync(const char* context = "stream synchronize");
+  // Number of host synchronizations so far (each read_flags / to_host /
+  // sync counts one); the presolve step profiler reports it per step.
+  int64_t sync_count() const { return syncs_; }
 
   template <class T>
   std::vector<T> to_host(const T* dev, int64_t count, const char* context) {
@@ -104,6 +107,7 @@ class Ctx {
   uint64_t* d_status_ = nullptr;
   int64_t status_cap_ = 0;
   uint32_t epoch_ = 0;
+  int64_t syncs_ = 0;
   int32_t* h_error_ = nullptr;  // host-mapped
   int32_t* d_error_ = nullptr;
   int32_t* h_flags_ = nullptr;

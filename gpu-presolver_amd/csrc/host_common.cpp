@@ -110,6 +110,7 @@ const int32_t* Ctx::read_flags(int count) {
 }
 
 void Ctx::sync(const char* context) {
+  ++syncs_;
   check(hipStreamSynchronize(stream_), context);
   if (*h_error_ != 0) {
     *h_error_ = 0;
@@ -307,13 +308,15 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
   DBuf<int32_t> jn(A.nnz, "compact entry map");
   hipStream_t s = c.stream();
   check(gpk_build_rank(col_map, A.cols, rank, s), "compact column rank");
-  check(gpk_compact_map(A.rowPtr, A.colVal, m_old, row_map, rank, jn, ptr, s), "compact map");
-  check(gpk_scan_lb_i32(ptr, ptr, m_new, c.lookback(gpk_lookback_tiles(m_new)), s), "compact row pointers");
+  const int32_t tiles = gpk_compact_tiles(m_old);
+  DBuf<int32_t> tile_off(static_cast<int64_t>(tiles) + 1, "compact tile offsets");
+  check(gpk_compact_map(A.rowPtr, A.colVal, m_old, row_map, rank, jn, ptr, tile_off, s), "compact map");
+  check(gpk_scan_lb_i32(tile_off, tile_off, tiles, c.lookback(gpk_lookback_tiles(tiles)), s), "compact tile offsets");
   // Kept entries are at most A.nnz: the outputs are sized for that and the
   // kept count is read back once the entries are written.
   DBuf<int32_t> idx(A.nnz, "compact colVal");
   DBuf<double> val(A.nnz, "compact nzVal");
-  check(gpk_compact_write(A.rowPtr, A.nzVal, m_old, row_map, jn, ptr, idx, val, s), "compact write");
+  check(gpk_compact_write(A.rowPtr, A.nzVal, m_old, row_map, jn, tile_off, ptr, m_new, idx, val, s), "compact write");
   out.nnz = known_nnz >= 0 ? known_nnz : c.scalar_to_host(ptr.get() + m_new, "compact kept count");
   out.rowPtr = ptr.release();
   out.colVal = idx.release();

@@ -458,9 +458,10 @@ class StepProfile {
   StepProfile() : on_(std::getenv("GP_PRESOLVE_PROFILE") != nullptr) {}
   ~StepProfile() {
     if (!on_) return;
-    std::fprintf(stderr, "presolve profile (ms, calls):\n");
+    std::fprintf(stderr, "presolve profile (ms, calls, host synchronizations):\n");
     for (const auto& kv : t_)
-      std::fprintf(stderr, "  %-34s %9.3f %6d\n", kv.first.c_str(), 1e3 * kv.second.second, kv.second.first);
+      std::fprintf(stderr, "  %-34s %9.3f %6d %7lld\n", kv.first.c_str(), 1e3 * kv.second.t, kv.second.calls,
+                   static_cast<long long>(kv.second.syncs));
   }
   template <class F>
   void run(const char* name, F&& f) {
@@ -469,17 +470,25 @@ class StepProfile {
       return;
     }
     ctx().sync("profile");
+    const int64_t s0 = ctx().sync_count();
     const auto t0 = std::chrono::steady_clock::now();
     f();
+    const int64_t s1 = ctx().sync_count();
     ctx().sync("profile");
     auto& e = t_[name];
-    ++e.first;
-    e.second += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    ++e.calls;
+    e.syncs += s1 - s0;
+    e.t += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 
  private:
+  struct Entry {
+    int calls = 0;
+    int64_t syncs = 0;
+    double t = 0.0;
+  };
   bool on_;
-  std::map<std::string, std::pair<int, double>> t_;
+  std::map<std::string, Entry> t_;
 };
 
 // run_gpu_presolve_impl (gpu_presolve.cu:2162-3400) without the six

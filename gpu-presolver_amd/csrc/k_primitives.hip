@@ -121,8 +121,9 @@ __global__ __launch_bounds__(kBlock) void k_row_activity_lu(const int32_t* __res
                                                             double* __restrict__ max_fin,
                                                             int32_t* __restrict__ neg_inf,
                                                             int32_t* __restrict__ pos_inf) {
-  __shared__ double s_tmin[kActCap];
-  __shared__ double s_tmax[kActCap];
+  // One term array (the min terms, then the max terms): 16 KB of LDS per
+  // block, so eight blocks fit a CU and their load phases overlap.
+  __shared__ double s_t[kActCap];
   __shared__ int32_t s_ptr[kActRows + 1];
   const int32_t r0 = blockIdx.x * kActRows;
   const int32_t rows = min(kActRows, m - r0);
@@ -153,24 +154,30 @@ __global__ __launch_bounds__(kBlock) void k_row_activity_lu(const int32_t* __res
       if (w + tid + k * kBlock >= we) a[k] = 0.0;
       b[k] = lu[fabs(a[k]) > ztol ? j[k] : 0];
     }
+    const int32_t rb = max(my_b, w), re = min(my_e, we);
+    double tmx[kActItems];  // the max terms wait in registers while the min terms are summed
 #pragma unroll
     for (int k = 0; k < kActItems; ++k) {
       const int32_t p = w + tid + k * kBlock;
-      if (p >= we) break;
-      double tmn = 0.0, tmx = 0.0;
-      if (fabs(a[k]) > ztol) {
-        tmn = a[k] >= 0.0 ? a[k] * b[k].x : a[k] * b[k].y;
-        tmx = a[k] >= 0.0 ? a[k] * b[k].y : a[k] * b[k].x;
-      }
-      s_tmin[p - w] = tmn;
-      s_tmax[p - w] = tmx;
+      const bool on = fabs(a[k]) > ztol;
+      tmx[k] = on ? (a[k] >= 0.0 ? a[k] * b[k].y : a[k] * b[k].x) : 0.0;
+      if (p < we) s_t[p - w] = on ? (a[k] >= 0.0 ? a[k] * b[k].x : a[k] * b[k].y) : 0.0;
     }
     __syncthreads();
-    const int32_t rb = max(my_b, w), re = min(my_e, we);
     for (int32_t p = rb; p < re; ++p) {
-      const double tmn = s_tmin[p - w], tmx = s_tmax[p - w];
-      if (isfinite(tmn)) mn += tmn; else if (tmn < 0.0) ++ni;
-      if (isfinite(tmx)) mx += tmx; else if (tmx > 0.0) ++pi;
+      const double t = s_t[p - w];
+      if (isfinite(t)) mn += t; else if (t < 0.0) ++ni;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kActItems; ++k) {
+      const int32_t p = w + tid + k * kBlock;
+      if (p < we) s_t[p - w] = tmx[k];
+    }
+    __syncthreads();
+    for (int32_t p = rb; p < re; ++p) {
+      const double t = s_t[p - w];
+      if (isfinite(t)) mx += t; else if (t > 0.0) ++pi;
     }
     __syncthreads();
   }
@@ -405,13 +412,15 @@ __global__ __launch_bounds__(kSmallBlock) void k_scan_small(const In* in, int32_
   }
 }
 
-// Compaction in three launches (gpu_presolve.cu:1881-1978): k_compact_map
-// maps every entry once (row kept and column rank, one cache-resident rank
-// gather per entry), stores the new column index or -1 per entry and each
-// kept row's kept count at out_ptr[new row]; an exclusive scan makes out_ptr
-// the new row pointers; k_compact_write streams the mapped indices and the
-// values and writes each tile's kept entries, positioned by wave ballots and
-// staged in LDS for coalesced stores.  Tiles are 256 old rows.
+// Compaction in three launches (gpu_presolve.cu:1881-1978) over tiles of 256
+// old rows: k_compact_map maps every entry once (row kept and column rank,
+// one cache-resident rank gather per entry), stores the new column index or
+// -1 per entry, each kept row's kept count at out_ptr[new row] and the
+// tile's kept count; an exclusive scan of the tile counts (one small array,
+// m/256 entries) gives every tile its output offset; k_compact_write turns
+// its rows' counts into row pointers (a block scan plus the tile offset),
+// then streams the mapped indices and the values and writes the kept
+// entries, positioned by wave ballots and staged in LDS for coalesced stores.
 constexpr int kCompRows = kBlock;
 
 // Rank form of a keep map (map[j] = new index or -1), one 16-byte word per 64
@@ -465,8 +474,10 @@ __global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restric
                                                         const int32_t* __restrict__ idx, int32_t m_old,
                                                         const int32_t* __restrict__ row_map,
                                                         const gpk_rank_word* __restrict__ col_rank,
-                                                        int32_t* __restrict__ jn_out, int32_t* __restrict__ out_ptr) {
+                                                        int32_t* __restrict__ jn_out, int32_t* __restrict__ out_ptr,
+                                                        int32_t* __restrict__ tile_cnt) {
   __shared__ int32_t s_cnt[kCompRows];
+  __shared__ int32_t s_wsum[kBlock / 64];
   __shared__ uint8_t s_keep[kCompRows];
   __shared__ uint8_t s_row[kCompChunk];  // tile-local row of each entry of the chunk
   static_assert(kCompRows <= 256, "tile-local rows are stored in one byte");
@@ -508,32 +519,52 @@ __global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restric
     }
     __syncthreads();  // s_row is relabelled for the next chunk
   }
-  if (my_new_row >= 0) out_ptr[my_new_row] = s_cnt[tid];
+  const int32_t mine = my_new_row >= 0 ? s_cnt[tid] : 0;
+  if (my_new_row >= 0) out_ptr[my_new_row] = mine;
+  const int32_t inc = wave_inclusive_sum(mine);
+  if (lane_id() == 63) s_wsum[tid >> 6] = inc;
+  __syncthreads();
+  if (tid == 0) {
+    int32_t t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += s_wsum[w];
+    tile_cnt[blockIdx.x] = t;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restrict__ ptr,
                                                           const double* __restrict__ val, int32_t m_old,
                                                           const int32_t* __restrict__ row_map,
                                                           const int32_t* __restrict__ jn_in,
-                                                          const int32_t* __restrict__ out_ptr,
+                                                          const int32_t* __restrict__ tile_off,
+                                                          int32_t* __restrict__ out_ptr, int32_t m_new,
                                                           int32_t* __restrict__ out_idx,
                                                           double* __restrict__ out_val) {
   constexpr int kWaves = kBlock / 64;
   __shared__ int32_t s_oidx[kCompChunk];
   __shared__ double s_oval[kCompChunk];
   __shared__ int32_t s_wcnt[kCompItems * kWaves];
-  __shared__ int32_t s_first, s_total;
+  __shared__ int32_t s_wsum[kWaves];
+  __shared__ int32_t s_total;
   const int32_t r0 = blockIdx.x * kCompRows;
   const int rows = min(kCompRows, m_old - r0);
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = tid >> 6;
   const int32_t my_new_row = tid < rows ? row_map[r0 + tid] : -1;
-  if (tid == 0) s_first = INT_MAX;
-  __syncthreads();
-  if (my_new_row >= 0) atomicMin(&s_first, my_new_row);
-  __syncthreads();
-  if (s_first == INT_MAX) return;  // no kept row in the tile
-  int32_t base = out_ptr[s_first];
+  int32_t base = tile_off[blockIdx.x];
+  if (blockIdx.x == gridDim.x - 1 && tid == 0) out_ptr[m_new] = tile_off[gridDim.x];
+  {  // row pointers of the tile's kept rows: tile offset + block exclusive scan of the counts
+    const int32_t cnt = my_new_row >= 0 ? out_ptr[my_new_row] : 0;
+    const int32_t inc = wave_inclusive_sum(cnt);
+    if (lane == 63) s_wsum[wave] = inc;
+    __syncthreads();
+    int32_t before = 0, all = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      before += w < wave ? s_wsum[w] : 0;
+      all += s_wsum[w];
+    }
+    if (my_new_row >= 0) out_ptr[my_new_row] = base + before + inc - cnt;
+    if (all == 0) return;  // no kept entry in the tile (the same in every thread)
+  }
   const int32_t seg_b = ptr[r0], seg_e = ptr[r0 + rows];
   const uint64_t below = lanemask_lt(lane);
   for (int32_t c = seg_b; c < seg_e; c += kCompChunk) {
@@ -878,20 +909,23 @@ hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hip
   return hipGetLastError();
 }
 
+int32_t gpk_compact_tiles(int32_t m_old) { return m_old > 0 ? static_cast<int32_t>(grid_for(m_old, kCompRows)) : 0; }
+
 hipError_t gpk_compact_map(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
-                           const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, hipStream_t s) {
+                           const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, int32_t* tile_cnt,
+                           hipStream_t s) {
   if (m_old <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_compact_map, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, idx, m_old, row_map,
-                     col_rank, jn, out_ptr);
+                     col_rank, jn, out_ptr, tile_cnt);
   return hipGetLastError();
 }
 
 hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
-                             const int32_t* jn, const int32_t* out_ptr, int32_t* out_idx, double* out_val,
-                             hipStream_t s) {
+                             const int32_t* jn, const int32_t* tile_off, int32_t* out_ptr, int32_t m_new,
+                             int32_t* out_idx, double* out_val, hipStream_t s) {
   if (m_old <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_compact_write, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, val, m_old, row_map,
-                     jn, out_ptr, out_idx, out_val);
+                     jn, tile_off, out_ptr, m_new, out_idx, out_val);
   return hipGetLastError();
 }
 

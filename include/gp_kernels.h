@@ -72,18 +72,23 @@ typedef struct __attribute__((aligned(16))) gpk_rank_word {
 } gpk_rank_word;
 hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hipStream_t stream);
 /* Stable compaction of a CSR by an old-to-new row map and the rank form of
- * the column map, in three launches over tiles of 256 old rows:
+ * the column map, over tiles of 256 old rows (gpk_compact_tiles(m_old)):
  *   gpk_compact_map    jn[p] = new column of entry p, or -1 when its row or
  *                      column is dropped; out_ptr[new row] = kept count;
- *   gpk_scan_lb_i32(out_ptr, out_ptr, m_new, ...) turns the counts into row
- *                      pointers (out_ptr[m_new] = kept entries);
- *   gpk_compact_write  writes the kept (column, value) pairs.
- * jn holds one int32 per entry of A. */
+ *                      tile_cnt[t] = kept entries of tile t;
+ *   gpk_scan_lb_i32(tile_cnt, tile_cnt, tiles, ...) makes the tile counts
+ *                      exclusive offsets (tile_cnt[tiles] = kept entries);
+ *   gpk_compact_write  turns out_ptr into the new row pointers
+ *                      (out_ptr[m_new] = kept entries) and writes the kept
+ *                      (column, value) pairs.
+ * jn holds one int32 per entry of A, tile_cnt tiles + 1. */
+int32_t gpk_compact_tiles(int32_t m_old);
 hipError_t gpk_compact_map(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
-                           const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, hipStream_t stream);
+                           const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, int32_t* tile_cnt,
+                           hipStream_t stream);
 hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
-                             const int32_t* jn, const int32_t* out_ptr, int32_t* out_idx, double* out_val,
-                             hipStream_t stream);
+                             const int32_t* jn, const int32_t* tile_off, int32_t* out_ptr, int32_t m_new,
+                             int32_t* out_idx, double* out_val, hipStream_t stream);
 
 /* old_to_new[i] = mask[i] ? prefix[i] : -1. */
 hipError_t gpk_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* old_to_new, int32_t n,
