@@ -55,12 +55,27 @@ class Ctx {
   // Returns kFlags zeroed device flag words.
   int32_t* flags();
   // Copies the first `count` words of the flags last returned by flags() to
-  // the host (synchronizes the stream).
+  // the host (waits for the stream).
   const int32_t* read_flags(int count = kFlags);
+  // Waits for everything enqueued on the stream so far.
   void sync(const char* context = "stream synchronize");
+  // Enqueues one small kernel that copies `words` (<= kMailWords) 32-bit
+  // words from device memory into host-mapped memory and then stores a new
+  // sequence number there; the host polls that word.  Returns the words once
+  // the whole stream up to that kernel has finished.  This avoids a copy
+  // command plus hipStreamSynchronize per decision, which is what limits
+  // many presolve jobs sharing one GPU.
+  const uint32_t* mail(const void* dev, int words, const char* context);
+  // A device-to-host copy into ordinary host memory was enqueued: the next
+  // sync() uses hipStreamSynchronize, which also completes such copies.
+  void note_host_copy() { ++host_copies_; }
+  static constexpr int kMailWords = 16;
   // Number of host synchronizations so far (each read_flags / to_host /
   // sync counts one); the presolve step profiler reports it per step.
   int64_t sync_count() const { return syncs_; }
+  // Seconds the host has spent waiting for the stream (mail polls and
+  // stream synchronizations).
+  double wait_seconds() const { return wait_s_; }
 
   template <class T>
   std::vector<T> to_host(const T* dev, int64_t count, const char* context) {
@@ -68,6 +83,7 @@ class Ctx {
     if (count > 0) {
       check(hipMemcpyAsync(out.data(), dev, sizeof(T) * static_cast<size_t>(count), hipMemcpyDeviceToHost, stream_),
             context);
+      note_host_copy();
       sync(context);
     }
     return out;
@@ -77,15 +93,22 @@ class Ctx {
   template <class T>
   void to_host_async(std::vector<T>& out, const T* dev, int64_t count, const char* context) {
     out.resize(static_cast<size_t>(count > 0 ? count : 0));
-    if (count > 0)
+    if (count > 0) {
       check(hipMemcpyAsync(out.data(), dev, sizeof(T) * static_cast<size_t>(count), hipMemcpyDeviceToHost, stream_),
             context);
+      note_host_copy();
+    }
   }
   template <class T>
   T scalar_to_host(const T* dev, const char* context) {
+    static_assert(sizeof(T) % 4 == 0 && sizeof(T) <= 4 * kMailWords, "scalar of whole 32-bit words");
+    T v;
+    if (host_copies_ == 0) {
+      std::memcpy(&v, mail(dev, static_cast<int>(sizeof(T) / 4), context), sizeof(T));
+      return v;
+    }
     check(hipMemcpyAsync(pinned_, dev, sizeof(T), hipMemcpyDeviceToHost, stream_), context);
     sync(context);
-    T v;
     std::memcpy(&v, pinned_, sizeof(T));
     return v;
   }
@@ -93,6 +116,7 @@ class Ctx {
   void to_device(T* dev, const T* host, int64_t count, const char* context) {
     if (count <= 0) return;
     check(hipMemcpyAsync(dev, host, sizeof(T) * static_cast<size_t>(count), hipMemcpyHostToDevice, stream_), context);
+    note_host_copy();  // the source is ordinary host memory: the next sync() completes the copy fully
   }
 
  private:
@@ -108,6 +132,12 @@ class Ctx {
   int64_t status_cap_ = 0;
   uint32_t epoch_ = 0;
   int64_t syncs_ = 0;
+  double wait_s_ = 0.0;
+  int64_t host_copies_ = 0;  // enqueued copies into ordinary host memory since the last full sync
+  uint32_t* h_mail_ = nullptr;  // host-mapped: [0] sequence, [4, 4 + kMailWords) payload
+  uint32_t* d_mail_ = nullptr;
+  uint32_t mail_seq_ = 0;
+  void check_error_flag(const char* context);
   int32_t* h_error_ = nullptr;  // host-mapped
   int32_t* d_error_ = nullptr;
   int32_t* h_flags_ = nullptr;
@@ -179,6 +209,33 @@ class DBuf {
   T* p_ = nullptr;
   int64_t n_ = 0;
   hipStream_t s_ = nullptr;
+};
+
+// Device copies and zero fills collected and issued together: one launch
+// per GPK_COPY_MAX ranges on the context's stream (in place of one copy or
+// fill command each).
+class CopyBatch {
+ public:
+  template <class T>
+  CopyBatch& copy(T* dst, const T* src, int64_t n) {
+    if (n > 0) segs_.push_back(gpk_copy_seg{dst, src, static_cast<int64_t>(sizeof(T)) * n});
+    return *this;
+  }
+  template <class T>
+  CopyBatch& zero(T* dst, int64_t n) {
+    if (n > 0) segs_.push_back(gpk_copy_seg{dst, nullptr, static_cast<int64_t>(sizeof(T)) * n});
+    return *this;
+  }
+  void run(const char* context) {
+    for (size_t k = 0; k < segs_.size(); k += GPK_COPY_MAX) {
+      const size_t cnt = segs_.size() - k < GPK_COPY_MAX ? segs_.size() - k : GPK_COPY_MAX;
+      check(gpk_copy_many(segs_.data() + k, static_cast<int32_t>(cnt), ctx().stream()), context);
+    }
+    segs_.clear();
+  }
+
+ private:
+  std::vector<gpk_copy_seg> segs_;
 };
 
 // ---- views -----------------------------------------------------------------

@@ -3,9 +3,12 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
 #include <limits>
 #include <mutex>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -56,6 +59,10 @@ Ctx::Ctx(hipStream_t stream, bool own_stream) {
   check(hipHostMalloc(&h_error_, sizeof(int32_t), hipHostMallocMapped), "hipHostMalloc error flag");
   *h_error_ = 0;
   check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_error_), h_error_, 0), "hipHostGetDevicePointer");
+  check(hipHostMalloc(&h_mail_, sizeof(uint32_t) * (4 + kMailWords), hipHostMallocMapped | hipHostMallocCoherent),
+        "hipHostMalloc mail");
+  std::memset(h_mail_, 0, sizeof(uint32_t) * (4 + kMailWords));
+  check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_mail_), h_mail_, 0), "hipHostGetDevicePointer mail");
 }
 
 gpk_lookback Ctx::lookback(int64_t tiles) {
@@ -86,6 +93,7 @@ Ctx::~Ctx() {
   (void)hipFree(d_tickets_);
   if (d_status_) (void)hipFree(d_status_);
   (void)hipHostFree(h_error_);
+  (void)hipHostFree(h_mail_);
   (void)hipHostFree(h_flags_);
   (void)hipHostFree(pinned_);
   if (own_stream_) (void)hipStreamDestroy(stream_);
@@ -103,19 +111,60 @@ int32_t* Ctx::flags() {
 }
 
 const int32_t* Ctx::read_flags(int count) {
-  check(hipMemcpyAsync(h_flags_, cur_ ? cur_ : d_flags_, sizeof(int32_t) * count, hipMemcpyDeviceToHost, stream_),
-        "read flags");
+  const int32_t* src = cur_ ? cur_ : d_flags_;
+  if (host_copies_ == 0 && count <= kMailWords)
+    return reinterpret_cast<const int32_t*>(mail(src, count, "read flags"));
+  check(hipMemcpyAsync(h_flags_, src, sizeof(int32_t) * count, hipMemcpyDeviceToHost, stream_), "read flags");
   sync("read flags");
   return h_flags_;
 }
 
-void Ctx::sync(const char* context) {
-  ++syncs_;
-  check(hipStreamSynchronize(stream_), context);
+void Ctx::check_error_flag(const char* context) {
   if (*h_error_ != 0) {
     *h_error_ = 0;
     throw std::runtime_error(std::string(context) + ": a single-pass scan exceeded its spin limit");
   }
+}
+
+void Ctx::sync(const char* context) {
+  if (host_copies_ == 0) {
+    (void)mail(nullptr, 0, context);
+    return;
+  }
+  ++syncs_;
+  const auto t0 = std::chrono::steady_clock::now();
+  const hipError_t e = hipStreamSynchronize(stream_);
+  wait_s_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  check(e, context);
+  host_copies_ = 0;
+  check_error_flag(context);
+}
+
+const uint32_t* Ctx::mail(const void* dev, int words, const char* context) {
+  ++syncs_;
+  if (++mail_seq_ == 0) mail_seq_ = 1;
+  const uint32_t seq = mail_seq_;
+  check(gpk_mail(static_cast<const uint32_t*>(dev), words, d_mail_, seq, stream_), context);
+  volatile uint32_t* word = h_mail_;
+  const auto t0 = std::chrono::steady_clock::now();
+  // Spin briefly (most decisions wait a few microseconds), then yield; every
+  // few thousand polls the stream is asked for an error so a failed launch
+  // does not leave the host waiting.
+  for (int64_t polls = 0; word[0] != seq; ++polls) {
+    if (polls < 2048) {
+      __builtin_ia32_pause();
+      continue;
+    }
+    if ((polls & 4095) == 0) {
+      const hipError_t e = hipStreamQuery(stream_);
+      if (e != hipSuccess && e != hipErrorNotReady) throw HipError(context, e);
+    }
+    std::this_thread::yield();
+  }
+  std::atomic_thread_fence(std::memory_order_acquire);
+  wait_s_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  check_error_flag(context);
+  return const_cast<const uint32_t*>(h_mail_ + 4);
 }
 
 Ctx& ctx() {
@@ -335,7 +384,7 @@ DeviceCsrMatrix transpose_csr(const DeviceCsrMatrix& A) {
   T.nnz = A.nnz;
   DBuf<int32_t> ptr(static_cast<int64_t>(A.cols) + 1, "transpose rowPtr");
   DBuf<int32_t> counts(static_cast<int64_t>(A.cols) + 1, "transpose counts");
-  check(hipMemsetAsync(counts.get(), 0, sizeof(int32_t) * (static_cast<size_t>(A.cols) + 1), s), "memset counts");
+  CopyBatch().zero(counts.get(), static_cast<int64_t>(A.cols) + 1).run("transpose counts");
   if (A.nnz > 0) check(gpk_col_histogram(A.colVal, A.nnz, counts, s), "col histogram");
   {
     DBuf<int32_t> scratch(gpk_scan_scratch_ints(A.cols));

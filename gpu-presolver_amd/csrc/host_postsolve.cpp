@@ -454,12 +454,8 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
     const int32_t nf = static_cast<int32_t>(record.fixed_val.size());
     fval.alloc(nf);
     c.to_device(fval.get(), record.fixed_val.data(), nf, "fixed_val upload");
-    if (n0 > 0) {
-      check(hipMemsetAsync(res.x_org, 0, sizeof(double) * n0, s), "x_org clear");
-      check(hipMemsetAsync(res.z_org, 0, sizeof(double) * n0, s), "z_org clear");
-      check(hipMemsetAsync(zret.get(), 0, n0, s), "z_retrieved clear");
-    }
-    if (m0 > 0) check(hipMemsetAsync(res.y_org, 0, sizeof(double) * m0, s), "y_org clear");
+    amd::CopyBatch().zero(res.x_org, n0).zero(res.z_org, n0).zero(zret.get(), n0).zero(res.y_org, m0).run(
+        "postsolve output clear");
     if (x_red) check(gpk_scatter(x_red, col_map, record.n1, res.x_org, s), "scatter x");
     if (y_red && m0 > 0) check(gpk_scatter(y_red, row_map, record.m1, res.y_org, s), "scatter y");
     if (z_red) check(gpk_scatter(z_red, col_map, record.n1, res.z_org, s), "scatter z");

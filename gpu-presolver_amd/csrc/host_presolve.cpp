@@ -97,7 +97,9 @@ void d2d(T* dst, const T* src, int64_t n, const char* what) {
 // (gpu_presolve.cu:197-247), in one contiguous block.
 struct PhasePlan {
   PresolvePlanGpu p;
-  explicit PhasePlan(const LPInfoGpu& lp) {
+  // init = false leaves the vectors to PhaseStats(lp, &plan), which fills
+  // them in the same launch as the statistics.
+  explicit PhasePlan(const LPInfoGpu& lp, bool init = true) {
     const size_t m = static_cast<size_t>(lp.A.rows), n = static_cast<size_t>(lp.A.cols);
     const size_t bytes = align256(8 * n) * 3 + align256(8 * m) * 2 + align256(m) + align256(n);
     char* b = static_cast<char*>(amd::dev_alloc(bytes, ctx().stream(), "plan storage"));
@@ -115,8 +117,9 @@ struct PhasePlan {
     p.keep_row_mask = reinterpret_cast<uint8_t*>(b);
     b += align256(m);
     p.keep_col_mask = reinterpret_cast<uint8_t*>(b);
-    check(gpk_plan_init(lp.A.rows, lp.A.cols, amd::plan_view(p), lp.c, lp.l, lp.u, lp.AL, lp.AU, ctx().stream()),
-          "plan init");
+    if (init)
+      check(gpk_plan_init(lp.A.rows, lp.A.cols, amd::plan_view(p), lp.c, lp.l, lp.u, lp.AL, lp.AU, ctx().stream()),
+            "plan init");
   }
   ~PhasePlan() {
     if (p.has_new_A) amd::free_csr(p.new_A);
@@ -131,7 +134,8 @@ struct PhasePlan {
 // (gpu_presolve.cu:249-276, 696-732).
 struct PhaseStats {
   PresolveStatsGpu s;
-  explicit PhaseStats(const LPInfoGpu& lp) {
+  // fresh_plan: a PhasePlan built with init = false, initialised here too.
+  explicit PhaseStats(const LPInfoGpu& lp, PresolvePlanGpu* fresh_plan = nullptr) {
     const size_t m = static_cast<size_t>(lp.A.rows), n = static_cast<size_t>(lp.A.cols);
     const size_t bytes = align256(4 * m) * 2 + align256(8 * m) + align256(m) + align256(4 * n) * 2 +
                          align256(8 * n) + align256(n) * 2;
@@ -155,6 +159,15 @@ struct PhaseStats {
     b += align256(n);
     s.column_singleton_mask = reinterpret_cast<uint8_t*>(b);
     hipStream_t st = ctx().stream();
+    if (fresh_plan) {
+      const gpk_phase_stats v{s.row_nnz, s.singleton_row_mask, s.singleton_row_col, s.singleton_row_val, s.col_nnz,
+                              s.empty_col_mask, s.column_singleton_mask, s.column_singleton_row,
+                              s.column_singleton_val};
+      check(gpk_phase_init(lp.A.rows, lp.A.cols, amd::plan_view(*fresh_plan), lp.c, lp.l, lp.u, lp.AL, lp.AU,
+                           amd::csr_view(lp.A), amd::csr_view(lp.AT), v, st),
+            "phase init");
+      return;
+    }
     if (m > 0)
       check(gpk_csr_stats(lp.A.rowPtr, lp.A.colVal, lp.A.nzVal, lp.A.rows, s.row_nnz, nullptr, s.singleton_row_mask,
                           s.singleton_row_col, s.singleton_row_val, st),
@@ -317,6 +330,7 @@ void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t
           "fixed records idx");
     check(hipMemcpyAsync(rec.fixed_val.data() + base, ov.get(), sizeof(double) * k, hipMemcpyDeviceToHost, s),
           "fixed records val");
+    c.note_host_copy();
     c.sync("fixed records");
   }
   append_global_tape(rs, p.tape);
@@ -360,11 +374,13 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   const double obj_new = cur.lp.obj_constant + p.obj_constant_delta;
   if (cur.owns && !p.has_new_A && m_new == m && n_new == n) {
     update_record(rs, p, m, n, row_map, col_map, m_new, n_new, fixed);
-    d2d(cur.lp.c, p.new_c, n, "apply c");
-    d2d(cur.lp.l, p.new_l, n, "apply l");
-    d2d(cur.lp.u, p.new_u, n, "apply u");
-    d2d(cur.lp.AL, p.new_AL, m, "apply AL");
-    d2d(cur.lp.AU, p.new_AU, m, "apply AU");
+    amd::CopyBatch()
+        .copy(cur.lp.c, p.new_c, n)
+        .copy(cur.lp.l, p.new_l, n)
+        .copy(cur.lp.u, p.new_u, n)
+        .copy(cur.lp.AL, p.new_AL, m)
+        .copy(cur.lp.AU, p.new_AU, m)
+        .run("apply vectors");
     cur.lp.obj_constant = obj_new;
     return;
   }
@@ -405,6 +421,7 @@ LPInfoGpu export_lp(const LPInfoGpu& w) {
   hipStream_t s = c.stream();
   LPInfoGpu out = w;
   const bool pooled = amd::job_mode().pooled_outputs;
+  amd::CopyBatch copies;
   auto take = [&](auto*& dst, const auto* src, int64_t count, const char* what) {
     using T = std::remove_const_t<std::remove_pointer_t<decltype(src)>>;
     const size_t bytes = sizeof(T) * static_cast<size_t>(count > 0 ? count : 1);
@@ -412,8 +429,7 @@ LPInfoGpu export_lp(const LPInfoGpu& w) {
     if (pooled) p = amd::dev_alloc(bytes, s, what);
     else check(hipMalloc(&p, bytes), what);
     dst = static_cast<T*>(p);
-    if (count > 0)
-      check(hipMemcpyAsync(dst, src, sizeof(T) * static_cast<size_t>(count), hipMemcpyDeviceToDevice, s), what);
+    copies.copy(dst, src, count);
   };
   take(out.A.rowPtr, w.A.rowPtr, static_cast<int64_t>(w.A.rows) + 1, "reduced A rowPtr");
   take(out.A.colVal, w.A.colVal, w.A.nnz, "reduced A colVal");
@@ -426,6 +442,7 @@ LPInfoGpu export_lp(const LPInfoGpu& w) {
   take(out.u, w.u, w.A.cols, "reduced u");
   take(out.AL, w.AL, w.A.rows, "reduced AL");
   take(out.AU, w.AU, w.A.rows, "reduced AU");
+  copies.run("reduced LP export");
   c.sync("reduced LP export");
   return out;
 }
@@ -549,8 +566,8 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     std::optional<PhasePlan> opp;
     std::optional<PhaseStats> ost;
     prof.run("plan + stats", [&] {
-      opp.emplace(cur.lp);
-      ost.emplace(cur.lp);
+      opp.emplace(cur.lp, false);
+      ost.emplace(cur.lp, &opp->p);
     });
     PhasePlan& pp = *opp;
     PhaseStats& st = *ost;
@@ -576,8 +593,8 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     std::optional<PhasePlan> opp;
     std::optional<PhaseStats> ost;
     prof.run("plan + stats", [&] {
-      opp.emplace(cur.lp);
-      ost.emplace(cur.lp);
+      opp.emplace(cur.lp, false);
+      ost.emplace(cur.lp, &opp->p);
     });
     PhasePlan& pp = *opp;
     PhaseStats& st = *ost;
@@ -663,8 +680,8 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     const LPInfoGpu& v = cur.lp;
     const int32_t m = v.A.rows, n = v.A.cols;
     hipStream_t s = ctx().stream();
-    PhasePlan pp(v);
-    PhaseStats st(v);
+    PhasePlan pp(v, false);
+    PhaseStats st(v, &pp.p);
     PresolvePlanGpu& p = pp.p;
     const gpk_plan pv = amd::plan_view(p);
     const gpk_csr av = amd::csr_view(v.A), atv = amd::csr_view(v.AT);
@@ -766,8 +783,8 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     int rounds = 0, bound_only = 0, limit = cfg_bound_only;
     while (rounds < max_rounds && !terminal && !time_exceeded()) {
       limit = effective_limit();
-      PhasePlan pp(cur.lp);
-      PhaseStats st(cur.lp);
+      PhasePlan pp(cur.lp, false);
+      PhaseStats st(cur.lp, &pp.p);
       PresolvePlanGpu& p = pp.p;
       bool group_changed = false, structural = false;
       while (rounds < max_rounds && !terminal && !time_exceeded()) {

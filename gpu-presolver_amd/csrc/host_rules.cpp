@@ -50,7 +50,7 @@ void d2d(T* dst, const T* src, int64_t n, const char* context) {
 
 template <class T>
 void zero(T* dst, int64_t n, const char* context) {
-  if (n > 0) check(hipMemsetAsync(dst, 0, sizeof(T) * static_cast<size_t>(n), ctx().stream()), context);
+  amd::CopyBatch().zero(dst, n).run(context);
 }
 
 // Packs the indices i with mask[i] != 0 (ascending) and copies them to the host.
@@ -122,8 +122,7 @@ void apply_rule_singleton_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
   hipStream_t s = c.stream();
   DBuf<double> cl(n), cu(n);
   DBuf<uint8_t> remove(m);
-  d2d(cl.get(), plan.new_l, n, "singleton rows cand_l");
-  d2d(cu.get(), plan.new_u, n, "singleton rows cand_u");
+  amd::CopyBatch().copy(cl.get(), plan.new_l, n).copy(cu.get(), plan.new_u, n).run("singleton rows candidates");
   const gpk_plan P = plan_view(plan);
   check(gpk_singleton_rows_bounds(m, n, P, stats.singleton_row_mask, stats.singleton_row_col, stats.singleton_row_val,
                                   prm.zero_tol, cl, cu, remove, s),
@@ -316,6 +315,7 @@ void apply_rule_duplicate_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
       check(hipMemcpyAsync(hto.data(), d_to, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s), "duplicate rows");
       check(hipMemcpyAsync(hstep.data(), d_step, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s), "duplicate rows");
       check(hipMemcpyAsync(hrec.data(), d_rec, sizeof(double) * 5 * k, hipMemcpyDeviceToHost, s), "duplicate rows");
+      c.note_host_copy();
       c.sync("duplicate rows tape");
     }
     // Tape order: by merge step, then by deleted row.
@@ -446,13 +446,15 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
   }
   DBuf<double> AL_new(m), AU_new(m), side(n), contrib(n), c_new(n);
   DBuf<uint8_t> row_del(m), row_changed(m), col_del(n);
-  zero(row_del.get(), m, "cs row_del");
-  zero(row_changed.get(), m, "cs row_changed");
-  zero(col_del.get(), n, "cs col_del");
-  zero(side.get(), n, "cs side");
-  zero(contrib.get(), n, "cs contrib");
-  d2d(AL_new.get(), plan.new_AL, m, "cs AL copy");
-  d2d(AU_new.get(), plan.new_AU, m, "cs AU copy");
+  amd::CopyBatch()
+      .zero(row_del.get(), m)
+      .zero(row_changed.get(), m)
+      .zero(col_del.get(), n)
+      .zero(side.get(), n)
+      .zero(contrib.get(), n)
+      .copy(AL_new.get(), plan.new_AL, m)
+      .copy(AU_new.get(), plan.new_AU, m)
+      .run("column singletons workspace");
   f = c.flags();
   check(gpk_cs_process(n, m, P, stats.column_singleton_mask, stats.column_singleton_row, stats.column_singleton_val,
                        owner, rc, prm.bound_tol, prm.zero_tol, eq_mode ? 1 : 0, AL_new, AU_new, row_del, row_changed,
@@ -592,8 +594,7 @@ bool dbl_batch_once(PresolvePlanGpu& plan, const LPInfoGpu& lp, const DeviceCsrM
   amd::scan_mask_count(d.selected, m, prefix);
   check(gpk_pack_index(d.selected, prefix, m, sel, s), "doubleton pack selected");
   gpk_dbl_cols cols = w.cols();
-  zero(cols.is_elim, n, "doubleton is_elim");
-  zero(cols.is_fixed_keep, n, "doubleton is_fixed_keep");
+  amd::CopyBatch().zero(cols.is_elim, n).zero(cols.is_fixed_keep, n).run("doubleton column flags");
   DBuf<uint8_t> subst(k), fixedp(k);
   DBuf<double> osub(k), ofix(k), rv(12 * static_cast<int64_t>(k));
   DBuf<int32_t> ri(3 * static_cast<int64_t>(k));
@@ -921,8 +922,7 @@ void apply_rule_duplicate_columns(PresolvePlanGpu& plan, const LPInfoGpu& lp, co
   DBuf<uint8_t> del(n), fixed(n);
   DBuf<double> fixed_val(n), rec5(5 * static_cast<int64_t>(n)), contrib(n);
   DBuf<int32_t> merge_to(n);
-  zero(del.get(), n, "duplicate cols delete mask");
-  zero(fixed.get(), n, "duplicate cols fixed mask");
+  amd::CopyBatch().zero(del.get(), n).zero(fixed.get(), n).run("duplicate cols masks");
   int32_t* f = c.flags();
   check(gpk_dup_col_groups(AT, P, hash, cols, zt, ctol, otol, max_ratio, use_guard ? guard.get() : nullptr,
                            prm.duplicate_columns_scale_guard_min_groups, del, fixed, fixed_val, merge_to, rec5, f, s),

@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <deque>
 #include <memory>
 #include <mutex>
@@ -75,6 +77,10 @@ class Server {
     cv_work_.notify_all();
     for (auto& t : workers_) t.join();
     ctxs_.clear();
+    if (std::getenv("GP_SERVE_PROFILE") != nullptr && jobs_run_ > 0)
+      std::fprintf(stderr, "serving profile: %lld jobs, per job %.3f ms on a worker, of which %.3f ms waiting for "
+                   "the GPU\n", static_cast<long long>(jobs_run_), 1e3 * job_s_ / jobs_run_,
+                   1e3 * wait_s_ / jobs_run_);
   }
 
   double now() const {
@@ -113,6 +119,9 @@ class Server {
     double bytes;
   };
 
+  int64_t jobs_run_ = 0;  // worker-side totals for GP_SERVE_PROFILE
+  double job_s_ = 0.0, wait_s_ = 0.0;
+
   // FIFO admission: the head job starts when a worker is free and it fits
   // the budget next to the jobs in flight (or nothing else is in flight).
   bool admissible() const {
@@ -136,9 +145,16 @@ class Server {
         inflight_bytes_ += j.bytes;
         ++inflight_jobs_;
       }
+      const double w0 = ctx().wait_seconds();
+      const auto t0 = std::chrono::steady_clock::now();
       gp_job_result r = run(j, w);
+      const double dt = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      const double dw = ctx().wait_seconds() - w0;
       {
         std::lock_guard<std::mutex> lock(mu_);
+        ++jobs_run_;
+        job_s_ += dt;
+        wait_s_ += dw;
         inflight_bytes_ -= j.bytes;
         --inflight_jobs_;
         done_.push_back(r);

@@ -613,6 +613,38 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restr
   }
 }
 
+__global__ void k_mail(const uint32_t* __restrict__ src, int32_t words, uint32_t* mail, uint32_t seq) {
+  const int t = threadIdx.x;
+  if (t < words) mail[4 + t] = src[t];
+  __threadfence_system();
+  __syncthreads();
+  if (t == 0) *reinterpret_cast<volatile uint32_t*>(mail) = seq;
+}
+
+struct CopyList {
+  gpk_copy_seg seg[GPK_COPY_MAX];
+};
+
+template <class U>
+__device__ inline void copy_units(const gpk_copy_seg& g) {
+  const int64_t units = g.bytes / static_cast<int64_t>(sizeof(U));
+  U* d = static_cast<U*>(g.dst);
+  const U* src = static_cast<const U*>(g.src);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < units; i += stride)
+    d[i] = src ? src[i] : U{};
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_many(CopyList list) {
+  const gpk_copy_seg g = list.seg[blockIdx.y];
+  const uint64_t a = reinterpret_cast<uint64_t>(g.dst) | reinterpret_cast<uint64_t>(g.src) |
+                     static_cast<uint64_t>(g.bytes);
+  if ((a & 15u) == 0) copy_units<uint4>(g);
+  else if ((a & 7u) == 0) copy_units<uint2>(g);
+  else if ((a & 3u) == 0) copy_units<uint32_t>(g);
+  else copy_units<uint8_t>(g);
+}
+
 __global__ void k_gather5(const int32_t* map, int32_t n_old, const double* s0, double* d0, const double* s1,
                           double* d1, const double* s2, double* d2, const double* s3, double* d3,
                           const double* s4, double* d4) {
@@ -644,6 +676,9 @@ __global__ void k_gather_i32(const int32_t* map, int32_t n_old, const int32_t* s
 constexpr int kRsTile = 2048;
 constexpr int kRsWaveItems = 512;
 constexpr int kRsWaves = kBlock / 64;
+// Up to this many tiles, each scatter block reads the whole histogram
+// (256 x tiles words) instead of waiting for a scan launch.
+constexpr int kRsHistTilesMax = 64;
 
 template <class K>
 __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, int32_t n, int shift,
@@ -660,15 +695,35 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ key
   hist[static_cast<int64_t>(threadIdx.x) * tiles + blockIdx.x] = s_h[threadIdx.x];
 }
 
-template <class K>
+// kFromHist: offs is the raw histogram (digit-major, tiles per digit) and
+// every block derives its own digit offsets from it (for few tiles, this
+// saves the separate scan launch); otherwise offs is the scanned histogram.
+template <class K, bool kFromHist>
 __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ kin, const int32_t* __restrict__ vin,
                                                          K* __restrict__ kout, int32_t* __restrict__ vout,
                                                          int32_t n, int shift, int32_t tiles,
                                                          const int32_t* __restrict__ offs) {
   __shared__ int32_t s_cnt[kRsWaves][256];
   __shared__ int32_t s_off[kRsWaves][256];
+  __shared__ int32_t s_base[256];
+  __shared__ int32_t s_wsum[kRsWaves];
   const int tid = threadIdx.x;
   for (int i = tid; i < kRsWaves * 256; i += kBlock) s_cnt[i >> 8][i & 255] = 0;
+  if (kFromHist) {
+    static_assert(kBlock == 256, "one thread per digit");
+    int32_t total = 0, before = 0;
+    for (int32_t t = 0; t < tiles; ++t) {
+      const int32_t h = offs[static_cast<int64_t>(tid) * tiles + t];
+      total += h;
+      before += t < static_cast<int32_t>(blockIdx.x) ? h : 0;
+    }
+    const int32_t inc = wave_inclusive_sum(total);
+    if ((tid & 63) == 63) s_wsum[tid >> 6] = inc;
+    __syncthreads();
+    int32_t wbase = 0;
+    for (int w = 0; w < (tid >> 6); ++w) wbase += s_wsum[w];
+    s_base[tid] = wbase + inc - total + before;
+  }
   __syncthreads();
   const int wave = tid >> 6, lane = tid & 63;
   const uint64_t lt = lanemask_lt(lane);
@@ -719,7 +774,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
     const int64_t e = base + k * 64 + lane;
     if (e < n) {
       const int d = dig[k];
-      const int32_t pos = offs[static_cast<int64_t>(d) * tiles + blockIdx.x] + s_off[wave][d] + rank[k];
+      const int32_t tile_base = kFromHist ? s_base[d] : offs[static_cast<int64_t>(d) * tiles + blockIdx.x];
+      const int32_t pos = tile_base + s_off[wave][d] + rank[k];
       kout[pos] = kk[k];
       vout[pos] = vv[k];
     }
@@ -778,10 +834,61 @@ __global__ void k_sort_rows(const int32_t* ptr, int32_t* idx, double* val, int32
   }
 }
 
+// Sorts up to kSortSmallMax (key, value) pairs in one workgroup: a bitonic
+// network in LDS over (key, input position), so the order equals that of the
+// stable radix sort; the values follow through the positions.
+constexpr int kSortSmallMax = 4096;
+constexpr int kSortSmallBlock = 1024;
+
+template <class K>
+__global__ __launch_bounds__(kSortSmallBlock) void k_sort_small(K* __restrict__ keys, int32_t* __restrict__ vals,
+                                                               int32_t n) {
+  __shared__ K s_k[kSortSmallMax];
+  __shared__ int32_t s_v[kSortSmallMax];
+  __shared__ int16_t s_p[kSortSmallMax];
+  static_assert(kSortSmallMax <= 32768, "positions are stored in 16 bits");
+  const int tid = threadIdx.x;
+  int len = 1;
+  while (len < n) len <<= 1;
+  for (int i = tid; i < len; i += kSortSmallBlock) {
+    s_k[i] = i < n ? keys[i] : static_cast<K>(~static_cast<K>(0));
+    s_p[i] = static_cast<int16_t>(i);
+    if (i < n) s_v[i] = vals[i];
+  }
+  __syncthreads();
+  for (int size = 2; size <= len; size <<= 1) {
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = tid; t < (len >> 1); t += kSortSmallBlock) {
+        const int i = 2 * t - (t & (stride - 1));  // lower index of the pair
+        const int j = i + stride;
+        const bool up = (i & size) == 0;
+        const K ki = s_k[i], kj = s_k[j];
+        const int pi = s_p[i], pj = s_p[j];
+        const bool gt = ki > kj || (ki == kj && pi > pj);
+        if (gt == up) {
+          s_k[i] = kj;
+          s_k[j] = ki;
+          s_p[i] = static_cast<int16_t>(pj);
+          s_p[j] = static_cast<int16_t>(pi);
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = tid; i < n; i += kSortSmallBlock) {
+    keys[i] = s_k[i];
+    vals[i] = s_v[s_p[i]];
+  }
+}
+
 template <class In>
 hipError_t scan_impl(const In* in, int32_t* out, int64_t n, int32_t* scratch, hipStream_t s) {
   if (n <= 0) {
     return hipMemsetAsync(out, 0, sizeof(int32_t), s);
+  }
+  if (n <= kSmallScanMax) {
+    hipLaunchKernelGGL((k_scan_small<In, 0>), dim3(1), dim3(kSmallBlock), 0, s, in, out, n, nullptr);
+    return hipGetLastError();
   }
   const int64_t tiles = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_scan_tiles<In>, dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, in, n, scratch);
@@ -795,10 +902,15 @@ template <class K>
 hipError_t radix_impl(K* keys, int32_t* vals, K* keys_alt, int32_t* vals_alt, int32_t n, int32_t key_bits,
                       int32_t* scratch, hipStream_t s) {
   if (n <= 1) return hipSuccess;
+  if (n <= kSortSmallMax) {
+    hipLaunchKernelGGL(k_sort_small<K>, dim3(1), dim3(kSortSmallBlock), 0, s, keys, vals, n);
+    return hipGetLastError();
+  }
   const int32_t tiles = (n + kRsTile - 1) / kRsTile;
   int32_t* hist = scratch;
   int32_t* scan_scratch = scratch + static_cast<int64_t>(256) * tiles + 1;
   const int passes = key_bits <= 0 ? 1 : (key_bits + 7) / 8;
+  const bool from_hist = tiles <= kRsHistTilesMax;
   K* ka = keys;
   int32_t* va = vals;
   K* kb = keys_alt;
@@ -806,10 +918,15 @@ hipError_t radix_impl(K* keys, int32_t* vals, K* keys_alt, int32_t* vals_alt, in
   for (int pass = 0; pass < passes; ++pass) {
     const int shift = pass * 8;
     hipLaunchKernelGGL(k_radix_hist<K>, dim3(tiles), dim3(kBlock), 0, s, ka, n, shift, tiles, hist);
-    hipError_t e = scan_impl<int32_t>(hist, hist, static_cast<int64_t>(256) * tiles, scan_scratch, s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_radix_scatter<K>, dim3(tiles), dim3(kBlock), 0, s, ka, va, kb, vb, n, shift, tiles,
-                       static_cast<const int32_t*>(hist));
+    if (from_hist) {
+      hipLaunchKernelGGL((k_radix_scatter<K, true>), dim3(tiles), dim3(kBlock), 0, s, ka, va, kb, vb, n, shift,
+                         tiles, static_cast<const int32_t*>(hist));
+    } else {
+      hipError_t e = scan_impl<int32_t>(hist, hist, static_cast<int64_t>(256) * tiles, scan_scratch, s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_radix_scatter<K, false>), dim3(tiles), dim3(kBlock), 0, s, ka, va, kb, vb, n, shift,
+                         tiles, static_cast<const int32_t*>(hist));
+    }
     K* tk = ka; ka = kb; kb = tk;
     int32_t* tv = va; va = vb; vb = tv;
   }
@@ -962,6 +1079,30 @@ hipError_t gpk_csr_stats(const int32_t* ptr, const int32_t* idx, const double* v
   if (rows <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_csr_stats, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, val, rows, nnz_out,
                      empty_mask, singleton_mask, singleton_idx, singleton_val);
+  return hipGetLastError();
+}
+
+hipError_t gpk_mail(const uint32_t* src, int32_t words, uint32_t* mail, uint32_t seq, hipStream_t s) {
+  if (words < 0 || words > 28 || (words > 0 && src == nullptr) || mail == nullptr) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_mail, dim3(1), dim3(64), 0, s, src, words, mail, seq);
+  return hipGetLastError();
+}
+
+hipError_t gpk_copy_many(const gpk_copy_seg* segs, int32_t count, hipStream_t s) {
+  if (count < 0 || count > GPK_COPY_MAX) return hipErrorInvalidValue;
+  CopyList list{};
+  int32_t used = 0;
+  int64_t most = 0;
+  for (int32_t k = 0; k < count; ++k) {
+    if (segs[k].bytes <= 0) continue;
+    list.seg[used++] = segs[k];
+    most = segs[k].bytes > most ? segs[k].bytes : most;
+  }
+  if (used == 0) return hipSuccess;
+  const int64_t per_block = static_cast<int64_t>(kBlock) * 16;
+  const int64_t blocks = (most + per_block - 1) / per_block;
+  hipLaunchKernelGGL(k_copy_many, dim3(static_cast<unsigned>(blocks < 1024 ? blocks : 1024), used), dim3(kBlock), 0,
+                     s, list);
   return hipGetLastError();
 }
 

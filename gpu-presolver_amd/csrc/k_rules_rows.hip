@@ -63,6 +63,35 @@ __global__ void k_plan_init(int32_t m, int32_t n, gpk_plan p, const double* c, c
   }
 }
 
+__global__ void k_phase_init(int32_t m, int32_t n, gpk_plan p, const double* c, const double* l, const double* u,
+                             const double* AL, const double* AU, gpk_csr A, gpk_csr AT, gpk_phase_stats st) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    p.keep_col[i] = 1;
+    p.c[i] = c[i];
+    p.l[i] = l[i];
+    p.u[i] = u[i];
+    const int32_t b = AT.ptr[i];
+    const int32_t cnt = AT.ptr[i + 1] - b;
+    st.col_nnz[i] = cnt;
+    st.empty_col_mask[i] = cnt == 0 ? 1 : 0;
+    st.column_singleton_mask[i] = cnt == 1 ? 1 : 0;
+    st.column_singleton_row[i] = cnt == 1 ? AT.idx[b] : -1;
+    st.column_singleton_val[i] = cnt == 1 ? AT.val[b] : 0.0;
+  }
+  if (i < m) {
+    p.keep_row[i] = 1;
+    p.AL[i] = AL[i];
+    p.AU[i] = AU[i];
+    const int32_t b = A.ptr[i];
+    const int32_t cnt = A.ptr[i + 1] - b;
+    st.row_nnz[i] = cnt;
+    st.singleton_row_mask[i] = cnt == 1 ? 1 : 0;
+    st.singleton_row_col[i] = cnt == 1 ? A.idx[b] : -1;
+    st.singleton_row_val[i] = cnt == 1 ? A.val[b] : 0.0;
+  }
+}
+
 __global__ void k_empty_rows(int32_t m, gpk_plan p, const int32_t* row_nnz, double tol, int32_t apply,
                              int32_t* flags) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -713,6 +742,12 @@ hipError_t gpk_pack_f64(const uint8_t* mask, const double* vals, const int32_t* 
 hipError_t gpk_pack_index(const uint8_t* mask, const int32_t* prefix, int32_t n, int32_t* out, hipStream_t stream) {
   GPK_LAUNCH(k_pack_index, n, mask, prefix, n, out);
 }
+hipError_t gpk_phase_init(int32_t m, int32_t n, gpk_plan plan, const double* c, const double* l, const double* u,
+                          const double* AL, const double* AU, gpk_csr A, gpk_csr AT, gpk_phase_stats stats,
+                          hipStream_t stream) {
+  GPK_LAUNCH(k_phase_init, m > n ? m : n, m, n, plan, c, l, u, AL, AU, A, AT, stats);
+}
+
 hipError_t gpk_plan_init(int32_t m, int32_t n, gpk_plan plan, const double* c, const double* l, const double* u,
                          const double* AL, const double* AU, hipStream_t stream) {
   GPK_LAUNCH(k_plan_init, m > n ? m : n, m, n, plan, c, l, u, AL, AU);

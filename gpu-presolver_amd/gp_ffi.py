@@ -241,6 +241,27 @@ def compact_csr(m, n, ptr, idx, val, keep_row, keep_col):
     return (out.rows, out.cols) + _take_csr(out)
 
 
+def radix_sort(keys, vals, key_bits):
+    """Stable sort of (keys, vals) by the low key_bits bits of the keys
+    (uint32 or uint64) through gpk_radix_sort_u32_i32 / _u64_i32 on the null
+    stream; returns the sorted (keys, vals)."""
+    keys = np.ascontiguousarray(keys)
+    vals = np.ascontiguousarray(vals, dtype=np.int32)
+    n = keys.size
+    L = lib()
+    fn = L.gpk_radix_sort_u64_i32 if keys.dtype == np.uint64 else L.gpk_radix_sort_u32_i32
+    vp = C.c_void_p
+    fn.argtypes = [vp, vp, vp, vp, C.c_int32, C.c_int32, vp, vp]
+    L.gpk_radix_scratch_ints.restype = C.c_int64
+    L.gpk_radix_scratch_ints.argtypes = [C.c_int64]
+    dk, dv = DeviceArray(keys), DeviceArray(vals)
+    ka, va = DeviceArray(size=max(n, 1), dtype=keys.dtype), DeviceArray(size=max(n, 1), dtype=np.int32)
+    scratch = DeviceArray(size=max(1, L.gpk_radix_scratch_ints(n)), dtype=np.int32)
+    hip_check(fn(dk.ptr, dv.ptr, ka.ptr, va.ptr, n, key_bits, scratch.ptr, None), "radix sort")
+    hip_check(hip().hipDeviceSynchronize(), "radix sort synchronize")
+    return dk.numpy(), dv.numpy()
+
+
 def transpose_csr(m, n, ptr, idx, val):
     A = DeviceCsr(m, n, ptr, idx, val)
     out = T.GpCsr()

@@ -90,6 +90,23 @@ hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_ol
                              const int32_t* jn, const int32_t* tile_off, int32_t* out_ptr, int32_t m_new,
                              int32_t* out_idx, double* out_val, hipStream_t stream);
 
+/* Up to GPK_COPY_MAX copies (src != NULL) or zero fills (src == NULL) of
+ * device byte ranges in one launch (block row y serves segment y; the unit
+ * is the widest of 16/8/4/1 bytes that divides both addresses and the size). */
+#define GPK_COPY_MAX 12
+typedef struct gpk_copy_seg {
+  void* dst;
+  const void* src;
+  int64_t bytes;
+} gpk_copy_seg;
+hipError_t gpk_copy_many(const gpk_copy_seg* segs, int32_t count, hipStream_t stream);
+
+/* Host mailbox (no reference counterpart): one workgroup copies `words`
+ * (<= 28) 32-bit words from src (may be NULL when words = 0) to mail[4..],
+ * makes them visible system-wide, then stores seq to mail[0].  mail is the
+ * device address of host-mapped, coherent pinned memory. */
+hipError_t gpk_mail(const uint32_t* src, int32_t words, uint32_t* mail, uint32_t seq, hipStream_t stream);
+
 /* old_to_new[i] = mask[i] ? prefix[i] : -1. */
 hipError_t gpk_old_to_new(const uint8_t* mask, const int32_t* prefix, int32_t* old_to_new, int32_t n,
                           hipStream_t stream);
@@ -171,6 +188,24 @@ typedef struct gpk_plan {
  * copies of the working LP's c, l, u (n) and AL, AU (m), in one launch. */
 hipError_t gpk_plan_init(int32_t m, int32_t n, gpk_plan plan, const double* c, const double* l, const double* u,
                          const double* AL, const double* AU, hipStream_t stream);
+
+/* Fresh phase plan and phase statistics in one launch: gpk_plan_init plus
+ * gpk_csr_stats of A (row counts, singleton rows) and of AT (column counts,
+ * empty and singleton columns), gpu_presolve.cu:197-276. */
+typedef struct gpk_phase_stats {
+  int32_t* row_nnz;
+  uint8_t* singleton_row_mask;
+  int32_t* singleton_row_col;
+  double* singleton_row_val;
+  int32_t* col_nnz;
+  uint8_t* empty_col_mask;
+  uint8_t* column_singleton_mask;
+  int32_t* column_singleton_row;
+  double* column_singleton_val;
+} gpk_phase_stats;
+hipError_t gpk_phase_init(int32_t m, int32_t n, gpk_plan plan, const double* c, const double* l, const double* u,
+                          const double* AL, const double* AU, gpk_csr A, gpk_csr AT, gpk_phase_stats stats,
+                          hipStream_t stream);
 
 /* Serving runtime (no reference counterpart).  gpk_synthetic_point: seeded
  * reduced-space point standing in for a solver (x in [l, u] on the 2^-10

@@ -92,3 +92,21 @@ def test_transpose(m, n, seed):
     assert got[:2] == want[:2]
     for name, a, b in zip(("ptr", "idx", "val"), got[2:], want[2:]):
         assert_same_array(name, a, b)
+
+
+# The sort has three launch shapes: one workgroup (n <= 4096), per-pass
+# histogram + scatter that derives its offsets from the histogram (up to 64
+# tiles of 2048 keys), and histogram + scan + scatter beyond that.
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 100, 4095, 4096, 4097, 20000, 131072, 131073, 300000])
+@pytest.mark.parametrize("dtype,bits", [(np.uint32, 13), (np.uint32, 31), (np.uint64, 64)])
+def test_radix_sort_stable(n, dtype, bits):
+    rng = np.random.default_rng(n * 7 + bits)
+    hi = min(2 ** bits, 2 ** 63)
+    # Few distinct keys, so equal keys are common and stability is tested.
+    pool = rng.integers(0, hi, size=max(1, n // 8), dtype=np.uint64).astype(dtype)
+    keys = pool[rng.integers(0, pool.size, size=n)] if n else np.zeros(0, dtype)
+    vals = rng.permutation(n).astype(np.int32)
+    got_k, got_v = gp_ffi.radix_sort(keys, vals, bits)
+    order = np.argsort(keys, kind="stable")
+    assert_same_array("keys", got_k, keys[order])
+    assert_same_array("values", got_v, vals[order])
