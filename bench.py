@@ -427,6 +427,32 @@ def bench_serving(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_small(args):
+    """Latency of one small LP from the batched size distribution (m rows,
+    n = 1.5 m, mean row length 6) presolved one at a time through the plain
+    entry point: ms per presolve, and its launch count under a kernel trace."""
+    import numpy as np
+
+    import gp_ffi
+    import lp_synth
+
+    m = args.small_rows
+    lp = lp_synth.generate(m, int(round(1.5 * m)), seed=7 + m, mean_row_len=6.0)
+    d = gp_ffi.DeviceLp(lp)
+    prm = gp_ffi.params(enable_structure_specific_rules=0)
+    for _ in range(args.warmup):
+        gp_ffi.Presolved(d, prm, keep_reduced=True).free()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        gp_ffi.Presolved(d, prm, keep_reduced=True).free()
+    ms = 1e3 * (time.perf_counter() - t) / args.steps
+    print(json.dumps({"metric": "small-LP presolve latency", "value": ms, "unit": "ms", "n_gpus": 1,
+                      "steps": args.steps, "warmup": args.warmup, "higher_is_better": False, "dtype": "f64",
+                      "data": "synthetic (seeded, dyadic values, planted reductions)",
+                      "config": {"workload": "small LP, plain presolve", "m": m, "n": int(round(1.5 * m)),
+                                 "nnz": int(len(lp["A_idx"]))}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -438,9 +464,11 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the row-activity and compaction kernel timings plus a 256 MiB device copy "
                          "(for hardware-counter runs)")
-    ap.add_argument("--workload", choices=("plain", "batched", "serving"), default="plain",
+    ap.add_argument("--workload", choices=("plain", "batched", "serving", "small"), default="plain",
                     help="plain: the single large LP (default); batched: 4,096 small LPs at once against "
-                         "one-at-a-time plain presolve; serving: Poisson presolve+postsolve requests")
+                         "one-at-a-time plain presolve; serving: Poisson presolve+postsolve requests; "
+                         "small: latency of one small LP (--small-rows)")
+    ap.add_argument("--small-rows", type=int, default=1400)
     ap.add_argument("--batch-lps", type=int, default=4096)
     ap.add_argument("--workers", type=str, default="1,2,4,8,12,16",
                     help="worker counts the autotuner tries (batched and serving)")
@@ -456,6 +484,8 @@ def main():
         return bench_batched(args)
     if args.workload == "serving":
         return bench_serving(args)
+    if args.workload == "small":
+        return bench_small(args)
 
     import numpy as np
 
