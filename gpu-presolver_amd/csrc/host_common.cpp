@@ -7,7 +7,9 @@
 #include <chrono>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -189,54 +191,84 @@ ScopedCtx::~ScopedCtx() { t_current = prev_; }
 // next allocation on that stream without any synchronization; the device is
 // only asked for memory (hipMallocAsync) on a miss.  This replaces one
 // hipMallocAsync/hipFreeAsync pair per temporary buffer of every rule call.
+// Each stream has its own arena and lock, so the serving workers (one stream
+// each) never wait for one another here.
 namespace {
-struct BlockInfo {
-  hipStream_t stream;
-  size_t bytes;
+struct StreamArena {
+  std::mutex mu;
+  std::unordered_map<size_t, std::vector<void*>> free_lists;  // size class -> blocks
+  std::unordered_map<void*, size_t> live;                     // block -> size class
 };
 struct Arena {
-  std::mutex mu;
-  std::unordered_map<void*, BlockInfo> live;
-  std::unordered_map<hipStream_t, std::unordered_map<size_t, std::vector<void*>>> free_lists;
+  std::shared_mutex mu;  // guards the map (arenas are never removed)
+  std::unordered_map<hipStream_t, std::unique_ptr<StreamArena>> streams;
 };
 Arena& arena() {
   static auto* a = new Arena();  // never destroyed: blocks may be released at exit
   return *a;
+}
+StreamArena& stream_arena(hipStream_t stream) {
+  thread_local hipStream_t last_stream = nullptr;
+  thread_local StreamArena* last = nullptr;
+  if (last && last_stream == stream) return *last;
+  Arena& a = arena();
+  {
+    std::shared_lock<std::shared_mutex> lock(a.mu);
+    auto it = a.streams.find(stream);
+    if (it != a.streams.end()) {
+      last_stream = stream;
+      last = it->second.get();
+      return *last;
+    }
+  }
+  std::unique_lock<std::shared_mutex> lock(a.mu);
+  auto& slot = a.streams[stream];
+  if (!slot) {
+    slot = std::make_unique<StreamArena>();
+    slot->live.reserve(1024);
+  }
+  last_stream = stream;
+  last = slot.get();
+  return *last;
 }
 size_t size_class(size_t bytes) {
   size_t c = 512;
   while (c < bytes) c <<= 1;
   return c;
 }
+// Returns a live block to the free list of the arena that allocated it.
+bool recycle(StreamArena& sa, void* p) {
+  std::lock_guard<std::mutex> lock(sa.mu);
+  auto it = sa.live.find(p);
+  if (it == sa.live.end()) return false;
+  sa.free_lists[it->second].push_back(p);
+  sa.live.erase(it);
+  return true;
+}
 }  // namespace
 
 void release_cached_blocks(hipStream_t stream) noexcept {
-  Arena& a = arena();
+  StreamArena& sa = stream_arena(stream);
   std::vector<void*> blocks;
   {
-    std::lock_guard<std::mutex> lock(a.mu);
-    auto it = a.free_lists.find(stream);
-    if (it == a.free_lists.end()) return;
-    for (auto& kv : it->second) blocks.insert(blocks.end(), kv.second.begin(), kv.second.end());
-    a.free_lists.erase(it);
+    std::lock_guard<std::mutex> lock(sa.mu);
+    for (auto& kv : sa.free_lists) blocks.insert(blocks.end(), kv.second.begin(), kv.second.end());
+    sa.free_lists.clear();
   }
   for (void* p : blocks) (void)hipFreeAsync(p, stream);
 }
 
 void* dev_alloc(size_t bytes, hipStream_t stream, const char* context) {
-  Arena& a = arena();
+  StreamArena& sa = stream_arena(stream);
   const size_t cls = size_class(bytes);
   {
-    std::lock_guard<std::mutex> lock(a.mu);
-    auto s = a.free_lists.find(stream);
-    if (s != a.free_lists.end()) {
-      auto f = s->second.find(cls);
-      if (f != s->second.end() && !f->second.empty()) {
-        void* p = f->second.back();
-        f->second.pop_back();
-        a.live.emplace(p, BlockInfo{stream, cls});
-        return p;
-      }
+    std::lock_guard<std::mutex> lock(sa.mu);
+    auto f = sa.free_lists.find(cls);
+    if (f != sa.free_lists.end() && !f->second.empty()) {
+      void* p = f->second.back();
+      f->second.pop_back();
+      sa.live.emplace(p, cls);
+      return p;
     }
   }
   void* p = nullptr;
@@ -247,22 +279,19 @@ void* dev_alloc(size_t bytes, hipStream_t stream, const char* context) {
     st = hipMallocAsync(&p, cls, stream);
   }
   check(st, context);
-  std::lock_guard<std::mutex> lock(a.mu);
-  a.live.emplace(p, BlockInfo{stream, cls});
+  std::lock_guard<std::mutex> lock(sa.mu);
+  sa.live.emplace(p, cls);
   return p;
 }
 
 void dev_free(void* p, hipStream_t stream) noexcept {
   if (!p) return;
-  Arena& a = arena();
-  {
-    std::lock_guard<std::mutex> lock(a.mu);
-    auto it = a.live.find(p);
-    if (it != a.live.end()) {
-      a.free_lists[it->second.stream][it->second.bytes].push_back(p);
-      a.live.erase(it);
-      return;
-    }
+  if (recycle(stream_arena(stream), p)) return;
+  {  // allocated on another stream
+    Arena& a = arena();
+    std::shared_lock<std::shared_mutex> lock(a.mu);
+    for (auto& kv : a.streams)
+      if (recycle(*kv.second, p)) return;
   }
   // Memory the caller allocated with hipMalloc (e.g. a plan's new_A).
   (void)hipStreamSynchronize(stream);
