@@ -473,7 +473,7 @@ def main():
                          "small: latency of one small LP (--small-rows)")
     ap.add_argument("--small-rows", type=int, default=1400)
     ap.add_argument("--batch-lps", type=int, default=4096)
-    ap.add_argument("--workers", type=str, default="1,2,4,8,12,16",
+    ap.add_argument("--workers", type=str, default="1,2,4,8,12,16,20,24",
                     help="worker counts the autotuner tries (batched and serving)")
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
