@@ -4,7 +4,10 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -461,6 +464,16 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
     if (z_red) check(gpk_scatter(z_red, col_map, record.n1, res.z_org, s), "scatter z");
     check(gpk_mark(col_map, record.n1, zret, s), "mark retrieved");
     check(gpk_restore_fixed(fidx, fval, nf, res.x_org, s), "restore fixed");
+    // GP_PRESOLVE_PROFILE=1: host time of the postsolve steps on stderr.
+    const bool prof = std::getenv("GP_PRESOLVE_PROFILE") != nullptr;
+    auto tick = std::chrono::steady_clock::now();
+    auto lap = [&](const char* what) {
+      if (!prof) return;
+      const auto now = std::chrono::steady_clock::now();
+      std::fprintf(stderr, "  postsolve %-28s %8.3f ms\n", what,
+                   1e3 * std::chrono::duration<double>(now - tick).count());
+      tick = now;
+    };
     DeviceTape tape(record.tape);
     const gp_orig_dev orig = orig_view(original_model);
     double* y = m0 > 0 ? res.y_org : nullptr;
@@ -475,11 +488,21 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
       pattern.a_rows = gather_lines(o.A.rowPtr, o.A.colVal, m0, rows);
       pattern.at_cols = gather_lines(o.AT.rowPtr, o.AT.colVal, n0, cols);
     }
+    lap("tape upload, scan, lines");
     const int32_t count = tape.view.count;
-    run_schedule(level_schedule(scanner, count, n0, m0, true, has_orig, has_y, &pattern), tape.view, orig, res.x_org,
-                 y, res.z_org, zret, true, kPostsolveTol);
-    run_schedule(level_schedule(scanner, count, n0, m0, false, has_orig, has_y, &pattern), tape.view, orig, res.x_org,
-                 y, res.z_org, zret, false, kPostsolveTol);
+    const Schedule primal_sch = level_schedule(scanner, count, n0, m0, true, has_orig, has_y, &pattern);
+    lap("primal schedule");
+    run_schedule(primal_sch, tape.view, orig, res.x_org, y, res.z_org, zret, true, kPostsolveTol);
+    const Schedule dual_sch = level_schedule(scanner, count, n0, m0, false, has_orig, has_y, &pattern);
+    lap("dual schedule");
+    run_schedule(dual_sch, tape.view, orig, res.x_org, y, res.z_org, zret, false, kPostsolveTol);
+    if (prof) {
+      c.sync("postsolve profile");
+      lap("replay (wait)");
+      std::fprintf(stderr, "  postsolve levels: primal %zu in %zu launches, dual %zu in %zu launches\n",
+                   primal_sch.level_start.size() - 1, primal_sch.segs.size(), dual_sch.level_start.size() - 1,
+                   dual_sch.segs.size());
+    }
     // project_column_duals_from_original (gpu_postsolve.cu:1370-1435)
     if (original_model != nullptr && y != nullptr && original_model->AT.rows > 0 && original_model->c &&
         original_model->l && original_model->u) {
@@ -491,6 +514,7 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
             "project duals");
     }
     c.sync("postsolve_gpu synchronize");
+    lap("projection (wait)");
   } catch (...) {
     (void)hipStreamSynchronize(s);
     amd::dev_free(res.x_org, s);

@@ -163,17 +163,55 @@ __device__ void dbl_recover_z(double xk, double zk_hat, double al, double okl, d
   else { *zk = zk_hat; *ze = 0.0; }
 }
 
-// Reverse replay of tape records [begin, end) (gpu_postsolve.cu:605-1046).
-// The records form a dependency chain in general, so one lane walks them in
-// order; every other postsolve step is data-parallel.
+// s (+/-)= sum over q in [0, count) with idx[q] != skip of val[q] * vec[idx[q]],
+// accumulated in q order exactly as the plain loop would (every product
+// rounded, then added or subtracted one at a time).  The indices, values and
+// gathered entries of eight terms are loaded before they are used, so a long
+// column costs count / 8 memory round trips instead of count.
+template <bool kSub>
+__device__ inline double dot_in_order(double s, const int32_t* __restrict__ idx, const double* __restrict__ val,
+                                      const double* __restrict__ vec, int32_t count, int32_t skip) {
+  constexpr int U = 8;
+  int32_t q = 0;
+  for (; q + U <= count; q += U) {
+    int32_t j[U];
+    double a[U], v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      j[k] = idx[q + k];
+      a[k] = val[q + k];
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) v[k] = vec[j[k]];
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (j[k] != skip) s = kSub ? s - a[k] * v[k] : s + a[k] * v[k];
+  }
+  for (; q < count; ++q) {
+    const int32_t j = idx[q];
+    if (j != skip) s = kSub ? s - val[q] * vec[j] : s + val[q] * vec[j];
+  }
+  return s;
+}
+
+// Where a record's indices and values sit in the tape (immutable, so it can
+// be loaded ahead of the record's turn).
+struct RecHead {
+  int32_t type, i0, i1, v0, v1;
+};
+__device__ inline RecHead load_head(const gp_tape_dev& T, int32_t k) {
+  return RecHead{T.types[k], T.index_starts[k], T.index_starts[k + 1] - 1, T.value_starts[k],
+                 T.value_starts[k + 1] - 1};
+}
+
 // One record of the reverse replay (_kernel_replay_postsolve_tape,
 // gpu_postsolve.cu:605-1046).
 __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x, double* y, double* z,
-                           uint8_t* zret, int32_t k, bool primal, bool dual, double tol) {
+                           uint8_t* zret, const RecHead& h, bool primal, bool dual, double tol) {
   const bool has_orig = O.present != 0;
-  const int32_t type = T.types[k];
-  const int32_t i0 = T.index_starts[k], i1 = T.index_starts[k + 1] - 1;
-  const int32_t v0 = T.value_starts[k], v1 = T.value_starts[k + 1] - 1;
+  const int32_t type = h.type;
+  const int32_t i0 = h.i0, i1 = h.i1;
+  const int32_t v0 = h.v0, v1 = h.v1;
   const int32_t* I = T.indices;
   const double* V = T.vals;
   if (type == 3) {  // DuplicateColumn
@@ -200,11 +238,11 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
       double so = 0.0;
       double yrow;
       if (payload) {
-        for (int32_t t = 0; t < I[i0 + 3]; ++t) so += V[v0 + 12 + t] * y[I[i0 + 4 + t]];
+        so = dot_in_order<false>(so, I + i0 + 4, V + v0 + 12, y, I[i0 + 3], -1);
         yrow = (eobj - so - ze) / ecoef;
       } else {
-        for (int32_t q = O.at_ptr[ec]; q < O.at_ptr[ec + 1]; ++q)
-          if (O.at_idx[q] != row) so += O.at_val[q] * y[O.at_idx[q]];
+        const int32_t qb = O.at_ptr[ec];
+        so = dot_in_order<false>(so, O.at_idx + qb, O.at_val + qb, y, O.at_ptr[ec + 1] - qb, row);
         yrow = (O.c[ec] - so - ze) / ecoef;
       }
       y[row] = yrow;
@@ -219,8 +257,7 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
     const int32_t ec = I[i0], row = I[i0 + 1], sc = I[i0 + 2];
     const double piv = V[v0], rhs = V[v0 + 1], eobj = V[v0 + 4];
     const bool row_deleted = V[v0 + 5] != 0.0;
-    double part = 0.0;
-    for (int32_t t = 0; t < sc; ++t) part += V[v0 + 6 + t] * x[I[i0 + 3 + t]];
+    const double part = dot_in_order<false>(0.0, I + i0 + 3, V + v0 + 6, x, sc, -1);
     if (primal) x[ec] = (rhs - part) / piv;
     if (dual && y && row >= 0) {
       if (row_deleted) y[row] = eobj / piv; else y[row] += eobj / piv;
@@ -234,10 +271,7 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
     if (primal) x[col] = V[v0];
     if (dual && z) {
       double zj = V[v0 + 1];
-      for (int32_t pos = i0 + 1; pos <= i1; ++pos) {
-        const double coef = V[v0 + 2 + (pos - (i0 + 1))];
-        if (y) zj -= coef * y[I[pos]];
-      }
+      if (y) zj = dot_in_order<true>(zj, I + i0 + 1, V + v0 + 2, y, i1 - i0, -1);
       z[col] = zj;
       if (zret) zret[col] = 1;
     }
@@ -294,9 +328,9 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
     const double dsl = r >= 0.0 ? a : b, dsu = r >= 0.0 ? b : a;
     if (dsl <= kal + tol && kau <= dsu + tol) { y[dr] = 0.0; return; }
     if (has_orig) {
-      double ap = 0.0, ad = 0.0;
-      for (int32_t q = O.a_ptr[kr]; q < O.a_ptr[kr + 1]; ++q) ap += O.a_val[q] * x[O.a_idx[q]];
-      for (int32_t q = O.a_ptr[dr]; q < O.a_ptr[dr + 1]; ++q) ad += O.a_val[q] * x[O.a_idx[q]];
+      const int32_t kb = O.a_ptr[kr], db = O.a_ptr[dr];
+      const double ap = dot_in_order<false>(0.0, O.a_idx + kb, O.a_val + kb, x, O.a_ptr[kr + 1] - kb, -1);
+      const double ad = dot_in_order<false>(0.0, O.a_idx + db, O.a_val + db, x, O.a_ptr[dr + 1] - db, -1);
       const uint8_t ks = row_state(ap, kal, kau, tol), ds = row_state(ad, dal, dau, tol);
       double ydl, ydh, ypl, yph;
       state_interval(ds, &ydl, &ydh);
@@ -316,8 +350,10 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
     const double oal = V[v0], oau = V[v0 + 1], aij = V[v0 + 2];
     if (fabs(aij) <= tol) return;
     double gamma = O.c[col];
-    for (int32_t q = O.at_ptr[col]; q < O.at_ptr[col + 1]; ++q)
-      if (O.at_idx[q] != row && y) gamma -= O.at_val[q] * y[O.at_idx[q]];
+    if (y) {
+      const int32_t qb = O.at_ptr[col];
+      gamma = dot_in_order<true>(gamma, O.at_idx + qb, O.at_val + qb, y, O.at_ptr[col + 1] - qb, row);
+    }
     const double xj = x[col], zbar = z[col];
     const double act = aij * xj;
     const bool ral = pfin(oal) && act <= oal + tol;
@@ -377,7 +413,8 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
 __global__ void k_replay(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
                          int32_t begin, int32_t end, int32_t primal, int32_t dual, double tol) {
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  for (int32_t k = end - 1; k >= begin; --k) replay_one(T, O, x, y, z, zret, k, primal != 0, dual != 0, tol);
+  for (int32_t k = end - 1; k >= begin; --k)
+    replay_one(T, O, x, y, z, zret, load_head(T, k), primal != 0, dual != 0, tol);
 }
 
 // One level of a dependency-level schedule: its records touch disjoint
@@ -385,23 +422,47 @@ __global__ void k_replay(gp_tape_dev T, gp_orig_dev O, double* x, double* y, dou
 __global__ void k_replay_list(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
                               const int32_t* records, int32_t count, int32_t primal, int32_t dual, double tol) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < count) replay_one(T, O, x, y, z, zret, records[i], primal != 0, dual != 0, tol);
+  if (i < count) replay_one(T, O, x, y, z, zret, load_head(T, records[i]), primal != 0, dual != 0, tol);
 }
 
 // A run of small levels in one workgroup: the threads share each level's
 // records and meet at a barrier before the next level.  All waves of the
 // workgroup run on one CU, so the barrier's workgroup-scope fence makes one
 // level's global stores visible to the next level's loads.
+// The level bounds of up to kLevelChunk levels sit in LDS, and each thread
+// loads the header of its record of the next level while the current level
+// runs, so a level's critical path starts at the tape's indices and values.
 constexpr int kLevelBlock = 512;
+constexpr int kLevelChunk = 4096;
 __global__ void __launch_bounds__(kLevelBlock)
     k_replay_levels(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
                     const int32_t* records, const int32_t* level_start, int32_t nlevels, int32_t primal,
                     int32_t dual, double tol) {
-  for (int32_t lv = 0; lv < nlevels; ++lv) {
-    const int32_t e = level_start[lv + 1];
-    for (int32_t i = level_start[lv] + static_cast<int32_t>(threadIdx.x); i < e; i += kLevelBlock)
-      replay_one(T, O, x, y, z, zret, records[i], primal != 0, dual != 0, tol);
+  __shared__ int32_t s_start[kLevelChunk + 2];
+  const int32_t tid = static_cast<int32_t>(threadIdx.x);
+  for (int32_t c0 = 0; c0 < nlevels; c0 += kLevelChunk) {
+    const int32_t cl = min(kLevelChunk, nlevels - c0);
+    // level_start[c0 .. c0 + cl + 1]: the bounds of this chunk's levels and of the next level
+    const int32_t have = min(cl + 2, nlevels + 1 - c0);
+    for (int32_t t = tid; t < have; t += kLevelBlock) s_start[t] = level_start[c0 + t];
     __syncthreads();
+    int32_t b = s_start[0], e = s_start[1];
+    RecHead cur{};
+    if (b + tid < e) cur = load_head(T, records[b + tid]);
+    for (int32_t l = 0; l < cl; ++l) {
+      const bool more = l + 1 < have - 1;  // the next level's bounds are in LDS
+      const int32_t nb = more ? s_start[l + 1] : 0, ne = more ? s_start[l + 2] : 0;
+      RecHead nxt{};
+      if (nb + tid < ne) nxt = load_head(T, records[nb + tid]);
+      if (b + tid < e) replay_one(T, O, x, y, z, zret, cur, primal != 0, dual != 0, tol);
+      for (int32_t i = b + tid + kLevelBlock; i < e; i += kLevelBlock)
+        replay_one(T, O, x, y, z, zret, load_head(T, records[i]), primal != 0, dual != 0, tol);
+      __syncthreads();
+      b = nb;
+      e = ne;
+      cur = nxt;
+    }
+    __syncthreads();  // s_start is reloaded for the next chunk
   }
 }
 
@@ -435,8 +496,7 @@ __global__ void k_project_duals(gp_orig_dev O, const uint8_t* prot, const double
     long_cols[atomicAdd(long_count, 1)] = col;
     return;
   }
-  double aty = 0.0;
-  for (int32_t q = b; q < e; ++q) aty += O.at_val[q] * y[O.at_idx[q]];
+  const double aty = dot_in_order<false>(0.0, O.at_idx + b, O.at_val + b, y, e - b, -1);
   z[col] = project_dual(O, col, aty, x, tol);
 }
 
