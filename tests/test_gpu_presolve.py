@@ -110,6 +110,15 @@ def test_empty_lp():
     _run_both(lp)
 
 
+def test_config2_full_size():
+    """The block-structured LP (2,000 blocks of 40 x 60 plus 200 linking rows of
+    3,000 nnz: 80,200 x 120,000, ~1.3M nnz) with the structure-specific rules
+    off, end to end."""
+    lp = lp_synth.config2(seed=2)
+    so, ro, lo = _run_both(lp, enable_structure_specific_rules=0)
+    _postsolve_both(lp, ro, lo, seed=5)
+
+
 def test_config1_full_size():
     """The single large LP workload (1M x 1.5M, ~7.5M nnz) end to end."""
     lp = lp_synth.config1(seed=1)

@@ -83,6 +83,25 @@ def test_compact(m, n, seed, frac):
         assert_same_array(name, a, b)
 
 
+def test_compact_many_tiles():
+    """More than 16,384 tiles of 256 rows: the tile offsets come from the
+    multi-workgroup look-back scan instead of the one-workgroup scan."""
+    m, n = 4_200_000, 50_000
+    rng = np.random.default_rng(9)
+    lens = rng.integers(0, 4, size=m)
+    rows = np.repeat(np.arange(m, dtype=np.int64), lens)
+    cols = rng.integers(0, n, size=rows.size)
+    vals = rng.choice([-1.5, 0.5, 2.0], size=rows.size)
+    ptr, idx, val = lp_synth.csr_from_coo(m, n, rows, cols, vals)
+    kr = (rng.random(m) >= 0.2).astype(np.uint8)
+    kc = (rng.random(n) >= 0.2).astype(np.uint8)
+    got = gp_ffi.compact_csr(m, n, ptr, idx, val, kr, kc)
+    want = O.compact_csr(m, n, ptr, idx, val, kr, kc)
+    assert got[:2] == want[:2]
+    for name, a, b in zip(("ptr", "idx", "val"), got[2:], want[2:]):
+        assert_same_array(name, a, b)
+
+
 @pytest.mark.parametrize("m,n,seed", [(0, 3, 1), (3, 0, 2), (1, 1, 3), (300, 400, 4), (30000, 20000, 5),
                                       (2000, 70000, 6)])
 def test_transpose(m, n, seed):
