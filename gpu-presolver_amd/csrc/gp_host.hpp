@@ -55,7 +55,8 @@ class Ctx {
   // Returns kFlags zeroed device flag words.
   int32_t* flags();
   // Copies the first `count` words of the flags last returned by flags() to
-  // the host (waits for the stream).
+  // the host (waits for the stream).  The words stay valid until the next
+  // read_flags, scalar_to_host or mail call on this context.
   const int32_t* read_flags(int count = kFlags);
   // Waits for everything enqueued on the stream so far.
   void sync(const char* context = "stream synchronize");
