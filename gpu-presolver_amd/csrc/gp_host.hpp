@@ -67,9 +67,6 @@ class Ctx {
   // command plus hipStreamSynchronize per decision, which is what limits
   // many presolve jobs sharing one GPU.
   const uint32_t* mail(const void* dev, int words, const char* context);
-  // A device-to-host copy into ordinary host memory was enqueued: the next
-  // sync() uses hipStreamSynchronize, which also completes such copies.
-  void note_host_copy() { ++host_copies_; }
   static constexpr int kMailWords = 16;
   // Number of host synchronizations so far (each read_flags / to_host /
   // sync counts one); the presolve step profiler reports it per step.
@@ -78,13 +75,16 @@ class Ctx {
   // stream synchronizations).
   double wait_seconds() const { return wait_s_; }
 
+  // Enqueues a copy of `bytes` of device memory into host memory at `dst`:
+  // a kernel writes it into the context's host-mapped staging area and the
+  // next wait (sync, read_flags, scalar_to_host, mail) moves it to `dst`.
+  // `dst` must stay valid until then.
+  void download(void* dst, const void* dev, size_t bytes, const char* context);
   template <class T>
   std::vector<T> to_host(const T* dev, int64_t count, const char* context) {
     std::vector<T> out(static_cast<size_t>(count > 0 ? count : 0));
     if (count > 0) {
-      check(hipMemcpyAsync(out.data(), dev, sizeof(T) * static_cast<size_t>(count), hipMemcpyDeviceToHost, stream_),
-            context);
-      note_host_copy();
+      download(out.data(), dev, sizeof(T) * static_cast<size_t>(count), context);
       sync(context);
     }
     return out;
@@ -94,30 +94,20 @@ class Ctx {
   template <class T>
   void to_host_async(std::vector<T>& out, const T* dev, int64_t count, const char* context) {
     out.resize(static_cast<size_t>(count > 0 ? count : 0));
-    if (count > 0) {
-      check(hipMemcpyAsync(out.data(), dev, sizeof(T) * static_cast<size_t>(count), hipMemcpyDeviceToHost, stream_),
-            context);
-      note_host_copy();
-    }
+    if (count > 0) download(out.data(), dev, sizeof(T) * static_cast<size_t>(count), context);
   }
   template <class T>
   T scalar_to_host(const T* dev, const char* context) {
     static_assert(sizeof(T) % 4 == 0 && sizeof(T) <= 4 * kMailWords, "scalar of whole 32-bit words");
     T v;
-    if (host_copies_ == 0) {
-      std::memcpy(&v, mail(dev, static_cast<int>(sizeof(T) / 4), context), sizeof(T));
-      return v;
-    }
-    check(hipMemcpyAsync(pinned_, dev, sizeof(T), hipMemcpyDeviceToHost, stream_), context);
-    sync(context);
-    std::memcpy(&v, pinned_, sizeof(T));
+    std::memcpy(&v, mail(dev, static_cast<int>(sizeof(T) / 4), context), sizeof(T));
     return v;
   }
   template <class T>
   void to_device(T* dev, const T* host, int64_t count, const char* context) {
     if (count <= 0) return;
+    // Stream-ordered: any later wait on this stream finds the copy complete.
     check(hipMemcpyAsync(dev, host, sizeof(T) * static_cast<size_t>(count), hipMemcpyHostToDevice, stream_), context);
-    note_host_copy();  // the source is ordinary host memory: the next sync() completes the copy fully
   }
 
  private:
@@ -134,15 +124,21 @@ class Ctx {
   uint32_t epoch_ = 0;
   int64_t syncs_ = 0;
   double wait_s_ = 0.0;
-  int64_t host_copies_ = 0;  // enqueued copies into ordinary host memory since the last full sync
   uint32_t* h_mail_ = nullptr;  // host-mapped: [0] sequence, [4, 4 + kMailWords) payload
   uint32_t* d_mail_ = nullptr;
   uint32_t mail_seq_ = 0;
+  struct Download {
+    void* dst;
+    size_t offset, bytes;
+  };
+  char* h_stage_ = nullptr;  // host-mapped staging area of download()
+  char* d_stage_ = nullptr;
+  size_t stage_cap_ = 0, stage_used_ = 0;
+  std::vector<Download> downloads_;
   void check_error_flag(const char* context);
+  void finish_downloads();
   int32_t* h_error_ = nullptr;  // host-mapped
   int32_t* d_error_ = nullptr;
-  int32_t* h_flags_ = nullptr;
-  void* pinned_ = nullptr;
 };
 
 // Context of the calling thread.  ScopedCtx installs another one (the

@@ -54,8 +54,6 @@ Ctx::Ctx(hipStream_t stream, bool own_stream) {
   }
   check(hipMalloc(&d_flags_, sizeof(int32_t) * kFlags * kFlagSlots), "hipMalloc flags");
   check(hipMemsetAsync(d_flags_, 0, sizeof(int32_t) * kFlags * kFlagSlots, stream_), "hipMemset flags");
-  check(hipHostMalloc(&h_flags_, sizeof(int32_t) * kFlags, hipHostMallocDefault), "hipHostMalloc flags");
-  check(hipHostMalloc(&pinned_, 64, hipHostMallocDefault), "hipHostMalloc staging");
   check(hipMalloc(&d_tickets_, sizeof(int32_t) * kTickets), "hipMalloc tickets");
   check(hipMemsetAsync(d_tickets_, 0, sizeof(int32_t) * kTickets, stream_), "hipMemset tickets");
   check(hipHostMalloc(&h_error_, sizeof(int32_t), hipHostMallocMapped), "hipHostMalloc error flag");
@@ -96,8 +94,7 @@ Ctx::~Ctx() {
   if (d_status_) (void)hipFree(d_status_);
   (void)hipHostFree(h_error_);
   (void)hipHostFree(h_mail_);
-  (void)hipHostFree(h_flags_);
-  (void)hipHostFree(pinned_);
+  if (h_stage_) (void)hipHostFree(h_stage_);
   if (own_stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -113,12 +110,9 @@ int32_t* Ctx::flags() {
 }
 
 const int32_t* Ctx::read_flags(int count) {
-  const int32_t* src = cur_ ? cur_ : d_flags_;
-  if (host_copies_ == 0 && count <= kMailWords)
-    return reinterpret_cast<const int32_t*>(mail(src, count, "read flags"));
-  check(hipMemcpyAsync(h_flags_, src, sizeof(int32_t) * count, hipMemcpyDeviceToHost, stream_), "read flags");
-  sync("read flags");
-  return h_flags_;
+  static_assert(kFlags <= kMailWords, "a flag slot fits the mailbox");
+  if (count < 0 || count > kFlags) throw std::invalid_argument("read_flags: count out of range");
+  return reinterpret_cast<const int32_t*>(mail(cur_ ? cur_ : d_flags_, count, "read flags"));
 }
 
 void Ctx::check_error_flag(const char* context) {
@@ -128,18 +122,36 @@ void Ctx::check_error_flag(const char* context) {
   }
 }
 
-void Ctx::sync(const char* context) {
-  if (host_copies_ == 0) {
-    (void)mail(nullptr, 0, context);
-    return;
+void Ctx::sync(const char* context) { (void)mail(nullptr, 0, context); }
+
+void Ctx::download(void* dst, const void* dev, size_t bytes, const char* context) {
+  if (bytes == 0) return;
+  size_t off = (stage_used_ + 255) & ~static_cast<size_t>(255);
+  if (off + bytes > stage_cap_) {
+    if (!downloads_.empty()) sync(context);  // the staged copies in flight land first
+    off = 0;
+    if (bytes > stage_cap_) {
+      if (h_stage_) check(hipHostFree(h_stage_), "hipHostFree staging");
+      h_stage_ = d_stage_ = nullptr;
+      stage_cap_ = 0;
+      size_t cap = size_t{1} << 20;
+      while (cap < bytes) cap <<= 1;
+      check(hipHostMalloc(reinterpret_cast<void**>(&h_stage_), cap, hipHostMallocMapped | hipHostMallocCoherent),
+            "hipHostMalloc staging");
+      check(hipHostGetDevicePointer(reinterpret_cast<void**>(&d_stage_), h_stage_, 0), "staging device pointer");
+      stage_cap_ = cap;
+    }
   }
-  ++syncs_;
-  const auto t0 = std::chrono::steady_clock::now();
-  const hipError_t e = hipStreamSynchronize(stream_);
-  wait_s_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  check(e, context);
-  host_copies_ = 0;
-  check_error_flag(context);
+  const gpk_copy_seg seg{d_stage_ + off, dev, static_cast<int64_t>(bytes)};
+  check(gpk_copy_many(&seg, 1, stream_), context);
+  downloads_.push_back(Download{dst, off, bytes});
+  stage_used_ = off + bytes;
+}
+
+void Ctx::finish_downloads() {
+  for (const Download& d : downloads_) std::memcpy(d.dst, h_stage_ + d.offset, d.bytes);
+  downloads_.clear();
+  stage_used_ = 0;
 }
 
 const uint32_t* Ctx::mail(const void* dev, int words, const char* context) {
@@ -165,6 +177,7 @@ const uint32_t* Ctx::mail(const void* dev, int words, const char* context) {
   }
   std::atomic_thread_fence(std::memory_order_acquire);
   wait_s_ += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  finish_downloads();
   check_error_flag(context);
   return const_cast<const uint32_t*>(h_mail_ + 4);
 }

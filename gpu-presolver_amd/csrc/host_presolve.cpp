@@ -326,11 +326,8 @@ void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t
     const size_t base = rec.fixed_idx.size();
     rec.fixed_idx.resize(base + static_cast<size_t>(k));
     rec.fixed_val.resize(base + static_cast<size_t>(k));
-    check(hipMemcpyAsync(rec.fixed_idx.data() + base, oi.get(), sizeof(int32_t) * k, hipMemcpyDeviceToHost, s),
-          "fixed records idx");
-    check(hipMemcpyAsync(rec.fixed_val.data() + base, ov.get(), sizeof(double) * k, hipMemcpyDeviceToHost, s),
-          "fixed records val");
-    c.note_host_copy();
+    c.download(rec.fixed_idx.data() + base, oi.get(), sizeof(int32_t) * k, "fixed records idx");
+    c.download(rec.fixed_val.data() + base, ov.get(), sizeof(double) * k, "fixed records val");
     c.sync("fixed records");
   }
   append_global_tape(rs, p.tape);

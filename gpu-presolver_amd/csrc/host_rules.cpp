@@ -311,11 +311,10 @@ void apply_rule_duplicate_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
     std::vector<int32_t> hrow(k), hto(k), hstep(k);
     std::vector<double> hrec(5 * static_cast<size_t>(k));
     if (k > 0) {
-      check(hipMemcpyAsync(hrow.data(), d_row, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s), "duplicate rows");
-      check(hipMemcpyAsync(hto.data(), d_to, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s), "duplicate rows");
-      check(hipMemcpyAsync(hstep.data(), d_step, sizeof(int32_t) * k, hipMemcpyDeviceToHost, s), "duplicate rows");
-      check(hipMemcpyAsync(hrec.data(), d_rec, sizeof(double) * 5 * k, hipMemcpyDeviceToHost, s), "duplicate rows");
-      c.note_host_copy();
+      c.download(hrow.data(), d_row, sizeof(int32_t) * k, "duplicate rows");
+      c.download(hto.data(), d_to, sizeof(int32_t) * k, "duplicate rows");
+      c.download(hstep.data(), d_step, sizeof(int32_t) * k, "duplicate rows");
+      c.download(hrec.data(), d_rec, sizeof(double) * 5 * k, "duplicate rows");
       c.sync("duplicate rows tape");
     }
     // Tape order: by merge step, then by deleted row.
