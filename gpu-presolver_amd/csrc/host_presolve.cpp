@@ -311,8 +311,11 @@ struct FixedCols {
   int32_t k = 0;
 };
 
+// row_r2o_next / col_r2o_next: the maps already gathered through row_map /
+// col_map (the structural path gathers them with the LP's vectors), or null.
 void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t n, const int32_t* row_map,
-                   const int32_t* col_map, int32_t m_new, int32_t n_new, const FixedCols& fixed) {
+                   const int32_t* col_map, int32_t m_new, int32_t n_new, const FixedCols& fixed,
+                   DBuf<int32_t>* row_r2o_next = nullptr, DBuf<int32_t>* col_r2o_next = nullptr) {
   auto& c = ctx();
   hipStream_t s = c.stream();
   PresolveRecordGpu& rec = *rs.rec;
@@ -332,15 +335,23 @@ void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t
   }
   append_global_tape(rs, p.tape);
   if (m_new < m) {
-    DBuf<int32_t> next(m_new);
-    check(gpk_gather_i32(row_map, m, rs.row_r2o, next, s), "row red2org");
-    rs.row_r2o = std::move(next);
+    if (row_r2o_next) {
+      rs.row_r2o = std::move(*row_r2o_next);
+    } else {
+      DBuf<int32_t> next(m_new);
+      check(gpk_gather_i32(row_map, m, rs.row_r2o, next, s), "row red2org");
+      rs.row_r2o = std::move(next);
+    }
     rs.m = m_new;
   }
   if (n_new < n) {
-    DBuf<int32_t> next(n_new);
-    check(gpk_gather_i32(col_map, n, rs.col_r2o, next, s), "col red2org");
-    rs.col_r2o = std::move(next);
+    if (col_r2o_next) {
+      rs.col_r2o = std::move(*col_r2o_next);
+    } else {
+      DBuf<int32_t> next(n_new);
+      check(gpk_gather_i32(col_map, n, rs.col_r2o, next, s), "col red2org");
+      rs.col_r2o = std::move(next);
+    }
     rs.n = n_new;
   }
 }
@@ -357,14 +368,9 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   fixed.mask.alloc(n);
   fixed.map.alloc(static_cast<int64_t>(n) + 1);
   int32_t* counts = c.flags();
-  if (m > 0)
-    check(gpk_mask_to_map(p.keep_row_mask, row_map, m, counts, c.lookback(gpk_lookback_tiles(m)), s), "row map");
-  if (n > 0) {
-    check(gpk_mask_to_map(p.keep_col_mask, col_map, n, counts + 1, c.lookback(gpk_lookback_tiles(n)), s), "col map");
-    check(gpk_removed_fixed_mask(n, p.keep_col_mask, p.new_l, p.new_u, fixed.mask, s), "removed fixed mask");
-    check(gpk_mask_to_map(fixed.mask, fixed.map, n, counts + 2, c.lookback(gpk_lookback_tiles(n)), s),
-          "removed fixed map");
-  }
+  const gpk_apply_maps_args maps{m, n, p.keep_row_mask, p.keep_col_mask, p.new_l, p.new_u, row_map, col_map,
+                                 fixed.mask, fixed.map, counts};
+  check(gpk_apply_maps(maps, c.lookback(gpk_apply_maps_tiles(m, n)), s), "keep maps");
   const int32_t* h = c.read_flags(3);
   const int32_t m_new = h[0], n_new = h[1];
   fixed.k = h[2];
@@ -396,15 +402,19 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   next.u = pool_f64(n_new, "working u");
   next.AL = pool_f64(m_new, "working AL");
   next.AU = pool_f64(m_new, "working AU");
-  if (n > 0)
-    check(gpk_gather5(col_map, n, p.new_c, next.c, p.new_l, next.l, p.new_u, next.u, nullptr, nullptr, nullptr,
-                      nullptr, s),
-          "gather column vectors");
-  if (m > 0)
-    check(gpk_gather5(row_map, m, p.new_AL, next.AL, p.new_AU, next.AU, nullptr, nullptr, nullptr, nullptr, nullptr,
-                      nullptr, s),
-          "gather row vectors");
-  update_record(rs, p, m, n, row_map, col_map, m_new, n_new, fixed);
+  // The record's reduced-to-original maps shrink with the LP; they are
+  // gathered into new buffers, so the record update below still reads the
+  // current ones.
+  DBuf<int32_t> row_r2o_next, col_r2o_next;
+  if (m_new < m) row_r2o_next.alloc(m_new);
+  if (n_new < n) col_r2o_next.alloc(n_new);
+  const gpk_apply_gathers_args ga{m, n, col_map, p.new_c, p.new_l, p.new_u, rs.col_r2o,
+                                  next.c, next.l, next.u, n_new < n ? col_r2o_next.get() : nullptr,
+                                  row_map, p.new_AL, p.new_AU, rs.row_r2o,
+                                  next.AL, next.AU, m_new < m ? row_r2o_next.get() : nullptr};
+  check(gpk_apply_gathers(ga, s), "gather vectors");
+  update_record(rs, p, m, n, row_map, col_map, m_new, n_new, fixed, m_new < m ? &row_r2o_next : nullptr,
+                n_new < n ? &col_r2o_next : nullptr);
   next.obj_constant = obj_new;
   cur.release();
   cur.lp = next;

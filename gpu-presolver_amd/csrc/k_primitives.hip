@@ -253,8 +253,8 @@ __device__ inline uint64_t lb_word(uint32_t epoch, uint32_t flag, int32_t value)
 constexpr int kLbPerLane = 4;
 constexpr int kLbWindow = 64 * kLbPerLane;
 
-__device__ inline uint64_t lb_wait(const gpk_lookback& lb, int32_t j) {
-  if (j < 0) return lb_word(lb.epoch, 2u, 0);  // before tile 0: an inclusive prefix of 0
+__device__ inline uint64_t lb_wait(const gpk_lookback& lb, int32_t j, int32_t first = 0) {
+  if (j < first) return lb_word(lb.epoch, 2u, 0);  // before the chain's first tile: an inclusive prefix of 0
   uint64_t spins = 0;
   while (true) {
     const uint64_t w = __hip_atomic_load(lb.status + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -267,17 +267,20 @@ __device__ inline uint64_t lb_wait(const gpk_lookback& lb, int32_t j) {
   }
 }
 
-__device__ int32_t lb_exclusive(const gpk_lookback& lb, int32_t tile, int32_t aggregate) {
+// `first`: the first tile of this tile's chain (several independent scans
+// can share one launch and one status array, chain after chain in ticket
+// order).
+__device__ int32_t lb_exclusive(const gpk_lookback& lb, int32_t tile, int32_t aggregate, int32_t first = 0) {
   const int lane = lane_id();
   if (lane == 0)
-    __hip_atomic_store(lb.status + tile, lb_word(lb.epoch, tile == 0 ? 2u : 1u, aggregate), __ATOMIC_RELAXED,
+    __hip_atomic_store(lb.status + tile, lb_word(lb.epoch, tile == first ? 2u : 1u, aggregate), __ATOMIC_RELAXED,
                        __HIP_MEMORY_SCOPE_AGENT);
-  if (tile == 0) return 0;
+  if (tile == first) return 0;
   int32_t excl = 0;
   for (int32_t j0 = tile - 1;; j0 -= kLbWindow) {
     uint64_t w[kLbPerLane];
 #pragma unroll
-    for (int q = 0; q < kLbPerLane; ++q) w[q] = lb_wait(lb, j0 - lane * kLbPerLane - q);
+    for (int q = 0; q < kLbPerLane; ++q) w[q] = lb_wait(lb, j0 - lane * kLbPerLane - q, first);
     int q_incl = kLbPerLane;  // this lane's nearest inclusive prefix
 #pragma unroll
     for (int q = kLbPerLane - 1; q >= 0; --q)
@@ -359,6 +362,72 @@ __global__ __launch_bounds__(kBlock) void k_scan_lb(const In* in, int32_t* out, 
     const int64_t i = base + k * kBlock;
     const int32_t prefix = excl + s_wbase[k * kWaves + wave] + inc[k] - v[k];
     if (i < n) out[i] = kMode == 1 ? (v[k] ? prefix : -1) : prefix;
+  }
+}
+
+// The three keep maps of one plan application in one launch
+// (build_old_to_new for rows and columns, gpu_presolve.cu:878-934, and the
+// map of removed columns with a fixed interval): three look-back chains in
+// one status array, ticket order rows, then columns, then fixed columns.
+// The fixed-column mask is computed from keep_col, l and u on the way and
+// stored (removed, both bounds finite, |u - l| <= 1e-9).
+__global__ __launch_bounds__(kBlock) void k_apply_maps(gpk_apply_maps_args a, int32_t row_tiles, int32_t col_tiles,
+                                                       gpk_lookback lb) {
+  constexpr int kWaves = kBlock / 64;
+  __shared__ int32_t s_wbase[kScanItems * kWaves];
+  __shared__ int32_t s_excl;
+  const int32_t tile = lb_ticket(lb);
+  if (tile >= static_cast<int32_t>(gridDim.x)) {  // the ticket word was not zeroed
+    if (threadIdx.x == 0) __hip_atomic_store(lb.error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    return;
+  }
+  const int chain = tile < row_tiles ? 0 : (tile < row_tiles + col_tiles ? 1 : 2);
+  const int32_t first = chain == 0 ? 0 : (chain == 1 ? row_tiles : row_tiles + col_tiles);
+  const int64_t n = chain == 0 ? a.m : a.n;
+  int32_t* out = chain == 0 ? a.row_map : (chain == 1 ? a.col_map : a.fixed_map);
+  const int tid = threadIdx.x, lane = lane_id(), wave = tid >> 6;
+  const int64_t base = static_cast<int64_t>(tile - first) * kScanTile + tid;
+  int32_t v[kScanItems], inc[kScanItems];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k * kBlock;
+    int32_t x = 0;
+    if (i < n) {
+      if (chain == 0) {
+        x = a.keep_row[i] != 0;
+      } else if (chain == 1) {
+        x = a.keep_col[i] != 0;
+      } else {
+        const double lo = a.l[i], hi = a.u[i];
+        x = (!a.keep_col[i] && isfinite(lo) && isfinite(hi) && fabs(lo - hi) <= 1.0e-9) ? 1 : 0;
+        a.fixed_mask[i] = static_cast<uint8_t>(x);
+      }
+    }
+    v[k] = x;
+  }
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    inc[k] = wave_inclusive_sum(v[k]);
+    if (lane == 63) s_wbase[k * kWaves + wave] = inc[k];
+  }
+  __syncthreads();
+  if (tid < 64) {
+    const int32_t c = tid < kScanItems * kWaves ? s_wbase[tid] : 0;
+    const int32_t ci = wave_inclusive_sum(c);
+    if (tid < kScanItems * kWaves) s_wbase[tid] = ci - c;
+    const int32_t total = __shfl(ci, 63, 64);
+    const int32_t e = lb_exclusive(lb, tile, total, first);
+    if (tid == 0) s_excl = e;
+    const int32_t last = chain == 0 ? row_tiles - 1 : (chain == 1 ? row_tiles + col_tiles - 1 : gridDim.x - 1);
+    if (tid == 0 && tile == last) a.counts[chain] = e + total;
+  }
+  __syncthreads();
+  const int32_t excl = s_excl;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    const int64_t i = base + k * kBlock;
+    const int32_t prefix = excl + s_wbase[k * kWaves + wave] + inc[k] - v[k];
+    if (i < n) out[i] = v[k] ? prefix : -1;
   }
 }
 
@@ -643,6 +712,31 @@ __global__ __launch_bounds__(kBlock) void k_copy_many(CopyList list) {
   else if ((a & 7u) == 0) copy_units<uint2>(g);
   else if ((a & 3u) == 0) copy_units<uint32_t>(g);
   else copy_units<uint8_t>(g);
+}
+
+// The vector gathers of one plan application in one launch: c, l, u and
+// the column reduced-to-original map through col_map; AL, AU and the row
+// map through row_map (dst[map[i]] = src[i] for map[i] >= 0; a null
+// destination is skipped).
+__global__ void k_apply_gathers(gpk_apply_gathers_args a) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.n) {
+    const int32_t j = a.col_map[i];
+    if (j >= 0) {
+      a.c_out[j] = a.c_in[i];
+      a.l_out[j] = a.l_in[i];
+      a.u_out[j] = a.u_in[i];
+      if (a.col_r2o_out) a.col_r2o_out[j] = a.col_r2o_in[i];
+    }
+  }
+  if (i < a.m) {
+    const int32_t r = a.row_map[i];
+    if (r >= 0) {
+      a.al_out[r] = a.al_in[i];
+      a.au_out[r] = a.au_in[i];
+      if (a.row_r2o_out) a.row_r2o_out[r] = a.row_r2o_in[i];
+    }
+  }
 }
 
 __global__ void k_gather5(const int32_t* map, int32_t n_old, const double* s0, double* d0, const double* s1,
@@ -1005,6 +1099,22 @@ hipError_t gpk_scan_lb_mask(const uint8_t* mask, int32_t* out, int64_t n, gpk_lo
   return hipGetLastError();
 }
 
+int64_t gpk_apply_maps_tiles(int32_t m, int32_t n) {
+  const int64_t rt = m > 0 ? (m + kScanTile - 1) / kScanTile : 0;
+  const int64_t ct = n > 0 ? (n + kScanTile - 1) / kScanTile : 0;
+  return rt + 2 * ct;
+}
+
+hipError_t gpk_apply_maps(gpk_apply_maps_args args, gpk_lookback lb, hipStream_t s) {
+  const int64_t rt = args.m > 0 ? (args.m + kScanTile - 1) / kScanTile : 0;
+  const int64_t ct = args.n > 0 ? (args.n + kScanTile - 1) / kScanTile : 0;
+  const int64_t tiles = rt + 2 * ct;
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_maps, dim3(static_cast<unsigned>(tiles)), dim3(kBlock), 0, s, args,
+                     static_cast<int32_t>(rt), static_cast<int32_t>(ct), lb);
+  return hipGetLastError();
+}
+
 hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t* count, gpk_lookback lb,
                            hipStream_t s) {
   if (n <= 0) return hipSuccess;
@@ -1103,6 +1213,13 @@ hipError_t gpk_copy_many(const gpk_copy_seg* segs, int32_t count, hipStream_t s)
   const int64_t blocks = (most + per_block - 1) / per_block;
   hipLaunchKernelGGL(k_copy_many, dim3(static_cast<unsigned>(blocks < 1024 ? blocks : 1024), used), dim3(kBlock), 0,
                      s, list);
+  return hipGetLastError();
+}
+
+hipError_t gpk_apply_gathers(gpk_apply_gathers_args args, hipStream_t s) {
+  const int32_t len = args.m > args.n ? args.m : args.n;
+  if (len <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_apply_gathers, dim3(grid_for(len)), dim3(kBlock), 0, s, args);
   return hipGetLastError();
 }
 

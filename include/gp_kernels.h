@@ -62,6 +62,27 @@ hipError_t gpk_scan_lb_mask(const uint8_t* mask, int32_t* out, int64_t n, gpk_lo
 /* map[i] = mask[i] ? (number of set mask entries before i) : -1; *count = total. */
 hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t* count, gpk_lookback lb,
                            hipStream_t stream);
+/* The three keep maps of a plan application in one launch (three look-back
+ * chains): row_map from keep_row (m), col_map from keep_col (n), and
+ * fixed_map from the removed columns with a fixed finite interval
+ * (!keep_col && |u - l| <= 1e-9; the mask is stored in fixed_mask);
+ * map[i] = rank among the set entries or -1; counts[0..2] = the three totals.
+ * The look-back workspace covers gpk_apply_maps_tiles(m, n) tiles. */
+typedef struct gpk_apply_maps_args {
+  int32_t m, n;
+  const uint8_t* keep_row;
+  const uint8_t* keep_col;
+  const double* l;
+  const double* u;
+  int32_t* row_map;
+  int32_t* col_map;
+  uint8_t* fixed_mask;
+  int32_t* fixed_map;
+  int32_t* counts;
+} gpk_apply_maps_args;
+int64_t gpk_apply_maps_tiles(int32_t m, int32_t n);
+hipError_t gpk_apply_maps(gpk_apply_maps_args args, gpk_lookback lb, hipStream_t stream);
+
 /* Rank form of an old-to-new map, one word per 64 entries: bit b of word w
  * is set when entry 64w + b is kept; base is the new index of the word's
  * first kept entry.  gpk_build_rank writes ceil(n / 64) words. */
@@ -128,6 +149,29 @@ hipError_t gpk_csr_stats(const int32_t* ptr, const int32_t* idx, const double* v
                          int32_t* singleton_idx, double* singleton_val, hipStream_t stream);
 
 /* ---- compaction helpers ---------------------------------------------- */
+/* The gathers of a plan application in one launch (gpu_presolve.cu:2021-2147):
+ * c, l, u and the column reduced-to-original map through col_map (n); AL, AU
+ * and the row map through row_map (m).  Null r2o outputs are skipped. */
+typedef struct gpk_apply_gathers_args {
+  int32_t m, n;
+  const int32_t* col_map;
+  const double* c_in;
+  const double* l_in;
+  const double* u_in;
+  const int32_t* col_r2o_in;
+  double* c_out;
+  double* l_out;
+  double* u_out;
+  int32_t* col_r2o_out;
+  const int32_t* row_map;
+  const double* al_in;
+  const double* au_in;
+  const int32_t* row_r2o_in;
+  double* al_out;
+  double* au_out;
+  int32_t* row_r2o_out;
+} gpk_apply_gathers_args;
+hipError_t gpk_apply_gathers(gpk_apply_gathers_args args, hipStream_t stream);
 /* dst[map[i]] = src[i] for map[i] >= 0, for up to 5 vectors of the same length. */
 hipError_t gpk_gather5(const int32_t* map, int32_t n_old, const double* s0, double* d0,
                        const double* s1, double* d1, const double* s2, double* d2,
