@@ -103,11 +103,13 @@ class Ctx {
     std::memcpy(&v, mail(dev, static_cast<int>(sizeof(T) / 4), context), sizeof(T));
     return v;
   }
+  // Copies host memory into the context's host-mapped staging area at once
+  // (the source may be released on return) and enqueues the kernel that moves
+  // it to device memory at `dev`.
+  void upload(void* dev, const void* host, size_t bytes, const char* context);
   template <class T>
   void to_device(T* dev, const T* host, int64_t count, const char* context) {
-    if (count <= 0) return;
-    // Stream-ordered: any later wait on this stream finds the copy complete.
-    check(hipMemcpyAsync(dev, host, sizeof(T) * static_cast<size_t>(count), hipMemcpyHostToDevice, stream_), context);
+    if (count > 0) upload(dev, host, sizeof(T) * static_cast<size_t>(count), context);
   }
 
  private:
@@ -131,12 +133,15 @@ class Ctx {
     void* dst;
     size_t offset, bytes;
   };
-  char* h_stage_ = nullptr;  // host-mapped staging area of download()
+  char* h_stage_ = nullptr;  // host-mapped staging area of download() and upload()
   char* d_stage_ = nullptr;
   size_t stage_cap_ = 0, stage_used_ = 0;
   std::vector<Download> downloads_;
   void check_error_flag(const char* context);
   void finish_downloads();
+  // Offset of `bytes` of free staging space (waits for the stream and grows
+  // the area when it is full); released at the next wait.
+  size_t stage_reserve(size_t bytes, const char* context);
   int32_t* h_error_ = nullptr;  // host-mapped
   int32_t* d_error_ = nullptr;
 };

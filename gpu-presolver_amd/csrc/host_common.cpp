@@ -124,11 +124,10 @@ void Ctx::check_error_flag(const char* context) {
 
 void Ctx::sync(const char* context) { (void)mail(nullptr, 0, context); }
 
-void Ctx::download(void* dst, const void* dev, size_t bytes, const char* context) {
-  if (bytes == 0) return;
+size_t Ctx::stage_reserve(size_t bytes, const char* context) {
   size_t off = (stage_used_ + 255) & ~static_cast<size_t>(255);
   if (off + bytes > stage_cap_) {
-    if (!downloads_.empty()) sync(context);  // the staged copies in flight land first
+    if (stage_used_ > 0) sync(context);  // staged copies in flight complete (and the area is released)
     off = 0;
     if (bytes > stage_cap_) {
       if (h_stage_) check(hipHostFree(h_stage_), "hipHostFree staging");
@@ -142,10 +141,24 @@ void Ctx::download(void* dst, const void* dev, size_t bytes, const char* context
       stage_cap_ = cap;
     }
   }
+  stage_used_ = off + bytes;
+  return off;
+}
+
+void Ctx::download(void* dst, const void* dev, size_t bytes, const char* context) {
+  if (bytes == 0) return;
+  const size_t off = stage_reserve(bytes, context);
   const gpk_copy_seg seg{d_stage_ + off, dev, static_cast<int64_t>(bytes)};
   check(gpk_copy_many(&seg, 1, stream_), context);
   downloads_.push_back(Download{dst, off, bytes});
-  stage_used_ = off + bytes;
+}
+
+void Ctx::upload(void* dev, const void* host, size_t bytes, const char* context) {
+  if (bytes == 0) return;
+  const size_t off = stage_reserve(bytes, context);
+  std::memcpy(h_stage_ + off, host, bytes);
+  const gpk_copy_seg seg{dev, d_stage_ + off, static_cast<int64_t>(bytes)};
+  check(gpk_copy_many(&seg, 1, stream_), context);
 }
 
 void Ctx::finish_downloads() {

@@ -275,26 +275,29 @@ void tape_index_kinds(const PostsolveTape& t, std::vector<uint8_t>& kind) {
 void append_global_tape(RecordState& rs, const PostsolveTape& src) {
   if (src.types.empty()) return;
   PostsolveTape& dst = rs.rec->tape;
-  std::vector<int32_t> idx = src.indices;
-  if (!idx.empty()) {
+  const int64_t cnt = static_cast<int64_t>(src.indices.size());
+  const int32_t base_idx = static_cast<int32_t>(dst.indices.size());
+  const int32_t base_val = static_cast<int32_t>(dst.vals.size());
+  dst.indices.resize(dst.indices.size() + static_cast<size_t>(cnt));
+  if (cnt > 0) {
     std::vector<uint8_t> kind;
     tape_index_kinds(src, kind);
     auto& c = ctx();
-    const int64_t cnt = static_cast<int64_t>(idx.size());
     DBuf<int32_t> d_idx(cnt);
     DBuf<uint8_t> d_kind(cnt);
-    c.to_device(d_idx.get(), idx.data(), cnt, "tape indices upload");
+    c.to_device(d_idx.get(), src.indices.data(), cnt, "tape indices upload");
     c.to_device(d_kind.get(), kind.data(), cnt, "tape kinds upload");
     check(gpk_map_tape_indices(static_cast<int32_t>(cnt), d_idx, d_kind, rs.row_r2o, rs.m, rs.col_r2o, rs.n,
                                c.stream()),
           "tape globalize");
-    idx = c.to_host(d_idx.get(), cnt, "tape indices download");
+    // The globalized indices land in the record at the context's next wait:
+    // every plan application waits before it appends again, and presolve
+    // waits before it returns, so no synchronization is spent here.
+    c.download(dst.indices.data() + base_idx, d_idx.get(), sizeof(int32_t) * static_cast<size_t>(cnt),
+               "tape indices download");
   }
-  const int32_t base_idx = static_cast<int32_t>(dst.indices.size());
-  const int32_t base_val = static_cast<int32_t>(dst.vals.size());
   dst.types.insert(dst.types.end(), src.types.begin(), src.types.end());
   dst.dual_modes.insert(dst.dual_modes.end(), src.dual_modes.begin(), src.dual_modes.end());
-  dst.indices.insert(dst.indices.end(), idx.begin(), idx.end());
   dst.vals.insert(dst.vals.end(), src.vals.begin(), src.vals.end());
   for (size_t k = 1; k < src.index_starts.size(); ++k) {
     dst.index_starts.push_back(base_idx + src.index_starts[k]);
@@ -329,9 +332,9 @@ void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t
     const size_t base = rec.fixed_idx.size();
     rec.fixed_idx.resize(base + static_cast<size_t>(k));
     rec.fixed_val.resize(base + static_cast<size_t>(k));
+    // Land at the context's next wait, like the tape indices below.
     c.download(rec.fixed_idx.data() + base, oi.get(), sizeof(int32_t) * k, "fixed records idx");
     c.download(rec.fixed_val.data() + base, ov.get(), sizeof(double) * k, "fixed records val");
-    c.sync("fixed records");
   }
   append_global_tape(rs, p.tape);
   if (m_new < m) {
