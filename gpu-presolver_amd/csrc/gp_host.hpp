@@ -284,6 +284,9 @@ void row_activity(const int32_t* ptr, const int32_t* idx, const double* val, int
                   hipStream_t stream);
 // Exclusive scan of a uint8 mask into out[0..n]; returns the count.
 int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out);
+// The same scans without reading the total back (out[n] holds it).
+void scan_mask_launch(const uint8_t* mask, int32_t n, int32_t* out);
+void scan_counts_launch(const int32_t* in, int32_t n, int32_t* out);
 // Exclusive scan of int32 counts into out[0..n]; returns the total.
 int32_t scan_counts(const int32_t* in, int32_t n, int32_t* out);
 // Sum, in ascending index order, of vals[i] over mask[i] != 0 (computed on

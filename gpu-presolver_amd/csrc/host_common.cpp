@@ -337,38 +337,54 @@ void row_activity(const int32_t* ptr, const int32_t* idx, const double* val, int
   check(st, "row activity");
 }
 
-int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out) {
+void scan_mask_launch(const uint8_t* mask, int32_t n, int32_t* out) {
   Ctx& c = ctx();
-  if (n <= 0) {
-    check(gpk_fill_i32(out, 0, 1, c.stream()), "scan empty");
-    return 0;
-  }
-  check(gpk_scan_lb_mask(mask, out, n, c.lookback(gpk_lookback_tiles(n)), c.stream()), "scan mask");
-  return c.scalar_to_host(out + n, "scan total");
+  if (n <= 0) check(gpk_fill_i32(out, 0, 1, c.stream()), "scan empty");
+  else check(gpk_scan_lb_mask(mask, out, n, c.lookback(gpk_lookback_tiles(n)), c.stream()), "scan mask");
+}
+
+int32_t scan_mask_count(const uint8_t* mask, int32_t n, int32_t* out) {
+  scan_mask_launch(mask, n, out);
+  return n <= 0 ? 0 : ctx().scalar_to_host(out + n, "scan total");
+}
+
+void scan_counts_launch(const int32_t* in, int32_t n, int32_t* out) {
+  Ctx& c = ctx();
+  if (n <= 0) check(gpk_fill_i32(out, 0, 1, c.stream()), "scan empty");
+  else check(gpk_scan_lb_i32(in, out, n, c.lookback(gpk_lookback_tiles(n)), c.stream()), "scan counts");
 }
 
 int32_t scan_counts(const int32_t* in, int32_t n, int32_t* out) {
-  Ctx& c = ctx();
-  if (n <= 0) {
-    check(gpk_fill_i32(out, 0, 1, c.stream()), "scan empty");
-    return 0;
-  }
-  check(gpk_scan_lb_i32(in, out, n, c.lookback(gpk_lookback_tiles(n)), c.stream()), "scan counts");
-  return c.scalar_to_host(out + n, "scan total");
+  scan_counts_launch(in, n, out);
+  return n <= 0 ? 0 : ctx().scalar_to_host(out + n, "scan total");
 }
+
+// Terms fetched together with their count (32 KB).
+constexpr int32_t kSpeculativeTerms = 4096;
 
 double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n) {
   if (n <= 0) return 0.0;
   Ctx& c = ctx();
+  hipStream_t s = c.stream();
   DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1);
-  const int32_t k = scan_mask_count(mask, n, prefix);
-  if (k == 0) return 0.0;
-  DBuf<double> packed(k);
-  check(gpk_pack_f64(mask, vals, prefix, n, packed, c.stream()), "pack objective terms");
-  const std::vector<double> h = c.to_host(packed.get(), k, "objective terms");
-  double s = 0.0;
-  for (double v : h) s += v;
-  return s;
+  DBuf<double> packed(n);
+  check(gpk_scan_lb_mask(mask, prefix, n, c.lookback(gpk_lookback_tiles(n)), s), "scan objective mask");
+  check(gpk_pack_f64(mask, vals, prefix, n, packed, s), "pack objective terms");
+  // The first kSpeculativeTerms packed terms travel with the count, so the
+  // usual case (few terms) costs one wait.
+  const int32_t head = std::min<int32_t>(n, kSpeculativeTerms);
+  std::vector<double> h(static_cast<size_t>(head));
+  c.download(h.data(), packed.get(), sizeof(double) * static_cast<size_t>(head), "objective terms");
+  const int32_t k = c.scalar_to_host(prefix.get() + n, "objective term count");
+  if (k > head) {
+    h.resize(static_cast<size_t>(k));
+    c.download(h.data() + head, packed.get() + head, sizeof(double) * static_cast<size_t>(k - head),
+               "objective terms");
+    c.sync("objective terms");
+  }
+  double sum = 0.0;
+  for (int32_t t = 0; t < k; ++t) sum += h[static_cast<size_t>(t)];
+  return sum;
 }
 
 int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map) {

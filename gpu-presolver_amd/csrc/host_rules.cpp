@@ -54,14 +54,26 @@ void zero(T* dst, int64_t n, const char* context) {
 }
 
 // Packs the indices i with mask[i] != 0 (ascending) and copies them to the host.
+// Ascending indices of the set mask entries; the first 4096 travel with
+// the count, so a short list costs one wait.
 std::vector<int32_t> masked_indices(const uint8_t* mask, int32_t n) {
   if (n <= 0) return {};
-  DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1);
-  const int32_t k = amd::scan_mask_count(mask, n, prefix);
-  if (k == 0) return {};
-  DBuf<int32_t> out(k);
-  check(gpk_pack_index(mask, prefix, n, out, ctx().stream()), "pack indices");
-  return ctx().to_host(out.get(), k, "indices to host");
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  DBuf<int32_t> prefix(static_cast<int64_t>(n) + 1), out(n);
+  check(gpk_scan_lb_mask(mask, prefix, n, c.lookback(gpk_lookback_tiles(n)), s), "scan mask");
+  check(gpk_pack_index(mask, prefix, n, out, s), "pack indices");
+  const int32_t head = std::min<int32_t>(n, 4096);
+  std::vector<int32_t> h(static_cast<size_t>(head));
+  c.download(h.data(), out.get(), sizeof(int32_t) * static_cast<size_t>(head), "indices to host");
+  const int32_t k = c.scalar_to_host(prefix.get() + n, "index count");
+  if (k > head) {
+    h.resize(static_cast<size_t>(k));
+    c.download(h.data() + head, out.get() + head, sizeof(int32_t) * static_cast<size_t>(k - head), "indices to host");
+    c.sync("indices to host");
+  }
+  h.resize(static_cast<size_t>(k));
+  return h;
 }
 
 }  // namespace
@@ -474,8 +486,12 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
     DBuf<uint8_t> sel(m);
     DBuf<int32_t> sel_prefix(static_cast<int64_t>(m) + 1), sup(static_cast<int64_t>(m) + 1);
     check(gpk_cs_subcol_counts(A, P, owner, col_del, sel, sup, s), "subcol counts");
-    const int32_t records = amd::scan_mask_count(sel, m, sel_prefix);
-    const int32_t support = amd::scan_counts(sup, m, sup);
+    amd::scan_mask_launch(sel, m, sel_prefix);
+    amd::scan_counts_launch(sup, m, sup);
+    int32_t records = 0, support = 0;  // both totals in one wait
+    c.download(&records, sel_prefix.get() + m, sizeof(int32_t), "subcol record count");
+    c.download(&support, sup.get() + m, sizeof(int32_t), "subcol support count");
+    c.sync("subcol counts");
     if (records > 0) {
       DBuf<int32_t> oi3(3 * static_cast<int64_t>(records)), scol(support);
       DBuf<double> ov6(6 * static_cast<int64_t>(records)), sval(support);
