@@ -481,8 +481,9 @@ __global__ __launch_bounds__(kSmallBlock) void k_scan_small(const In* in, int32_
   }
 }
 
-// Compaction in three launches (gpu_presolve.cu:1881-1978) over tiles of 256
-// old rows: k_compact_map maps every entry once (row kept and column rank,
+// Compaction (gpu_presolve.cu:1881-1978): after the column map's rank
+// words, three launches over tiles of 256 old rows: k_compact_map maps
+// every entry once (row kept and column rank,
 // one cache-resident rank gather per entry), stores the new column index or
 // -1 per entry, each kept row's kept count at out_ptr[new row] and the
 // tile's kept count; an exclusive scan of the tile counts (one small array,
