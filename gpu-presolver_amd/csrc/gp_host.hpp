@@ -295,8 +295,10 @@ double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n);
 // Compacts A by row/column keep maps; the result owns fresh device arrays.
 // known_nnz >= 0 is the kept count when the caller knows it (the transpose
 // of a matrix compacted with the same maps): the read-back is skipped.
+// col_rank: the column map's rank words when the caller has them.
 presolve::DeviceCsrMatrix compact_csr(const presolve::DeviceCsrMatrix& A, const int32_t* row_map,
-                                      const int32_t* col_map, int32_t m_new, int32_t n_new, int32_t known_nnz = -1);
+                                      const int32_t* col_map, int32_t m_new, int32_t n_new, int32_t known_nnz = -1,
+                                      const gpk_rank_word* col_rank = nullptr);
 // Stable transpose (rows of the result sorted); fresh device arrays.
 presolve::DeviceCsrMatrix transpose_csr(const presolve::DeviceCsrMatrix& A);
 void free_csr(presolve::DeviceCsrMatrix& A) noexcept;

@@ -411,7 +411,7 @@ void free_csr(DeviceCsrMatrix& A) noexcept {
 // per-row kept counts), an exclusive scan into the row pointers, then the
 // coalesced write of the kept entries.
 DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, const int32_t* col_map, int32_t m_new,
-                            int32_t n_new, int32_t known_nnz) {
+                            int32_t n_new, int32_t known_nnz, const gpk_rank_word* col_rank) {
   Ctx& c = ctx();
   DeviceCsrMatrix out;
   out.rows = m_new;
@@ -424,13 +424,17 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
     return out;
   }
   const int32_t m_old = A.rows;
-  DBuf<gpk_rank_word> rank((static_cast<int64_t>(A.cols) + 63) / 64, "compact column rank");
+  DBuf<gpk_rank_word> rank;
   DBuf<int32_t> jn(A.nnz, "compact entry map");
   hipStream_t s = c.stream();
-  check(gpk_build_rank(col_map, A.cols, rank, s), "compact column rank");
+  if (col_rank == nullptr) {
+    rank.alloc((static_cast<int64_t>(A.cols) + 63) / 64, "compact column rank");
+    check(gpk_build_rank(col_map, A.cols, rank, s), "compact column rank");
+    col_rank = rank.get();
+  }
   const int32_t tiles = gpk_compact_tiles(m_old);
   DBuf<int32_t> tile_off(static_cast<int64_t>(tiles) + 1, "compact tile offsets");
-  check(gpk_compact_map(A.rowPtr, A.colVal, m_old, row_map, rank, jn, ptr, tile_off, s), "compact map");
+  check(gpk_compact_map(A.rowPtr, A.colVal, m_old, row_map, col_rank, jn, ptr, tile_off, s), "compact map");
   check(gpk_scan_lb_i32(tile_off, tile_off, tiles, c.lookback(gpk_lookback_tiles(tiles)), s), "compact tile offsets");
   // Kept entries are at most A.nnz: the outputs are sized for that and the
   // kept count is read back once the entries are written.

@@ -371,8 +371,11 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   fixed.mask.alloc(n);
   fixed.map.alloc(static_cast<int64_t>(n) + 1);
   int32_t* counts = c.flags();
+  // Rank words of both maps (the column map's for compacting A, the row
+  // map's for compacting AT) come out of the same launch.
+  DBuf<gpk_rank_word> row_rank((static_cast<int64_t>(m) + 63) / 64), col_rank((static_cast<int64_t>(n) + 63) / 64);
   const gpk_apply_maps_args maps{m, n, p.keep_row_mask, p.keep_col_mask, p.new_l, p.new_u, row_map, col_map,
-                                 fixed.mask, fixed.map, counts};
+                                 fixed.mask, fixed.map, counts, row_rank.get(), col_rank.get()};
   check(gpk_apply_maps(maps, c.lookback(gpk_apply_maps_tiles(m, n)), s), "keep maps");
   const int32_t* h = c.read_flags(3);
   const int32_t m_new = h[0], n_new = h[1];
@@ -392,13 +395,13 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   }
   const DeviceCsrMatrix& src = p.has_new_A ? p.new_A : cur.lp.A;
   LPInfoGpu next;
-  next.A = amd::compact_csr(src, row_map, col_map, m_new, n_new);
+  next.A = amd::compact_csr(src, row_map, col_map, m_new, n_new, -1, col_rank.get());
   // A kept AT is compacted with the maps swapped; a rewritten A, or the
   // caller's AT on the first change, is re-transposed.
   if (p.has_new_A || !cur.owns) {
     next.AT = amd::transpose_csr(next.A);
   } else {
-    next.AT = amd::compact_csr(cur.lp.AT, col_map, row_map, n_new, m_new, next.A.nnz);
+    next.AT = amd::compact_csr(cur.lp.AT, col_map, row_map, n_new, m_new, next.A.nnz, row_rank.get());
   }
   next.c = pool_f64(n_new, "working c");
   next.l = pool_f64(n_new, "working l");

@@ -423,11 +423,27 @@ __global__ __launch_bounds__(kBlock) void k_apply_maps(gpk_apply_maps_args a, in
   }
   __syncthreads();
   const int32_t excl = s_excl;
+  gpk_rank_word* rank = chain == 0 ? a.row_rank : (chain == 1 ? a.col_rank : nullptr);
 #pragma unroll
   for (int k = 0; k < kScanItems; ++k) {
     const int64_t i = base + k * kBlock;
     const int32_t prefix = excl + s_wbase[k * kWaves + wave] + inc[k] - v[k];
     if (i < n) out[i] = v[k] ? prefix : -1;
+    if (rank) {
+      // A wave's 64 items of slice k are 64 consecutive, 64-aligned entries:
+      // one rank word (kept bits and the new index of the first kept one).
+      const uint64_t bits = __ballot(v[k] != 0);
+      const int lead = bits ? __ffsll(static_cast<unsigned long long>(bits)) - 1 : 0;
+      const int32_t first_new = __shfl(prefix, lead, 64);
+      const int64_t i0 = i - lane;
+      if (lane == 0 && i0 < n) {
+        gpk_rank_word w;
+        w.bits = bits;
+        w.base = bits ? first_new : 0;
+        w.pad = 0;
+        rank[i0 >> 6] = w;
+      }
+    }
   }
 }
 

@@ -62,11 +62,22 @@ hipError_t gpk_scan_lb_mask(const uint8_t* mask, int32_t* out, int64_t n, gpk_lo
 /* map[i] = mask[i] ? (number of set mask entries before i) : -1; *count = total. */
 hipError_t gpk_mask_to_map(const uint8_t* mask, int32_t* map, int64_t n, int32_t* count, gpk_lookback lb,
                            hipStream_t stream);
+/* Rank form of an old-to-new map, one word per 64 entries: bit b of word w
+ * is set when entry 64w + b is kept; base is the new index of the word's
+ * first kept entry.  gpk_build_rank writes ceil(n / 64) words. */
+typedef struct __attribute__((aligned(16))) gpk_rank_word {
+  uint64_t bits;
+  int32_t base;
+  int32_t pad;
+} gpk_rank_word;
+hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hipStream_t stream);
 /* The three keep maps of a plan application in one launch (three look-back
  * chains): row_map from keep_row (m), col_map from keep_col (n), and
  * fixed_map from the removed columns with a fixed finite interval
  * (!keep_col && |u - l| <= 1e-9; the mask is stored in fixed_mask);
- * map[i] = rank among the set entries or -1; counts[0..2] = the three totals.
+ * map[i] = rank among the set entries or -1; counts[0..2] = the three totals;
+ * row_rank / col_rank, when given, receive the maps' rank words (as
+ * gpk_build_rank).
  * The look-back workspace covers gpk_apply_maps_tiles(m, n) tiles. */
 typedef struct gpk_apply_maps_args {
   int32_t m, n;
@@ -79,19 +90,12 @@ typedef struct gpk_apply_maps_args {
   uint8_t* fixed_mask;
   int32_t* fixed_map;
   int32_t* counts;
+  gpk_rank_word* row_rank; /* optional: rank words of row_map ((m + 63) / 64) */
+  gpk_rank_word* col_rank; /* optional: rank words of col_map ((n + 63) / 64) */
 } gpk_apply_maps_args;
 int64_t gpk_apply_maps_tiles(int32_t m, int32_t n);
 hipError_t gpk_apply_maps(gpk_apply_maps_args args, gpk_lookback lb, hipStream_t stream);
 
-/* Rank form of an old-to-new map, one word per 64 entries: bit b of word w
- * is set when entry 64w + b is kept; base is the new index of the word's
- * first kept entry.  gpk_build_rank writes ceil(n / 64) words. */
-typedef struct __attribute__((aligned(16))) gpk_rank_word {
-  uint64_t bits;
-  int32_t base;
-  int32_t pad;
-} gpk_rank_word;
-hipError_t gpk_build_rank(const int32_t* map, int32_t n, gpk_rank_word* out, hipStream_t stream);
 /* Stable compaction of a CSR by an old-to-new row map and the rank form of
  * the column map, over tiles of 256 old rows (gpk_compact_tiles(m_old)):
  *   gpk_compact_map    jn[p] = new column of entry p, or -1 when its row or
