@@ -410,6 +410,10 @@ void free_csr(DeviceCsrMatrix& A) noexcept {
 // rank form, one mapping pass over the entries (per-entry new column and
 // per-row kept counts), an exclusive scan into the row pointers, then the
 // coalesced write of the kept entries.
+// Up to this many 256-row tiles the write pass derives the tile offsets
+// itself (gpk_compact_write_counts), one launch fewer.
+constexpr int32_t kCompactFewTiles = 256;
+
 DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, const int32_t* col_map, int32_t m_new,
                             int32_t n_new, int32_t known_nnz, const gpk_rank_word* col_rank) {
   Ctx& c = ctx();
@@ -435,12 +439,20 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
   const int32_t tiles = gpk_compact_tiles(m_old);
   DBuf<int32_t> tile_off(static_cast<int64_t>(tiles) + 1, "compact tile offsets");
   check(gpk_compact_map(A.rowPtr, A.colVal, m_old, row_map, col_rank, jn, ptr, tile_off, s), "compact map");
-  check(gpk_scan_lb_i32(tile_off, tile_off, tiles, c.lookback(gpk_lookback_tiles(tiles)), s), "compact tile offsets");
+  const bool few_tiles = tiles <= kCompactFewTiles;
+  if (!few_tiles)
+    check(gpk_scan_lb_i32(tile_off, tile_off, tiles, c.lookback(gpk_lookback_tiles(tiles)), s),
+          "compact tile offsets");
   // Kept entries are at most A.nnz: the outputs are sized for that and the
   // kept count is read back once the entries are written.
   DBuf<int32_t> idx(A.nnz, "compact colVal");
   DBuf<double> val(A.nnz, "compact nzVal");
-  check(gpk_compact_write(A.rowPtr, A.nzVal, m_old, row_map, jn, tile_off, ptr, m_new, idx, val, s), "compact write");
+  if (few_tiles)
+    check(gpk_compact_write_counts(A.rowPtr, A.nzVal, m_old, row_map, jn, tile_off, ptr, m_new, idx, val, s),
+          "compact write");
+  else
+    check(gpk_compact_write(A.rowPtr, A.nzVal, m_old, row_map, jn, tile_off, ptr, m_new, idx, val, s),
+          "compact write");
   out.nnz = known_nnz >= 0 ? known_nnz : c.scalar_to_host(ptr.get() + m_new, "compact kept count");
   out.rowPtr = ptr.release();
   out.colVal = idx.release();

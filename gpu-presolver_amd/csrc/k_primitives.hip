@@ -617,6 +617,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restric
   }
 }
 
+template <bool kOffsetsFromCounts>
 __global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restrict__ ptr,
                                                           const double* __restrict__ val, int32_t m_old,
                                                           const int32_t* __restrict__ row_map,
@@ -636,8 +637,31 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restr
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = tid >> 6;
   const int32_t my_new_row = tid < rows ? row_map[r0 + tid] : -1;
-  int32_t base = tile_off[blockIdx.x];
-  if (blockIdx.x == gridDim.x - 1 && tid == 0) out_ptr[m_new] = tile_off[gridDim.x];
+  int32_t base;
+  if (kOffsetsFromCounts) {
+    // Few tiles (gridDim.x <= kBlock): tile_off holds the raw per-tile kept
+    // counts and each block sums those of the tiles before it (and the last
+    // block all of them) instead of waiting for a scan launch.
+    const int32_t t = tid < static_cast<int>(gridDim.x) ? tile_off[tid] : 0;
+    const int32_t before = wave_inclusive_sum(tid < static_cast<int>(blockIdx.x) ? t : 0);
+    const int32_t all = wave_inclusive_sum(t);
+    if (lane == 63) {
+      s_wsum[wave] = before;
+      s_wcnt[wave] = all;  // s_wcnt is free until the chunk loop
+    }
+    __syncthreads();
+    int32_t b = 0, a = 0;
+    for (int w = 0; w < kWaves; ++w) {
+      b += s_wsum[w];
+      a += s_wcnt[w];
+    }
+    base = b;
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) out_ptr[m_new] = a;
+    __syncthreads();  // s_wsum is reused below
+  } else {
+    base = tile_off[blockIdx.x];
+    if (blockIdx.x == gridDim.x - 1 && tid == 0) out_ptr[m_new] = tile_off[gridDim.x];
+  }
   {  // row pointers of the tile's kept rows: tile offset + block exclusive scan of the counts
     const int32_t cnt = my_new_row >= 0 ? out_ptr[my_new_row] : 0;
     const int32_t inc = wave_inclusive_sum(cnt);
@@ -1168,8 +1192,19 @@ hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_ol
                              const int32_t* jn, const int32_t* tile_off, int32_t* out_ptr, int32_t m_new,
                              int32_t* out_idx, double* out_val, hipStream_t s) {
   if (m_old <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_compact_write, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, val, m_old, row_map,
-                     jn, tile_off, out_ptr, m_new, out_idx, out_val);
+  hipLaunchKernelGGL(k_compact_write<false>, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, val, m_old,
+                     row_map, jn, tile_off, out_ptr, m_new, out_idx, out_val);
+  return hipGetLastError();
+}
+
+hipError_t gpk_compact_write_counts(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
+                                    const int32_t* jn, const int32_t* tile_cnt, int32_t* out_ptr, int32_t m_new,
+                                    int32_t* out_idx, double* out_val, hipStream_t s) {
+  if (m_old <= 0) return hipSuccess;
+  const unsigned tiles = grid_for(m_old, kCompRows);
+  if (tiles > static_cast<unsigned>(kBlock)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_compact_write<true>, dim3(tiles), dim3(kBlock), 0, s, ptr, val, m_old, row_map, jn, tile_cnt,
+                     out_ptr, m_new, out_idx, out_val);
   return hipGetLastError();
 }
 

@@ -114,6 +114,11 @@ hipError_t gpk_compact_map(const int32_t* ptr, const int32_t* idx, int32_t m_old
 hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
                              const int32_t* jn, const int32_t* tile_off, int32_t* out_ptr, int32_t m_new,
                              int32_t* out_idx, double* out_val, hipStream_t stream);
+/* The same without the scan, for at most 256 tiles (m_old <= 65,536): it
+ * takes the raw tile counts of gpk_compact_map and derives the offsets. */
+hipError_t gpk_compact_write_counts(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
+                                    const int32_t* jn, const int32_t* tile_cnt, int32_t* out_ptr, int32_t m_new,
+                                    int32_t* out_idx, double* out_val, hipStream_t stream);
 
 /* Up to GPK_COPY_MAX copies (src != NULL) or zero fills (src == NULL) of
  * device byte ranges in one launch (block row y serves segment y; the unit

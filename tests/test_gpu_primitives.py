@@ -69,8 +69,11 @@ def test_col_max_abs(m, n, seed):
     assert_same_array("col_max_abs", gp_ffi.col_max_abs(n, m, tp, ti, tv), O.col_max_abs(n, m, tp, ti, tv))
 
 
+# 65,536 rows are 256 tiles (offsets derived in the write pass), 65,537 rows
+# 257 tiles (a scan of the tile counts).
 @pytest.mark.parametrize("m,n,seed,frac", [(0, 4, 1, 0.5), (5, 0, 2, 0.5), (300, 400, 3, 0.3),
-                                          (20000, 25000, 4, 0.15), (1000, 800, 5, 1.0), (1000, 800, 6, 0.0)])
+                                          (20000, 25000, 4, 0.15), (1000, 800, 5, 1.0), (1000, 800, 6, 0.0),
+                                          (65536, 3000, 7, 0.2), (65537, 3000, 8, 0.2)])
 def test_compact(m, n, seed, frac):
     ptr, idx, val = _random_csr(m, n, seed)
     rng = np.random.default_rng(seed)
