@@ -107,6 +107,12 @@ class Ctx {
   // (the source may be released on return) and enqueues the kernel that moves
   // it to device memory at `dev`.
   void upload(void* dev, const void* host, size_t bytes, const char* context);
+  // Copies host memory into the staging area and returns its device address,
+  // for a kernel to read (and rewrite) in place over the bus; with
+  // `dst_after`, the area's first `dst_bytes` bytes, as the kernels left
+  // them, are copied there at the next wait.  The address is valid until
+  // that wait.
+  void* stage(const void* host, size_t bytes, void* dst_after, size_t dst_bytes, const char* context);
   template <class T>
   void to_device(T* dev, const T* host, int64_t count, const char* context) {
     if (count > 0) upload(dev, host, sizeof(T) * static_cast<size_t>(count), context);

@@ -153,6 +153,15 @@ void Ctx::download(void* dst, const void* dev, size_t bytes, const char* context
   downloads_.push_back(Download{dst, off, bytes});
 }
 
+void* Ctx::stage(const void* host, size_t bytes, void* dst_after, size_t dst_bytes, const char* context) {
+  if (bytes == 0) return nullptr;
+  if (dst_bytes > bytes) throw std::invalid_argument("Ctx::stage: dst_bytes exceeds the staged bytes");
+  const size_t off = stage_reserve(bytes, context);
+  std::memcpy(h_stage_ + off, host, bytes);
+  if (dst_after && dst_bytes > 0) downloads_.push_back(Download{dst_after, off, dst_bytes});
+  return d_stage_ + off;
+}
+
 void Ctx::upload(void* dev, const void* host, size_t bytes, const char* context) {
   if (bytes == 0) return;
   const size_t off = stage_reserve(bytes, context);

@@ -12,12 +12,13 @@
 #include <algorithm>
 #include <atomic>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <optional>
 #include <string>
-#include <cmath>
 #include <utility>
 #include <vector>
 
@@ -283,18 +284,22 @@ void append_global_tape(RecordState& rs, const PostsolveTape& src) {
     std::vector<uint8_t> kind;
     tape_index_kinds(src, kind);
     auto& c = ctx();
-    DBuf<int32_t> d_idx(cnt);
-    DBuf<uint8_t> d_kind(cnt);
-    c.to_device(d_idx.get(), src.indices.data(), cnt, "tape indices upload");
-    c.to_device(d_kind.get(), kind.data(), cnt, "tape kinds upload");
-    check(gpk_map_tape_indices(static_cast<int32_t>(cnt), d_idx, d_kind, rs.row_r2o, rs.m, rs.col_r2o, rs.n,
+    // The indices and their kinds go to host-mapped staging memory, the
+    // kernel rewrites the indices there in place, and they land in the record
+    // at the context's next wait (every plan application waits before it
+    // appends again, and presolve waits before it returns): one launch, no
+    // synchronization of its own.
+    // One staged block: the indices, then their kinds.
+    const size_t bytes = sizeof(int32_t) * static_cast<size_t>(cnt);
+    std::vector<char> block(bytes + static_cast<size_t>(cnt));
+    std::memcpy(block.data(), src.indices.data(), bytes);
+    std::memcpy(block.data() + bytes, kind.data(), static_cast<size_t>(cnt));
+    char* d_block = static_cast<char*>(
+        c.stage(block.data(), block.size(), dst.indices.data() + base_idx, bytes, "tape indices"));
+    check(gpk_map_tape_indices(static_cast<int32_t>(cnt), reinterpret_cast<int32_t*>(d_block),
+                               reinterpret_cast<const uint8_t*>(d_block + bytes), rs.row_r2o, rs.m, rs.col_r2o, rs.n,
                                c.stream()),
           "tape globalize");
-    // The globalized indices land in the record at the context's next wait:
-    // every plan application waits before it appends again, and presolve
-    // waits before it returns, so no synchronization is spent here.
-    c.download(dst.indices.data() + base_idx, d_idx.get(), sizeof(int32_t) * static_cast<size_t>(cnt),
-               "tape indices download");
   }
   dst.types.insert(dst.types.end(), src.types.begin(), src.types.end());
   dst.dual_modes.insert(dst.dual_modes.end(), src.dual_modes.begin(), src.dual_modes.end());
