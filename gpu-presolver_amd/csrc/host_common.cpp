@@ -145,8 +145,17 @@ size_t Ctx::stage_reserve(size_t bytes, const char* context) {
   return off;
 }
 
+// Copies larger than this bypass the staging area (a direct copy and a
+// stream synchronization), so the pinned area stays modest.
+constexpr size_t kStageMaxBytes = size_t{64} << 20;
+
 void Ctx::download(void* dst, const void* dev, size_t bytes, const char* context) {
   if (bytes == 0) return;
+  if (bytes > kStageMaxBytes) {
+    check(hipMemcpyAsync(dst, dev, bytes, hipMemcpyDeviceToHost, stream_), context);
+    check(hipStreamSynchronize(stream_), context);
+    return;
+  }
   const size_t off = stage_reserve(bytes, context);
   const gpk_copy_seg seg{d_stage_ + off, dev, static_cast<int64_t>(bytes)};
   check(gpk_copy_many(&seg, 1, stream_), context);
@@ -156,6 +165,7 @@ void Ctx::download(void* dst, const void* dev, size_t bytes, const char* context
 void* Ctx::stage(const void* host, size_t bytes, void* dst_after, size_t dst_bytes, const char* context) {
   if (bytes == 0) return nullptr;
   if (dst_bytes > bytes) throw std::invalid_argument("Ctx::stage: dst_bytes exceeds the staged bytes");
+  if (bytes > kStageMaxBytes) throw std::length_error("Ctx::stage: more than 64 MB to stage");
   const size_t off = stage_reserve(bytes, context);
   std::memcpy(h_stage_ + off, host, bytes);
   if (dst_after && dst_bytes > 0) downloads_.push_back(Download{dst_after, off, dst_bytes});
@@ -164,6 +174,11 @@ void* Ctx::stage(const void* host, size_t bytes, void* dst_after, size_t dst_byt
 
 void Ctx::upload(void* dev, const void* host, size_t bytes, const char* context) {
   if (bytes == 0) return;
+  if (bytes > kStageMaxBytes) {
+    check(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, stream_), context);
+    check(hipStreamSynchronize(stream_), context);
+    return;
+  }
   const size_t off = stage_reserve(bytes, context);
   std::memcpy(h_stage_ + off, host, bytes);
   const gpk_copy_seg seg{dev, d_stage_ + off, static_cast<int64_t>(bytes)};
@@ -538,24 +553,34 @@ double fixed_col_tape(PostsolveTape* tape, const uint8_t* fixed_mask, const uint
   DBuf<uint8_t> sel(n);
   DBuf<int32_t> sel_prefix(static_cast<int64_t>(n) + 1), nnz_prefix(static_cast<int64_t>(n) + 1);
   check(gpk_fixed_col_counts(at, fixed_mask, excluded, keep_row, sel, nnz_prefix, s), "fixed col counts");
-  const int32_t records = scan_mask_count(sel, n, sel_prefix);
+  scan_mask_launch(sel, n, sel_prefix);
+  scan_counts_launch(nnz_prefix, n, nnz_prefix);
+  int32_t records = 0, active = 0;  // both totals in one wait
+  c.download(&records, sel_prefix.get() + n, sizeof(int32_t), "fixed col records");
+  c.download(&active, nnz_prefix.get() + n, sizeof(int32_t), "fixed col entries");
+  c.sync("fixed col counts");
   if (records == 0) return 0.0;
-  const int32_t active = scan_counts(nnz_prefix, n, nnz_prefix);
   DBuf<int32_t> cols(records), rows(active), rstart(records);
   DBuf<double> v(records), cc(records), coef(active);
   check(gpk_fixed_col_pack(at, sel, sel_prefix, nnz_prefix, fixed_val, cvec, keep_row, cols, v, cc, rstart, rows, coef,
                            s),
         "fixed col pack");
-  const std::vector<double> hv = c.to_host(v.get(), records, "fixed col values");
-  const std::vector<double> hc = c.to_host(cc.get(), records, "fixed col costs");
+  // Everything the objective and the tape need, in one wait.
+  std::vector<double> hv, hc, hcoef;
+  std::vector<int32_t> hcol, hrow, starts;
+  c.to_host_async(hv, v.get(), records, "fixed col values");
+  c.to_host_async(hc, cc.get(), records, "fixed col costs");
+  if (tape) {
+    c.to_host_async(hcol, cols.get(), records, "fixed col ids");
+    c.to_host_async(hrow, rows.get(), active, "fixed col rows");
+    c.to_host_async(hcoef, coef.get(), active, "fixed col coefs");
+    c.to_host_async(starts, rstart.get(), records, "fixed col starts");
+  }
+  c.sync("fixed col tape");
   long double obj = 0.0L;
   for (int32_t r = 0; r < records; ++r)
     obj += static_cast<long double>(hc[r]) * static_cast<long double>(hv[r]);
   if (tape) {
-    const std::vector<int32_t> hcol = c.to_host(cols.get(), records, "fixed col ids");
-    const std::vector<int32_t> hrow = c.to_host(rows.get(), active, "fixed col rows");
-    const std::vector<double> hcoef = c.to_host(coef.get(), active, "fixed col coefs");
-    std::vector<int32_t> starts = c.to_host(rstart.get(), records, "fixed col starts");
     starts.push_back(active);
     std::vector<int32_t> idx;
     std::vector<double> vals;

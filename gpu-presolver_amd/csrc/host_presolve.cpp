@@ -273,6 +273,9 @@ void tape_index_kinds(const PostsolveTape& t, std::vector<uint8_t>& kind) {
   }
 }
 
+// Tape chunks up to this size are globalized in the host-mapped staging area.
+constexpr size_t kStagedTapeMaxBytes = size_t{32} << 20;
+
 void append_global_tape(RecordState& rs, const PostsolveTape& src) {
   if (src.types.empty()) return;
   PostsolveTape& dst = rs.rec->tape;
@@ -291,15 +294,27 @@ void append_global_tape(RecordState& rs, const PostsolveTape& src) {
     // synchronization of its own.
     // One staged block: the indices, then their kinds.
     const size_t bytes = sizeof(int32_t) * static_cast<size_t>(cnt);
-    std::vector<char> block(bytes + static_cast<size_t>(cnt));
-    std::memcpy(block.data(), src.indices.data(), bytes);
-    std::memcpy(block.data() + bytes, kind.data(), static_cast<size_t>(cnt));
-    char* d_block = static_cast<char*>(
-        c.stage(block.data(), block.size(), dst.indices.data() + base_idx, bytes, "tape indices"));
-    check(gpk_map_tape_indices(static_cast<int32_t>(cnt), reinterpret_cast<int32_t*>(d_block),
-                               reinterpret_cast<const uint8_t*>(d_block + bytes), rs.row_r2o, rs.m, rs.col_r2o, rs.n,
-                               c.stream()),
-          "tape globalize");
+    if (bytes + static_cast<size_t>(cnt) <= kStagedTapeMaxBytes) {
+      std::vector<char> block(bytes + static_cast<size_t>(cnt));
+      std::memcpy(block.data(), src.indices.data(), bytes);
+      std::memcpy(block.data() + bytes, kind.data(), static_cast<size_t>(cnt));
+      char* d_block = static_cast<char*>(
+          c.stage(block.data(), block.size(), dst.indices.data() + base_idx, bytes, "tape indices"));
+      check(gpk_map_tape_indices(static_cast<int32_t>(cnt), reinterpret_cast<int32_t*>(d_block),
+                                 reinterpret_cast<const uint8_t*>(d_block + bytes), rs.row_r2o, rs.m, rs.col_r2o,
+                                 rs.n, c.stream()),
+            "tape globalize");
+    } else {  // very long tape chunks go through device buffers
+      DBuf<int32_t> d_idx(cnt);
+      DBuf<uint8_t> d_kind(cnt);
+      c.to_device(d_idx.get(), src.indices.data(), cnt, "tape indices upload");
+      c.to_device(d_kind.get(), kind.data(), cnt, "tape kinds upload");
+      check(gpk_map_tape_indices(static_cast<int32_t>(cnt), d_idx, d_kind, rs.row_r2o, rs.m, rs.col_r2o, rs.n,
+                                 c.stream()),
+            "tape globalize");
+      c.download(dst.indices.data() + base_idx, d_idx.get(), bytes, "tape indices download");
+      c.sync("tape indices");
+    }
   }
   dst.types.insert(dst.types.end(), src.types.begin(), src.types.end());
   dst.dual_modes.insert(dst.dual_modes.end(), src.dual_modes.begin(), src.dual_modes.end());

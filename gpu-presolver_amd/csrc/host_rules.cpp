@@ -149,11 +149,15 @@ void apply_rule_singleton_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
   const bool removed = h[1] != 0, changed = h[2] != 0;
   if (!(removed || changed)) return;
   if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu && removed) {
-    const std::vector<uint8_t> hr = c.to_host(remove.get(), m, "singleton tape rows");
-    const std::vector<int32_t> hc = c.to_host(stats.singleton_row_col, m, "singleton tape cols");
-    const std::vector<double> hv = c.to_host(stats.singleton_row_val, m, "singleton tape vals");
-    const std::vector<double> hal = c.to_host(plan.new_AL, m, "singleton tape AL");
-    const std::vector<double> hau = c.to_host(plan.new_AU, m, "singleton tape AU");
+    std::vector<uint8_t> hr;
+    std::vector<int32_t> hc;
+    std::vector<double> hv, hal, hau;
+    c.to_host_async(hr, remove.get(), m, "singleton tape rows");
+    c.to_host_async(hc, stats.singleton_row_col, m, "singleton tape cols");
+    c.to_host_async(hv, stats.singleton_row_val, m, "singleton tape vals");
+    c.to_host_async(hal, plan.new_AL, m, "singleton tape AL");
+    c.to_host_async(hau, plan.new_AU, m, "singleton tape AU");
+    c.sync("singleton tape");
     for (int32_t i = 0; i < m; ++i)
       if (hr[i]) append_record(plan.tape, RT::DeletedRow, {i, hc[i]}, {hal[i], hau[i], hv[i]}, DM::Minimal);
   }
@@ -182,11 +186,14 @@ void apply_rule_infeasible_redundant_rows(PresolvePlanGpu& plan, const LPInfoGpu
   }
   if (!h[1]) return;
   if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
-    const std::vector<uint8_t> hf = c.to_host(full.get(), m, "redundant tape full");
-    const std::vector<uint8_t> hl = c.to_host(dlo.get(), m, "redundant tape lo");
-    const std::vector<uint8_t> hh = c.to_host(dhi.get(), m, "redundant tape hi");
-    const std::vector<double> hal = c.to_host(plan.new_AL, m, "redundant tape AL");
-    const std::vector<double> hau = c.to_host(plan.new_AU, m, "redundant tape AU");
+    std::vector<uint8_t> hf, hl, hh;
+    std::vector<double> hal, hau;
+    c.to_host_async(hf, full.get(), m, "redundant tape full");
+    c.to_host_async(hl, dlo.get(), m, "redundant tape lo");
+    c.to_host_async(hh, dhi.get(), m, "redundant tape hi");
+    c.to_host_async(hal, plan.new_AL, m, "redundant tape AL");
+    c.to_host_async(hau, plan.new_AU, m, "redundant tape AU");
+    c.sync("redundant tape");
     const double inf = HUGE_VAL;
     for (int32_t i = 0; i < m; ++i)
       if (hf[i]) append_record(plan.tape, RT::DeletedRow, {i}, {hal[i], hau[i]}, DM::Minimal);
@@ -359,12 +366,16 @@ void apply_rule_infeasible_fixed_variables(PresolvePlanGpu& plan, const LPInfoGp
   if (c.read_flags(1)[0] == 0) return;
   if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
     const std::vector<int32_t> cols = masked_indices(fixed, n);
-    const std::vector<double> hv = c.to_host(v.get(), n, "fixed vars tape v");
-    const std::vector<double> hc = c.to_host(plan.new_c, n, "fixed vars tape c");
-    const std::vector<uint8_t> kr = c.to_host(plan.keep_row_mask, m, "fixed vars tape keep_row");
-    const std::vector<int32_t> ap = c.to_host(lp.AT.rowPtr, static_cast<int64_t>(n) + 1, "fixed vars tape AT ptr");
-    const std::vector<int32_t> ai = c.to_host(lp.AT.colVal, lp.AT.nnz, "fixed vars tape AT idx");
-    const std::vector<double> av = c.to_host(lp.AT.nzVal, lp.AT.nnz, "fixed vars tape AT val");
+    std::vector<double> hv, hc, av;
+    std::vector<uint8_t> kr;
+    std::vector<int32_t> ap, ai;
+    c.to_host_async(hv, v.get(), n, "fixed vars tape v");
+    c.to_host_async(hc, plan.new_c, n, "fixed vars tape c");
+    c.to_host_async(kr, plan.keep_row_mask, m, "fixed vars tape keep_row");
+    c.to_host_async(ap, lp.AT.rowPtr, static_cast<int64_t>(n) + 1, "fixed vars tape AT ptr");
+    c.to_host_async(ai, lp.AT.colVal, lp.AT.nnz, "fixed vars tape AT idx");
+    c.to_host_async(av, lp.AT.nzVal, lp.AT.nnz, "fixed vars tape AT val");
+    c.sync("fixed vars tape");
     for (int32_t j : cols) {
       std::vector<int32_t> idx{j};
       std::vector<double> vals{hv[j], hc[j]};
@@ -410,8 +421,10 @@ void apply_rule_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Pre
   if (!h[2]) return;
   if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
     const std::vector<int32_t> cols = masked_indices(fixed, n);
-    const std::vector<double> hv = c.to_host(v.get(), n, "empty cols tape v");
-    const std::vector<double> hc = c.to_host(plan.new_c, n, "empty cols tape c");
+    std::vector<double> hv, hc;
+    c.to_host_async(hv, v.get(), n, "empty cols tape v");
+    c.to_host_async(hc, plan.new_c, n, "empty cols tape c");
+    c.sync("empty cols tape");
     for (int32_t j : cols) append_record(plan.tape, RT::FixedCol, {j}, {hv[j], hc[j]}, DM::Minimal);
   }
   check(gpk_fix_cols_apply(n, P, fixed, v, s), "empty cols apply");
@@ -853,17 +866,21 @@ void apply_rule_dual_fix(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Preso
       // FixedColInf: the column, its infinite side and every live row it
       // touches with that row's entries (rule_dual_fix.cu:386-470).
       const std::vector<int32_t> cols = masked_indices(inf_fix, n);
-      const std::vector<double> hv = c.to_host(v.get(), n, "dual fix tape v");
-      const std::vector<double> hl = c.to_host(plan.new_l, n, "dual fix tape l");
-      const std::vector<double> hu = c.to_host(plan.new_u, n, "dual fix tape u");
-      const std::vector<uint8_t> kr = c.to_host(plan.keep_row_mask, m, "dual fix tape keep_row");
-      const std::vector<double> hal = c.to_host(plan.new_AL, m, "dual fix tape AL");
-      const std::vector<double> hau = c.to_host(plan.new_AU, m, "dual fix tape AU");
-      const std::vector<int32_t> tp = c.to_host(lp.AT.rowPtr, static_cast<int64_t>(n) + 1, "dual fix tape AT ptr");
-      const std::vector<int32_t> ti = c.to_host(lp.AT.colVal, lp.AT.nnz, "dual fix tape AT idx");
-      const std::vector<int32_t> ap = c.to_host(lp.A.rowPtr, static_cast<int64_t>(m) + 1, "dual fix tape A ptr");
-      const std::vector<int32_t> ai = c.to_host(lp.A.colVal, lp.A.nnz, "dual fix tape A idx");
-      const std::vector<double> av = c.to_host(lp.A.nzVal, lp.A.nnz, "dual fix tape A val");
+      std::vector<double> hv, hl, hu, hal, hau, av;
+      std::vector<uint8_t> kr;
+      std::vector<int32_t> tp, ti, ap, ai;
+      c.to_host_async(hv, v.get(), n, "dual fix tape v");
+      c.to_host_async(hl, plan.new_l, n, "dual fix tape l");
+      c.to_host_async(hu, plan.new_u, n, "dual fix tape u");
+      c.to_host_async(kr, plan.keep_row_mask, m, "dual fix tape keep_row");
+      c.to_host_async(hal, plan.new_AL, m, "dual fix tape AL");
+      c.to_host_async(hau, plan.new_AU, m, "dual fix tape AU");
+      c.to_host_async(tp, lp.AT.rowPtr, static_cast<int64_t>(n) + 1, "dual fix tape AT ptr");
+      c.to_host_async(ti, lp.AT.colVal, lp.AT.nnz, "dual fix tape AT idx");
+      c.to_host_async(ap, lp.A.rowPtr, static_cast<int64_t>(m) + 1, "dual fix tape A ptr");
+      c.to_host_async(ai, lp.A.colVal, lp.A.nnz, "dual fix tape A idx");
+      c.to_host_async(av, lp.A.nzVal, lp.A.nnz, "dual fix tape A val");
+      c.sync("dual fix tape");
       std::vector<int32_t> idx;
       std::vector<double> vals;
       for (int32_t j : cols) {
@@ -951,8 +968,11 @@ void apply_rule_duplicate_columns(PresolvePlanGpu& plan, const LPInfoGpu& lp, co
   if (!(merged || any_fixed)) return;
   if (merged && prm.record_postsolve_tape) {
     const std::vector<int32_t> deleted = masked_indices(del, n);
-    const std::vector<int32_t> hto = c.to_host(merge_to.get(), n, "duplicate cols merge_to");
-    const std::vector<double> hr = c.to_host(rec5.get(), 5 * static_cast<int64_t>(n), "duplicate cols values");
+    std::vector<int32_t> hto;
+    std::vector<double> hr;
+    c.to_host_async(hto, merge_to.get(), n, "duplicate cols merge_to");
+    c.to_host_async(hr, rec5.get(), 5 * static_cast<int64_t>(n), "duplicate cols values");
+    c.sync("duplicate cols tape");
     for (int32_t k : deleted) {
       const double* r = hr.data() + 5 * static_cast<size_t>(k);
       append_record(plan.tape, RT::DuplicateColumn, {k, hto[k]}, {r[0], r[1], r[2], r[3], r[4]}, DM::Minimal);
@@ -995,10 +1015,13 @@ void apply_rule_redundant_bounds(PresolvePlanGpu& plan, const LPInfoGpu& lp, con
     check(gpk_rb_select(A, AT, dlo, dhi, rkey, sl, su, f, s), "redundant bounds select");
     if (c.read_flags(1)[0] == 0) break;
     if (prm.record_postsolve_tape) {
-      const std::vector<uint8_t> hsl = c.to_host(sl.get(), n, "redundant bounds lo");
-      const std::vector<uint8_t> hsu = c.to_host(su.get(), n, "redundant bounds hi");
-      const std::vector<double> hl = c.to_host(plan.new_l, n, "redundant bounds l");
-      const std::vector<double> hu = c.to_host(plan.new_u, n, "redundant bounds u");
+      std::vector<uint8_t> hsl, hsu;
+      std::vector<double> hl, hu;
+      c.to_host_async(hsl, sl.get(), n, "redundant bounds lo");
+      c.to_host_async(hsu, su.get(), n, "redundant bounds hi");
+      c.to_host_async(hl, plan.new_l, n, "redundant bounds l");
+      c.to_host_async(hu, plan.new_u, n, "redundant bounds u");
+      c.sync("redundant bounds tape");
       for (int32_t j = 0; j < n; ++j) {
         if (!hsl[j] && !hsu[j]) continue;
         const double nl = hsl[j] ? -HUGE_VAL : hl[j];
