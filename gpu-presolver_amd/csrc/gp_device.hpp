@@ -31,13 +31,18 @@ __device__ inline bool wave_leader(uint64_t* active) {
   *active = m;
   return lane_id() == __ffsll(static_cast<unsigned long long>(m)) - 1;
 }
+// OR and max only ever grow the word, so a leader that already sees its bits
+// (or a value at least its own) skips the atomic: when most waves of a large
+// grid set the same flag, they would otherwise queue on one address.
 __device__ inline void wave_or(int32_t* addr, int32_t v) {
   uint64_t m;
-  if (wave_leader(&m)) atomicOr(addr, v);
+  if (wave_leader(&m) && (__hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & v) != v)
+    atomicOr(addr, v);
 }
 __device__ inline void wave_max(int32_t* addr, int32_t v) {
   uint64_t m;
-  if (wave_leader(&m)) atomicMax(addr, v);
+  if (wave_leader(&m) && __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < v)
+    atomicMax(addr, v);
 }
 __device__ inline void wave_add1(int32_t* addr) {
   uint64_t m;
