@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <initializer_list>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -244,6 +245,35 @@ class CopyBatch {
 
  private:
   std::vector<gpk_copy_seg> segs_;
+};
+
+// ---- prepared postsolve ------------------------------------------------------
+// Everything a postsolve of one record needs that does not depend on the
+// reduced point: the validated, uploaded tape and record maps, the two replay
+// schedules (built on the host and uploaded), the protected-column mask and
+// the work buffers.  enqueue() only launches kernels on the context's stream
+// (no allocation, copy or wait), so it can be captured into a graph.
+class PostsolvePlan {
+ public:
+  // Throws std::runtime_error when the record or the original LP cannot be
+  // postsolved (the checks of postsolve_gpu).
+  static void validate(const presolve::PresolveRecordGpu& record, const presolve::LPInfoGpu* original);
+  PostsolvePlan(const presolve::PresolveRecordGpu& record, const presolve::LPInfoGpu* original);
+  ~PostsolvePlan();
+  PostsolvePlan(const PostsolvePlan&) = delete;
+  PostsolvePlan& operator=(const PostsolvePlan&) = delete;
+  // x_red, y_red, z_red may be null (as in postsolve_gpu); outputs are device
+  // arrays of n0, m0 and n0 doubles.
+  void enqueue(const double* x_red, const double* y_red, const double* z_red, double* x_org, double* y_org,
+               double* z_org);
+  int32_t n0() const;
+  int32_t m0() const;
+  int32_t n1() const;
+  int32_t m1() const;
+
+ private:
+  struct Impl;
+  std::unique_ptr<Impl> d_;
 };
 
 // ---- views -----------------------------------------------------------------

@@ -615,22 +615,137 @@ gp_status gp_postsolve(const gp_record_view* record, const double* x_red, const 
     LPInfoGpu orig;
     if (original != nullptr) orig = to_lp(*original);
     require_device();
-    GpuPostsolveResult r = postsolve_gpu(const_cast<double*>(x_red), const_cast<double*>(y_red),
-                                         const_cast<double*>(z_red), rec, original ? &orig : nullptr, nullptr);
-    hipStream_t s = amd::ctx().stream();
-    hipError_t st = hipSuccess;
-    if (rec.n0 > 0) {
-      st = hipMemcpyAsync(x_org, r.x_org, sizeof(double) * rec.n0, hipMemcpyDeviceToDevice, s);
-      if (st == hipSuccess) st = hipMemcpyAsync(z_org, r.z_org, sizeof(double) * rec.n0, hipMemcpyDeviceToDevice, s);
-    }
-    if (st == hipSuccess && rec.m0 > 0)
-      st = hipMemcpyAsync(y_org, r.y_org, sizeof(double) * rec.m0, hipMemcpyDeviceToDevice, s);
-    if (st == hipSuccess) st = hipStreamSynchronize(s);
-    (void)hipFree(r.x_org);
-    (void)hipFree(r.y_org);
-    (void)hipFree(r.z_org);
-    amd::check(st, "gp_postsolve copy-out");
+    amd::PostsolvePlan::validate(rec, original ? &orig : nullptr);
+    if (amd::job_mode().device_sync_on_entry) amd::check(hipDeviceSynchronize(), "gp_postsolve entry synchronize");
+    // Straight into the caller's outputs.
+    amd::PostsolvePlan plan(rec, original ? &orig : nullptr);
+    plan.enqueue(x_red, y_red, z_red, x_org, y_org, z_org);
+    amd::ctx().sync("gp_postsolve");
   });
+}
+
+}  // extern "C"
+
+struct gp_postsolve_plan {
+  PresolveRecordGpu record;
+  LPInfoGpu orig;
+  bool has_orig = false;
+  bool use_graph = false;
+  bool graph_failed = false;
+  std::unique_ptr<amd::PostsolvePlan> plan;
+  amd::DBuf<double> in_x, in_y, in_z, out_x, out_y, out_z;  // the graph's own inputs and outputs
+  hipGraphExec_t exec = nullptr;
+  int64_t graph_runs = 0;
+  ~gp_postsolve_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+  }
+};
+
+namespace {
+
+// Captures the plan's launch sequence over its own buffers; false (and the
+// stream left out of capture) when the runtime refuses.
+bool capture_postsolve(gp_postsolve_plan& p) {
+  const amd::PostsolvePlan& plan = *p.plan;
+  p.in_x.alloc(plan.n1());
+  p.in_y.alloc(plan.m1());
+  p.in_z.alloc(plan.n1());
+  p.out_x.alloc(plan.n0());
+  p.out_y.alloc(plan.m0());
+  p.out_z.alloc(plan.n0());
+  hipStream_t s = amd::ctx().stream();
+  amd::ctx().sync("postsolve graph capture");  // nothing of this stream's earlier work enters the graph
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  bool ok = true;
+  try {
+    p.plan->enqueue(p.in_x, p.in_y, p.in_z, p.out_x, p.out_y, p.out_z);
+  } catch (...) {
+    ok = false;
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t end = hipStreamEndCapture(s, &g);
+  if (!ok || end != hipSuccess || g == nullptr) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return false;
+  }
+  const hipError_t inst = hipGraphInstantiate(&p.exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (inst != hipSuccess) {
+    p.exec = nullptr;
+    (void)hipGetLastError();
+    return false;
+  }
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+gp_status gp_postsolve_prepare(const gp_record_view* record, const gp_lp* original, int32_t use_graph,
+                               gp_postsolve_plan** out) {
+  return guarded([&] {
+    require(record != nullptr && out != nullptr, "null record or output handle");
+    *out = nullptr;
+    auto p = std::make_unique<gp_postsolve_plan>();
+    p->record = to_record(*record);
+    if (original != nullptr) {
+      p->orig = to_lp(*original);
+      p->has_orig = true;
+    }
+    p->use_graph = use_graph != 0;
+    require_device();
+    amd::PostsolvePlan::validate(p->record, p->has_orig ? &p->orig : nullptr);
+    if (amd::job_mode().device_sync_on_entry) amd::check(hipDeviceSynchronize(), "gp_postsolve_prepare entry");
+    p->plan = std::make_unique<amd::PostsolvePlan>(p->record, p->has_orig ? &p->orig : nullptr);
+    amd::ctx().sync("gp_postsolve_prepare");
+    *out = p.release();
+  });
+}
+
+gp_status gp_postsolve_plan_run(gp_postsolve_plan* p, const double* x_red, const double* y_red, const double* z_red,
+                                double* x_org, double* y_org, double* z_org) {
+  return guarded([&] {
+    require(p != nullptr && p->plan, "null postsolve plan");
+    amd::PostsolvePlan& plan = *p->plan;
+    require(plan.n0() == 0 || (x_org && z_org), "null x_org/z_org");
+    require(plan.m0() == 0 || y_org, "null y_org");
+    require_device();
+    if (amd::job_mode().device_sync_on_entry) amd::check(hipDeviceSynchronize(), "gp_postsolve_plan_run entry");
+    const bool all_inputs = x_red && (y_red || plan.m1() == 0) && z_red;
+    if (p->use_graph && all_inputs && !p->graph_failed && p->exec == nullptr && !capture_postsolve(*p))
+      p->graph_failed = true;
+    if (p->use_graph && all_inputs && p->exec != nullptr) {
+      hipStream_t s = amd::ctx().stream();
+      amd::CopyBatch()
+          .copy(p->in_x.get(), x_red, plan.n1())
+          .copy(p->in_y.get(), y_red, plan.m1())
+          .copy(p->in_z.get(), z_red, plan.n1())
+          .run("postsolve graph inputs");
+      amd::check(hipGraphLaunch(p->exec, s), "postsolve graph launch");
+      amd::CopyBatch()
+          .copy(x_org, p->out_x.get(), plan.n0())
+          .copy(y_org, p->out_y.get(), plan.m0())
+          .copy(z_org, p->out_z.get(), plan.n0())
+          .run("postsolve graph outputs");
+      ++p->graph_runs;
+    } else {
+      plan.enqueue(x_red, y_red, z_red, x_org, y_org, z_org);
+    }
+    amd::ctx().sync("gp_postsolve_plan_run");
+  });
+}
+
+int64_t gp_postsolve_plan_graph_runs(const gp_postsolve_plan* p) { return p ? p->graph_runs : 0; }
+
+void gp_postsolve_plan_free(gp_postsolve_plan* p) {
+  if (p == nullptr) return;
+  (void)guarded([&] { amd::ctx().sync("gp_postsolve_plan_free"); });
+  delete p;
 }
 
 }  // extern "C"

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -373,17 +374,22 @@ Schedule level_schedule(const TapeScanner& scan, int32_t count, int32_t n0, int3
   return sch;
 }
 
-void run_schedule(const Schedule& sch, const gp_tape_dev& tape, const gp_orig_dev& orig, double* x, double* y,
-                  double* z, uint8_t* zret, bool primal, double tol) {
-  if (sch.order.empty()) return;
-  auto& c = ctx();
-  hipStream_t s = c.stream();
+// The schedule's record order and level bounds in one device block.
+DBuf<int32_t> upload_schedule(const Schedule& sch) {
   const int64_t nr = static_cast<int64_t>(sch.order.size()), nl = static_cast<int64_t>(sch.level_start.size());
   DBuf<int32_t> buf(nr + nl);
-  c.to_device(buf.get(), sch.order.data(), nr, "replay schedule upload");
-  c.to_device(buf.get() + nr, sch.level_start.data(), nl, "replay schedule upload");
-  const int32_t* order = buf.get();
-  const int32_t* starts = buf.get() + nr;
+  ctx().to_device(buf.get(), sch.order.data(), nr, "replay schedule upload");
+  ctx().to_device(buf.get() + nr, sch.level_start.data(), nl, "replay schedule upload");
+  return buf;
+}
+
+// Launches only: no allocation, copy or wait, so it can be captured.
+void enqueue_schedule(const Schedule& sch, const int32_t* buf, const gp_tape_dev& tape, const gp_orig_dev& orig,
+                      double* x, double* y, double* z, uint8_t* zret, bool primal, double tol) {
+  if (sch.order.empty()) return;
+  hipStream_t s = ctx().stream();
+  const int32_t* order = buf;
+  const int32_t* starts = buf + sch.order.size();
   const int32_t p = primal ? 1 : 0;
   const char* what = primal ? "replay primal" : "replay dual";
   for (const Segment& g : sch.segs) {
@@ -406,6 +412,131 @@ void upload_map(DBuf<int32_t>& dst, const std::vector<int32_t>& src, int32_t bou
 
 }  // namespace
 
+}  // namespace gpu_presolver::presolve
+
+namespace gpu_presolver::amd {
+
+using presolve::kPostsolveTol;
+using presolve::LPInfoGpu;
+using presolve::PresolveRecordGpu;
+
+void PostsolvePlan::validate(const PresolveRecordGpu& record, const LPInfoGpu* original) {
+  if (!record.primal_recovery_timeline.empty() || !record.structural_primal_recoveries.empty() ||
+      !record.antipodal_component_recoveries.empty())
+    throw std::runtime_error(
+        "postsolve_gpu: the record holds structure-specific recovery steps, which this library does not implement");
+  if (static_cast<int32_t>(record.col_red2org.size()) != record.n1 ||
+      static_cast<int32_t>(record.row_red2org.size()) != record.m1)
+    throw std::runtime_error("postsolve_gpu: record maps do not match m1/n1");
+  if (record.fixed_idx.size() != record.fixed_val.size())
+    throw std::runtime_error("postsolve_gpu: fixed_idx and fixed_val differ in length");
+  if (original != nullptr && (original->A.rows != record.m0 || original->A.cols != record.n0))
+    throw std::runtime_error("postsolve_gpu: original model does not match the record's dimensions");
+}
+
+struct PostsolvePlan::Impl {
+  int32_t n0 = 0, m0 = 0, n1 = 0, m1 = 0, nf = 0;
+  bool project = false;
+  gp_orig_dev orig{};
+  std::unique_ptr<presolve::DeviceTape> tape;
+  DBuf<int32_t> col_map, row_map, fidx;
+  DBuf<double> fval;
+  DBuf<uint8_t> zret, prot;
+  DBuf<int32_t> long_cols, long_count;
+  presolve::Schedule primal, dual;
+  DBuf<int32_t> primal_buf, dual_buf;
+};
+
+PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* original) : d_(new Impl) {
+  validate(record, original);
+  Impl& d = *d_;
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  d.n0 = record.n0;
+  d.m0 = record.m0;
+  d.n1 = record.n1;
+  d.m1 = record.m1;
+  // GP_PRESOLVE_PROFILE=1: host time of the preparation steps on stderr.
+  const bool prof = std::getenv("GP_PRESOLVE_PROFILE") != nullptr;
+  auto tick = std::chrono::steady_clock::now();
+  auto lap = [&](const char* what) {
+    if (!prof) return;
+    const auto now = std::chrono::steady_clock::now();
+    std::fprintf(stderr, "  postsolve %-28s %8.3f ms\n", what, 1e3 * std::chrono::duration<double>(now - tick).count());
+    tick = now;
+  };
+  d.zret.alloc(d.n0);
+  presolve::upload_map(d.col_map, record.col_red2org, d.n0, "col_red2org upload");
+  presolve::upload_map(d.row_map, record.row_red2org, d.m0, "row_red2org upload");
+  presolve::upload_map(d.fidx, record.fixed_idx, d.n0, "fixed_idx upload");
+  d.nf = static_cast<int32_t>(record.fixed_val.size());
+  d.fval.alloc(d.nf);
+  c.to_device(d.fval.get(), record.fixed_val.data(), d.nf, "fixed_val upload");
+  d.tape = std::make_unique<presolve::DeviceTape>(record.tape);
+  d.orig = presolve::orig_view(original);
+  const presolve::TapeScanner scanner(record.tape, d.n0, d.m0);
+  const bool has_orig = original != nullptr;
+  const bool has_y = d.m0 > 0;
+  presolve::OrigPattern pattern;
+  if (has_orig) {
+    std::vector<int32_t> rows, cols;
+    scanner.pattern_lines(rows, cols);
+    pattern.a_rows = presolve::gather_lines(original->A.rowPtr, original->A.colVal, d.m0, rows);
+    pattern.at_cols = presolve::gather_lines(original->AT.rowPtr, original->AT.colVal, d.n0, cols);
+  }
+  lap("tape upload, scan, lines");
+  const int32_t count = d.tape->view.count;
+  d.primal = presolve::level_schedule(scanner, count, d.n0, d.m0, true, has_orig, has_y, &pattern);
+  d.primal_buf = presolve::upload_schedule(d.primal);
+  lap("primal schedule");
+  d.dual = presolve::level_schedule(scanner, count, d.n0, d.m0, false, has_orig, has_y, &pattern);
+  d.dual_buf = presolve::upload_schedule(d.dual);
+  lap("dual schedule");
+  // project_column_duals_from_original (gpu_postsolve.cu:1370-1435): the
+  // columns the tape recovers duals for keep them.
+  d.project = original != nullptr && has_y && original->AT.rows > 0 && original->c && original->l && original->u;
+  if (d.project) {
+    d.prot.alloc(d.n0);
+    CopyBatch().zero(d.prot.get(), d.n0).run("protected columns clear");
+    check(gpk_protect_columns(d.tape->view, d.n0, d.prot, s), "protected columns");
+    d.long_cols.alloc(d.n0);
+    d.long_count.alloc(1);
+  }
+}
+
+PostsolvePlan::~PostsolvePlan() = default;
+int32_t PostsolvePlan::n0() const { return d_->n0; }
+int32_t PostsolvePlan::m0() const { return d_->m0; }
+int32_t PostsolvePlan::n1() const { return d_->n1; }
+int32_t PostsolvePlan::m1() const { return d_->m1; }
+
+void PostsolvePlan::enqueue(const double* x_red, const double* y_red, const double* z_red, double* x_org,
+                            double* y_org, double* z_org) {
+  Impl& d = *d_;
+  hipStream_t s = ctx().stream();
+  double* y = d.m0 > 0 ? y_org : nullptr;
+  CopyBatch batch;
+  batch.zero(x_org, d.n0).zero(z_org, d.n0).zero(d.zret.get(), d.n0).zero(y_org, d.m0);
+  if (d.project) batch.zero(d.long_count.get(), 1);
+  batch.run("postsolve output clear");
+  if (x_red) check(gpk_scatter(x_red, d.col_map, d.n1, x_org, s), "scatter x");
+  if (y_red && d.m0 > 0) check(gpk_scatter(y_red, d.row_map, d.m1, y_org, s), "scatter y");
+  if (z_red) check(gpk_scatter(z_red, d.col_map, d.n1, z_org, s), "scatter z");
+  check(gpk_mark(d.col_map, d.n1, d.zret, s), "mark retrieved");
+  check(gpk_restore_fixed(d.fidx, d.fval, d.nf, x_org, s), "restore fixed");
+  presolve::enqueue_schedule(d.primal, d.primal_buf, d.tape->view, d.orig, x_org, y, z_org, d.zret, true,
+                             kPostsolveTol);
+  presolve::enqueue_schedule(d.dual, d.dual_buf, d.tape->view, d.orig, x_org, y, z_org, d.zret, false,
+                             kPostsolveTol);
+  if (d.project)
+    check(gpk_project_duals(d.orig, d.prot, x_org, y, z_org, d.n0, kPostsolveTol, d.long_count, d.long_cols, s),
+          "project duals");
+}
+
+}  // namespace gpu_presolver::amd
+
+namespace gpu_presolver::presolve {
+
 void postsolve_restore_structural_primal_gpu(double* x_org, const std::vector<StructuralL1PrimalRecoveryStep>& recoveries,
                                              const double* original_l) {
   (void)x_org;
@@ -425,18 +556,7 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
 GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, const PresolveRecordGpu& record,
                                  const LPInfoGpu* original_model, const double* original_l) {
   (void)original_l;
-  if (!record.primal_recovery_timeline.empty() || !record.structural_primal_recoveries.empty() ||
-      !record.antipodal_component_recoveries.empty())
-    throw std::runtime_error(
-        "postsolve_gpu: the record holds structure-specific recovery steps, which this library does not implement");
-  if (static_cast<int32_t>(record.col_red2org.size()) != record.n1 ||
-      static_cast<int32_t>(record.row_red2org.size()) != record.m1)
-    throw std::runtime_error("postsolve_gpu: record maps do not match m1/n1");
-  if (record.fixed_idx.size() != record.fixed_val.size())
-    throw std::runtime_error("postsolve_gpu: fixed_idx and fixed_val differ in length");
-  if (original_model != nullptr &&
-      (original_model->A.rows != record.m0 || original_model->A.cols != record.n0))
-    throw std::runtime_error("postsolve_gpu: original model does not match the record's dimensions");
+  amd::PostsolvePlan::validate(record, original_model);
   if (amd::job_mode().device_sync_on_entry) check(hipDeviceSynchronize(), "postsolve entry synchronize");
   auto& c = ctx();
   hipStream_t s = c.stream();
@@ -448,73 +568,9 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
     res.x_org = malloc_f64(n0, "hipMalloc postsolve x_org");
     res.z_org = malloc_f64(n0, "hipMalloc postsolve z_org");
     res.y_org = malloc_f64(m0, "hipMalloc postsolve y_org");
-    DBuf<uint8_t> zret(n0);
-    DBuf<int32_t> col_map, row_map, fidx;
-    DBuf<double> fval;
-    upload_map(col_map, record.col_red2org, n0, "col_red2org upload");
-    upload_map(row_map, record.row_red2org, m0, "row_red2org upload");
-    upload_map(fidx, record.fixed_idx, n0, "fixed_idx upload");
-    const int32_t nf = static_cast<int32_t>(record.fixed_val.size());
-    fval.alloc(nf);
-    c.to_device(fval.get(), record.fixed_val.data(), nf, "fixed_val upload");
-    amd::CopyBatch().zero(res.x_org, n0).zero(res.z_org, n0).zero(zret.get(), n0).zero(res.y_org, m0).run(
-        "postsolve output clear");
-    if (x_red) check(gpk_scatter(x_red, col_map, record.n1, res.x_org, s), "scatter x");
-    if (y_red && m0 > 0) check(gpk_scatter(y_red, row_map, record.m1, res.y_org, s), "scatter y");
-    if (z_red) check(gpk_scatter(z_red, col_map, record.n1, res.z_org, s), "scatter z");
-    check(gpk_mark(col_map, record.n1, zret, s), "mark retrieved");
-    check(gpk_restore_fixed(fidx, fval, nf, res.x_org, s), "restore fixed");
-    // GP_PRESOLVE_PROFILE=1: host time of the postsolve steps on stderr.
-    const bool prof = std::getenv("GP_PRESOLVE_PROFILE") != nullptr;
-    auto tick = std::chrono::steady_clock::now();
-    auto lap = [&](const char* what) {
-      if (!prof) return;
-      const auto now = std::chrono::steady_clock::now();
-      std::fprintf(stderr, "  postsolve %-28s %8.3f ms\n", what,
-                   1e3 * std::chrono::duration<double>(now - tick).count());
-      tick = now;
-    };
-    DeviceTape tape(record.tape);
-    const gp_orig_dev orig = orig_view(original_model);
-    double* y = m0 > 0 ? res.y_org : nullptr;
-    const TapeScanner scanner(record.tape, n0, m0);
-    const bool has_orig = original_model != nullptr;
-    const bool has_y = y != nullptr;
-    OrigPattern pattern;
-    if (has_orig) {
-      const LPInfoGpu& o = *original_model;
-      std::vector<int32_t> rows, cols;
-      scanner.pattern_lines(rows, cols);
-      pattern.a_rows = gather_lines(o.A.rowPtr, o.A.colVal, m0, rows);
-      pattern.at_cols = gather_lines(o.AT.rowPtr, o.AT.colVal, n0, cols);
-    }
-    lap("tape upload, scan, lines");
-    const int32_t count = tape.view.count;
-    const Schedule primal_sch = level_schedule(scanner, count, n0, m0, true, has_orig, has_y, &pattern);
-    lap("primal schedule");
-    run_schedule(primal_sch, tape.view, orig, res.x_org, y, res.z_org, zret, true, kPostsolveTol);
-    const Schedule dual_sch = level_schedule(scanner, count, n0, m0, false, has_orig, has_y, &pattern);
-    lap("dual schedule");
-    run_schedule(dual_sch, tape.view, orig, res.x_org, y, res.z_org, zret, false, kPostsolveTol);
-    if (prof) {
-      c.sync("postsolve profile");
-      lap("replay (wait)");
-      std::fprintf(stderr, "  postsolve levels: primal %zu in %zu launches, dual %zu in %zu launches\n",
-                   primal_sch.level_start.size() - 1, primal_sch.segs.size(), dual_sch.level_start.size() - 1,
-                   dual_sch.segs.size());
-    }
-    // project_column_duals_from_original (gpu_postsolve.cu:1370-1435)
-    if (original_model != nullptr && y != nullptr && original_model->AT.rows > 0 && original_model->c &&
-        original_model->l && original_model->u) {
-      DBuf<uint8_t> prot(n0);
-      check(hipMemsetAsync(prot.get(), 0, n0, s), "protected columns clear");
-      check(gpk_protect_columns(tape.view, n0, prot, s), "protected columns");
-      DBuf<int32_t> long_cols(n0);
-      check(gpk_project_duals(orig, prot, res.x_org, y, res.z_org, n0, kPostsolveTol, c.flags(), long_cols, s),
-            "project duals");
-    }
+    amd::PostsolvePlan plan(record, original_model);
+    plan.enqueue(x_red, y_red, z_red, res.x_org, res.y_org, res.z_org);
     c.sync("postsolve_gpu synchronize");
-    lap("projection (wait)");
   } catch (...) {
     (void)hipStreamSynchronize(s);
     amd::dev_free(res.x_org, s);

@@ -391,6 +391,11 @@ class PreparedPostsolve:
         _check(lib().gp_postsolve(C.byref(self.rk.view), *[d.ptr for d in self.ins], C.byref(self.orig.g),
                                   dx.ptr, dy.ptr, dz.ptr))
 
+    def run_plan(self, plan):
+        """The same postsolve through a PostsolvePlan of this record."""
+        dx, dy, dz = self.out
+        _check(lib().gp_postsolve_plan_run(plan.h, *[d.ptr for d in self.ins], dx.ptr, dy.ptr, dz.ptr))
+
 
 def postsolve(rec, x_red, y_red, z_red, original=None, original_dev=None):
     """original: LP dict (uploaded here) or original_dev: a resident DeviceLp."""
@@ -402,6 +407,48 @@ def postsolve(rec, x_red, y_red, z_red, original=None, original_dev=None):
     _check(lib().gp_postsolve(C.byref(rk.view), *[d.ptr if d is not None else None for d in ins],
                               C.byref(orig.g) if orig is not None else None, dx.ptr, dy.ptr, dz.ptr))
     return dx.numpy(), dy.numpy(), dz.numpy()
+
+
+class PostsolvePlan:
+    """gp_postsolve_prepare / gp_postsolve_plan_run: the record's tape, maps
+    and replay schedules built once, any number of reduced points postsolved
+    with them (replayed from a captured hipGraph when use_graph is set and
+    x, y, z are all given)."""
+
+    def __init__(self, rec, original=None, original_dev=None, use_graph=True):
+        L = lib()
+        L.gp_postsolve_prepare.argtypes = [C.c_void_p, C.c_void_p, C.c_int32, C.POINTER(C.c_void_p)]
+        L.gp_postsolve_plan_run.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.gp_postsolve_plan_graph_runs.restype = C.c_int64
+        L.gp_postsolve_plan_graph_runs.argtypes = [C.c_void_p]
+        L.gp_postsolve_plan_free.argtypes = [C.c_void_p]
+        self._rk = T.RecordKeepAlive(rec)  # the record view must outlive the plan
+        self._orig = original_dev if original_dev is not None else (
+            DeviceLp(original) if original is not None else None)
+        self.n0, self.m0 = rec["n0"], rec["m0"]
+        self.h = C.c_void_p()
+        _check(L.gp_postsolve_prepare(C.addressof(self._rk.view),
+                                      C.addressof(self._orig.g) if self._orig is not None else None,
+                                      1 if use_graph else 0, C.byref(self.h)))
+
+    def run(self, x_red, y_red, z_red):
+        dx, dy, dz = DeviceArray(size=self.n0), DeviceArray(size=self.m0), DeviceArray(size=self.n0)
+        ins = [DeviceArray(np.asarray(v, np.float64)) if v is not None else None for v in (x_red, y_red, z_red)]
+        _check(lib().gp_postsolve_plan_run(self.h, *[d.ptr if d is not None else None for d in ins],
+                                           dx.ptr, dy.ptr, dz.ptr))
+        return dx.numpy(), dy.numpy(), dz.numpy()
+
+    def graph_runs(self):
+        return lib().gp_postsolve_plan_graph_runs(self.h)
+
+    def free(self):
+        if self.h:
+            lib().gp_postsolve_plan_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            self.free()
 
 
 # ---- timing ----------------------------------------------------------------

@@ -264,6 +264,25 @@ gp_status gp_postsolve(const gp_record_view* record, const double* x_red, const 
                        const double* z_red, const gp_lp* original, double* x_org, double* y_org,
                        double* z_org);
 
+/* ---- prepared postsolve (no counterpart in the reference) ---------------
+ * gp_postsolve_prepare builds what a postsolve of one record needs apart from
+ * the reduced point (device tape and maps, both replay schedules, the
+ * protected-column mask); gp_postsolve_plan_run then postsolves any reduced
+ * point of that record exactly as gp_postsolve does, without rebuilding them.
+ * With use_graph != 0 and all three of x_red, y_red, z_red given, the run's
+ * launches are captured into a hipGraph on first use and replayed afterwards
+ * (the inputs and outputs pass through buffers the plan owns).  The record
+ * is copied; the original LP must stay valid while the plan exists.  A plan
+ * runs one postsolve at a time. */
+typedef struct gp_postsolve_plan gp_postsolve_plan;
+gp_status gp_postsolve_prepare(const gp_record_view* record, const gp_lp* original, int32_t use_graph,
+                               gp_postsolve_plan** out);
+gp_status gp_postsolve_plan_run(gp_postsolve_plan* plan, const double* x_red, const double* y_red,
+                                const double* z_red, double* x_org, double* y_org, double* z_org);
+/* Number of runs so far that replayed the captured graph. */
+int64_t gp_postsolve_plan_graph_runs(const gp_postsolve_plan* plan);
+void gp_postsolve_plan_free(gp_postsolve_plan* plan);
+
 /* ---- serving and batching (no counterpart in the reference) ------------
  * The reference presolves one LP per call on the legacy stream.  The serving
  * runtime keeps several jobs in flight, each on its own stream and execution
