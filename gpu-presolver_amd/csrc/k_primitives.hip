@@ -943,7 +943,7 @@ __global__ void k_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t 
   if (i >= rows) return;
   for (int32_t p = ptr[i] + 1; p < ptr[i + 1]; ++p) {
     if (idx[p] < idx[p - 1]) {
-      wave_or(flag, 1);
+      wave_set_flag(flag);
       return;
     }
   }

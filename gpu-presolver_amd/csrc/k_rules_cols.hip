@@ -45,11 +45,11 @@ __global__ void k_empty_cols(int32_t n, gpk_plan p, const uint8_t* empty, double
   if (!(p.keep_col[j] && empty[j])) return;
   const double cj = p.c[j], lj = p.l[j], uj = p.u[j];
   if (lj > uj + btol) {
-    wave_or(&flags[0], 1);
+    wave_set_flag(&flags[0]);
     return;
   }
   if ((cj > ztol && !isfinite(lj)) || (cj < -ztol && !isfinite(uj))) {
-    wave_or(&flags[1], 1);
+    wave_set_flag(&flags[1]);
     return;
   }
   double val = 0.0;
@@ -60,7 +60,7 @@ __global__ void k_empty_cols(int32_t n, gpk_plan p, const uint8_t* empty, double
   fixed[j] = 1;
   v[j] = val;
   contrib[j] = cj * val;
-  wave_or(&flags[2], 1);
+  wave_set_flag(&flags[2]);
 }
 
 __global__ void k_fix_cols_apply(int32_t n, gpk_plan p, const uint8_t* fixed, const double* v) {
@@ -235,7 +235,7 @@ __global__ void k_cs_owner(gpk_csr A, gpk_plan p, const uint8_t* mask, const int
   if (live <= 1) return;
   const double lhs = p.AL[row], rhs = p.AU[row], cj = p.c[j];
   if (!eq_mode && cs_direct_unbounded(cj, a, lhs, rhs, p.l[j], p.u[j], zt)) {
-    wave_or(&flags[0], 1);
+    wave_set_flag(&flags[0]);
     return;
   }
   bool is_eq, above, below;
@@ -258,7 +258,7 @@ __global__ void k_cs_process(int32_t n, int32_t m, gpk_plan p, const uint8_t* ma
   if (fabs(a) <= zt) return;
   const double lhs = AL_new[row], rhs = AU_new[row], cj = p.c[j];
   if (!eq_mode && cs_direct_unbounded(cj, a, lhs, rhs, p.l[j], p.u[j], zt)) {
-    wave_or(&flags[0], 1);
+    wave_set_flag(&flags[0]);
     return;
   }
   bool is_eq, above, below;
@@ -282,7 +282,7 @@ __global__ void k_cs_process(int32_t n, int32_t m, gpk_plan p, const uint8_t* ma
       }
     }
     col_del[j] = 1;
-    wave_or(&flags[1], 1);
+    wave_set_flag(&flags[1]);
     return;
   }
   double side = 0.0;
@@ -299,7 +299,7 @@ __global__ void k_cs_process(int32_t n, int32_t m, gpk_plan p, const uint8_t* ma
     row_changed[row] = 1;
     AU_new[row] = lhs;
   }
-  wave_or(&flags[2], 1);
+  wave_set_flag(&flags[2]);
 }
 
 // rule_column_singletons.cu:1063-1091 (AT order, no atomics).
@@ -443,8 +443,8 @@ __global__ void k_dual_fix_candidates(gpk_csr AT, gpk_plan p, const uint8_t* can
     finite_fix[j] = infx ? 0 : 1;
     v[j] = vj;
     contrib[j] = infx ? 0.0 : cj * vj;
-    wave_or(&flags[1], 1);
-    if (infx) wave_or(&flags[2], 1);
+    wave_set_flag(&flags[1]);
+    if (infx) wave_set_flag(&flags[2]);
   }
 }
 
@@ -465,7 +465,7 @@ __global__ void k_dual_fix_rows(gpk_csr A, gpk_plan p, const uint8_t* finite_fix
   }
   shift[i] = s;
   remove[i] = rm;
-  if (rm) wave_or(&flags[3], 1);
+  if (rm) wave_set_flag(&flags[3]);
 }
 
 __global__ void k_dual_fix_apply(int32_t m, int32_t n, gpk_plan p, const uint8_t* fixed, const double* v,
@@ -637,7 +637,7 @@ __global__ void k_dup_col_groups(gpk_csr AT, gpk_plan p, const uint64_t* sh, con
         p.l[j] = nl;
         p.u[j] = nu;
         col_del[k] = 1;
-        wave_or(&flags[1], 1);
+        wave_set_flag(&flags[1]);
         continue;
       }
       bool k_lo = false, k_hi = false, j_lo = false, j_hi = false;
@@ -649,18 +649,18 @@ __global__ void k_dup_col_groups(gpk_csr AT, gpk_plan p, const uint64_t* sh, con
         else { k_hi = !isfinite(tu); j_hi = !isfinite(su); }
       }
       if (k_lo) {
-        if (!isfinite(sl)) { wave_or(&flags[0], 1); return; }
-        fixed[k] = 1; fixed_val[k] = sl; wave_or(&flags[2], 1); continue;
+        if (!isfinite(sl)) { wave_set_flag(&flags[0]); return; }
+        fixed[k] = 1; fixed_val[k] = sl; wave_set_flag(&flags[2]); continue;
       } else if (k_hi) {
-        if (!isfinite(su)) { wave_or(&flags[0], 1); return; }
-        fixed[k] = 1; fixed_val[k] = su; wave_or(&flags[2], 1); continue;
+        if (!isfinite(su)) { wave_set_flag(&flags[0]); return; }
+        fixed[k] = 1; fixed_val[k] = su; wave_set_flag(&flags[2]); continue;
       }
       if (j_lo) {
-        if (!isfinite(tl)) { wave_or(&flags[0], 1); return; }
-        fixed[j] = 1; fixed_val[j] = tl; wave_or(&flags[2], 1); break;
+        if (!isfinite(tl)) { wave_set_flag(&flags[0]); return; }
+        fixed[j] = 1; fixed_val[j] = tl; wave_set_flag(&flags[2]); break;
       } else if (j_hi) {
-        if (!isfinite(tu)) { wave_or(&flags[0], 1); return; }
-        fixed[j] = 1; fixed_val[j] = tu; wave_or(&flags[2], 1); break;
+        if (!isfinite(tu)) { wave_set_flag(&flags[0]); return; }
+        fixed[j] = 1; fixed_val[j] = tu; wave_set_flag(&flags[2]); break;
       }
     }
   }
@@ -861,7 +861,7 @@ __global__ void k_dbl_apply(int32_t k, const int32_t* sel, gpk_plan p, gpk_dbl d
   const double nkl = isfinite(ml) ? fmax(ol_k, ml) : ol_k;
   const double nku = isfinite(mu) ? fmin(ou_k, mu) : ou_k;
   if (nkl > nku + tol) {
-    wave_or(&flags[0], 1);
+    wave_set_flag(&flags[0]);
     return;
   }
   const bool keep_fixed = isfinite(nkl) && isfinite(nku) && nku <= nkl + tol;
@@ -1122,7 +1122,7 @@ __global__ void k_rb_select(gpk_csr AT, const uint8_t* dlo, const uint8_t* dhi, 
     if (rkey[AT.idx[q]] != key) return;
   sl[j] = dlo[j];
   su[j] = dhi[j];
-  wave_or(&flags[0], 1);
+  wave_set_flag(&flags[0]);
 }
 
 __global__ void k_rb_apply(int32_t n, gpk_plan p, const uint8_t* sl, const uint8_t* su) {

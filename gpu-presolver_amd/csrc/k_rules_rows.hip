@@ -97,9 +97,9 @@ __global__ void k_empty_rows(int32_t m, gpk_plan p, const int32_t* row_nnz, doub
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m || !p.keep_row[i] || row_nnz[i] != 0) return;
   if (p.AL[i] <= tol && p.AU[i] >= -tol) {
-    if (apply) p.keep_row[i] = 0; else wave_or(&flags[1], 1);
+    if (apply) p.keep_row[i] = 0; else wave_set_flag(&flags[1]);
   } else if (!apply) {
-    wave_or(&flags[0], 1);
+    wave_set_flag(&flags[0]);
   }
 }
 
@@ -124,10 +124,10 @@ __global__ void k_singleton_rows_check(int32_t m, int32_t n, gpk_plan p, const d
                                        const uint8_t* remove, double tol, int32_t* flags) {
   const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < n) {
-    if (cl[t] > cu[t] + tol) wave_or(&flags[0], 1);
-    if (cl[t] != p.l[t] || cu[t] != p.u[t]) wave_or(&flags[2], 1);
+    if (cl[t] > cu[t] + tol) wave_set_flag(&flags[0]);
+    if (cl[t] != p.l[t] || cu[t] != p.u[t]) wave_set_flag(&flags[2]);
   }
-  if (t < m && remove[t]) wave_or(&flags[1], 1);
+  if (t < m && remove[t]) wave_set_flag(&flags[1]);
 }
 
 __global__ void k_singleton_rows_commit(int32_t m, int32_t n, gpk_plan p, const double* cl, const double* cu,
@@ -166,20 +166,20 @@ __global__ void k_redundant_rows_classify(gpk_csr A, gpk_plan p, const int32_t* 
     }
   }
   if ((lf && rmax < lo - tol) || (hf && rmin > hi + tol)) {
-    wave_or(&flags[0], 1);
+    wave_set_flag(&flags[0]);
     return;
   }
   const bool lo_imp = !lf || rmin >= lo - tol;
   const bool hi_imp = !hf || rmax <= hi + tol;
   if (lo_imp && hi_imp) {
     full[i] = 1;
-    wave_or(&flags[1], 1);
+    wave_set_flag(&flags[1]);
   } else if (lf && lo_imp) {
     dlo[i] = 1;
-    wave_or(&flags[1], 1);
+    wave_set_flag(&flags[1]);
   } else if (hf && hi_imp) {
     dhi[i] = 1;
-    wave_or(&flags[1], 1);
+    wave_set_flag(&flags[1]);
   }
 }
 
@@ -273,7 +273,7 @@ __global__ void k_implied_finalize(int32_t n, gpk_plan p, const double* ref_l, c
   if (isfinite(cl)) el = fmax(el, cl);
   if (isfinite(cu)) eu = fmin(eu, cu);
   if (el > eu + ftol) {
-    wave_or(&flags[0], 1);
+    wave_set_flag(&flags[0]);
     return;
   }
   const bool was_free = !isfinite(ref_l[j]) && !isfinite(ref_u[j]);
@@ -286,7 +286,7 @@ __global__ void k_implied_finalize(int32_t n, gpk_plan p, const double* ref_l, c
   }
   if (isfinite(cl) && cl > lj && isfinite(uj)) {
     if (cl >= uj + ftol) {
-      wave_or(&flags[0], 1);
+      wave_set_flag(&flags[0]);
       return;
     }
     if (cl >= uj || (uj - cl) * vm <= ftol) {
@@ -306,7 +306,7 @@ __global__ void k_implied_finalize(int32_t n, gpk_plan p, const double* ref_l, c
   }
   if (!fx && isfinite(cu) && cu < uj && isfinite(nl)) {
     if (cu <= nl - ftol) {
-      wave_or(&flags[0], 1);
+      wave_set_flag(&flags[0]);
       return;
     }
     if (cu <= nl || (cu - nl) * vm <= ftol) {
@@ -326,9 +326,9 @@ __global__ void k_implied_finalize(int32_t n, gpk_plan p, const double* ref_l, c
   if (fx) {
     lc = 0;
     uc = 0;
-    wave_or(&flags[2], 1);
+    wave_set_flag(&flags[2]);
   }
-  if (lc || uc) wave_or(&flags[1], 1);
+  if (lc || uc) wave_set_flag(&flags[1]);
   fl[j] = nl;
   fu[j] = nu;
   lch[j] = lc;
@@ -415,9 +415,9 @@ __global__ void k_remove_empty_rows_after_fix(gpk_csr A, gpk_plan p, double ftol
   if (cnt != 0) return;
   if (p.AL[i] <= ftol && p.AU[i] >= -ftol) {
     p.keep_row[i] = 0;
-    wave_or(&flags[4], 1);
+    wave_set_flag(&flags[4]);
   } else {
-    wave_or(&flags[3], 1);
+    wave_set_flag(&flags[3]);
   }
 }
 
@@ -570,7 +570,7 @@ __global__ void k_dup_row_groups(gpk_csr A, gpk_plan p, const uint64_t* sh, cons
     double sl, su;
     scaled_interval(p.AL[row], p.AU[row], r, &sl, &su);
     if (fmax(rl, sl) > fmin(ru, su) + itol) {
-      wave_or(&flags[0], 1);
+      wave_set_flag(&flags[0]);
       return;
     }
     ++step;
