@@ -394,15 +394,45 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
       if (ok) return;
     }
     if (fabs(xj - implied) > tol) return;
+    // The row's entries are read eight at a time with every load in flight
+    // together (the columns of a row are distinct, so the updates of one
+    // chunk are independent); the results are those of the entry-by-entry
+    // loops.
+    constexpr int U = 8;
+    const int32_t qb = O.a_ptr[row], qe = O.a_ptr[row + 1];
     double aij = 0.0;
-    for (int32_t q = O.a_ptr[row]; q < O.a_ptr[row + 1]; ++q)
-      if (O.a_idx[q] == col) { aij = O.a_val[q]; break; }
+    for (int32_t q0 = qb; q0 < qe; q0 += U) {
+      int32_t j[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) j[k] = O.a_idx[min(q0 + k, qe - 1)];
+      int hit = -1;
+#pragma unroll
+      for (int k = U - 1; k >= 0; --k)
+        if (q0 + k < qe && j[k] == col) hit = k;
+      if (hit >= 0) {
+        aij = O.a_val[q0 + hit];
+        break;
+      }
+    }
     if (fabs(aij) <= tol) return;
     if (y) y[row] += zj / aij;
-    for (int32_t q = O.a_ptr[row]; q < O.a_ptr[row + 1]; ++q) {
-      const int32_t kc = O.a_idx[q];
-      if (kc == col || (zret && !zret[kc])) continue;
-      z[kc] -= (O.a_val[q] / aij) * zj;
+    for (int32_t q0 = qb; q0 < qe; q0 += U) {
+      int32_t kc[U];
+      double a[U], zk[U];
+      bool on[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int32_t q = min(q0 + k, qe - 1);
+        kc[k] = O.a_idx[q];
+        a[k] = O.a_val[q];
+      }
+#pragma unroll
+      for (int k = 0; k < U; ++k) on[k] = q0 + k < qe && kc[k] != col && (!zret || zret[kc[k]]);
+#pragma unroll
+      for (int k = 0; k < U; ++k) zk[k] = z[kc[k]];
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (on[k]) z[kc[k]] = zk[k] - (a[k] / aij) * zj;
     }
     z[col] = 0.0;
     if (zret) zret[col] = 1;
