@@ -122,7 +122,10 @@ hipError_t gpk_compact_write_counts(const int32_t* ptr, const double* val, int32
 
 /* Up to GPK_COPY_MAX copies (src != NULL) or zero fills (src == NULL) of
  * device byte ranges in one launch (block row y serves segment y; the unit
- * is the widest of 16/8/4/1 bytes that divides both addresses and the size). */
+ * is the widest of 16/8/4/1 bytes that divides both addresses and the size).
+ * No reference counterpart: it batches what the reference issues as one
+ * cudaMemcpyAsync / cudaMemsetAsync per buffer (e.g. the plan's vector copies,
+ * gpu_presolve.cu:197-247). */
 #define GPK_COPY_MAX 12
 typedef struct gpk_copy_seg {
   void* dst;
