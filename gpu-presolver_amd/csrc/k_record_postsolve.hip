@@ -530,7 +530,21 @@ __global__ void k_project_duals(gp_orig_dev O, const uint8_t* prot, const double
   z[col] = project_dual(O, col, aty, x, tol);
 }
 
+// Lane t's double, in every lane (t the same in all lanes).
+__device__ inline double lane_f64(double v, int t) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), t);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), t);
+  return __hiloint2double(hi, lo);
+}
+
 // One wave per listed column; a fixed grid whose waves stride over the list.
+// The column's products are formed kLongSlices x 64 at a time (all loads of
+// the batch in flight together) and added to the running sum one by one in
+// AT order, each read from its lane into scalar registers, so the serial
+// chain costs one add per entry.
+constexpr int kLongSlices = 4;
+constexpr int kLongGrid = 2048;
 __global__ void k_project_duals_long(gp_orig_dev O, const double* x, const double* y, double* z, double tol,
                                      const int32_t* long_count, const int32_t* long_cols) {
   const int lane = lane_id();
@@ -540,11 +554,27 @@ __global__ void k_project_duals_long(gp_orig_dev O, const double* x, const doubl
     const int32_t col = long_cols[w];
     const int32_t b = O.at_ptr[col], e = O.at_ptr[col + 1];
     double aty = 0.0;
-    for (int32_t base = b; base < e; base += 64) {
-      const int32_t q = base + lane;
-      const double prod = q < e ? O.at_val[q] * y[O.at_idx[q]] : 0.0;
-      const int32_t k = min(64, e - base);
-      for (int32_t t = 0; t < k; ++t) aty += __shfl(prod, t, 64);
+    for (int32_t base = b; base < e; base += 64 * kLongSlices) {
+      int32_t j[kLongSlices];
+      double a[kLongSlices], p[kLongSlices];
+#pragma unroll
+      for (int u = 0; u < kLongSlices; ++u) {
+        const int32_t q = base + 64 * u + lane;
+        j[u] = q < e ? O.at_idx[q] : -1;
+        a[u] = q < e ? O.at_val[q] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kLongSlices; ++u) p[u] = j[u] >= 0 ? a[u] * y[j[u]] : 0.0;
+#pragma unroll
+      for (int u = 0; u < kLongSlices; ++u) {
+        const int32_t s0 = base + 64 * u;
+        if (s0 + 64 <= e) {
+#pragma unroll
+          for (int t = 0; t < 64; ++t) aty += lane_f64(p[u], t);
+        } else {
+          for (int t = 0; t < e - s0; ++t) aty += lane_f64(p[u], t);
+        }
+      }
     }
     if (lane == 0) z[col] = project_dual(O, col, aty, x, tol);
   }
@@ -650,7 +680,7 @@ hipError_t gpk_project_duals(gp_orig_dev orig, const uint8_t* prot, const double
   if (n <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_project_duals, dim3(grid_for(n)), dim3(kBlock), 0, stream, orig, prot, x, y, z, n, tol,
                      long_count, long_cols);
-  hipLaunchKernelGGL(k_project_duals_long, dim3(256), dim3(kBlock), 0, stream, orig, x, y, z, tol, long_count,
+  hipLaunchKernelGGL(k_project_duals_long, dim3(kLongGrid), dim3(kBlock), 0, stream, orig, x, y, z, tol, long_count,
                      long_cols);
   return hipGetLastError();
 }
