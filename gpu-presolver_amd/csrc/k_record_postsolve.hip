@@ -168,10 +168,9 @@ __device__ void dbl_recover_z(double xk, double zk_hat, double al, double okl, d
 // rounded, then added or subtracted one at a time).  The indices, values and
 // gathered entries of eight terms are loaded before they are used, so a long
 // column costs count / 8 memory round trips instead of count.
-template <bool kSub>
+template <bool kSub, int U = 8>
 __device__ inline double dot_in_order(double s, const int32_t* __restrict__ idx, const double* __restrict__ val,
                                       const double* __restrict__ vec, int32_t count, int32_t skip) {
-  constexpr int U = 8;
   int32_t q = 0;
   for (; q + U <= count; q += U) {
     int32_t j[U];
@@ -196,13 +195,24 @@ __device__ inline double dot_in_order(double s, const int32_t* __restrict__ idx,
 
 // Where a record's indices and values sit in the tape (immutable, so it can
 // be loaded ahead of the record's turn).
+// The header also carries the record's first three index words (-1 past
+// its end), so a prefetched header leaves the record's variables one load
+// away.
 struct RecHead {
   int32_t type, i0, i1, v0, v1;
+  int32_t a0, a1, a2;
 };
 __device__ inline RecHead load_head(const gp_tape_dev& T, int32_t k) {
-  return RecHead{T.types[k], T.index_starts[k], T.index_starts[k + 1] - 1, T.value_starts[k],
-                 T.value_starts[k + 1] - 1};
+  RecHead h{T.types[k], T.index_starts[k], T.index_starts[k + 1] - 1, T.value_starts[k], T.value_starts[k + 1] - 1,
+            -1, -1, -1};
+  if (h.i0 <= h.i1) h.a0 = T.indices[h.i0];
+  if (h.i0 + 1 <= h.i1) h.a1 = T.indices[h.i0 + 1];
+  if (h.i0 + 2 <= h.i1) h.a2 = T.indices[h.i0 + 2];
+  return h;
 }
+
+// Dot products inside a record keep this many entries' loads in flight.
+constexpr int kRecDotU = 32;
 
 // One record of the reverse replay (_kernel_replay_postsolve_tape,
 // gpu_postsolve.cu:605-1046).
@@ -225,11 +235,14 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
     }
     if (dual && z) { z[from] = r * z[to]; if (zret) zret[from] = 1; }
   } else if (type == 13) {  // DoubletonEquation
-    const int32_t ec = I[i0], kc = I[i0 + 1], row = I[i0 + 2];
+    const int32_t ec = h.a0, kc = h.a1, row = h.a2;
     const double ecoef = V[v0], kcoef = V[v0 + 1], rhs = V[v0 + 2], eobj = V[v0 + 9];
     const double al = -kcoef / ecoef, be = rhs / ecoef;
     if (primal) x[ec] = be + al * x[kc];
     if (dual && z && y && has_orig) {
+      // The column bounds and c[ec] load together with x[kc] and z[kc].
+      const int32_t qb = O.at_ptr[ec], qe = O.at_ptr[ec + 1];
+      const double cec = O.c[ec];
       const double ls = v1 >= v0 + 10 ? V[v0 + 10] : 0.0;
       const double us = v1 >= v0 + 11 ? V[v0 + 11] : 0.0;
       double zk = 0.0, ze = 0.0;
@@ -238,12 +251,11 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
       double so = 0.0;
       double yrow;
       if (payload) {
-        so = dot_in_order<false>(so, I + i0 + 4, V + v0 + 12, y, I[i0 + 3], -1);
+        so = dot_in_order<false, kRecDotU>(so, I + i0 + 4, V + v0 + 12, y, I[i0 + 3], -1);
         yrow = (eobj - so - ze) / ecoef;
       } else {
-        const int32_t qb = O.at_ptr[ec];
-        so = dot_in_order<false>(so, O.at_idx + qb, O.at_val + qb, y, O.at_ptr[ec + 1] - qb, row);
-        yrow = (O.c[ec] - so - ze) / ecoef;
+        so = dot_in_order<false, kRecDotU>(so, O.at_idx + qb, O.at_val + qb, y, qe - qb, row);
+        yrow = (cec - so - ze) / ecoef;
       }
       y[row] = yrow;
       z[kc] = zk;
@@ -254,10 +266,10 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
       if (y && row >= 0) y[row] = eobj / ecoef;
     }
   } else if (type == 2) {  // SubCol
-    const int32_t ec = I[i0], row = I[i0 + 1], sc = I[i0 + 2];
+    const int32_t ec = h.a0, row = h.a1, sc = h.a2;
     const double piv = V[v0], rhs = V[v0 + 1], eobj = V[v0 + 4];
     const bool row_deleted = V[v0 + 5] != 0.0;
-    const double part = dot_in_order<false>(0.0, I + i0 + 3, V + v0 + 6, x, sc, -1);
+    const double part = dot_in_order<false, kRecDotU>(0.0, I + i0 + 3, V + v0 + 6, x, sc, -1);
     if (primal) x[ec] = (rhs - part) / piv;
     if (dual && y && row >= 0) {
       if (row_deleted) y[row] = eobj / piv; else y[row] += eobj / piv;
@@ -267,11 +279,11 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
       if (zret) zret[ec] = 1;
     }
   } else if (type == 0) {  // FixedCol
-    const int32_t col = I[i0];
+    const int32_t col = h.a0;
     if (primal) x[col] = V[v0];
     if (dual && z) {
       double zj = V[v0 + 1];
-      if (y) zj = dot_in_order<true>(zj, I + i0 + 1, V + v0 + 2, y, i1 - i0, -1);
+      if (y) zj = dot_in_order<true, kRecDotU>(zj, I + i0 + 1, V + v0 + 2, y, i1 - i0, -1);
       z[col] = zj;
       if (zret) zret[col] = 1;
     }
@@ -329,8 +341,8 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
     if (dsl <= kal + tol && kau <= dsu + tol) { y[dr] = 0.0; return; }
     if (has_orig) {
       const int32_t kb = O.a_ptr[kr], db = O.a_ptr[dr];
-      const double ap = dot_in_order<false>(0.0, O.a_idx + kb, O.a_val + kb, x, O.a_ptr[kr + 1] - kb, -1);
-      const double ad = dot_in_order<false>(0.0, O.a_idx + db, O.a_val + db, x, O.a_ptr[dr + 1] - db, -1);
+      const double ap = dot_in_order<false, kRecDotU>(0.0, O.a_idx + kb, O.a_val + kb, x, O.a_ptr[kr + 1] - kb, -1);
+      const double ad = dot_in_order<false, kRecDotU>(0.0, O.a_idx + db, O.a_val + db, x, O.a_ptr[dr + 1] - db, -1);
       const uint8_t ks = row_state(ap, kal, kau, tol), ds = row_state(ad, dal, dau, tol);
       double ydl, ydh, ypl, yph;
       state_interval(ds, &ydl, &ydh);
@@ -346,14 +358,12 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
       if (doc_dual_split(y[kr], r, kal, kau, dsl, dsu, tol, &ypd, &ydd)) { y[kr] = ypd; y[dr] = ydd; }
     }
   } else if (dual && type == 5 && has_orig && i1 >= i0 + 1 && v1 >= v0 + 2) {  // DeletedRow
-    const int32_t row = I[i0], col = I[i0 + 1];
+    const int32_t row = h.a0, col = h.a1;
     const double oal = V[v0], oau = V[v0 + 1], aij = V[v0 + 2];
+    const int32_t qb = O.at_ptr[col], qe = O.at_ptr[col + 1];
     if (fabs(aij) <= tol) return;
     double gamma = O.c[col];
-    if (y) {
-      const int32_t qb = O.at_ptr[col];
-      gamma = dot_in_order<true>(gamma, O.at_idx + qb, O.at_val + qb, y, O.at_ptr[col + 1] - qb, row);
-    }
+    if (y) gamma = dot_in_order<true, kRecDotU>(gamma, O.at_idx + qb, O.at_val + qb, y, qe - qb, row);
     const double xj = x[col], zbar = z[col];
     const double act = aij * xj;
     const bool ral = pfin(oal) && act <= oal + tol;
@@ -380,39 +390,76 @@ __device__ void replay_one(const gp_tape_dev& T, const gp_orig_dev& O, double* x
     if (y) y[row] = (gamma - zn) / aij;
     if (zret) zret[col] = 1;
   } else if (dual && type == 12 && has_orig && i1 >= i0 + 1) {  // BoundChangeTheRow
-    const int32_t col = I[i0], row = I[i0 + 1];
-    if (zret && !zret[col]) return;
+    const int32_t col = h.a0, row = h.a1;
+    // Every load that depends only on the record goes out at once; the
+    // decisions below then read registers.
+    const uint8_t col_set = zret ? zret[col] : 1;
     const double ol = V[v0], ou = V[v0 + 1], nl = V[v0 + 2], nu = V[v0 + 3];
+    const double xj = x[col], zj = z[col];
+    const int32_t qb = O.a_ptr[row], qe = O.a_ptr[row + 1];
+    if (!col_set) return;
     const bool lc = nl > ol + tol, uc = nu < ou - tol;
     if (lc == uc) return;
     const double implied = lc ? nl : nu;
     const double other = lc ? ou : ol;
     const bool other_is_lower = !lc;
-    const double xj = x[col], zj = z[col];
     if (pfin(other) && fabs(xj - other) <= tol) {
       const bool ok = other_is_lower ? (zj >= -tol) : (zj <= tol);
       if (ok) return;
     }
     if (fabs(xj - implied) > tol) return;
-    // The row's entries are read eight at a time with every load in flight
+    // The row's entries are read U at a time with every load in flight
     // together (the columns of a row are distinct, so the updates of one
     // chunk are independent); the results are those of the entry-by-entry
-    // loops.
-    constexpr int U = 8;
-    const int32_t qb = O.a_ptr[row], qe = O.a_ptr[row + 1];
+    // loops.  A row of at most U entries is loaded once for both passes.
+    constexpr int U = 16;
+    if (qe <= qb) return;  // an empty row holds no a_ij
+    if (qe - qb <= U) {
+      int32_t kc[U];
+      double a[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int32_t q = min(qb + k, qe - 1);
+        kc[k] = O.a_idx[q];
+        a[k] = O.a_val[q];
+      }
+      double aij = 0.0;
+#pragma unroll
+      for (int k = U - 1; k >= 0; --k)
+        if (qb + k < qe && kc[k] == col) aij = a[k];
+      if (fabs(aij) <= tol) return;
+      bool on[U];
+      double zk[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) on[k] = qb + k < qe && kc[k] != col && (!zret || zret[kc[k]]);
+#pragma unroll
+      for (int k = 0; k < U; ++k) zk[k] = z[kc[k]];
+      if (y) y[row] += zj / aij;
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (on[k]) z[kc[k]] = zk[k] - (a[k] / aij) * zj;
+      z[col] = 0.0;
+      if (zret) zret[col] = 1;
+      return;
+    }
     double aij = 0.0;
     for (int32_t q0 = qb; q0 < qe; q0 += U) {
       int32_t j[U];
+      double a[U];
 #pragma unroll
-      for (int k = 0; k < U; ++k) j[k] = O.a_idx[min(q0 + k, qe - 1)];
-      int hit = -1;
+      for (int k = 0; k < U; ++k) {
+        const int32_t q = min(q0 + k, qe - 1);
+        j[k] = O.a_idx[q];
+        a[k] = O.a_val[q];
+      }
+      bool hit = false;
 #pragma unroll
       for (int k = U - 1; k >= 0; --k)
-        if (q0 + k < qe && j[k] == col) hit = k;
-      if (hit >= 0) {
-        aij = O.a_val[q0 + hit];
-        break;
-      }
+        if (q0 + k < qe && j[k] == col) {
+          aij = a[k];
+          hit = true;
+        }
+      if (hit) break;
     }
     if (fabs(aij) <= tol) return;
     if (y) y[row] += zj / aij;
