@@ -496,7 +496,7 @@ __global__ void k_replay(gp_tape_dev T, gp_orig_dev O, double* x, double* y, dou
 
 // One level of a dependency-level schedule: its records touch disjoint
 // variables and are replayed one per thread.
-__global__ void k_replay_list(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
+__global__ void __launch_bounds__(kBlock) k_replay_list(gp_tape_dev T, gp_orig_dev O, double* x, double* y, double* z, uint8_t* zret,
                               const int32_t* records, int32_t count, int32_t primal, int32_t dual, double tol) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) replay_one(T, O, x, y, z, zret, load_head(T, records[i]), primal != 0, dual != 0, tol);
