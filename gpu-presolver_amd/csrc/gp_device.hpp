@@ -31,14 +31,14 @@ __device__ inline bool wave_leader(uint64_t* active) {
   *active = m;
   return lane_id() == __ffsll(static_cast<unsigned long long>(m)) - 1;
 }
-// A yes/no flag word (0 until some thread sets it to 1): the wave's leader
-// stores 1 unless it already reads 1.  A plain store, not an atomic: when
-// most waves of a large grid set the same flag they would otherwise queue on
-// one address for read-modify-write.
+// A yes/no flag word (0 until some thread sets it to 1, read only after the
+// kernel): the wave's leader stores 1 with a plain store, which lands in its
+// XCD's L2 and is written back at the end of the kernel.  No read-modify-
+// write and no device-coherent load: when most waves of a large grid set the
+// same flag, those would all queue on one address at one memory channel.
 __device__ inline void wave_set_flag(int32_t* addr) {
   uint64_t m;
-  if (wave_leader(&m) && __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
-    __hip_atomic_store(addr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (wave_leader(&m)) __hip_atomic_store(addr, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 // Max only ever grows the word, so a leader that reads a value at least its
 // own skips the atomic.

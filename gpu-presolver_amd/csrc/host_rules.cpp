@@ -449,9 +449,10 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
   check(gpk_cs_precheck(n, m, P, stats.column_singleton_mask, stats.column_singleton_row, prm.bound_tol, needed, f, s),
         "column singletons precheck");
   if (!skip_precheck) {
-    const int32_t* h = c.read_flags(2);
-    const int32_t count = h[0], eq = h[1];
-    if (count == 0 || (eq_mode && eq == 0) || (!eq_mode && count == eq)) return;
+    // Singletons at all; on equality rows (what eq mode uses); on other rows
+    // (what dual inference uses).
+    const int32_t* h = c.read_flags(3);
+    if (!h[0] || (eq_mode && !h[1]) || (!eq_mode && !h[2])) return;
   }
   DBuf<double> fmn(m), fmx(m);
   DBuf<int32_t> mn(m), mp(m), xn(m), xp(m);

@@ -30,7 +30,7 @@ __global__ void k_fixed_vars_candidates(int32_t n, gpk_plan p, double tol, uint8
     fixed[j] = 1;
     v[j] = vj;
     contrib[j] = p.c[j] * vj;
-    wave_add1(count);
+    wave_set_flag(count);
   }
 }
 
@@ -145,17 +145,19 @@ __device__ bool cs_rest_bounds(const gpk_row_cache& rc, int32_t row, int32_t col
   return !isnan(*rest_min) && !isnan(*rest_max);
 }
 
-// counts[0]: live singleton columns, counts[1]: of which on equality rows.
+// flags[0]: some live singleton column; flags[1]: some on an equality row;
+// flags[2]: some on a row that is not an equality.
 __global__ void k_cs_precheck(int32_t n, int32_t m, gpk_plan p, const uint8_t* mask, const int32_t* srow,
-                              double tol, uint8_t* needed, int32_t* counts) {
+                              double tol, uint8_t* needed, int32_t* flags) {
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n || !p.keep_col[j] || !mask[j]) return;
   const int32_t row = srow[j];
   if (row < 0 || row >= m || !p.keep_row[row]) return;
   needed[row] = 1;
-  wave_add1(&counts[0]);
+  wave_set_flag(&flags[0]);
   const double lo = p.AL[row], hi = p.AU[row];
-  if (isfinite(lo) && isfinite(hi) && fabs(hi - lo) <= tol) wave_add1(&counts[1]);
+  if (isfinite(lo) && isfinite(hi) && fabs(hi - lo) <= tol) wave_set_flag(&flags[1]);
+  else wave_set_flag(&flags[2]);
 }
 
 // rule_column_singletons.cu:455-535 (sequential along the row).
@@ -1154,8 +1156,8 @@ hipError_t gpk_fix_cols_apply(int32_t n, gpk_plan plan, const uint8_t* fixed, co
   GPK_LAUNCH(k_fix_cols_apply, n, n, plan, fixed, v);
 }
 hipError_t gpk_cs_precheck(int32_t n, int32_t m, gpk_plan plan, const uint8_t* mask, const int32_t* srow,
-                           double bound_tol, uint8_t* needed, int32_t* counts, hipStream_t stream) {
-  GPK_LAUNCH(k_cs_precheck, n, n, m, plan, mask, srow, bound_tol, needed, counts);
+                           double bound_tol, uint8_t* needed, int32_t* flags, hipStream_t stream) {
+  GPK_LAUNCH(k_cs_precheck, n, n, m, plan, mask, srow, bound_tol, needed, flags);
 }
 hipError_t gpk_cs_activity_cache(gpk_csr A, gpk_plan plan, const uint8_t* needed, gpk_row_cache cache,
                                  hipStream_t stream) {

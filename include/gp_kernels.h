@@ -362,7 +362,8 @@ hipError_t gpk_dup_row_pack(int32_t m, const uint8_t* del, const int32_t* prefix
                             const int32_t* step, const double* rec, int32_t* out_row, int32_t* out_to,
                             int32_t* out_step, double* out_rec, hipStream_t stream);
 
-/* rule_infeasible_fixed_variables.cu:42-104, rule_empty_cols.cu:42-108 */
+/* rule_infeasible_fixed_variables.cu:42-104, rule_empty_cols.cu:42-108
+ * (*count: set to 1 when some column is fixed) */
 hipError_t gpk_fixed_vars_candidates(int32_t n, gpk_plan plan, double bound_tol, uint8_t* fixed, double* v,
                                      double* contrib, int32_t* count, hipStream_t stream);
 hipError_t gpk_empty_cols(int32_t n, gpk_plan plan, const uint8_t* empty_mask, double bound_tol, double zero_tol,
@@ -380,8 +381,10 @@ typedef struct gpk_row_cache {
   int32_t* xp;
   uint8_t* invalid;
 } gpk_row_cache;
+/* flags[0] = 1: some live singleton column; flags[1]: some on an equality
+ * row; flags[2]: some on another row. */
 hipError_t gpk_cs_precheck(int32_t n, int32_t m, gpk_plan plan, const uint8_t* mask, const int32_t* srow,
-                           double bound_tol, uint8_t* needed, int32_t* counts, hipStream_t stream);
+                           double bound_tol, uint8_t* needed, int32_t* flags, hipStream_t stream);
 hipError_t gpk_cs_activity_cache(gpk_csr A, gpk_plan plan, const uint8_t* needed, gpk_row_cache cache,
                                  hipStream_t stream);
 hipError_t gpk_cs_owner(gpk_csr A, gpk_plan plan, const uint8_t* mask, const int32_t* srow, const double* sval,
