@@ -8,8 +8,9 @@ planted reductions of the workload description).  `value` is LPs presolved
 per second.
 
 Extra objects on the JSON line:
-  roofline           the single-pass compaction kernel k_compact_lb (the
-                     largest share of presolve kernel time) on the full A with
+  roofline           the compaction (k_build_rank, k_compact_map, the tile-offset
+                     scan and k_compact_write; the largest share of presolve
+                     kernel time after the replay) on the full A with
                      15% of rows and columns dropped, timed with HIP events
                      around back-to-back launches on one stream, against 8 TB/s;
                      traffic comes from the committed rocprofv3 PMC summary;
