@@ -591,11 +591,11 @@ __global__ void k_dup_row_groups(gpk_csr A, gpk_plan p, const uint64_t* sh, cons
   }
 }
 
-__global__ void k_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count) {
+__global__ void k_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* deleted) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < m && row_delete[i]) {
     keep_row[i] = 0;
-    wave_add1(count);
+    wave_set_flag(deleted);
   }
 }
 
@@ -836,9 +836,9 @@ hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_h
   GPK_LAUNCH(k_dup_row_groups, A.rows, A, plan, sorted_hash, sorted_rows, coeff_tol, interval_tol, row_delete,
              merge_to, merge_step, rec_vals, flags);
 }
-hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count,
+hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* deleted,
                            hipStream_t stream) {
-  GPK_LAUNCH(k_delete_rows, m, m, keep_row, row_delete, count);
+  GPK_LAUNCH(k_delete_rows, m, m, keep_row, row_delete, deleted);
 }
 hipError_t gpk_dup_row_pack(int32_t m, const uint8_t* del, const int32_t* prefix, const int32_t* merge_to,
                             const int32_t* step, const double* rec, int32_t* out_row, int32_t* out_to,

@@ -354,7 +354,8 @@ hipError_t gpk_dup_row_hash(gpk_csr A, const uint8_t* keep_row, double coeff_tol
 hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_hash, const int32_t* sorted_rows,
                               double coeff_tol, double interval_tol, uint8_t* row_delete, int32_t* merge_to,
                               int32_t* merge_step, double* rec_vals, int32_t* flags, hipStream_t stream);
-hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* count,
+/* keep_row[i] = 0 where row_delete[i]; *deleted set to 1 when some row is. */
+hipError_t gpk_delete_rows(int32_t m, uint8_t* keep_row, const uint8_t* row_delete, int32_t* deleted,
                            hipStream_t stream);
 /* Deleted duplicate rows (del[i] != 0, prefix = exclusive scan of del) packed
  * in ascending row order: row, merge_to, step and the 5 record values. */
