@@ -225,33 +225,75 @@ __device__ inline void implied_from_row(double a, double lo, double hi, double r
   }
 }
 
+// Walks column j of AT and calls f(i, il, iu) for each entry on a live row
+// (kept, more than one entry) with |a| > ztol, in AT order, where il / iu is
+// the bound that row implies for the column (the column's own term taken
+// out of the row's activity).  Entries go U at a time: their row indices and
+// values load together, then every row quantity of the U rows, so a chunk
+// costs two dependent loads instead of three per entry.
+template <class F>
+__device__ inline void implied_walk(const gpk_csr& AT, const gpk_plan& p, const int32_t* row_nnz, const double* rmin,
+                                    const double* rmax, const int32_t* rni, const int32_t* rpi, double ztol, int32_t j,
+                                    double ol, double ou, F&& f) {
+  constexpr int U = 4;
+  const int32_t qb = AT.ptr[j], qe = AT.ptr[j + 1];
+  for (int32_t q0 = qb; q0 < qe; q0 += U) {
+    int32_t ri[U];
+    double av[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int32_t q = min(q0 + k, qe - 1);
+      ri[k] = AT.idx[q];
+      av[k] = AT.val[q];
+    }
+    bool live[U];
+    double smin[U], smax[U], lo[U], hi[U];
+    int32_t ni[U], pi[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int32_t i = ri[k];
+      live[k] = p.keep_row[i] != 0 && row_nnz[i] > 1;
+      smin[k] = rmin[i];
+      smax[k] = rmax[i];
+      ni[k] = rni[i];
+      pi[k] = rpi[i];
+      lo[k] = p.AL[i];
+      hi[k] = p.AU[i];
+    }
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      if (q0 + k >= qe || !live[k]) continue;
+      const double a = av[k];
+      if (fabs(a) <= ztol) continue;
+      double tmin, tmax;
+      term_interval(a, ol, ou, &tmin, &tmax);
+      const double rs_min = resid_min(smin[k], ni[k], tmin);
+      const double rs_max = resid_max(smax[k], pi[k], tmax);
+      double il, iu;
+      implied_from_row(a, lo[k], hi[k], rs_min, rs_max, &il, &iu);
+      f(ri[k], il, iu);
+    }
+  }
+}
+
 // Candidate bounds per column, pulled along AT (no atomics): the max (min)
 // over the column's live rows of the implied lower (upper) bound, seeded with
 // the current bound.  Same values as the reference's per-entry CAS push.
-__global__ void k_implied_candidates(gpk_csr AT, gpk_plan p, const int32_t* row_nnz, const double* rmin,
-                                     const double* rmax, const int32_t* rni, const int32_t* rpi, double ztol,
-                                     double* cand_l, double* cand_u) {
+__global__ __launch_bounds__(kBlock) void k_implied_candidates(gpk_csr AT, gpk_plan p, const int32_t* row_nnz,
+                                                               const double* rmin, const double* rmax,
+                                                               const int32_t* rni, const int32_t* rpi, double ztol,
+                                                               double* cand_l, double* cand_u) {
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= AT.rows) return;
   const double ol = p.l[j], ou = p.u[j];
   double cl = ol, cu = ou;
-  for (int32_t q = AT.ptr[j]; q < AT.ptr[j + 1]; ++q) {
-    const int32_t i = AT.idx[q];
-    if (!p.keep_row[i] || row_nnz[i] <= 1) continue;
-    const double a = AT.val[q];
-    if (fabs(a) <= ztol) continue;
-    double tmin, tmax;
-    term_interval(a, ol, ou, &tmin, &tmax);
-    const double rs_min = resid_min(rmin[i], rni[i], tmin);
-    const double rs_max = resid_max(rmax[i], rpi[i], tmax);
-    double il, iu;
-    implied_from_row(a, p.AL[i], p.AU[i], rs_min, rs_max, &il, &iu);
+  implied_walk(AT, p, row_nnz, rmin, rmax, rni, rpi, ztol, j, ol, ou, [&](int32_t, double il, double iu) {
     double nl = ol, nu = ou;
     if (isfinite(il)) nl = fmax(nl, il);
     if (isfinite(iu)) nu = fmin(nu, iu);
     if (nl > ol && !(cl >= nl)) cl = nl;
     if (nu < ou && !(cu <= nu)) cu = nu;
-  }
+  });
   cand_l[j] = cl;
   cand_u[j] = cu;
 }
@@ -338,10 +380,12 @@ __global__ void k_implied_finalize(int32_t n, gpk_plan p, const double* ref_l, c
 }
 
 // rule_implied_variable_bounds.cu:668-750 plus the per-column tape record count.
-__global__ void k_implied_support(gpk_csr AT, gpk_plan p, const int32_t* row_nnz, const double* rmin,
-                                  const double* rmax, const int32_t* rni, const int32_t* rpi, const uint8_t* lch,
-                                  const uint8_t* uch, const double* fl, const double* fu, double ztol, double tol,
-                                  double stol, int32_t* sup_l, int32_t* sup_u, int32_t* rec_count) {
+__global__ __launch_bounds__(kBlock) void k_implied_support(gpk_csr AT, gpk_plan p, const int32_t* row_nnz,
+                                                            const double* rmin, const double* rmax,
+                                                            const int32_t* rni, const int32_t* rpi,
+                                                            const uint8_t* lch, const uint8_t* uch, const double* fl,
+                                                            const double* fu, double ztol, double tol, double stol,
+                                                            int32_t* sup_l, int32_t* sup_u, int32_t* rec_count) {
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= AT.rows) return;
   int32_t sl = INT_MAX, su = INT_MAX;
@@ -349,20 +393,10 @@ __global__ void k_implied_support(gpk_csr AT, gpk_plan p, const int32_t* row_nnz
   if (need_l || need_u) {
     const double ol = p.l[j], ou = p.u[j];
     const double cl = fl[j], cu = fu[j];
-    for (int32_t q = AT.ptr[j]; q < AT.ptr[j + 1]; ++q) {
-      const int32_t i = AT.idx[q];
-      if (!p.keep_row[i] || row_nnz[i] <= 1) continue;
-      const double a = AT.val[q];
-      if (fabs(a) <= ztol) continue;
-      double tmin, tmax;
-      term_interval(a, ol, ou, &tmin, &tmax);
-      const double rs_min = resid_min(rmin[i], rni[i], tmin);
-      const double rs_max = resid_max(rmax[i], rpi[i], tmax);
-      double il, iu;
-      implied_from_row(a, p.AL[i], p.AU[i], rs_min, rs_max, &il, &iu);
+    implied_walk(AT, p, row_nnz, rmin, rmax, rni, rpi, ztol, j, ol, ou, [&](int32_t i, double il, double iu) {
       if (need_l && cl > ol + tol && isfinite(il) && fabs(il - cl) <= stol) sl = min(sl, i);
       if (need_u && cu < ou - tol && isfinite(iu) && fabs(iu - cu) <= stol) su = min(su, i);
-    }
+    });
   }
   sup_l[j] = sl;
   sup_u[j] = su;
