@@ -161,32 +161,56 @@ __global__ void k_cs_precheck(int32_t n, int32_t m, gpk_plan p, const uint8_t* m
 }
 
 // rule_column_singletons.cu:455-535 (sequential along the row).
-__global__ void k_cs_activity_cache(gpk_csr A, gpk_plan p, const uint8_t* needed, gpk_row_cache rc) {
+// The row is walked U entries at a time: their columns and values load
+// together, then the columns' keep flags and bounds, and the terms are then
+// added in CSR order (the sums are those of the entry-by-entry loop).
+__global__ __launch_bounds__(kBlock) void k_cs_activity_cache(gpk_csr A, gpk_plan p, const uint8_t* needed,
+                                                              gpk_row_cache rc) {
+  constexpr int U = 8;
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.rows || !needed[i]) return;
   double fmn = 0.0, fmx = 0.0;
   int32_t mn = 0, mp = 0, xn = 0, xp = 0;
   bool invalid = false;
   if (p.keep_row[i]) {
-    for (int32_t q = A.ptr[i]; q < A.ptr[i + 1]; ++q) {
-      const int32_t col = A.idx[q];
-      if (!p.keep_col[col]) continue;
-      const double a = A.val[q];
-      const double tmin = a >= 0.0 ? a * p.l[col] : a * p.u[col];
-      const double tmax = a >= 0.0 ? a * p.u[col] : a * p.l[col];
-      if (isnan(tmin) || isnan(tmax)) {
-        invalid = true;
-        continue;
+    const int32_t qb = A.ptr[i], qe = A.ptr[i + 1];
+    for (int32_t q0 = qb; q0 < qe; q0 += U) {
+      int32_t cc[U];
+      double av[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        const int32_t q = min(q0 + k, qe - 1);
+        cc[k] = A.idx[q];
+        av[k] = A.val[q];
       }
-      if (isinf(tmin)) {
-        if (tmin < 0.0) ++mn; else ++mp;
-      } else {
-        fmn += tmin;
+      bool kept[U];
+      double lv[U], uv[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        kept[k] = p.keep_col[cc[k]] != 0;
+        lv[k] = p.l[cc[k]];
+        uv[k] = p.u[cc[k]];
       }
-      if (isinf(tmax)) {
-        if (tmax < 0.0) ++xn; else ++xp;
-      } else {
-        fmx += tmax;
+#pragma unroll
+      for (int k = 0; k < U; ++k) {
+        if (q0 + k >= qe || !kept[k]) continue;
+        const double a = av[k];
+        const double tmin = a >= 0.0 ? a * lv[k] : a * uv[k];
+        const double tmax = a >= 0.0 ? a * uv[k] : a * lv[k];
+        if (isnan(tmin) || isnan(tmax)) {
+          invalid = true;
+          continue;
+        }
+        if (isinf(tmin)) {
+          if (tmin < 0.0) ++mn; else ++mp;
+        } else {
+          fmn += tmin;
+        }
+        if (isinf(tmax)) {
+          if (tmax < 0.0) ++xn; else ++xp;
+        } else {
+          fmx += tmax;
+        }
       }
     }
   }
