@@ -1007,7 +1007,17 @@ __global__ __launch_bounds__(kSortSmallBlock) void k_sort_small(K* __restrict__ 
           s_p[j] = static_cast<int16_t>(pi);
         }
       }
-      __syncthreads();
+      // A stage with stride <= 64 only touches its wave's own 128-element
+      // segments (pair t of thread t lies in [128 (t / 64), +128)), so between
+      // two such stages the wave's LDS order suffices; any wider stage, before
+      // or after, meets the other waves at the workgroup barrier.
+      const int next = stride > 1 ? stride >> 1 : (size < len ? size : kSortSmallBlock);
+      if (stride > 64 || next > 64) {
+        __syncthreads();
+      } else {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
     }
   }
   for (int i = tid; i < n; i += kSortSmallBlock) {
