@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <exception>
 #include <initializer_list>
 #include <memory>
 #include <stdexcept>
@@ -29,6 +30,14 @@ class HipError : public std::runtime_error {
 
  private:
   hipError_t status_;
+};
+
+// An input that breaks the LP contract (bad CSR arrays, mismatched sizes):
+// a std::runtime_error like every other failure of the reference's API; the
+// C-ABI reports it as GP_ERR_INVALID_ARGUMENT.
+class InputError : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
 };
 
 inline void check(hipError_t status, const char* context) {
@@ -118,6 +127,12 @@ class Ctx {
   void to_device(T* dev, const T* host, int64_t count, const char* context) {
     if (count > 0) upload(dev, host, sizeof(T) * static_cast<size_t>(count), context);
   }
+  // Drops every pending download without copying it (its destination may
+  // already be gone), waits for the stream so the staging area is free, and
+  // clears the spin-limit error word.  Called when an exception leaves a
+  // library call, so the next wait on this context does not write into
+  // released host memory.
+  void abandon() noexcept;
 
  private:
   hipStream_t stream_ = nullptr;
@@ -163,6 +178,25 @@ class ScopedCtx {
 
  private:
   Ctx* prev_;
+};
+
+// Calls abandon() on the calling thread's context, if it has one.
+void abandon_pending_copies() noexcept;
+
+// Placed at the top of every public entry point: when an exception leaves
+// the scope, the context's pending downloads (whose destinations are host
+// vectors being destroyed by the unwinding) are dropped.
+class AbandonOnThrow {
+ public:
+  AbandonOnThrow() : depth_(std::uncaught_exceptions()) {}
+  ~AbandonOnThrow() {
+    if (std::uncaught_exceptions() > depth_) abandon_pending_copies();
+  }
+  AbandonOnThrow(const AbandonOnThrow&) = delete;
+  AbandonOnThrow& operator=(const AbandonOnThrow&) = delete;
+
+ private:
+  int depth_;
 };
 
 // Caching stream-ordered allocation (host_common.cpp).  dev_free returns the
@@ -310,6 +344,10 @@ uint64_t checksum_xyz(const double* x, int32_t n, const double* y, int32_t m, co
 // closure (process-wide; 10,000,000 as in the reference).
 void set_batched_closure_min_nnz(int32_t nnz);
 int32_t batched_closure_min_nnz();
+// Testing hook: the calling thread's count-th record update from now throws
+// std::runtime_error right after it has queued its host copies (0 disarms).
+void fail_after_record_updates(int32_t count);
+void record_update_checkpoint();
 
 // ---- shared host helpers (host_common.cpp) ---------------------------------
 // Row activity (gpu_presolve_kernels.cu:18-111) on `stream`: the bounds are
@@ -340,6 +378,10 @@ presolve::DeviceCsrMatrix transpose_csr(const presolve::DeviceCsrMatrix& A);
 void free_csr(presolve::DeviceCsrMatrix& A) noexcept;
 // Sorts the rows of A in place by column when some row is unsorted.
 void sort_rows_if_needed(const presolve::DeviceCsrMatrix& A);
+// Checks A and AT of an LP on the device (row pointers, index range, no
+// column repeated within a row; one wait) and throws InputError naming the
+// first problem.  Unsorted rows are accepted.
+void check_lp_arrays(const presolve::LPInfoGpu& lp, const char* who);
 // old_to_new map of a keep mask; returns the kept count.
 int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map);
 // FixedCol records and the long-double objective delta

@@ -57,10 +57,14 @@ struct InvalidArgument : std::invalid_argument {
 template <class F>
 gp_status guarded(F&& f) {
   try {
+    amd::AbandonOnThrow abandon_guard;
     f();
     t_error.clear();
     return GP_OK;
   } catch (const InvalidArgument& e) {
+    t_error = e.what();
+    return GP_ERR_INVALID_ARGUMENT;
+  } catch (const amd::InputError& e) {
     t_error = e.what();
     return GP_ERR_INVALID_ARGUMENT;
   } catch (const amd::HipError& e) {
@@ -598,6 +602,7 @@ gp_status gp_checksum(const double* x, int32_t n, const double* y, int32_t m, co
 
 void gp_set_batched_closure_min_nnz(int32_t nnz) { amd::set_batched_closure_min_nnz(nnz); }
 int32_t gp_batched_closure_min_nnz(void) { return amd::batched_closure_min_nnz(); }
+void gp_test_fail_after_record_updates(int32_t count) { amd::fail_after_record_updates(count); }
 
 void gp_presolve_free(gp_presolve_handle* h) {
   if (h == nullptr) return;

@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <shared_mutex>
+#include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -40,6 +41,7 @@ void configure_pool_once() {
 }
 
 thread_local Ctx* t_current = nullptr;
+thread_local Ctx* t_own = nullptr;  // the thread's own context (created on first use)
 
 }  // namespace
 
@@ -221,9 +223,20 @@ const uint32_t* Ctx::mail(const void* dev, int words, const char* context) {
 
 Ctx& ctx() {
   if (t_current) return *t_current;
-  thread_local Ctx* own = nullptr;
-  if (!own) own = new Ctx();  // lives for the thread
-  return *own;
+  if (!t_own) t_own = new Ctx();  // lives for the thread
+  return *t_own;
+}
+
+void Ctx::abandon() noexcept {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  downloads_.clear();
+  stage_used_ = 0;
+  if (h_error_) *h_error_ = 0;
+}
+
+void abandon_pending_copies() noexcept {
+  Ctx* c = t_current ? t_current : t_own;
+  if (c) c->abandon();
 }
 
 JobMode& job_mode() {
@@ -339,9 +352,24 @@ void dev_free(void* p, hipStream_t stream) noexcept {
   if (recycle(stream_arena(stream), p)) return;
   {  // allocated on another stream
     Arena& a = arena();
-    std::shared_lock<std::shared_mutex> lock(a.mu);
-    for (auto& kv : a.streams)
-      if (recycle(*kv.second, p)) return;
+    StreamArena* owner = nullptr;
+    {
+      std::shared_lock<std::shared_mutex> lock(a.mu);
+      for (auto& kv : a.streams) {
+        std::lock_guard<std::mutex> l2(kv.second->mu);
+        if (kv.second->live.count(p)) {
+          owner = kv.second.get();
+          break;
+        }
+      }
+    }
+    if (owner) {
+      // The owning stream's next allocation may hand this block out with no
+      // ordering against work still queued on the freeing stream: that work
+      // finishes first.
+      (void)hipStreamSynchronize(stream);
+      if (recycle(*owner, p)) return;
+    }
   }
   // Memory the caller allocated with hipMalloc (e.g. a plan's new_A).
   (void)hipStreamSynchronize(stream);
@@ -531,6 +559,21 @@ void sort_rows_if_needed(const DeviceCsrMatrix& A) {
   check(gpk_rows_unsorted(A.rowPtr, A.colVal, A.rows, f, c.stream()), "rows unsorted check");
   if (c.read_flags(1)[0] == 0) return;
   check(gpk_sort_rows_in_place(A.rowPtr, A.colVal, A.nzVal, A.rows, c.stream()), "sort rows");
+}
+
+void check_lp_arrays(const presolve::LPInfoGpu& lp, const char* who) {
+  if (lp.A.rows <= 0 && lp.AT.rows <= 0) return;
+  Ctx& c = ctx();
+  int32_t* f = c.flags();
+  check(gpk_csr_check(lp.A.rowPtr, lp.A.colVal, lp.A.rows, lp.A.cols, lp.A.nnz, f, c.stream()), "check A");
+  check(gpk_csr_check(lp.AT.rowPtr, lp.AT.colVal, lp.AT.rows, lp.AT.cols, lp.AT.nnz, f + 4, c.stream()), "check AT");
+  const int32_t* h = c.read_flags(8);
+  static const char* const what[3] = {"row pointers are not monotone from 0 to nnz",
+                                      "a column index is outside [0, cols)", "a row repeats a column index"};
+  for (int m = 0; m < 2; ++m)
+    for (int k = 0; k < 3; ++k)
+      if (h[4 * m + k])
+        throw InputError(std::string(who) + ": " + (m == 0 ? "A" : "AT") + ": " + what[k]);
 }
 
 void append_record(PostsolveTape& tape, PostsolveReductionType type, const int32_t* idx, size_t ni,

@@ -448,7 +448,11 @@ struct PostsolvePlan::Impl {
 };
 
 PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* original) : d_(new Impl) {
+  AbandonOnThrow abandon_guard;
   validate(record, original);
+  if (original != nullptr && original->A.rowPtr != nullptr && original->AT.rowPtr != nullptr &&
+      original->AT.rows == original->A.cols && original->AT.nnz == original->A.nnz)
+    check_lp_arrays(*original, "postsolve original LP");
   Impl& d = *d_;
   auto& c = ctx();
   hipStream_t s = c.stream();
@@ -555,6 +559,7 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
 
 GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, const PresolveRecordGpu& record,
                                  const LPInfoGpu* original_model, const double* original_l) {
+  amd::AbandonOnThrow abandon_guard;
   (void)original_l;
   amd::PostsolvePlan::validate(record, original_model);
   if (amd::job_mode().device_sync_on_entry) check(hipDeviceSynchronize(), "postsolve entry synchronize");

@@ -42,8 +42,15 @@ namespace gpu_presolver::amd {
 namespace {
 // kMinNnzForBatchedClosure (gpu_presolve.cu:2506).
 std::atomic<int32_t> g_closure_min_nnz{10000000};
+thread_local int32_t t_fail_after_updates = 0;
 }  // namespace
 void set_batched_closure_min_nnz(int32_t nnz) { g_closure_min_nnz.store(nnz); }
+void fail_after_record_updates(int32_t count) { t_fail_after_updates = count > 0 ? count : 0; }
+// Throws on the armed record update (after its host copies are queued).
+void record_update_checkpoint() {
+  if (t_fail_after_updates > 0 && --t_fail_after_updates == 0)
+    throw std::runtime_error("presolve: injected failure after a record update");
+}
 int32_t batched_closure_min_nnz() { return g_closure_min_nnz.load(); }
 }  // namespace gpu_presolver::amd
 
@@ -377,6 +384,7 @@ void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t
     }
     rs.n = n_new;
   }
+  amd::record_update_checkpoint();
 }
 
 // apply_plan_to_working_lp (gpu_presolve.cu:2021-2147).
@@ -546,8 +554,10 @@ class StepProfile {
 // patterns, so both values of enable_structure_specific_rules schedule the
 // same phases here).
 GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool keep_reduced) {
+  amd::AbandonOnThrow abandon_guard;
   validate_lp(lp);
   if (amd::job_mode().device_sync_on_entry) check(hipDeviceSynchronize(), "presolve entry synchronize");
+  amd::check_lp_arrays(lp, "presolve input");
   GpuPresolveSummary res;
   res.original_rows = lp.A.rows;
   res.original_cols = lp.A.cols;

@@ -82,6 +82,7 @@ std::vector<int32_t> masked_indices(const uint8_t* mask, int32_t n) {
 void compute_row_activity_summary(double* row_min_fin, double* row_max_fin, std::int32_t* row_min_neg_inf_count,
                                   std::int32_t* row_max_pos_inf_count, const DeviceCsrMatrix& A_csr, const double* l,
                                   const double* u, double zero_tol) {
+  amd::AbandonOnThrow abandon_guard;
   if (A_csr.rows == 0) return;
   row_activity(A_csr.rowPtr, A_csr.colVal, A_csr.nzVal, A_csr.rows, A_csr.cols, l, u, zero_tol, row_min_fin,
                row_max_fin, row_min_neg_inf_count, row_max_pos_inf_count, ctx().stream());
@@ -89,6 +90,7 @@ void compute_row_activity_summary(double* row_min_fin, double* row_max_fin, std:
 
 // gpu_presolve_kernels.cu:113-124
 void compute_col_max_abs(double* col_max_abs, const DeviceCsrMatrix& AT_csr) {
+  amd::AbandonOnThrow abandon_guard;
   if (AT_csr.rows == 0) return;
   check(gpk_col_max_abs(AT_csr.rowPtr, AT_csr.nzVal, AT_csr.rows, col_max_abs, ctx().stream()), "col max abs");
 }
@@ -98,6 +100,7 @@ double append_compact_fixed_col_tape_from_device(PostsolveTape* tape, const std:
                                                  const std::uint8_t* excluded_mask, const double* fixed_val,
                                                  const std::uint8_t* keep_row, const double* c,
                                                  const DeviceCsrMatrix& AT, std::int32_t n, const char* context) {
+  amd::AbandonOnThrow abandon_guard;
   (void)context;
   return amd::fixed_col_tape(tape, fixed_mask, excluded_mask, fixed_val, keep_row, c, AT, n);
 }
@@ -106,6 +109,7 @@ double append_compact_fixed_col_tape_from_device(PostsolveTape* tape, const std:
 // rule_empty_rows.cu:188-240
 void apply_rule_empty_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                            const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows;
   if (m == 0) return;
@@ -127,6 +131,7 @@ void apply_rule_empty_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Pre
 // rule_singleton_rows.cu:127-248
 void apply_rule_singleton_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0) return;
@@ -169,6 +174,7 @@ void apply_rule_singleton_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
 // rule_infeasible_redundant_rows.cu:119-243
 void apply_rule_infeasible_redundant_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                           const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows;
   if (m == 0) return;
@@ -210,6 +216,7 @@ void apply_rule_infeasible_redundant_rows(PresolvePlanGpu& plan, const LPInfoGpu
 // rule_implied_variable_bounds.cu:886-1164
 void apply_rule_implied_variable_bounds(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                         const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0 || n == 0) return;
@@ -289,6 +296,7 @@ void release_implied_variable_bounds_workspace() noexcept {}
 // rule_duplicate_rows.cu:424-613
 void apply_rule_duplicate_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   (void)stats;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows;
@@ -353,6 +361,7 @@ void apply_rule_duplicate_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
 
 // rule_infeasible_fixed_variables.cu:117-266
 void apply_rule_infeasible_fixed_variables(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t n = lp.A.cols, m = lp.A.rows;
   if (n == 0) return;
@@ -398,6 +407,7 @@ void apply_rule_infeasible_fixed_variables(PresolvePlanGpu& plan, const LPInfoGp
 // rule_empty_cols.cu:110-198
 void apply_rule_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                            const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t n = lp.A.cols;
   if (n == 0) return;
@@ -550,11 +560,13 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
 
 void apply_rule_column_singletons_eq(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                      const PresolveParams& prm, bool skip_precheck) {
+  amd::AbandonOnThrow abandon_guard;
   column_singletons_mode(plan, lp, stats, prm, true, skip_precheck);
 }
 
 void apply_rule_column_singletons_dual_infer(PresolvePlanGpu& plan, const LPInfoGpu& lp,
                                              const PresolveStatsGpu& stats, const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   column_singletons_mode(plan, lp, stats, prm, false, false);
 }
 
@@ -776,6 +788,7 @@ void dbl_scalar(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams
 // rule_doubleton_equations.cu:1679-1777 (batch driver) and 1913-2084
 void apply_rule_doubleton_equations(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                     const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   (void)stats;
   if (plan.has_infeasible || plan.has_unbounded) return;
   if (!prm.doubleton_equations_scan) {
@@ -832,6 +845,7 @@ void apply_rule_doubleton_equations(PresolvePlanGpu& plan, const LPInfoGpu& lp, 
 // rule_dual_fix.cu:174-554
 void apply_rule_dual_fix(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                          const PresolveParams& prm, const std::uint8_t* candidate_col_mask) {
+  amd::AbandonOnThrow abandon_guard;
   (void)stats;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t n = lp.A.cols, m = lp.A.rows;
@@ -923,6 +937,7 @@ void apply_rule_dual_fix(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Preso
 // rule_duplicate_columns.cu:95-842
 void apply_rule_duplicate_columns(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                   const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   (void)stats;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t n = lp.A.cols, m = lp.A.rows;
@@ -997,6 +1012,7 @@ void apply_rule_duplicate_columns(PresolvePlanGpu& plan, const LPInfoGpu& lp, co
 // rule_redundant_bounds.cu:31-411 (opt-in)
 void apply_rule_redundant_bounds(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                  const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
   (void)stats;
   if (plan.has_infeasible || plan.has_unbounded) return;
   const int32_t m = lp.A.rows, n = lp.A.cols;

@@ -173,6 +173,7 @@ class Server {
     r.t_start = now();
     Ctx& c = ctx();
     try {
+      AbandonOnThrow abandon_guard;
       GpuPresolveSummary sum = presolve::run_gpu_presolve_with_reduced_lp(j.lp, prm_);
       r.reduced_rows = sum.reduced_rows;
       r.reduced_cols = sum.reduced_cols;

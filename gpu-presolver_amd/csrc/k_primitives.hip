@@ -949,6 +949,46 @@ __global__ void k_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t 
   }
 }
 
+// Input check of a CSR matrix, one row per thread: flags[0] row pointers
+// that are not monotone or do not start at 0 / end at nnz, flags[1] a column
+// index outside [0, cols), flags[2] a column repeated within a row, flags[3]
+// a row whose columns are not ascending.  Sorted rows are checked for repeats
+// by neighbours; an unsorted row compares every pair of its entries.
+__global__ void k_csr_check(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t cols, int32_t nnz,
+                            int32_t* flags) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const int32_t b = ptr[i], e = ptr[i + 1];
+  if ((i == 0 && b != 0) || (i == rows - 1 && e != nnz) || b > e || b < 0 || e > nnz) {
+    wave_set_flag(flags);
+    return;
+  }
+  bool range = false, repeat = false, unsorted = false;
+  int32_t prev = -1;
+  for (int32_t p = b; p < e; ++p) {
+    const int32_t j = idx[p];
+    range = range || j < 0 || j >= cols;
+    if (p > b) {
+      repeat = repeat || j == prev;
+      unsorted = unsorted || j < prev;
+    }
+    prev = j;
+  }
+  if (unsorted && !repeat) {
+    for (int32_t p = b + 1; p < e && !repeat; ++p) {
+      const int32_t j = idx[p];
+      for (int32_t q = b; q < p; ++q)
+        if (idx[q] == j) {
+          repeat = true;
+          break;
+        }
+    }
+  }
+  if (range) wave_set_flag(flags + 1);
+  if (repeat) wave_set_flag(flags + 2);
+  if (unsorted) wave_set_flag(flags + 3);
+}
+
 // Stable insertion sort of one row per thread (used only for inputs whose
 // rows arrive unsorted).
 __global__ void k_sort_rows(const int32_t* ptr, int32_t* idx, double* val, int32_t rows) {
@@ -1338,6 +1378,13 @@ hipError_t gpk_transpose_gather(const int32_t* perm, const int32_t* row_of, cons
 hipError_t gpk_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t* flag, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_rows_unsorted, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, rows, flag);
+  return hipGetLastError();
+}
+
+hipError_t gpk_csr_check(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t cols, int32_t nnz,
+                         int32_t* flags, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_check, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, rows, cols, nnz, flags);
   return hipGetLastError();
 }
 

@@ -256,6 +256,12 @@ int32_t gp_has_good_nnz_progress(int32_t nnz_before, int32_t nnz_after, double r
 void gp_set_batched_closure_min_nnz(int32_t nnz);
 int32_t gp_batched_closure_min_nnz(void);
 
+/* Testing hook (no counterpart in the reference): the calling thread's
+ * count-th presolve record update from now fails with an internal error right
+ * after it has queued the record's device-to-host copies; 0 disarms it.  Used
+ * to check that a failed call leaves the thread's next call unaffected. */
+void gp_test_fail_after_record_updates(int32_t count);
+
 /* ---- postsolve (reference gpu_postsolve.hpp:16-27) --------------------- */
 /* postsolve_gpu with caller-provided outputs: x_org, z_org hold n0 doubles,
  * y_org m0 doubles (device).  x_red/y_red/z_red may be null.  `original`
