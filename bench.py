@@ -366,7 +366,7 @@ def bench_batched(args):
         "value": st["lp_per_s"], "unit": "LP/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f64", "data": "synthetic (seeded, dyadic values, planted reductions)",
         "config": {"workload": "batched presolve", "lps": len(dlps), "m": "logU[100,20000]", "n": "1.5 m",
-                   "row_len": "Poisson(6), >= 2", "sum_m": int(ms.sum()), "sum_nnz": int(nnz.sum()),
+                   "row_len": "Poisson(6), >= 1", "sum_m": int(ms.sum()), "sum_nnz": int(nnz.sum()),
                    "workers": best},
         "makespan_s": st["makespan_s"],
         "plain_one_at_a_time": {"value": plain, "unit": "LP/s", "seconds": plain_s},

@@ -336,9 +336,23 @@ class Server;
 Server* server_create(const presolve::PresolveParams& prm, const ::gp_serve_config& cfg);
 void server_destroy(Server* s);
 int64_t server_submit(Server* s, const presolve::LPInfoGpu& lp, uint64_t tag, double t_arrival);
+int64_t server_submit_spec(Server* s, const gpk_lpgen_spec& spec, uint64_t tag, double t_arrival);
+// Seeded LP generated on the device into blocks of the calling thread's pool
+// (gp_generate_lp); released with free_generated_lp on the same thread.
+void check_spec(const gpk_lpgen_spec& spec);
+presolve::LPInfoGpu generate_lp(const gpk_lpgen_spec& spec);
+void free_generated_lp(presolve::LPInfoGpu& lp) noexcept;
+// run_job on the LP of spec, generated first; *t_ready gets the host's
+// steady-clock seconds once the LP is resident.
+gp_job_result run_generated_job(const gpk_lpgen_spec& spec, const presolve::PresolveParams& prm, bool postsolve,
+                                uint64_t seed, uint64_t tag, double* t_ready);
 int server_wait(Server* s, ::gp_job_result* out, int max, double timeout_s);
 double server_now(const Server* s);
 uint64_t checksum_xyz(const double* x, int32_t n, const double* y, int32_t m, const double* z);
+// One presolve (+ seeded point + postsolve + checksum) job on the calling
+// thread's context, in the thread's current JobMode.
+gp_job_result run_job(const presolve::LPInfoGpu& lp, const presolve::PresolveParams& prm, bool postsolve,
+                      uint64_t seed, uint64_t tag);
 
 // Working-LP nnz from which each fast phase opens with the batched singleton
 // closure (process-wide; 10,000,000 as in the reference).
@@ -398,3 +412,11 @@ inline void append_record(presolve::PostsolveTape& tape, presolve::PostsolveRedu
 }
 
 }  // namespace gpu_presolver::amd
+
+namespace gpu_presolver::presolve {
+// apply_rule_doubleton_equations that also hands back the transpose (rows
+// sorted) of the A it rebuilds, in library blocks the caller frees with
+// free_csr; new_at->rowPtr stays null when A is not rebuilt.
+void apply_rule_doubleton_equations_with_at(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm,
+                                            DeviceCsrMatrix* new_at);
+}  // namespace gpu_presolver::presolve

@@ -131,6 +131,10 @@ gp_lp from_lp(const LPInfoGpu& lp) {
   return p;
 }
 
+gpk_lpgen_spec to_spec(const gp_lp_spec& q) {
+  return gpk_lpgen_spec{q.seed, q.m, q.n, q.len_lo, q.len_hi, q.mean_len, q.min_len};
+}
+
 PresolveParams to_params(const gp_params* q) {
   PresolveParams p;
   if (q == nullptr) return p;
@@ -569,6 +573,115 @@ gp_status gp_serve_trace(const gp_lp* lps, int32_t lp_count, const int32_t* lp_o
       const double wait = t - amd::server_now(srv.get());
       if (wait > 0.0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
       amd::server_submit(srv.get(), L[lp_of_job[k]], static_cast<uint64_t>(k), t);
+    }
+    std::vector<gp_job_result> buf(64);
+    for (int32_t got = 0; got < jobs;) {
+      const int n = amd::server_wait(srv.get(), buf.data(), static_cast<int>(buf.size()), 1.0);
+      for (int i = 0; i < n; ++i) results[buf[i].tag] = buf[i];
+      got += n;
+    }
+  });
+}
+
+gp_status gp_plain_job(const gp_lp* lp, const gp_params* params, uint64_t seed, uint64_t tag, int32_t postsolve,
+                       gp_job_result* out) {
+  return guarded([&] {
+    require(lp != nullptr && out != nullptr, "null argument");
+    const LPInfoGpu L = to_lp(*lp);
+    require_device();
+    const auto clock = [] {
+      return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    };
+    const double t0 = clock();
+    *out = amd::run_job(L, to_params(params), postsolve != 0, seed, tag);
+    out->job = 0;
+    out->worker = -1;
+    out->t_arrival = out->t_start = out->t_ready = t0;
+    out->t_end = clock();
+  });
+}
+
+gp_status gp_plain_job_spec(const gp_lp_spec* spec, const gp_params* params, uint64_t seed, uint64_t tag,
+                            int32_t postsolve, gp_job_result* out) {
+  return guarded([&] {
+    require(spec != nullptr && out != nullptr, "null argument");
+    const gpk_lpgen_spec sp = to_spec(*spec);
+    amd::check_spec(sp);
+    require_device();
+    const auto clock = [] {
+      return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    };
+    const double t0 = clock();
+    double t_ready = t0;
+    *out = amd::run_generated_job(sp, to_params(params), postsolve != 0, seed, tag, &t_ready);
+    out->job = 0;
+    out->worker = -1;
+    out->t_arrival = out->t_start = t0;
+    out->t_ready = t_ready;
+    out->t_end = clock();
+  });
+}
+
+gp_status gp_generate_lp(const gp_lp_spec* spec, gp_lp* out) {
+  return guarded([&] {
+    require(spec != nullptr && out != nullptr, "null argument");
+    const gpk_lpgen_spec sp = to_spec(*spec);
+    amd::check_spec(sp);
+    require_device();
+    LPInfoGpu lp = amd::generate_lp(sp);
+    try {
+      amd::ctx().sync("gp_generate_lp");
+    } catch (...) {
+      amd::free_generated_lp(lp);
+      throw;
+    }
+    *out = from_lp(lp);
+  });
+}
+
+gp_status gp_free_lp(gp_lp* lp) {
+  return guarded([&] {
+    if (lp == nullptr) return;
+    LPInfoGpu L;
+    L.A = to_csr(lp->A);
+    L.AT = to_csr(lp->AT);
+    L.c = lp->c;
+    L.l = lp->l;
+    L.u = lp->u;
+    L.AL = lp->AL;
+    L.AU = lp->AU;
+    amd::free_generated_lp(L);
+    amd::ctx().sync("gp_free_lp");
+    *lp = gp_lp{};
+  });
+}
+
+gp_status gp_server_submit_spec(gp_server* server, const gp_lp_spec* spec, uint64_t tag, double t_arrival,
+                                int64_t* job) {
+  return guarded([&] {
+    require(server != nullptr && spec != nullptr, "null argument");
+    const int64_t id = amd::server_submit_spec(server->impl, to_spec(*spec), tag, t_arrival);
+    if (job) *job = id;
+  });
+}
+
+gp_status gp_serve_spec_trace(const gp_lp_spec* specs, int32_t jobs, const double* arrival_s, const gp_params* params,
+                              const gp_serve_config* config, gp_job_result* results) {
+  return guarded([&] {
+    require(specs != nullptr && config != nullptr && results != nullptr, "null argument");
+    require(jobs >= 0 && config->workers >= 1, "invalid trace size or worker count");
+    for (int32_t k = 0; k < jobs; ++k) {
+      amd::check_spec(to_spec(specs[k]));
+      if (arrival_s) require(k == 0 || arrival_s[k] >= arrival_s[k - 1], "arrival times must be non-decreasing");
+    }
+    require_device();
+    std::unique_ptr<amd::Server, void (*)(amd::Server*)> srv(amd::server_create(to_params(params), *config),
+                                                             amd::server_destroy);
+    for (int32_t k = 0; k < jobs; ++k) {
+      const double t = arrival_s ? arrival_s[k] : 0.0;
+      const double wait = t - amd::server_now(srv.get());
+      if (wait > 0.0) std::this_thread::sleep_for(std::chrono::duration<double>(wait));
+      amd::server_submit_spec(srv.get(), to_spec(specs[k]), static_cast<uint64_t>(k), t);
     }
     std::vector<gp_job_result> buf(64);
     for (int32_t got = 0; got < jobs;) {

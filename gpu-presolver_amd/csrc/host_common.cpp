@@ -512,8 +512,10 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
   return out;
 }
 
-// Stable transpose: LSD radix sort of entries (already in row-major order)
-// by column, which leaves every AT row sorted by row index.
+// Stable transpose: LSD radix sort of the column indices (entries already in
+// row-major order, so every AT row comes out sorted by row index) carrying
+// the entry positions; the row pointers come from the sorted keys (no
+// per-column atomic counters), then one gather writes rows and values.
 DeviceCsrMatrix transpose_csr(const DeviceCsrMatrix& A) {
   Ctx& c = ctx();
   hipStream_t s = c.stream();
@@ -522,27 +524,27 @@ DeviceCsrMatrix transpose_csr(const DeviceCsrMatrix& A) {
   T.cols = A.rows;
   T.nnz = A.nnz;
   DBuf<int32_t> ptr(static_cast<int64_t>(A.cols) + 1, "transpose rowPtr");
-  DBuf<int32_t> counts(static_cast<int64_t>(A.cols) + 1, "transpose counts");
-  CopyBatch().zero(counts.get(), static_cast<int64_t>(A.cols) + 1).run("transpose counts");
-  if (A.nnz > 0) check(gpk_col_histogram(A.colVal, A.nnz, counts, s), "col histogram");
-  {
-    DBuf<int32_t> scratch(gpk_scan_scratch_ints(A.cols));
-    check(gpk_scan_exclusive_i32(counts, ptr, A.cols, scratch, s), "transpose scan");
-  }
   if (A.nnz == 0) {
+    check(gpk_col_bounds(nullptr, 0, A.cols, ptr, s), "transpose rowPtr");
     T.rowPtr = ptr.release();
     return T;
   }
   const int32_t nnz = A.nnz;
   int32_t bits = 1;
   while (bits < 31 && (int64_t{1} << bits) < A.cols) ++bits;
-  DBuf<uint32_t> keys(nnz), keys_alt(nnz);
-  DBuf<int32_t> perm(nnz), perm_alt(nnz), row_of(nnz);
-  check(hipMemcpyAsync(keys.get(), A.colVal, sizeof(int32_t) * nnz, hipMemcpyDeviceToDevice, s), "transpose keys");
-  check(gpk_iota_i32(perm, nnz, s), "transpose iota");
+  DBuf<uint32_t> ka(nnz), kb(nnz);
+  DBuf<int32_t> va(nnz), vb(nnz);
+  int32_t in_b = 0;
+  {
+    DBuf<int32_t> scratch(gpk_radix_scratch_ints(nnz));
+    check(gpk_transpose_sort(A.colVal, nnz, bits, ka, va, kb, vb, scratch, &in_b, s), "transpose sort");
+  }
+  const uint32_t* keys = in_b ? kb.get() : ka.get();
+  const int32_t* perm = in_b ? vb.get() : va.get();
+  check(gpk_col_bounds(keys, nnz, A.cols, ptr, s), "transpose rowPtr");
+  // the other key buffer holds the row of every entry
+  int32_t* row_of = reinterpret_cast<int32_t*>(in_b ? ka.get() : kb.get());
   check(gpk_expand_rows(A.rowPtr, A.rows, row_of, s), "expand rows");
-  DBuf<int32_t> scratch(gpk_radix_scratch_ints(nnz));
-  check(gpk_radix_sort_u32_i32(keys, perm, keys_alt, perm_alt, nnz, bits, scratch, s), "transpose sort");
   DBuf<int32_t> idx(nnz, "transpose colVal");
   DBuf<double> val(nnz, "transpose nzVal");
   check(gpk_transpose_gather(perm, row_of, A.nzVal, nnz, idx, val, s), "transpose gather");

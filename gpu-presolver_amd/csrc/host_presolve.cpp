@@ -388,7 +388,9 @@ void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t
 }
 
 // apply_plan_to_working_lp (gpu_presolve.cu:2021-2147).
-void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
+// new_at: the transpose of p.new_A (rows sorted) when the rule that rebuilt
+// A formed it, else null.
+void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs, const DeviceCsrMatrix* new_at = nullptr) {
   auto& c = ctx();
   hipStream_t s = c.stream();
   const int32_t m = cur.lp.A.rows, n = cur.lp.A.cols;
@@ -405,7 +407,13 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   const gpk_apply_maps_args maps{m, n, p.keep_row_mask, p.keep_col_mask, p.new_l, p.new_u, row_map, col_map,
                                  fixed.mask, fixed.map, counts, row_rank.get(), col_rank.get()};
   check(gpk_apply_maps(maps, c.lookback(gpk_apply_maps_tiles(m, n)), s), "keep maps");
-  const int32_t* h = c.read_flags(3);
+  // The caller's AT (first change) is compacted like a working AT when its
+  // rows are sorted; the flag comes back with the counts.
+  const bool callers_at = !cur.owns && !p.has_new_A;
+  if (callers_at && cur.lp.AT.rows > 0 && cur.lp.AT.nnz > 1)
+    check(gpk_rows_unsorted(cur.lp.AT.rowPtr, cur.lp.AT.colVal, cur.lp.AT.rows, counts + 3, s), "AT sorted check");
+  const int32_t* h = c.read_flags(4);
+  const bool callers_at_sorted = callers_at && h[3] == 0;
   const int32_t m_new = h[0], n_new = h[1];
   fixed.k = h[2];
   const double obj_new = cur.lp.obj_constant + p.obj_constant_delta;
@@ -424,12 +432,16 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs) {
   const DeviceCsrMatrix& src = p.has_new_A ? p.new_A : cur.lp.A;
   LPInfoGpu next;
   next.A = amd::compact_csr(src, row_map, col_map, m_new, n_new, -1, col_rank.get());
-  // A kept AT is compacted with the maps swapped; a rewritten A, or the
-  // caller's AT on the first change, is re-transposed.
-  if (p.has_new_A || !cur.owns) {
-    next.AT = amd::transpose_csr(next.A);
+  // An AT with sorted rows that is the exact transpose of the A being
+  // compacted (the working AT, the caller's AT when sorted, the rebuilding
+  // rule's AT of a new A) is compacted with the maps swapped, which gives
+  // the transpose of the compacted A; otherwise the compacted A is
+  // re-transposed.
+  const DeviceCsrMatrix* at_src = p.has_new_A ? new_at : (cur.owns || callers_at_sorted ? &cur.lp.AT : nullptr);
+  if (at_src != nullptr) {
+    next.AT = amd::compact_csr(*at_src, col_map, row_map, n_new, m_new, next.A.nnz, row_rank.get());
   } else {
-    next.AT = amd::compact_csr(cur.lp.AT, col_map, row_map, n_new, m_new, next.A.nnz, row_rank.get());
+    next.AT = amd::transpose_csr(next.A);
   }
   next.c = pool_f64(n_new, "working c");
   next.l = pool_f64(n_new, "working l");
@@ -591,15 +603,24 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     if (!has_limit) return false;
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count() >= prm.max_time;
   };
+  // The doubleton rule's transpose of the A it rebuilt, for the plan's
+  // application (freed after every plan).
+  struct OwnedCsr {
+    DeviceCsrMatrix m;
+    ~OwnedCsr() { amd::free_csr(m); }
+  } dbl_at;
   auto finish_plan = [&](PresolvePlanGpu& p) -> bool {
     res.has_infeasible = p.has_infeasible;
     res.has_unbounded = p.has_unbounded;
     terminal = p.has_infeasible || p.has_unbounded;
     const bool changed = !terminal && p.has_change;
     if (changed) {
-      prof.run("apply plan", [&] { apply_plan(cur, p, rs); });
+      const DeviceCsrMatrix* at = p.has_new_A && dbl_at.m.rowPtr ? &dbl_at.m : nullptr;
+      prof.run("apply plan", [&] { apply_plan(cur, p, rs, at); });
       invalidate_clean();
     }
+    amd::free_csr(dbl_at.m);
+    dbl_at.m = DeviceCsrMatrix{};
     return changed;
   };
 
@@ -654,7 +675,10 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     if (!time_exceeded() && cs_eq && prm.enable_column_singletons_eq)
       prof.run("rule column_singletons_eq", [&] { apply_rule_column_singletons_eq(p, v, st.s, prm, false); });
     if (!time_exceeded() && doubleton && prm.enable_doubleton_equations)
-      prof.run("rule doubleton_equations", [&] { apply_rule_doubleton_equations(p, v, st.s, prm); });
+      prof.run("rule doubleton_equations", [&] {
+        amd::free_csr(dbl_at.m);
+        apply_rule_doubleton_equations_with_at(p, v, prm, &dbl_at.m);
+      });
     if (!time_exceeded() && dual_fix && prm.enable_dual_fix)
       prof.run("rule dual_fix", [&] { apply_rule_dual_fix(p, v, st.s, prm, nullptr); });
     if (!time_exceeded() && dup_cols && prm.enable_duplicate_columns)

@@ -604,8 +604,52 @@ struct DblTapeRec {
 // One batch (rule_doubleton_equations.cu:1432-1677).  Returns true when the
 // plan changed; os/of receive the substitution and fixed-pair objective
 // deltas of the batch.
+// The transpose of A + delta, built from AT: the delta (rows sorted by
+// column) is transposed (it is small), then every row of AT is merged with
+// its row of the transposed delta by k_dbl_merge, the kernel that builds the
+// new A.  Per entry the same terms are added in the same order (the old
+// coefficient, then the delta terms in row-major delta order, which a stable
+// transpose keeps) and the same |value| <= zero_tol test drops it, so the
+// result is bit for bit the transpose of the merged A; nnz is known (the new
+// A's), so no count is read back.
+DeviceCsrMatrix merged_transpose(const DeviceCsrMatrix& AT, const int32_t* dptr, const int32_t* dcol,
+                                 const double* dval, int32_t total, int32_t m, double zt, int32_t nnz) {
+  auto& c = ctx();
+  hipStream_t s = c.stream();
+  const int32_t n = AT.rows;
+  const DeviceCsrMatrix delta{m, n, total, const_cast<int32_t*>(dptr), const_cast<int32_t*>(dcol),
+                              const_cast<double*>(dval)};
+  DeviceCsrMatrix dT = amd::transpose_csr(delta);
+  DeviceCsrMatrix out;
+  out.rows = n;
+  out.cols = m;
+  out.nnz = nnz;
+  try {
+    DBuf<int32_t> tptr(static_cast<int64_t>(n) + 1);
+    check(gpk_dbl_merge(csr_view(AT), dT.rowPtr, dT.colVal, dT.nzVal, zt, nullptr, tptr, nullptr, nullptr, s),
+          "doubleton AT merge count");
+    amd::scan_counts_launch(tptr, n, tptr);
+    DBuf<int32_t> tidx(nnz);
+    DBuf<double> tval(nnz);
+    check(gpk_dbl_merge(csr_view(AT), dT.rowPtr, dT.colVal, dT.nzVal, zt, tptr, nullptr, tidx, tval, s),
+          "doubleton AT merge");
+    out.rowPtr = tptr.release();
+    out.colVal = tidx.release();
+    out.nzVal = tval.release();
+  } catch (...) {
+    amd::free_csr(dT);
+    throw;
+  }
+  amd::free_csr(dT);
+  return out;
+}
+
+// at_out (when not null and A is rebuilt): receives the transpose of the new
+// A, formed as AT_src merged with the transposed delta.  AT_src must then be
+// the exact transpose of this round's A with sorted rows.
 bool dbl_batch_once(PresolvePlanGpu& plan, const LPInfoGpu& lp, const DeviceCsrMatrix& AT_src,
-                    const PresolveParams& prm, DblWork& w, std::vector<DblTapeRec>* recs, double* os, double* of) {
+                    const PresolveParams& prm, DblWork& w, std::vector<DblTapeRec>* recs, double* os, double* of,
+                    DeviceCsrMatrix* at_out) {
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0 || n == 0) return false;
   auto& c = ctx();
@@ -689,6 +733,7 @@ bool dbl_batch_once(PresolvePlanGpu& plan, const LPInfoGpu& lp, const DeviceCsrM
     DBuf<int32_t> oidx(out.nnz);
     DBuf<double> oval(out.nnz);
     check(gpk_dbl_merge(A, dptr, dcol, dval, zt, optr, nullptr, oidx, oval, s), "doubleton merge");
+    if (at_out) *at_out = merged_transpose(AT_src, dptr, dcol, dval, total, m, zt, out.nnz);
     out.rowPtr = optr.release();
     out.colVal = oidx.release();
     out.nzVal = oval.release();
@@ -788,8 +833,16 @@ void dbl_scalar(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams
 // rule_doubleton_equations.cu:1679-1777 (batch driver) and 1913-2084
 void apply_rule_doubleton_equations(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
                                     const PresolveParams& prm) {
-  amd::AbandonOnThrow abandon_guard;
   (void)stats;
+  DeviceCsrMatrix at;
+  apply_rule_doubleton_equations_with_at(plan, lp, prm, &at);
+  amd::free_csr(at);
+}
+
+void apply_rule_doubleton_equations_with_at(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm,
+                                            DeviceCsrMatrix* new_at) {
+  amd::AbandonOnThrow abandon_guard;
+  *new_at = DeviceCsrMatrix{};
   if (plan.has_infeasible || plan.has_unbounded) return;
   if (!prm.doubleton_equations_scan) {
     dbl_scalar(plan, lp, prm);
@@ -798,6 +851,10 @@ void apply_rule_doubleton_equations(PresolvePlanGpu& plan, const LPInfoGpu& lp, 
   const int32_t m = lp.A.rows, n = lp.A.cols;
   if (m == 0 || n == 0) return;
   amd::sort_rows_if_needed(lp.AT);
+  // The AT of each round's A: the working AT while A is the working A, then
+  // the merged transpose of every rebuilt A (a plan that arrives with a
+  // rebuilt A has no AT of it: those rounds transpose).
+  const bool merge = !plan.has_new_A;
   DeviceCsrMatrix at_own;
   bool owns_at = false;
   const DeviceCsrMatrix* at = &lp.AT;
@@ -817,12 +874,18 @@ void apply_rule_doubleton_equations(PresolvePlanGpu& plan, const LPInfoGpu& lp, 
         if (el.count() >= prm.doubleton_equations_max_time) break;
       }
       double os = 0.0, of = 0.0;
-      const bool changed = dbl_batch_once(plan, lp, *at, prm, w, prm.record_postsolve_tape ? &recs : nullptr, &os, &of);
-      if (plan.has_infeasible || plan.has_unbounded || !changed) break;
+      DeviceCsrMatrix merged;
+      const int32_t* a_before = plan.has_new_A ? plan.new_A.rowPtr : nullptr;
+      const bool changed = dbl_batch_once(plan, lp, *at, prm, w, prm.record_postsolve_tape ? &recs : nullptr, &os, &of,
+                                          merge ? &merged : nullptr);
+      if (plan.has_infeasible || plan.has_unbounded || !changed) {
+        amd::free_csr(merged);
+        break;
+      }
       objs.push_back(os);
       objs.push_back(of);
-      if (plan.has_new_A) {
-        DeviceCsrMatrix t = amd::transpose_csr(plan.new_A);
+      if (plan.has_new_A && plan.new_A.rowPtr != a_before) {  // A rebuilt this round
+        DeviceCsrMatrix t = merged.rowPtr ? merged : amd::transpose_csr(plan.new_A);
         if (owns_at) amd::free_csr(at_own);
         at_own = t;
         owns_at = true;
@@ -833,7 +896,10 @@ void apply_rule_doubleton_equations(PresolvePlanGpu& plan, const LPInfoGpu& lp, 
     if (owns_at) amd::free_csr(at_own);
     throw;
   }
-  if (owns_at) amd::free_csr(at_own);
+  if (owns_at) {
+    if (plan.has_new_A && !plan.has_infeasible && !plan.has_unbounded) *new_at = at_own;
+    else amd::free_csr(at_own);
+  }
   for (const DblTapeRec& r : recs)
     append_record(plan.tape, RT::DoubletonEquation, {r.i3[0], r.i3[1], r.i3[2], 0}, {r.v12[0], r.v12[1], r.v12[2],
                   r.v12[3], r.v12[4], r.v12[5], r.v12[6], r.v12[7], r.v12[8], r.v12[9], r.v12[10], r.v12[11]},

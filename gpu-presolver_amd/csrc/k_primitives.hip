@@ -875,7 +875,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
     int d = 0;
     if (valid) {
       kk[k] = kin[e];
-      vv[k] = vin[e];
+      vv[k] = vin ? vin[e] : static_cast<int32_t>(e);  // no values: the input positions
       d = static_cast<int>((kk[k] >> shift) & 255u);
     }
     dig[k] = d;
@@ -922,6 +922,17 @@ __global__ void k_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) row_of[p] = i;
+}
+
+// Row pointers of the transpose from its sorted column keys: position p
+// starts every column in (key[p-1], key[p]] (p = nnz closes the columns up
+// to n), so each pointer is written once and no counter is shared.
+__global__ void k_col_bounds(const uint32_t* keys, int32_t nnz, int32_t n, int32_t* ptr) {
+  const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p > nnz) return;
+  const int32_t hi = p < nnz ? static_cast<int32_t>(keys[p]) : n;
+  const int32_t lo = p > 0 ? static_cast<int32_t>(keys[p - 1]) : -1;
+  for (int32_t j = lo + 1; j <= hi; ++j) ptr[j] = p;
 }
 
 __global__ void k_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts) {
@@ -1353,6 +1364,56 @@ hipError_t gpk_radix_sort_u64_i32(uint64_t* keys, int32_t* vals, uint64_t* keys_
 hipError_t gpk_radix_sort_u32_i32(uint32_t* keys, int32_t* vals, uint32_t* keys_alt, int32_t* vals_alt, int32_t n,
                                   int32_t key_bits, int32_t* scratch, hipStream_t s) {
   return radix_impl<uint32_t>(keys, vals, keys_alt, vals_alt, n, key_bits, scratch, s);
+}
+
+hipError_t gpk_transpose_sort(const int32_t* col, int32_t nnz, int32_t key_bits, uint32_t* ka, int32_t* va,
+                              uint32_t* kb, int32_t* vb, int32_t* scratch, int32_t* result_in_b, hipStream_t s) {
+  *result_in_b = 0;
+  if (nnz <= 0) return hipSuccess;
+  if (nnz <= kSortSmallMax) {
+    hipError_t e = hipMemcpyAsync(ka, col, sizeof(uint32_t) * nnz, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_iota, dim3(grid_for(nnz)), dim3(kBlock), 0, s, va, nnz);
+    hipLaunchKernelGGL(k_sort_small<uint32_t>, dim3(1), dim3(kSortSmallBlock), 0, s, ka, va, nnz);
+    return hipGetLastError();
+  }
+  // Stable LSD passes; the first reads the column indices in place with the
+  // entry positions as values.
+  const int32_t tiles = (nnz + kRsTile - 1) / kRsTile;
+  int32_t* hist = scratch;
+  int32_t* scan_scratch = scratch + static_cast<int64_t>(256) * tiles + 1;
+  const int passes = key_bits <= 0 ? 1 : (key_bits + 7) / 8;
+  const bool from_hist = tiles <= kRsHistTilesMax;
+  const uint32_t* kin = reinterpret_cast<const uint32_t*>(col);
+  const int32_t* vin = nullptr;
+  uint32_t* kout = ka;
+  int32_t* vout = va;
+  for (int pass = 0; pass < passes; ++pass) {
+    const int shift = pass * 8;
+    hipLaunchKernelGGL(k_radix_hist<uint32_t>, dim3(tiles), dim3(kBlock), 0, s, kin, nnz, shift, tiles, hist);
+    if (from_hist) {
+      hipLaunchKernelGGL((k_radix_scatter<uint32_t, true>), dim3(tiles), dim3(kBlock), 0, s, kin, vin, kout, vout, nnz,
+                         shift, tiles, static_cast<const int32_t*>(hist));
+    } else {
+      hipError_t e = scan_impl<int32_t>(hist, hist, static_cast<int64_t>(256) * tiles, scan_scratch, s);
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL((k_radix_scatter<uint32_t, false>), dim3(tiles), dim3(kBlock), 0, s, kin, vin, kout, vout,
+                         nnz, shift, tiles, static_cast<const int32_t*>(hist));
+    }
+    kin = kout;
+    vin = vout;
+    const bool to_b = kout == ka;
+    kout = to_b ? kb : ka;
+    vout = to_b ? vb : va;
+  }
+  *result_in_b = kin == kb ? 1 : 0;
+  return hipGetLastError();
+}
+
+hipError_t gpk_col_bounds(const uint32_t* sorted_keys, int32_t nnz, int32_t n, int32_t* ptr, hipStream_t s) {
+  hipLaunchKernelGGL(k_col_bounds, dim3(grid_for(static_cast<int64_t>(nnz) + 1)), dim3(kBlock), 0, s, sorted_keys,
+                     nnz, n, ptr);
+  return hipGetLastError();
 }
 
 hipError_t gpk_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of, hipStream_t s) {

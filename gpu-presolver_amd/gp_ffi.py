@@ -106,6 +106,16 @@ def lib():
         L.gp_serve_trace.argtypes = [C.POINTER(T.GpLp), C.c_int32, vp, vp, C.c_int32, vp,
                                      C.POINTER(T.GpServeConfig), C.POINTER(T.GpJobResult)]
         L.gp_synthetic_point.argtypes = [C.c_uint64, C.POINTER(T.GpLp), vp, vp, vp]
+        L.gp_plain_job.argtypes = [C.POINTER(T.GpLp), vp, C.c_uint64, C.c_uint64, C.c_int32,
+                                   C.POINTER(T.GpJobResult)]
+        L.gp_plain_job_spec.argtypes = [C.POINTER(T.GpLpSpec), vp, C.c_uint64, C.c_uint64, C.c_int32,
+                                        C.POINTER(T.GpJobResult)]
+        L.gp_generate_lp.argtypes = [C.POINTER(T.GpLpSpec), C.POINTER(T.GpLp)]
+        L.gp_free_lp.argtypes = [C.POINTER(T.GpLp)]
+        L.gp_server_submit_spec.argtypes = [vp, C.POINTER(T.GpLpSpec), C.c_uint64, C.c_double,
+                                            C.POINTER(C.c_int64)]
+        L.gp_serve_spec_trace.argtypes = [C.POINTER(T.GpLpSpec), C.c_int32, vp, vp,
+                                          C.POINTER(T.GpServeConfig), C.POINTER(T.GpJobResult)]
         L.gp_checksum.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
@@ -546,6 +556,76 @@ class Server:
     def __del__(self):
         if _lib is not None:
             self.close()
+
+
+def plain_job(dlp, prm, seed, tag, postsolve=True):
+    """One serving job (presolve, seeded point seed ^ tag, postsolve,
+    checksum) through the plain synchronous entry points on this thread."""
+    out = T.GpJobResult()
+    _check(lib().gp_plain_job(C.byref(dlp.g), C.byref(prm), int(seed), int(tag), int(bool(postsolve)),
+                              C.byref(out)))
+    return _results_dict([out])[0]
+
+
+def lp_spec(seed, m, n=None, row_len=None, mean_len=0.0, min_len=1):
+    """gp_lp_spec: general rows of U[row_len] entries, or max(min_len,
+    Poisson(mean_len)) when mean_len > 0; n defaults to round(1.5 m)."""
+    n = int(round(1.5 * m)) if n is None else int(n)
+    lo, hi = row_len if row_len is not None else (0, 0)
+    return T.GpLpSpec(int(seed), int(m), n, int(lo), int(hi), float(mean_len), int(min_len))
+
+
+class GeneratedLp:
+    """An LP generated on the device from a gp_lp_spec (library memory,
+    released by gp_free_lp); .g is its gp_lp, as DeviceLp.g."""
+
+    def __init__(self, spec):
+        self.spec = spec
+        self.g = T.GpLp()
+        _check(lib().gp_generate_lp(C.byref(spec), C.byref(self.g)))
+        self.m, self.n = self.g.A.rows, self.g.A.cols
+
+    def host(self):
+        """The LP as a host dict (the layout of lp_synth)."""
+        g = self.g
+        d = {"m": g.A.rows, "n": g.A.cols, "obj": g.obj_constant}
+        for name, csr in (("A", g.A), ("AT", g.AT)):
+            d[name + "_ptr"] = download(csr.row_ptr, csr.rows + 1, np.int32)
+            d[name + "_idx"] = download(csr.col_idx, csr.nnz, np.int32)
+            d[name + "_val"] = download(csr.val, csr.nnz, np.float64)
+        for k, n in (("c", g.A.cols), ("l", g.A.cols), ("u", g.A.cols), ("AL", g.A.rows), ("AU", g.A.rows)):
+            d[k] = download(getattr(g, k), n, np.float64)
+        return d
+
+    def free(self):
+        if self.g is not None and _lib is not None:
+            _check(lib().gp_free_lp(C.byref(self.g)))
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def plain_job_spec(spec, prm, seed, tag, postsolve=True):
+    out = T.GpJobResult()
+    _check(lib().gp_plain_job_spec(C.byref(spec), C.byref(prm), int(seed), int(tag), int(bool(postsolve)),
+                                   C.byref(out)))
+    return _results_dict([out])[0]
+
+
+def serve_spec_trace(specs, prm, cfg, arrival_s=None):
+    """Job k generates specs[k] on its worker, arrives at arrival_s[k] (None:
+    all at once) and is presolved (+ postsolved); returns result dicts."""
+    jobs = len(specs)
+    arr_specs = (T.GpLpSpec * max(jobs, 1))(*specs)
+    arr = None if arrival_s is None else np.ascontiguousarray(arrival_s, dtype=np.float64)
+    res = (T.GpJobResult * max(jobs, 1))()
+    _check(lib().gp_serve_spec_trace(arr_specs, jobs, None if arr is None else arr.ctypes.data, C.byref(prm),
+                                     C.byref(cfg), res))
+    return _results_dict(res[:jobs])
 
 
 def synthetic_point(seed, reduced_g):

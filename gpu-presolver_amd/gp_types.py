@@ -107,7 +107,12 @@ class GpJobResult(C.Structure):
                 ("reduced_rows", C.c_int32), ("reduced_cols", C.c_int32), ("reduced_nnz", C.c_int32),
                 ("iterations", C.c_int32), ("has_infeasible", C.c_int32), ("has_unbounded", C.c_int32),
                 ("checksum", C.c_uint64), ("t_arrival", C.c_double), ("t_start", C.c_double),
-                ("t_end", C.c_double)]
+                ("t_end", C.c_double), ("t_ready", C.c_double)]
+
+
+class GpLpSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("m", C.c_int32), ("n", C.c_int32), ("len_lo", C.c_int32),
+                ("len_hi", C.c_int32), ("mean_len", C.c_double), ("min_len", C.c_int32)]
 
 
 class GpRecordView(C.Structure):

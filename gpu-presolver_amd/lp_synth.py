@@ -85,12 +85,13 @@ def _finish(rng, m, n, rows, cols, vals, c, l, u, x0, row_kind, dup_src, dup_fac
             "AL": AL.astype(np.float64), "AU": AU.astype(np.float64), "obj": 0.0}
 
 
-def generate(m, n, seed, row_len=(2, 10), mean_row_len=None, zipf=None, zipf_cap=5000,
+def generate(m, n, seed, row_len=(2, 10), mean_row_len=None, min_row_len=2, zipf=None, zipf_cap=5000,
              frac_singleton_rows=0.05, frac_singleton_cols=0.05, frac_doubleton=0.03,
              frac_dup_rows=0.02, frac_fixed_cols=0.02, frac_empty_cols=0.01,
              frac_eq=0.30, frac_one_sided=0.10, frac_inf_u=0.20):
     """General planted LP.  Row lengths of general rows: Poisson(mean_row_len)
-    clipped to >= 2 when mean_row_len is given, else uniform in row_len.
+    clipped to >= min_row_len when mean_row_len is given, else uniform in
+    row_len.
     Column choice is uniform, or Zipf(zipf) popularity with a per-column cap."""
     rng = np.random.Generator(np.random.PCG64(seed))
     c = _dyadic(rng.normal(0.0, 1.0, size=n))
@@ -131,7 +132,7 @@ def generate(m, n, seed, row_len=(2, 10), mean_row_len=None, zipf=None, zipf_cap
 
     # General rows
     if mean_row_len is not None:
-        lens = np.maximum(2, rng.poisson(mean_row_len, size=len(grows)))
+        lens = np.maximum(min_row_len, rng.poisson(mean_row_len, size=len(grows)))
     else:
         lens = rng.integers(row_len[0], row_len[1] + 1, size=len(grows))
     total = int(lens.sum())

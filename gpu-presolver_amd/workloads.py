@@ -2,7 +2,8 @@
 arrival traces, latency statistics and the worker-count autotuner.
 
   batched  4,096 independent LPs, m log-uniform in [100, 20,000], n = 1.5 m,
-           mean row length 6, presolved together (all arrive at once);
+           row lengths Poisson(6) with minimum 1, presolved together (all
+           arrive at once);
   serving  Poisson arrivals (200 requests/s) of presolve + postsolve jobs,
            m log-uniform in [1,000, 500,000], n = 1.5 m, 4-10 nonzeros per row.
 
@@ -42,7 +43,7 @@ def generate_pool(sizes, seed, workers=None, **kw):
 def batched_pool(count=4096, seed=3, workers=None):
     rng = np.random.default_rng(seed)
     sizes = log_uniform_sizes(rng, count, 100, 20_000)
-    return generate_pool(sizes, seed, workers, mean_row_len=6.0)
+    return generate_pool(sizes, seed, workers, mean_row_len=6.0, min_row_len=1)
 
 
 def serving_pool(count=48, seed=4, workers=None):

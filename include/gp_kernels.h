@@ -207,6 +207,13 @@ hipError_t gpk_radix_sort_u32_i32(uint32_t* keys, int32_t* vals, uint32_t* keys_
 /* ---- transpose -------------------------------------------------------------
  * Expands the row index of every entry (row_of[p] = i), for use as the sort
  * payload of a transpose, and the final gather of the transpose. */
+/* Transpose building blocks: stable sort of the column indices of A (read
+ * in place) carrying the entry positions, key_bits wide, through ka/va and
+ * kb/vb (*result_in_b tells which pair holds the result); row pointers of
+ * the transpose (n + 1) from the sorted keys. */
+hipError_t gpk_transpose_sort(const int32_t* col, int32_t nnz, int32_t key_bits, uint32_t* ka, int32_t* va,
+                              uint32_t* kb, int32_t* vb, int32_t* scratch, int32_t* result_in_b, hipStream_t stream);
+hipError_t gpk_col_bounds(const uint32_t* sorted_keys, int32_t nnz, int32_t n, int32_t* ptr, hipStream_t stream);
 hipError_t gpk_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of, hipStream_t stream);
 hipError_t gpk_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts, hipStream_t stream);
 hipError_t gpk_transpose_gather(const int32_t* perm, const int32_t* row_of, const double* val,
@@ -276,6 +283,30 @@ hipError_t gpk_phase_init(int32_t m, int32_t n, gpk_plan plan, const double* c, 
 hipError_t gpk_synthetic_point(uint64_t seed, int32_t n, int32_t m, const double* l, const double* u, double* x,
                                double* y, double* z, hipStream_t stream);
 hipError_t gpk_checksum_f64(const double* v, int32_t n, uint64_t salt, unsigned long long* acc, hipStream_t stream);
+
+/* Seeded LP generator (k_lpgen.hip).  Steps: columns (class, c, l, u, x0);
+ * row classes, kinds and raw entry counts (extra: zeroed m words); after an
+ * exclusive scan of raw_count into raw_ptr, the raw entries (own, cursor:
+ * zeroed m words), per-row column sort and the final row lengths len; after
+ * a scan of len into ptr, the CSR and the row bounds. */
+typedef struct gpk_lpgen_spec {
+  uint64_t seed;
+  int32_t m, n;
+  int32_t len_lo, len_hi; /* general row length U[len_lo, len_hi] when mean_len <= 0 */
+  double mean_len;        /* > 0: general row length max(min_len, Poisson(mean_len)) */
+  int32_t min_len;
+} gpk_lpgen_spec;
+hipError_t gpk_lpgen_columns(gpk_lpgen_spec spec, uint8_t* col_class, double* c, double* l, double* u, double* x0,
+                             hipStream_t stream);
+hipError_t gpk_lpgen_row_counts(gpk_lpgen_spec spec, const uint8_t* col_class, uint8_t* row_class, uint8_t* kind,
+                                int32_t* raw_count, int32_t* scol_row, int32_t* extra, hipStream_t stream);
+hipError_t gpk_lpgen_fill(gpk_lpgen_spec spec, const uint8_t* col_class, const uint8_t* row_class,
+                          const int32_t* raw_ptr, const int32_t* scol_row, int32_t* own, int32_t* cursor,
+                          int32_t* raw_col, double* raw_val, int32_t* unique, int32_t* src, int32_t* len,
+                          hipStream_t stream);
+hipError_t gpk_lpgen_write(gpk_lpgen_spec spec, const uint8_t* row_class, const uint8_t* kind, const int32_t* raw_ptr,
+                           const int32_t* raw_col, const double* raw_val, const int32_t* src, const int32_t* ptr,
+                           int32_t* idx, double* val, const double* x0, double* AL, double* AU, hipStream_t stream);
 
 /* Batched singleton closure (reference gpu_presolve.cu:98-247, 2506-2831).
  * gpk_closure_degrees: row_nnz (col_nnz) lose the live entries of columns

@@ -317,7 +317,29 @@ typedef struct gp_job_result {
   int32_t has_infeasible, has_unbounded;
   uint64_t checksum;                 /* 0 without postsolve */
   double t_arrival, t_start, t_end;  /* seconds since the server was created */
+  double t_ready; /* the job's LP resident: t_start, or the end of its generation (gp_lp_spec jobs) */
 } gp_job_result;
+
+/* A seeded synthetic LP, generated on the device (no counterpart in the
+ * reference): the same spec gives the same LP bit for bit.  Dyadic values
+ * (coefficients +-U[0.5,2] rounded to 1/8; c ~ N(0,1), u ~ U[1,10], the
+ * feasible point x0 and the row-bound offsets U[0,1], rounded to 2^-10);
+ * l = 0, u = +inf on 20% of the columns (only where c >= 0); 1% empty, 2%
+ * fixed and 5% singleton columns; 5% singleton rows, 3% doubleton
+ * equalities, 2% scaled duplicates of general rows; 30% equality and 10%
+ * one-sided rows.  General rows have U[len_lo, len_hi] entries, or
+ * max(min_len, Poisson(mean_len)) when mean_len > 0. */
+typedef struct gp_lp_spec {
+  uint64_t seed;
+  int32_t m, n;
+  int32_t len_lo, len_hi;
+  double mean_len;
+  int32_t min_len;
+} gp_lp_spec;
+/* Generates the LP into device memory owned by the library (A, AT rows
+ * sorted, c, AL, AU, l, u); release it with gp_free_lp. */
+gp_status gp_generate_lp(const gp_lp_spec* spec, gp_lp* out);
+gp_status gp_free_lp(gp_lp* lp);
 
 typedef struct gp_server gp_server;
 gp_status gp_server_create(const gp_params* params, const gp_serve_config* config, gp_server** out);
@@ -330,12 +352,31 @@ double gp_server_now(const gp_server* server);
 /* Finishes every queued job, then stops the workers. */
 void gp_server_destroy(gp_server* server);
 
+/* Queues a job whose LP the worker first generates from `spec` on its own
+ * stream (released after the job); t_ready marks the end of the generation. */
+gp_status gp_server_submit_spec(gp_server* server, const gp_lp_spec* spec, uint64_t tag, double t_arrival,
+                                int64_t* job);
+/* The arrival trace form of gp_server_submit_spec: job k generates
+ * specs[k], arrives at arrival_s[k] (NULL: all at once) and has tag k. */
+gp_status gp_serve_spec_trace(const gp_lp_spec* specs, int32_t jobs, const double* arrival_s, const gp_params* params,
+                              const gp_serve_config* config, gp_job_result* results);
 /* A whole arrival trace on a fresh server: job k runs lps[lp_of_job[k]] and
  * arrives arrival_s[k] seconds after the start (non-decreasing; NULL: all at
  * once, the batched mode); its tag is k and results[k] receives its result. */
 gp_status gp_serve_trace(const gp_lp* lps, int32_t lp_count, const int32_t* lp_of_job, const double* arrival_s,
                          int32_t jobs, const gp_params* params, const gp_serve_config* config,
                          gp_job_result* results);
+
+/* One job exactly as a serving worker runs it (seed ^ tag for the reduced
+ * point), but through the plain synchronous entry points on the calling
+ * thread: the one-at-a-time baseline of the batched and serving modes, and
+ * the reference result a served job is compared with.  out->worker is -1;
+ * times are seconds on the host's steady clock around the call. */
+gp_status gp_plain_job(const gp_lp* lp, const gp_params* params, uint64_t seed, uint64_t tag, int32_t postsolve,
+                       gp_job_result* out);
+/* gp_plain_job on the LP of `spec`, generated first on the calling thread. */
+gp_status gp_plain_job_spec(const gp_lp_spec* spec, const gp_params* params, uint64_t seed, uint64_t tag,
+                            int32_t postsolve, gp_job_result* out);
 
 /* The seeded reduced-space point of a serving job: x in [l, u] on the 2^-10
  * grid, y and z uniform in [-1, 1); lp is the reduced LP (device). */
