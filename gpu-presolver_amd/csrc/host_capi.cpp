@@ -308,6 +308,29 @@ gp_status gp_row_activity(const gp_csr* A, const double* l, const double* u, dou
   });
 }
 
+gp_status gp_row_activity_noalloc(const gp_csr* A, const double* l, const double* u, double zero_tol,
+                                  double* row_min_fin, double* row_max_fin, int32_t* row_min_neg_inf_count,
+                                  int32_t* row_max_pos_inf_count, void* stream) {
+  // No library context, pool or staging area is touched here: nothing in
+  // this call allocates.
+  return guarded([&] {
+    require(A != nullptr, "A is null");
+    check_csr(*A, "A");
+    require(A->rows == 0 || (row_min_fin && row_max_fin && row_min_neg_inf_count && row_max_pos_inf_count),
+            "null output");
+    require(A->cols == 0 || (l && u), "null bounds");
+    require(A->cols <= GP_ROW_ACTIVITY_NOALLOC_MAX_COLS,
+            "gp_row_activity_noalloc: more columns than its fixed staging array holds (2,097,152)");
+    if (A->rows == 0) return;
+    require_device();
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    amd::check(gpk_row_activity_noalloc(A->row_ptr, A->col_idx, A->val, A->rows, A->cols, l, u, zero_tol,
+                                        row_min_fin, row_max_fin, row_min_neg_inf_count, row_max_pos_inf_count, s),
+               "gp_row_activity_noalloc");
+    if (s == nullptr) amd::check(hipStreamSynchronize(nullptr), "gp_row_activity_noalloc synchronize");
+  });
+}
+
 gp_status gp_col_max_abs(const gp_csr* AT, double* col_max_abs, void* stream) {
   return guarded([&] {
     require(AT != nullptr, "AT is null");

@@ -76,6 +76,7 @@ def lib():
         L.gp_params_default.argtypes = [C.POINTER(T.GpParams)]
         L.gp_device_free.argtypes = [vp]
         L.gp_row_activity.argtypes = [C.POINTER(T.GpCsr), vp, vp, C.c_double, vp, vp, vp, vp, vp]
+        L.gp_row_activity_noalloc.argtypes = [C.POINTER(T.GpCsr), vp, vp, C.c_double, vp, vp, vp, vp, vp]
         L.gp_col_max_abs.argtypes = [C.POINTER(T.GpCsr), vp, vp]
         L.gp_compact_csr.argtypes = [C.POINTER(T.GpCsr), vp, vp, C.POINTER(T.GpCsr)]
         L.gp_transpose_csr.argtypes = [C.POINTER(T.GpCsr), C.POINTER(T.GpCsr)]

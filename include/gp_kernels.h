@@ -155,6 +155,13 @@ hipError_t gpk_row_activity_lu(const int32_t* row_ptr, const int32_t* col_idx, c
                                int32_t* pos_inf, hipStream_t stream);
 hipError_t gpk_col_max_abs(const int32_t* at_ptr, const double* at_val, int32_t n, double* out,
                            hipStream_t stream);
+/* Row activity with the (l, u) pairs staged in a fixed file-scope device
+ * array (k_noalloc.hip) instead of a pool block: no allocation.  n at most
+ * GP_ROW_ACTIVITY_NOALLOC_MAX_COLS (hipErrorInvalidValue beyond).  Calls that
+ * overlap on one GPU share that array and race. */
+hipError_t gpk_row_activity_noalloc(const int32_t* row_ptr, const int32_t* col_idx, const double* val, int32_t m,
+                                    int32_t n, const double* l, const double* u, double ztol, double* min_fin,
+                                    double* max_fin, int32_t* neg_inf, int32_t* pos_inf, hipStream_t stream);
 /* Row stats of a CSR (counts include explicit zeros; singleton = first entry). */
 hipError_t gpk_csr_stats(const int32_t* ptr, const int32_t* idx, const double* val, int32_t rows,
                          int32_t* nnz_out, uint8_t* empty_mask, uint8_t* singleton_mask,

@@ -211,6 +211,19 @@ gp_status gp_row_activity(const gp_csr* A, const double* l, const double* u, dou
                           double* row_min_fin, double* row_max_fin,
                           int32_t* row_min_neg_inf_count, int32_t* row_max_pos_inf_count,
                           void* stream);
+/* No counterpart in the reference.  gp_row_activity without any allocation,
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, serving loops that forbid hipMalloc): the staging of the (l, u)
+ * pairs uses a fixed 32 MiB device array of the library (one per GPU, held
+ * until the process ends) instead of a pool block.  A->cols must be at most
+ * GP_ROW_ACTIVITY_NOALLOC_MAX_COLS, else GP_ERR_INVALID_ARGUMENT.  `stream` NULL: the null stream, and the call
+ * returns when the results are written; otherwise the kernels are only
+ * enqueued.  Calls that overlap on one GPU share the array and race; the
+ * results are bit-identical to gp_row_activity's. */
+gp_status gp_row_activity_noalloc(const gp_csr* A, const double* l, const double* u, double zero_tol,
+                                  double* row_min_fin, double* row_max_fin, int32_t* row_min_neg_inf_count,
+                                  int32_t* row_max_pos_inf_count, void* stream);
+#define GP_ROW_ACTIVITY_NOALLOC_MAX_COLS 2097152
 /* Replaces compute_col_max_abs (reference gpu_presolve_kernels.hpp:16); `stream` as above. */
 gp_status gp_col_max_abs(const gp_csr* AT, double* col_max_abs, void* stream);
 /* Mask compaction of a CSR matrix and its re-transpose, the apply step of the
