@@ -18,8 +18,19 @@ Extra objects on the JSON line:
                      stream: k_pack_lu then k_row_activity_lu), algorithmic
                      bytes 4(m+1) + 12 nnz + 16 n + 24 m, with both kernels
                      also timed apart;
-  cpu_baseline       the serial CPU oracle presolving the same LP once on the
-                     host (single thread).
+  roofline_transpose the transpose of the full A (three LSD sort passes over
+                     the column indices, row pointers from the sorted keys,
+                     one gather), algorithmic bytes
+                     4(m+1) + 12 nnz + 4(n+1) + 12 nnz + 8 n;
+  cpu_baseline       the C++ oracle presolving the same LP on the host:
+                     single thread and the OpenMP build on every usable core,
+                     median of five runs after a warm-up, with the core count
+                     and CPU model;
+  batched            4,096 LPs generated on the device, presolved together,
+                     against one-at-a-time plain presolve (LP/s, speedup);
+  serving            60 s of Poisson(200/s) presolve + postsolve requests whose
+                     LPs are generated per request (p50/p99, LP/s), with the
+                     capacity against serial plain processing.
 """
 import argparse
 import json
@@ -76,7 +87,8 @@ def kernel_rooflines(gp_ffi, dlp, lp, launches):
     act["kernels"] = measure_activity_kernels(gp_ffi, dlp, lp, stream, launches, mn.numpy(), mx.numpy())
     comp = measure_compaction(gp_ffi, dlp, lp, stream, launches)
     log(f"compaction: {comp['us']:.1f} us/call, {comp['achieved']:.0f} GB/s")
-    return act, comp
+    tr = measure_transpose(gp_ffi, dlp, lp, stream, max(5, launches // 5))
+    return act, comp, tr
 
 
 def measure_activity_kernels(gp_ffi, dlp, lp, stream, launches, want_mn, want_mx):
@@ -277,6 +289,78 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
                         f"-> {m_new} rows, {nnz_new} nnz"}
 
 
+def transpose_bytes(m, n, nnz):
+    """Algorithmic bytes of one transpose of an m x n CSR: read the row
+    pointers and entries, write the transpose's row pointers and entries,
+    and the column counts and their scan (8 n)."""
+    return 4 * (m + 1) + 12 * nnz + 4 * (n + 1) + 12 * nnz + 8 * n
+
+
+def measure_transpose(gp_ffi, dlp, lp, stream, launches):
+    """Times the library's transpose (stable sort of the column indices
+    carrying entry positions, row pointers from the sorted keys, rows of the
+    entries, one gather of rows and values) on the full A, back to back on
+    one stream, each kernel group also timed apart; the result must equal
+    the LP's own AT."""
+    import ctypes as C
+
+    import numpy as np
+
+    m, n, nnz = lp["m"], lp["n"], len(lp["A_idx"])
+    L = gp_ffi.lib()
+    vp = C.c_void_p
+    L.gpk_radix_scratch_ints.restype = C.c_int64
+    L.gpk_radix_scratch_ints.argtypes = [C.c_int64]
+    L.gpk_transpose_sort.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.POINTER(C.c_int32), vp]
+    L.gpk_col_bounds.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
+    L.gpk_expand_rows.argtypes = [vp, C.c_int32, vp, vp]
+    L.gpk_transpose_gather.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp]
+    bits = max(1, int(n - 1).bit_length())
+    ka, kb = gp_ffi.DeviceArray(size=nnz, dtype=np.uint32), gp_ffi.DeviceArray(size=nnz, dtype=np.uint32)
+    va, vb = gp_ffi.DeviceArray(size=nnz, dtype=np.int32), gp_ffi.DeviceArray(size=nnz, dtype=np.int32)
+    scratch = gp_ffi.DeviceArray(size=L.gpk_radix_scratch_ints(nnz), dtype=np.int32)
+    ptr = gp_ffi.DeviceArray(size=n + 1, dtype=np.int32)
+    idx, val = gp_ffi.DeviceArray(size=nnz, dtype=np.int32), gp_ffi.DeviceArray(size=nnz)
+    A = dlp.A.g
+    in_b = C.c_int32(0)
+    ck = gp_ffi.hip_check
+
+    def sort():
+        ck(L.gpk_transpose_sort(A.col_idx, nnz, bits, ka.ptr, va.ptr, kb.ptr, vb.ptr, scratch.ptr, C.byref(in_b),
+                                stream.ptr), "transpose sort")
+
+    def finish():
+        keys, perm, other = (kb, vb, ka) if in_b.value else (ka, va, kb)
+        ck(L.gpk_col_bounds(keys.ptr, nnz, n, ptr.ptr, stream.ptr), "col bounds")
+        ck(L.gpk_expand_rows(A.row_ptr, m, other.ptr, stream.ptr), "expand rows")
+        ck(L.gpk_transpose_gather(perm.ptr, other.ptr, A.val, nnz, idx.ptr, val.ptr, stream.ptr), "gather")
+
+    for _ in range(2):
+        sort()
+        finish()
+    stream.sync()
+    e = [gp_ffi.Event() for _ in range(3)]
+    e[0].record(stream)
+    for _ in range(launches):
+        sort()
+        finish()
+    e[1].record(stream)
+    for _ in range(launches):
+        sort()
+    e[2].record(stream)
+    us = 1e3 * e[0].elapsed_ms(e[1]) / launches
+    sort_us = 1e3 * e[1].elapsed_ms(e[2]) / launches
+    if not (np.array_equal(ptr.numpy(), lp["AT_ptr"]) and np.array_equal(idx.numpy(), lp["AT_idx"])
+            and np.array_equal(val.numpy(), lp["AT_val"])):
+        raise SystemExit("bench.py: the transpose differs from the LP's AT")
+    b = transpose_bytes(m, n, nnz)
+    log(f"transpose: {us:.1f} us/call ({b / (us * 1e-6) / 1e9:.0f} GB/s); sort {sort_us:.1f} us, "
+        f"bounds + rows + gather {us - sort_us:.1f} us")
+    return {"us": us, "bytes": b, "achieved": b / (us * 1e-6) / 1e9,
+            "kernels_us": {"sort (3 LSD passes)": sort_us, "bounds + rows + gather": us - sort_us},
+            "workload": f"A of the LP ({m}x{n}, {nnz} nnz) -> AT ({n}x{m})"}
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "compaction_row_activity_hbm_traffic.md")
 
 
@@ -301,6 +385,15 @@ def activity_traffic():
     the committed PMC summary, or None."""
     parts = [profiled_traffic(k) for k in ("k_pack_lu", "k_row_activity_lu")]
     return None if any(p is None for p in parts) else sum(parts)
+
+
+def transpose_traffic():
+    """HBM bytes of one transpose from the committed PMC summary, or None."""
+    parts = [profiled_traffic(k) for k in ("k_radix_hist<unsigned int>", "k_radix_scatter<unsigned int, false>",
+                                           "k_col_bounds", "k_expand_rows", "k_transpose_gather")]
+    if any(p is None for p in parts):
+        return None
+    return 3 * (parts[0] + parts[1]) + sum(parts[2:])
 
 
 def compaction_traffic():
@@ -457,6 +550,179 @@ def bench_small(args):
                                  "nnz": int(len(lp["A_idx"]))}}), flush=True)
 
 
+def _sizes(seed, count, lo, hi):
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size=count)).astype(np.int64)
+
+
+def many_batched(gp_ffi, args):
+    """The batched workload on LPs generated on the device: 4,096 LPs,
+    m log-uniform in [100, 20,000], n = 1.5 m, row lengths Poisson(6) with
+    minimum 1, all submitted at once to the serving runtime (presolve only;
+    worker count from a 256-LP sample), against the same LPs presolved one
+    at a time through the plain synchronous entry points."""
+    import numpy as np
+
+    import workloads
+
+    sizes = _sizes(3, args.batch_lps, 100, 20_000)
+    t = time.time()
+    specs = [gp_ffi.lp_spec(0xB000 + k, int(m), mean_len=6.0, min_len=1) for k, m in enumerate(sizes)]
+    lps = [gp_ffi.GeneratedLp(s) for s in specs]
+    nnz = sum(g.g.A.nnz for g in lps)
+    log(f"batched: generated {len(lps)} LPs on the device (sum m {int(sizes.sum())}, sum nnz {nnz}) "
+        f"in {time.time() - t:.1f}s")
+    prm = gp_ffi.params(enable_structure_specific_rules=0)
+    sample = np.random.default_rng(11).choice(len(lps), size=min(256, len(lps)), replace=False).astype(np.int32)
+
+    def run(w, jobs=sample):
+        return gp_ffi.serve_trace(lps, jobs, prm, gp_ffi.serve_config(workers=w, postsolve=False))
+
+    run(4, sample[:32])
+    best, table = workloads.autotune_workers(run, _worker_candidates(args))
+    res = run(best, np.arange(len(lps), dtype=np.int32))
+    if any(r["status"] != 0 for r in res):
+        raise SystemExit("bench.py: batched jobs failed")
+    st = workloads.latency_stats(res)
+    t = time.perf_counter()
+    for k, g in enumerate(lps):
+        p = gp_ffi.plain_job(g, prm, 0, k, postsolve=False)
+        if (p["status"], p["reduced_rows"], p["reduced_nnz"]) != (0, res[k]["reduced_rows"], res[k]["reduced_nnz"]):
+            raise SystemExit(f"bench.py: batched job {k} differs from its plain run")
+    plain_s = time.perf_counter() - t
+    for g in lps:
+        g.free()
+    plain = len(lps) / plain_s
+    log(f"batched: {st['lp_per_s']:.0f} LP/s with {best} workers (autotune {table}); plain one at a time "
+        f"{plain:.0f} LP/s -> {st['lp_per_s'] / plain:.2f}x")
+    return {"value": st["lp_per_s"], "unit": "LP/s", "lps": len(lps), "sum_m": int(sizes.sum()), "sum_nnz": int(nnz),
+            "m": "logU[100,20000]", "n": "1.5 m", "row_len": "Poisson(6), >= 1", "workers": best,
+            "makespan_s": st["makespan_s"], "plain_one_at_a_time_lp_per_s": plain,
+            "speedup_vs_plain": st["lp_per_s"] / plain, "autotune_lp_per_s": {str(k): v for k, v in table.items()},
+            "check": "every job's reduced dimensions equal its plain run's"}
+
+
+def many_serving(gp_ffi, args):
+    """The serving workload: Poisson(rate) presolve + postsolve requests for
+    --serve-seconds, each generating its own LP on a worker from the request
+    size distribution (m log-uniform in [1,000, 500,000], n = 1.5 m, 4-10
+    entries per general row), admitted under a 200 GB budget; p50/p99
+    latency (arrival to result, generation included) and LP/s.  Beside it:
+    the capacity with a sample of requests all at once, and serial plain
+    processing (gp_plain_job_spec, one at a time) of the same requests,
+    both as capacity and replayed against the trace's first arrivals."""
+    import numpy as np
+
+    import workloads
+
+    rate, secs = args.serve_rate, args.serve_seconds
+    arrival, _ = workloads.poisson_trace(rate, secs, 1, seed=5)
+    sizes = _sizes(4, len(arrival), 1_000, 500_000)
+    specs = [gp_ffi.lp_spec(0xC000 + k, int(m), row_len=(4, 10)) for k, m in enumerate(sizes)]
+    prm = gp_ffi.params(enable_structure_specific_rules=0)
+    budget = 200e9
+    cfg = gp_ffi.serve_config(workers=args.serve_workers, postsolve=True, hbm_budget_bytes=budget, seed=99)
+    gp_ffi.serve_spec_trace(specs[:16], prm, cfg)  # warm-up
+    k_cap = min(len(specs), 400)
+    cap = workloads.latency_stats(gp_ffi.serve_spec_trace(specs[:k_cap], prm, cfg))
+    res = gp_ffi.serve_spec_trace(specs, prm, cfg, arrival)
+    if any(r["status"] != 0 for r in res):
+        raise SystemExit("bench.py: served requests failed")
+    st = workloads.latency_stats(res)
+    gen_ms = 1e3 * float(np.mean([r["t_ready"] - r["t_start"] for r in res]))
+    # serial plain processing, one request at a time, of the first requests
+    t_end, lat, busy = 0.0, [], 0.0
+    k_ser = min(len(specs), k_cap)
+    for k in range(k_ser):
+        p = gp_ffi.plain_job_spec(specs[k], prm, 99, k, True)
+        if p["checksum"] != res[k]["checksum"]:
+            raise SystemExit(f"bench.py: served request {k} differs from its plain run")
+        dt = p["t_end"] - p["t_start"]
+        busy += dt
+        t_end = max(t_end, arrival[k]) + dt
+        lat.append(t_end - arrival[k])
+    serial_cap = k_ser / busy
+    lat = np.array(lat)
+    log(f"serving: {st['lp_per_s']:.0f} LP/s at {rate:.0f} req/s offered, p50 {st['p50_ms']:.1f} ms, "
+        f"p99 {st['p99_ms']:.1f} ms ({len(res)} requests, generation {gen_ms:.1f} ms mean); capacity "
+        f"{cap['lp_per_s']:.0f} LP/s vs serial plain {serial_cap:.0f} LP/s -> {cap['lp_per_s'] / serial_cap:.2f}x")
+    return {"value": st["lp_per_s"], "unit": "LP/s", "requests": len(res), "rate_per_s": rate, "seconds": secs,
+            "m": "logU[1000,500000]", "n": "1.5 m", "row_len": "U[4,10]", "mean_m": float(sizes.mean()),
+            "hbm_budget_bytes": budget, "workers": args.serve_workers, "p50_ms": st["p50_ms"],
+            "p99_ms": st["p99_ms"], "max_ms": st["max_ms"], "generation_ms_mean": gen_ms,
+            "capacity_lp_per_s": cap["lp_per_s"], "capacity_sample": k_cap,
+            "serial_plain_capacity_lp_per_s": serial_cap, "capacity_vs_serial": cap["lp_per_s"] / serial_cap,
+            "serial_plain_trace": {"requests": k_ser, "p50_ms": 1e3 * float(np.percentile(lat, 50)),
+                                   "p99_ms": 1e3 * float(np.percentile(lat, 99))},
+            "check": "every compared request's postsolve checksum equals its plain run's"}
+
+
+def cpu_leg(args):
+    """One CPU baseline leg in its own process: the oracle build named by
+    ORACLE_LIB presolves the plain workload's LP once to warm up, then
+    --cpu-runs more times; prints the times and the reduced sizes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import lp_synth
+    import oracle_ffi
+
+    lp = lp_synth.config1(seed=args.seed)
+    prm = oracle_ffi.params(enable_structure_specific_rules=0)
+    times = []
+    for k in range(args.cpu_runs + 1):
+        t = time.perf_counter()
+        s, _, _ = oracle_ffi.presolve(lp, prm, False)
+        if k > 0:
+            times.append(time.perf_counter() - t)
+    print(json.dumps({"times": times, "reduced": [s["reduced_rows"], s["reduced_cols"], s["reduced_nnz"]]}),
+          flush=True)
+
+
+def _cpu_model():
+    import subprocess
+
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return "unknown"
+
+
+def cpu_baseline(args, summary, m, n):
+    """The oracle (serial C++ restatement of the reference's passes) on the
+    host cores: single thread, and the OpenMP build on all the threads the
+    job may use (OMP_NUM_THREADS, else the usable cores); median of
+    --cpu-runs runs after one warm-up, each leg in its own process."""
+    import statistics
+    import subprocess
+
+    threads = int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0)))
+    legs = {}
+    for name, lib, nt in (("single_thread", "liboracle.so", 1), ("all_core", "liboracle_omp.so", threads)):
+        env = dict(os.environ, ORACLE_LIB=os.path.join(ROOT, "oracle", "build", lib), OMP_NUM_THREADS=str(nt))
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-leg", "--seed", str(args.seed),
+                            "--cpu-runs", str(args.cpu_runs)], capture_output=True, text=True, env=env, timeout=600)
+        if r.returncode != 0:
+            raise SystemExit(f"bench.py: cpu baseline leg {name} failed: {r.stderr[-2000:]}")
+        res = json.loads(r.stdout.strip().splitlines()[-1])
+        if res["reduced"] != [summary["reduced_rows"], summary["reduced_cols"], summary["reduced_nnz"]]:
+            raise SystemExit("bench.py: GPU and oracle reduced sizes differ")
+        med = statistics.median(res["times"])
+        legs[name] = {"seconds_median": med, "seconds": res["times"], "threads": nt, "lp_per_s": 1.0 / med}
+        log(f"cpu oracle presolve ({name}, {nt} threads): median {med:.2f} s of {res['times']}")
+    best = legs["all_core"]
+    return {"value": best["lp_per_s"], "unit": "LP/s", "cores": best["threads"], "kind": "port",
+            "sample": f"full presolves of the same {m}x{n} LP (seed {args.seed}) by the C++ oracle, median of "
+                      f"{args.cpu_runs} after one warm-up; all-core = OpenMP build (transpose, compaction and "
+                      f"row activity threaded, identical results)",
+            "single_thread": legs["single_thread"], "all_core": legs["all_core"],
+            "host": {"nproc": os.cpu_count(), "usable_cores": len(os.sched_getaffinity(0)), "cpu_model": _cpu_model()}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -479,7 +745,14 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=60.0)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool", type=int, default=48)
+    ap.add_argument("--cpu-runs", type=int, default=5, help="timed oracle runs per CPU baseline leg")
+    ap.add_argument("--serve-workers", type=int, default=12)
+    ap.add_argument("--no-many", action="store_true",
+                    help="plain workload only (no batched / serving objects on the line)")
+    ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_leg:
+        return cpu_leg(args)
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported: this library runs one LP presolve per GPU and "
               "the benchmark measures one GPU only (N=1).", file=sys.stderr)
@@ -507,9 +780,10 @@ def main():
     H = gp_ffi.hip()
 
     if args.kernels_only:
-        act, comp = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
+        act, comp, tr = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
         calibration_copy(gp_ffi)
-        print(json.dumps({"kernels_only": True, "row_activity": act, "compaction": comp}), flush=True)
+        print(json.dumps({"kernels_only": True, "row_activity": act, "compaction": comp, "transpose": tr}),
+              flush=True)
         return
 
     def one_presolve():
@@ -565,22 +839,10 @@ def main():
     log(f"postsolve (inputs resident; record tape uploaded by the call): {post_ms:.1f} ms (runs {post}); "
         f"prepared: {prep_ms:.1f} ms once, then {plan_ms:.1f} ms per run (runs {runs})")
 
-    act, comp = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
+    act, comp, tr = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
     cpu = None
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))
-        import oracle_ffi
-
-        t = time.perf_counter()
-        s_cpu, _, _ = oracle_ffi.presolve(lp, oracle_ffi.params(enable_structure_specific_rules=0), False)
-        cpu_s = time.perf_counter() - t
-        log(f"cpu oracle presolve: {cpu_s:.2f} s")
-        if (s_cpu["reduced_rows"], s_cpu["reduced_cols"], s_cpu["reduced_nnz"]) != (
-                summary["reduced_rows"], summary["reduced_cols"], summary["reduced_nnz"]):
-            raise SystemExit("bench.py: GPU and oracle reduced sizes differ")
-        cpu = {"value": 1.0 / cpu_s, "unit": "LP/s", "cores": 1, "kind": "port",
-               "sample": f"one full presolve of the same {m}x{n} LP (seed {args.seed}) by the serial C++ oracle "
-                         f"({cpu_s:.2f} s)"}
+        cpu = cpu_baseline(args, summary, m, n)
 
     out = {
         "metric": "presolve ms + HBM GB/s %peak per LP; LPs/s and p99 latency in batched serving",
@@ -615,8 +877,18 @@ def main():
                               "us_per_call": act["us"], "algorithmic_bytes": act["bytes"],
                               "effective_bytes": act["effective_bytes"],
                               "effective_frac": act["effective_achieved"] / HBM_PEAK_GBS, **act["kernels"]},
+        "roofline_transpose": {"bound": "hbm", "kernel": "transpose: 3 LSD sort passes + row pointers from the "
+                                                          "sorted keys + entry rows + gather",
+                               "achieved": tr["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": tr["achieved"] / HBM_PEAK_GBS, "traffic": transpose_traffic(),
+                               "us_per_call": tr["us"], "algorithmic_bytes": tr["bytes"],
+                               "workload": tr["workload"], "kernels_us": tr["kernels_us"]},
         "cpu_baseline": cpu,
     }
+    if not args.no_many:
+        del dlp
+        out["batched"] = many_batched(gp_ffi, args)
+        out["serving"] = many_serving(gp_ffi, args)
     print(json.dumps(out), flush=True)
 
 

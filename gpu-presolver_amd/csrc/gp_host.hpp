@@ -292,7 +292,10 @@ class PostsolvePlan {
   // Throws std::runtime_error when the record or the original LP cannot be
   // postsolved (the checks of postsolve_gpu).
   static void validate(const presolve::PresolveRecordGpu& record, const presolve::LPInfoGpu* original);
-  PostsolvePlan(const presolve::PresolveRecordGpu& record, const presolve::LPInfoGpu* original);
+  // original_l: the original lower bounds the max-slack recovery reads
+  // (null: original->l, or none).
+  PostsolvePlan(const presolve::PresolveRecordGpu& record, const presolve::LPInfoGpu* original,
+                const double* original_l = nullptr);
   ~PostsolvePlan();
   PostsolvePlan(const PostsolvePlan&) = delete;
   PostsolvePlan& operator=(const PostsolvePlan&) = delete;

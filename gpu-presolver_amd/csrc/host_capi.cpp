@@ -7,6 +7,7 @@
 
 #include <chrono>
 #include <memory>
+#include <sstream>
 #include <new>
 #include <stdexcept>
 #include <thread>
@@ -29,6 +30,8 @@
 #include "gpu_presolver/presolve/rules/rule_infeasible_redundant_rows.hpp"
 #include "gpu_presolver/presolve/rules/rule_redundant_bounds.hpp"
 #include "gpu_presolver/presolve/rules/rule_singleton_rows.hpp"
+#include "gpu_presolver/tools.hpp"
+#include "mps_reader/mpsreader.hpp"
 
 using namespace gpu_presolver::presolve;
 namespace amd = gpu_presolver::amd;
@@ -44,6 +47,20 @@ struct gp_tape_handle {
 
 struct gp_server {
   amd::Server* impl = nullptr;
+};
+
+// A read MPS model with int32 copies of its index arrays for the view.
+struct gp_mps_model {
+  mps_reader::MpsModel mps;
+  std::vector<int32_t> csr_ptr, csr_idx, csc_ptr, csc_idx;
+  void fill() {
+    const auto& a = mps.lp.csr();
+    const auto& t = mps.lp.csc();
+    csr_ptr.assign(a.row_ptr().begin(), a.row_ptr().end());
+    csr_idx.assign(a.col_idx().begin(), a.col_idx().end());
+    csc_ptr.assign(t.col_ptr().begin(), t.col_ptr().end());
+    csc_idx.assign(t.row_idx().begin(), t.row_idx().end());
+  }
 };
 
 namespace {
@@ -739,6 +756,62 @@ gp_status gp_checksum(const double* x, int32_t n, const double* y, int32_t m, co
 void gp_set_batched_closure_min_nnz(int32_t nnz) { amd::set_batched_closure_min_nnz(nnz); }
 int32_t gp_batched_closure_min_nnz(void) { return amd::batched_closure_min_nnz(); }
 void gp_test_fail_after_record_updates(int32_t count) { amd::fail_after_record_updates(count); }
+
+gp_status gp_mps_read(const char* path, gp_mps_model** out) {
+  return guarded([&] {
+    require(path != nullptr && out != nullptr, "null argument");
+    *out = nullptr;
+    auto m = std::make_unique<gp_mps_model>(gp_mps_model{gpu_presolver::tools::read_mps_file(path), {}, {}, {}, {}});
+    m->fill();
+    *out = m.release();
+  });
+}
+
+gp_status gp_mps_parse(const char* text, size_t length, gp_mps_model** out) {
+  return guarded([&] {
+    require((text != nullptr || length == 0) && out != nullptr, "null argument");
+    *out = nullptr;
+    std::istringstream in(std::string(text ? text : "", length));
+    auto m = std::make_unique<gp_mps_model>(gp_mps_model{mps_reader::read_mps(in), {}, {}, {}, {}});
+    m->fill();
+    *out = m.release();
+  });
+}
+
+gp_status gp_mps_get_view(const gp_mps_model* model, gp_mps_view* out) {
+  return guarded([&] {
+    require(model != nullptr && out != nullptr, "null argument");
+    const mps_reader::LpModel& lp = model->mps.lp;
+    out->name = model->mps.name.c_str();
+    out->rows = lp.num_rows();
+    out->cols = lp.num_cols();
+    out->nnz = lp.csr().nnz();
+    out->csr_row_ptr = model->csr_ptr.data();
+    out->csr_col_idx = model->csr_idx.data();
+    out->csr_val = lp.csr().values().data();
+    out->csc_col_ptr = model->csc_ptr.data();
+    out->csc_row_idx = model->csc_idx.data();
+    out->csc_val = lp.csc().values().data();
+    out->c = lp.objective().data();
+    out->row_lower = lp.row_lower().data();
+    out->row_upper = lp.row_upper().data();
+    out->col_lower = lp.col_lower().data();
+    out->col_upper = lp.col_upper().data();
+    out->obj_constant = lp.obj_constant();
+  });
+}
+
+gp_status gp_mps_upload(const gp_mps_model* model, gp_lp* out) {
+  return guarded([&] {
+    require(model != nullptr && out != nullptr, "null argument");
+    require_device();
+    gpu_presolver::tools::DeviceLpOwner owner = gpu_presolver::tools::upload_lp(model->mps.lp);
+    *out = from_lp(owner.lp);
+    owner.lp = LPInfoGpu{};  // the caller owns the arrays now (gp_free_lp)
+  });
+}
+
+void gp_mps_free(gp_mps_model* model) { delete model; }
 
 void gp_presolve_free(gp_presolve_handle* h) {
   if (h == nullptr) return;

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -337,31 +338,33 @@ class TapeScanner {
   int32_t n0_, m0_;
 };
 
-Schedule level_schedule(const TapeScanner& scan, int32_t count, int32_t n0, int32_t m0, bool primal, bool has_orig,
-                        bool has_y, const OrigPattern* op) {
+// Records [lo, hi) of the tape, replayed in reverse.
+Schedule level_schedule(const TapeScanner& scan, int32_t lo, int32_t hi, int32_t n0, int32_t m0, bool primal,
+                        bool has_orig, bool has_y, const OrigPattern* op) {
   std::vector<int32_t> lw(static_cast<size_t>(n0) + m0, -1), lr(static_cast<size_t>(n0) + m0, -1);
-  std::vector<int32_t> level(count, -1);
+  const int32_t span = hi > lo ? hi - lo : 0;
+  std::vector<int32_t> level(span, -1);
   std::vector<int32_t> rd, wr;
   int32_t levels = 0;
-  for (int32_t k = count - 1; k >= 0; --k) {
+  for (int32_t k = hi - 1; k >= lo; --k) {
     if (!scan.sets(k, primal, has_orig, has_y, op, rd, wr)) continue;
     int32_t lv = 0;
     for (int32_t v : rd) lv = std::max(lv, lw[v] + 1);
     for (int32_t v : wr) lv = std::max(lv, std::max(lw[v], lr[v]) + 1);
     for (int32_t v : wr) lw[v] = lv;
     for (int32_t v : rd) lr[v] = std::max(lr[v], lv);
-    level[k] = lv;
+    level[k - lo] = lv;
     levels = std::max(levels, lv + 1);
   }
   std::vector<int32_t> start(static_cast<size_t>(levels) + 1, 0);
-  for (int32_t k = 0; k < count; ++k)
+  for (int32_t k = 0; k < span; ++k)
     if (level[k] >= 0) ++start[level[k] + 1];
   for (int32_t l = 0; l < levels; ++l) start[l + 1] += start[l];
   Schedule sch;
   sch.order.resize(start[levels]);
   std::vector<int32_t> fill(start.begin(), start.end() - 1);
-  for (int32_t k = count - 1; k >= 0; --k)
-    if (level[k] >= 0) sch.order[fill[level[k]]++] = k;  // descending k within a level
+  for (int32_t k = hi - 1; k >= lo; --k)
+    if (level[k - lo] >= 0) sch.order[fill[level[k - lo]]++] = k;  // descending k within a level
   for (int32_t l = 0; l < levels; ++l) {
     const bool big = start[l + 1] - start[l] >= kGridLevelMin;
     if (!big && !sch.segs.empty() && !sch.segs.back().grid) {
@@ -420,11 +423,164 @@ using presolve::kPostsolveTol;
 using presolve::LPInfoGpu;
 using presolve::PresolveRecordGpu;
 
+namespace {
+
+void check_recovery_col(int32_t col, int32_t n0, const char* what) {
+  if (col < 0 || col >= n0) throw std::runtime_error(std::string(what) + " column is out of range");
+}
+
+// validate_primal_recovery_timeline (gpu_postsolve.cu:1485-1616): the
+// checkpoints are chronological, each payload has exactly one checkpoint
+// and every column a payload names is an original column.
+void validate_recovery_timeline(const PresolveRecordGpu& r) {
+  const auto limit = static_cast<size_t>(std::numeric_limits<int32_t>::max());
+  if (r.structural_primal_recoveries.size() > limit || r.antipodal_component_recoveries.size() > limit)
+    throw std::runtime_error("primal recovery payload exceeds 32-bit checkpoint range");
+  std::vector<uint8_t> seen_s(r.structural_primal_recoveries.size(), 0), seen_a(r.antipodal_component_recoveries.size(), 0);
+  const int32_t tape_count = static_cast<int32_t>(r.tape.types.size());
+  int32_t prev = 0;
+  for (const presolve::PrimalRecoveryCheckpoint& cp : r.primal_recovery_timeline) {
+    if (cp.tape_position < prev || cp.tape_position > tape_count)
+      throw std::runtime_error("primal recovery checkpoints are not chronological");
+    prev = cp.tape_position;
+    if (cp.kind == presolve::PrimalRecoveryKind::StructuralL1) {
+      if (cp.payload_index < 0 || cp.payload_index >= static_cast<int32_t>(seen_s.size()) || seen_s[cp.payload_index])
+        throw std::runtime_error("invalid or duplicate structural recovery checkpoint");
+      seen_s[cp.payload_index] = 1;
+      const auto& st = r.structural_primal_recoveries[cp.payload_index];
+      for (const auto& x : st.splits) {
+        check_recovery_col(x.t_col, r.n0, "structural split recovery");
+        check_recovery_col(x.e_col, r.n0, "structural split recovery");
+      }
+      for (const auto& x : st.outer_pairs) {
+        check_recovery_col(x.bound_col, r.n0, "structural outer-pair recovery");
+        check_recovery_col(x.free_col, r.n0, "structural outer-pair recovery");
+      }
+      for (const auto& x : st.linked_slacks) {
+        check_recovery_col(x.slack_col, r.n0, "structural linked-slack recovery");
+        check_recovery_col(x.t_col, r.n0, "structural linked-slack recovery");
+      }
+      for (const auto& x : st.max_slacks) {
+        if (x.t_cols.size() != x.factors.size())
+          throw std::runtime_error("structural max-slack recovery has mismatched payloads");
+        check_recovery_col(x.slack_col, r.n0, "structural max-slack recovery");
+        for (int32_t col : x.t_cols) check_recovery_col(col, r.n0, "structural max-slack recovery");
+      }
+    } else if (cp.kind == presolve::PrimalRecoveryKind::AntipodalComponent) {
+      if (cp.payload_index < 0 || cp.payload_index >= static_cast<int32_t>(seen_a.size()) || seen_a[cp.payload_index])
+        throw std::runtime_error("invalid or duplicate antipodal recovery checkpoint");
+      seen_a[cp.payload_index] = 1;
+      const auto& a = r.antipodal_component_recoveries[cp.payload_index];
+      const size_t k = a.plus_cols.size();
+      if (a.minus_cols.size() != k || a.root_plus_cols.size() != k || a.root_minus_cols.size() != k ||
+          a.plus_lower.size() != k || a.minus_lower.size() != k || k > limit)
+        throw std::runtime_error("antipodal recovery has mismatched payloads");
+      for (size_t i = 0; i < k; ++i) {
+        check_recovery_col(a.plus_cols[i], r.n0, "antipodal recovery");
+        check_recovery_col(a.minus_cols[i], r.n0, "antipodal recovery");
+        check_recovery_col(a.root_plus_cols[i], r.n0, "antipodal recovery");
+        check_recovery_col(a.root_minus_cols[i], r.n0, "antipodal recovery");
+      }
+    } else {
+      throw std::runtime_error("unknown primal recovery checkpoint kind");
+    }
+  }
+  for (uint8_t v : seen_s)
+    if (!v) throw std::runtime_error("structural recovery payload is missing a checkpoint");
+  for (uint8_t v : seen_a)
+    if (!v) throw std::runtime_error("antipodal recovery payload is missing a checkpoint");
+}
+
+// One recovery step on the device: its payload arrays uploaded once.
+struct RecoveryStep {
+  bool antipodal = false;
+  int32_t nsplit = 0, nouter = 0, nlinked = 0, nmax = 0, nanti = 0;
+  DBuf<int32_t> split_t, split_e, outer_b, outer_f, linked_s, linked_t, max_s, max_start, max_t;
+  DBuf<double> split_rho, linked_f, max_f;
+  DBuf<int32_t> a_plus, a_minus, a_rplus, a_rminus;
+  DBuf<double> a_plower, a_mlower, a_pcache, a_mcache;
+
+  template <class T>
+  static void up(DBuf<T>& d, const std::vector<T>& v, const char* what) {
+    d.alloc(static_cast<int64_t>(v.size()));
+    ctx().to_device(d.get(), v.data(), static_cast<int64_t>(v.size()), what);
+  }
+
+  explicit RecoveryStep(const presolve::StructuralL1PrimalRecoveryStep& st) {
+    std::vector<int32_t> t, e, b, f, ls, lt, ms, mstart{0}, mt;
+    std::vector<double> rho, lf, mf;
+    for (const auto& x : st.splits) {
+      t.push_back(x.t_col);
+      e.push_back(x.e_col);
+      rho.push_back(x.rho);
+    }
+    for (const auto& x : st.outer_pairs) {
+      b.push_back(x.bound_col);
+      f.push_back(x.free_col);
+    }
+    for (const auto& x : st.linked_slacks) {
+      ls.push_back(x.slack_col);
+      lt.push_back(x.t_col);
+      lf.push_back(x.factor);
+    }
+    for (const auto& x : st.max_slacks) {
+      if (x.t_cols.empty()) continue;  // the reference skips an empty max slack
+      ms.push_back(x.slack_col);
+      mt.insert(mt.end(), x.t_cols.begin(), x.t_cols.end());
+      mf.insert(mf.end(), x.factors.begin(), x.factors.end());
+      mstart.push_back(static_cast<int32_t>(mt.size()));
+    }
+    nsplit = static_cast<int32_t>(t.size());
+    nouter = static_cast<int32_t>(b.size());
+    nlinked = static_cast<int32_t>(ls.size());
+    nmax = static_cast<int32_t>(ms.size());
+    up(split_t, t, "split recovery upload");
+    up(split_e, e, "split recovery upload");
+    up(split_rho, rho, "split recovery upload");
+    up(outer_b, b, "outer-pair recovery upload");
+    up(outer_f, f, "outer-pair recovery upload");
+    up(linked_s, ls, "linked-slack recovery upload");
+    up(linked_t, lt, "linked-slack recovery upload");
+    up(linked_f, lf, "linked-slack recovery upload");
+    up(max_s, ms, "max-slack recovery upload");
+    up(max_start, mstart, "max-slack recovery upload");
+    up(max_t, mt, "max-slack recovery upload");
+    up(max_f, mf, "max-slack recovery upload");
+  }
+
+  explicit RecoveryStep(const presolve::AntipodalComponentPrimalRecoveryStep& a) : antipodal(true) {
+    nanti = static_cast<int32_t>(a.plus_cols.size());
+    up(a_plus, a.plus_cols, "antipodal recovery upload");
+    up(a_minus, a.minus_cols, "antipodal recovery upload");
+    up(a_rplus, a.root_plus_cols, "antipodal recovery upload");
+    up(a_rminus, a.root_minus_cols, "antipodal recovery upload");
+    up(a_plower, a.plus_lower, "antipodal recovery upload");
+    up(a_mlower, a.minus_lower, "antipodal recovery upload");
+    a_pcache.alloc(nanti);
+    a_mcache.alloc(nanti);
+  }
+
+  // restore_structural_primal_step / restore_antipodal_component
+  // (gpu_postsolve.cu:1150-1212): splits, outer pairs, linked slacks, then
+  // max slacks.
+  void enqueue(double* x, const double* original_l, hipStream_t s) const {
+    if (antipodal) {
+      check(gpk_restore_antipodal(x, a_plus, a_minus, a_rplus, a_rminus, a_plower, a_mlower, a_pcache, a_mcache, nanti,
+                                  s),
+            "antipodal recovery");
+      return;
+    }
+    check(gpk_restore_splits(x, split_t, split_e, split_rho, nsplit, s), "split recovery");
+    check(gpk_restore_outer_pairs(x, outer_b, outer_f, nouter, s), "outer-pair recovery");
+    check(gpk_restore_linked_slacks(x, linked_s, linked_t, linked_f, nlinked, s), "linked-slack recovery");
+    check(gpk_restore_max_slacks(x, max_s, max_start, max_t, max_f, nmax, original_l, s), "max-slack recovery");
+  }
+};
+
+}  // namespace
+
 void PostsolvePlan::validate(const PresolveRecordGpu& record, const LPInfoGpu* original) {
-  if (!record.primal_recovery_timeline.empty() || !record.structural_primal_recoveries.empty() ||
-      !record.antipodal_component_recoveries.empty())
-    throw std::runtime_error(
-        "postsolve_gpu: the record holds structure-specific recovery steps, which this library does not implement");
+  validate_recovery_timeline(record);
   if (static_cast<int32_t>(record.col_red2org.size()) != record.n1 ||
       static_cast<int32_t>(record.row_red2org.size()) != record.m1)
     throw std::runtime_error("postsolve_gpu: record maps do not match m1/n1");
@@ -443,11 +599,21 @@ struct PostsolvePlan::Impl {
   DBuf<double> fval;
   DBuf<uint8_t> zret, prot;
   DBuf<int32_t> long_cols, long_count;
-  presolve::Schedule primal, dual;
-  DBuf<int32_t> primal_buf, dual_buf;
+  presolve::Schedule dual;
+  DBuf<int32_t> dual_buf;
+  // The primal pass in replay order: a schedule over a tape range, then the
+  // recovery step at that range's start (none after the last range).
+  struct Stage {
+    presolve::Schedule sched;
+    DBuf<int32_t> buf;
+    std::unique_ptr<RecoveryStep> step;
+  };
+  std::vector<Stage> primal;
+  const double* original_l = nullptr;
 };
 
-PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* original) : d_(new Impl) {
+PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* original, const double* original_l)
+    : d_(new Impl) {
   AbandonOnThrow abandon_guard;
   validate(record, original);
   if (original != nullptr && original->A.rowPtr != nullptr && original->AT.rowPtr != nullptr &&
@@ -490,10 +656,31 @@ PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* o
   }
   lap("tape upload, scan, lines");
   const int32_t count = d.tape->view.count;
-  d.primal = presolve::level_schedule(scanner, count, d.n0, d.m0, true, has_orig, has_y, &pattern);
-  d.primal_buf = presolve::upload_schedule(d.primal);
+  d.original_l = original_l != nullptr ? original_l : (original != nullptr ? original->l : nullptr);
+  // postsolve_gpu (gpu_postsolve.cu:1691-1713): the primal pass replays the
+  // tape from its end down to each recovery checkpoint, in reverse
+  // chronological order, and applies that checkpoint's recovery step there.
+  int32_t range_end = count;
+  for (auto it = record.primal_recovery_timeline.rbegin(); it != record.primal_recovery_timeline.rend(); ++it) {
+    Impl::Stage st;
+    st.sched = presolve::level_schedule(scanner, it->tape_position, range_end, d.n0, d.m0, true, has_orig, has_y,
+                                        &pattern);
+    st.buf = presolve::upload_schedule(st.sched);
+    if (it->kind == presolve::PrimalRecoveryKind::StructuralL1)
+      st.step = std::make_unique<RecoveryStep>(record.structural_primal_recoveries[it->payload_index]);
+    else
+      st.step = std::make_unique<RecoveryStep>(record.antipodal_component_recoveries[it->payload_index]);
+    d.primal.push_back(std::move(st));
+    range_end = it->tape_position;
+  }
+  {
+    Impl::Stage st;
+    st.sched = presolve::level_schedule(scanner, 0, range_end, d.n0, d.m0, true, has_orig, has_y, &pattern);
+    st.buf = presolve::upload_schedule(st.sched);
+    d.primal.push_back(std::move(st));
+  }
   lap("primal schedule");
-  d.dual = presolve::level_schedule(scanner, count, d.n0, d.m0, false, has_orig, has_y, &pattern);
+  d.dual = presolve::level_schedule(scanner, 0, count, d.n0, d.m0, false, has_orig, has_y, &pattern);
   d.dual_buf = presolve::upload_schedule(d.dual);
   lap("dual schedule");
   // project_column_duals_from_original (gpu_postsolve.cu:1370-1435): the
@@ -528,8 +715,10 @@ void PostsolvePlan::enqueue(const double* x_red, const double* y_red, const doub
   if (z_red) check(gpk_scatter(z_red, d.col_map, d.n1, z_org, s), "scatter z");
   check(gpk_mark(d.col_map, d.n1, d.zret, s), "mark retrieved");
   check(gpk_restore_fixed(d.fidx, d.fval, d.nf, x_org, s), "restore fixed");
-  presolve::enqueue_schedule(d.primal, d.primal_buf, d.tape->view, d.orig, x_org, y, z_org, d.zret, true,
-                             kPostsolveTol);
+  for (const Impl::Stage& st : d.primal) {
+    presolve::enqueue_schedule(st.sched, st.buf, d.tape->view, d.orig, x_org, y, z_org, d.zret, true, kPostsolveTol);
+    if (st.step) st.step->enqueue(x_org, d.original_l, s);
+  }
   presolve::enqueue_schedule(d.dual, d.dual_buf, d.tape->view, d.orig, x_org, y, z_org, d.zret, false,
                              kPostsolveTol);
   if (d.project)
@@ -541,14 +730,19 @@ void PostsolvePlan::enqueue(const double* x_red, const double* y_red, const doub
 
 namespace gpu_presolver::presolve {
 
+// gpu_postsolve.cu:1618-1628: the steps in reverse order on x_org, then a
+// device synchronization.
 void postsolve_restore_structural_primal_gpu(double* x_org, const std::vector<StructuralL1PrimalRecoveryStep>& recoveries,
                                              const double* original_l) {
-  (void)x_org;
-  (void)original_l;
-  if (!recoveries.empty())
-    throw std::runtime_error(
-        "postsolve_restore_structural_primal_gpu: structural L1 recoveries are produced by the structure-specific "
-        "rules, which this library does not implement");
+  amd::AbandonOnThrow abandon_guard;
+  if (!recoveries.empty()) {
+    check(hipDeviceSynchronize(), "structural primal recovery entry synchronize");
+    for (auto it = recoveries.rbegin(); it != recoveries.rend(); ++it) {
+      amd::RecoveryStep step(*it);
+      step.enqueue(x_org, original_l, ctx().stream());
+      ctx().sync("structural primal recovery");
+    }
+  }
   check(hipDeviceSynchronize(), "postsolve_restore_structural_primal_gpu synchronize");
 }
 
@@ -560,7 +754,6 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
 GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, const PresolveRecordGpu& record,
                                  const LPInfoGpu* original_model, const double* original_l) {
   amd::AbandonOnThrow abandon_guard;
-  (void)original_l;
   amd::PostsolvePlan::validate(record, original_model);
   if (amd::job_mode().device_sync_on_entry) check(hipDeviceSynchronize(), "postsolve entry synchronize");
   auto& c = ctx();
@@ -573,7 +766,7 @@ GpuPostsolveResult postsolve_gpu(double* x_red, double* y_red, double* z_red, co
     res.x_org = malloc_f64(n0, "hipMalloc postsolve x_org");
     res.z_org = malloc_f64(n0, "hipMalloc postsolve z_org");
     res.y_org = malloc_f64(m0, "hipMalloc postsolve y_org");
-    amd::PostsolvePlan plan(record, original_model);
+    amd::PostsolvePlan plan(record, original_model, original_l);
     plan.enqueue(x_red, y_red, z_red, res.x_org, res.y_org, res.z_org);
     c.sync("postsolve_gpu synchronize");
   } catch (...) {

@@ -118,6 +118,12 @@ def lib():
         L.gp_serve_spec_trace.argtypes = [C.POINTER(T.GpLpSpec), C.c_int32, vp, vp,
                                           C.POINTER(T.GpServeConfig), C.POINTER(T.GpJobResult)]
         L.gp_checksum.argtypes = [vp, C.c_int32, vp, C.c_int32, vp, C.POINTER(C.c_uint64)]
+        L.gp_mps_read.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.gp_mps_parse.argtypes = [C.c_char_p, C.c_size_t, C.POINTER(vp)]
+        L.gp_mps_get_view.argtypes = [vp, C.POINTER(T.GpMpsView)]
+        L.gp_mps_upload.argtypes = [vp, C.POINTER(T.GpLp)]
+        L.gp_mps_free.argtypes = [vp]
+        L.gp_mps_free.restype = None
         _lib = L
     return _lib
 
@@ -627,6 +633,49 @@ def serve_spec_trace(specs, prm, cfg, arrival_s=None):
     _check(lib().gp_serve_spec_trace(arr_specs, jobs, None if arr is None else arr.ctypes.data, C.byref(prm),
                                      C.byref(cfg), res))
     return _results_dict(res[:jobs])
+
+
+def _mps_dict(view):
+    m, n, nnz = view.rows, view.cols, view.nnz
+    a = T.mps_view_arrays(view)
+    return dict(a, name=view.name.decode(), m=m, n=n, obj=view.obj_constant)
+
+
+class MpsModel:
+    """A model read by the library's MPS reader (host memory); .lp is the
+    LP dict (layout of lp_synth) plus its name."""
+
+    def __init__(self, text=None, path=None):
+        self.h = C.c_void_p()
+        if path is not None:
+            _check(lib().gp_mps_read(os.fsencode(path), C.byref(self.h)))
+        else:
+            data = text.encode() if isinstance(text, str) else bytes(text)
+            _check(lib().gp_mps_parse(data, len(data), C.byref(self.h)))
+        v = T.GpMpsView()
+        _check(lib().gp_mps_get_view(self.h, C.byref(v)))
+        self.lp = _mps_dict(v)
+
+    def upload(self):
+        """The model as a device LP (library memory, released with
+        gp_free_lp); returned as a GeneratedLp-like holder."""
+        return _UploadedLp(self)
+
+    def close(self):
+        if self.h and _lib is not None:
+            lib().gp_mps_free(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+
+class _UploadedLp(GeneratedLp):
+    def __init__(self, model):
+        self.spec = None
+        self.g = T.GpLp()
+        _check(lib().gp_mps_upload(model.h, C.byref(self.g)))
+        self.m, self.n = self.g.A.rows, self.g.A.cols
 
 
 def synthetic_point(seed, reduced_g):

@@ -115,6 +115,28 @@ class GpLpSpec(C.Structure):
                 ("len_hi", C.c_int32), ("mean_len", C.c_double), ("min_len", C.c_int32)]
 
 
+class GpMpsView(C.Structure):
+    _fields_ = [("name", C.c_char_p), ("rows", C.c_int32), ("cols", C.c_int32), ("nnz", C.c_int32),
+                ("csr_row_ptr", i32p), ("csr_col_idx", i32p), ("csr_val", f64p),
+                ("csc_col_ptr", i32p), ("csc_row_idx", i32p), ("csc_val", f64p),
+                ("c", f64p), ("row_lower", f64p), ("row_upper", f64p), ("col_lower", f64p),
+                ("col_upper", f64p), ("obj_constant", C.c_double)]
+
+
+def mps_view_arrays(v) -> dict:
+    """Copies of the arrays of a gp_mps_view (or the same layout), keyed as
+    an LP dict: A (CSR), AT (the CSC read as the transpose's CSR), c, l, u,
+    AL, AU."""
+    m, n, nnz = v.rows, v.cols, v.nnz
+    ptr = _arr(v.csr_row_ptr, m + 1, np.int32) if m + 1 > 0 else np.zeros(1, np.int32)
+    tptr = _arr(v.csc_col_ptr, n + 1, np.int32)
+    return {"A_ptr": ptr, "A_idx": _arr(v.csr_col_idx, nnz, np.int32), "A_val": _arr(v.csr_val, nnz, np.float64),
+            "AT_ptr": tptr, "AT_idx": _arr(v.csc_row_idx, nnz, np.int32), "AT_val": _arr(v.csc_val, nnz, np.float64),
+            "c": _arr(v.c, n, np.float64), "l": _arr(v.col_lower, n, np.float64),
+            "u": _arr(v.col_upper, n, np.float64), "AL": _arr(v.row_lower, m, np.float64),
+            "AU": _arr(v.row_upper, m, np.float64)}
+
+
 class GpRecordView(C.Structure):
     _fields_ = [("m0", C.c_int32), ("n0", C.c_int32), ("m1", C.c_int32), ("n1", C.c_int32),
                 ("row_red2org", i32p), ("col_red2org", i32p),

@@ -21,6 +21,7 @@
 #ifndef GP_PRESOLVE_CAPI_H
 #define GP_PRESOLVE_CAPI_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -390,6 +391,36 @@ gp_status gp_plain_job(const gp_lp* lp, const gp_params* params, uint64_t seed, 
 /* gp_plain_job on the LP of `spec`, generated first on the calling thread. */
 gp_status gp_plain_job_spec(const gp_lp_spec* spec, const gp_params* params, uint64_t seed, uint64_t tag,
                             int32_t postsolve, gp_job_result* out);
+
+/* ---- MPS input (reference cpp/mps_reader, tools/gpu_presolver_tool_common.hpp)
+ * Host-only reading of a continuous MPS model (free format, then the fixed
+ * column layout), with the reference reader's decisions (number parsing,
+ * duplicate summing, default bounds, RANGES, first RHS/RANGES/BOUNDS set).
+ * gp_mps_view gives host pointers valid until gp_mps_free; gp_mps_upload
+ * copies the LP to the device (A from the CSR, AT from the CSC) in arrays
+ * released by gp_free_lp. */
+typedef struct gp_mps_model gp_mps_model;
+typedef struct gp_mps_view {
+  const char* name;
+  int32_t rows, cols, nnz;
+  const int32_t* csr_row_ptr;
+  const int32_t* csr_col_idx;
+  const double* csr_val;
+  const int32_t* csc_col_ptr;
+  const int32_t* csc_row_idx;
+  const double* csc_val;
+  const double* c;
+  const double* row_lower;
+  const double* row_upper;
+  const double* col_lower;
+  const double* col_upper;
+  double obj_constant;
+} gp_mps_view;
+gp_status gp_mps_read(const char* path, gp_mps_model** out);
+gp_status gp_mps_parse(const char* text, size_t length, gp_mps_model** out);
+gp_status gp_mps_get_view(const gp_mps_model* model, gp_mps_view* out);
+gp_status gp_mps_upload(const gp_mps_model* model, gp_lp* out);
+void gp_mps_free(gp_mps_model* model);
 
 /* The seeded reduced-space point of a serving job: x in [l, u] on the 2^-10
  * grid, y and z uniform in [-1, 1); lp is the reduced LP (device). */

@@ -80,6 +80,28 @@ hipError_t gpk_protect_columns(gp_tape_dev tape, int32_t n, uint8_t* prot, hipSt
 hipError_t gpk_project_duals(gp_orig_dev orig, const uint8_t* prot, const double* x, const double* y, double* z,
                              int32_t n, double tol, int32_t* long_count, int32_t* long_cols, hipStream_t stream);
 
+/* Primal recovery steps of the structure-specific reductions
+ * (gpu_postsolve.cu:44-141), in place on x (k_recovery.hip).  Splits:
+ * x[t] = t + e, x[e] = (t - e) / rho; outer pairs: x[free] = x[bound];
+ * linked slacks: x[slack] = factor * x[t]; max slacks (one per slack, entries
+ * starts[s] .. starts[s+1]): x[slack] = max(lower, max factor * x[t]) with
+ * lower the finite original lower bound or 0 (original_l may be null);
+ * antipodal: the root values are cached first, then x[plus] = plus_lower +
+ * root_plus value and x[minus] = minus_lower + root_minus value. */
+hipError_t gpk_restore_splits(double* x, const int32_t* t_cols, const int32_t* e_cols, const double* rho, int32_t k,
+                              hipStream_t stream);
+hipError_t gpk_restore_outer_pairs(double* x, const int32_t* bound_cols, const int32_t* free_cols, int32_t k,
+                                   hipStream_t stream);
+hipError_t gpk_restore_linked_slacks(double* x, const int32_t* slack_cols, const int32_t* t_cols,
+                                     const double* factors, int32_t k, hipStream_t stream);
+hipError_t gpk_restore_max_slacks(double* x, const int32_t* slack_cols, const int32_t* starts, const int32_t* t_cols,
+                                  const double* factors, int32_t slacks, const double* original_l,
+                                  hipStream_t stream);
+hipError_t gpk_restore_antipodal(double* x, const int32_t* plus_cols, const int32_t* minus_cols,
+                                 const int32_t* root_plus, const int32_t* root_minus, const double* plus_lower,
+                                 const double* minus_lower, double* plus_cache, double* minus_cache, int32_t k,
+                                 hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -15,7 +15,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(os.path.dirname(_HERE), "gpu-presolver_amd"))
 import gp_types as T  # noqa: E402
 
-LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# ORACLE_LIB selects another build of the same sources (the CPU baseline's
+# OpenMP leg loads build/liboracle_omp.so).
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "build", "liboracle.so")
 _lib = None
 
 

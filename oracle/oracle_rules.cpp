@@ -8,6 +8,9 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 namespace orc {
 
@@ -99,9 +102,108 @@ Csr copy_csr(const CsrV& a) {
   return out;
 }
 
+#ifdef _OPENMP
+// All-core builds only (the CPU baseline's OpenMP leg): the same transpose
+// and compaction with the work split over threads; outputs identical to the
+// serial versions below.  Rows are split into one contiguous block per
+// thread; each thread counts its block's entries per column, a per-column
+// exclusive sum over the threads gives every thread its own cursors, and
+// the blocks are scattered in row order, so every column keeps ascending
+// rows.
+Csr transpose_threads(const CsrV& a) {
+  const int T = omp_get_max_threads();
+  const int64_t n = a.cols;
+  Csr t;
+  t.rows = a.cols;
+  t.cols = a.rows;
+  t.ptr.assign(static_cast<size_t>(n) + 1, 0);
+  t.idx.resize(a.nnz);
+  t.val.resize(a.nnz);
+  std::vector<int32_t> cnt(static_cast<size_t>(T) * static_cast<size_t>(n), 0);
+  auto block = [&](int k) { return static_cast<int32_t>(static_cast<int64_t>(a.rows) * k / T); };
+#pragma omp parallel num_threads(T)
+  {
+    const int k = omp_get_thread_num();
+    int32_t* c = cnt.data() + static_cast<size_t>(k) * static_cast<size_t>(n);
+    for (int32_t p = a.ptr[block(k)]; p < a.ptr[block(k + 1)]; ++p) ++c[a.idx[p]];
+  }
+#pragma omp parallel for schedule(static)
+  for (int64_t j = 0; j < n; ++j) {
+    int32_t run = 0;
+    for (int k = 0; k < T; ++k) {
+      int32_t& c = cnt[static_cast<size_t>(k) * static_cast<size_t>(n) + static_cast<size_t>(j)];
+      const int32_t v = c;
+      c = run;
+      run += v;
+    }
+    t.ptr[j + 1] = run;
+  }
+  for (int64_t j = 0; j < n; ++j) t.ptr[j + 1] += t.ptr[j];
+#pragma omp parallel num_threads(T)
+  {
+    const int k = omp_get_thread_num();
+    int32_t* c = cnt.data() + static_cast<size_t>(k) * static_cast<size_t>(n);
+    for (int32_t i = block(k); i < block(k + 1); ++i)
+      for (int32_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p) {
+        const int32_t j = a.idx[p];
+        const int32_t q = t.ptr[j] + c[j]++;
+        t.idx[q] = i;
+        t.val[q] = a.val[p];
+      }
+  }
+  return t;
+}
+
+// Kept entries per kept row counted in parallel, a serial prefix over the
+// rows, then the kept entries written in parallel.
+Csr compact_threads(const CsrV& a, const uint8_t* keep_row, const uint8_t* keep_col) {
+  Csr out;
+  std::vector<int32_t> col_new(a.cols, -1);
+  int32_t n_new = 0;
+  for (int32_t j = 0; j < a.cols; ++j) col_new[j] = keep_col[j] ? n_new++ : -1;
+  out.cols = n_new;
+  std::vector<int32_t> row_new(a.rows, -1);
+  int32_t m_new = 0;
+  for (int32_t i = 0; i < a.rows; ++i) row_new[i] = keep_row[i] ? m_new++ : -1;
+  out.rows = m_new;
+  out.ptr.assign(static_cast<size_t>(m_new) + 1, 0);
+  if (n_new > 0) {
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < a.rows; ++i) {
+      if (row_new[i] < 0) continue;
+      int32_t k = 0;
+      for (int32_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p) k += col_new[a.idx[p]] >= 0;
+      out.ptr[row_new[i] + 1] = k;
+    }
+  }
+  for (int32_t r = 0; r < m_new; ++r) out.ptr[r + 1] += out.ptr[r];
+  out.idx.resize(out.ptr[m_new]);
+  out.val.resize(out.ptr[m_new]);
+  if (n_new > 0) {
+#pragma omp parallel for schedule(static)
+    for (int32_t i = 0; i < a.rows; ++i) {
+      if (row_new[i] < 0) continue;
+      int32_t w = out.ptr[row_new[i]];
+      for (int32_t p = a.ptr[i]; p < a.ptr[i + 1]; ++p) {
+        const int32_t jn = col_new[a.idx[p]];
+        if (jn >= 0) {
+          out.idx[w] = jn;
+          out.val[w] = a.val[p];
+          ++w;
+        }
+      }
+    }
+  }
+  return out;
+}
+#endif
+
 // Transpose with rows of the result in ascending row-of-A order, the output
 // of cusparseCsr2cscEx2 ALG1 used by the reference (gpu_presolve.cu:1804-1879).
 Csr transpose(const CsrV& a) {
+#ifdef _OPENMP
+  if (omp_get_max_threads() > 1 && a.nnz > 65536) return transpose_threads(a);
+#endif
   Csr t;
   t.rows = a.cols;
   t.cols = a.rows;
@@ -123,6 +225,9 @@ Csr transpose(const CsrV& a) {
 
 // Stable row/column compaction (gpu_presolve.cu:297-531, 1881-1978).
 Csr compact(const CsrV& a, const uint8_t* keep_row, const uint8_t* keep_col) {
+#ifdef _OPENMP
+  if (omp_get_max_threads() > 1 && a.nnz > 65536) return compact_threads(a, keep_row, keep_col);
+#endif
   Csr out;
   std::vector<int32_t> col_new(a.cols, -1);
   int32_t n_new = 0;
@@ -174,6 +279,7 @@ void sort_rows_in_place(const CsrV& a) {
 // gpu_presolve_kernels.cu:18-66
 void row_activity(const CsrV& A, const double* l, const double* u, double ztol, double* min_fin,
                   double* max_fin, int32_t* neg_inf, int32_t* pos_inf) {
+#pragma omp parallel for schedule(static) if (A.nnz > 65536)
   for (int32_t i = 0; i < A.rows; ++i) {
     double mn = 0.0, mx = 0.0;
     int32_t ni = 0, pi = 0;
