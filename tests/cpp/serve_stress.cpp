@@ -35,6 +35,10 @@ int main(int argc, char** argv) {
   int done = 0, failed = 0;
   while (done < jobs) {
     const int n = gp_server_wait(srv, res.data(), static_cast<int>(res.size()), 5.0);
+    if (n < 0) {
+      std::printf("wait failed: %s\n", gp_last_error());
+      return 2;
+    }
     for (int i = 0; i < n; ++i) failed += res[static_cast<size_t>(i)].status != GP_OK;
     done += n;
   }
