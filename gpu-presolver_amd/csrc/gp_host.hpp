@@ -40,6 +40,15 @@ class InputError : public std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// The LP has a block structure on which one of the structure-specific rules
+// would act; those rules are not implemented, so the presolve stops instead
+// of returning a record that differs from the reference's.  The C-ABI
+// reports it as GP_ERR_UNSUPPORTED.
+class UnsupportedStructure : public std::runtime_error {
+ public:
+  using std::runtime_error::runtime_error;
+};
+
 inline void check(hipError_t status, const char* context) {
   if (status == hipSuccess) return;
   throw HipError(context, status);
@@ -386,7 +395,8 @@ double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n);
 // Compacts A by row/column keep maps; the result owns fresh device arrays.
 // known_nnz >= 0 is the kept count when the caller knows it (the transpose
 // of a matrix compacted with the same maps): the read-back is skipped.
-// col_rank: the column map's rank words when the caller has them.
+// col_rank: the column map's rank words when the caller has them (unused
+// when n_new == A.cols: no column is dropped and col_map is the identity).
 presolve::DeviceCsrMatrix compact_csr(const presolve::DeviceCsrMatrix& A, const int32_t* row_map,
                                       const int32_t* col_map, int32_t m_new, int32_t n_new, int32_t known_nnz = -1,
                                       const gpk_rank_word* col_rank = nullptr);
@@ -399,6 +409,10 @@ void sort_rows_if_needed(const presolve::DeviceCsrMatrix& A);
 // column repeated within a row; one wait) and throws InputError naming the
 // first problem.  Unsorted rows are accepted.
 void check_lp_arrays(const presolve::LPInfoGpu& lp, const char* who);
+// Fail-closed screens of the structure-specific rules (host_screens.cpp):
+// each throws UnsupportedStructure when its rule would act on the LP.
+void screen_bootstrap_structures(const presolve::LPInfoGpu& lp, const presolve::PresolveParams& prm);
+void screen_structural_l1(const presolve::LPInfoGpu& lp, const presolve::PresolveParams& prm);
 // old_to_new map of a keep mask; returns the kept count.
 int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map);
 // FixedCol records and the long-double objective delta

@@ -84,6 +84,9 @@ gp_status guarded(F&& f) {
   } catch (const amd::InputError& e) {
     t_error = e.what();
     return GP_ERR_INVALID_ARGUMENT;
+  } catch (const amd::UnsupportedStructure& e) {
+    t_error = e.what();
+    return GP_ERR_UNSUPPORTED;
   } catch (const amd::HipError& e) {
     t_error = e.what();
     return e.status() == hipErrorNoDevice || e.status() == hipErrorInvalidDevice ? GP_ERR_NO_DEVICE : GP_ERR_DEVICE;
@@ -195,6 +198,25 @@ PresolveParams to_params(const gp_params* q) {
   p.use_tiered_scheduler = q->use_tiered_scheduler != 0;
   p.enable_tiered_bootstrap = q->enable_tiered_bootstrap != 0;
   p.implied_variable_bounds_bound_only_nnz_round_budget = q->implied_variable_bounds_bound_only_nnz_round_budget;
+  p.enable_bounded_two_row_projection = q->enable_bounded_two_row_projection != 0;
+  p.bounded_two_row_projection_max_row_nnz = q->bounded_two_row_projection_max_row_nnz;
+  p.bounded_two_row_projection_safety_tol = q->bounded_two_row_projection_safety_tol;
+  p.bounded_two_row_projection_min_candidates = q->bounded_two_row_projection_min_candidates;
+  p.bounded_two_row_projection_min_candidate_ratio = q->bounded_two_row_projection_min_candidate_ratio;
+  p.enable_orphan_mccormick_projection = q->enable_orphan_mccormick_projection != 0;
+  p.orphan_mccormick_projection_min_candidates = q->orphan_mccormick_projection_min_candidates;
+  p.orphan_mccormick_projection_min_candidate_ratio = q->orphan_mccormick_projection_min_candidate_ratio;
+  p.enable_linf_components = q->enable_linf_components != 0;
+  p.linf_probe_pairs = q->linf_probe_pairs;
+  p.linf_min_pairs = q->linf_min_pairs;
+  p.linf_max_tail_cols = q->linf_max_tail_cols;
+  p.enable_structural_l1_substitution = q->enable_structural_l1_substitution != 0;
+  p.enable_antipodal_components = q->enable_antipodal_components != 0;
+  p.antipodal_probe_pairs = q->antipodal_probe_pairs;
+  p.antipodal_min_pairs = q->antipodal_min_pairs;
+  p.enable_covering_cost_dominance = q->enable_covering_cost_dominance != 0;
+  p.covering_cost_dominance_min_fixed_cols = q->covering_cost_dominance_min_fixed_cols;
+  p.covering_cost_dominance_min_candidate_ratio = q->covering_cost_dominance_min_candidate_ratio;
   return p;
 }
 
@@ -292,6 +314,25 @@ void gp_params_default(gp_params* q) {
   q->use_tiered_scheduler = p.use_tiered_scheduler;
   q->enable_tiered_bootstrap = p.enable_tiered_bootstrap;
   q->implied_variable_bounds_bound_only_nnz_round_budget = p.implied_variable_bounds_bound_only_nnz_round_budget;
+  q->enable_bounded_two_row_projection = p.enable_bounded_two_row_projection;
+  q->bounded_two_row_projection_max_row_nnz = p.bounded_two_row_projection_max_row_nnz;
+  q->bounded_two_row_projection_safety_tol = p.bounded_two_row_projection_safety_tol;
+  q->bounded_two_row_projection_min_candidates = p.bounded_two_row_projection_min_candidates;
+  q->bounded_two_row_projection_min_candidate_ratio = p.bounded_two_row_projection_min_candidate_ratio;
+  q->enable_orphan_mccormick_projection = p.enable_orphan_mccormick_projection;
+  q->orphan_mccormick_projection_min_candidates = p.orphan_mccormick_projection_min_candidates;
+  q->orphan_mccormick_projection_min_candidate_ratio = p.orphan_mccormick_projection_min_candidate_ratio;
+  q->enable_linf_components = p.enable_linf_components;
+  q->linf_probe_pairs = p.linf_probe_pairs;
+  q->linf_min_pairs = p.linf_min_pairs;
+  q->linf_max_tail_cols = p.linf_max_tail_cols;
+  q->enable_structural_l1_substitution = p.enable_structural_l1_substitution;
+  q->enable_antipodal_components = p.enable_antipodal_components;
+  q->antipodal_probe_pairs = p.antipodal_probe_pairs;
+  q->antipodal_min_pairs = p.antipodal_min_pairs;
+  q->enable_covering_cost_dominance = p.enable_covering_cost_dominance;
+  q->covering_cost_dominance_min_fixed_cols = p.covering_cost_dominance_min_fixed_cols;
+  q->covering_cost_dominance_min_candidate_ratio = p.covering_cost_dominance_min_candidate_ratio;
 }
 
 gp_status gp_device_free(void* device_ptr) {

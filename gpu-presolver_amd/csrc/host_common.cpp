@@ -483,7 +483,9 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
   DBuf<gpk_rank_word> rank;
   DBuf<int32_t> jn(A.nnz, "compact entry map");
   hipStream_t s = c.stream();
-  if (col_rank == nullptr) {
+  if (n_new == A.cols) {
+    col_rank = nullptr;  // no column dropped: every entry keeps its column index
+  } else if (col_rank == nullptr) {
     rank.alloc((static_cast<int64_t>(A.cols) + 63) / 64, "compact column rank");
     check(gpk_build_rank(col_map, A.cols, rank, s), "compact column rank");
     col_rank = rank.get();

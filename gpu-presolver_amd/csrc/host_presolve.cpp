@@ -561,10 +561,11 @@ class StepProfile {
   std::map<std::string, Entry> t_;
 };
 
-// run_gpu_presolve_impl (gpu_presolve.cu:2162-3400) without the six
-// structure-specific rules (they find nothing on LPs without their block
-// patterns, so both values of enable_structure_specific_rules schedule the
-// same phases here).
+// run_gpu_presolve_impl (gpu_presolve.cu:2162-3400).  The six
+// structure-specific rules are not implemented: with
+// enable_structure_specific_rules they are screened where the reference runs
+// them, and an LP on which one would act is refused (UnsupportedStructure);
+// on every other LP they do nothing, as in the reference.
 GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool keep_reduced) {
   amd::AbandonOnThrow abandon_guard;
   validate_lp(lp);
@@ -942,26 +943,38 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     }
   } else {
     if (prm.enable_tiered_bootstrap && prm.max_iters > 0 && !time_exceeded()) {
-      // Bootstrap (gpu_presolve.cu:3082-3332) without the structure detectors.
-      row_phase(true, false, false, false, false);
-      row_phase(false, true, false, false, false);
-      row_phase(false, false, true, false, false);
-      row_phase(false, false, false, false, true);
+      // Bootstrap (gpu_presolve.cu:3082-3332).  The structure-specific rules
+      // are screened where the reference runs them (the model-wide analyzers
+      // and the structural L1 phase first, the structural L1 phase again
+      // after a bootstrap that changed the LP); a screen that finds their
+      // structure stops the presolve.
+      amd::screen_bootstrap_structures(cur.lp, prm);
+      bool generic = false;
+      generic = row_phase(true, false, false, false, false) || generic;
+      generic = row_phase(false, true, false, false, false) || generic;
+      generic = row_phase(false, false, true, false, false) || generic;
+      generic = row_phase(false, false, false, false, true) || generic;
       bool ch = col_phase(true, true, false, false, false, false, false, false);
-      if (ch) trivial_cleanup();
-      col_phase(false, true, false, false, false, false, false, false);
+      generic = ch || generic;
+      if (ch) generic = trivial_cleanup() || generic;
+      generic = col_phase(false, true, false, false, false, false, false, false) || generic;
       do {
         ch = col_phase(false, false, true, false, false, false, false, false);
-        if (ch) trivial_cleanup();
+        generic = ch || generic;
+        if (ch) generic = trivial_cleanup() || generic;
       } while (ch && !terminal && !time_exceeded());
       do {
         ch = col_phase(false, false, false, true, false, false, false, false);
-        if (ch) trivial_cleanup();
+        generic = ch || generic;
+        if (ch) generic = trivial_cleanup() || generic;
       } while (ch && !terminal && !time_exceeded());
       ch = col_phase(false, false, false, false, false, true, false, false);
-      if (ch) trivial_cleanup();
+      generic = ch || generic;
+      if (ch) generic = trivial_cleanup() || generic;
       ch = col_phase(false, false, false, false, false, false, true, false);
-      if (ch) trivial_cleanup();
+      generic = ch || generic;
+      if (ch) generic = trivial_cleanup() || generic;
+      if (generic && !terminal && !time_exceeded()) amd::screen_structural_l1(cur.lp, prm);
       ++res.iterations;
     }
     bool fast = true;

@@ -500,7 +500,8 @@ __global__ __launch_bounds__(kSmallBlock) void k_scan_small(const In* in, int32_
 // Compaction (gpu_presolve.cu:1881-1978): after the column map's rank
 // words, three launches over tiles of 256 old rows: k_compact_map maps
 // every entry once (row kept and column rank,
-// one cache-resident rank gather per entry), stores the new column index or
+// one cache-resident rank gather per entry; with no rank words, when no
+// column is dropped, the column keeps its index), stores the new column index or
 // -1 per entry, each kept row's kept count at out_ptr[new row] and the
 // tile's kept count; an exclusive scan of the tile counts (one small array,
 // m/256 entries) gives every tile its output offset; k_compact_write turns
@@ -594,12 +595,14 @@ __global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restric
 #pragma unroll
     for (int k = 0; k < kCompItems; ++k) cj[k] = idx[min(c + k * kBlock + tid, seg_e - 1)];
     uint4 w[kCompItems];
+    if (col_rank != nullptr) {
 #pragma unroll
-    for (int k = 0; k < kCompItems; ++k) w[k] = rank_word(col_rank, live[k] ? cj[k] : 0);
+      for (int k = 0; k < kCompItems; ++k) w[k] = rank_word(col_rank, live[k] ? cj[k] : 0);
+    }
 #pragma unroll
     for (int k = 0; k < kCompItems; ++k) {
       const int32_t p = c + k * kBlock + tid;
-      const int32_t jn = live[k] ? rank_of(w[k], cj[k]) : -1;
+      const int32_t jn = !live[k] ? -1 : col_rank != nullptr ? rank_of(w[k], cj[k]) : cj[k];
       if (p < seg_e) jn_out[p] = jn;
       comp_count_run(rr[k], rr[k] >= 0, jn >= 0, s_cnt);
     }

@@ -74,6 +74,25 @@ _PARAM_FIELDS = [
     ("enable_zero_cost_redundant_box_bounds", C.c_int32), ("enable_redundant_bounds", C.c_int32),
     ("use_tiered_scheduler", C.c_int32), ("enable_tiered_bootstrap", C.c_int32),
     ("implied_variable_bounds_bound_only_nnz_round_budget", C.c_int64),
+    ("enable_bounded_two_row_projection", C.c_int32),
+    ("bounded_two_row_projection_max_row_nnz", C.c_int32),
+    ("bounded_two_row_projection_safety_tol", C.c_double),
+    ("bounded_two_row_projection_min_candidates", C.c_int32),
+    ("bounded_two_row_projection_min_candidate_ratio", C.c_double),
+    ("enable_orphan_mccormick_projection", C.c_int32),
+    ("orphan_mccormick_projection_min_candidates", C.c_int32),
+    ("orphan_mccormick_projection_min_candidate_ratio", C.c_double),
+    ("enable_linf_components", C.c_int32),
+    ("linf_probe_pairs", C.c_int32),
+    ("linf_min_pairs", C.c_int32),
+    ("linf_max_tail_cols", C.c_int32),
+    ("enable_structural_l1_substitution", C.c_int32),
+    ("enable_antipodal_components", C.c_int32),
+    ("antipodal_probe_pairs", C.c_int32),
+    ("antipodal_min_pairs", C.c_int32),
+    ("enable_covering_cost_dominance", C.c_int32),
+    ("covering_cost_dominance_min_fixed_cols", C.c_int32),
+    ("covering_cost_dominance_min_candidate_ratio", C.c_double),
 ]
 
 

@@ -106,7 +106,8 @@ hipError_t gpk_apply_maps(gpk_apply_maps_args args, gpk_lookback lb, hipStream_t
  *   gpk_compact_write  turns out_ptr into the new row pointers
  *                      (out_ptr[m_new] = kept entries) and writes the kept
  *                      (column, value) pairs.
- * jn holds one int32 per entry of A, tile_cnt tiles + 1. */
+ * jn holds one int32 per entry of A, tile_cnt tiles + 1.  col_rank NULL
+ * means no column is dropped (every kept entry keeps its column index). */
 int32_t gpk_compact_tiles(int32_t m_old);
 hipError_t gpk_compact_map(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
                            const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, int32_t* tile_cnt,
@@ -530,6 +531,43 @@ hipError_t gpk_rb_select(gpk_csr A, gpk_csr AT, const uint8_t* dlo, const uint8_
                          uint8_t* sel_lo, uint8_t* sel_hi, int32_t* flags, hipStream_t stream);
 hipError_t gpk_rb_apply(int32_t n, gpk_plan plan, const uint8_t* sel_lo, const uint8_t* sel_hi,
                         hipStream_t stream);
+
+/* Fail-closed screens of the structure-specific rules (k_screens.hip).
+ * Counter words in `out` start at zero.
+ *   covering:   out[0] != 0 when a row, column or entry breaks the unit set
+ *               covering form (rule_covering_cost_dominance.cu:112-196),
+ *               out[1] = columns whose cost exceeds their row count;
+ *   bounded:    *selected = non-conflicting projectable columns
+ *               (rule_bounded_two_row_projection.cu:236-335); cand: n bytes,
+ *               row_hits: m zeroed words;
+ *   mccormick:  the same for orphan McCormick columns
+ *               (rule_orphan_mccormick_projection.cu:117-244); ends: 2n words;
+ *   pair probe: kind 0 antipodal, 1 Linf: out[0] != 0 when a sampled pair
+ *               fails, out[1] = sampled merge rows
+ *               (rule_antipodal_components.cu:41-125, rule_linf_components.cu:108-220);
+ *   structural: *out = 1 when rows 0..3 open an L1 split, outer pair or
+ *               graph block (rule_structural_l1_substitution.cu:2214-2376). */
+hipError_t gpk_screen_covering(const int32_t* ptr, const int32_t* idx, const double* val, const int32_t* at_ptr,
+                               const double* at_val, const double* c, const double* AL, const double* AU,
+                               const double* l, const double* u, int32_t m, int32_t n, int32_t nnz, int32_t* out,
+                               hipStream_t stream);
+hipError_t gpk_screen_bounded_two_row(const int32_t* ptr, const int32_t* idx, const double* val,
+                                      const int32_t* at_ptr, const int32_t* at_idx, const double* c,
+                                      const double* AL, const double* AU, const double* l, const double* u,
+                                      int32_t n, int32_t max_row, double ztol, double tol, uint8_t* cand,
+                                      int32_t* row_hits, int32_t* selected, hipStream_t stream);
+hipError_t gpk_screen_mccormick(const int32_t* ptr, const int32_t* idx, const double* val, const int32_t* at_ptr,
+                                const int32_t* at_idx, const double* c, const double* AL, const double* AU,
+                                const double* l, const double* u, int32_t n, uint8_t* cand, int32_t* ends,
+                                int32_t* row_hits, int32_t* selected, hipStream_t stream);
+hipError_t gpk_screen_pair_probe(int32_t kind, const int32_t* ptr, const int32_t* idx, const double* val,
+                                 const int32_t* at_ptr, const int32_t* at_idx, const double* at_val,
+                                 const double* c, const double* AL, const double* AU, const double* l,
+                                 const double* u, int32_t m, int32_t n, int32_t pairs, int32_t pair_probes,
+                                 int32_t row_probes, int32_t* out, hipStream_t stream);
+hipError_t gpk_screen_structural_prefix(const int32_t* ptr, const int32_t* idx, const double* val,
+                                        const double* AL, const double* AU, int32_t m, int32_t* out,
+                                        hipStream_t stream);
 
 #ifdef __cplusplus
 }

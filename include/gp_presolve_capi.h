@@ -34,6 +34,9 @@ typedef enum gp_status {
   GP_ERR_DEVICE = 2,      /* a HIP runtime call failed */
   GP_ERR_INTERNAL = 3,    /* std::exception inside the library */
   GP_ERR_NO_DEVICE = 4,   /* no usable gfx950 device */
+  GP_ERR_UNSUPPORTED = 5, /* a structure-specific rule would act on the LP: those rules are not
+                             implemented, so the presolve is refused (disable the rule or
+                             enable_structure_specific_rules to presolve without it) */
 } gp_status;
 
 /* int32 zero-based CSR (reference DeviceCsrMatrix, presolve_structs.hpp:137-144). */
@@ -102,6 +105,28 @@ typedef struct gp_params {
   int32_t use_tiered_scheduler;
   int32_t enable_tiered_bootstrap;
   int64_t implied_variable_bounds_bound_only_nnz_round_budget;
+  /* The structure-specific rules' switches and the thresholds their
+   * screens read (presolve_structs.hpp:75-115, in that order); appended after
+   * the fields above. */
+  int32_t enable_bounded_two_row_projection;
+  int32_t bounded_two_row_projection_max_row_nnz;
+  double bounded_two_row_projection_safety_tol;
+  int32_t bounded_two_row_projection_min_candidates;
+  double bounded_two_row_projection_min_candidate_ratio;
+  int32_t enable_orphan_mccormick_projection;
+  int32_t orphan_mccormick_projection_min_candidates;
+  double orphan_mccormick_projection_min_candidate_ratio;
+  int32_t enable_linf_components;
+  int32_t linf_probe_pairs;
+  int32_t linf_min_pairs;
+  int32_t linf_max_tail_cols;
+  int32_t enable_structural_l1_substitution;
+  int32_t enable_antipodal_components;
+  int32_t antipodal_probe_pairs;
+  int32_t antipodal_min_pairs;
+  int32_t enable_covering_cost_dominance;
+  int32_t covering_cost_dominance_min_fixed_cols;
+  double covering_cost_dominance_min_candidate_ratio;
 } gp_params;
 
 /* Summary scalars of one presolve (reference GpuPresolveSummary, gpu_presolve.hpp:9-29). */

@@ -90,6 +90,27 @@ void default_params(Params* p) {
   p->use_tiered_scheduler = 1;
   p->enable_tiered_bootstrap = 1;
   p->implied_variable_bounds_bound_only_nnz_round_budget = 8000000;
+  // The structure-specific rules (presolve_structs.hpp:75-115): the oracle
+  // does not run them; the fields only keep the struct at its defaults.
+  p->enable_bounded_two_row_projection = 1;
+  p->bounded_two_row_projection_max_row_nnz = 8;
+  p->bounded_two_row_projection_safety_tol = 1.0e-8;
+  p->bounded_two_row_projection_min_candidates = 128;
+  p->bounded_two_row_projection_min_candidate_ratio = 0.01;
+  p->enable_orphan_mccormick_projection = 1;
+  p->orphan_mccormick_projection_min_candidates = 128;
+  p->orphan_mccormick_projection_min_candidate_ratio = 0.01;
+  p->enable_linf_components = 1;
+  p->linf_probe_pairs = 256;
+  p->linf_min_pairs = 4096;
+  p->linf_max_tail_cols = 8;
+  p->enable_structural_l1_substitution = 1;
+  p->enable_antipodal_components = 1;
+  p->antipodal_probe_pairs = 256;
+  p->antipodal_min_pairs = 4096;
+  p->enable_covering_cost_dominance = 1;
+  p->covering_cost_dominance_min_fixed_cols = 4096;
+  p->covering_cost_dominance_min_candidate_ratio = 0.20;
 }
 
 Csr copy_csr(const CsrV& a) {

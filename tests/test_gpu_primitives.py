@@ -86,6 +86,22 @@ def test_compact(m, n, seed, frac):
         assert_same_array(name, a, b)
 
 
+@pytest.mark.parametrize("m,n,seed,frac_r,frac_c", [(20000, 25000, 10, 0.3, 0.0), (20000, 25000, 11, 0.0, 0.3),
+                                                   (65537, 3000, 12, 0.2, 0.0)])
+def test_compact_one_side(m, n, seed, frac_r, frac_c):
+    """Only rows or only columns dropped: with every column kept the entries
+    keep their column indices without the rank words."""
+    ptr, idx, val = _random_csr(m, n, seed)
+    rng = np.random.default_rng(seed)
+    kr = (rng.random(m) >= frac_r).astype(np.uint8)
+    kc = (rng.random(n) >= frac_c).astype(np.uint8)
+    got = gp_ffi.compact_csr(m, n, ptr, idx, val, kr, kc)
+    want = O.compact_csr(m, n, ptr, idx, val, kr, kc)
+    assert got[:2] == want[:2]
+    for name, a, b in zip(("ptr", "idx", "val"), got[2:], want[2:]):
+        assert_same_array(name, a, b)
+
+
 def test_compact_many_tiles():
     """More than 16,384 tiles of 256 rows: the tile offsets come from the
     multi-workgroup look-back scan instead of the one-workgroup scan."""
