@@ -26,6 +26,9 @@ Extra objects on the JSON line:
                      single thread and the OpenMP build on every usable core,
                      median of five runs after a warm-up, with the core count
                      and CPU model;
+  long_rows          the same LP shape with row lengths U[2,64] (~30.5M nnz), on
+                     which the batched singleton closure runs at its default
+                     10M-nonzero threshold (presolve ms, reduced sizes);
   batched            4,096 LPs generated on the device, presolved together,
                      against one-at-a-time plain presolve (LP/s, speedup);
   serving            60 s of Poisson(200/s) presolve + postsolve requests whose
@@ -557,6 +560,43 @@ def _sizes(seed, count, lo, hi):
     return np.exp(rng.uniform(np.log(lo), np.log(hi), size=count)).astype(np.int64)
 
 
+def long_rows(gp_ffi, args):
+    """The single large LP with row lengths U[2,64] (1M x 1.5M, about 30.5M
+    nonzeros): the working LP keeps more than 10M nonzeros into the fast
+    phase, so the batched singleton closure runs at its default threshold.
+    Presolve time (mean of 3 after one warm-up) and the reduced sizes, which
+    must equal the oracle's (tests/golden/long_rows_presolve.json)."""
+    import lp_synth
+
+    t = time.time()
+    lp = lp_synth.config1(seed=1, row_len=(2, 64))
+    nnz = len(lp["A_idx"])
+    log(f"long rows: generated LP {lp['m']}x{lp['n']} nnz={nnz} in {time.time() - t:.1f}s")
+    d = gp_ffi.DeviceLp(lp)
+    del lp
+    prm = gp_ffi.params(enable_structure_specific_rules=0)
+    gp_ffi.Presolved(d, prm, keep_reduced=True).free()
+    times, s = [], None
+    for _ in range(3):
+        gp_ffi.hip_check(gp_ffi.hip().hipDeviceSynchronize(), "sync")
+        t = time.perf_counter()
+        h = gp_ffi.Presolved(d, prm, keep_reduced=True)
+        s = h.summary()
+        gp_ffi.hip_check(gp_ffi.hip().hipDeviceSynchronize(), "sync")
+        times.append(1e3 * (time.perf_counter() - t))
+        h.free()
+    with open(os.path.join(ROOT, "tests", "golden", "long_rows_presolve.json")) as f:
+        want = json.load(f)["summary"]
+    got = [s["reduced_rows"], s["reduced_cols"], s["reduced_nnz"]]
+    if got != [want["reduced_rows"], want["reduced_cols"], want["reduced_nnz"]]:
+        raise SystemExit(f"bench.py: long-row presolve gives {got}, the oracle {want}")
+    ms = sum(times) / len(times)
+    log(f"long rows: presolve {ms:.1f} ms (runs {[round(x, 1) for x in times]}) -> {got}")
+    return {"ms_per_presolve": ms, "lp_per_s": 1e3 / ms, "m": 1_000_000, "n": 1_500_000, "nnz": nnz,
+            "row_len": "U[2,64]", "reduced": got, "iterations": s["iterations"],
+            "check": "reduced sizes equal the oracle's (record digests are compared in the GPU tests)"}
+
+
 def many_batched(gp_ffi, args):
     """The batched workload on LPs generated on the device: 4,096 LPs,
     m log-uniform in [100, 20,000], n = 1.5 m, row lengths Poisson(6) with
@@ -623,9 +663,19 @@ def many_serving(gp_ffi, args):
     specs = [gp_ffi.lp_spec(0xC000 + k, int(m), row_len=(4, 10)) for k, m in enumerate(sizes)]
     prm = gp_ffi.params(enable_structure_specific_rules=0)
     budget = 200e9
-    cfg = gp_ffi.serve_config(workers=args.serve_workers, postsolve=True, hbm_budget_bytes=budget, seed=99)
-    gp_ffi.serve_spec_trace(specs[:16], prm, cfg)  # warm-up
     k_cap = min(len(specs), 400)
+
+    def cfg_for(w):
+        return gp_ffi.serve_config(workers=w, postsolve=True, hbm_budget_bytes=budget, seed=99)
+
+    gp_ffi.serve_spec_trace(specs[:16], prm, cfg_for(4))  # warm-up
+    if args.serve_workers > 0:
+        workers, table = args.serve_workers, {}
+    else:  # capacity of the first requests at each worker count
+        cands = tuple(w for w in _worker_candidates(args) if w >= 8) or _worker_candidates(args)
+        workers, table = workloads.autotune_workers(lambda w: gp_ffi.serve_spec_trace(specs[:k_cap], prm, cfg_for(w)),
+                                                    cands)
+    cfg = cfg_for(workers)
     cap = workloads.latency_stats(gp_ffi.serve_spec_trace(specs[:k_cap], prm, cfg))
     res = gp_ffi.serve_spec_trace(specs, prm, cfg, arrival)
     if any(r["status"] != 0 for r in res):
@@ -650,7 +700,8 @@ def many_serving(gp_ffi, args):
         f"{cap['lp_per_s']:.0f} LP/s vs serial plain {serial_cap:.0f} LP/s -> {cap['lp_per_s'] / serial_cap:.2f}x")
     return {"value": st["lp_per_s"], "unit": "LP/s", "requests": len(res), "rate_per_s": rate, "seconds": secs,
             "m": "logU[1000,500000]", "n": "1.5 m", "row_len": "U[4,10]", "mean_m": float(sizes.mean()),
-            "hbm_budget_bytes": budget, "workers": args.serve_workers, "p50_ms": st["p50_ms"],
+            "hbm_budget_bytes": budget, "workers": workers,
+            "autotune_capacity_lp_per_s": {str(k): v for k, v in table.items()}, "p50_ms": st["p50_ms"],
             "p99_ms": st["p99_ms"], "max_ms": st["max_ms"], "generation_ms_mean": gen_ms,
             "capacity_lp_per_s": cap["lp_per_s"], "capacity_sample": k_cap,
             "serial_plain_capacity_lp_per_s": serial_cap, "capacity_vs_serial": cap["lp_per_s"] / serial_cap,
@@ -746,7 +797,8 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-pool", type=int, default=48)
     ap.add_argument("--cpu-runs", type=int, default=5, help="timed oracle runs per CPU baseline leg")
-    ap.add_argument("--serve-workers", type=int, default=12)
+    ap.add_argument("--serve-workers", type=int, default=0,
+                    help="serving workers (0: the count with the highest capacity on the first requests)")
     ap.add_argument("--no-many", action="store_true",
                     help="plain workload only (no batched / serving objects on the line)")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
@@ -887,6 +939,7 @@ def main():
     }
     if not args.no_many:
         del dlp
+        out["long_rows"] = long_rows(gp_ffi, args)
         out["batched"] = many_batched(gp_ffi, args)
         out["serving"] = many_serving(gp_ffi, args)
     print(json.dumps(out), flush=True)

@@ -74,6 +74,7 @@ int32_t orc_has_good_nnz_progress(int32_t before, int32_t after, double ratio) {
 
 void orc_set_batched_closure_min_nnz(int32_t nnz) { orc::set_batched_closure_min_nnz(nnz); }
 int32_t orc_batched_closure_min_nnz(void) { return orc::batched_closure_min_nnz(); }
+int64_t orc_batched_closure_runs(void) { return orc::batched_closure_runs(); }
 
 int orc_row_activity(const gp_csr* A, const double* l, const double* u, double ztol, double* mn, double* mx,
                      int32_t* ni, int32_t* pi) {

@@ -187,6 +187,7 @@ bool has_good_nnz_progress(int32_t before, int32_t after, double ratio);
 // closure (gpu_presolve.cu:2506: 10,000,000).
 void set_batched_closure_min_nnz(int32_t nnz);
 int32_t batched_closure_min_nnz();
+int64_t batched_closure_runs();
 
 struct RecordV {
   int32_t m0, n0, m1, n1;

@@ -51,6 +51,8 @@ def lib():
         L.orc_set_batched_closure_min_nnz.restype = None
         L.orc_batched_closure_min_nnz.argtypes = []
         L.orc_batched_closure_min_nnz.restype = C.c_int32
+        L.orc_batched_closure_runs.argtypes = []
+        L.orc_batched_closure_runs.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -78,6 +80,11 @@ def set_batched_closure_min_nnz(nnz):
 
 def batched_closure_min_nnz():
     return int(lib().orc_batched_closure_min_nnz())
+
+
+def batched_closure_runs():
+    """Times the oracle's batched singleton closure has run in this process."""
+    return int(lib().orc_batched_closure_runs())
 
 
 def _ptr(a):

@@ -258,10 +258,14 @@ namespace {
 // kMinNnzForBatchedClosure (gpu_presolve.cu:2506); adjustable so the closure
 // can be exercised on small LPs.
 int32_t g_closure_min_nnz = 10000000;
+// Times the closure ran since the process started (tests read it to show
+// that a workload reaches the closure at the default threshold).
+int64_t g_closure_runs = 0;
 }  // namespace
 
 void set_batched_closure_min_nnz(int32_t nnz) { g_closure_min_nnz = nnz; }
 int32_t batched_closure_min_nnz() { return g_closure_min_nnz; }
+int64_t batched_closure_runs() { return g_closure_runs; }
 
 bool has_good_nnz_progress(int32_t before, int32_t after, double ratio) {
   if (before <= 0) return false;
@@ -410,6 +414,7 @@ Result presolve(const LpV& lp, const Params& prm, bool keep_reduced) {
     if (terminal || time_exceeded() || cur.lp.A.nnz() < g_closure_min_nnz || !prm.enable_column_singletons_eq ||
         !prm.enable_dual_fix || !prm.enable_singleton_rows)
       return false;
+    ++g_closure_runs;
     OwnedPlan op(cur.lp);
     OwnedStats os(cur.lp);
     const LpV v = cur.lp.view();

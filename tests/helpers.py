@@ -68,3 +68,17 @@ def assert_same_lp(la, lb):
     for k in LP_KEYS:
         assert_same_array("lp " + k, la[k], lb[k])
     assert la["obj"] == lb["obj"], f"obj {la['obj']!r} vs {lb['obj']!r}"
+
+
+def digests(d, keys):
+    """SHA-256 of each array (dtype, shape and bytes), for comparing results
+    too large to keep as fixtures."""
+    import hashlib
+
+    out = {}
+    for k in keys:
+        a = np.ascontiguousarray(d[k])
+        h = hashlib.sha256(f"{a.dtype.str}{a.shape}".encode())
+        h.update(a.tobytes())
+        out[k] = h.hexdigest()
+    return out

@@ -1,6 +1,8 @@
 """Full presolve and postsolve on the GPU against the oracle: summary,
 record (maps, fixed values, tape) and reduced LP bit-exact on dyadic inputs;
 postsolve (x, y, z) within 1e-12."""
+import os
+
 import numpy as np
 import pytest
 
@@ -117,6 +119,46 @@ def test_config2_full_size():
     lp = lp_synth.config2(seed=2)
     so, ro, lo = _run_both(lp, enable_structure_specific_rules=0)
     _postsolve_both(lp, ro, lo, seed=5)
+
+
+def test_config1_long_rows_full_size():
+    """The single large LP with row lengths U[2,64] (1M x 1.5M, ~30.5M nnz):
+    the working LP still holds more than 10M nonzeros when the fast phase
+    starts, so the batched singleton closure runs at its default threshold
+    (the oracle ran it once on this LP).  The oracle needs about two minutes
+    here, so its result is a committed fixture (digests of the record and
+    reduced-LP arrays, written by tests/golden/make_long_rows_fixture.py);
+    the GPU must match it bit for bit, and its postsolve of the same seeded
+    reduced point must match the oracle's sums and sums of squares to 1e-12
+    (relative)."""
+    import json
+
+    from helpers import LP_KEYS, RECORD_KEYS, digests
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "long_rows_presolve.json")) as f:
+        want = json.load(f)
+    assert want["closure_runs"] >= 1
+    assert gp_ffi.batched_closure_min_nnz() == 10_000_000
+    lp = lp_synth.config1(seed=1, row_len=(2, 64))
+    assert len(lp["A_idx"]) == want["nnz"]
+    s, rec, red = gp_ffi.presolve(lp, gp_ffi.params(**want["params"]), True)
+    for k, v in want["summary"].items():
+        assert int(s[k]) == v, k
+    assert s["obj_constant_delta"] == want["obj_constant_delta"]
+    assert [int(rec[k]) for k in ("m0", "n0", "m1", "n1")] == want["record_dims"]
+    assert rec["obj_constant_new"] == want["obj_constant_new"]
+    got = digests(rec, RECORD_KEYS)
+    for k in RECORD_KEYS:
+        assert got[k] == want["record"][k], f"record {k}"
+    got = digests(red, LP_KEYS)
+    for k in LP_KEYS:
+        assert got[k] == want["reduced"][k], f"reduced LP {k}"
+    x, y, z = lp_synth.reduced_solution(want["reduced_solution_seed"], red["m"], red["n"], red["l"], red["u"])
+    for tag, original in (("original", lp), ("record", None)):
+        for name, v in zip("xyz", gp_ffi.postsolve(rec, x, y, z, original)):
+            ref = want["postsolve_sum_sumsq"][tag][name]
+            for g, r in zip((float(np.sum(v)), float(np.sum(v * v))), ref):
+                assert abs(g - r) <= 1e-12 * max(1.0, abs(r)), (tag, name, g, r)
 
 
 def test_config1_full_size():
