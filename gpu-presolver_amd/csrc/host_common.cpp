@@ -571,7 +571,19 @@ void check_lp_arrays(const presolve::LPInfoGpu& lp, const char* who) {
   int32_t* f = c.flags();
   check(gpk_csr_check(lp.A.rowPtr, lp.A.colVal, lp.A.rows, lp.A.cols, lp.A.nnz, f, c.stream()), "check A");
   check(gpk_csr_check(lp.AT.rowPtr, lp.AT.colVal, lp.AT.rows, lp.AT.cols, lp.AT.nnz, f + 4, c.stream()), "check AT");
-  const int32_t* h = c.read_flags(8);
+  int32_t h[8];
+  std::memcpy(h, c.read_flags(8), sizeof(h));
+  const bool a_again = h[0] == 0 && h[2] == 0 && h[3] != 0, at_again = h[4] == 0 && h[6] == 0 && h[7] != 0;
+  if (a_again || at_again) {  // unsorted rows: look for columns repeated at a distance
+    f = c.flags();
+    if (a_again) check(gpk_csr_check_rows(lp.A.rowPtr, lp.A.colVal, lp.A.rows, lp.A.cols, lp.A.nnz, f, c.stream()),
+                       "check A rows");
+    if (at_again)
+      check(gpk_csr_check_rows(lp.AT.rowPtr, lp.AT.colVal, lp.AT.rows, lp.AT.cols, lp.AT.nnz, f + 4, c.stream()),
+            "check AT rows");
+    const int32_t* g = c.read_flags(8);
+    for (int k = 0; k < 8; ++k) h[k] |= g[k];
+  }
   static const char* const what[3] = {"row pointers are not monotone from 0 to nnz",
                                       "a column index is outside [0, cols)", "a row repeats a column index"};
   for (int m = 0; m < 2; ++m)

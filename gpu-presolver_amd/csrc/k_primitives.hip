@@ -952,18 +952,57 @@ __global__ void k_transpose_gather(const int32_t* perm, const int32_t* row_of, c
   at_val[k] = val[p];
 }
 
-__global__ void k_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t* flag) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows) return;
-  for (int32_t p = ptr[i] + 1; p < ptr[i + 1]; ++p) {
-    if (idx[p] < idx[p - 1]) {
-      wave_set_flag(flag);
-      return;
+// CSR checks over tiles of 256 rows with one thread per entry, so a long
+// row (a popular column of AT) is shared by the whole tile instead of walked
+// by one thread.  The tile's row pointers go to LDS; every entry finds its
+// row by a binary search of them and is compared with the entry before it in
+// the same row.  kFull: flags[0] bad row pointers (not monotone, not from 0
+// to nnz: the tile then reads no entry), flags[1] a column outside [0, cols),
+// flags[2] two equal neighbours, flags[3] a descending pair.  Otherwise the
+// pointers are trusted and only a descending pair is flagged, in flags[0].
+template <bool kFull>
+__global__ __launch_bounds__(kBlock) void k_csr_tiles(const int32_t* __restrict__ ptr,
+                                                      const int32_t* __restrict__ idx, int32_t rows, int32_t cols,
+                                                      int32_t nnz, int32_t* flags) {
+  __shared__ int32_t s_ptr[kBlock + 1];
+  __shared__ int32_t s_bad;
+  const int32_t r0 = blockIdx.x * kBlock;
+  const int nr = min(kBlock, rows - r0);
+  const int tid = threadIdx.x;
+  if (tid == 0) s_bad = 0;
+  for (int t = tid; t <= nr; t += kBlock) s_ptr[t] = ptr[r0 + t];
+  __syncthreads();
+  if (kFull && tid < nr) {
+    const int32_t b = s_ptr[tid], e = s_ptr[tid + 1];
+    if ((r0 + tid == 0 && b != 0) || (r0 + tid == rows - 1 && e != nnz) || b > e || b < 0 || e > nnz) s_bad = 1;
+  }
+  __syncthreads();
+  if (s_bad != 0) {
+    if (tid == 0) __hip_atomic_store(flags, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
+  bool range = false, equal = false, desc = false;
+  for (int32_t p = s_ptr[0] + tid; p < s_ptr[nr]; p += kBlock) {
+    const int row = find_row(s_ptr, nr, p);
+    const int32_t j = idx[p];
+    if (kFull) range = range || j < 0 || j >= cols;
+    if (p > s_ptr[row]) {
+      const int32_t q = idx[p - 1];
+      equal = equal || j == q;
+      desc = desc || j < q;
     }
+  }
+  if (kFull) {
+    if (range) wave_set_flag(flags + 1);
+    if (equal) wave_set_flag(flags + 2);
+    if (desc) wave_set_flag(flags + 3);
+  } else if (desc) {
+    wave_set_flag(flags);
   }
 }
 
-// Input check of a CSR matrix, one row per thread: flags[0] row pointers
+// Input check of a CSR matrix, one row per thread (for matrices whose rows
+// arrive unsorted: a column can repeat at a distance): flags[0] row pointers
 // that are not monotone or do not start at 0 / end at nnz, flags[1] a column
 // index outside [0, cols), flags[2] a column repeated within a row, flags[3]
 // a row whose columns are not ascending.  Sorted rows are checked for repeats
@@ -1441,12 +1480,19 @@ hipError_t gpk_transpose_gather(const int32_t* perm, const int32_t* row_of, cons
 
 hipError_t gpk_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t* flag, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_rows_unsorted, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, rows, flag);
+  hipLaunchKernelGGL(k_csr_tiles<false>, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, rows, 0, 0, flag);
   return hipGetLastError();
 }
 
 hipError_t gpk_csr_check(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t cols, int32_t nnz,
                          int32_t* flags, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_csr_tiles<true>, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, rows, cols, nnz, flags);
+  return hipGetLastError();
+}
+
+hipError_t gpk_csr_check_rows(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t cols, int32_t nnz,
+                              int32_t* flags, hipStream_t s) {
   if (rows <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_csr_check, dim3(grid_for(rows)), dim3(kBlock), 0, s, ptr, idx, rows, cols, nnz, flags);
   return hipGetLastError();

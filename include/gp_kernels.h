@@ -226,12 +226,18 @@ hipError_t gpk_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of, hipSt
 hipError_t gpk_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts, hipStream_t stream);
 hipError_t gpk_transpose_gather(const int32_t* perm, const int32_t* row_of, const double* val,
                                 int32_t nnz, int32_t* at_idx, double* at_val, hipStream_t stream);
-/* flag[0] |= 1 if some row of the CSR is not sorted by column. */
-/* Input check of a CSR matrix: flags[0] bad row pointers, flags[1] a column
- * index outside [0, cols), flags[2] a column repeated within a row, flags[3]
- * some row not sorted by column (flags are set to 1, never cleared). */
+/* Input check of a CSR matrix, one thread per entry over tiles of 256 rows:
+ * flags[0] bad row pointers, flags[1] a column index outside [0, cols),
+ * flags[2] two equal neighbours in a row, flags[3] a row not sorted by column
+ * (flags are set to 1, never cleared).  A row that is not sorted can repeat a
+ * column at a distance: gpk_csr_check_rows (one thread per row, every pair of
+ * an unsorted row) sets flags[2] for those as well. */
 hipError_t gpk_csr_check(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t cols, int32_t nnz,
                          int32_t* flags, hipStream_t stream);
+hipError_t gpk_csr_check_rows(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t cols, int32_t nnz,
+                              int32_t* flags, hipStream_t stream);
+/* flag[0] = 1 if some row of the CSR is not sorted by column (row pointers
+ * trusted). */
 hipError_t gpk_rows_unsorted(const int32_t* ptr, const int32_t* idx, int32_t rows, int32_t* flag,
                              hipStream_t stream);
 /* In-place sort of every row of a CSR by column (values follow). */
