@@ -82,3 +82,25 @@ def digests(d, keys):
         h.update(a.tobytes())
         out[k] = h.hexdigest()
     return out
+
+
+def lp_from_rows(rows, n, c, AL, AU, l, u):
+    """LP dict from rows given as lists of (column, value) in column order."""
+    m = len(rows)
+    ptr = np.zeros(m + 1, np.int32)
+    idx, val = [], []
+    for i, r in enumerate(rows):
+        for j, v in r:
+            idx.append(j)
+            val.append(v)
+        ptr[i + 1] = len(idx)
+    idx, val = np.array(idx, np.int32), np.array(val, np.float64)
+    order = np.argsort(idx, kind="stable")
+    tptr = np.zeros(n + 1, np.int32)
+    np.add.at(tptr, idx + 1, 1)
+    tptr = np.cumsum(tptr).astype(np.int32)
+    row_of = np.repeat(np.arange(m, dtype=np.int32), np.diff(ptr))
+    f = lambda a: np.array(a, np.float64)  # noqa: E731
+    return {"m": m, "n": n, "A_ptr": ptr, "A_idx": idx, "A_val": val, "AT_ptr": tptr,
+            "AT_idx": row_of[order].astype(np.int32), "AT_val": val[order], "c": f(c), "l": f(l), "u": f(u),
+            "AL": f(AL), "AU": f(AU), "obj": 0.0}
