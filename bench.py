@@ -22,6 +22,9 @@ Extra objects on the JSON line:
                      the column indices, row pointers from the sorted keys,
                      one gather), algorithmic bytes
                      4(m+1) + 12 nnz + 4(n+1) + 12 nnz + 8 n;
+  roofline_postsolve the prepared postsolve of the last record: tape records,
+                     dependency levels of the two replay passes, time per level
+                     and the bytes model (tape read per pass, outputs written);
   cpu_baseline       the C++ oracle presolving the same LP on the host:
                      single thread and the OpenMP build on every usable core,
                      median of five runs after a warm-up, with the core count
@@ -364,7 +367,7 @@ def measure_transpose(gp_ffi, dlp, lp, stream, launches):
             "workload": f"A of the LP ({m}x{n}, {nnz} nnz) -> AT ({n}x{m})"}
 
 
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "compaction_row_activity_hbm_traffic.md")
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_activity_compaction_transpose_1Mx1.5M.md")
 
 
 def profiled_traffic(kernel):
@@ -392,17 +395,19 @@ def activity_traffic():
 
 def transpose_traffic():
     """HBM bytes of one transpose from the committed PMC summary, or None."""
-    parts = [profiled_traffic(k) for k in ("k_radix_hist<unsigned int>", "k_radix_scatter<unsigned int, false>",
-                                           "k_col_bounds", "k_expand_rows", "k_transpose_gather")]
-    if any(p is None for p in parts):
+    per_pass = [profiled_traffic(k) for k in ("k_radix_hist<unsigned int>", "k_scan_tiles<int>", "k_scan_tile_sums",
+                                              "k_scan_down<int>", "k_radix_scatter<unsigned int, false>")]
+    once = [profiled_traffic(k) for k in ("k_col_bounds", "k_expand_rows", "k_transpose_gather")]
+    if any(p is None for p in per_pass + once):
         return None
-    return 3 * (parts[0] + parts[1]) + sum(parts[2:])
+    return 3 * sum(per_pass) + sum(once)
 
 
 def compaction_traffic():
     """HBM bytes of one compaction call: the sum over its kernels in the
     committed PMC summary, or None when one of them is missing."""
-    parts = [profiled_traffic(k) for k in ("k_build_rank", "k_compact_map", "k_scan_small<int, 0>", "k_compact_write")]
+    parts = [profiled_traffic(k) for k in ("k_build_rank", "k_compact_map", "k_scan_small<int, 0>",
+                                           "k_compact_write<false>")]
     return None if any(p is None for p in parts) else sum(parts)
 
 
@@ -886,10 +891,41 @@ def main():
         ps.run_plan(plan)
         runs.append(1e3 * (time.perf_counter() - t))
     plan_ms = min(runs[1:])  # the first run captures the graph
+    shape = plan.shape()
     plan.free()
     last.free()
     log(f"postsolve (inputs resident; record tape uploaded by the call): {post_ms:.1f} ms (runs {post}); "
         f"prepared: {prep_ms:.1f} ms once, then {plan_ms:.1f} ms per run (runs {runs})")
+
+    # Postsolve model: each replay pass (primal, dual) reads the tape once, the
+    # outputs x, z (n) and y (m) are cleared and written and the reduced point
+    # scattered; the passes are chains of dependent levels, so the bound is the
+    # level count times a level's latency, not bandwidth.
+    levels = shape["primal_levels"] + shape["dual_levels"]
+    post_bytes = 2 * shape["tape_bytes"] + 8 * 2 * (2 * n + m) + 8 * (2 * red["n"] + red["m"])
+    post_roof = {"bound": "latency (dependent replay levels)", "records": shape["records"],
+                 "primal_levels": shape["primal_levels"], "dual_levels": shape["dual_levels"],
+                 "tape_bytes": shape["tape_bytes"], "ms_prepared_run": plan_ms,
+                 "us_per_level": 1e3 * plan_ms / max(1, levels), "algorithmic_bytes": post_bytes,
+                 "achieved": post_bytes / (plan_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": post_bytes / (plan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"postsolve: {shape['records']} records, {shape['primal_levels']} + {shape['dual_levels']} levels, "
+        f"{post_roof['us_per_level']:.1f} us per level")
+
+    # Postsolve model: each replay pass (primal, dual) reads the tape once, the
+    # outputs x, z (n) and y (m) are cleared and written and the reduced point
+    # scattered; the passes are chains of dependent levels, so the bound is the
+    # level count times a level's latency, not bandwidth.
+    levels = shape["primal_levels"] + shape["dual_levels"]
+    post_bytes = 2 * shape["tape_bytes"] + 8 * 2 * (2 * n + m) + 8 * (2 * red["n"] + red["m"])
+    post_roof = {"bound": "latency (dependent replay levels)", "records": shape["records"],
+                 "primal_levels": shape["primal_levels"], "dual_levels": shape["dual_levels"],
+                 "tape_bytes": shape["tape_bytes"], "ms_prepared_run": plan_ms,
+                 "us_per_level": 1e3 * plan_ms / max(1, levels), "algorithmic_bytes": post_bytes,
+                 "achieved": post_bytes / (plan_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                 "frac": post_bytes / (plan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+    log(f"postsolve: {shape['records']} records, {shape['primal_levels']} + {shape['dual_levels']} levels, "
+        f"{post_roof['us_per_level']:.1f} us per level")
 
     act, comp, tr = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
     cpu = None
@@ -935,6 +971,7 @@ def main():
                                "frac": tr["achieved"] / HBM_PEAK_GBS, "traffic": transpose_traffic(),
                                "us_per_call": tr["us"], "algorithmic_bytes": tr["bytes"],
                                "workload": tr["workload"], "kernels_us": tr["kernels_us"]},
+        "roofline_postsolve": post_roof,
         "cpu_baseline": cpu,
     }
     if not args.no_many:

@@ -316,6 +316,14 @@ class PostsolvePlan {
   int32_t m0() const;
   int32_t n1() const;
   int32_t m1() const;
+  // Tape records, dependency levels of the primal pass (over all its ranges)
+  // and of the dual pass, and the tape's bytes on the device.
+  struct Shape {
+    int64_t records = 0;
+    int32_t primal_levels = 0, dual_levels = 0;
+    int64_t tape_bytes = 0;
+  };
+  Shape shape() const;
 
  private:
   struct Impl;

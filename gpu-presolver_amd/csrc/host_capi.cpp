@@ -997,6 +997,14 @@ gp_status gp_postsolve_plan_run(gp_postsolve_plan* p, const double* x_red, const
 
 int64_t gp_postsolve_plan_graph_runs(const gp_postsolve_plan* p) { return p ? p->graph_runs : 0; }
 
+gp_status gp_postsolve_plan_shape(const gp_postsolve_plan* p, gp_postsolve_shape* out) {
+  return guarded([&] {
+    require(p != nullptr && p->plan != nullptr && out != nullptr, "null argument");
+    const amd::PostsolvePlan::Shape sh = p->plan->shape();
+    *out = gp_postsolve_shape{sh.records, sh.primal_levels, sh.dual_levels, sh.tape_bytes};
+  });
+}
+
 void gp_postsolve_plan_free(gp_postsolve_plan* p) {
   if (p == nullptr) return;
   (void)guarded([&] { amd::ctx().sync("gp_postsolve_plan_free"); });

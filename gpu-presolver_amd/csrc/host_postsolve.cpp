@@ -610,6 +610,7 @@ struct PostsolvePlan::Impl {
   };
   std::vector<Stage> primal;
   const double* original_l = nullptr;
+  int64_t records = 0, tape_bytes = 0;
 };
 
 PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* original, const double* original_l)
@@ -626,6 +627,13 @@ PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* o
   d.m0 = record.m0;
   d.n1 = record.n1;
   d.m1 = record.m1;
+  {
+    const presolve::PostsolveTape& t = record.tape;
+    d.records = static_cast<int64_t>(t.types.size());
+    d.tape_bytes = static_cast<int64_t>(sizeof(int32_t) * (t.types.size() + t.index_starts.size() +
+                                                            t.value_starts.size() + t.indices.size()) +
+                                        t.dual_modes.size() + sizeof(double) * t.vals.size());
+  }
   // GP_PRESOLVE_PROFILE=1: host time of the preparation steps on stderr.
   const bool prof = std::getenv("GP_PRESOLVE_PROFILE") != nullptr;
   auto tick = std::chrono::steady_clock::now();
@@ -724,6 +732,16 @@ void PostsolvePlan::enqueue(const double* x_red, const double* y_red, const doub
   if (d.project)
     check(gpk_project_duals(d.orig, d.prot, x_org, y, z_org, d.n0, kPostsolveTol, d.long_count, d.long_cols, s),
           "project duals");
+}
+
+PostsolvePlan::Shape PostsolvePlan::shape() const {
+  Shape out;
+  out.records = d_->records;
+  out.tape_bytes = d_->tape_bytes;
+  for (const auto& st : d_->primal)
+    out.primal_levels += static_cast<int32_t>(st.sched.level_start.empty() ? 0 : st.sched.level_start.size() - 1);
+  out.dual_levels = static_cast<int32_t>(d_->dual.level_start.empty() ? 0 : d_->dual.level_start.size() - 1);
+  return out;
 }
 
 }  // namespace gpu_presolver::amd

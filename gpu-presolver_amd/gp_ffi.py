@@ -458,6 +458,15 @@ class PostsolvePlan:
     def graph_runs(self):
         return lib().gp_postsolve_plan_graph_runs(self.h)
 
+    def shape(self):
+        """Tape records, primal and dual dependency levels, tape bytes."""
+        L = lib()
+        L.gp_postsolve_plan_shape.argtypes = [C.c_void_p, C.POINTER(T.GpPostsolveShape)]
+        out = T.GpPostsolveShape()
+        _check(L.gp_postsolve_plan_shape(self.h, C.byref(out)))
+        return {"records": out.records, "primal_levels": out.primal_levels, "dual_levels": out.dual_levels,
+                "tape_bytes": out.tape_bytes}
+
     def free(self):
         if self.h:
             lib().gp_postsolve_plan_free(self.h)

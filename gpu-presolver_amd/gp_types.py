@@ -129,6 +129,11 @@ class GpJobResult(C.Structure):
                 ("t_end", C.c_double), ("t_ready", C.c_double)]
 
 
+class GpPostsolveShape(C.Structure):
+    _fields_ = [("records", C.c_int64), ("primal_levels", C.c_int32), ("dual_levels", C.c_int32),
+                ("tape_bytes", C.c_int64)]
+
+
 class GpLpSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("m", C.c_int32), ("n", C.c_int32), ("len_lo", C.c_int32),
                 ("len_hi", C.c_int32), ("mean_len", C.c_double), ("min_len", C.c_int32)]

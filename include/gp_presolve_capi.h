@@ -326,6 +326,16 @@ gp_status gp_postsolve_plan_run(gp_postsolve_plan* plan, const double* x_red, co
                                 const double* z_red, double* x_org, double* y_org, double* z_org);
 /* Number of runs so far that replayed the captured graph. */
 int64_t gp_postsolve_plan_graph_runs(const gp_postsolve_plan* plan);
+/* Shape of a prepared postsolve: tape records, dependency levels of the
+ * primal replay (over all its ranges) and of the dual replay, and the bytes
+ * of the tape on the device. */
+typedef struct gp_postsolve_shape {
+  int64_t records;
+  int32_t primal_levels;
+  int32_t dual_levels;
+  int64_t tape_bytes;
+} gp_postsolve_shape;
+gp_status gp_postsolve_plan_shape(const gp_postsolve_plan* plan, gp_postsolve_shape* out);
 void gp_postsolve_plan_free(gp_postsolve_plan* plan);
 
 /* ---- serving and batching (no counterpart in the reference) ------------

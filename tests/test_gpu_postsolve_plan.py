@@ -51,3 +51,21 @@ def test_plan_missing_inputs_and_no_original():
             assert_same_array(name, a, b)
     assert plan.graph_runs() == 1  # only the run with all three inputs replays the graph
     plan.free()
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_plan_shape(case):
+    """gp_postsolve_plan_shape: records and tape bytes as the record holds
+    them; a pass has at least one level when the tape is not empty and no
+    more levels than records."""
+    lp, rec, red = _presolved(CASES[case])
+    plan = gp_ffi.PostsolvePlan(rec, original=lp, use_graph=False)
+    sh = plan.shape()
+    plan.free()
+    R = len(rec["types"])
+    assert sh["records"] == R
+    want = 4 * (R + len(rec["index_starts"]) + len(rec["value_starts"]) + len(rec["indices"])) + \
+        len(rec["dual_modes"]) + 8 * len(rec["vals"])
+    assert sh["tape_bytes"] == want
+    assert 0 <= sh["dual_levels"] <= R and 0 <= sh["primal_levels"] <= R
+    assert R == 0 or sh["dual_levels"] + sh["primal_levels"] >= 1
