@@ -319,14 +319,15 @@ def measure_transpose(gp_ffi, dlp, lp, stream, launches):
     L.gpk_radix_scratch_ints.argtypes = [C.c_int64]
     L.gpk_transpose_sort.argtypes = [vp, C.c_int32, C.c_int32, vp, vp, vp, vp, vp, C.POINTER(C.c_int32), vp]
     L.gpk_col_bounds.argtypes = [vp, C.c_int32, C.c_int32, vp, vp]
-    L.gpk_expand_rows.argtypes = [vp, C.c_int32, vp, vp]
-    L.gpk_transpose_gather.argtypes = [vp, vp, vp, C.c_int32, vp, vp, vp]
+    L.gpk_expand_rows.argtypes = [vp, vp, C.c_int32, vp, vp]
+    L.gpk_transpose_gather.argtypes = [vp, vp, C.c_int32, vp, vp, vp]
     bits = max(1, int(n - 1).bit_length())
     ka, kb = gp_ffi.DeviceArray(size=nnz, dtype=np.uint32), gp_ffi.DeviceArray(size=nnz, dtype=np.uint32)
     va, vb = gp_ffi.DeviceArray(size=nnz, dtype=np.int32), gp_ffi.DeviceArray(size=nnz, dtype=np.int32)
     scratch = gp_ffi.DeviceArray(size=L.gpk_radix_scratch_ints(nnz), dtype=np.int32)
     ptr = gp_ffi.DeviceArray(size=n + 1, dtype=np.int32)
     idx, val = gp_ffi.DeviceArray(size=nnz, dtype=np.int32), gp_ffi.DeviceArray(size=nnz)
+    rv = gp_ffi.DeviceArray(size=2 * nnz)  # 16-byte (row, value) pairs
     A = dlp.A.g
     in_b = C.c_int32(0)
     ck = gp_ffi.hip_check
@@ -336,10 +337,10 @@ def measure_transpose(gp_ffi, dlp, lp, stream, launches):
                                 stream.ptr), "transpose sort")
 
     def finish():
-        keys, perm, other = (kb, vb, ka) if in_b.value else (ka, va, kb)
+        keys, perm = (kb, vb) if in_b.value else (ka, va)
         ck(L.gpk_col_bounds(keys.ptr, nnz, n, ptr.ptr, stream.ptr), "col bounds")
-        ck(L.gpk_expand_rows(A.row_ptr, m, other.ptr, stream.ptr), "expand rows")
-        ck(L.gpk_transpose_gather(perm.ptr, other.ptr, A.val, nnz, idx.ptr, val.ptr, stream.ptr), "gather")
+        ck(L.gpk_expand_rows(A.row_ptr, A.val, m, rv.ptr, stream.ptr), "expand rows")
+        ck(L.gpk_transpose_gather(perm.ptr, rv.ptr, nnz, idx.ptr, val.ptr, stream.ptr), "gather")
 
     for _ in range(2):
         sort()

@@ -517,7 +517,8 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
 // Stable transpose: LSD radix sort of the column indices (entries already in
 // row-major order, so every AT row comes out sorted by row index) carrying
 // the entry positions; the row pointers come from the sorted keys (no
-// per-column atomic counters), then one gather writes rows and values.
+// per-column atomic counters), then one gather of packed (row, value) pairs
+// writes rows and values.
 DeviceCsrMatrix transpose_csr(const DeviceCsrMatrix& A) {
   Ctx& c = ctx();
   hipStream_t s = c.stream();
@@ -544,12 +545,11 @@ DeviceCsrMatrix transpose_csr(const DeviceCsrMatrix& A) {
   const uint32_t* keys = in_b ? kb.get() : ka.get();
   const int32_t* perm = in_b ? vb.get() : va.get();
   check(gpk_col_bounds(keys, nnz, A.cols, ptr, s), "transpose rowPtr");
-  // the other key buffer holds the row of every entry
-  int32_t* row_of = reinterpret_cast<int32_t*>(in_b ? ka.get() : kb.get());
-  check(gpk_expand_rows(A.rowPtr, A.rows, row_of, s), "expand rows");
+  DBuf<gpk_row_val> rv(nnz, "transpose rows and values");
+  check(gpk_expand_rows(A.rowPtr, A.nzVal, A.rows, rv, s), "expand rows");
   DBuf<int32_t> idx(nnz, "transpose colVal");
   DBuf<double> val(nnz, "transpose nzVal");
-  check(gpk_transpose_gather(perm, row_of, A.nzVal, nnz, idx, val, s), "transpose gather");
+  check(gpk_transpose_gather(perm, rv, nnz, idx, val, s), "transpose gather");
   T.rowPtr = ptr.release();
   T.colVal = idx.release();
   T.nzVal = val.release();

@@ -921,10 +921,24 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
 }
 
 // ------------------------------------------------------------- transpose ----
-__global__ void k_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) row_of[p] = i;
+// (row, value) of every entry of A packed into 16 bytes, so that the
+// transpose's gather reads one pair per entry instead of two scattered
+// words: one thread per entry over tiles of 256 rows (an entry finds its row
+// by a binary search of the tile's row pointers in LDS), so reads and writes
+// stay coalesced whatever the row lengths.
+__global__ __launch_bounds__(kBlock) void k_expand_rows(const int32_t* __restrict__ ptr,
+                                                        const double* __restrict__ val, int32_t m,
+                                                        gpk_row_val* __restrict__ rv) {
+  __shared__ int32_t s_ptr[kBlock + 1];
+  const int32_t r0 = blockIdx.x * kBlock;
+  const int nr = min(kBlock, m - r0);
+  for (int t = threadIdx.x; t <= nr; t += kBlock) s_ptr[t] = ptr[r0 + t];
+  __syncthreads();
+  for (int32_t p = s_ptr[0] + threadIdx.x; p < s_ptr[nr]; p += kBlock) {
+    const double v = val[p];
+    reinterpret_cast<int4*>(rv)[p] =
+        make_int4(r0 + find_row(s_ptr, nr, p), 0, __double2loint(v), __double2hiint(v));
+  }
 }
 
 // Row pointers of the transpose from its sorted column keys: position p
@@ -938,18 +952,13 @@ __global__ void k_col_bounds(const uint32_t* keys, int32_t nnz, int32_t n, int32
   for (int32_t j = lo + 1; j <= hi; ++j) ptr[j] = p;
 }
 
-__global__ void k_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts) {
-  const int32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < nnz) atomicAdd(&counts[idx[p]], 1);
-}
-
-__global__ void k_transpose_gather(const int32_t* perm, const int32_t* row_of, const double* val, int32_t nnz,
-                                   int32_t* at_idx, double* at_val) {
+__global__ void k_transpose_gather(const int32_t* __restrict__ perm, const gpk_row_val* __restrict__ rv,
+                                   int32_t nnz, int32_t* __restrict__ at_idx, double* __restrict__ at_val) {
   const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= nnz) return;
-  const int32_t p = perm[k];
-  at_idx[k] = row_of[p];
-  at_val[k] = val[p];
+  const int4 w = reinterpret_cast<const int4*>(rv)[perm[k]];  // one 16-byte load
+  at_idx[k] = w.x;
+  at_val[k] = __hiloint2double(w.w, w.z);
 }
 
 // CSR checks over tiles of 256 rows with one thread per entry, so a long
@@ -1458,23 +1467,16 @@ hipError_t gpk_col_bounds(const uint32_t* sorted_keys, int32_t nnz, int32_t n, i
   return hipGetLastError();
 }
 
-hipError_t gpk_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of, hipStream_t s) {
+hipError_t gpk_expand_rows(const int32_t* ptr, const double* val, int32_t m, gpk_row_val* rv, hipStream_t s) {
   if (m <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_expand_rows, dim3(grid_for(m)), dim3(kBlock), 0, s, ptr, m, row_of);
+  hipLaunchKernelGGL(k_expand_rows, dim3(grid_for(m)), dim3(kBlock), 0, s, ptr, val, m, rv);
   return hipGetLastError();
 }
 
-hipError_t gpk_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts, hipStream_t s) {
+hipError_t gpk_transpose_gather(const int32_t* perm, const gpk_row_val* rv, int32_t nnz, int32_t* at_idx,
+                                double* at_val, hipStream_t s) {
   if (nnz <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_col_histogram, dim3(grid_for(nnz)), dim3(kBlock), 0, s, idx, nnz, counts);
-  return hipGetLastError();
-}
-
-hipError_t gpk_transpose_gather(const int32_t* perm, const int32_t* row_of, const double* val, int32_t nnz,
-                                int32_t* at_idx, double* at_val, hipStream_t s) {
-  if (nnz <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_transpose_gather, dim3(grid_for(nnz)), dim3(kBlock), 0, s, perm, row_of, val, nnz, at_idx,
-                     at_val);
+  hipLaunchKernelGGL(k_transpose_gather, dim3(grid_for(nnz)), dim3(kBlock), 0, s, perm, rv, nnz, at_idx, at_val);
   return hipGetLastError();
 }
 

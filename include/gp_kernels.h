@@ -222,10 +222,18 @@ hipError_t gpk_radix_sort_u32_i32(uint32_t* keys, int32_t* vals, uint32_t* keys_
 hipError_t gpk_transpose_sort(const int32_t* col, int32_t nnz, int32_t key_bits, uint32_t* ka, int32_t* va,
                               uint32_t* kb, int32_t* vb, int32_t* scratch, int32_t* result_in_b, hipStream_t stream);
 hipError_t gpk_col_bounds(const uint32_t* sorted_keys, int32_t nnz, int32_t n, int32_t* ptr, hipStream_t stream);
-hipError_t gpk_expand_rows(const int32_t* ptr, int32_t m, int32_t* row_of, hipStream_t stream);
-hipError_t gpk_col_histogram(const int32_t* idx, int32_t nnz, int32_t* counts, hipStream_t stream);
-hipError_t gpk_transpose_gather(const int32_t* perm, const int32_t* row_of, const double* val,
-                                int32_t nnz, int32_t* at_idx, double* at_val, hipStream_t stream);
+/* The row and value of every entry of A, packed per entry (16 bytes, 16-byte
+ * aligned array of nnz), then the transpose's columns and values gathered
+ * through the sorted positions: at_idx[k] = rv[perm[k]].row, at_val[k] =
+ * rv[perm[k]].val. */
+typedef struct gpk_row_val {
+  int32_t row;
+  int32_t pad;
+  double val;
+} gpk_row_val;
+hipError_t gpk_expand_rows(const int32_t* ptr, const double* val, int32_t m, gpk_row_val* rv, hipStream_t stream);
+hipError_t gpk_transpose_gather(const int32_t* perm, const gpk_row_val* rv, int32_t nnz, int32_t* at_idx,
+                                double* at_val, hipStream_t stream);
 /* Input check of a CSR matrix, one thread per entry over tiles of 256 rows:
  * flags[0] bad row pointers, flags[1] a column index outside [0, cols),
  * flags[2] two equal neighbours in a row, flags[3] a row not sorted by column
