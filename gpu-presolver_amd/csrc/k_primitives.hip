@@ -810,7 +810,8 @@ __global__ void k_gather_i32(const int32_t* map, int32_t n_old, const int32_t* s
 // Per 64-key step the lanes holding the same digit are found with eight
 // ballots, ranked by lane order, and appended to a per-wave digit counter in
 // LDS; digit offsets across waves and tiles come from the global histogram
-// scan, so equal digits keep their input order.
+// scan, so equal digits keep their input order.  The ranked tile is staged
+// in LDS in digit order and stored run by run.
 constexpr int kRsTile = 2048;
 constexpr int kRsWaveItems = 512;
 constexpr int kRsWaves = kBlock / 64;
@@ -844,7 +845,10 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
   __shared__ int32_t s_cnt[kRsWaves][256];
   __shared__ int32_t s_off[kRsWaves][256];
   __shared__ int32_t s_base[256];
+  __shared__ int32_t s_start[256];
   __shared__ int32_t s_wsum[kRsWaves];
+  __shared__ K s_key[kRsTile];
+  __shared__ int32_t s_val[kRsTile];
   const int tid = threadIdx.x;
   for (int i = tid; i < kRsWaves * 256; i += kBlock) s_cnt[i >> 8][i & 255] = 0;
   if (kFromHist) {
@@ -899,24 +903,40 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
     __builtin_amdgcn_wave_barrier();
   }
   __syncthreads();
+  // Thread `tid` is digit tid: offsets of the waves' runs inside the tile's
+  // run of the digit, and the digit's run start in the tile's digit-sorted
+  // order.
+  int32_t tile_cnt;
   {
     int32_t run = 0;
     for (int w = 0; w < kRsWaves; ++w) {
       s_off[w][tid] = run;
       run += s_cnt[w][tid];
     }
+    s_start[tid] = block_exclusive_sum(run, &tile_cnt);
+    if (!kFromHist) s_base[tid] = offs[static_cast<int64_t>(tid) * tiles + blockIdx.x];
   }
   __syncthreads();
+  // The tile is staged in LDS in digit order, then written out by
+  // consecutive threads to consecutive positions of each digit's run, so
+  // the stores of a digit's run are contiguous.
 #pragma unroll
   for (int k = 0; k < kSteps; ++k) {
     const int64_t e = base + k * 64 + lane;
     if (e < n) {
       const int d = dig[k];
-      const int32_t tile_base = kFromHist ? s_base[d] : offs[static_cast<int64_t>(d) * tiles + blockIdx.x];
-      const int32_t pos = tile_base + s_off[wave][d] + rank[k];
-      kout[pos] = kk[k];
-      vout[pos] = vv[k];
+      const int32_t local = s_start[d] + s_off[wave][d] + rank[k];
+      s_key[local] = kk[k];
+      s_val[local] = vv[k];
     }
+  }
+  __syncthreads();
+  for (int i = tid; i < tile_cnt; i += kBlock) {
+    const K key = s_key[i];
+    const int d = static_cast<int>((key >> shift) & 255u);
+    const int32_t pos = s_base[d] + (i - s_start[d]);
+    kout[pos] = key;
+    vout[pos] = s_val[i];
   }
 }
 
