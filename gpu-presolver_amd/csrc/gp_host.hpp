@@ -420,6 +420,32 @@ void check_lp_arrays(const presolve::LPInfoGpu& lp, const char* who);
 // Fail-closed screens of the structure-specific rules (host_screens.cpp):
 // each throws UnsupportedStructure when its rule would act on the LP.
 void screen_bootstrap_structures(const presolve::LPInfoGpu& lp, const presolve::PresolveParams& prm);
+
+// ---- on-device LP solver (host_pdhg.cpp) -----------------------------------
+// Restarted PDHG on the Ruiz-scaled LP, with every vector resident on the
+// device; the host only reads the KKT sums once per check period.
+struct PdhgOptions {
+  int32_t max_iters = 200000;
+  double max_time = 3600.0;
+  double tol = 1e-4;          // relative KKT tolerance
+  int32_t check_every = 64;   // iterations per check period (one graph launch)
+  int32_t ruiz_iters = 10;
+  int32_t power_iters = 40;
+  int32_t use_graph = 1;
+  int32_t verbose = 0;
+};
+struct PdhgReport {
+  int32_t status = 0;  // 0 optimal, 1 iteration limit, 2 time limit
+  int32_t iterations = 0, restarts = 0, checks = 0;
+  double primal_obj = 0.0, dual_obj = 0.0;
+  double primal_res = 0.0, dual_res = 0.0, gap = 0.0;  // relative
+  double elapsed_seconds = 0.0, norm_estimate = 0.0, primal_weight = 0.0;
+  int64_t graph_launches = 0;
+};
+// x, z: device arrays of lp.A.cols doubles, y of lp.A.rows; on return they
+// hold the best iterate (unscaled), z the reduced costs projected on the
+// cone of the column bounds.
+PdhgReport pdhg_solve(const presolve::LPInfoGpu& lp, const PdhgOptions& opt, double* x, double* y, double* z);
 void screen_structural_l1(const presolve::LPInfoGpu& lp, const presolve::PresolveParams& prm);
 // old_to_new map of a keep mask; returns the kept count.
 int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map);

@@ -1012,3 +1012,34 @@ void gp_postsolve_plan_free(gp_postsolve_plan* p) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+void gp_pdhg_params_default(gp_pdhg_params* params) {
+  if (params == nullptr) return;
+  const amd::PdhgOptions d;
+  *params = gp_pdhg_params{d.max_iters, d.max_time, d.tol, d.check_every, d.ruiz_iters, d.power_iters, d.use_graph,
+                           d.verbose};
+}
+
+gp_status gp_pdhg_solve(const gp_lp* lp, const gp_pdhg_params* params, double* x, double* y, double* z,
+                        gp_pdhg_result* out) {
+  return guarded([&] {
+    require(lp != nullptr && out != nullptr, "null argument");
+    *out = gp_pdhg_result{};
+    const LPInfoGpu L = to_lp(*lp);
+    require(L.A.cols == 0 || (L.c && L.l && L.u), "null c/l/u");
+    require(L.A.rows == 0 || (L.AL && L.AU), "null AL/AU");
+    require_device();
+    amd::PdhgOptions o;
+    if (params != nullptr)
+      o = amd::PdhgOptions{params->max_iters, params->max_time, params->tol, params->check_every,
+                           params->ruiz_iters, params->power_iters, params->use_graph, params->verbose};
+    const amd::PdhgReport r = amd::pdhg_solve(L, o, x, y, z);
+    amd::ctx().sync("gp_pdhg_solve");
+    *out = gp_pdhg_result{r.status, r.iterations, r.restarts, r.checks, r.primal_obj, r.dual_obj, r.primal_res,
+                          r.dual_res, r.gap, r.elapsed_seconds, r.norm_estimate, r.primal_weight, r.graph_launches};
+  });
+}
+
+}  // extern "C"

@@ -124,6 +124,10 @@ def lib():
         L.gp_mps_upload.argtypes = [vp, C.POINTER(T.GpLp)]
         L.gp_mps_free.argtypes = [vp]
         L.gp_mps_free.restype = None
+        L.gp_pdhg_params_default.argtypes = [C.POINTER(T.GpPdhgParams)]
+        L.gp_pdhg_params_default.restype = None
+        L.gp_pdhg_solve.argtypes = [C.POINTER(T.GpLp), C.POINTER(T.GpPdhgParams), vp, vp, vp,
+                                    C.POINTER(T.GpPdhgResult)]
         _lib = L
     return _lib
 
@@ -475,6 +479,37 @@ class PostsolvePlan:
     def __del__(self):
         if getattr(self, "h", None) and _lib is not None:
             self.free()
+
+
+# ---- solver hand-off ---------------------------------------------------------
+def pdhg_params(**kw):
+    p = T.GpPdhgParams()
+    lib().gp_pdhg_params_default(C.byref(p))
+    names = {f[0] for f in T.GpPdhgParams._fields_}
+    for k, v in kw.items():
+        if k not in names:
+            raise KeyError(f"unknown gp_pdhg_params field {k!r}")
+        setattr(p, k, v)
+    return p
+
+
+def pdhg_solve_view(g, prm=None, out=None):
+    """gp_pdhg_solve on a device gp_lp (a DeviceLp's .g or a presolve
+    handle's reduced_view()); returns (result dict, (x, y, z) DeviceArrays)."""
+    m, n = g.A.rows, g.A.cols
+    if out is None:
+        out = (DeviceArray(size=n), DeviceArray(size=m), DeviceArray(size=n))
+    r = T.GpPdhgResult()
+    prm = prm if prm is not None else pdhg_params()
+    _check(lib().gp_pdhg_solve(C.byref(g), C.byref(prm), out[0].ptr, out[1].ptr, out[2].ptr, C.byref(r)))
+    return {k: getattr(r, k) for k, _ in T.GpPdhgResult._fields_}, out
+
+
+def pdhg_solve(lp, **kw):
+    """Uploads an LP dict, solves it on the device; returns (result, x, y, z)."""
+    dlp = DeviceLp(lp)
+    res, (x, y, z) = pdhg_solve_view(dlp.g, pdhg_params(**kw))
+    return res, x.numpy(), y.numpy(), z.numpy()
 
 
 # ---- timing ----------------------------------------------------------------

@@ -134,6 +134,20 @@ class GpPostsolveShape(C.Structure):
                 ("tape_bytes", C.c_int64)]
 
 
+class GpPdhgParams(C.Structure):
+    _fields_ = [("max_iters", C.c_int32), ("max_time", C.c_double), ("tol", C.c_double),
+                ("check_every", C.c_int32), ("ruiz_iters", C.c_int32), ("power_iters", C.c_int32),
+                ("use_graph", C.c_int32), ("verbose", C.c_int32)]
+
+
+class GpPdhgResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("restarts", C.c_int32),
+                ("checks", C.c_int32), ("primal_obj", C.c_double), ("dual_obj", C.c_double),
+                ("primal_res", C.c_double), ("dual_res", C.c_double), ("gap", C.c_double),
+                ("elapsed_seconds", C.c_double), ("norm_estimate", C.c_double), ("primal_weight", C.c_double),
+                ("graph_launches", C.c_int64)]
+
+
 class GpLpSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("m", C.c_int32), ("n", C.c_int32), ("len_lo", C.c_int32),
                 ("len_hi", C.c_int32), ("mean_len", C.c_double), ("min_len", C.c_int32)]

@@ -583,6 +583,57 @@ hipError_t gpk_screen_structural_prefix(const int32_t* ptr, const int32_t* idx, 
                                         const double* AL, const double* AU, int32_t m, int32_t* out,
                                         hipStream_t stream);
 
+/* ======================================================================
+ * On-device LP solver of the reduced LP (restarted PDHG; no counterpart in
+ * the reference, which hands the reduced LP to an external GPU solver).
+ * A, AT: the scaled matrices; step[0] = tau (primal), step[1] = sigma (dual)
+ * are read from device memory so a captured run of iterations can be
+ * replayed after the host changes them.
+ * ====================================================================== */
+/* `iterations` PDHG steps: x <- proj_[l,u](x - tau (c - A^T y)),
+ * xbar = 2 x_new - x_old, y <- the projected dual step on xbar; x_sum and
+ * y_sum accumulate the new iterates (the averages of a restart period). */
+hipError_t gpk_pdhg_iterations(gpk_csr A, gpk_csr AT, const double* c, const double* l, const double* u,
+                               const double* AL, const double* AU, double* x, double* xbar, double* x_sum, double* y,
+                               double* y_sum, const double* step, int32_t iterations, hipStream_t stream);
+/* out = a x + b y (y may be null: out = a x). */
+hipError_t gpk_pdhg_axpby(double a, const double* x, double b, const double* y, double* out, int32_t n,
+                          hipStream_t stream);
+hipError_t gpk_pdhg_mul(const double* x, const double* d, double* out, int32_t n, hipStream_t stream);
+hipError_t gpk_pdhg_spmv(gpk_csr M, const double* x, double* y, hipStream_t stream);
+/* One Ruiz step: d[i] /= sqrt(max_p |M.val[p]|) over row i (rows with no
+ * nonzero keep d). */
+hipError_t gpk_pdhg_ruiz(gpk_csr M, double* d, hipStream_t stream);
+/* out[p] = M.val[p] * drow[row(p)] * dcol[idx[p]]. */
+hipError_t gpk_pdhg_scale_csr(gpk_csr M, const double* drow, const double* dcol, double* out, hipStream_t stream);
+/* out = v * d (mul != 0) or v / d. */
+hipError_t gpk_pdhg_scale_vec(const double* v, const double* d, double* out, int32_t n, int32_t mul,
+                              hipStream_t stream);
+/* Doubles of the per-block partial-sum buffer the *_terms launchers need for
+ * a vector of n entries. */
+int32_t gpk_pdhg_partials(int32_t n);
+/* KKT sums over the rows of A (out[0..3]: squared row violation of Ax, the
+ * rows' dual-objective terms of y, squared bound magnitudes, squared y on
+ * rows whose bound of y's sign is infinite) and over the columns (out[0..3]:
+ * squared dual residual of c - A^T y against the reduced-cost cone of the
+ * bounds, the columns' dual-objective terms, squared c, c^T x; z, if not
+ * null, gets the projected reduced costs).  Block sums are added in block
+ * order by one workgroup: the results do not vary run to run. */
+hipError_t gpk_pdhg_row_terms(gpk_csr A, const double* AL, const double* AU, const double* x, const double* y,
+                              double* partial, double* out, hipStream_t stream);
+hipError_t gpk_pdhg_col_terms(gpk_csr AT, const double* c, const double* l, const double* u, const double* x,
+                              const double* y, double* z, double* partial, double* out, hipStream_t stream);
+/* out[0] = sum (a - b)^2 (b null: sum a^2), out[1] = sum a^2. */
+hipError_t gpk_pdhg_sq_terms(const double* a, const double* b, int32_t n, double* partial, double* out,
+                             hipStream_t stream);
+/* p[i] = 0.5 + frac(i * golden ratio): the power iteration's start vector. */
+hipError_t gpk_pdhg_seed(double* p, int32_t n, hipStream_t stream);
+/* out = the projection of xs * d on [l, u]. */
+hipError_t gpk_pdhg_unscale_x(const double* xs, const double* d, const double* l, const double* u, double* out,
+                              int32_t n, hipStream_t stream);
+/* x = the projection of 0 on [l, u]. */
+hipError_t gpk_pdhg_project_zero(const double* l, const double* u, double* x, int32_t n, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -463,6 +463,48 @@ gp_status gp_synthetic_point(uint64_t seed, const gp_lp* lp, double* x, double* 
 /* Order-free checksum of the bit patterns of x (n), y (m) and z (n). */
 gp_status gp_checksum(const double* x, int32_t n, const double* y, int32_t m, const double* z, uint64_t* out);
 
+/* ---- solver hand-off (the external GPU solver's place in the reference's
+ * pipeline, README.md:145-169) ------------------------------------------
+ * gp_pdhg_solve solves a device LP (normally gp_presolve_reduced_lp's view,
+ * so the reduced LP never leaves the GPU) with restarted PDHG: Ruiz
+ * scaling, step sizes from a power-iteration estimate of ||A||_2, restarts
+ * by KKT error and primal-weight updates, every vector in device memory, one
+ * wait per check period.  x, z: device arrays of lp->A.cols doubles, y of
+ * lp->A.rows, written with the returned iterate; z holds c - A^T y projected
+ * on the reduced-cost cone of the column bounds (so c - A^T y - z is the dual
+ * residual).  Tolerances are relative: ||row violation|| / (1 + ||b||),
+ * ||dual residual|| / (1 + ||c||), |primal - dual objective| /
+ * (1 + |primal| + |dual|), b the larger finite |AL|, |AU| of each row.
+ * Deterministic: the same LP and parameters give the same bits. */
+typedef struct gp_pdhg_params {
+  int32_t max_iters;   /* default 200000 */
+  double max_time;     /* seconds, default 3600 */
+  double tol;          /* relative KKT tolerance, default 1e-4 */
+  int32_t check_every; /* iterations between KKT checks, default 64 */
+  int32_t ruiz_iters;  /* default 10 */
+  int32_t power_iters; /* default 40 */
+  int32_t use_graph;   /* 1: each check period is one graph launch (default) */
+  int32_t verbose;     /* 1: one line per check on stderr */
+} gp_pdhg_params;
+typedef struct gp_pdhg_result {
+  int32_t status; /* 0 optimal within tol, 1 iteration limit, 2 time limit */
+  int32_t iterations;
+  int32_t restarts;
+  int32_t checks;
+  double primal_obj; /* c^T x + obj_constant */
+  double dual_obj;
+  double primal_res; /* relative, as above */
+  double dual_res;
+  double gap;
+  double elapsed_seconds;
+  double norm_estimate; /* ||A||_2 of the scaled A */
+  double primal_weight;
+  int64_t graph_launches;
+} gp_pdhg_result;
+void gp_pdhg_params_default(gp_pdhg_params* params);
+gp_status gp_pdhg_solve(const gp_lp* lp, const gp_pdhg_params* params, double* x, double* y, double* z,
+                        gp_pdhg_result* out);
+
 #ifdef __cplusplus
 }
 #endif
