@@ -25,6 +25,12 @@ Extra objects on the JSON line:
   roofline_postsolve the prepared postsolve of the last record: tape records,
                      dependency levels of the two replay passes, time per level
                      and the bytes model (tape read per pass, outputs written);
+  end_to_end         the solver hand-off on the same LP: presolve, the reduced
+                     LP solved in HBM by gp_pdhg_solve (restarted PDHG, relative
+                     KKT tolerance --solve-tol), one-shot postsolve of that
+                     solution; then the same solver on the original LP without
+                     presolve, with the per-iteration bytes model
+                     24 nnz + 76 n + 60 m against 8 TB/s;
   cpu_baseline       the C++ oracle presolving the same LP on the host:
                      single thread and the OpenMP build on every usable core,
                      median of five runs after a warm-up, with the core count
@@ -368,6 +374,86 @@ def measure_transpose(gp_ffi, dlp, lp, stream, launches):
             "workload": f"A of the LP ({m}x{n}, {nnz} nnz) -> AT ({n}x{m})"}
 
 
+def pdhg_iteration_bytes(m, n, nnz):
+    """Algorithmic bytes of one PDHG iteration over an m x n LP: the primal
+    step reads AT (row pointers 4 n, 12 B per entry), y once (8 m), c, l, u,
+    x, x_sum (40 n) and writes x, xbar, x_sum (24 n); the dual step reads A
+    (4 m, 12 B per entry), xbar once (8 n), AL, AU, y, y_sum (32 m) and
+    writes y, y_sum (16 m)."""
+    return 24 * nnz + 76 * n + 60 * m
+
+
+def measure_pdhg_iterations(gp_ffi, dlp, lp, stream, launches, per_launch=64):
+    """Times the solver's iteration kernels (k_rows<OpPrimal> over AT, then
+    k_rows<OpDual> over A, per iteration) on the original LP, `per_launch`
+    iterations per gpk_pdhg_iterations call, back to back on one stream with
+    HIP events; the two sparse products alone (gpk_pdhg_spmv over A and AT)
+    are timed apart.  The iterates are not those of a solve (unscaled LP,
+    fixed small steps): the kernels' memory traffic does not depend on them."""
+    import ctypes as C
+
+    import numpy as np
+
+    import gp_types as T
+
+    class GpkRows(C.Structure):
+        _fields_ = [("M", T.GpCsr), ("blk", C.c_void_p), ("blocks", C.c_int32)]
+
+    m, n, nnz = lp["m"], lp["n"], len(lp["A_idx"])
+    L = gp_ffi.lib()
+    vp = C.c_void_p
+    L.gpk_pdhg_row_blocks.restype = C.c_int32
+    L.gpk_pdhg_row_blocks.argtypes = [vp, C.c_int32, vp]
+    L.gpk_pdhg_iterations.argtypes = [GpkRows, GpkRows] + [vp] * 11 + [C.c_int32, vp]
+    L.gpk_pdhg_spmv.argtypes = [GpkRows, vp, vp, vp]
+    keep = []
+
+    def rows(g, host_ptr):
+        ptr = np.ascontiguousarray(host_ptr, np.int32)
+        blk = np.zeros(len(ptr), np.int32)
+        nb = L.gpk_pdhg_row_blocks(ptr.ctypes.data, len(ptr) - 1, blk.ctypes.data)
+        d = gp_ffi.DeviceArray(blk[:nb + 1])
+        keep.append(d)
+        return GpkRows(g, d.ptr, nb)
+
+    A, AT = rows(dlp.A.g, lp["A_ptr"]), rows(dlp.AT.g, lp["AT_ptr"])
+    v = dlp.vec
+    x, xbar, xs = (gp_ffi.DeviceArray(np.zeros(n)) for _ in range(3))
+    y, ys = (gp_ffi.DeviceArray(np.zeros(m)) for _ in range(2))
+    step = gp_ffi.DeviceArray(np.array([1e-3, 1e-3]))
+    ck = gp_ffi.hip_check
+
+    def iterate():
+        ck(L.gpk_pdhg_iterations(A, AT, v["c"].ptr, v["l"].ptr, v["u"].ptr, v["AL"].ptr, v["AU"].ptr, x.ptr,
+                                 xbar.ptr, xs.ptr, y.ptr, ys.ptr, step.ptr, per_launch, stream.ptr), "iterations")
+
+    def products():
+        ck(L.gpk_pdhg_spmv(A, x.ptr, y.ptr, stream.ptr), "spmv A")
+        ck(L.gpk_pdhg_spmv(AT, y.ptr, x.ptr, stream.ptr), "spmv AT")
+
+    iterate()
+    stream.sync()
+    e = [gp_ffi.Event() for _ in range(3)]
+    e[0].record(stream)
+    for _ in range(launches):
+        iterate()
+    e[1].record(stream)
+    for _ in range(launches * per_launch):
+        products()
+    e[2].record(stream)
+    stream.sync()
+    it_us = 1e3 * e[0].elapsed_ms(e[1]) / (launches * per_launch)
+    prod_us = 1e3 * e[1].elapsed_ms(e[2]) / (launches * per_launch)
+    b = pdhg_iteration_bytes(m, n, nnz)
+    log(f"pdhg iteration kernels: {it_us:.1f} us/iteration ({b / (it_us * 1e-6) / 1e9:.0f} GB/s); "
+        f"the two products alone {prod_us:.1f} us; blocks A {A.blocks}, AT {AT.blocks}")
+    return {"bound": "hbm", "kernels": "k_rows<OpPrimal> over AT + k_rows<OpDual> over A",
+            "workload": f"original {m}x{n} LP, {nnz} nnz, {launches} x {per_launch} iterations",
+            "us_per_iteration": it_us, "bytes_per_iteration": b, "achieved": b / (it_us * 1e-6) / 1e9,
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": b / (it_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+            "us_two_products": prod_us, "row_blocks": [A.blocks, AT.blocks]}
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hbm_traffic_activity_compaction_transpose_1Mx1.5M.md")
 
 
@@ -603,6 +689,68 @@ def long_rows(gp_ffi, args):
             "check": "reduced sizes equal the oracle's (record digests are compared in the GPU tests)"}
 
 
+def end_to_end(gp_ffi, dlp, lp, prm, tol):
+    """The solver hand-off: presolve, the reduced LP solved on the device by
+    gp_pdhg_solve without leaving HBM, and the one-shot postsolve of that
+    solution against the original LP; then the same solver on the original
+    LP (no presolve), for the solve-time ratio.  Each stage is timed
+    wall-clock with the device synchronised around it."""
+    import ctypes as C
+
+    import numpy as np
+
+    import gp_types as T
+
+    H = gp_ffi.hip()
+    m, n, nnz = lp["m"], lp["n"], len(lp["A_idx"])
+    sync = lambda: gp_ffi.hip_check(H.hipDeviceSynchronize(), "sync")
+    sync()
+    t = time.perf_counter()
+    h = gp_ffi.Presolved(dlp, prm, keep_reduced=True)
+    sync()
+    pre_ms = 1e3 * (time.perf_counter() - t)
+    try:
+        view = h.reduced_view()
+        m1, n1, nnz1 = view.A.rows, view.A.cols, view.A.nnz
+        t = time.perf_counter()
+        res, (dx, dy, dz) = gp_ffi.pdhg_solve_view(view, gp_ffi.pdhg_params(tol=tol))
+        sync()
+        solve_ms = 1e3 * (time.perf_counter() - t)
+        rec = h.record()
+    finally:
+        h.free()
+    rk = T.RecordKeepAlive(rec)
+    out = (gp_ffi.DeviceArray(size=n), gp_ffi.DeviceArray(size=m), gp_ffi.DeviceArray(size=n))
+    sync()
+    t = time.perf_counter()
+    gp_ffi._check(gp_ffi.lib().gp_postsolve(C.byref(rk.view), dx.ptr, dy.ptr, dz.ptr, C.byref(dlp.g),
+                                            out[0].ptr, out[1].ptr, out[2].ptr))
+    sync()
+    post_ms = 1e3 * (time.perf_counter() - t)
+    x = out[0].numpy()
+    obj = float(np.dot(lp["c"], x)) + float(lp.get("obj", 0.0))
+    log(f"end to end: presolve {pre_ms:.1f} ms, solve {solve_ms:.1f} ms ({res['iterations']} iterations, "
+        f"reduced {m1}x{n1}), postsolve {post_ms:.1f} ms; objective {obj:.10g}")
+    t = time.perf_counter()
+    direct, _ = gp_ffi.pdhg_solve_view(dlp.g, gp_ffi.pdhg_params(tol=tol, max_time=120.0))
+    sync()
+    direct_ms = 1e3 * (time.perf_counter() - t)
+    log(f"no presolve: solve {direct_ms:.0f} ms ({direct['iterations']} iterations, status {direct['status']}); "
+        f"objective {direct['primal_obj']:.10g}")
+    us_it = 1e6 * direct["elapsed_seconds"] / max(1, direct["iterations"])
+    total = pre_ms + solve_ms + post_ms
+    return {"tol": tol, "presolve_ms": pre_ms, "solve_ms": solve_ms, "postsolve_ms": post_ms, "total_ms": total,
+            "reduced": [m1, n1, nnz1], "solve_status": res["status"], "solve_iterations": res["iterations"],
+            "solve_restarts": res["restarts"], "objective": obj,
+            "reduced_kkt": [res["primal_res"], res["dual_res"], res["gap"]],
+            "no_presolve_solve_ms": direct_ms, "no_presolve_status": direct["status"],
+            "no_presolve_iterations": direct["iterations"], "no_presolve_objective": direct["primal_obj"],
+            "no_presolve_kkt": [direct["primal_res"], direct["dual_res"], direct["gap"]],
+            "speedup_total_vs_no_presolve": direct_ms / total,
+            "no_presolve_us_per_iteration_with_checks": us_it,
+            "iteration_roofline": measure_pdhg_iterations(gp_ffi, dlp, lp, gp_ffi.Stream(), 10)}
+
+
 def many_batched(gp_ffi, args):
     """The batched workload on LPs generated on the device: 4,096 LPs,
     m log-uniform in [100, 20,000], n = 1.5 m, row lengths Poisson(6) with
@@ -807,6 +955,8 @@ def main():
                     help="serving workers (0: the count with the highest capacity on the first requests)")
     ap.add_argument("--no-many", action="store_true",
                     help="plain workload only (no batched / serving objects on the line)")
+    ap.add_argument("--no-solve", action="store_true", help="skip the presolve + solve + postsolve object")
+    ap.add_argument("--solve-tol", type=float, default=1e-4, help="relative KKT tolerance of gp_pdhg_solve")
     ap.add_argument("--cpu-leg", action="store_true", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.cpu_leg:
@@ -839,9 +989,10 @@ def main():
 
     if args.kernels_only:
         act, comp, tr = kernel_rooflines(gp_ffi, dlp, lp, args.activity_launches)
+        pdhg = measure_pdhg_iterations(gp_ffi, dlp, lp, gp_ffi.Stream(), 10)
         calibration_copy(gp_ffi)
-        print(json.dumps({"kernels_only": True, "row_activity": act, "compaction": comp, "transpose": tr}),
-              flush=True)
+        print(json.dumps({"kernels_only": True, "row_activity": act, "compaction": comp, "transpose": tr,
+                          "pdhg_iterations": pdhg}), flush=True)
         return
 
     def one_presolve():
@@ -895,23 +1046,9 @@ def main():
     shape = plan.shape()
     plan.free()
     last.free()
+    e2e = None if args.no_solve else end_to_end(gp_ffi, dlp, lp, prm, args.solve_tol)
     log(f"postsolve (inputs resident; record tape uploaded by the call): {post_ms:.1f} ms (runs {post}); "
         f"prepared: {prep_ms:.1f} ms once, then {plan_ms:.1f} ms per run (runs {runs})")
-
-    # Postsolve model: each replay pass (primal, dual) reads the tape once, the
-    # outputs x, z (n) and y (m) are cleared and written and the reduced point
-    # scattered; the passes are chains of dependent levels, so the bound is the
-    # level count times a level's latency, not bandwidth.
-    levels = shape["primal_levels"] + shape["dual_levels"]
-    post_bytes = 2 * shape["tape_bytes"] + 8 * 2 * (2 * n + m) + 8 * (2 * red["n"] + red["m"])
-    post_roof = {"bound": "latency (dependent replay levels)", "records": shape["records"],
-                 "primal_levels": shape["primal_levels"], "dual_levels": shape["dual_levels"],
-                 "tape_bytes": shape["tape_bytes"], "ms_prepared_run": plan_ms,
-                 "us_per_level": 1e3 * plan_ms / max(1, levels), "algorithmic_bytes": post_bytes,
-                 "achieved": post_bytes / (plan_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                 "frac": post_bytes / (plan_ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
-    log(f"postsolve: {shape['records']} records, {shape['primal_levels']} + {shape['dual_levels']} levels, "
-        f"{post_roof['us_per_level']:.1f} us per level")
 
     # Postsolve model: each replay pass (primal, dual) reads the tape once, the
     # outputs x, z (n) and y (m) are cleared and written and the reduced point
@@ -973,6 +1110,7 @@ def main():
                                "us_per_call": tr["us"], "algorithmic_bytes": tr["bytes"],
                                "workload": tr["workload"], "kernels_us": tr["kernels_us"]},
         "roofline_postsolve": post_roof,
+        "end_to_end": e2e,
         "cpu_baseline": cpu,
     }
     if not args.no_many:

@@ -15,6 +15,7 @@
 // the captured iterations read the current step sizes.  The KKT sums of the
 // current and the average iterate are reduced on the device and read back
 // with one wait per period.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -77,14 +78,27 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
   Ctx& cx = ctx();
   const hipStream_t s = cx.stream();
   const char* where = "pdhg";
+  check_lp_arrays(lp, "pdhg input");
+
+  // ---- row blocks of A and AT (from the host copies of the row pointers) ---
+  const auto row_blocks = [&](const presolve::DeviceCsrMatrix& M, DBuf<int32_t>& blk) {
+    const std::vector<int32_t> ptr = cx.to_host(M.rowPtr, static_cast<int64_t>(M.rows) + 1, where);
+    std::vector<int32_t> b(static_cast<size_t>(M.rows) + 1);
+    const int32_t nb = gpk_pdhg_row_blocks(ptr.data(), M.rows, b.data());
+    blk.alloc(static_cast<int64_t>(nb) + 1, where);
+    cx.to_device(blk.get(), b.data(), static_cast<int64_t>(nb) + 1, where);
+    return nb;
+  };
+  DBuf<int32_t> blk_a, blk_at;
+  const int32_t nb_a = row_blocks(lp.A, blk_a), nb_at = row_blocks(lp.AT, blk_at);
 
   // ---- Ruiz equilibration: As = Dr A Dc, ATs = Dc AT Dr ------------------
   DBuf<double> As(nnz, where), ATs(nnz, where), dr(m, where), dc(n, where);
   check(gpk_fill_f64(dr, 1.0, m, s), where);
   check(gpk_fill_f64(dc, 1.0, n, s), where);
   CopyBatch().copy(As.get(), lp.A.nzVal, nnz).copy(ATs.get(), lp.AT.nzVal, nnz).run(where);
-  const gpk_csr A0 = view(lp.A, lp.A.nzVal), AT0 = view(lp.AT, lp.AT.nzVal);
-  const gpk_csr A = view(lp.A, As), AT = view(lp.AT, ATs);
+  const gpk_rows A0{view(lp.A, lp.A.nzVal), blk_a, nb_a}, AT0{view(lp.AT, lp.AT.nzVal), blk_at, nb_at};
+  const gpk_rows A{view(lp.A, As), blk_a, nb_a}, AT{view(lp.AT, ATs), blk_at, nb_at};
   for (int32_t k = 0; k < opt.ruiz_iters && nnz > 0; ++k) {
     check(gpk_pdhg_ruiz(A, dr, s), where);
     check(gpk_pdhg_ruiz(AT, dc, s), where);
@@ -101,13 +115,16 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
   // ---- work vectors --------------------------------------------------------
   DBuf<double> x(n, where), xbar(n, where), xsum(n, where), xr(n, where), xa(n, where), xo(n, where);
   DBuf<double> y(m, where), ysum(m, where), yr(m, where), ya(m, where), yo(m, where);
-  DBuf<double> partial(std::max(gpk_pdhg_partials(m), gpk_pdhg_partials(n)), where);
+  DBuf<double> partial(std::max({gpk_pdhg_partials(nb_a), gpk_pdhg_partials(nb_at),
+                                 gpk_pdhg_partials(static_cast<int32_t>((std::max(m, n) + 255) / 256))}),
+                       where);
   DBuf<double> res(24, where), step(2, where);
   check(gpk_pdhg_project_zero(ls, us, x, n, s), where);
   CopyBatch().zero(y.get(), m).zero(xsum.get(), n).zero(ysum.get(), m).copy(xr.get(), x.get(), n).zero(yr.get(), m)
       .zero(xbar.get(), n).run(where);
 
-  // ---- ||As||_2 by power iteration on As^T As ------------------------------
+  // ---- ||As||_2 by power iteration on As^T As (normalised on the device;
+  // one wait at the end) ------------------------------------------------------
   double norm = 0.0;
   if (nnz > 0) {
     DBuf<double> v(n, where), w(m, where);
@@ -116,11 +133,10 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
       check(gpk_pdhg_spmv(A, v, w, s), where);
       check(gpk_pdhg_spmv(AT, w, v, s), where);
       check(gpk_pdhg_sq_terms(v, nullptr, n, partial, res, s), where);
-      const double len = std::sqrt(cx.to_host(res.get(), 1, where)[0]);
-      if (!(len > 0.0)) break;
-      norm = std::sqrt(len);  // ||As^T As v|| with ||v|| = 1 tends to ||As||^2
-      check(gpk_pdhg_axpby(1.0 / len, v, 0.0, nullptr, v, n, s), where);
+      check(gpk_pdhg_normalize(v, res, n, s), where);
     }
+    // ||As^T As v|| with ||v|| = 1 tends to ||As||^2
+    norm = std::sqrt(std::sqrt(cx.to_host(res.get(), 1, where)[0]));
   }
   if (!(norm > 0.0)) norm = 1.0;
   const double eta = 0.998 / norm;

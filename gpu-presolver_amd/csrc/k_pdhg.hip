@@ -1,9 +1,19 @@
 // Kernels of the on-device LP solver that takes the reduced LP from presolve
 // (restarted primal-dual hybrid gradient, PDHG, on the Ruiz-scaled LP):
 // the two half steps of an iteration, Ruiz scaling, sparse products for the
-// norm estimate, and the residual sums of the KKT check.  Every sum over a
-// vector goes through per-block partial sums reduced in block order by one
-// workgroup, so a solve is deterministic run to run.
+// norm estimate, and the residual sums of the KKT check.
+//
+// Every kernel that walks the rows of a CSR matrix (A, or AT for the
+// columns) runs over row blocks built once per solve: either one row,
+// reduced by the whole workgroup (every row of more than
+// GPK_PDHG_LONG_ROW entries gets such a block), or up to 256 consecutive
+// short rows holding at most GPK_PDHG_ENTRY_BUDGET entries, whose products
+// are staged in LDS by coalesced loads and then summed one row per thread.
+// A column of AT with thousands of entries therefore never runs on a single
+// lane.  Each row's sum has a fixed order (given by the blocks, which depend
+// only on the matrix), and every sum over a vector goes through per-block
+// partial sums reduced in block order by one workgroup, so a solve is
+// deterministic run to run.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -16,98 +26,256 @@ using namespace gpdev;
 
 namespace {
 
+constexpr int kEntryBudget = GPK_PDHG_ENTRY_BUDGET;
+constexpr int kPartials = 4;
+
 __device__ inline double clampd(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
 
-// Sum of v over the block in a fixed order (lanes by shuffle tree, waves in
-// index order); thread 0 returns it.
-__device__ inline double block_sum(double v, double* s_w) {
+struct SumOp {
+  static __device__ double identity() { return 0.0; }
+  static __device__ double combine(double a, double b) { return a + b; }
+};
+struct MaxOp {
+  static __device__ double identity() { return 0.0; }
+  static __device__ double combine(double a, double b) { return fmax(a, b); }
+};
+
+// Combination of v over the block in a fixed order (lanes by shuffle tree,
+// waves in index order); every thread gets the result.
+template <class C>
+__device__ inline double block_combine(double v, double* s_w) {
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1) v += __shfl_down(v, d, 64);
+  for (int d = 32; d >= 1; d >>= 1) v = C::combine(v, __shfl_down(v, d, 64));
   if (lane_id() == 0) s_w[threadIdx.x >> 6] = v;
   __syncthreads();
-  double t = 0.0;
-  if (threadIdx.x == 0)
-    for (int w = 0; w < kBlock / 64; ++w) t += s_w[w];
+  double t = s_w[0];
+  for (int w = 1; w < kBlock / 64; ++w) t = C::combine(t, s_w[w]);
   __syncthreads();
   return t;
 }
 
-// x <- proj_[l,u](x - tau (c - A^T y)); xbar = 2 x_new - x_old; x_sum += x_new.
-// One thread per column (row j of AT).
-__global__ void k_pdhg_primal(const int32_t* __restrict__ at_ptr, const int32_t* __restrict__ at_idx,
-                              const double* __restrict__ at_val, const double* __restrict__ c,
-                              const double* __restrict__ l, const double* __restrict__ u,
-                              const double* __restrict__ y, double* __restrict__ x, double* __restrict__ xbar,
-                              double* __restrict__ x_sum, const double* __restrict__ step, int32_t n) {
-  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  double g = c[j];
-  for (int32_t p = at_ptr[j]; p < at_ptr[j + 1]; ++p) g -= at_val[p] * y[at_idx[p]];
-  const double xo = x[j];
-  const double xn = clampd(xo - step[0] * g, l[j], u[j]);
-  x[j] = xn;
-  xbar[j] = 2.0 * xn - xo;
-  x_sum[j] += xn;
+// ---- row operations --------------------------------------------------------
+// elem(p, col, val): the term entry p contributes; row(r, acc, t): consumes
+// the row's combined terms (and, with kTerms, adds its KKT terms to t).
+
+// y = M v.
+struct OpSpmv {
+  using Comb = SumOp;
+  static constexpr bool kTerms = false;
+  const double* v;
+  double* out;
+  __device__ double elem(int32_t, int32_t col, double val) const { return val * v[col]; }
+  __device__ void row(int32_t r, double acc, double*) const { out[r] = acc; }
+};
+
+// Ruiz: d[r] /= sqrt(max |entry| of row r) (rows without a nonzero keep d).
+struct OpRuiz {
+  using Comb = MaxOp;
+  static constexpr bool kTerms = false;
+  double* d;
+  __device__ double elem(int32_t, int32_t, double val) const { return fabs(val); }
+  __device__ void row(int32_t r, double acc, double*) const {
+    if (acc > 0.0) d[r] /= sqrt(acc);
+  }
+};
+
+// Primal half step over the rows of AT: x <- proj_[l,u](x - tau (c - A^T y)),
+// xbar = 2 x_new - x_old, x_sum += x_new.
+struct OpPrimal {
+  using Comb = SumOp;
+  static constexpr bool kTerms = false;
+  const double* y;
+  const double* c;
+  const double* l;
+  const double* u;
+  const double* step;
+  double* x;
+  double* xbar;
+  double* x_sum;
+  __device__ double elem(int32_t, int32_t col, double val) const { return val * y[col]; }
+  __device__ void row(int32_t j, double aty, double*) const {
+    const double g = c[j] - aty;
+    const double xo = x[j];
+    const double xn = clampd(xo - step[0] * g, l[j], u[j]);
+    x[j] = xn;
+    xbar[j] = 2.0 * xn - xo;
+    x_sum[j] += xn;
+  }
+};
+
+// Dual half step over the rows of A: y <- yt + sigma proj_[AL,AU](-yt / sigma)
+// with yt = y - sigma A xbar (the Moreau form of the projected dual ascent),
+// y_sum += y_new.
+struct OpDual {
+  using Comb = SumOp;
+  static constexpr bool kTerms = false;
+  const double* xbar;
+  const double* AL;
+  const double* AU;
+  const double* step;
+  double* y;
+  double* y_sum;
+  __device__ double elem(int32_t, int32_t col, double val) const { return val * xbar[col]; }
+  __device__ void row(int32_t i, double ax, double*) const {
+    const double sigma = step[1];
+    const double yt = y[i] - sigma * ax;
+    const double yn = yt + sigma * clampd(-yt / sigma, AL[i], AU[i]);
+    y[i] = yn;
+    y_sum[i] += yn;
+  }
+};
+
+// KKT terms of the rows (A, v = x): [0] squared violation of Ax against
+// [AL, AU], [1] the rows' dual-objective terms of y, [2] squared bound
+// magnitude (the larger finite |AL|, |AU|), [3] y^2 where the bound of y's
+// sign is infinite (a dual infeasibility).
+struct OpRowTerms {
+  using Comb = SumOp;
+  static constexpr bool kTerms = true;
+  const double* x;
+  const double* AL;
+  const double* AU;
+  const double* y;
+  __device__ double elem(int32_t, int32_t col, double val) const { return val * x[col]; }
+  __device__ void row(int32_t i, double ax, double* t) const {
+    const double lo = AL[i], hi = AU[i];
+    const double viol = ax - clampd(ax, lo, hi);
+    t[0] += viol * viol;
+    const double yi = y[i];
+    if (yi > 0.0) {
+      if (isfinite(lo)) t[1] += yi * lo;
+      else t[3] += yi * yi;
+    } else if (yi < 0.0) {
+      if (isfinite(hi)) t[1] += yi * hi;
+      else t[3] += yi * yi;
+    }
+    const double b = fmax(isfinite(lo) ? fabs(lo) : 0.0, isfinite(hi) ? fabs(hi) : 0.0);
+    t[2] += b * b;
+  }
+};
+
+// KKT terms of the columns (AT, v = y): g = c - A^T y and lambda its
+// projection on the reduced-cost cone of the column's bounds (lambda >= 0
+// needs a finite lower bound, lambda <= 0 a finite upper one); [0] squared
+// dual residual (g - lambda), [1] the columns' dual-objective terms, [2] c^2,
+// [3] c x; z = lambda when z is given.
+struct OpColTerms {
+  using Comb = SumOp;
+  static constexpr bool kTerms = true;
+  const double* y;
+  const double* c;
+  const double* l;
+  const double* u;
+  const double* x;
+  double* z;
+  __device__ double elem(int32_t, int32_t col, double val) const { return val * y[col]; }
+  __device__ void row(int32_t j, double aty, double* t) const {
+    const double g = c[j] - aty;
+    const bool lo_fin = isfinite(l[j]), hi_fin = isfinite(u[j]);
+    const double lam = clampd(g, hi_fin ? -INFINITY : 0.0, lo_fin ? INFINITY : 0.0);
+    if (z) z[j] = lam;
+    t[0] += (g - lam) * (g - lam);
+    if (lam > 0.0) t[1] += lam * l[j];
+    else if (lam < 0.0) t[1] += lam * u[j];
+    t[2] += c[j] * c[j];
+    t[3] += c[j] * x[j];
+  }
+};
+
+template <class Op>
+__global__ __launch_bounds__(kBlock) void k_rows(gpk_csr M, const int32_t* __restrict__ blk, Op op,
+                                                 double* __restrict__ partial) {
+  using C = typename Op::Comb;
+  __shared__ double s_val[kEntryBudget];
+  __shared__ int32_t s_ptr[kBlock + 1];
+  __shared__ double s_w[kBlock / 64];
+  const int32_t r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+  const int32_t nr = r1 - r0;
+  const int tid = threadIdx.x;
+  double t[kPartials] = {0.0, 0.0, 0.0, 0.0};
+  if (nr == 1) {
+    // one (long) row: strided over the block, then a fixed-order tree
+    double acc = C::identity();
+    for (int32_t p = M.ptr[r0] + tid; p < M.ptr[r0 + 1]; p += kBlock)
+      acc = C::combine(acc, op.elem(p, M.idx[p], M.val[p]));
+    acc = block_combine<C>(acc, s_w);
+    if (tid == 0) op.row(r0, acc, t);
+  } else if (nr > 1) {
+    const int32_t e0 = M.ptr[r0];
+    const int32_t cnt = M.ptr[r1] - e0;
+    if (nr <= kBlock && cnt <= kEntryBudget) {
+      if (tid < nr) s_ptr[tid] = M.ptr[r0 + tid] - e0;
+      if (tid == 0) s_ptr[nr] = cnt;
+      for (int32_t q = tid; q < cnt; q += kBlock) s_val[q] = op.elem(e0 + q, M.idx[e0 + q], M.val[e0 + q]);
+      __syncthreads();
+      if (tid < nr) {
+        double acc = C::identity();
+        for (int32_t q = s_ptr[tid]; q < s_ptr[tid + 1]; ++q) acc = C::combine(acc, s_val[q]);
+        op.row(r0 + tid, acc, t);
+      }
+    } else {
+      // not a block the host builds (more rows or entries than the block
+      // stages): still correct, one row per thread from global memory
+      for (int32_t r = r0 + tid; r < r1; r += kBlock) {
+        double acc = C::identity();
+        for (int32_t p = M.ptr[r]; p < M.ptr[r + 1]; ++p) acc = C::combine(acc, op.elem(p, M.idx[p], M.val[p]));
+        op.row(r, acc, t);
+      }
+    }
+  }
+  if constexpr (Op::kTerms) {
+    for (int k = 0; k < kPartials; ++k) {
+      const double v = block_combine<SumOp>(t[k], s_w);
+      if (tid == 0) partial[static_cast<int64_t>(blockIdx.x) * kPartials + k] = v;
+    }
+  }
 }
 
-// y <- yt + sigma proj_[AL,AU](-yt / sigma) with yt = y - sigma A xbar (the
-// Moreau form of the projected dual ascent); y_sum += y_new.  One thread per row.
-__global__ void k_pdhg_dual(const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
-                            const double* __restrict__ val, const double* __restrict__ AL,
-                            const double* __restrict__ AU, const double* __restrict__ xbar, double* __restrict__ y,
-                            double* __restrict__ y_sum, const double* __restrict__ step, int32_t m) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  double ax = 0.0;
-  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) ax += val[p] * xbar[idx[p]];
-  const double sigma = step[1];
-  const double yt = y[i] - sigma * ax;
-  const double yn = yt + sigma * clampd(-yt / sigma, AL[i], AU[i]);
-  y[i] = yn;
-  y_sum[i] += yn;
+// out[p] = M.val[p] * drow[row(p)] * dcol[idx[p]], over the same row blocks
+// (entries strided over the block; the row of an entry of a multi-row block
+// by binary search in the block's row pointers).
+__global__ __launch_bounds__(kBlock) void k_rows_scale(gpk_csr M, const int32_t* __restrict__ blk,
+                                                       const double* __restrict__ drow,
+                                                       const double* __restrict__ dcol, double* __restrict__ out) {
+  __shared__ int32_t s_ptr[kBlock + 1];
+  const int32_t r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+  const int32_t nr = r1 - r0;
+  const int tid = threadIdx.x;
+  if (nr == 1) {
+    const double dr = drow[r0];
+    for (int32_t p = M.ptr[r0] + tid; p < M.ptr[r0 + 1]; p += kBlock) out[p] = M.val[p] * dr * dcol[M.idx[p]];
+  } else if (nr > 1 && nr <= kBlock) {
+    const int32_t e0 = M.ptr[r0];
+    const int32_t cnt = M.ptr[r1] - e0;
+    if (tid < nr) s_ptr[tid] = M.ptr[r0 + tid] - e0;
+    if (tid == 0) s_ptr[nr] = cnt;
+    __syncthreads();
+    for (int32_t q = tid; q < cnt; q += kBlock) {
+      int32_t lo = 0, hi = nr - 1;  // last k with s_ptr[k] <= q
+      while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (s_ptr[mid] <= q) lo = mid;
+        else hi = mid - 1;
+      }
+      const int32_t p = e0 + q;
+      out[p] = M.val[p] * drow[r0 + lo] * dcol[M.idx[p]];
+    }
+  } else if (nr > kBlock) {
+    for (int32_t r = r0 + tid; r < r1; r += kBlock)
+      for (int32_t p = M.ptr[r]; p < M.ptr[r + 1]; ++p) out[p] = M.val[p] * drow[r] * dcol[M.idx[p]];
+  }
 }
 
 // out = a * x + b * y (any of the inputs may alias the output).
-__global__ void k_pdhg_axpby(double a, const double* __restrict__ x, double b, const double* __restrict__ y,
-                             double* out, int32_t n) {
+__global__ void k_pdhg_axpby(double a, const double* x, double b, const double* y, double* out, int32_t n) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = a * x[i] + (y ? b * y[i] : 0.0);
 }
 
-__global__ void k_pdhg_mul(const double* __restrict__ x, const double* __restrict__ d, double* out, int32_t n) {
+__global__ void k_pdhg_mul(const double* __restrict__ x, const double* __restrict__ d, double* __restrict__ out,
+                           int32_t n) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = x[i] * d[i];
-}
-
-// y = M x for a CSR M.
-__global__ void k_pdhg_spmv(const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
-                            const double* __restrict__ val, const double* __restrict__ x, double* __restrict__ y,
-                            int32_t rows) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows) return;
-  double s = 0.0;
-  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) s += val[p] * x[idx[p]];
-  y[i] = s;
-}
-
-// Ruiz step: the largest |entry| of every row of the scaled matrix; the row
-// scale is divided by its square root.
-__global__ void k_pdhg_ruiz(const int32_t* __restrict__ ptr, const double* __restrict__ val, double* d, int32_t rows) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows) return;
-  double mx = 0.0;
-  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) mx = fmax(mx, fabs(val[p]));
-  if (mx > 0.0) d[i] /= sqrt(mx);
-}
-
-// out[p] = val[p] * drow[i] * dcol[idx[p]] for every entry p of row i.
-__global__ void k_pdhg_scale_csr(const int32_t* __restrict__ ptr, const int32_t* __restrict__ idx,
-                                 const double* __restrict__ val, const double* __restrict__ drow,
-                                 const double* __restrict__ dcol, double* __restrict__ out, int32_t rows) {
-  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= rows) return;
-  const double r = drow[i];
-  for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) out[p] = val[p] * r * dcol[idx[p]];
 }
 
 // Scaled vectors: multiply (mul) or divide (!mul) by d; infinities stay.
@@ -117,82 +285,15 @@ __global__ void k_pdhg_scale_vec(const double* __restrict__ v, const double* __r
   if (i < n) out[i] = mul ? v[i] * d[i] : v[i] / d[i];
 }
 
-// Per-block partial sums, kPartials words per block.
-//   rows (original A, unscaled x, y): [0] sum viol^2 where viol = Ax - proj_[AL,AU](Ax),
-//     [1] sum of the rows' bound terms of the dual objective, [2] sum b^2 (the
-//     larger finite |AL|, |AU| of each row), [3] sum of y^2 on rows whose bound of
-//     y's sign is infinite (a dual infeasibility)
-//   cols (original AT): [0] sum (g - lambda)^2 with g = c - A^T y and lambda its
-//     projection on the reduced-cost cone of the column's bounds, [1] the
-//     columns' bound terms of the dual objective, [2] sum c^2, [3] c^T x; z = lambda.
-constexpr int kPartials = 4;
-
-__global__ __launch_bounds__(kBlock) void k_pdhg_row_terms(const int32_t* __restrict__ ptr,
-                                                           const int32_t* __restrict__ idx,
-                                                           const double* __restrict__ val,
-                                                           const double* __restrict__ AL,
-                                                           const double* __restrict__ AU,
-                                                           const double* __restrict__ x,
-                                                           const double* __restrict__ y, int32_t m,
-                                                           double* __restrict__ partial) {
-  __shared__ double s_w[kBlock / 64];
+// v *= 1 / sqrt(sumsq[0]) when sumsq[0] > 0 (the power iteration's
+// normalisation, read from device memory so the host need not wait).
+__global__ void k_pdhg_normalize(double* __restrict__ v, const double* __restrict__ sumsq, int32_t n) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  double t[kPartials] = {0.0, 0.0, 0.0, 0.0};
-  if (i < m) {
-    double ax = 0.0;
-    for (int32_t p = ptr[i]; p < ptr[i + 1]; ++p) ax += val[p] * x[idx[p]];
-    const double lo = AL[i], hi = AU[i];
-    const double viol = ax - clampd(ax, lo, hi);
-    t[0] = viol * viol;
-    const double yi = y[i];
-    if (yi > 0.0) {
-      if (isfinite(lo)) t[1] = yi * lo;
-      else t[3] = yi * yi;
-    } else if (yi < 0.0) {
-      if (isfinite(hi)) t[1] = yi * hi;
-      else t[3] = yi * yi;
-    }
-    const double b = fmax(isfinite(lo) ? fabs(lo) : 0.0, isfinite(hi) ? fabs(hi) : 0.0);
-    t[2] = b * b;
-  }
-  for (int k = 0; k < kPartials; ++k) {
-    const double v = block_sum(t[k], s_w);
-    if (threadIdx.x == 0) partial[static_cast<int64_t>(blockIdx.x) * kPartials + k] = v;
-  }
+  const double s = sumsq[0];
+  if (i < n && s > 0.0) v[i] *= 1.0 / sqrt(s);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pdhg_col_terms(const int32_t* __restrict__ at_ptr,
-                                                           const int32_t* __restrict__ at_idx,
-                                                           const double* __restrict__ at_val,
-                                                           const double* __restrict__ c,
-                                                           const double* __restrict__ l,
-                                                           const double* __restrict__ u,
-                                                           const double* __restrict__ x,
-                                                           const double* __restrict__ y, int32_t n,
-                                                           double* __restrict__ z, double* __restrict__ partial) {
-  __shared__ double s_w[kBlock / 64];
-  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  double t[kPartials] = {0.0, 0.0, 0.0, 0.0};
-  if (j < n) {
-    double g = c[j];
-    for (int32_t p = at_ptr[j]; p < at_ptr[j + 1]; ++p) g -= at_val[p] * y[at_idx[p]];
-    const bool lo_fin = isfinite(l[j]), hi_fin = isfinite(u[j]);
-    // lambda >= 0 needs a finite lower bound, lambda <= 0 a finite upper one
-    const double lam = clampd(g, hi_fin ? -INFINITY : 0.0, lo_fin ? INFINITY : 0.0);
-    if (z) z[j] = lam;
-    t[0] = (g - lam) * (g - lam);
-    if (lam > 0.0) t[1] = lam * l[j];
-    else if (lam < 0.0) t[1] = lam * u[j];
-    t[2] = c[j] * c[j];
-    t[3] = c[j] * x[j];
-  }
-  for (int k = 0; k < kPartials; ++k) {
-    const double v = block_sum(t[k], s_w);
-    if (threadIdx.x == 0) partial[static_cast<int64_t>(blockIdx.x) * kPartials + k] = v;
-  }
-}
-
-// sum of (a - b)^2 and of a^2 (b may be null) into partials [0], [1].
+// Sum of (a - b)^2 and of a^2 (b may be null) into partials [0], [1].
 __global__ __launch_bounds__(kBlock) void k_pdhg_sq_terms(const double* __restrict__ a,
                                                           const double* __restrict__ b, int32_t n,
                                                           double* __restrict__ partial) {
@@ -204,13 +305,14 @@ __global__ __launch_bounds__(kBlock) void k_pdhg_sq_terms(const double* __restri
     d2 = d * d;
     a2 = a[i] * a[i];
   }
-  double v = block_sum(d2, s_w);
-  if (threadIdx.x == 0) partial[static_cast<int64_t>(blockIdx.x) * kPartials] = v;
-  v = block_sum(a2, s_w);
-  if (threadIdx.x == 0) partial[static_cast<int64_t>(blockIdx.x) * kPartials + 1] = v;
+  d2 = block_combine<SumOp>(d2, s_w);
+  a2 = block_combine<SumOp>(a2, s_w);
   if (threadIdx.x == 0) {
-    partial[static_cast<int64_t>(blockIdx.x) * kPartials + 2] = 0.0;
-    partial[static_cast<int64_t>(blockIdx.x) * kPartials + 3] = 0.0;
+    double* o = partial + static_cast<int64_t>(blockIdx.x) * kPartials;
+    o[0] = d2;
+    o[1] = a2;
+    o[2] = 0.0;
+    o[3] = 0.0;
   }
 }
 
@@ -222,100 +324,10 @@ __global__ __launch_bounds__(kBlock) void k_pdhg_reduce(const double* __restrict
   for (int k = 0; k < kPartials; ++k) {
     double t = 0.0;
     for (int32_t b = threadIdx.x; b < blocks; b += kBlock) t += partial[static_cast<int64_t>(b) * kPartials + k];
-    const double v = block_sum(t, s_w);
+    const double v = block_combine<SumOp>(t, s_w);
     if (threadIdx.x == 0) out[k] = v;
   }
 }
-
-}  // namespace
-
-extern "C" {
-
-hipError_t gpk_pdhg_iterations(gpk_csr A, gpk_csr AT, const double* c, const double* l, const double* u,
-                               const double* AL, const double* AU, double* x, double* xbar, double* x_sum, double* y,
-                               double* y_sum, const double* step, int32_t iterations, hipStream_t s) {
-  const int32_t m = A.rows, n = A.cols;
-  for (int32_t k = 0; k < iterations; ++k) {
-    if (n > 0)
-      hipLaunchKernelGGL(k_pdhg_primal, dim3(grid_for(n)), dim3(kBlock), 0, s, AT.ptr, AT.idx, AT.val, c, l,
-                         u, y, x, xbar, x_sum, step, n);
-    if (m > 0)
-      hipLaunchKernelGGL(k_pdhg_dual, dim3(grid_for(m)), dim3(kBlock), 0, s, A.ptr, A.idx, A.val, AL, AU,
-                         xbar, y, y_sum, step, m);
-  }
-  return hipGetLastError();
-}
-
-hipError_t gpk_pdhg_axpby(double a, const double* x, double b, const double* y, double* out, int32_t n,
-                          hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pdhg_axpby, dim3(grid_for(n)), dim3(kBlock), 0, s, a, x, b, y, out, n);
-  return hipGetLastError();
-}
-
-hipError_t gpk_pdhg_mul(const double* x, const double* d, double* out, int32_t n, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pdhg_mul, dim3(grid_for(n)), dim3(kBlock), 0, s, x, d, out, n);
-  return hipGetLastError();
-}
-
-hipError_t gpk_pdhg_spmv(gpk_csr M, const double* x, double* y, hipStream_t s) {
-  if (M.rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pdhg_spmv, dim3(grid_for(M.rows)), dim3(kBlock), 0, s, M.ptr, M.idx, M.val, x, y,
-                     M.rows);
-  return hipGetLastError();
-}
-
-hipError_t gpk_pdhg_ruiz(gpk_csr M, double* d, hipStream_t s) {
-  if (M.rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pdhg_ruiz, dim3(grid_for(M.rows)), dim3(kBlock), 0, s, M.ptr, M.val, d, M.rows);
-  return hipGetLastError();
-}
-
-hipError_t gpk_pdhg_scale_csr(gpk_csr M, const double* drow, const double* dcol, double* out, hipStream_t s) {
-  if (M.rows <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pdhg_scale_csr, dim3(grid_for(M.rows)), dim3(kBlock), 0, s, M.ptr, M.idx, M.val, drow,
-                     dcol, out, M.rows);
-  return hipGetLastError();
-}
-
-hipError_t gpk_pdhg_scale_vec(const double* v, const double* d, double* out, int32_t n, int32_t mul, hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pdhg_scale_vec, dim3(grid_for(n)), dim3(kBlock), 0, s, v, d, out, n, mul);
-  return hipGetLastError();
-}
-
-int32_t gpk_pdhg_partials(int32_t n) { return n > 0 ? static_cast<int32_t>(grid_for(n)) * kPartials : kPartials; }
-
-hipError_t gpk_pdhg_row_terms(gpk_csr A, const double* AL, const double* AU, const double* x, const double* y,
-                              double* partial, double* out, hipStream_t s) {
-  const unsigned blocks = grid_for(A.rows);
-  hipLaunchKernelGGL(k_pdhg_row_terms, dim3(blocks), dim3(kBlock), 0, s, A.ptr, A.idx, A.val, AL, AU, x, y,
-                     A.rows, partial);
-  hipLaunchKernelGGL(k_pdhg_reduce, dim3(1), dim3(kBlock), 0, s, partial, static_cast<int32_t>(blocks), out);
-  return hipGetLastError();
-}
-
-hipError_t gpk_pdhg_col_terms(gpk_csr AT, const double* c, const double* l, const double* u, const double* x,
-                              const double* y, double* z, double* partial, double* out, hipStream_t s) {
-  const unsigned blocks = grid_for(AT.rows);
-  hipLaunchKernelGGL(k_pdhg_col_terms, dim3(blocks), dim3(kBlock), 0, s, AT.ptr, AT.idx, AT.val, c, l, u, x,
-                     y, AT.rows, z, partial);
-  hipLaunchKernelGGL(k_pdhg_reduce, dim3(1), dim3(kBlock), 0, s, partial, static_cast<int32_t>(blocks), out);
-  return hipGetLastError();
-}
-
-hipError_t gpk_pdhg_sq_terms(const double* a, const double* b, int32_t n, double* partial, double* out,
-                             hipStream_t s) {
-  const unsigned blocks = grid_for(n);
-  hipLaunchKernelGGL(k_pdhg_sq_terms, dim3(blocks), dim3(kBlock), 0, s, a, b, n, partial);
-  hipLaunchKernelGGL(k_pdhg_reduce, dim3(1), dim3(kBlock), 0, s, partial, static_cast<int32_t>(blocks), out);
-  return hipGetLastError();
-}
-
-}  // extern "C"
-
-namespace {
 
 // Start vector of the power iteration: 0.5 + frac(i * golden ratio).
 __global__ void k_pdhg_seed(double* __restrict__ p, int32_t n) {
@@ -342,9 +354,119 @@ __global__ void k_pdhg_project_zero(const double* __restrict__ l, const double* 
   if (j < n) x[j] = clampd(0.0, l[j], u[j]);
 }
 
+template <class Op>
+hipError_t launch_rows(const gpk_rows& R, const Op& op, double* partial, hipStream_t s) {
+  if (R.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows<Op>, dim3(static_cast<unsigned>(R.blocks)), dim3(kBlock), 0, s, R.M, R.blk, op,
+                     partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const double* partial, int32_t blocks, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_pdhg_reduce, dim3(1), dim3(kBlock), 0, s, partial, blocks, out);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 extern "C" {
+
+int32_t gpk_pdhg_row_blocks(const int32_t* ptr, int32_t rows, int32_t* blk) {
+  int32_t nb = 0, r = 0;
+  if (blk) blk[0] = 0;
+  while (r < rows) {
+    if (ptr[r + 1] - ptr[r] > GPK_PDHG_LONG_ROW) {
+      ++r;
+    } else {
+      const int32_t r_start = r;
+      int32_t cnt = 0;
+      while (r < rows && r - r_start < kBlock) {
+        const int32_t len = ptr[r + 1] - ptr[r];
+        if (len > GPK_PDHG_LONG_ROW || cnt + len > kEntryBudget) break;
+        cnt += len;
+        ++r;
+      }
+    }
+    ++nb;
+    if (blk) blk[nb] = r;
+  }
+  return nb;
+}
+
+hipError_t gpk_pdhg_iterations(gpk_rows A, gpk_rows AT, const double* c, const double* l, const double* u,
+                               const double* AL, const double* AU, double* x, double* xbar, double* x_sum, double* y,
+                               double* y_sum, const double* step, int32_t iterations, hipStream_t s) {
+  const OpPrimal primal{y, c, l, u, step, x, xbar, x_sum};
+  const OpDual dual{xbar, AL, AU, step, y, y_sum};
+  for (int32_t k = 0; k < iterations; ++k) {
+    hipError_t e = launch_rows(AT, primal, nullptr, s);
+    if (e != hipSuccess) return e;
+    e = launch_rows(A, dual, nullptr, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t gpk_pdhg_axpby(double a, const double* x, double b, const double* y, double* out, int32_t n,
+                          hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pdhg_axpby, dim3(grid_for(n)), dim3(kBlock), 0, s, a, x, b, y, out, n);
+  return hipGetLastError();
+}
+
+hipError_t gpk_pdhg_mul(const double* x, const double* d, double* out, int32_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pdhg_mul, dim3(grid_for(n)), dim3(kBlock), 0, s, x, d, out, n);
+  return hipGetLastError();
+}
+
+hipError_t gpk_pdhg_spmv(gpk_rows M, const double* x, double* y, hipStream_t s) {
+  return launch_rows(M, OpSpmv{x, y}, nullptr, s);
+}
+
+hipError_t gpk_pdhg_ruiz(gpk_rows M, double* d, hipStream_t s) { return launch_rows(M, OpRuiz{d}, nullptr, s); }
+
+hipError_t gpk_pdhg_scale_csr(gpk_rows M, const double* drow, const double* dcol, double* out, hipStream_t s) {
+  if (M.blocks <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_rows_scale, dim3(static_cast<unsigned>(M.blocks)), dim3(kBlock), 0, s, M.M, M.blk, drow, dcol,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t gpk_pdhg_scale_vec(const double* v, const double* d, double* out, int32_t n, int32_t mul, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pdhg_scale_vec, dim3(grid_for(n)), dim3(kBlock), 0, s, v, d, out, n, mul);
+  return hipGetLastError();
+}
+
+hipError_t gpk_pdhg_normalize(double* v, const double* sumsq, int32_t n, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pdhg_normalize, dim3(grid_for(n)), dim3(kBlock), 0, s, v, sumsq, n);
+  return hipGetLastError();
+}
+
+int32_t gpk_pdhg_partials(int32_t blocks) { return (blocks > 0 ? blocks : 1) * kPartials; }
+
+hipError_t gpk_pdhg_row_terms(gpk_rows A, const double* AL, const double* AU, const double* x, const double* y,
+                              double* partial, double* out, hipStream_t s) {
+  const hipError_t e = launch_rows(A, OpRowTerms{x, AL, AU, y}, partial, s);
+  if (e != hipSuccess) return e;
+  return launch_reduce(partial, A.blocks > 0 ? A.blocks : 0, out, s);
+}
+
+hipError_t gpk_pdhg_col_terms(gpk_rows AT, const double* c, const double* l, const double* u, const double* x,
+                              const double* y, double* z, double* partial, double* out, hipStream_t s) {
+  const hipError_t e = launch_rows(AT, OpColTerms{y, c, l, u, x, z}, partial, s);
+  if (e != hipSuccess) return e;
+  return launch_reduce(partial, AT.blocks > 0 ? AT.blocks : 0, out, s);
+}
+
+hipError_t gpk_pdhg_sq_terms(const double* a, const double* b, int32_t n, double* partial, double* out,
+                             hipStream_t s) {
+  const unsigned blocks = grid_for(n);
+  hipLaunchKernelGGL(k_pdhg_sq_terms, dim3(blocks), dim3(kBlock), 0, s, a, b, n, partial);
+  return launch_reduce(partial, static_cast<int32_t>(blocks), out, s);
+}
 
 hipError_t gpk_pdhg_seed(double* p, int32_t n, hipStream_t s) {
   if (n <= 0) return hipSuccess;

@@ -586,32 +586,50 @@ hipError_t gpk_screen_structural_prefix(const int32_t* ptr, const int32_t* idx, 
 /* ======================================================================
  * On-device LP solver of the reduced LP (restarted PDHG; no counterpart in
  * the reference, which hands the reduced LP to an external GPU solver).
- * A, AT: the scaled matrices; step[0] = tau (primal), step[1] = sigma (dual)
- * are read from device memory so a captured run of iterations can be
- * replayed after the host changes them.
+ * step[0] = tau (primal), step[1] = sigma (dual) are read from device memory
+ * so a captured run of iterations can be replayed after the host changes
+ * them.  Row kernels take a gpk_rows: the CSR matrix (A, or AT for the
+ * columns) and its row blocks, built on the host from the row pointers by
+ * gpk_pdhg_row_blocks: a block is one row (every row of more than
+ * GPK_PDHG_LONG_ROW entries is alone, reduced by the whole workgroup) or up
+ * to 256 consecutive shorter rows with at most GPK_PDHG_ENTRY_BUDGET entries
+ * in all (staged in LDS).
  * ====================================================================== */
+#define GPK_PDHG_LONG_ROW 256
+#define GPK_PDHG_ENTRY_BUDGET 2048
+typedef struct gpk_rows {
+  gpk_csr M;
+  const int32_t* blk; /* device: blocks + 1 row starts, blk[blocks] = M.rows */
+  int32_t blocks;
+} gpk_rows;
+/* Host: the row blocks of a CSR from its host row pointers (rows + 1);
+ * writes blocks + 1 starts to blk (room for rows + 1; NULL only counts) and
+ * returns the block count. */
+int32_t gpk_pdhg_row_blocks(const int32_t* ptr, int32_t rows, int32_t* blk);
 /* `iterations` PDHG steps: x <- proj_[l,u](x - tau (c - A^T y)),
  * xbar = 2 x_new - x_old, y <- the projected dual step on xbar; x_sum and
  * y_sum accumulate the new iterates (the averages of a restart period). */
-hipError_t gpk_pdhg_iterations(gpk_csr A, gpk_csr AT, const double* c, const double* l, const double* u,
+hipError_t gpk_pdhg_iterations(gpk_rows A, gpk_rows AT, const double* c, const double* l, const double* u,
                                const double* AL, const double* AU, double* x, double* xbar, double* x_sum, double* y,
                                double* y_sum, const double* step, int32_t iterations, hipStream_t stream);
 /* out = a x + b y (y may be null: out = a x). */
 hipError_t gpk_pdhg_axpby(double a, const double* x, double b, const double* y, double* out, int32_t n,
                           hipStream_t stream);
 hipError_t gpk_pdhg_mul(const double* x, const double* d, double* out, int32_t n, hipStream_t stream);
-hipError_t gpk_pdhg_spmv(gpk_csr M, const double* x, double* y, hipStream_t stream);
-/* One Ruiz step: d[i] /= sqrt(max_p |M.val[p]|) over row i (rows with no
+hipError_t gpk_pdhg_spmv(gpk_rows M, const double* x, double* y, hipStream_t stream);
+/* One Ruiz step: d[i] /= sqrt(max |M.val[p]| over row i) (rows with no
  * nonzero keep d). */
-hipError_t gpk_pdhg_ruiz(gpk_csr M, double* d, hipStream_t stream);
+hipError_t gpk_pdhg_ruiz(gpk_rows M, double* d, hipStream_t stream);
 /* out[p] = M.val[p] * drow[row(p)] * dcol[idx[p]]. */
-hipError_t gpk_pdhg_scale_csr(gpk_csr M, const double* drow, const double* dcol, double* out, hipStream_t stream);
+hipError_t gpk_pdhg_scale_csr(gpk_rows M, const double* drow, const double* dcol, double* out, hipStream_t stream);
 /* out = v * d (mul != 0) or v / d. */
 hipError_t gpk_pdhg_scale_vec(const double* v, const double* d, double* out, int32_t n, int32_t mul,
                               hipStream_t stream);
-/* Doubles of the per-block partial-sum buffer the *_terms launchers need for
- * a vector of n entries. */
-int32_t gpk_pdhg_partials(int32_t n);
+/* v *= 1 / sqrt(sumsq[0]) when sumsq[0] > 0 (sumsq in device memory). */
+hipError_t gpk_pdhg_normalize(double* v, const double* sumsq, int32_t n, hipStream_t stream);
+/* Doubles of partial sums a *_terms launch over `blocks` workgroups needs
+ * (row blocks for row/col terms, ceil(n / 256) for sq_terms). */
+int32_t gpk_pdhg_partials(int32_t blocks);
 /* KKT sums over the rows of A (out[0..3]: squared row violation of Ax, the
  * rows' dual-objective terms of y, squared bound magnitudes, squared y on
  * rows whose bound of y's sign is infinite) and over the columns (out[0..3]:
@@ -619,9 +637,9 @@ int32_t gpk_pdhg_partials(int32_t n);
  * bounds, the columns' dual-objective terms, squared c, c^T x; z, if not
  * null, gets the projected reduced costs).  Block sums are added in block
  * order by one workgroup: the results do not vary run to run. */
-hipError_t gpk_pdhg_row_terms(gpk_csr A, const double* AL, const double* AU, const double* x, const double* y,
+hipError_t gpk_pdhg_row_terms(gpk_rows A, const double* AL, const double* AU, const double* x, const double* y,
                               double* partial, double* out, hipStream_t stream);
-hipError_t gpk_pdhg_col_terms(gpk_csr AT, const double* c, const double* l, const double* u, const double* x,
+hipError_t gpk_pdhg_col_terms(gpk_rows AT, const double* c, const double* l, const double* u, const double* x,
                               const double* y, double* z, double* partial, double* out, hipStream_t stream);
 /* out[0] = sum (a - b)^2 (b null: sum a^2), out[1] = sum a^2. */
 hipError_t gpk_pdhg_sq_terms(const double* a, const double* b, int32_t n, double* partial, double* out,
