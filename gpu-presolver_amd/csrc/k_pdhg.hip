@@ -27,7 +27,10 @@ using namespace gpdev;
 namespace {
 
 constexpr int kEntryBudget = GPK_PDHG_ENTRY_BUDGET;
+constexpr int kStage = kEntryBudget / kBlock;  // staged entries per thread
+constexpr int kUnroll = 4;                     // chains per thread on a long row
 constexpr int kPartials = 4;
+static_assert(kEntryBudget % kBlock == 0, "entry budget in whole strides");
 
 __device__ inline double clampd(double v, double lo, double hi) { return fmin(fmax(v, lo), hi); }
 
@@ -194,19 +197,62 @@ __global__ __launch_bounds__(kBlock) void k_rows(gpk_csr M, const int32_t* __res
   const int tid = threadIdx.x;
   double t[kPartials] = {0.0, 0.0, 0.0, 0.0};
   if (nr == 1) {
-    // one (long) row: strided over the block, then a fixed-order tree
-    double acc = C::identity();
-    for (int32_t p = M.ptr[r0] + tid; p < M.ptr[r0 + 1]; p += kBlock)
-      acc = C::combine(acc, op.elem(p, M.idx[p], M.val[p]));
-    acc = block_combine<C>(acc, s_w);
-    if (tid == 0) op.row(r0, acc, t);
+    // one (long) row: kUnroll independent strided chains per thread (their
+    // loads in flight together), combined in a fixed order, then the tree
+    const int32_t b = M.ptr[r0], e = M.ptr[r0 + 1];
+    double acc[kUnroll];
+#pragma unroll
+    for (int k = 0; k < kUnroll; ++k) acc[k] = C::identity();
+    int32_t p = b + tid;
+    for (; p + (kUnroll - 1) * kBlock < e; p += kUnroll * kBlock) {
+      int32_t ci[kUnroll];
+      double cv[kUnroll];
+#pragma unroll
+      for (int k = 0; k < kUnroll; ++k) {
+        ci[k] = M.idx[p + k * kBlock];
+        cv[k] = M.val[p + k * kBlock];
+      }
+#pragma unroll
+      for (int k = 0; k < kUnroll; ++k) acc[k] = C::combine(acc[k], op.elem(p + k * kBlock, ci[k], cv[k]));
+    }
+#pragma unroll
+    for (int k = 0; k < kUnroll - 1; ++k) {
+      const int32_t q = p + k * kBlock;
+      if (q < e) acc[k] = C::combine(acc[k], op.elem(q, M.idx[q], M.val[q]));
+    }
+    double a = acc[0];
+#pragma unroll
+    for (int k = 1; k < kUnroll; ++k) a = C::combine(a, acc[k]);
+    a = block_combine<C>(a, s_w);
+    if (tid == 0) op.row(r0, a, t);
   } else if (nr > 1) {
     const int32_t e0 = M.ptr[r0];
     const int32_t cnt = M.ptr[r1] - e0;
     if (nr <= kBlock && cnt <= kEntryBudget) {
       if (tid < nr) s_ptr[tid] = M.ptr[r0 + tid] - e0;
       if (tid == 0) s_ptr[nr] = cnt;
-      for (int32_t q = tid; q < cnt; q += kBlock) s_val[q] = op.elem(e0 + q, M.idx[e0 + q], M.val[e0 + q]);
+      // every load of the thread's (up to kStage) entries issued before the
+      // gathers, every gather before the LDS stores
+      int32_t ci[kStage];
+      double cv[kStage];
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int32_t q = tid + k * kBlock;
+        if (q < cnt) {
+          ci[k] = M.idx[e0 + q];
+          cv[k] = M.val[e0 + q];
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int32_t q = tid + k * kBlock;
+        if (q < cnt) cv[k] = op.elem(e0 + q, ci[k], cv[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < kStage; ++k) {
+        const int32_t q = tid + k * kBlock;
+        if (q < cnt) s_val[q] = cv[k];
+      }
       __syncthreads();
       if (tid < nr) {
         double acc = C::identity();
