@@ -408,6 +408,20 @@ double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n);
 presolve::DeviceCsrMatrix compact_csr(const presolve::DeviceCsrMatrix& A, const int32_t* row_map,
                                       const int32_t* col_map, int32_t m_new, int32_t n_new, int32_t known_nnz = -1,
                                       const gpk_rank_word* col_rank = nullptr);
+// The same compaction in two halves, for a caller that reads the new sizes
+// in its own wait: compact_start launches the map pass and the tile-offset
+// scan while the sizes are still unknown (every kept column goes through
+// col_rank, which must be given) and copies the kept entry count to the
+// device word `kept`; compact_finish then writes the compacted matrix.  The
+// row-pointer array is sized for A.rows + 1.
+struct CompactStart {
+  const presolve::DeviceCsrMatrix* A = nullptr;
+  const int32_t* row_map = nullptr;
+  DBuf<int32_t> ptr, jn, tile_off;
+};
+CompactStart compact_start(const presolve::DeviceCsrMatrix& A, const int32_t* row_map, const gpk_rank_word* col_rank,
+                           int32_t* kept);
+presolve::DeviceCsrMatrix compact_finish(CompactStart& st, int32_t m_new, int32_t n_new, int32_t nnz_new);
 // Stable transpose (rows of the result sorted); fresh device arrays.
 presolve::DeviceCsrMatrix transpose_csr(const presolve::DeviceCsrMatrix& A);
 void free_csr(presolve::DeviceCsrMatrix& A) noexcept;

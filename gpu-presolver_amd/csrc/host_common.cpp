@@ -514,6 +514,49 @@ DeviceCsrMatrix compact_csr(const DeviceCsrMatrix& A, const int32_t* row_map, co
   return out;
 }
 
+CompactStart compact_start(const DeviceCsrMatrix& A, const int32_t* row_map, const gpk_rank_word* col_rank,
+                           int32_t* kept) {
+  Ctx& c = ctx();
+  hipStream_t s = c.stream();
+  CompactStart st;
+  st.A = &A;
+  st.row_map = row_map;
+  st.ptr.alloc(static_cast<int64_t>(A.rows) + 1, "compact rowPtr");
+  if (A.rows == 0 || A.nnz == 0) return st;  // *kept stays as the caller zeroed it
+  if (col_rank == nullptr) throw std::logic_error("compact_start: the column rank words are required");
+  const int32_t tiles = gpk_compact_tiles(A.rows);
+  st.jn.alloc(A.nnz, "compact entry map");
+  st.tile_off.alloc(static_cast<int64_t>(tiles) + 1, "compact tile offsets");
+  check(gpk_compact_map(A.rowPtr, A.colVal, A.rows, row_map, col_rank, st.jn, st.ptr, st.tile_off, s), "compact map");
+  check(gpk_scan_lb_i32(st.tile_off, st.tile_off, tiles, c.lookback(gpk_lookback_tiles(tiles)), s),
+        "compact tile offsets");
+  CopyBatch().copy(kept, st.tile_off.get() + tiles, 1).run("compact kept count");
+  return st;
+}
+
+DeviceCsrMatrix compact_finish(CompactStart& st, int32_t m_new, int32_t n_new, int32_t nnz_new) {
+  Ctx& c = ctx();
+  hipStream_t s = c.stream();
+  const DeviceCsrMatrix& A = *st.A;
+  DeviceCsrMatrix out;
+  out.rows = m_new;
+  out.cols = n_new;
+  if (m_new == 0 || n_new == 0 || A.nnz == 0 || st.jn.get() == nullptr) {
+    CopyBatch().zero(st.ptr.get(), static_cast<int64_t>(m_new) + 1).run("compact empty rowPtr");
+    out.rowPtr = st.ptr.release();
+    return out;
+  }
+  DBuf<int32_t> idx(A.nnz, "compact colVal");
+  DBuf<double> val(A.nnz, "compact nzVal");
+  check(gpk_compact_write(A.rowPtr, A.nzVal, A.rows, st.row_map, st.jn, st.tile_off, st.ptr, m_new, idx, val, s),
+        "compact write");
+  out.nnz = nnz_new;
+  out.rowPtr = st.ptr.release();
+  out.colVal = idx.release();
+  out.nzVal = val.release();
+  return out;
+}
+
 // Stable transpose: LSD radix sort of the column indices (entries already in
 // row-major order, so every AT row comes out sorted by row index) carrying
 // the entry positions; the row pointers come from the sorted keys (no

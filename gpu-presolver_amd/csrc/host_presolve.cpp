@@ -412,9 +412,17 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs, const De
   const bool callers_at = !cur.owns && !p.has_new_A;
   if (callers_at && cur.lp.AT.rows > 0 && cur.lp.AT.nnz > 1)
     check(gpk_rows_unsorted(cur.lp.AT.rowPtr, cur.lp.AT.colVal, cur.lp.AT.rows, counts + 3, s), "AT sorted check");
-  const int32_t* h = c.read_flags(4);
+  // A plan that may drop rows or columns (or rebuilt A) has A's map pass
+  // and tile scan launched before the sizes are read, so the kept entry
+  // count comes back in the same wait as the three counts.
+  const DeviceCsrMatrix& src = p.has_new_A ? p.new_A : cur.lp.A;
+  const bool early = p.has_new_A || p.has_row_action || p.has_col_action;
+  amd::CompactStart a_start;
+  if (early) a_start = amd::compact_start(src, row_map, col_rank.get(), counts + 4);
+  const int32_t* h = c.read_flags(early ? 5 : 4);
   const bool callers_at_sorted = callers_at && h[3] == 0;
   const int32_t m_new = h[0], n_new = h[1];
+  const int32_t a_nnz = early ? h[4] : -1;
   fixed.k = h[2];
   const double obj_new = cur.lp.obj_constant + p.obj_constant_delta;
   if (cur.owns && !p.has_new_A && m_new == m && n_new == n) {
@@ -429,9 +437,9 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs, const De
     cur.lp.obj_constant = obj_new;
     return;
   }
-  const DeviceCsrMatrix& src = p.has_new_A ? p.new_A : cur.lp.A;
   LPInfoGpu next;
-  next.A = amd::compact_csr(src, row_map, col_map, m_new, n_new, -1, col_rank.get());
+  next.A = early ? amd::compact_finish(a_start, m_new, n_new, a_nnz)
+                 : amd::compact_csr(src, row_map, col_map, m_new, n_new, -1, col_rank.get());
   // An AT with sorted rows that is the exact transpose of the A being
   // compacted (the working AT, the caller's AT when sorted, the rebuilding
   // rule's AT of a new A) is compacted with the maps swapped, which gives
