@@ -516,6 +516,32 @@ __global__ void k_dup_row_hash(gpk_csr A, const uint8_t* keep_row, double ctol, 
   hash[i] = h;
 }
 
+// rule_duplicate_rows.cu:116-178, the hash the reference uses when
+// nnz > 32 m (:484-486): one FNV pair hash (col, a / a_first) per entry,
+// XOR-combined (order-free, so 64 lanes give the value 32 do) and mixed into
+// the length seed.  One wave per row.
+__global__ void k_dup_row_hash_wide(gpk_csr A, const uint8_t* keep_row, double ctol, uint64_t* hash) {
+  const int32_t i = static_cast<int32_t>((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = lane_id();
+  if (i >= A.rows) return;
+  if (!keep_row[i]) {
+    if (lane == 0) hash[i] = static_cast<uint64_t>(i);
+    return;
+  }
+  const int32_t b = A.ptr[i], e = A.ptr[i + 1];
+  if (e - b <= 0 || fabs(A.val[b]) <= ctol) {
+    if (lane == 0) hash[i] = fnv_mix(kFnvOffset, static_cast<uint64_t>(i));
+    return;
+  }
+  const double piv = A.val[b];
+  uint64_t agg = 0;
+  for (int32_t q = b + lane; q < e; q += 64)
+    agg ^= fnv_mix(fnv_mix(kFnvOffset, static_cast<uint64_t>(static_cast<int64_t>(A.idx[q]))), dbits(A.val[q] / piv));
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) agg ^= static_cast<uint64_t>(__shfl_xor(static_cast<unsigned long long>(agg), d, 64));
+  if (lane == 0) hash[i] = fnv_mix(fnv_mix(kFnvOffset, static_cast<uint64_t>(e - b)), agg);
+}
+
 __device__ bool dup_row_ratio(const gpk_csr& A, int32_t i, int32_t k, double ctol, double* r_out) {
   const int32_t bi = A.ptr[i], li = A.ptr[i + 1] - bi;
   const int32_t bk = A.ptr[k], lk = A.ptr[k + 1] - bk;
@@ -862,6 +888,12 @@ hipError_t gpk_fixed_col_pack(gpk_csr AT, const uint8_t* selected, const int32_t
 }
 hipError_t gpk_dup_row_hash(gpk_csr A, const uint8_t* keep_row, double coeff_tol, uint64_t* hash,
                             hipStream_t stream) {
+  if (static_cast<int64_t>(A.nnz) > 32LL * static_cast<int64_t>(A.rows)) {
+    if (A.rows > 0)
+      hipLaunchKernelGGL(k_dup_row_hash_wide, dim3(grid_for(static_cast<int64_t>(A.rows) * 64)), dim3(kBlock), 0,
+                         stream, A, keep_row, coeff_tol, hash);
+    return hipGetLastError();
+  }
   GPK_LAUNCH(k_dup_row_hash, A.rows, A, keep_row, coeff_tol, hash);
 }
 hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_hash, const int32_t* sorted_rows,

@@ -406,7 +406,10 @@ hipError_t gpk_fixed_col_pack(gpk_csr AT, const uint8_t* selected, const int32_t
                               const int32_t* nnz_prefix, const double* fixed_val, const double* c,
                               const uint8_t* keep_row, int32_t* out_col, double* out_v, double* out_c,
                               int32_t* out_start, int32_t* out_rows, double* out_coef, hipStream_t stream);
-/* rule_duplicate_rows.cu:78-418 */
+/* rule_duplicate_rows.cu:78-418.  gpk_dup_row_hash computes the hash the
+ * reference picks (:484-497): sequential FNV over (length, column,
+ * a / a_first) per row, or, when A.nnz > 32 A.rows, the XOR of per-entry
+ * pair hashes mixed into the length seed (:116-178), one wave per row. */
 hipError_t gpk_dup_row_hash(gpk_csr A, const uint8_t* keep_row, double coeff_tol, uint64_t* hash,
                             hipStream_t stream);
 hipError_t gpk_dup_row_groups(gpk_csr A, gpk_plan plan, const uint64_t* sorted_hash, const int32_t* sorted_rows,

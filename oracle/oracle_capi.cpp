@@ -76,6 +76,12 @@ void orc_set_batched_closure_min_nnz(int32_t nnz) { orc::set_batched_closure_min
 int32_t orc_batched_closure_min_nnz(void) { return orc::batched_closure_min_nnz(); }
 int64_t orc_batched_closure_runs(void) { return orc::batched_closure_runs(); }
 
+// rules/rule_duplicate_rows.cu:78-178 (the hash the rule sorts rows by).
+int orc_dup_row_hashes(const gp_csr* A, const uint8_t* keep_row, double ctol, uint64_t* out) {
+  orc::duplicate_row_hashes(csr_of(*A), keep_row, ctol, out);
+  return 0;
+}
+
 int orc_row_activity(const gp_csr* A, const double* l, const double* u, double ztol, double* mn, double* mx,
                      int32_t* ni, int32_t* pi) {
   orc::row_activity(csr_of(*A), l, u, ztol, mn, mx, ni, pi);

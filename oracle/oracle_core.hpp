@@ -188,6 +188,10 @@ bool has_good_nnz_progress(int32_t before, int32_t after, double ratio);
 void set_batched_closure_min_nnz(int32_t nnz);
 int32_t batched_closure_min_nnz();
 int64_t batched_closure_runs();
+// The duplicate-row hash of every row (the rule's first step), as the
+// reference computes it for this matrix (rules/rule_duplicate_rows.cu:78-178,
+// 484-497).
+void duplicate_row_hashes(const CsrV& A, const uint8_t* keep_row, double ctol, uint64_t* h);
 
 struct RecordV {
   int32_t m0, n0, m1, n1;

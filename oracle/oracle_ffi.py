@@ -53,6 +53,7 @@ def lib():
         L.orc_batched_closure_min_nnz.restype = C.c_int32
         L.orc_batched_closure_runs.argtypes = []
         L.orc_batched_closure_runs.restype = C.c_int64
+        L.orc_dup_row_hashes.argtypes = [C.POINTER(T.GpCsr), vp, C.c_double, vp]
         _lib = L
     return _lib
 
@@ -85,6 +86,19 @@ def batched_closure_min_nnz():
 def batched_closure_runs():
     """Times the oracle's batched singleton closure has run in this process."""
     return int(lib().orc_batched_closure_runs())
+
+
+def dup_row_hashes(m, n, ptr, idx, val, keep_row, ctol):
+    """The duplicate-row hash of every row, as the rule computes it."""
+    ptr = np.ascontiguousarray(ptr, np.int32)
+    idx = np.ascontiguousarray(idx, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    keep = np.ascontiguousarray(keep_row, np.uint8)
+    out = np.zeros(max(m, 1), np.uint64)
+    g = T.GpCsr(m, n, len(idx), ptr.ctypes.data, idx.ctypes.data if idx.size else None,
+                val.ctypes.data if val.size else None)
+    _check(lib().orc_dup_row_hashes(C.byref(g), keep.ctypes.data, ctol, out.ctypes.data))
+    return out[:m]
 
 
 def _ptr(a):

@@ -663,6 +663,36 @@ void rule_implied_variable_bounds(Plan& p, const LpV& lp, const Stats& s, const 
   p.has_change = true;
 }
 
+// _kernel_duplicate_row_hashes (rules/rule_duplicate_rows.cu:78-114):
+// sequential FNV over (len, col, a / a_first); when nnz > 32 m (:484-486)
+// the reference switches to _kernel_duplicate_row_hashes_warp (:116-178), a
+// different hash: the XOR of one FNV pair hash per entry, mixed into the
+// length seed.
+void duplicate_row_hashes(const CsrV& A, const uint8_t* keep_row, double ctol, uint64_t* h) {
+  const int32_t m = A.rows;
+  const bool wide = static_cast<int64_t>(A.nnz) > 32LL * static_cast<int64_t>(m);
+  for (int32_t i = 0; i < m; ++i) {
+    if (!keep_row[i]) { h[i] = static_cast<uint64_t>(i); continue; }
+    const int32_t b = A.ptr[i], e = A.ptr[i + 1];
+    if (e - b <= 0 || std::fabs(A.val[b]) <= ctol) { h[i] = fnv_mix(FNV_OFFSET, static_cast<uint64_t>(i)); continue; }
+    const double piv = A.val[b];
+    if (wide) {
+      uint64_t agg = 0;
+      for (int32_t q = b; q < e; ++q)
+        agg ^= fnv_mix(fnv_mix(FNV_OFFSET, static_cast<uint64_t>(static_cast<int64_t>(A.idx[q]))),
+                       dbits(A.val[q] / piv));
+      h[i] = fnv_mix(fnv_mix(FNV_OFFSET, static_cast<uint64_t>(e - b)), agg);
+      continue;
+    }
+    uint64_t x = fnv_mix(FNV_OFFSET, static_cast<uint64_t>(e - b));
+    for (int32_t q = b; q < e; ++q) {
+      x = fnv_mix(x, static_cast<uint64_t>(static_cast<int64_t>(A.idx[q])));
+      x = fnv_mix(x, dbits(A.val[q] / piv));
+    }
+    h[i] = x;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // rules/rule_duplicate_rows.cu:78-613
 void rule_duplicate_rows(Plan& p, const LpV& lp, const Stats&, const Params& prm) {
@@ -671,20 +701,8 @@ void rule_duplicate_rows(Plan& p, const LpV& lp, const Stats&, const Params& prm
   if (m <= 1) return;
   const double ctol = prm.zero_tol, itol = prm.feasibility_tol;
   const CsrV& A = lp.A;
-  // _kernel_duplicate_row_hashes (:78-114)
   std::vector<uint64_t> h(m);
-  for (int32_t i = 0; i < m; ++i) {
-    if (!p.keep_row[i]) { h[i] = static_cast<uint64_t>(i); continue; }
-    const int32_t b = A.ptr[i], e = A.ptr[i + 1];
-    if (e - b <= 0 || std::fabs(A.val[b]) <= ctol) { h[i] = fnv_mix(FNV_OFFSET, static_cast<uint64_t>(i)); continue; }
-    const double piv = A.val[b];
-    uint64_t x = fnv_mix(FNV_OFFSET, static_cast<uint64_t>(e - b));
-    for (int32_t q = b; q < e; ++q) {
-      x = fnv_mix(x, static_cast<uint64_t>(static_cast<int64_t>(A.idx[q])));
-      x = fnv_mix(x, dbits(A.val[q] / piv));
-    }
-    h[i] = x;
-  }
+  duplicate_row_hashes(A, p.keep_row, ctol, h.data());
   std::vector<int32_t> order(m);
   std::iota(order.begin(), order.end(), 0);
   std::sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
