@@ -136,6 +136,14 @@ class Ctx {
   void to_device(T* dev, const T* host, int64_t count, const char* context) {
     if (count > 0) upload(dev, host, sizeof(T) * static_cast<size_t>(count), context);
   }
+  // Staged copies (download, to_host_async, upload, to_device) issued
+  // between begin_copy_group() and end_copy_group() go out together, one
+  // k_copy_many launch per GPK_COPY_MAX of them, instead of one launch each.
+  // Nothing else may be launched on the stream inside a group (the copies
+  // are enqueued at its end or when a group fills); a wait (sync,
+  // read_flags, scalar_to_host, mail) also ends the group, its copies first.
+  void begin_copy_group() { grouping_ = true; }
+  void end_copy_group(const char* context);
   // Drops every pending download without copying it (its destination may
   // already be gone), waits for the stream so the staging area is free, and
   // clears the spin-limit error word.  Called when an exception leaves a
@@ -168,6 +176,10 @@ class Ctx {
   char* d_stage_ = nullptr;
   size_t stage_cap_ = 0, stage_used_ = 0;
   std::vector<Download> downloads_;
+  bool grouping_ = false;
+  std::vector<gpk_copy_seg> pending_;  // copies of the open group
+  void enqueue_copy(const gpk_copy_seg& seg, const char* context);
+  void flush_pending(const char* context);
   void check_error_flag(const char* context);
   void finish_downloads();
   // Offset of `bytes` of free staging space (waits for the stream and grows

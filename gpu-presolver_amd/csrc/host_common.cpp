@@ -159,9 +159,29 @@ void Ctx::download(void* dst, const void* dev, size_t bytes, const char* context
     return;
   }
   const size_t off = stage_reserve(bytes, context);
-  const gpk_copy_seg seg{d_stage_ + off, dev, static_cast<int64_t>(bytes)};
-  check(gpk_copy_many(&seg, 1, stream_), context);
+  enqueue_copy(gpk_copy_seg{d_stage_ + off, dev, static_cast<int64_t>(bytes)}, context);
   downloads_.push_back(Download{dst, off, bytes});
+}
+
+void Ctx::enqueue_copy(const gpk_copy_seg& seg, const char* context) {
+  if (!grouping_) {
+    check(gpk_copy_many(&seg, 1, stream_), context);
+    return;
+  }
+  pending_.push_back(seg);
+  if (pending_.size() == GPK_COPY_MAX) flush_pending(context);
+}
+
+void Ctx::flush_pending(const char* context) {
+  if (pending_.empty()) return;
+  const hipError_t e = gpk_copy_many(pending_.data(), static_cast<int32_t>(pending_.size()), stream_);
+  pending_.clear();
+  check(e, context);
+}
+
+void Ctx::end_copy_group(const char* context) {
+  grouping_ = false;
+  flush_pending(context);
 }
 
 void* Ctx::stage(const void* host, size_t bytes, void* dst_after, size_t dst_bytes, const char* context) {
@@ -183,8 +203,7 @@ void Ctx::upload(void* dev, const void* host, size_t bytes, const char* context)
   }
   const size_t off = stage_reserve(bytes, context);
   std::memcpy(h_stage_ + off, host, bytes);
-  const gpk_copy_seg seg{dev, d_stage_ + off, static_cast<int64_t>(bytes)};
-  check(gpk_copy_many(&seg, 1, stream_), context);
+  enqueue_copy(gpk_copy_seg{dev, d_stage_ + off, static_cast<int64_t>(bytes)}, context);
 }
 
 void Ctx::finish_downloads() {
@@ -194,6 +213,8 @@ void Ctx::finish_downloads() {
 }
 
 const uint32_t* Ctx::mail(const void* dev, int words, const char* context) {
+  grouping_ = false;  // a wait closes the group; its copies precede the wait
+  flush_pending(context);
   ++syncs_;
   if (++mail_seq_ == 0) mail_seq_ = 1;
   const uint32_t seq = mail_seq_;
@@ -229,6 +250,8 @@ Ctx& ctx() {
 
 void Ctx::abandon() noexcept {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  grouping_ = false;
+  pending_.clear();
   downloads_.clear();
   stage_used_ = 0;
   if (h_error_) *h_error_ = 0;
@@ -658,6 +681,7 @@ double fixed_col_tape(PostsolveTape* tape, const uint8_t* fixed_mask, const uint
   scan_mask_launch(sel, n, sel_prefix);
   scan_counts_launch(nnz_prefix, n, nnz_prefix);
   int32_t records = 0, active = 0;  // both totals in one wait
+  c.begin_copy_group();
   c.download(&records, sel_prefix.get() + n, sizeof(int32_t), "fixed col records");
   c.download(&active, nnz_prefix.get() + n, sizeof(int32_t), "fixed col entries");
   c.sync("fixed col counts");
@@ -670,6 +694,7 @@ double fixed_col_tape(PostsolveTape* tape, const uint8_t* fixed_mask, const uint
   // Everything the objective and the tape need, in one wait.
   std::vector<double> hv, hc, hcoef;
   std::vector<int32_t> hcol, hrow, starts;
+  c.begin_copy_group();
   c.to_host_async(hv, v.get(), records, "fixed col values");
   c.to_host_async(hc, cc.get(), records, "fixed col costs");
   if (tape) {

@@ -381,8 +381,11 @@ Schedule level_schedule(const TapeScanner& scan, int32_t lo, int32_t hi, int32_t
 DBuf<int32_t> upload_schedule(const Schedule& sch) {
   const int64_t nr = static_cast<int64_t>(sch.order.size()), nl = static_cast<int64_t>(sch.level_start.size());
   DBuf<int32_t> buf(nr + nl);
-  ctx().to_device(buf.get(), sch.order.data(), nr, "replay schedule upload");
-  ctx().to_device(buf.get() + nr, sch.level_start.data(), nl, "replay schedule upload");
+  amd::Ctx& c = ctx();
+  c.begin_copy_group();
+  c.to_device(buf.get(), sch.order.data(), nr, "replay schedule upload");
+  c.to_device(buf.get() + nr, sch.level_start.data(), nl, "replay schedule upload");
+  c.end_copy_group("replay schedule upload");
   return buf;
 }
 
@@ -644,6 +647,8 @@ PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* o
     tick = now;
   };
   d.zret.alloc(d.n0);
+  // The record's maps and tape go up together (copies only in this group).
+  c.begin_copy_group();
   presolve::upload_map(d.col_map, record.col_red2org, d.n0, "col_red2org upload");
   presolve::upload_map(d.row_map, record.row_red2org, d.m0, "row_red2org upload");
   presolve::upload_map(d.fidx, record.fixed_idx, d.n0, "fixed_idx upload");
@@ -651,6 +656,7 @@ PostsolvePlan::PostsolvePlan(const PresolveRecordGpu& record, const LPInfoGpu* o
   d.fval.alloc(d.nf);
   c.to_device(d.fval.get(), record.fixed_val.data(), d.nf, "fixed_val upload");
   d.tape = std::make_unique<presolve::DeviceTape>(record.tape);
+  c.end_copy_group("record upload");
   d.orig = presolve::orig_view(original);
   const presolve::TapeScanner scanner(record.tape, d.n0, d.m0);
   const bool has_orig = original != nullptr;

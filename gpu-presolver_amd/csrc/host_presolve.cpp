@@ -360,8 +360,10 @@ void update_record(RecordState& rs, const PresolvePlanGpu& p, int32_t m, int32_t
     rec.fixed_idx.resize(base + static_cast<size_t>(k));
     rec.fixed_val.resize(base + static_cast<size_t>(k));
     // Land at the context's next wait, like the tape indices below.
+    c.begin_copy_group();
     c.download(rec.fixed_idx.data() + base, oi.get(), sizeof(int32_t) * k, "fixed records idx");
     c.download(rec.fixed_val.data() + base, ov.get(), sizeof(double) * k, "fixed records val");
+    c.end_copy_group("fixed records");
   }
   append_global_tape(rs, p.tape);
   if (m_new < m) {

@@ -157,6 +157,7 @@ void apply_rule_singleton_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
     std::vector<uint8_t> hr;
     std::vector<int32_t> hc;
     std::vector<double> hv, hal, hau;
+    c.begin_copy_group();
     c.to_host_async(hr, remove.get(), m, "singleton tape rows");
     c.to_host_async(hc, stats.singleton_row_col, m, "singleton tape cols");
     c.to_host_async(hv, stats.singleton_row_val, m, "singleton tape vals");
@@ -194,6 +195,7 @@ void apply_rule_infeasible_redundant_rows(PresolvePlanGpu& plan, const LPInfoGpu
   if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
     std::vector<uint8_t> hf, hl, hh;
     std::vector<double> hal, hau;
+    c.begin_copy_group();
     c.to_host_async(hf, full.get(), m, "redundant tape full");
     c.to_host_async(hl, dlo.get(), m, "redundant tape lo");
     c.to_host_async(hh, dhi.get(), m, "redundant tape hi");
@@ -258,6 +260,7 @@ void apply_rule_implied_variable_bounds(PresolvePlanGpu& plan, const LPInfoGpu& 
       check(gpk_implied_pack_tape(n, P, lch, uch, sl, su, fl, fu, cnt, rcol, rrow, rval, s), "implied bounds pack");
       std::vector<int32_t> hc, hr;
       std::vector<double> hv;
+      c.begin_copy_group();
       c.to_host_async(hc, rcol.get(), records, "implied tape cols");
       c.to_host_async(hr, rrow.get(), records, "implied tape rows");
       c.to_host_async(hv, rval.get(), 4 * static_cast<int64_t>(records), "implied tape vals");
@@ -338,6 +341,7 @@ void apply_rule_duplicate_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
     std::vector<int32_t> hrow(k), hto(k), hstep(k);
     std::vector<double> hrec(5 * static_cast<size_t>(k));
     if (k > 0) {
+      c.begin_copy_group();
       c.download(hrow.data(), d_row, sizeof(int32_t) * k, "duplicate rows");
       c.download(hto.data(), d_to, sizeof(int32_t) * k, "duplicate rows");
       c.download(hstep.data(), d_step, sizeof(int32_t) * k, "duplicate rows");
@@ -378,6 +382,7 @@ void apply_rule_infeasible_fixed_variables(PresolvePlanGpu& plan, const LPInfoGp
     std::vector<double> hv, hc, av;
     std::vector<uint8_t> kr;
     std::vector<int32_t> ap, ai;
+    c.begin_copy_group();
     c.to_host_async(hv, v.get(), n, "fixed vars tape v");
     c.to_host_async(hc, plan.new_c, n, "fixed vars tape c");
     c.to_host_async(kr, plan.keep_row_mask, m, "fixed vars tape keep_row");
@@ -432,6 +437,7 @@ void apply_rule_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Pre
   if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
     const std::vector<int32_t> cols = masked_indices(fixed, n);
     std::vector<double> hv, hc;
+    c.begin_copy_group();
     c.to_host_async(hv, v.get(), n, "empty cols tape v");
     c.to_host_async(hc, plan.new_c, n, "empty cols tape c");
     c.sync("empty cols tape");
@@ -513,6 +519,7 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
     amd::scan_mask_launch(sel, m, sel_prefix);
     amd::scan_counts_launch(sup, m, sup);
     int32_t records = 0, support = 0;  // both totals in one wait
+    c.begin_copy_group();
     c.download(&records, sel_prefix.get() + m, sizeof(int32_t), "subcol record count");
     c.download(&support, sup.get() + m, sizeof(int32_t), "subcol support count");
     c.sync("subcol counts");
@@ -525,6 +532,7 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
       // entries [start_r, start_{r+1}) of the packed arrays.
       std::vector<int32_t> hi3, hsc;
       std::vector<double> hv6, hsv;
+      c.begin_copy_group();
       c.to_host_async(hi3, oi3.get(), 3 * static_cast<int64_t>(records), "subcol ids");
       c.to_host_async(hv6, ov6.get(), 6 * static_cast<int64_t>(records), "subcol vals");
       c.to_host_async(hsc, scol.get(), support, "subcol support cols");
@@ -698,6 +706,7 @@ bool dbl_batch_once(PresolvePlanGpu& plan, const LPInfoGpu& lp, const DeviceCsrM
     std::vector<uint8_t> hs;
     std::vector<int32_t> hi;
     std::vector<double> hv;
+    c.begin_copy_group();
     c.to_host_async(hs, subst.get(), k, "doubleton tape mask");
     c.to_host_async(hi, ri.get(), 3 * static_cast<int64_t>(k), "doubleton tape ids");
     c.to_host_async(hv, rv.get(), 12 * static_cast<int64_t>(k), "doubleton tape vals");
@@ -950,6 +959,7 @@ void apply_rule_dual_fix(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Preso
       std::vector<double> hv, hl, hu, hal, hau, av;
       std::vector<uint8_t> kr;
       std::vector<int32_t> tp, ti, ap, ai;
+      c.begin_copy_group();
       c.to_host_async(hv, v.get(), n, "dual fix tape v");
       c.to_host_async(hl, plan.new_l, n, "dual fix tape l");
       c.to_host_async(hu, plan.new_u, n, "dual fix tape u");
@@ -1052,6 +1062,7 @@ void apply_rule_duplicate_columns(PresolvePlanGpu& plan, const LPInfoGpu& lp, co
     const std::vector<int32_t> deleted = masked_indices(del, n);
     std::vector<int32_t> hto;
     std::vector<double> hr;
+    c.begin_copy_group();
     c.to_host_async(hto, merge_to.get(), n, "duplicate cols merge_to");
     c.to_host_async(hr, rec5.get(), 5 * static_cast<int64_t>(n), "duplicate cols values");
     c.sync("duplicate cols tape");
@@ -1100,6 +1111,7 @@ void apply_rule_redundant_bounds(PresolvePlanGpu& plan, const LPInfoGpu& lp, con
     if (prm.record_postsolve_tape) {
       std::vector<uint8_t> hsl, hsu;
       std::vector<double> hl, hu;
+      c.begin_copy_group();
       c.to_host_async(hsl, sl.get(), n, "redundant bounds lo");
       c.to_host_async(hsu, su.get(), n, "redundant bounds hi");
       c.to_host_async(hl, plan.new_l, n, "redundant bounds l");
