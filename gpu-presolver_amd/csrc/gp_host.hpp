@@ -459,6 +459,7 @@ struct PdhgOptions {
   int32_t power_iters = 40;
   int32_t use_graph = 1;
   int32_t verbose = 0;
+  int32_t method = 0;  // 0 restarted averaged PDHG, 1 restarted reflected Halpern PDHG
 };
 struct PdhgReport {
   int32_t status = 0;  // 0 optimal, 1 iteration limit, 2 time limit

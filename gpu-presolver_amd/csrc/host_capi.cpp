@@ -1019,7 +1019,7 @@ void gp_pdhg_params_default(gp_pdhg_params* params) {
   if (params == nullptr) return;
   const amd::PdhgOptions d;
   *params = gp_pdhg_params{d.max_iters, d.max_time, d.tol, d.check_every, d.ruiz_iters, d.power_iters, d.use_graph,
-                           d.verbose};
+                           d.verbose, d.method};
 }
 
 gp_status gp_pdhg_solve(const gp_lp* lp, const gp_pdhg_params* params, double* x, double* y, double* z,
@@ -1033,8 +1033,9 @@ gp_status gp_pdhg_solve(const gp_lp* lp, const gp_pdhg_params* params, double* x
     require_device();
     amd::PdhgOptions o;
     if (params != nullptr)
-      o = amd::PdhgOptions{params->max_iters, params->max_time, params->tol, params->check_every,
-                           params->ruiz_iters, params->power_iters, params->use_graph, params->verbose};
+      o = amd::PdhgOptions{params->max_iters,   params->max_time,    params->tol,
+                           params->check_every, params->ruiz_iters,  params->power_iters,
+                           params->use_graph,   params->verbose,     params->method};
     const amd::PdhgReport r = amd::pdhg_solve(L, o, x, y, z);
     amd::ctx().sync("gp_pdhg_solve");
     *out = gp_pdhg_result{r.status, r.iterations, r.restarts, r.checks, r.primal_obj, r.dual_obj, r.primal_res,

@@ -1,14 +1,19 @@
 // Restarted primal-dual hybrid gradient (PDHG) for the reduced LP
 //   min c^T x + k  s.t.  AL <= A x <= AU,  l <= x <= u,
 // run on the LP that presolve leaves in device memory, so the reduced LP
-// never leaves the GPU between presolve, solve and postsolve.  The method is
+// never leaves the GPU between presolve, solve and postsolve.  Method 0 is
 // the published restarted-average PDHG of Applegate et al. ("Practical
 // large-scale linear programming using primal-dual hybrid gradient",
 // NeurIPS 2021): Ruiz equilibration, step sizes from a power-iteration
 // estimate of ||A||_2, restarts to the better of the current and the
 // average iterate by the KKT error (sufficient 0.2, necessary 0.8,
 // artificial 0.36 of the iterations so far), and the primal weight
-// rebalanced at every restart from the distance travelled.
+// rebalanced at every restart from the distance travelled.  Method 1
+// replaces the averaging by the reflected Halpern iteration of Lu and Yang
+// ("Restarted Halpern PDHG for linear programming", 2024), the
+// Halpern-Peaceman-Rachford form: z <- (k+1)/(k+2) (2 T(z) - z) +
+// 1/(k+2) z_restart with T the PDHG step; the KKT error is taken at T(z),
+// which is also the restart point and the returned point.
 //
 // Each check period of `check_every` iterations is one graph launch (or the
 // same kernels launched directly); tau and sigma live in device memory so
@@ -70,8 +75,10 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   };
   const int32_t m = lp.A.rows, n = lp.A.cols, nnz = lp.A.nnz;
-  if (opt.check_every < 1 || opt.max_iters < 0 || opt.ruiz_iters < 0 || opt.power_iters < 1 || !(opt.tol > 0.0))
-    throw InputError("pdhg: check_every, power_iters >= 1, max_iters, ruiz_iters >= 0 and tol > 0 are required");
+  if (opt.check_every < 1 || opt.max_iters < 0 || opt.ruiz_iters < 0 || opt.power_iters < 1 || !(opt.tol > 0.0) ||
+      (opt.method != 0 && opt.method != 1))
+    throw InputError(
+        "pdhg: check_every, power_iters >= 1, max_iters, ruiz_iters >= 0, tol > 0 and method 0 or 1 are required");
   if (n > 0 && x_out == nullptr) throw InputError("pdhg: null x");
   if (m > 0 && y_out == nullptr) throw InputError("pdhg: null y");
   if (n > 0 && z_out == nullptr) throw InputError("pdhg: null z");
@@ -118,7 +125,7 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
   DBuf<double> partial(std::max({gpk_pdhg_partials(nb_a), gpk_pdhg_partials(nb_at),
                                  gpk_pdhg_partials(static_cast<int32_t>((std::max(m, n) + 255) / 256))}),
                        where);
-  DBuf<double> res(24, where), step(2, where);
+  DBuf<double> res(24, where), step(3, where);
   check(gpk_pdhg_project_zero(ls, us, x, n, s), where);
   CopyBatch().zero(y.get(), m).zero(xsum.get(), n).zero(ysum.get(), m).copy(xr.get(), x.get(), n).zero(yr.get(), m)
       .zero(xbar.get(), n).run(where);
@@ -147,11 +154,14 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
   std::vector<double> h = cx.to_host(res.get(), 8, where);
   const double nc = std::sqrt(h[6]), nb = std::sqrt(h[2]);
   double omega = nc > 1e-10 && nb > 1e-10 ? nc / nb : 1.0;
+  // step = {tau, sigma, iterations since the restart (the Halpern weights)}
+  int32_t since = 0;
   const auto set_steps = [&] {
-    const double st[2] = {eta / omega, eta * omega};
+    const double st[3] = {eta / omega, eta * omega, static_cast<double>(since)};
     cx.upload(step.get(), st, sizeof st, where);
   };
   set_steps();
+  const bool halpern = opt.method == 1;
 
   // ---- KKT of an iterate (scaled xv, yv) into res[off .. off + 8) ---------
   const auto enqueue_kkt = [&](const double* xv, const double* yv, double* z, int off) {
@@ -164,7 +174,8 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
   // ---- a check period: captured once, replayed -----------------------------
   const int32_t K = opt.check_every;
   const auto enqueue_period = [&] {
-    return gpk_pdhg_iterations(A, AT, cs, ls, us, ALs, AUs, x, xbar, xsum, y, ysum, step, K, s);
+    return gpk_pdhg_iterations(A, AT, cs, ls, us, ALs, AUs, x, xbar, xsum, y, ysum, xr, yr, step, halpern ? 1 : 0, K,
+                               s);
   };
   GraphExec graph;
   if (opt.use_graph) {
@@ -187,8 +198,9 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
   h = cx.to_host(res.get(), 8, where);
   Kkt last_restart = kkt_from(h.data(), lp.obj_constant);
   double prev_candidate = std::numeric_limits<double>::infinity();
-  int32_t since = 0;
-  bool use_average = false;
+  // Which buffers hold the candidate point: the iterate (x, y), the
+  // average (xa, ya) or, for Halpern, the PDHG point T(z) in (xsum, ysum).
+  enum class Point { Iterate, Average, Pdhg } point = Point::Iterate;
   rep.status = last_restart.optimal(opt.tol) ? 0 : 1;
 
   while (rep.status != 0 && rep.iterations < opt.max_iters) {
@@ -196,6 +208,7 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
       rep.status = 2;
       break;
     }
+    if (halpern && since > 0) set_steps();  // the period's first Halpern count
     if (graph.exec) {
       check(hipGraphLaunch(graph.exec, s), "pdhg graph launch");
       ++rep.graph_launches;
@@ -205,19 +218,29 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
     rep.iterations += K;
     since += K;
     ++rep.checks;
-    check(gpk_pdhg_axpby(1.0 / since, xsum, 0.0, nullptr, xa, n, s), where);
-    check(gpk_pdhg_axpby(1.0 / since, ysum, 0.0, nullptr, ya, m, s), where);
-    enqueue_kkt(x, y, nullptr, 0);
-    enqueue_kkt(xa, ya, nullptr, 8);
-    h = cx.to_host(res.get(), 16, where);
-    const Kkt cur = kkt_from(h.data(), lp.obj_constant), avg = kkt_from(h.data() + 8, lp.obj_constant);
-    use_average = avg.weighted(omega) < cur.weighted(omega);
-    const Kkt& cand = use_average ? avg : cur;
+    Kkt cand;
+    if (halpern) {
+      enqueue_kkt(xsum, ysum, nullptr, 0);
+      h = cx.to_host(res.get(), 8, where);
+      cand = kkt_from(h.data(), lp.obj_constant);
+      point = Point::Pdhg;
+    } else {
+      check(gpk_pdhg_axpby(1.0 / since, xsum, 0.0, nullptr, xa, n, s), where);
+      check(gpk_pdhg_axpby(1.0 / since, ysum, 0.0, nullptr, ya, m, s), where);
+      enqueue_kkt(x, y, nullptr, 0);
+      enqueue_kkt(xa, ya, nullptr, 8);
+      h = cx.to_host(res.get(), 16, where);
+      const Kkt cur = kkt_from(h.data(), lp.obj_constant), avg = kkt_from(h.data() + 8, lp.obj_constant);
+      const bool opt_cur = cur.optimal(opt.tol), opt_avg = avg.optimal(opt.tol);
+      const bool take_avg = opt_cur || opt_avg ? !opt_cur : avg.weighted(omega) < cur.weighted(omega);
+      point = take_avg ? Point::Average : Point::Iterate;
+      cand = take_avg ? avg : cur;
+    }
     if (opt.verbose)
       std::fprintf(stderr, "pdhg it %d %s pres %.3e dres %.3e gap %.3e obj %.10g omega %.3e\n", rep.iterations,
-                   use_average ? "avg" : "cur", cand.rel_p, cand.rel_d, cand.rel_gap, cand.pobj, omega);
-    if (cur.optimal(opt.tol) || avg.optimal(opt.tol)) {
-      use_average = !cur.optimal(opt.tol);
+                   point == Point::Average ? "avg" : point == Point::Pdhg ? "T(z)" : "cur", cand.rel_p, cand.rel_d,
+                   cand.rel_gap, cand.pobj, omega);
+    if (cand.optimal(opt.tol)) {
       rep.status = 0;
       break;
     }
@@ -227,25 +250,27 @@ PdhgReport pdhg_solve(const LPInfoGpu& lp, const PdhgOptions& opt, double* x_out
       prev_candidate = e;
       continue;
     }
-    if (use_average) CopyBatch().copy(x.get(), xa.get(), n).copy(y.get(), ya.get(), m).run(where);
+    if (point == Point::Average) CopyBatch().copy(x.get(), xa.get(), n).copy(y.get(), ya.get(), m).run(where);
+    if (point == Point::Pdhg) CopyBatch().copy(x.get(), xsum.get(), n).copy(y.get(), ysum.get(), m).run(where);
     check(gpk_pdhg_sq_terms(x, xr, n, partial, res.get() + 16, s), where);
     check(gpk_pdhg_sq_terms(y, yr, m, partial, res.get() + 20, s), where);
     h = cx.to_host(res.get() + 16, 8, where);
     const double dx = std::sqrt(h[0]), dy = std::sqrt(h[4]);
     if (dx > 1e-10 && dy > 1e-10) omega = std::exp(0.5 * std::log(dy / dx) + 0.5 * std::log(omega));
+    since = 0;
     set_steps();
+    // (xr, yr): the restart point, also the Halpern anchor of the next run
     CopyBatch().copy(xr.get(), x.get(), n).copy(yr.get(), y.get(), m).zero(xsum.get(), n).zero(ysum.get(), m)
         .run(where);
     last_restart = cand;
     prev_candidate = std::numeric_limits<double>::infinity();
-    since = 0;
-    use_average = false;  // x, y now hold the restart point
+    point = Point::Iterate;  // x, y now hold the restart point
     ++rep.restarts;
   }
 
   // ---- the chosen iterate, unscaled, with its projected reduced costs -----
-  const double* xs = use_average ? xa.get() : x.get();
-  const double* ys = use_average ? ya.get() : y.get();
+  const double* xs = point == Point::Average ? xa.get() : point == Point::Pdhg ? xsum.get() : x.get();
+  const double* ys = point == Point::Average ? ya.get() : point == Point::Pdhg ? ysum.get() : y.get();
   check(gpk_pdhg_unscale_x(xs, dc, lp.l, lp.u, x_out, n, s), where);
   check(gpk_pdhg_mul(ys, dr, y_out, m, s), where);
   check(gpk_pdhg_row_terms(A0, lp.AL, lp.AU, x_out, y_out, partial, res.get(), s), where);

@@ -82,8 +82,19 @@ struct OpRuiz {
   }
 };
 
-// Primal half step over the rows of AT: x <- proj_[l,u](x - tau (c - A^T y)),
-// xbar = 2 x_new - x_old, x_sum += x_new.
+// Halpern weight of iteration k since the restart: (k + 1) / (k + 2), with
+// k = step[2] (the count at the start of the launch) + j.
+__device__ inline double halpern_weight(const double* step, int32_t j) {
+  const double k = step[2] + static_cast<double>(j);
+  return (k + 1.0) / (k + 2.0);
+}
+
+// Primal half step over the rows of AT: xn = proj_[l,u](x - tau (c - A^T y))
+// and xbar = 2 xn - x (the extrapolated point the dual step reads).
+// Averaged PDHG: x <- xn, x_acc += xn (the running sum).  Reflected Halpern
+// PDHG: x_acc <- xn (the PDHG point T(z)), x <- w xbar + (1 - w) x0, since
+// xbar is also the reflected point 2 T(z) - z.
+template <bool kHalpern>
 struct OpPrimal {
   using Comb = SumOp;
   static constexpr bool kTerms = false;
@@ -94,21 +105,32 @@ struct OpPrimal {
   const double* step;
   double* x;
   double* xbar;
-  double* x_sum;
+  double* x_acc;
+  const double* x0;
+  int32_t j;
   __device__ double elem(int32_t, int32_t col, double val) const { return val * y[col]; }
-  __device__ void row(int32_t j, double aty, double*) const {
-    const double g = c[j] - aty;
-    const double xo = x[j];
-    const double xn = clampd(xo - step[0] * g, l[j], u[j]);
-    x[j] = xn;
-    xbar[j] = 2.0 * xn - xo;
-    x_sum[j] += xn;
+  __device__ void row(int32_t r, double aty, double*) const {
+    const double g = c[r] - aty;
+    const double xo = x[r];
+    const double xn = clampd(xo - step[0] * g, l[r], u[r]);
+    const double xb = 2.0 * xn - xo;
+    xbar[r] = xb;
+    if (kHalpern) {
+      const double w = halpern_weight(step, j);
+      x_acc[r] = xn;
+      x[r] = w * xb + (1.0 - w) * x0[r];
+    } else {
+      x[r] = xn;
+      x_acc[r] += xn;
+    }
   }
 };
 
-// Dual half step over the rows of A: y <- yt + sigma proj_[AL,AU](-yt / sigma)
-// with yt = y - sigma A xbar (the Moreau form of the projected dual ascent),
-// y_sum += y_new.
+// Dual half step over the rows of A: yn = yt + sigma proj_[AL,AU](-yt / sigma)
+// with yt = y - sigma A xbar (the Moreau form of the projected dual ascent).
+// Averaged: y <- yn, y_acc += yn.  Reflected Halpern: y_acc <- yn,
+// y <- w (2 yn - y) + (1 - w) y0.
+template <bool kHalpern>
 struct OpDual {
   using Comb = SumOp;
   static constexpr bool kTerms = false;
@@ -117,14 +139,23 @@ struct OpDual {
   const double* AU;
   const double* step;
   double* y;
-  double* y_sum;
+  double* y_acc;
+  const double* y0;
+  int32_t j;
   __device__ double elem(int32_t, int32_t col, double val) const { return val * xbar[col]; }
   __device__ void row(int32_t i, double ax, double*) const {
     const double sigma = step[1];
-    const double yt = y[i] - sigma * ax;
+    const double yo = y[i];
+    const double yt = yo - sigma * ax;
     const double yn = yt + sigma * clampd(-yt / sigma, AL[i], AU[i]);
-    y[i] = yn;
-    y_sum[i] += yn;
+    if (kHalpern) {
+      const double w = halpern_weight(step, j);
+      y_acc[i] = yn;
+      y[i] = w * (2.0 * yn - yo) + (1.0 - w) * y0[i];
+    } else {
+      y[i] = yn;
+      y_acc[i] += yn;
+    }
   }
 };
 
@@ -440,14 +471,18 @@ int32_t gpk_pdhg_row_blocks(const int32_t* ptr, int32_t rows, int32_t* blk) {
 }
 
 hipError_t gpk_pdhg_iterations(gpk_rows A, gpk_rows AT, const double* c, const double* l, const double* u,
-                               const double* AL, const double* AU, double* x, double* xbar, double* x_sum, double* y,
-                               double* y_sum, const double* step, int32_t iterations, hipStream_t s) {
-  const OpPrimal primal{y, c, l, u, step, x, xbar, x_sum};
-  const OpDual dual{xbar, AL, AU, step, y, y_sum};
+                               const double* AL, const double* AU, double* x, double* xbar, double* x_acc, double* y,
+                               double* y_acc, const double* x0, const double* y0, const double* step,
+                               int32_t halpern, int32_t iterations, hipStream_t s) {
   for (int32_t k = 0; k < iterations; ++k) {
-    hipError_t e = launch_rows(AT, primal, nullptr, s);
-    if (e != hipSuccess) return e;
-    e = launch_rows(A, dual, nullptr, s);
+    hipError_t e;
+    if (halpern) {
+      e = launch_rows(AT, OpPrimal<true>{y, c, l, u, step, x, xbar, x_acc, x0, k}, nullptr, s);
+      if (e == hipSuccess) e = launch_rows(A, OpDual<true>{xbar, AL, AU, step, y, y_acc, y0, k}, nullptr, s);
+    } else {
+      e = launch_rows(AT, OpPrimal<false>{y, c, l, u, step, x, xbar, x_acc, x0, k}, nullptr, s);
+      if (e == hipSuccess) e = launch_rows(A, OpDual<false>{xbar, AL, AU, step, y, y_acc, y0, k}, nullptr, s);
+    }
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -137,7 +137,7 @@ class GpPostsolveShape(C.Structure):
 class GpPdhgParams(C.Structure):
     _fields_ = [("max_iters", C.c_int32), ("max_time", C.c_double), ("tol", C.c_double),
                 ("check_every", C.c_int32), ("ruiz_iters", C.c_int32), ("power_iters", C.c_int32),
-                ("use_graph", C.c_int32), ("verbose", C.c_int32)]
+                ("use_graph", C.c_int32), ("verbose", C.c_int32), ("method", C.c_int32)]
 
 
 class GpPdhgResult(C.Structure):

@@ -609,12 +609,17 @@ typedef struct gpk_rows {
  * writes blocks + 1 starts to blk (room for rows + 1; NULL only counts) and
  * returns the block count. */
 int32_t gpk_pdhg_row_blocks(const int32_t* ptr, int32_t rows, int32_t* blk);
-/* `iterations` PDHG steps: x <- proj_[l,u](x - tau (c - A^T y)),
- * xbar = 2 x_new - x_old, y <- the projected dual step on xbar; x_sum and
- * y_sum accumulate the new iterates (the averages of a restart period). */
+/* `iterations` PDHG steps from z = (x, y): the PDHG point T(z) is
+ * xn = proj_[l,u](x - tau (c - A^T y)), xbar = 2 xn - x, yn the projected
+ * dual step on xbar.  halpern == 0 (averaged PDHG): z <- T(z) and x_acc,
+ * y_acc accumulate the iterates (the averages of a restart period).
+ * halpern != 0 (reflected Halpern PDHG): x_acc, y_acc <- T(z) and
+ * z <- w (2 T(z) - z) + (1 - w) (x0, y0) with w = (k + 1) / (k + 2), k the
+ * iterations since the restart: step[2] at the first of these iterations. */
 hipError_t gpk_pdhg_iterations(gpk_rows A, gpk_rows AT, const double* c, const double* l, const double* u,
-                               const double* AL, const double* AU, double* x, double* xbar, double* x_sum, double* y,
-                               double* y_sum, const double* step, int32_t iterations, hipStream_t stream);
+                               const double* AL, const double* AU, double* x, double* xbar, double* x_acc, double* y,
+                               double* y_acc, const double* x0, const double* y0, const double* step,
+                               int32_t halpern, int32_t iterations, hipStream_t stream);
 /* out = a x + b y (y may be null: out = a x). */
 hipError_t gpk_pdhg_axpby(double a, const double* x, double b, const double* y, double* out, int32_t n,
                           hipStream_t stream);

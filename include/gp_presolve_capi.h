@@ -466,7 +466,8 @@ gp_status gp_checksum(const double* x, int32_t n, const double* y, int32_t m, co
 /* ---- solver hand-off (the external GPU solver's place in the reference's
  * pipeline, README.md:145-169) ------------------------------------------
  * gp_pdhg_solve solves a device LP (normally gp_presolve_reduced_lp's view,
- * so the reduced LP never leaves the GPU) with restarted PDHG: Ruiz
+ * so the reduced LP never leaves the GPU) with restarted PDHG (averaged or
+ * reflected Halpern, by `method`): Ruiz
  * scaling, step sizes from a power-iteration estimate of ||A||_2, restarts
  * by KKT error and primal-weight updates, every vector in device memory, one
  * wait per check period.  x, z: device arrays of lp->A.cols doubles, y of
@@ -485,6 +486,9 @@ typedef struct gp_pdhg_params {
   int32_t power_iters; /* default 40 */
   int32_t use_graph;   /* 1: each check period is one graph launch (default) */
   int32_t verbose;     /* 1: one line per check on stderr */
+  int32_t method;      /* 0: restarted averaged PDHG (default); 1: restarted reflected Halpern PDHG,
+                          z <- (k+1)/(k+2) (2 T(z) - z) + 1/(k+2) z_restart, the iteration of the
+                          Halpern Peaceman-Rachford family */
 } gp_pdhg_params;
 typedef struct gp_pdhg_result {
   int32_t status; /* 0 optimal within tol, 1 iteration limit, 2 time limit */

@@ -80,24 +80,26 @@ CASES = {
 }
 
 
+@pytest.mark.parametrize("method", [0, 1])
 @pytest.mark.parametrize("case", list(CASES))
-def test_matches_highs(case):
+def test_matches_highs(case, method):
     lp = CASES[case]()
     tol = 1e-6
     st, f, *_ = K.solve(lp)
     assert st == 0
-    res, x, y, z = gp_ffi.pdhg_solve(lp, tol=tol)
+    res, x, y, z = gp_ffi.pdhg_solve(lp, tol=tol, method=method)
     assert res["status"] == 0, res
     assert max(res["primal_res"], res["dual_res"], res["gap"]) <= tol
     pobj = _host_checks(lp, res, x, y, z, tol)
     assert abs(pobj - f) <= 10 * tol * (1.0 + abs(f)), (pobj, f)
 
 
-def test_deterministic_and_graph_equals_direct():
+@pytest.mark.parametrize("method", [0, 1])
+def test_deterministic_and_graph_equals_direct(method):
     """Same LP, same parameters: the same bits, whether each check period is
     a replayed graph or its kernels launched one by one."""
     lp = lp_synth.generate(200, 300, seed=11, mean_row_len=4.0)
-    runs = [gp_ffi.pdhg_solve(lp, tol=1e-6, use_graph=g) for g in (1, 1, 0)]
+    runs = [gp_ffi.pdhg_solve(lp, tol=1e-6, use_graph=g, method=method) for g in (1, 1, 0)]
     for res, *vecs in runs[1:]:
         assert res["iterations"] == runs[0][0]["iterations"]
         assert res["restarts"] == runs[0][0]["restarts"]
@@ -116,12 +118,13 @@ def test_iteration_limit():
 def test_invalid_parameters():
     lp = lp_synth.generate(50, 70, seed=8, mean_row_len=4.0)
     dlp = gp_ffi.DeviceLp(lp)
-    for bad in (dict(tol=0.0), dict(check_every=0), dict(power_iters=0), dict(max_iters=-1)):
+    for bad in (dict(tol=0.0), dict(check_every=0), dict(power_iters=0), dict(max_iters=-1), dict(method=2)):
         with pytest.raises(gp_ffi.GpError, match="status 1"):
             gp_ffi.pdhg_solve_view(dlp.g, gp_ffi.pdhg_params(**bad))
 
 
-def test_presolve_solve_postsolve():
+@pytest.mark.parametrize("method", [0, 1])
+def test_presolve_solve_postsolve(method):
     """The hand-off: presolve on the device, the reduced LP solved in place
     (its device view goes straight into gp_pdhg_solve), the reduced solution
     postsolved on the device against the original LP.  The recovered x is
@@ -133,7 +136,7 @@ def test_presolve_solve_postsolve():
     try:
         view = h.reduced_view()
         m1, n1 = view.A.rows, view.A.cols
-        res, (dx, dy, dz) = gp_ffi.pdhg_solve_view(view, gp_ffi.pdhg_params(tol=tol))
+        res, (dx, dy, dz) = gp_ffi.pdhg_solve_view(view, gp_ffi.pdhg_params(tol=tol, method=method))
         assert res["status"] == 0, res
         rec = h.record()
         red = h.reduced()
@@ -147,3 +150,59 @@ def test_presolve_solve_postsolve():
     assert st == 0
     assert abs(K.objective(lp, xo) - f) <= 10 * tol * (1.0 + abs(f))
     assert K.primal_violation(lp, xo) <= 1e-4
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_free_columns_and_equalities(method):
+    """Free columns, equality rows and a one-sided row: min x0 + 2 x1 - x2
+    s.t. x0 + x1 = 3, x1 - x2 = 1, x0 + x2 >= 1, x0, x2 free, 0 <= x1 <= 5.
+    The optimum (from the equalities, x0 = 3 - x1, x2 = x1 - 1, so the
+    objective is 3 - x1 + 2 x1 - x1 + 1 = 4 on the whole feasible segment
+    x0 + x2 = 2 >= 1) is 4; HiGHS agrees."""
+    lp = _lp_from_dense([[1, 1, 0], [0, 1, -1], [1, 0, 1]], [3, 1, 1], [3, 1, np.inf], [1, 2, -1],
+                        [-np.inf, 0, -np.inf], [np.inf, 5, np.inf])
+    st, f, *_ = K.solve(lp)
+    assert st == 0 and abs(f - 4.0) <= 1e-9
+    res, x, y, z = gp_ffi.pdhg_solve(lp, tol=1e-9, method=method)
+    assert res["status"] == 0
+    assert abs(res["primal_obj"] - 4.0) <= 1e-6
+    _host_checks(lp, res, x, y, z, 1e-9)
+    assert z[0] == 0.0 and z[2] == 0.0  # free columns have no reduced cost
+
+
+def test_infeasible_lp_stops_at_the_limit():
+    """x0 + x1 <= 1 and x0 + x1 >= 3 with x >= 0 has no solution: the solver
+    must stop at its iteration limit (status 1) with a primal residual far
+    from zero, not report an optimum."""
+    lp = _lp_from_dense([[1, 1], [1, 1]], [-np.inf, 3], [1, np.inf], [1, 1], [0, 0], [np.inf, np.inf])
+    res, x, y, z = gp_ffi.pdhg_solve(lp, tol=1e-6, max_iters=4096)
+    assert res["status"] == 1 and res["iterations"] == 4096
+    assert res["primal_res"] > 1e-3
+
+
+@pytest.mark.parametrize("method", [0, 1])
+def test_hand_off_on_the_block_structured_lp(method):
+    """The hand-off on a block-structured LP (100 blocks of 40 x 60 and 10
+    linking rows): GPU presolve, the reduced view solved in place, GPU
+    postsolve; the recovered objective against HiGHS on the original LP.
+    This reduced LP is hard for restarted PDHG (about 47k iterations to a
+    relative KKT error of 1e-4; 1e-5 takes far longer), so the check runs at
+    1e-4: objective within 10 tol (relative), row and bound violations within
+    10 tol (1 + ||b||) after postsolve."""
+    lp = lp_synth.config2(seed=2, blocks=100, link_rows=10, link_nnz=1500)
+    tol = 1e-4
+    dlp = gp_ffi.DeviceLp(lp)
+    h = gp_ffi.Presolved(dlp, gp_ffi.params(enable_structure_specific_rules=0), True)
+    try:
+        res, (dx, dy, dz) = gp_ffi.pdhg_solve_view(h.reduced_view(), gp_ffi.pdhg_params(tol=tol, method=method))
+        rec = h.record()
+    finally:
+        h.free()
+    assert res["status"] == 0, res
+    xo, yo, zo = gp_ffi.postsolve(rec, dx.numpy(), dy.numpy(), dz.numpy(), original_dev=dlp)
+    st, f, *_ = K.solve(lp)
+    assert st == 0
+    assert abs(K.objective(lp, xo) - f) <= 10 * tol * (1.0 + abs(f))
+    b = np.maximum(np.where(np.isfinite(lp["AL"]), np.abs(lp["AL"]), 0.0),
+                   np.where(np.isfinite(lp["AU"]), np.abs(lp["AU"]), 0.0))
+    assert K.primal_violation(lp, xo) <= 10 * tol * (1.0 + np.linalg.norm(b))
