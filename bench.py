@@ -645,6 +645,32 @@ def bench_small(args):
                                  "nnz": int(len(lp["A_idx"]))}}), flush=True)
 
 
+def bench_requests(args):
+    """Served requests processed one at a time: the first --requests requests
+    of the serving workload's trace (m log-uniform in [1,000, 500,000],
+    n = 1.5 m, 4-10 entries per general row), each generated on the device,
+    presolved, given its seeded reduced point and postsolved through
+    gp_plain_job_spec, as the serial leg of the serving object does.  LP/s of
+    the serial processing; under a kernel trace, the kernels and launches one
+    request costs."""
+    import gp_ffi
+
+    sizes = _sizes(4, args.requests, 1_000, 500_000)
+    specs = [gp_ffi.lp_spec(0xC000 + k, int(m), row_len=(4, 10)) for k, m in enumerate(sizes)]
+    prm = gp_ffi.params(enable_structure_specific_rules=0)
+    for k in range(min(args.warmup, len(specs))):
+        gp_ffi.plain_job_spec(specs[k], prm, 99, k, True)
+    busy = 0.0
+    for k, spec in enumerate(specs):
+        p = gp_ffi.plain_job_spec(spec, prm, 99, k, True)
+        busy += p["t_end"] - p["t_start"]
+    print(json.dumps({"metric": "served requests, one at a time", "value": len(specs) / busy, "unit": "LP/s",
+                      "n_gpus": 1, "steps": len(specs), "warmup": args.warmup, "higher_is_better": True,
+                      "dtype": "f64", "data": "synthetic (seeded, generated on the device)",
+                      "config": {"workload": "serial served requests", "requests": len(specs),
+                                 "mean_m": float(sizes.mean())}}), flush=True)
+
+
 def _sizes(seed, count, lo, hi):
     import numpy as np
 
@@ -939,10 +965,12 @@ def main():
     ap.add_argument("--kernels-only", action="store_true",
                     help="only the row-activity and compaction kernel timings plus a 256 MiB device copy "
                          "(for hardware-counter runs)")
-    ap.add_argument("--workload", choices=("plain", "batched", "serving", "small"), default="plain",
+    ap.add_argument("--workload", choices=("plain", "batched", "serving", "small", "requests"), default="plain",
                     help="plain: the single large LP (default); batched: 4,096 small LPs at once against "
                          "one-at-a-time plain presolve; serving: Poisson presolve+postsolve requests; "
-                         "small: latency of one small LP (--small-rows)")
+                         "small: latency of one small LP (--small-rows); requests: the serving workload's "
+                         "first --requests requests one at a time")
+    ap.add_argument("--requests", type=int, default=200)
     ap.add_argument("--small-rows", type=int, default=1400)
     ap.add_argument("--batch-lps", type=int, default=4096)
     ap.add_argument("--workers", type=str, default="1,2,4,8,12,16,20,24",
@@ -971,6 +999,8 @@ def main():
         return bench_serving(args)
     if args.workload == "small":
         return bench_small(args)
+    if args.workload == "requests":
+        return bench_requests(args)
 
     import numpy as np
 
