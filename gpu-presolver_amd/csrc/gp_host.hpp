@@ -394,6 +394,14 @@ int32_t batched_closure_min_nnz();
 // std::runtime_error right after it has queued its host copies (0 disarms).
 void fail_after_record_updates(int32_t count);
 void record_update_checkpoint();
+// Testing hook: on the calling thread, every phase that takes over the last
+// phase's plan first compares it (and its statistics) with a freshly built
+// pair and throws std::logic_error on any difference.
+void check_plan_reuse(bool on);
+bool plan_reuse_checked();
+// Phases of the calling thread that took over their predecessor's plan.
+void count_plan_reuse();
+int64_t plan_reuses();
 
 // ---- shared host helpers (host_common.cpp) ---------------------------------
 // Row activity (gpu_presolve_kernels.cu:18-111) on `stream`: the bounds are

@@ -797,6 +797,8 @@ gp_status gp_checksum(const double* x, int32_t n, const double* y, int32_t m, co
 void gp_set_batched_closure_min_nnz(int32_t nnz) { amd::set_batched_closure_min_nnz(nnz); }
 int32_t gp_batched_closure_min_nnz(void) { return amd::batched_closure_min_nnz(); }
 void gp_test_fail_after_record_updates(int32_t count) { amd::fail_after_record_updates(count); }
+void gp_test_check_plan_reuse(int32_t on) { amd::check_plan_reuse(on != 0); }
+int64_t gp_test_plan_reuses(void) { return amd::plan_reuses(); }
 
 gp_status gp_mps_read(const char* path, gp_mps_model** out) {
   return guarded([&] {

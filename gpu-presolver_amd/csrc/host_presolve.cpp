@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <optional>
+#include <type_traits>
 #include <string>
 #include <utility>
 #include <vector>
@@ -43,8 +44,14 @@ namespace {
 // kMinNnzForBatchedClosure (gpu_presolve.cu:2506).
 std::atomic<int32_t> g_closure_min_nnz{10000000};
 thread_local int32_t t_fail_after_updates = 0;
+thread_local bool t_check_plan_reuse = false;
+thread_local int64_t t_plan_reuses = 0;
 }  // namespace
+void count_plan_reuse() { ++t_plan_reuses; }
+int64_t plan_reuses() { return t_plan_reuses; }
 void set_batched_closure_min_nnz(int32_t nnz) { g_closure_min_nnz.store(nnz); }
+void check_plan_reuse(bool on) { t_check_plan_reuse = on; }
+bool plan_reuse_checked() { return t_check_plan_reuse; }
 void fail_after_record_updates(int32_t count) { t_fail_after_updates = count > 0 ? count : 0; }
 // Throws on the armed record update (after its host copies are queued).
 void record_update_checkpoint() {
@@ -189,6 +196,54 @@ struct PhaseStats {
   PhaseStats(const PhaseStats&) = delete;
   PhaseStats& operator=(const PhaseStats&) = delete;
 };
+
+// A phase that changes nothing leaves its plan as a fresh plan of the same
+// LP (every flag clear, no tape, no objective delta): the next phase takes
+// it and its statistics over instead of building them again.
+bool plan_pristine(const PresolvePlanGpu& p) {
+  return !p.has_change && !p.has_row_action && !p.has_col_action && !p.has_new_A && !p.has_infeasible &&
+         !p.has_unbounded && p.obj_constant_delta == 0.0 && p.tape.types.empty() &&
+         !p.has_structural_primal_recovery && !p.has_antipodal_component_recovery &&
+         !p.has_covering_cost_dominance_reduction && !p.has_projected_auxiliary_reduction;
+}
+
+// Testing hook (gp_test_check_plan_reuse): a kept plan and its statistics
+// against a freshly built pair, compared bit for bit where the rules read
+// them (the singleton entries only where their masks are set).
+void verify_kept_plan(const PresolvePlanGpu& kp, const PresolveStatsGpu& ks, const LPInfoGpu& lp) {
+  PhasePlan fp(lp, false);
+  PhaseStats fs(lp, &fp.p);
+  auto& c = ctx();
+  const int64_t m = lp.A.rows, n = lp.A.cols;
+  auto same = [&](const auto* a, const auto* b, int64_t count, const char* what, const uint8_t* only = nullptr) {
+    using T = std::remove_const_t<std::remove_pointer_t<decltype(a)>>;
+    const std::vector<T> ha = c.to_host(a, count, what), hb = c.to_host(b, count, what);
+    std::vector<uint8_t> mask;
+    if (only) mask = c.to_host(only, count, what);
+    for (int64_t i = 0; i < count; ++i) {
+      if (only && !mask[static_cast<size_t>(i)]) continue;
+      if (std::memcmp(&ha[static_cast<size_t>(i)], &hb[static_cast<size_t>(i)], sizeof(T)) != 0)
+        throw std::logic_error(std::string("presolve: a kept phase plan differs from a fresh one in ") + what +
+                               " at " + std::to_string(i));
+    }
+  };
+  same(kp.new_c, fp.p.new_c, n, "c");
+  same(kp.new_l, fp.p.new_l, n, "l");
+  same(kp.new_u, fp.p.new_u, n, "u");
+  same(kp.new_AL, fp.p.new_AL, m, "AL");
+  same(kp.new_AU, fp.p.new_AU, m, "AU");
+  same(kp.keep_row_mask, fp.p.keep_row_mask, m, "keep_row");
+  same(kp.keep_col_mask, fp.p.keep_col_mask, n, "keep_col");
+  same(ks.row_nnz, fs.s.row_nnz, m, "row_nnz");
+  same(ks.singleton_row_mask, fs.s.singleton_row_mask, m, "singleton_row_mask");
+  same(ks.singleton_row_col, fs.s.singleton_row_col, m, "singleton_row_col", fs.s.singleton_row_mask);
+  same(ks.singleton_row_val, fs.s.singleton_row_val, m, "singleton_row_val", fs.s.singleton_row_mask);
+  same(ks.col_nnz, fs.s.col_nnz, n, "col_nnz");
+  same(ks.empty_col_mask, fs.s.empty_col_mask, n, "empty_col_mask");
+  same(ks.column_singleton_mask, fs.s.column_singleton_mask, n, "column_singleton_mask");
+  same(ks.column_singleton_row, fs.s.column_singleton_row, n, "column_singleton_row", fs.s.column_singleton_mask);
+  same(ks.column_singleton_val, fs.s.column_singleton_val, n, "column_singleton_val", fs.s.column_singleton_mask);
+}
 
 // ---- record state ----------------------------------------------------------
 struct RecordState {
@@ -606,9 +661,29 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
   const auto t0 = std::chrono::steady_clock::now();
   const bool has_limit = prm.max_time > 0.0 && std::isfinite(prm.max_time);
   bool cs_dual_clean = false, cs_eq_clean = false;
+  // The last phase's plan and statistics, kept while the LP is unchanged
+  // and the plan pristine (plan_pristine).
+  std::optional<PhasePlan> kept_plan;
+  std::optional<PhaseStats> kept_stats;
+  bool kept_valid = false;
   auto invalidate_clean = [&]() {
     cs_dual_clean = false;
     cs_eq_clean = false;
+    kept_valid = false;  // the LP changed
+  };
+  auto phase_begin = [&]() -> PhasePlan& {
+    if (kept_valid) {
+      if (amd::plan_reuse_checked()) verify_kept_plan(kept_plan->p, kept_stats->s, cur.lp);
+      amd::count_plan_reuse();
+      return *kept_plan;
+    }
+    kept_stats.reset();
+    kept_plan.reset();
+    prof.run("plan + stats", [&] {
+      kept_plan.emplace(cur.lp, false);
+      kept_stats.emplace(cur.lp, &kept_plan->p);
+    });
+    return *kept_plan;
   };
   auto time_exceeded = [&]() {
     if (!has_limit) return false;
@@ -632,20 +707,15 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     }
     amd::free_csr(dbl_at.m);
     dbl_at.m = DeviceCsrMatrix{};
+    kept_valid = !changed && !terminal && plan_pristine(p);
     return changed;
   };
 
   // run_row_phase (gpu_presolve.cu:2219-2301)
   auto row_phase = [&](bool empty, bool singleton, bool redundant, bool implied, bool dup) -> bool {
     if (terminal || time_exceeded()) return false;
-    std::optional<PhasePlan> opp;
-    std::optional<PhaseStats> ost;
-    prof.run("plan + stats", [&] {
-      opp.emplace(cur.lp, false);
-      ost.emplace(cur.lp, &opp->p);
-    });
-    PhasePlan& pp = *opp;
-    PhaseStats& st = *ost;
+    PhasePlan& pp = phase_begin();
+    PhaseStats& st = *kept_stats;
     PresolvePlanGpu& p = pp.p;
     const LPInfoGpu& v = cur.lp;
     if (!time_exceeded() && empty && prm.enable_empty_rows)
@@ -665,14 +735,8 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
   auto col_phase = [&](bool fixed, bool empty_cols, bool cs_dual, bool cs_eq, bool doubleton, bool dual_fix,
                        bool dup_cols, bool redundant_bounds) -> bool {
     if (terminal || time_exceeded()) return false;
-    std::optional<PhasePlan> opp;
-    std::optional<PhaseStats> ost;
-    prof.run("plan + stats", [&] {
-      opp.emplace(cur.lp, false);
-      ost.emplace(cur.lp, &opp->p);
-    });
-    PhasePlan& pp = *opp;
-    PhaseStats& st = *ost;
+    PhasePlan& pp = phase_begin();
+    PhaseStats& st = *kept_stats;
     PresolvePlanGpu& p = pp.p;
     const LPInfoGpu& v = cur.lp;
     const bool run_redundant =

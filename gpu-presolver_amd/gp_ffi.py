@@ -124,6 +124,10 @@ def lib():
         L.gp_mps_upload.argtypes = [vp, C.POINTER(T.GpLp)]
         L.gp_mps_free.argtypes = [vp]
         L.gp_mps_free.restype = None
+        L.gp_test_check_plan_reuse.argtypes = [C.c_int32]
+        L.gp_test_check_plan_reuse.restype = None
+        L.gp_test_plan_reuses.argtypes = []
+        L.gp_test_plan_reuses.restype = C.c_int64
         L.gp_pdhg_params_default.argtypes = [C.POINTER(T.GpPdhgParams)]
         L.gp_pdhg_params_default.restype = None
         L.gp_pdhg_solve.argtypes = [C.POINTER(T.GpLp), C.POINTER(T.GpPdhgParams), vp, vp, vp,
@@ -145,6 +149,16 @@ def params(**kw):
 
 def has_good_nnz_progress(before, after, ratio):
     return bool(lib().gp_has_good_nnz_progress(before, after, ratio))
+
+
+def check_plan_reuse(on):
+    """Testing hook: every reused phase plan is compared with a fresh one."""
+    lib().gp_test_check_plan_reuse(1 if on else 0)
+
+
+def plan_reuses():
+    """Phases on this thread that reused their predecessor's plan so far."""
+    return int(lib().gp_test_plan_reuses())
 
 
 def set_batched_closure_min_nnz(nnz):

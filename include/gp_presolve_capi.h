@@ -300,6 +300,14 @@ int32_t gp_batched_closure_min_nnz(void);
  * after it has queued the record's device-to-host copies; 0 disarms it.  Used
  * to check that a failed call leaves the thread's next call unaffected. */
 void gp_test_fail_after_record_updates(int32_t count);
+/* Testing hook (no counterpart in the reference): a phase whose predecessor
+ * changed nothing reuses that phase's plan and statistics instead of building
+ * them again.  With on != 0, the calling thread's presolves compare every
+ * reused pair with a freshly built one, bit for bit, and fail with
+ * GP_ERR_INTERNAL on any difference. */
+void gp_test_check_plan_reuse(int32_t on);
+/* Number of phases on the calling thread that reused their predecessor's plan. */
+int64_t gp_test_plan_reuses(void);
 
 /* ---- postsolve (reference gpu_postsolve.hpp:16-27) --------------------- */
 /* postsolve_gpu with caller-provided outputs: x_org, z_org hold n0 doubles,
