@@ -28,9 +28,11 @@ Extra objects on the JSON line:
   end_to_end         the solver hand-off on the same LP: presolve, the reduced
                      LP solved in HBM by gp_pdhg_solve (restarted PDHG, relative
                      KKT tolerance --solve-tol), one-shot postsolve of that
-                     solution; then the same solver on the original LP without
-                     presolve, with the per-iteration bytes model
-                     24 nnz + 76 n + 60 m against 8 TB/s;
+                     solution; the same as one gp_presolve_solve call (the
+                     postsolve prepared on a second stream during the solve);
+                     then the same solver on the original LP without presolve,
+                     with the per-iteration bytes model 24 nnz + 76 n + 60 m
+                     against 8 TB/s;
   cpu_baseline       the C++ oracle presolving the same LP on the host:
                      single thread and the OpenMP build on every usable core,
                      median of five runs after a warm-up, with the core count
@@ -404,7 +406,7 @@ def measure_pdhg_iterations(gp_ffi, dlp, lp, stream, launches, per_launch=64):
     vp = C.c_void_p
     L.gpk_pdhg_row_blocks.restype = C.c_int32
     L.gpk_pdhg_row_blocks.argtypes = [vp, C.c_int32, vp]
-    L.gpk_pdhg_iterations.argtypes = [GpkRows, GpkRows] + [vp] * 11 + [C.c_int32, vp]
+    L.gpk_pdhg_iterations.argtypes = [GpkRows, GpkRows] + [vp] * 13 + [C.c_int32, C.c_int32, vp]
     L.gpk_pdhg_spmv.argtypes = [GpkRows, vp, vp, vp]
     keep = []
 
@@ -420,12 +422,13 @@ def measure_pdhg_iterations(gp_ffi, dlp, lp, stream, launches, per_launch=64):
     v = dlp.vec
     x, xbar, xs = (gp_ffi.DeviceArray(np.zeros(n)) for _ in range(3))
     y, ys = (gp_ffi.DeviceArray(np.zeros(m)) for _ in range(2))
-    step = gp_ffi.DeviceArray(np.array([1e-3, 1e-3]))
+    step = gp_ffi.DeviceArray(np.array([1e-3, 1e-3, 0.0]))
     ck = gp_ffi.hip_check
 
     def iterate():
         ck(L.gpk_pdhg_iterations(A, AT, v["c"].ptr, v["l"].ptr, v["u"].ptr, v["AL"].ptr, v["AU"].ptr, x.ptr,
-                                 xbar.ptr, xs.ptr, y.ptr, ys.ptr, step.ptr, per_launch, stream.ptr), "iterations")
+                                 xbar.ptr, xs.ptr, y.ptr, ys.ptr, x.ptr, y.ptr, step.ptr, 0, per_launch, stream.ptr),
+           "iterations")
 
     def products():
         ck(L.gpk_pdhg_spmv(A, x.ptr, y.ptr, stream.ptr), "spmv A")
@@ -765,6 +768,23 @@ def end_to_end(gp_ffi, dlp, lp, prm, tol):
         f"objective {direct['primal_obj']:.10g}")
     us_it = 1e6 * direct["elapsed_seconds"] / max(1, direct["iterations"])
     total = pre_ms + solve_ms + post_ms
+    # The same three steps as one call (gp_presolve_solve): the postsolve is
+    # prepared on a second stream while the solver runs; best of three.
+    prm_s = gp_ffi.pdhg_params(tol=tol)
+    pipe_out = (gp_ffi.DeviceArray(size=n), gp_ffi.DeviceArray(size=m), gp_ffi.DeviceArray(size=n))
+    pipes = []
+    for _ in range(3):
+        sync()
+        t = time.perf_counter()
+        pr, _ = gp_ffi.presolve_solve(dlp, prm, prm_s, pipe_out)
+        sync()
+        pipes.append((1e3 * (time.perf_counter() - t), pr))
+    pipe_ms, pr = min(pipes, key=lambda q: q[0])
+    if not np.array_equal(pipe_out[0].numpy(), x):
+        raise SystemExit("bench.py: gp_presolve_solve differs from the three steps")
+    log(f"pipelined (gp_presolve_solve): {pipe_ms:.1f} ms (presolve {1e3 * pr['presolve_seconds']:.1f}, solve "
+        f"{1e3 * pr['solve_seconds']:.1f} with the postsolve prepared beside it in "
+        f"{1e3 * pr['prepare_seconds']:.1f}, postsolve {1e3 * pr['postsolve_seconds']:.1f})")
     return {"tol": tol, "presolve_ms": pre_ms, "solve_ms": solve_ms, "postsolve_ms": post_ms, "total_ms": total,
             "reduced": [m1, n1, nnz1], "solve_status": res["status"], "solve_iterations": res["iterations"],
             "solve_restarts": res["restarts"], "objective": obj,
@@ -773,6 +793,9 @@ def end_to_end(gp_ffi, dlp, lp, prm, tol):
             "no_presolve_iterations": direct["iterations"], "no_presolve_objective": direct["primal_obj"],
             "no_presolve_kkt": [direct["primal_res"], direct["dual_res"], direct["gap"]],
             "speedup_total_vs_no_presolve": direct_ms / total,
+            "pipelined_total_ms": pipe_ms,
+            "pipelined_parts_ms": {k: 1e3 * pr[k + "_seconds"] for k in ("presolve", "solve", "prepare", "postsolve")},
+            "speedup_pipelined_vs_no_presolve": direct_ms / pipe_ms,
             "no_presolve_us_per_iteration_with_checks": us_it,
             "iteration_roofline": measure_pdhg_iterations(gp_ffi, dlp, lp, gp_ffi.Stream(), 10)}
 

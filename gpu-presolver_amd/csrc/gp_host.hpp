@@ -481,6 +481,27 @@ struct PdhgReport {
 // hold the best iterate (unscaled), z the reduced costs projected on the
 // cone of the column bounds.
 PdhgReport pdhg_solve(const presolve::LPInfoGpu& lp, const PdhgOptions& opt, double* x, double* y, double* z);
+
+// ---- presolve + solve + postsolve (host_pipeline.cpp) ----------------------
+}  // namespace gpu_presolver::amd
+namespace gpu_presolver::presolve {
+struct GpuPresolveSummary;
+}  // namespace gpu_presolver::presolve
+namespace gpu_presolver::amd {
+struct PipelineReport {
+  int32_t reduced_rows = 0, reduced_cols = 0, reduced_nnz = 0, iterations = 0;
+  double obj_constant_delta = 0.0;
+  bool has_infeasible = false, has_unbounded = false;
+  PdhgReport solve;  // status -1: not solved (presolve found the LP infeasible or unbounded)
+  double presolve_seconds = 0.0, solve_seconds = 0.0, prepare_seconds = 0.0, postsolve_seconds = 0.0,
+         total_seconds = 0.0;
+  void copy_summary(const presolve::GpuPresolveSummary& s);
+};
+// Presolves lp, solves the reduced LP in device memory with pdhg_solve while
+// a second thread and stream prepare the postsolve, then postsolves into x,
+// z (lp.A.cols doubles) and y (lp.A.rows), device arrays.
+PipelineReport presolve_solve(const presolve::LPInfoGpu& lp, const presolve::PresolveParams& prm,
+                              const PdhgOptions& solve, double* x, double* y, double* z);
 void screen_structural_l1(const presolve::LPInfoGpu& lp, const presolve::PresolveParams& prm);
 // old_to_new map of a keep mask; returns the kept count.
 int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map);

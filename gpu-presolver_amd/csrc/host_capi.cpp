@@ -1024,6 +1024,49 @@ void gp_pdhg_params_default(gp_pdhg_params* params) {
                            d.verbose, d.method};
 }
 
+namespace {
+amd::PdhgOptions pdhg_options(const gp_pdhg_params* params) {
+  amd::PdhgOptions o;
+  if (params != nullptr)
+    o = amd::PdhgOptions{params->max_iters,   params->max_time,    params->tol,
+                         params->check_every, params->ruiz_iters,  params->power_iters,
+                         params->use_graph,   params->verbose,     params->method};
+  return o;
+}
+gp_pdhg_result pdhg_result(const amd::PdhgReport& r) {
+  return gp_pdhg_result{r.status,   r.iterations,      r.restarts,      r.checks,        r.primal_obj,
+                        r.dual_obj, r.primal_res,      r.dual_res,      r.gap,           r.elapsed_seconds,
+                        r.norm_estimate, r.primal_weight, r.graph_launches};
+}
+}  // namespace
+
+gp_status gp_presolve_solve(const gp_lp* lp, const gp_params* params, const gp_pdhg_params* solve_params,
+                            double* x, double* y, double* z, gp_pipeline_result* out) {
+  return guarded([&] {
+    require(lp != nullptr && out != nullptr, "null argument");
+    *out = gp_pipeline_result{};
+    const LPInfoGpu L = to_lp(*lp);
+    require(L.A.cols == 0 || (x && z), "null x/z");
+    require(L.A.rows == 0 || y, "null y");
+    require_device();
+    const PresolveParams prm = to_params(params);
+    const amd::PipelineReport r = amd::presolve_solve(L, prm, pdhg_options(solve_params), x, y, z);
+    out->reduced_rows = r.reduced_rows;
+    out->reduced_cols = r.reduced_cols;
+    out->reduced_nnz = r.reduced_nnz;
+    out->presolve_iterations = r.iterations;
+    out->obj_constant_delta = r.obj_constant_delta;
+    out->has_infeasible = r.has_infeasible ? 1 : 0;
+    out->has_unbounded = r.has_unbounded ? 1 : 0;
+    out->solve = pdhg_result(r.solve);
+    out->presolve_seconds = r.presolve_seconds;
+    out->solve_seconds = r.solve_seconds;
+    out->prepare_seconds = r.prepare_seconds;
+    out->postsolve_seconds = r.postsolve_seconds;
+    out->total_seconds = r.total_seconds;
+  });
+}
+
 gp_status gp_pdhg_solve(const gp_lp* lp, const gp_pdhg_params* params, double* x, double* y, double* z,
                         gp_pdhg_result* out) {
   return guarded([&] {
@@ -1033,15 +1076,9 @@ gp_status gp_pdhg_solve(const gp_lp* lp, const gp_pdhg_params* params, double* x
     require(L.A.cols == 0 || (L.c && L.l && L.u), "null c/l/u");
     require(L.A.rows == 0 || (L.AL && L.AU), "null AL/AU");
     require_device();
-    amd::PdhgOptions o;
-    if (params != nullptr)
-      o = amd::PdhgOptions{params->max_iters,   params->max_time,    params->tol,
-                           params->check_every, params->ruiz_iters,  params->power_iters,
-                           params->use_graph,   params->verbose,     params->method};
-    const amd::PdhgReport r = amd::pdhg_solve(L, o, x, y, z);
+    const amd::PdhgReport r = amd::pdhg_solve(L, pdhg_options(params), x, y, z);
     amd::ctx().sync("gp_pdhg_solve");
-    *out = gp_pdhg_result{r.status, r.iterations, r.restarts, r.checks, r.primal_obj, r.dual_obj, r.primal_res,
-                          r.dual_res, r.gap, r.elapsed_seconds, r.norm_estimate, r.primal_weight, r.graph_launches};
+    *out = pdhg_result(r);
   });
 }
 

@@ -132,6 +132,8 @@ def lib():
         L.gp_pdhg_params_default.restype = None
         L.gp_pdhg_solve.argtypes = [C.POINTER(T.GpLp), C.POINTER(T.GpPdhgParams), vp, vp, vp,
                                     C.POINTER(T.GpPdhgResult)]
+        L.gp_presolve_solve.argtypes = [C.POINTER(T.GpLp), C.POINTER(T.GpParams), C.POINTER(T.GpPdhgParams), vp, vp,
+                                        vp, C.POINTER(T.GpPipelineResult)]
         _lib = L
     return _lib
 
@@ -517,6 +519,22 @@ def pdhg_solve_view(g, prm=None, out=None):
     prm = prm if prm is not None else pdhg_params()
     _check(lib().gp_pdhg_solve(C.byref(g), C.byref(prm), out[0].ptr, out[1].ptr, out[2].ptr, C.byref(r)))
     return {k: getattr(r, k) for k, _ in T.GpPdhgResult._fields_}, out
+
+
+def presolve_solve(dlp, prm, sprm, out=None):
+    """gp_presolve_solve on a resident DeviceLp: presolve, the reduced LP
+    solved in HBM while the postsolve is prepared, then postsolved.  Returns
+    (result dict with the solve's dict under "solve", (x, y, z) DeviceArrays
+    in the original LP's space)."""
+    m, n = dlp.m, dlp.n
+    if out is None:
+        out = (DeviceArray(size=n), DeviceArray(size=m), DeviceArray(size=n))
+    r = T.GpPipelineResult()
+    _check(lib().gp_presolve_solve(C.byref(dlp.g), C.byref(prm), C.byref(sprm), out[0].ptr, out[1].ptr, out[2].ptr,
+                                   C.byref(r)))
+    d = {k: getattr(r, k) for k, _ in T.GpPipelineResult._fields_ if k != "solve"}
+    d["solve"] = {k: getattr(r.solve, k) for k, _ in T.GpPdhgResult._fields_}
+    return d, out
 
 
 def pdhg_solve(lp, **kw):

@@ -148,6 +148,14 @@ class GpPdhgResult(C.Structure):
                 ("graph_launches", C.c_int64)]
 
 
+class GpPipelineResult(C.Structure):
+    _fields_ = [("reduced_rows", C.c_int32), ("reduced_cols", C.c_int32), ("reduced_nnz", C.c_int32),
+                ("presolve_iterations", C.c_int32), ("obj_constant_delta", C.c_double),
+                ("has_infeasible", C.c_int32), ("has_unbounded", C.c_int32), ("solve", GpPdhgResult),
+                ("presolve_seconds", C.c_double), ("solve_seconds", C.c_double), ("prepare_seconds", C.c_double),
+                ("postsolve_seconds", C.c_double), ("total_seconds", C.c_double)]
+
+
 class GpLpSpec(C.Structure):
     _fields_ = [("seed", C.c_uint64), ("m", C.c_int32), ("n", C.c_int32), ("len_lo", C.c_int32),
                 ("len_hi", C.c_int32), ("mean_len", C.c_double), ("min_len", C.c_int32)]

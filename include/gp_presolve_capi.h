@@ -517,6 +517,32 @@ void gp_pdhg_params_default(gp_pdhg_params* params);
 gp_status gp_pdhg_solve(const gp_lp* lp, const gp_pdhg_params* params, double* x, double* y, double* z,
                         gp_pdhg_result* out);
 
+/* Presolve, solve and postsolve of one LP in one call: gp_presolve, then
+ * gp_pdhg_solve on the device-resident reduced LP while a second thread and
+ * stream prepare the postsolve (gp_postsolve_prepare's work, which needs only
+ * the record), then the prepared postsolve of the solution against `lp`.
+ * x, z: device arrays of lp->A.cols doubles, y of lp->A.rows; they receive
+ * exactly what those calls one after another give.  When presolve finds the
+ * LP infeasible or unbounded nothing is solved: solve.status is -1 and x, y,
+ * z are zero. */
+typedef struct gp_pipeline_result {
+  int32_t reduced_rows;
+  int32_t reduced_cols;
+  int32_t reduced_nnz;
+  int32_t presolve_iterations;
+  double obj_constant_delta;
+  int32_t has_infeasible;
+  int32_t has_unbounded;
+  gp_pdhg_result solve;
+  double presolve_seconds;
+  double solve_seconds;
+  double prepare_seconds;  /* postsolve preparation, overlapped with the solve */
+  double postsolve_seconds;
+  double total_seconds;
+} gp_pipeline_result;
+gp_status gp_presolve_solve(const gp_lp* lp, const gp_params* params, const gp_pdhg_params* solve_params,
+                            double* x, double* y, double* z, gp_pipeline_result* out);
+
 #ifdef __cplusplus
 }
 #endif

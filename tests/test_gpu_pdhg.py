@@ -206,3 +206,42 @@ def test_hand_off_on_the_block_structured_lp(method):
     b = np.maximum(np.where(np.isfinite(lp["AL"]), np.abs(lp["AL"]), 0.0),
                    np.where(np.isfinite(lp["AU"]), np.abs(lp["AU"]), 0.0))
     assert K.primal_violation(lp, xo) <= 10 * tol * (1.0 + np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("case", ["config0", "block"])
+def test_pipeline_equals_the_steps(case):
+    """gp_presolve_solve (postsolve prepared on a second stream while the
+    solver runs) gives exactly what gp_presolve, gp_pdhg_solve on the
+    reduced view and gp_postsolve give one after another."""
+    lp = lp_synth.config0(seed=0) if case == "config0" else \
+        lp_synth.config2(seed=2, blocks=100, link_rows=10, link_nnz=1500)
+    tol = 1e-6 if case == "config0" else 1e-4
+    prm = gp_ffi.params(enable_structure_specific_rules=0)
+    sprm = gp_ffi.pdhg_params(tol=tol)
+    dlp = gp_ffi.DeviceLp(lp)
+    res, (px, py, pz) = gp_ffi.presolve_solve(dlp, prm, sprm)
+    h = gp_ffi.Presolved(dlp, prm, True)
+    try:
+        sres, (dx, dy, dz) = gp_ffi.pdhg_solve_view(h.reduced_view(), sprm)
+        rec = h.record()
+        s = h.summary()
+    finally:
+        h.free()
+    xo, yo, zo = gp_ffi.postsolve(rec, dx.numpy(), dy.numpy(), dz.numpy(), original_dev=dlp)
+    assert (res["reduced_rows"], res["reduced_cols"], res["reduced_nnz"]) == \
+           (s["reduced_rows"], s["reduced_cols"], s["reduced_nnz"])
+    assert res["solve"]["status"] == 0 and res["solve"]["iterations"] == sres["iterations"]
+    for a, b in zip((px.numpy(), py.numpy(), pz.numpy()), (xo, yo, zo)):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert res["prepare_seconds"] > 0.0 and res["total_seconds"] >= res["presolve_seconds"]
+
+
+def test_pipeline_on_an_infeasible_lp():
+    """Presolve finds the LP infeasible (an empty row whose bounds exclude
+    zero): nothing is solved, the outputs are zero and the flag comes back."""
+    lp = _lp_from_dense([[1, 1], [0, 0]], [1, 1], [3, 2], [1, 1], [0, 0], [5, 5])
+    dlp = gp_ffi.DeviceLp(lp)
+    res, (px, py, pz) = gp_ffi.presolve_solve(dlp, gp_ffi.params(enable_structure_specific_rules=0),
+                                              gp_ffi.pdhg_params())
+    assert res["has_infeasible"] == 1 and res["solve"]["status"] == -1
+    assert not px.numpy().any() and not py.numpy().any() and not pz.numpy().any()
