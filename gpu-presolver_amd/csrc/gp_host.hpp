@@ -429,15 +429,17 @@ presolve::DeviceCsrMatrix compact_csr(const presolve::DeviceCsrMatrix& A, const 
                                       const int32_t* col_map, int32_t m_new, int32_t n_new, int32_t known_nnz = -1,
                                       const gpk_rank_word* col_rank = nullptr);
 // The same compaction in two halves, for a caller that reads the new sizes
-// in its own wait: compact_start launches the map pass and the tile-offset
-// scan while the sizes are still unknown (every kept column goes through
-// col_rank, which must be given) and copies the kept entry count to the
-// device word `kept`; compact_finish then writes the compacted matrix.  The
-// row-pointer array is sized for A.rows + 1.
+// in its own wait: compact_start launches the map pass (and, past 256 tiles,
+// the tile-offset scan) while the sizes are still unknown (every kept column
+// goes through col_rank, which must be given); the map pass adds the kept
+// entry count to the device word `kept`, which the caller zeroes.
+// compact_finish then writes the compacted matrix.  The row-pointer array is
+// sized for A.rows + 1.
 struct CompactStart {
   const presolve::DeviceCsrMatrix* A = nullptr;
   const int32_t* row_map = nullptr;
   DBuf<int32_t> ptr, jn, tile_off;
+  bool few_tiles = false;  // tile_off holds raw counts (no scan launched)
 };
 CompactStart compact_start(const presolve::DeviceCsrMatrix& A, const int32_t* row_map, const gpk_rank_word* col_rank,
                            int32_t* kept);

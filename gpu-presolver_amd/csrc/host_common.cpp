@@ -550,10 +550,12 @@ CompactStart compact_start(const DeviceCsrMatrix& A, const int32_t* row_map, con
   const int32_t tiles = gpk_compact_tiles(A.rows);
   st.jn.alloc(A.nnz, "compact entry map");
   st.tile_off.alloc(static_cast<int64_t>(tiles) + 1, "compact tile offsets");
-  check(gpk_compact_map(A.rowPtr, A.colVal, A.rows, row_map, col_rank, st.jn, st.ptr, st.tile_off, s), "compact map");
-  check(gpk_scan_lb_i32(st.tile_off, st.tile_off, tiles, c.lookback(gpk_lookback_tiles(tiles)), s),
-        "compact tile offsets");
-  CopyBatch().copy(kept, st.tile_off.get() + tiles, 1).run("compact kept count");
+  check(gpk_compact_map_total(A.rowPtr, A.colVal, A.rows, row_map, col_rank, st.jn, st.ptr, st.tile_off, kept, s),
+        "compact map");
+  st.few_tiles = tiles <= kCompactFewTiles;  // the write pass derives the offsets from the raw counts
+  if (!st.few_tiles)
+    check(gpk_scan_lb_i32(st.tile_off, st.tile_off, tiles, c.lookback(gpk_lookback_tiles(tiles)), s),
+          "compact tile offsets");
   return st;
 }
 
@@ -571,8 +573,13 @@ DeviceCsrMatrix compact_finish(CompactStart& st, int32_t m_new, int32_t n_new, i
   }
   DBuf<int32_t> idx(A.nnz, "compact colVal");
   DBuf<double> val(A.nnz, "compact nzVal");
-  check(gpk_compact_write(A.rowPtr, A.nzVal, A.rows, st.row_map, st.jn, st.tile_off, st.ptr, m_new, idx, val, s),
-        "compact write");
+  if (st.few_tiles)
+    check(gpk_compact_write_counts(A.rowPtr, A.nzVal, A.rows, st.row_map, st.jn, st.tile_off, st.ptr, m_new, idx, val,
+                                   s),
+          "compact write");
+  else
+    check(gpk_compact_write(A.rowPtr, A.nzVal, A.rows, st.row_map, st.jn, st.tile_off, st.ptr, m_new, idx, val, s),
+          "compact write");
   out.nnz = nnz_new;
   out.rowPtr = st.ptr.release();
   out.colVal = idx.release();

@@ -562,7 +562,8 @@ __global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restric
                                                         const int32_t* __restrict__ row_map,
                                                         const gpk_rank_word* __restrict__ col_rank,
                                                         int32_t* __restrict__ jn_out, int32_t* __restrict__ out_ptr,
-                                                        int32_t* __restrict__ tile_cnt) {
+                                                        int32_t* __restrict__ tile_cnt,
+                                                        int32_t* __restrict__ kept_total) {
   __shared__ int32_t s_cnt[kCompRows];
   __shared__ int32_t s_wsum[kBlock / 64];
   __shared__ uint8_t s_keep[kCompRows];
@@ -617,6 +618,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restric
     int32_t t = 0;
     for (int w = 0; w < kBlock / 64; ++w) t += s_wsum[w];
     tile_cnt[blockIdx.x] = t;
+    if (kept_total != nullptr && t != 0) atomicAdd(kept_total, t);  // integer: order-free
   }
 }
 
@@ -1314,9 +1316,15 @@ int32_t gpk_compact_tiles(int32_t m_old) { return m_old > 0 ? static_cast<int32_
 hipError_t gpk_compact_map(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
                            const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, int32_t* tile_cnt,
                            hipStream_t s) {
+  return gpk_compact_map_total(ptr, idx, m_old, row_map, col_rank, jn, out_ptr, tile_cnt, nullptr, s);
+}
+
+hipError_t gpk_compact_map_total(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
+                                 const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, int32_t* tile_cnt,
+                                 int32_t* kept_total, hipStream_t s) {
   if (m_old <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_compact_map, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, idx, m_old, row_map,
-                     col_rank, jn, out_ptr, tile_cnt);
+                     col_rank, jn, out_ptr, tile_cnt, kept_total);
   return hipGetLastError();
 }
 

@@ -115,6 +115,12 @@ hipError_t gpk_compact_map(const int32_t* ptr, const int32_t* idx, int32_t m_old
 hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
                              const int32_t* jn, const int32_t* tile_off, int32_t* out_ptr, int32_t m_new,
                              int32_t* out_idx, double* out_val, hipStream_t stream);
+/* gpk_compact_map that also adds the kept entry count to *kept_total (one
+ * integer atomic per tile; the caller zeroes it), so the count is known
+ * before any scan. */
+hipError_t gpk_compact_map_total(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
+                                 const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, int32_t* tile_cnt,
+                                 int32_t* kept_total, hipStream_t stream);
 /* The same without the scan, for at most 256 tiles (m_old <= 65,536): it
  * takes the raw tile counts of gpk_compact_map and derives the offsets. */
 hipError_t gpk_compact_write_counts(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
