@@ -590,13 +590,14 @@ constexpr int kDblInnerRounds = 1024;
 
 struct DblWork {
   DBuf<uint8_t> acceptable, active, selected, blocked;
-  DBuf<int32_t> cec, ckc, owner, live_nnz;
+  DBuf<int32_t> cec, ckc, live_nnz;
+  DBuf<uint64_t> owner;
   DBuf<double> cev, ckv;
   DBuf<uint8_t> is_elim, is_fixed_keep;
   DBuf<double> alpha_of, beta_of, fixed_val_of;
   DBuf<int32_t> row_of, keep_of;
   DblWork(int32_t m, int32_t n)
-      : acceptable(m), active(m), selected(m), blocked(n), cec(m), ckc(m), owner(n), live_nnz(n), cev(m), ckv(m),
+      : acceptable(m), active(m), selected(m), blocked(n), cec(m), ckc(m), live_nnz(n), owner(n), cev(m), ckv(m),
         is_elim(n), is_fixed_keep(n), alpha_of(n), beta_of(n), fixed_val_of(n), row_of(n), keep_of(n) {}
   gpk_dbl view() {
     return gpk_dbl{acceptable, active, selected, blocked, cec, ckc, cev, ckv, owner, live_nnz};
@@ -672,10 +673,10 @@ bool dbl_batch_once(PresolvePlanGpu& plan, const LPInfoGpu& lp, const DeviceCsrM
   int32_t* f = c.flags();
   check(gpk_dbl_candidates(A, AT, P, d, zt, tol, rtol, prm.doubleton_equations_max_fill_in_proxy, 1, f, s),
         "doubleton candidates");
-  zero(d.blocked, n, "doubleton blocked");
-  for (int r = 0; r < kDblMatchRounds; ++r) {
-    check(gpk_dbl_claim(m, n, d, s), "doubleton claim");
-    check(gpk_dbl_select(m, d, f, s), "doubleton select");
+  amd::CopyBatch().zero(d.blocked, n).zero(d.owner, n).run("doubleton blocked and owners");
+  for (int r = 1; r <= kDblMatchRounds; ++r) {
+    check(gpk_dbl_claim(m, d, r, s), "doubleton claim");
+    check(gpk_dbl_select(m, d, r, f, s), "doubleton select");
   }
   const int32_t* h = c.read_flags(2);
   const int32_t acceptable = h[0], k = h[1];

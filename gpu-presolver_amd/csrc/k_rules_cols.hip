@@ -829,27 +829,32 @@ __global__ void k_dbl_candidates(gpk_csr A, gpk_csr AT, gpk_plan p, gpk_dbl d, d
   wave_add1(&counts[0]);
 }
 
-__global__ void k_dbl_reset_owner(int32_t n, gpk_dbl d) {
-  const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j < n) d.owner[j] = INT_MAX;
+// A claim of column ownership in `round` by row i: larger for a later round,
+// and within a round larger for a lower row, so an atomic max keeps the
+// lowest claiming row of the latest round (a minimum-row claim, as the
+// reference's per-round atomicMin after a reset).
+__device__ inline unsigned long long dbl_stamp(int32_t round, int32_t i) {
+  return (static_cast<unsigned long long>(round) << 32) | (0xffffffffull - static_cast<uint32_t>(i));
 }
 
-__global__ void k_dbl_claim(int32_t m, gpk_dbl d) {
+__global__ void k_dbl_claim(int32_t m, gpk_dbl d, int32_t round) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m || !d.active[i]) return;
   const int32_t k = d.ckc[i], e = d.cec[i];
   if (d.blocked[k] || d.blocked[e]) return;
-  atomicMin(&d.owner[k], i);
-  atomicMin(&d.owner[e], i);
+  const unsigned long long v = dbl_stamp(round, i);
+  atomicMax(reinterpret_cast<unsigned long long*>(&d.owner[k]), v);
+  atomicMax(reinterpret_cast<unsigned long long*>(&d.owner[e]), v);
 }
 
-// counts[1]: selected rows.  A row wins when it owns both of its columns;
-// owners are unique, so winners never share a column.
-__global__ void k_dbl_select(int32_t m, gpk_dbl d, int32_t* counts) {
+// counts[1]: selected rows.  A row wins when it owns both of its columns in
+// this round; owners are unique, so winners never share a column.
+__global__ void k_dbl_select(int32_t m, gpk_dbl d, int32_t round, int32_t* counts) {
   const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m || !d.active[i]) return;
   const int32_t k = d.ckc[i], e = d.cec[i];
-  if (!d.blocked[k] && !d.blocked[e] && d.owner[k] == i && d.owner[e] == i) {
+  const uint64_t v = dbl_stamp(round, i);
+  if (!d.blocked[k] && !d.blocked[e] && d.owner[k] == v && d.owner[e] == v) {
     d.selected[i] = 1;
     d.active[i] = 0;
     d.blocked[k] = 1;
@@ -1266,12 +1271,11 @@ hipError_t gpk_dbl_candidates(gpk_csr A, gpk_csr AT, gpk_plan plan, gpk_dbl d, d
                               hipStream_t stream) {
   GPK_LAUNCH(k_dbl_candidates, A.rows, A, AT, plan, d, zero_tol, tol, ratio_tol, max_fill, live_fill, counts);
 }
-hipError_t gpk_dbl_claim(int32_t m, int32_t n, gpk_dbl d, hipStream_t stream) {
-  if (n > 0) hipLaunchKernelGGL(k_dbl_reset_owner, dim3(grid_for(n)), dim3(kBlock), 0, stream, n, d);
-  GPK_LAUNCH(k_dbl_claim, m, m, d);
+hipError_t gpk_dbl_claim(int32_t m, gpk_dbl d, int32_t round, hipStream_t stream) {
+  GPK_LAUNCH(k_dbl_claim, m, m, d, round);
 }
-hipError_t gpk_dbl_select(int32_t m, gpk_dbl d, int32_t* counts, hipStream_t stream) {
-  GPK_LAUNCH(k_dbl_select, m, m, d, counts);
+hipError_t gpk_dbl_select(int32_t m, gpk_dbl d, int32_t round, int32_t* counts, hipStream_t stream) {
+  GPK_LAUNCH(k_dbl_select, m, m, d, round, counts);
 }
 hipError_t gpk_dbl_apply(int32_t k, const int32_t* sel, gpk_plan plan, gpk_dbl d, gpk_dbl_cols cols, double tol,
                          uint8_t* subst, uint8_t* fixedp, double* osub, double* ofix, int32_t* rec_i3,

@@ -508,7 +508,7 @@ typedef struct gpk_dbl {
   int32_t* ckc;
   double* cev;
   double* ckv;
-  int32_t* owner;
+  uint64_t* owner; /* per column: (round << 32) | (0xffffffff - row) of the round's lowest claiming row */
   int32_t* live_nnz;
 } gpk_dbl;
 typedef struct gpk_dbl_cols {
@@ -524,8 +524,14 @@ hipError_t gpk_dbl_live_nnz(gpk_csr AT, gpk_plan plan, double zero_tol, int32_t*
 hipError_t gpk_dbl_candidates(gpk_csr A, gpk_csr AT, gpk_plan plan, gpk_dbl d, double zero_tol, double tol,
                               double ratio_tol, int32_t max_fill, int32_t live_fill, int32_t* counts,
                               hipStream_t stream);
-hipError_t gpk_dbl_claim(int32_t m, int32_t n, gpk_dbl d, hipStream_t stream);
-hipError_t gpk_dbl_select(int32_t m, gpk_dbl d, int32_t* counts, hipStream_t stream);
+/* One claim round of the matching (rule_doubleton_equations.cu:876-913): every
+ * active row with both columns unblocked claims them for itself, the lowest
+ * row index winning a column.  Owners carry their round (1, 2, ...; the
+ * owner array is zeroed once before round 1), so no reset between rounds is
+ * needed.  gpk_dbl_select then selects the rows that own both of their
+ * columns in that round and blocks those columns; counts[1] += selected. */
+hipError_t gpk_dbl_claim(int32_t m, gpk_dbl d, int32_t round, hipStream_t stream);
+hipError_t gpk_dbl_select(int32_t m, gpk_dbl d, int32_t round, int32_t* counts, hipStream_t stream);
 hipError_t gpk_dbl_apply(int32_t k, const int32_t* sel, gpk_plan plan, gpk_dbl d, gpk_dbl_cols cols, double tol,
                          uint8_t* subst, uint8_t* fixedp, double* osub, double* ofix, int32_t* rec_i3,
                          double* rec_v12, int32_t* flags, hipStream_t stream);
