@@ -229,9 +229,10 @@ void apply_rule_implied_variable_bounds(PresolvePlanGpu& plan, const LPInfoGpu& 
   DBuf<double> rmin(m), rmax(m);
   DBuf<int32_t> rni(m), rpi(m);
   row_activity(A.ptr, A.idx, A.val, m, n, plan.new_l, plan.new_u, prm.zero_tol, rmin, rmax, rni, rpi, s);
+  DBuf<gpk_implied_row> rows(m);
+  check(gpk_implied_pack_rows(m, P, stats.row_nnz, rmin, rmax, rni, rpi, rows, s), "implied bounds row records");
   DBuf<double> cl(n), cu(n), vmax(n);
-  check(gpk_implied_candidates(AT, P, stats.row_nnz, rmin, rmax, rni, rpi, prm.zero_tol, cl, cu, s),
-        "implied bounds candidates");
+  check(gpk_implied_candidates(AT, P, rows, prm.zero_tol, cl, cu, s), "implied bounds candidates");
   check(gpk_col_max_abs(AT.ptr, AT.val, n, vmax, s), "implied bounds col max");
   DBuf<double> fl(n), fu(n), fval(n);
   DBuf<uint8_t> lch(n), uch(n), fixed(n);
@@ -250,8 +251,8 @@ void apply_rule_implied_variable_bounds(PresolvePlanGpu& plan, const LPInfoGpu& 
   if (prm.record_postsolve_tape && tightened) {
     DBuf<int32_t> sl(n), su(n), cnt(static_cast<int64_t>(n) + 1);
     const double support_tol = std::fmax(10.0 * prm.bound_tol, 1.0e-10);
-    check(gpk_implied_support(AT, P, stats.row_nnz, rmin, rmax, rni, rpi, lch, uch, fl, fu, prm.zero_tol,
-                              prm.bound_tol, support_tol, sl, su, cnt, s),
+    check(gpk_implied_support(AT, P, rows, lch, uch, fl, fu, prm.zero_tol, prm.bound_tol, support_tol, sl, su, cnt,
+                              s),
           "implied bounds support rows");
     const int32_t records = amd::scan_counts(cnt, n, cnt);
     if (records > 0) {

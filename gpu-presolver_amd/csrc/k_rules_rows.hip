@@ -232,9 +232,8 @@ __device__ inline void implied_from_row(double a, double lo, double hi, double r
 // values load together, then every row quantity of the U rows, so a chunk
 // costs two dependent loads instead of three per entry.
 template <class F>
-__device__ inline void implied_walk(const gpk_csr& AT, const gpk_plan& p, const int32_t* row_nnz, const double* rmin,
-                                    const double* rmax, const int32_t* rni, const int32_t* rpi, double ztol, int32_t j,
-                                    double ol, double ou, F&& f) {
+__device__ inline void implied_walk(const gpk_csr& AT, const gpk_implied_row* __restrict__ rows, double ztol,
+                                    int32_t j, double ol, double ou, F&& f) {
   constexpr int U = 4;
   const int32_t qb = AT.ptr[j], qe = AT.ptr[j + 1];
   for (int32_t q0 = qb; q0 < qe; q0 += U) {
@@ -246,48 +245,55 @@ __device__ inline void implied_walk(const gpk_csr& AT, const gpk_plan& p, const 
       ri[k] = AT.idx[q];
       av[k] = AT.val[q];
     }
-    bool live[U];
-    double smin[U], smax[U], lo[U], hi[U];
-    int32_t ni[U], pi[U];
+    gpk_implied_row r[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) r[k] = rows[ri[k]];
 #pragma unroll
     for (int k = 0; k < U; ++k) {
-      const int32_t i = ri[k];
-      live[k] = p.keep_row[i] != 0 && row_nnz[i] > 1;
-      smin[k] = rmin[i];
-      smax[k] = rmax[i];
-      ni[k] = rni[i];
-      pi[k] = rpi[i];
-      lo[k] = p.AL[i];
-      hi[k] = p.AU[i];
-    }
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      if (q0 + k >= qe || !live[k]) continue;
+      if (q0 + k >= qe || !r[k].live) continue;
       const double a = av[k];
       if (fabs(a) <= ztol) continue;
       double tmin, tmax;
       term_interval(a, ol, ou, &tmin, &tmax);
-      const double rs_min = resid_min(smin[k], ni[k], tmin);
-      const double rs_max = resid_max(smax[k], pi[k], tmax);
+      const double rs_min = resid_min(r[k].smin, r[k].ninf, tmin);
+      const double rs_max = resid_max(r[k].smax, r[k].pinf, tmax);
       double il, iu;
-      implied_from_row(a, lo[k], hi[k], rs_min, rs_max, &il, &iu);
+      implied_from_row(a, r[k].lo, r[k].hi, rs_min, rs_max, &il, &iu);
       f(ri[k], il, iu);
     }
   }
 }
 
+// One record per row: the activity summary, the plan's row bounds and
+// whether the row takes part (kept, more than one entry).
+__global__ void k_implied_pack_rows(int32_t m, gpk_plan p, const int32_t* row_nnz, const double* rmin,
+                                    const double* rmax, const int32_t* rni, const int32_t* rpi,
+                                    gpk_implied_row* rows) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  gpk_implied_row r;
+  r.smin = rmin[i];
+  r.smax = rmax[i];
+  r.lo = p.AL[i];
+  r.hi = p.AU[i];
+  r.ninf = rni[i];
+  r.pinf = rpi[i];
+  r.live = p.keep_row[i] != 0 && row_nnz[i] > 1 ? 1 : 0;
+  for (int k = 0; k < 5; ++k) r.pad[k] = 0;
+  rows[i] = r;
+}
+
 // Candidate bounds per column, pulled along AT (no atomics): the max (min)
 // over the column's live rows of the implied lower (upper) bound, seeded with
 // the current bound.  Same values as the reference's per-entry CAS push.
-__global__ __launch_bounds__(kBlock) void k_implied_candidates(gpk_csr AT, gpk_plan p, const int32_t* row_nnz,
-                                                               const double* rmin, const double* rmax,
-                                                               const int32_t* rni, const int32_t* rpi, double ztol,
+__global__ __launch_bounds__(kBlock) void k_implied_candidates(gpk_csr AT, gpk_plan p,
+                                                               const gpk_implied_row* __restrict__ rows, double ztol,
                                                                double* cand_l, double* cand_u) {
   const int32_t j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= AT.rows) return;
   const double ol = p.l[j], ou = p.u[j];
   double cl = ol, cu = ou;
-  implied_walk(AT, p, row_nnz, rmin, rmax, rni, rpi, ztol, j, ol, ou, [&](int32_t, double il, double iu) {
+  implied_walk(AT, rows, ztol, j, ol, ou, [&](int32_t, double il, double iu) {
     double nl = ol, nu = ou;
     if (isfinite(il)) nl = fmax(nl, il);
     if (isfinite(iu)) nu = fmin(nu, iu);
@@ -380,9 +386,8 @@ __global__ void k_implied_finalize(int32_t n, gpk_plan p, const double* ref_l, c
 }
 
 // rule_implied_variable_bounds.cu:668-750 plus the per-column tape record count.
-__global__ __launch_bounds__(kBlock) void k_implied_support(gpk_csr AT, gpk_plan p, const int32_t* row_nnz,
-                                                            const double* rmin, const double* rmax,
-                                                            const int32_t* rni, const int32_t* rpi,
+__global__ __launch_bounds__(kBlock) void k_implied_support(gpk_csr AT, gpk_plan p,
+                                                            const gpk_implied_row* __restrict__ rows,
                                                             const uint8_t* lch, const uint8_t* uch, const double* fl,
                                                             const double* fu, double ztol, double tol, double stol,
                                                             int32_t* sup_l, int32_t* sup_u, int32_t* rec_count) {
@@ -393,7 +398,7 @@ __global__ __launch_bounds__(kBlock) void k_implied_support(gpk_csr AT, gpk_plan
   if (need_l || need_u) {
     const double ol = p.l[j], ou = p.u[j];
     const double cl = fl[j], cu = fu[j];
-    implied_walk(AT, p, row_nnz, rmin, rmax, rni, rpi, ztol, j, ol, ou, [&](int32_t i, double il, double iu) {
+    implied_walk(AT, rows, ztol, j, ol, ou, [&](int32_t i, double il, double iu) {
       if (need_l && cl > ol + tol && isfinite(il) && fabs(il - cl) <= stol) sl = min(sl, i);
       if (need_u && cu < ou - tol && isfinite(iu) && fabs(iu - cu) <= stol) su = min(su, i);
     });
@@ -839,10 +844,14 @@ hipError_t gpk_redundant_rows_apply(int32_t m, gpk_plan plan, const uint8_t* ful
                                     const uint8_t* dhi, hipStream_t stream) {
   GPK_LAUNCH(k_redundant_rows_apply, m, m, plan, full, dlo, dhi);
 }
-hipError_t gpk_implied_candidates(gpk_csr AT, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
-                                  const double* rmax, const int32_t* rni, const int32_t* rpi, double zero_tol,
+hipError_t gpk_implied_pack_rows(int32_t m, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
+                                 const double* rmax, const int32_t* rni, const int32_t* rpi, gpk_implied_row* rows,
+                                 hipStream_t stream) {
+  GPK_LAUNCH(k_implied_pack_rows, m, m, plan, row_nnz, rmin, rmax, rni, rpi, rows);
+}
+hipError_t gpk_implied_candidates(gpk_csr AT, gpk_plan plan, const gpk_implied_row* rows, double zero_tol,
                                   double* cand_l, double* cand_u, hipStream_t stream) {
-  GPK_LAUNCH(k_implied_candidates, AT.rows, AT, plan, row_nnz, rmin, rmax, rni, rpi, zero_tol, cand_l, cand_u);
+  GPK_LAUNCH(k_implied_candidates, AT.rows, AT, plan, rows, zero_tol, cand_l, cand_u);
 }
 hipError_t gpk_implied_finalize(int32_t n, gpk_plan plan, const double* ref_l, const double* ref_u,
                                 const double* cand_l, const double* cand_u, const double* vmax,
@@ -852,13 +861,12 @@ hipError_t gpk_implied_finalize(int32_t n, gpk_plan plan, const double* ref_l, c
   GPK_LAUNCH(k_implied_finalize, n, n, plan, ref_l, ref_u, cand_l, cand_u, vmax, preserve_free_zero_cost, zero_tol,
              feas_tol, fin_l, fin_u, lch, uch, fixed, fixed_val, flags);
 }
-hipError_t gpk_implied_support(gpk_csr AT, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
-                               const double* rmax, const int32_t* rni, const int32_t* rpi, const uint8_t* lch,
+hipError_t gpk_implied_support(gpk_csr AT, gpk_plan plan, const gpk_implied_row* rows, const uint8_t* lch,
                                const uint8_t* uch, const double* fin_l, const double* fin_u, double zero_tol,
                                double tol, double support_tol, int32_t* support_l, int32_t* support_u,
                                int32_t* rec_count, hipStream_t stream) {
-  GPK_LAUNCH(k_implied_support, AT.rows, AT, plan, row_nnz, rmin, rmax, rni, rpi, lch, uch, fin_l, fin_u, zero_tol,
-             tol, support_tol, support_l, support_u, rec_count);
+  GPK_LAUNCH(k_implied_support, AT.rows, AT, plan, rows, lch, uch, fin_l, fin_u, zero_tol, tol, support_tol,
+             support_l, support_u, rec_count);
 }
 hipError_t gpk_implied_pack_tape(int32_t n, gpk_plan plan, const uint8_t* lch, const uint8_t* uch,
                                  const int32_t* support_l, const int32_t* support_u, const double* fin_l,

@@ -383,17 +383,28 @@ hipError_t gpk_redundant_rows_classify(gpk_csr A, gpk_plan plan, const int32_t* 
                                        hipStream_t stream);
 hipError_t gpk_redundant_rows_apply(int32_t m, gpk_plan plan, const uint8_t* full, const uint8_t* dlo,
                                     const uint8_t* dhi, hipStream_t stream);
-/* rule_implied_variable_bounds.cu:339-866 */
-hipError_t gpk_implied_candidates(gpk_csr AT, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
-                                  const double* rmax, const int32_t* rni, const int32_t* rpi, double zero_tol,
+/* rule_implied_variable_bounds.cu:339-866.  The walks along AT read, for
+ * every entry, its row's activity summary and bounds: gpk_implied_pack_rows
+ * first packs them into one 64-byte record per row (one cache line per
+ * entry's row instead of eight gathers). */
+typedef struct __attribute__((aligned(64))) gpk_implied_row {
+  double smin, smax; /* finite parts of the row's min and max activity */
+  double lo, hi;     /* the plan's AL, AU */
+  int32_t ninf, pinf; /* infinite terms of the min and max activity */
+  int32_t live;      /* kept and more than one entry */
+  int32_t pad[5];
+} gpk_implied_row;
+hipError_t gpk_implied_pack_rows(int32_t m, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
+                                 const double* rmax, const int32_t* rni, const int32_t* rpi, gpk_implied_row* rows,
+                                 hipStream_t stream);
+hipError_t gpk_implied_candidates(gpk_csr AT, gpk_plan plan, const gpk_implied_row* rows, double zero_tol,
                                   double* cand_l, double* cand_u, hipStream_t stream);
 hipError_t gpk_implied_finalize(int32_t n, gpk_plan plan, const double* ref_l, const double* ref_u,
                                 const double* cand_l, const double* cand_u, const double* vmax,
                                 int32_t preserve_free_zero_cost, double zero_tol, double feas_tol, double* fin_l,
                                 double* fin_u, uint8_t* lch, uint8_t* uch, uint8_t* fixed, double* fixed_val,
                                 int32_t* flags, hipStream_t stream);
-hipError_t gpk_implied_support(gpk_csr AT, gpk_plan plan, const int32_t* row_nnz, const double* rmin,
-                               const double* rmax, const int32_t* rni, const int32_t* rpi, const uint8_t* lch,
+hipError_t gpk_implied_support(gpk_csr AT, gpk_plan plan, const gpk_implied_row* rows, const uint8_t* lch,
                                const uint8_t* uch, const double* fin_l, const double* fin_u, double zero_tol,
                                double tol, double support_tol, int32_t* support_l, int32_t* support_u,
                                int32_t* rec_count, hipStream_t stream);
