@@ -528,4 +528,11 @@ namespace gpu_presolver::presolve {
 // free_csr; new_at->rowPtr stays null when A is not rebuilt.
 void apply_rule_doubleton_equations_with_at(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm,
                                             DeviceCsrMatrix* new_at);
+// apply_rule_infeasible_fixed_variables, with the empty-column rule's
+// detection launched beside it on the same plan and both read in one wait.
+// Returns true when the empty-column rule is settled as well (the fixed-
+// variable rule fixed nothing, so its detection saw the plan it would have
+// seen after it); false when the caller still runs apply_rule_empty_cols.
+bool apply_rule_fixed_variables_and_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp,
+                                               const PresolveStatsGpu& stats, const PresolveParams& prm);
 }  // namespace gpu_presolver::presolve

@@ -741,9 +741,15 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     const LPInfoGpu& v = cur.lp;
     const bool run_redundant =
         redundant_bounds && (prm.enable_redundant_bounds || prm.enable_zero_cost_redundant_box_bounds);
-    if (!time_exceeded() && fixed && prm.enable_infeasible_fixed_variables)
-      prof.run("rule infeasible_fixed_variables", [&] { apply_rule_infeasible_fixed_variables(p, v, prm); });
-    if (!time_exceeded() && empty_cols && prm.enable_empty_cols)
+    bool empty_done = false;
+    if (!time_exceeded() && fixed && prm.enable_infeasible_fixed_variables) {
+      if (empty_cols && prm.enable_empty_cols)
+        prof.run("rule infeasible_fixed_variables",
+                 [&] { empty_done = apply_rule_fixed_variables_and_empty_cols(p, v, st.s, prm); });
+      else
+        prof.run("rule infeasible_fixed_variables", [&] { apply_rule_infeasible_fixed_variables(p, v, prm); });
+    }
+    if (!empty_done && !time_exceeded() && empty_cols && prm.enable_empty_cols)
       prof.run("rule empty_cols", [&] { apply_rule_empty_cols(p, v, st.s, prm); });
     if (!time_exceeded() && cs_dual && prm.enable_column_singletons_dual_infer)
       prof.run("rule column_singletons_dual_infer", [&] { apply_rule_column_singletons_dual_infer(p, v, st.s, prm); });

@@ -365,26 +365,40 @@ void apply_rule_duplicate_rows(PresolvePlanGpu& plan, const LPInfoGpu& lp, const
 }
 
 // rule_infeasible_fixed_variables.cu:117-266
-void apply_rule_infeasible_fixed_variables(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm) {
-  amd::AbandonOnThrow abandon_guard;
-  if (plan.has_infeasible || plan.has_unbounded) return;
+// Detection and decision halves of the fixed-variable and empty-column
+// rules.  A detection launch only writes its own scratch and flag words, so
+// the scheduler can launch both rules' detections and read their flags in
+// one wait when the first rule finds nothing
+// (apply_rule_fixed_variables_and_empty_cols).
+namespace {
+
+struct ColFixScratch {
+  DBuf<uint8_t> fixed;
+  DBuf<double> v, contrib;
+  explicit ColFixScratch(int32_t n) : fixed(n), v(n), contrib(n) {}
+};
+
+void fixed_vars_launch(const PresolvePlanGpu& plan, int32_t n, const PresolveParams& prm, ColFixScratch& w,
+                       int32_t* f) {
+  check(gpk_fixed_vars_candidates(n, plan_view(plan), prm.bound_tol, w.fixed, w.v, w.contrib, f, ctx().stream()),
+        "fixed vars candidates");
+}
+
+// h[0]: any fixed column.
+void fixed_vars_finish(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm, ColFixScratch& w,
+                       const int32_t* h) {
+  if (h[0] == 0) return;
   const int32_t n = lp.A.cols, m = lp.A.rows;
-  if (n == 0) return;
   auto& c = ctx();
   hipStream_t s = c.stream();
   const gpk_plan P = plan_view(plan);
-  DBuf<uint8_t> fixed(n);
-  DBuf<double> v(n), contrib(n);
-  int32_t* f = c.flags();
-  check(gpk_fixed_vars_candidates(n, P, prm.bound_tol, fixed, v, contrib, f, s), "fixed vars candidates");
-  if (c.read_flags(1)[0] == 0) return;
   if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
-    const std::vector<int32_t> cols = masked_indices(fixed, n);
+    const std::vector<int32_t> cols = masked_indices(w.fixed, n);
     std::vector<double> hv, hc, av;
     std::vector<uint8_t> kr;
     std::vector<int32_t> ap, ai;
     c.begin_copy_group();
-    c.to_host_async(hv, v.get(), n, "fixed vars tape v");
+    c.to_host_async(hv, w.v.get(), n, "fixed vars tape v");
     c.to_host_async(hc, plan.new_c, n, "fixed vars tape c");
     c.to_host_async(kr, plan.keep_row_mask, m, "fixed vars tape keep_row");
     c.to_host_async(ap, lp.AT.rowPtr, static_cast<int64_t>(n) + 1, "fixed vars tape AT ptr");
@@ -403,29 +417,24 @@ void apply_rule_infeasible_fixed_variables(PresolvePlanGpu& plan, const LPInfoGp
     }
   }
   // Row shift over every row (the reference does not test keep_row here).
-  check(gpk_row_shift(csr_view(lp.A), nullptr, fixed, v, -1.0, plan.new_AL, plan.new_AU, s), "fixed vars row shift");
-  check(gpk_fix_cols_apply(n, P, fixed, v, s), "fixed vars apply");
-  plan.obj_constant_delta += amd::ordered_masked_sum(fixed, contrib, n);
+  check(gpk_row_shift(csr_view(lp.A), nullptr, w.fixed, w.v, -1.0, plan.new_AL, plan.new_AU, s),
+        "fixed vars row shift");
+  check(gpk_fix_cols_apply(n, P, w.fixed, w.v, s), "fixed vars apply");
+  plan.obj_constant_delta += amd::ordered_masked_sum(w.fixed, w.contrib, n);
   plan.has_col_action = true;
   plan.has_change = true;
 }
 
-// rule_empty_cols.cu:110-198
-void apply_rule_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
-                           const PresolveParams& prm) {
-  amd::AbandonOnThrow abandon_guard;
-  if (plan.has_infeasible || plan.has_unbounded) return;
-  const int32_t n = lp.A.cols;
-  if (n == 0) return;
-  auto& c = ctx();
-  hipStream_t s = c.stream();
-  const gpk_plan P = plan_view(plan);
-  DBuf<uint8_t> fixed(n);
-  DBuf<double> v(n), contrib(n);
-  int32_t* f = c.flags();
-  check(gpk_empty_cols(n, P, stats.empty_col_mask, prm.bound_tol, prm.zero_tol, fixed, v, contrib, f, s),
+void empty_cols_launch(const PresolvePlanGpu& plan, int32_t n, const PresolveStatsGpu& stats,
+                       const PresolveParams& prm, ColFixScratch& w, int32_t* f) {
+  check(gpk_empty_cols(n, plan_view(plan), stats.empty_col_mask, prm.bound_tol, prm.zero_tol, w.fixed, w.v,
+                       w.contrib, f, ctx().stream()),
         "empty cols");
-  const int32_t* h = c.read_flags(3);
+}
+
+// h[0] infeasible, h[1] unbounded, h[2] any fixed column.
+void empty_cols_finish(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm, ColFixScratch& w,
+                       const int32_t* h) {
   if (h[0]) {
     plan.has_infeasible = true;
     return;
@@ -435,19 +444,70 @@ void apply_rule_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Pre
     return;
   }
   if (!h[2]) return;
+  const int32_t n = lp.A.cols;
+  auto& c = ctx();
   if (prm.record_postsolve_tape && prm.record_postsolve_tape_cpu) {
-    const std::vector<int32_t> cols = masked_indices(fixed, n);
+    const std::vector<int32_t> cols = masked_indices(w.fixed, n);
     std::vector<double> hv, hc;
     c.begin_copy_group();
-    c.to_host_async(hv, v.get(), n, "empty cols tape v");
+    c.to_host_async(hv, w.v.get(), n, "empty cols tape v");
     c.to_host_async(hc, plan.new_c, n, "empty cols tape c");
     c.sync("empty cols tape");
     for (int32_t j : cols) append_record(plan.tape, RT::FixedCol, {j}, {hv[j], hc[j]}, DM::Minimal);
   }
-  check(gpk_fix_cols_apply(n, P, fixed, v, s), "empty cols apply");
-  plan.obj_constant_delta += amd::ordered_masked_sum(fixed, contrib, n);
+  check(gpk_fix_cols_apply(n, plan_view(plan), w.fixed, w.v, c.stream()), "empty cols apply");
+  plan.obj_constant_delta += amd::ordered_masked_sum(w.fixed, w.contrib, n);
   plan.has_col_action = true;
   plan.has_change = true;
+}
+
+}  // namespace
+
+void apply_rule_infeasible_fixed_variables(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t n = lp.A.cols;
+  if (n == 0) return;
+  ColFixScratch w(n);
+  auto& c = ctx();
+  int32_t* f = c.flags();
+  fixed_vars_launch(plan, n, prm, w, f);
+  fixed_vars_finish(plan, lp, prm, w, c.read_flags(1));
+}
+
+// rule_empty_cols.cu:110-198
+void apply_rule_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp, const PresolveStatsGpu& stats,
+                           const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
+  if (plan.has_infeasible || plan.has_unbounded) return;
+  const int32_t n = lp.A.cols;
+  if (n == 0) return;
+  ColFixScratch w(n);
+  auto& c = ctx();
+  int32_t* f = c.flags();
+  empty_cols_launch(plan, n, stats, prm, w, f);
+  empty_cols_finish(plan, lp, prm, w, c.read_flags(3));
+}
+
+bool apply_rule_fixed_variables_and_empty_cols(PresolvePlanGpu& plan, const LPInfoGpu& lp,
+                                               const PresolveStatsGpu& stats, const PresolveParams& prm) {
+  amd::AbandonOnThrow abandon_guard;
+  if (plan.has_infeasible || plan.has_unbounded) return true;
+  const int32_t n = lp.A.cols;
+  if (n == 0) return true;
+  ColFixScratch wf(n), we(n);
+  auto& c = ctx();
+  int32_t* f = c.flags();
+  fixed_vars_launch(plan, n, prm, wf, f);
+  empty_cols_launch(plan, n, stats, prm, we, f + 4);
+  const int32_t* h = c.read_flags(7);
+  const int32_t hf[1] = {h[0]}, he[3] = {h[4], h[5], h[6]};
+  if (hf[0] != 0) {
+    fixed_vars_finish(plan, lp, prm, wf, hf);
+    return false;
+  }
+  empty_cols_finish(plan, lp, prm, we, he);
+  return true;
 }
 
 // rule_column_singletons.cu:1631-1928
