@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstring>
 #include <exception>
+#include <functional>
 #include <initializer_list>
 #include <memory>
 #include <stdexcept>
@@ -144,6 +145,15 @@ class Ctx {
   // read_flags, scalar_to_host, mail) also ends the group, its copies first.
   void begin_copy_group() { grouping_ = true; }
   void end_copy_group(const char* context);
+  // Host work run in the order given once the downloads of the next wait
+  // have landed, before that wait returns.  A callback may copy from the
+  // (then idle) stream synchronously but must not wait through this context.
+  void after_wait(std::function<void()> fn) { after_.push_back(std::move(fn)); }
+  bool has_after_wait() const { return !after_.empty(); }
+  // Set while the presolve scheduler runs: rules then leave their objective
+  // shifts to the next wait (add_ordered_masked_sum).  The scheduler reads
+  // a plan's shift only after a wait.
+  bool defer_objective_sums = false;
   // Drops every pending download without copying it (its destination may
   // already be gone), waits for the stream so the staging area is free, and
   // clears the spin-limit error word.  Called when an exception leaves a
@@ -178,6 +188,7 @@ class Ctx {
   std::vector<Download> downloads_;
   bool grouping_ = false;
   std::vector<gpk_copy_seg> pending_;  // copies of the open group
+  std::vector<std::function<void()>> after_;
   void enqueue_copy(const gpk_copy_seg& seg, const char* context);
   void flush_pending(const char* context);
   void check_error_flag(const char* context);
@@ -420,6 +431,11 @@ int32_t scan_counts(const int32_t* in, int32_t n, int32_t* out);
 // Sum, in ascending index order, of vals[i] over mask[i] != 0 (computed on
 // the host from the packed selection, so the rounding order is fixed).
 double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n);
+// *target += ordered_masked_sum(mask, vals, n).  With the context's
+// defer_objective_sums set, the terms are packed and downloaded without a
+// wait and the addition runs at the next wait (Ctx::after_wait), in the
+// order of the calls; the mask and values may be released once this returns.
+void add_ordered_masked_sum(double* target, const uint8_t* mask, const double* vals, int32_t n);
 // Compacts A by row/column keep maps; the result owns fresh device arrays.
 // known_nnz >= 0 is the kept count when the caller knows it (the transpose
 // of a matrix compacted with the same maps): the read-back is skipped.

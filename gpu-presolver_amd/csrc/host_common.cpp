@@ -210,6 +210,10 @@ void Ctx::finish_downloads() {
   for (const Download& d : downloads_) std::memcpy(d.dst, h_stage_ + d.offset, d.bytes);
   downloads_.clear();
   stage_used_ = 0;
+  if (after_.empty()) return;
+  std::vector<std::function<void()>> run;
+  run.swap(after_);
+  for (auto& fn : run) fn();
 }
 
 const uint32_t* Ctx::mail(const void* dev, int words, const char* context) {
@@ -253,6 +257,7 @@ void Ctx::abandon() noexcept {
   grouping_ = false;
   pending_.clear();
   downloads_.clear();
+  after_.clear();
   stage_used_ = 0;
   if (h_error_) *h_error_ = 0;
 }
@@ -460,6 +465,46 @@ double ordered_masked_sum(const uint8_t* mask, const double* vals, int32_t n) {
   double sum = 0.0;
   for (int32_t t = 0; t < k; ++t) sum += h[static_cast<size_t>(t)];
   return sum;
+}
+
+void add_ordered_masked_sum(double* target, const uint8_t* mask, const double* vals, int32_t n) {
+  Ctx& c = ctx();
+  if (!c.defer_objective_sums || n <= 0) {
+    *target += ordered_masked_sum(mask, vals, n);
+    return;
+  }
+  hipStream_t s = c.stream();
+  // The packed terms and the count stay allocated until the addition has run.
+  struct Terms {
+    DBuf<int32_t> prefix;
+    DBuf<double> packed;
+    std::vector<double> h;
+    int32_t k = 0;
+  };
+  auto t = std::make_shared<Terms>();
+  t->prefix.alloc(static_cast<int64_t>(n) + 1);
+  t->packed.alloc(n);
+  check(gpk_scan_lb_mask(mask, t->prefix, n, c.lookback(gpk_lookback_tiles(n)), s), "scan objective mask");
+  check(gpk_pack_f64(mask, vals, t->prefix, n, t->packed, s), "pack objective terms");
+  const int32_t head = std::min<int32_t>(n, kSpeculativeTerms);
+  t->h.resize(static_cast<size_t>(head));
+  c.begin_copy_group();
+  c.download(t->h.data(), t->packed.get(), sizeof(double) * static_cast<size_t>(head), "objective terms");
+  c.download(&t->k, t->prefix.get() + n, sizeof(int32_t), "objective term count");
+  c.end_copy_group("objective terms");
+  c.after_wait([t, target, head, s] {
+    const int32_t k = t->k;
+    if (k > head) {  // the stream is idle: the rest comes over directly
+      t->h.resize(static_cast<size_t>(k));
+      check(hipMemcpyAsync(t->h.data() + head, t->packed.get() + head, sizeof(double) * static_cast<size_t>(k - head),
+                           hipMemcpyDeviceToHost, s),
+            "objective terms");
+      check(hipStreamSynchronize(s), "objective terms");
+    }
+    double sum = 0.0;
+    for (int32_t q = 0; q < k; ++q) sum += t->h[static_cast<size_t>(q)];
+    *target += sum;
+  });
 }
 
 int32_t build_old_to_new(const uint8_t* mask, int32_t n, DBuf<int32_t>& map) {

@@ -631,8 +631,25 @@ class StepProfile {
 // enable_structure_specific_rules they are screened where the reference runs
 // them, and an LP on which one would act is refused (UnsupportedStructure);
 // on every other LP they do nothing, as in the reference.
+// While the scheduler runs, rules leave their objective shifts to the next
+// wait (amd::add_ordered_masked_sum).  A plan's shift is read only after a
+// wait (the one that applies it); settle() covers a plan discarded without
+// one.
+struct DeferObjectiveSums {
+  amd::Ctx& c;
+  bool prev;
+  DeferObjectiveSums() : c(amd::ctx()), prev(c.defer_objective_sums) { c.defer_objective_sums = true; }
+  ~DeferObjectiveSums() { c.defer_objective_sums = prev; }
+  DeferObjectiveSums(const DeferObjectiveSums&) = delete;
+  DeferObjectiveSums& operator=(const DeferObjectiveSums&) = delete;
+  void settle() {
+    if (c.has_after_wait()) c.sync("objective shifts");
+  }
+};
+
 GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool keep_reduced) {
   amd::AbandonOnThrow abandon_guard;
+  DeferObjectiveSums defer_sums;
   validate_lp(lp);
   if (amd::job_mode().device_sync_on_entry) check(hipDeviceSynchronize(), "presolve entry synchronize");
   amd::check_lp_arrays(lp, "presolve input");
@@ -707,6 +724,7 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
     }
     amd::free_csr(dbl_at.m);
     dbl_at.m = DeviceCsrMatrix{};
+    if (!changed) defer_sums.settle();
     kept_valid = !changed && !terminal && plan_pristine(p);
     return changed;
   };
@@ -890,6 +908,7 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
       apply_plan(cur, p, rs);
       invalidate_clean();
     }
+    defer_sums.settle();  // before the closure's plan goes
     return any;
   };
   // run_fast_phase (gpu_presolve.cu:2833-2863)
@@ -1090,6 +1109,7 @@ GpuPresolveSummary run_impl(const LPInfoGpu& lp, const PresolveParams& prm, bool
   res.reduced_rows = cur.lp.A.rows;
   res.reduced_cols = cur.lp.A.cols;
   res.reduced_nnz = cur.lp.A.nnz;
+  defer_sums.settle();
   res.obj_constant_delta = cur.lp.obj_constant - lp.obj_constant;
   rec.m1 = res.reduced_rows;
   rec.n1 = res.reduced_cols;

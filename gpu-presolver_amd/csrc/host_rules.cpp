@@ -420,7 +420,7 @@ void fixed_vars_finish(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Presolv
   check(gpk_row_shift(csr_view(lp.A), nullptr, w.fixed, w.v, -1.0, plan.new_AL, plan.new_AU, s),
         "fixed vars row shift");
   check(gpk_fix_cols_apply(n, P, w.fixed, w.v, s), "fixed vars apply");
-  plan.obj_constant_delta += amd::ordered_masked_sum(w.fixed, w.contrib, n);
+  amd::add_ordered_masked_sum(&plan.obj_constant_delta, w.fixed, w.contrib, n);
   plan.has_col_action = true;
   plan.has_change = true;
 }
@@ -456,7 +456,7 @@ void empty_cols_finish(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Presolv
     for (int32_t j : cols) append_record(plan.tape, RT::FixedCol, {j}, {hv[j], hc[j]}, DM::Minimal);
   }
   check(gpk_fix_cols_apply(n, plan_view(plan), w.fixed, w.v, c.stream()), "empty cols apply");
-  plan.obj_constant_delta += amd::ordered_masked_sum(w.fixed, w.contrib, n);
+  amd::add_ordered_masked_sum(&plan.obj_constant_delta, w.fixed, w.contrib, n);
   plan.has_col_action = true;
   plan.has_change = true;
 }
@@ -538,14 +538,13 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
   check(gpk_cs_activity_cache(A, P, needed, rc, s), "column singletons activity");
   DBuf<int32_t> owner(m);
   check(gpk_fill_i32(owner, INT_MAX, m, s), "column singletons owner fill");
+  // The owner pass's unbounded flag (word 0) and the process pass's three
+  // flags (words 4-6) come back in one wait: the process pass writes only
+  // this call's buffers, which an unbounded owner pass discards.
   f = c.flags();
   check(gpk_cs_owner(A, P, stats.column_singleton_mask, stats.column_singleton_row, stats.column_singleton_val, rc,
                      prm.bound_tol, prm.zero_tol, eq_mode ? 1 : 0, owner, f, s),
         "column singletons owner");
-  if (c.read_flags(1)[0]) {
-    plan.has_unbounded = true;
-    return;
-  }
   DBuf<double> AL_new(m), AU_new(m), side(n), contrib(n), c_new(n);
   DBuf<uint8_t> row_del(m), row_changed(m), col_del(n);
   amd::CopyBatch()
@@ -557,17 +556,16 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
       .copy(AL_new.get(), plan.new_AL, m)
       .copy(AU_new.get(), plan.new_AU, m)
       .run("column singletons workspace");
-  f = c.flags();
   check(gpk_cs_process(n, m, P, stats.column_singleton_mask, stats.column_singleton_row, stats.column_singleton_val,
                        owner, rc, prm.bound_tol, prm.zero_tol, eq_mode ? 1 : 0, AL_new, AU_new, row_del, row_changed,
-                       col_del, side, contrib, f, s),
+                       col_del, side, contrib, f + 4, s),
         "column singletons process");
-  const int32_t* h = c.read_flags(3);
-  if (h[0]) {
+  const int32_t* h = c.read_flags(7);
+  if (h[0] || h[4]) {
     plan.has_unbounded = true;
     return;
   }
-  const bool st_eq = h[1] != 0, st_dual = h[2] != 0;
+  const bool st_eq = h[5] != 0, st_dual = h[6] != 0;
   if (!(st_eq || st_dual)) return;
   if (st_eq && eq_mode)
     check(gpk_cs_objective(AT, P, col_del, owner, stats.column_singleton_val, c_new, s), "column singletons objective");
@@ -621,7 +619,7 @@ static void column_singletons_mode(PresolvePlanGpu& plan, const LPInfoGpu& lp, c
     for (int32_t row : rows) append_record(plan.tape, RT::EqToIneq, {row}, {0.0}, DM::Minimal);
   }
   check(gpk_cs_apply(m, n, P, row_del, col_del, AL_new, AU_new, c_new, s), "column singletons apply");
-  plan.obj_constant_delta += amd::ordered_masked_sum(col_del, contrib, n);
+  amd::add_ordered_masked_sum(&plan.obj_constant_delta, col_del, contrib, n);
   plan.has_row_action = true;
   plan.has_col_action = plan.has_col_action || st_eq;
   plan.has_change = true;
@@ -1065,7 +1063,7 @@ void apply_rule_dual_fix(PresolvePlanGpu& plan, const LPInfoGpu& lp, const Preso
     }
   }
   check(gpk_dual_fix_apply(m, n, P, fixed, v, shift, row_remove, s), "dual fix apply");
-  plan.obj_constant_delta += amd::ordered_masked_sum(fixed, contrib, n);
+  amd::add_ordered_masked_sum(&plan.obj_constant_delta, fixed, contrib, n);
   plan.has_row_action = plan.has_row_action || row_changed;
   plan.has_col_action = true;
   plan.has_change = true;
@@ -1142,7 +1140,7 @@ void apply_rule_duplicate_columns(PresolvePlanGpu& plan, const LPInfoGpu& lp, co
     plan.has_row_action = true;
   }
   check(gpk_dup_col_apply(n, P, del, fixed, fixed_val, contrib, s), "duplicate cols apply");
-  plan.obj_constant_delta += amd::ordered_masked_sum(fixed, contrib, n);
+  amd::add_ordered_masked_sum(&plan.obj_constant_delta, fixed, contrib, n);
   plan.has_col_action = true;
   plan.has_change = true;
 }
