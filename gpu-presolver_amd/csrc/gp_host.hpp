@@ -13,6 +13,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "gp_kernels.h"
@@ -460,6 +461,27 @@ struct CompactStart {
 CompactStart compact_start(const presolve::DeviceCsrMatrix& A, const int32_t* row_map, const gpk_rank_word* col_rank,
                            int32_t* kept);
 presolve::DeviceCsrMatrix compact_finish(CompactStart& st, int32_t m_new, int32_t n_new, int32_t nnz_new);
+// The two halves for A and a transpose of it with sorted rows, compacted by
+// the same maps (AT's rows by col_map, its columns by the row rank words):
+// compact_start_pair launches one map pass over both matrices' tiles and, past
+// 256 tiles on either side, one scan of both sides' tile counts;
+// compact_finish_pair writes both in one launch.  `kept` receives A's kept
+// count as in compact_start.
+struct CompactPairStart {
+  const presolve::DeviceCsrMatrix* A = nullptr;
+  const presolve::DeviceCsrMatrix* AT = nullptr;
+  const int32_t* row_map = nullptr;
+  const int32_t* col_map = nullptr;
+  DBuf<int32_t> ptr, t_ptr, jn, t_jn, tile_off;
+  bool from_counts = false;  // tile_off holds raw counts (no scan launched)
+};
+CompactPairStart compact_start_pair(const presolve::DeviceCsrMatrix& A, const int32_t* row_map,
+                                    const gpk_rank_word* col_rank, const presolve::DeviceCsrMatrix& AT,
+                                    const int32_t* col_map, const gpk_rank_word* row_rank, int32_t* kept);
+// {compacted A, compacted AT}; both hold nnz_new entries.
+std::pair<presolve::DeviceCsrMatrix, presolve::DeviceCsrMatrix> compact_finish_pair(CompactPairStart& st,
+                                                                                     int32_t m_new, int32_t n_new,
+                                                                                     int32_t nnz_new);
 // Stable transpose (rows of the result sorted); fresh device arrays.
 presolve::DeviceCsrMatrix transpose_csr(const presolve::DeviceCsrMatrix& A);
 void free_csr(presolve::DeviceCsrMatrix& A) noexcept;

@@ -632,6 +632,85 @@ DeviceCsrMatrix compact_finish(CompactStart& st, int32_t m_new, int32_t n_new, i
   return out;
 }
 
+namespace {
+gpk_compact_side compact_side(const DeviceCsrMatrix& M, const int32_t* row_map, const gpk_rank_word* col_rank,
+                              int32_t* jn, int32_t* out_ptr) {
+  return gpk_compact_side{M.rowPtr, M.colVal, M.nzVal, M.rows, 0, row_map, col_rank, jn, out_ptr, nullptr, nullptr};
+}
+}  // namespace
+
+CompactPairStart compact_start_pair(const DeviceCsrMatrix& A, const int32_t* row_map, const gpk_rank_word* col_rank,
+                                    const DeviceCsrMatrix& AT, const int32_t* col_map, const gpk_rank_word* row_rank,
+                                    int32_t* kept) {
+  Ctx& c = ctx();
+  hipStream_t s = c.stream();
+  if (AT.rows != A.cols || AT.cols != A.rows || AT.nnz != A.nnz)
+    throw std::logic_error("compact_start_pair: AT is not the transpose shape of A");
+  CompactPairStart st;
+  st.A = &A;
+  st.AT = &AT;
+  st.row_map = row_map;
+  st.col_map = col_map;
+  st.ptr.alloc(static_cast<int64_t>(A.rows) + 1, "compact rowPtr");
+  st.t_ptr.alloc(static_cast<int64_t>(AT.rows) + 1, "compact AT rowPtr");
+  if (A.rows == 0 || AT.rows == 0 || A.nnz == 0) return st;  // *kept stays as the caller zeroed it
+  if (col_rank == nullptr || row_rank == nullptr)
+    throw std::logic_error("compact_start_pair: both sides' rank words are required");
+  const int32_t ta = gpk_compact_tiles(A.rows), tt = gpk_compact_tiles(AT.rows);
+  st.jn.alloc(A.nnz, "compact entry map");
+  st.t_jn.alloc(AT.nnz, "compact AT entry map");
+  st.tile_off.alloc(static_cast<int64_t>(ta) + tt + 1, "compact tile offsets");
+  check(gpk_compact_map_pair(compact_side(A, row_map, col_rank, st.jn, st.ptr),
+                             compact_side(AT, col_map, row_rank, st.t_jn, st.t_ptr), st.tile_off, kept, s),
+        "compact map pair");
+  st.from_counts = ta <= kCompactFewTiles && tt <= kCompactFewTiles;
+  if (!st.from_counts)
+    check(gpk_scan_lb_i32(st.tile_off, st.tile_off, ta + tt, c.lookback(gpk_lookback_tiles(ta + tt)), s),
+          "compact tile offsets");
+  return st;
+}
+
+std::pair<DeviceCsrMatrix, DeviceCsrMatrix> compact_finish_pair(CompactPairStart& st, int32_t m_new, int32_t n_new,
+                                                                int32_t nnz_new) {
+  Ctx& c = ctx();
+  hipStream_t s = c.stream();
+  const DeviceCsrMatrix& A = *st.A;
+  const DeviceCsrMatrix& AT = *st.AT;
+  DeviceCsrMatrix a_out, t_out;
+  a_out.rows = m_new;
+  a_out.cols = n_new;
+  t_out.rows = n_new;
+  t_out.cols = m_new;
+  if (m_new == 0 || n_new == 0 || A.nnz == 0 || st.jn.get() == nullptr) {
+    CopyBatch()
+        .zero(st.ptr.get(), static_cast<int64_t>(m_new) + 1)
+        .zero(st.t_ptr.get(), static_cast<int64_t>(n_new) + 1)
+        .run("compact empty rowPtr");
+    a_out.rowPtr = st.ptr.release();
+    t_out.rowPtr = st.t_ptr.release();
+    return {a_out, t_out};
+  }
+  DBuf<int32_t> idx(A.nnz, "compact colVal"), t_idx(AT.nnz, "compact AT colVal");
+  DBuf<double> val(A.nnz, "compact nzVal"), t_val(AT.nnz, "compact AT nzVal");
+  gpk_compact_side a = compact_side(A, st.row_map, nullptr, st.jn, st.ptr);
+  gpk_compact_side t = compact_side(AT, st.col_map, nullptr, st.t_jn, st.t_ptr);
+  a.m_new = m_new;
+  a.out_idx = idx;
+  a.out_val = val;
+  t.m_new = n_new;
+  t.out_idx = t_idx;
+  t.out_val = t_val;
+  check(gpk_compact_write_pair(a, t, st.tile_off, st.from_counts ? 1 : 0, s), "compact write pair");
+  a_out.nnz = t_out.nnz = nnz_new;
+  a_out.rowPtr = st.ptr.release();
+  a_out.colVal = idx.release();
+  a_out.nzVal = val.release();
+  t_out.rowPtr = st.t_ptr.release();
+  t_out.colVal = t_idx.release();
+  t_out.nzVal = t_val.release();
+  return {a_out, t_out};
+}
+
 // Stable transpose: LSD radix sort of the column indices (entries already in
 // row-major order, so every AT row comes out sorted by row index) carrying
 // the entry positions; the row pointers come from the sorted keys (no

@@ -20,6 +20,7 @@
 #include <optional>
 #include <type_traits>
 #include <string>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -474,8 +475,17 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs, const De
   // count comes back in the same wait as the three counts.
   const DeviceCsrMatrix& src = p.has_new_A ? p.new_A : cur.lp.A;
   const bool early = p.has_new_A || p.has_row_action || p.has_col_action;
+  // When the AT to compact is known before the wait (the working AT, or the
+  // rebuilding rule's transpose of its new A), both matrices' map passes go
+  // out in one launch now and both write passes in one launch after it.
+  const DeviceCsrMatrix* at_early = !early ? nullptr : p.has_new_A ? new_at : cur.owns ? &cur.lp.AT : nullptr;
   amd::CompactStart a_start;
-  if (early) a_start = amd::compact_start(src, row_map, col_rank.get(), counts + 4);
+  amd::CompactPairStart pair_start;
+  if (at_early)
+    pair_start =
+        amd::compact_start_pair(src, row_map, col_rank.get(), *at_early, col_map, row_rank.get(), counts + 4);
+  else if (early)
+    a_start = amd::compact_start(src, row_map, col_rank.get(), counts + 4);
   const int32_t* h = c.read_flags(early ? 5 : 4);
   const bool callers_at_sorted = callers_at && h[3] == 0;
   const int32_t m_new = h[0], n_new = h[1];
@@ -495,18 +505,22 @@ void apply_plan(WorkLp& cur, const PresolvePlanGpu& p, RecordState& rs, const De
     return;
   }
   LPInfoGpu next;
-  next.A = early ? amd::compact_finish(a_start, m_new, n_new, a_nnz)
-                 : amd::compact_csr(src, row_map, col_map, m_new, n_new, -1, col_rank.get());
-  // An AT with sorted rows that is the exact transpose of the A being
-  // compacted (the working AT, the caller's AT when sorted, the rebuilding
-  // rule's AT of a new A) is compacted with the maps swapped, which gives
-  // the transpose of the compacted A; otherwise the compacted A is
-  // re-transposed.
-  const DeviceCsrMatrix* at_src = p.has_new_A ? new_at : (cur.owns || callers_at_sorted ? &cur.lp.AT : nullptr);
-  if (at_src != nullptr) {
-    next.AT = amd::compact_csr(*at_src, col_map, row_map, n_new, m_new, next.A.nnz, row_rank.get());
+  if (at_early) {
+    std::tie(next.A, next.AT) = amd::compact_finish_pair(pair_start, m_new, n_new, a_nnz);
   } else {
-    next.AT = amd::transpose_csr(next.A);
+    next.A = early ? amd::compact_finish(a_start, m_new, n_new, a_nnz)
+                   : amd::compact_csr(src, row_map, col_map, m_new, n_new, -1, col_rank.get());
+    // An AT with sorted rows that is the exact transpose of the A being
+    // compacted (the working AT, the caller's AT when sorted, the rebuilding
+    // rule's AT of a new A) is compacted with the maps swapped, which gives
+    // the transpose of the compacted A; otherwise the compacted A is
+    // re-transposed.
+    const DeviceCsrMatrix* at_src = p.has_new_A ? new_at : (cur.owns || callers_at_sorted ? &cur.lp.AT : nullptr);
+    if (at_src != nullptr) {
+      next.AT = amd::compact_csr(*at_src, col_map, row_map, n_new, m_new, next.A.nnz, row_rank.get());
+    } else {
+      next.AT = amd::transpose_csr(next.A);
+    }
   }
   next.c = pool_f64(n_new, "working c");
   next.l = pool_f64(n_new, "working l");

@@ -557,19 +557,19 @@ __device__ inline void comp_count_run(int row, bool valid, bool kept, int32_t* s
   if (c) atomicAdd(&s_cnt[row], c);
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restrict__ ptr,
-                                                        const int32_t* __restrict__ idx, int32_t m_old,
-                                                        const int32_t* __restrict__ row_map,
-                                                        const gpk_rank_word* __restrict__ col_rank,
-                                                        int32_t* __restrict__ jn_out, int32_t* __restrict__ out_ptr,
-                                                        int32_t* __restrict__ tile_cnt,
-                                                        int32_t* __restrict__ kept_total) {
+// The map pass of one 256-row tile (tile_cnt[tile]: its kept count).
+__device__ __forceinline__ void compact_map_tile(int32_t tile, const int32_t* __restrict__ ptr,
+                                                 const int32_t* __restrict__ idx, int32_t m_old,
+                                                 const int32_t* __restrict__ row_map,
+                                                 const gpk_rank_word* __restrict__ col_rank,
+                                                 int32_t* __restrict__ jn_out, int32_t* __restrict__ out_ptr,
+                                                 int32_t* __restrict__ tile_cnt, int32_t* __restrict__ kept_total) {
   __shared__ int32_t s_cnt[kCompRows];
   __shared__ int32_t s_wsum[kBlock / 64];
   __shared__ uint8_t s_keep[kCompRows];
   __shared__ uint8_t s_row[kCompChunk];  // tile-local row of each entry of the chunk
   static_assert(kCompRows <= 256, "tile-local rows are stored in one byte");
-  const int32_t r0 = blockIdx.x * kCompRows;
+  const int32_t r0 = tile * kCompRows;
   const int rows = min(kCompRows, m_old - r0);
   const int tid = threadIdx.x;
   const int32_t my_new_row = tid < rows ? row_map[r0 + tid] : -1;
@@ -617,38 +617,64 @@ __global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restric
   if (tid == 0) {
     int32_t t = 0;
     for (int w = 0; w < kBlock / 64; ++w) t += s_wsum[w];
-    tile_cnt[blockIdx.x] = t;
+    tile_cnt[tile] = t;
     if (kept_total != nullptr && t != 0) atomicAdd(kept_total, t);  // integer: order-free
   }
 }
 
-template <bool kOffsetsFromCounts>
-__global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restrict__ ptr,
-                                                          const double* __restrict__ val, int32_t m_old,
-                                                          const int32_t* __restrict__ row_map,
-                                                          const int32_t* __restrict__ jn_in,
-                                                          const int32_t* __restrict__ tile_off,
-                                                          int32_t* __restrict__ out_ptr, int32_t m_new,
-                                                          int32_t* __restrict__ out_idx,
-                                                          double* __restrict__ out_val) {
+__global__ __launch_bounds__(kBlock) void k_compact_map(const int32_t* __restrict__ ptr,
+                                                        const int32_t* __restrict__ idx, int32_t m_old,
+                                                        const int32_t* __restrict__ row_map,
+                                                        const gpk_rank_word* __restrict__ col_rank,
+                                                        int32_t* __restrict__ jn_out, int32_t* __restrict__ out_ptr,
+                                                        int32_t* __restrict__ tile_cnt,
+                                                        int32_t* __restrict__ kept_total) {
+  compact_map_tile(static_cast<int32_t>(blockIdx.x), ptr, idx, m_old, row_map, col_rank, jn_out, out_ptr, tile_cnt,
+                   kept_total);
+}
+
+// A and its transpose compacted with the same maps in one launch: blocks
+// [0, tiles_a) map A's tiles, the rest AT's; the tile counts go to one array
+// (A's, then AT's), so one scan serves both.
+__global__ __launch_bounds__(kBlock) void k_compact_map_pair(gpk_compact_side a, gpk_compact_side t,
+                                                             int32_t tiles_a, int32_t* __restrict__ tile_cnt,
+                                                             int32_t* __restrict__ kept_total) {
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  if (b < tiles_a)
+    compact_map_tile(b, a.ptr, a.idx, a.m_old, a.row_map, a.col_rank, a.jn, a.out_ptr, tile_cnt, kept_total);
+  else
+    compact_map_tile(b - tiles_a, t.ptr, t.idx, t.m_old, t.row_map, t.col_rank, t.jn, t.out_ptr,
+                     tile_cnt + tiles_a, nullptr);
+}
+
+// The write pass of tile `tile` of `ntiles`.  seg: the tiles' raw kept
+// counts (from_counts; ntiles <= kBlock) or an exclusive scan of them whose
+// first word is subtracted (a segment of a longer scan).
+__device__ __forceinline__ void compact_write_tile(int32_t tile, int32_t ntiles, bool from_counts,
+                                                   const int32_t* __restrict__ ptr, const double* __restrict__ val,
+                                                   int32_t m_old, const int32_t* __restrict__ row_map,
+                                                   const int32_t* __restrict__ jn_in,
+                                                   const int32_t* __restrict__ seg, int32_t* __restrict__ out_ptr,
+                                                   int32_t m_new, int32_t* __restrict__ out_idx,
+                                                   double* __restrict__ out_val) {
   constexpr int kWaves = kBlock / 64;
   __shared__ int32_t s_oidx[kCompChunk];
   __shared__ double s_oval[kCompChunk];
   __shared__ int32_t s_wcnt[kCompItems * kWaves];
   __shared__ int32_t s_wsum[kWaves];
   __shared__ int32_t s_total;
-  const int32_t r0 = blockIdx.x * kCompRows;
+  const int32_t r0 = tile * kCompRows;
   const int rows = min(kCompRows, m_old - r0);
   const int tid = threadIdx.x;
   const int lane = lane_id(), wave = tid >> 6;
   const int32_t my_new_row = tid < rows ? row_map[r0 + tid] : -1;
   int32_t base;
-  if (kOffsetsFromCounts) {
-    // Few tiles (gridDim.x <= kBlock): tile_off holds the raw per-tile kept
-    // counts and each block sums those of the tiles before it (and the last
-    // block all of them) instead of waiting for a scan launch.
-    const int32_t t = tid < static_cast<int>(gridDim.x) ? tile_off[tid] : 0;
-    const int32_t before = wave_inclusive_sum(tid < static_cast<int>(blockIdx.x) ? t : 0);
+  if (from_counts) {
+    // Few tiles (ntiles <= kBlock): seg holds the raw per-tile kept counts
+    // and each block sums those of the tiles before it (and the last block
+    // all of them) instead of waiting for a scan launch.
+    const int32_t t = tid < ntiles ? seg[tid] : 0;
+    const int32_t before = wave_inclusive_sum(tid < tile ? t : 0);
     const int32_t all = wave_inclusive_sum(t);
     if (lane == 63) {
       s_wsum[wave] = before;
@@ -661,11 +687,12 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restr
       a += s_wcnt[w];
     }
     base = b;
-    if (blockIdx.x == gridDim.x - 1 && tid == 0) out_ptr[m_new] = a;
+    if (tile == ntiles - 1 && tid == 0) out_ptr[m_new] = a;
     __syncthreads();  // s_wsum is reused below
   } else {
-    base = tile_off[blockIdx.x];
-    if (blockIdx.x == gridDim.x - 1 && tid == 0) out_ptr[m_new] = tile_off[gridDim.x];
+    const int32_t s0 = seg[0];
+    base = seg[tile] - s0;
+    if (tile == ntiles - 1 && tid == 0) out_ptr[m_new] = seg[ntiles] - s0;
   }
   {  // row pointers of the tile's kept rows: tile offset + block exclusive scan of the counts
     const int32_t cnt = my_new_row >= 0 ? out_ptr[my_new_row] : 0;
@@ -726,6 +753,33 @@ __global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restr
     base += total;
     __syncthreads();
   }
+}
+
+template <bool kOffsetsFromCounts>
+__global__ __launch_bounds__(kBlock) void k_compact_write(const int32_t* __restrict__ ptr,
+                                                          const double* __restrict__ val, int32_t m_old,
+                                                          const int32_t* __restrict__ row_map,
+                                                          const int32_t* __restrict__ jn_in,
+                                                          const int32_t* __restrict__ tile_off,
+                                                          int32_t* __restrict__ out_ptr, int32_t m_new,
+                                                          int32_t* __restrict__ out_idx,
+                                                          double* __restrict__ out_val) {
+  compact_write_tile(static_cast<int32_t>(blockIdx.x), static_cast<int32_t>(gridDim.x), kOffsetsFromCounts, ptr, val,
+                     m_old, row_map, jn_in, tile_off, out_ptr, m_new, out_idx, out_val);
+}
+
+// Both write passes of a pair map (k_compact_map_pair) in one launch.
+__global__ __launch_bounds__(kBlock) void k_compact_write_pair(gpk_compact_side a, gpk_compact_side t,
+                                                               int32_t tiles_a, int32_t tiles_t,
+                                                               const int32_t* __restrict__ tile_off,
+                                                               int32_t from_counts) {
+  const int32_t b = static_cast<int32_t>(blockIdx.x);
+  if (b < tiles_a)
+    compact_write_tile(b, tiles_a, from_counts != 0, a.ptr, a.val, a.m_old, a.row_map, a.jn, tile_off, a.out_ptr,
+                       a.m_new, a.out_idx, a.out_val);
+  else
+    compact_write_tile(b - tiles_a, tiles_t, from_counts != 0, t.ptr, t.val, t.m_old, t.row_map, t.jn,
+                       tile_off + tiles_a, t.out_ptr, t.m_new, t.out_idx, t.out_val);
 }
 
 __global__ void k_mail(const uint32_t* __restrict__ src, int32_t words, uint32_t* mail, uint32_t seq) {
@@ -1334,6 +1388,27 @@ hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_ol
   if (m_old <= 0) return hipSuccess;
   hipLaunchKernelGGL(k_compact_write<false>, dim3(grid_for(m_old, kCompRows)), dim3(kBlock), 0, s, ptr, val, m_old,
                      row_map, jn, tile_off, out_ptr, m_new, out_idx, out_val);
+  return hipGetLastError();
+}
+
+hipError_t gpk_compact_map_pair(gpk_compact_side a, gpk_compact_side t, int32_t* tile_cnt, int32_t* kept_total,
+                                hipStream_t s) {
+  if (a.m_old < 0 || t.m_old < 0) return hipErrorInvalidValue;
+  const int32_t ta = a.m_old > 0 ? static_cast<int32_t>(grid_for(a.m_old, kCompRows)) : 0;
+  const int32_t tt = t.m_old > 0 ? static_cast<int32_t>(grid_for(t.m_old, kCompRows)) : 0;
+  if (ta + tt == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_compact_map_pair, dim3(ta + tt), dim3(kBlock), 0, s, a, t, ta, tile_cnt, kept_total);
+  return hipGetLastError();
+}
+
+hipError_t gpk_compact_write_pair(gpk_compact_side a, gpk_compact_side t, const int32_t* tile_off, int32_t from_counts,
+                                  hipStream_t s) {
+  if (a.m_old < 0 || t.m_old < 0) return hipErrorInvalidValue;
+  const int32_t ta = a.m_old > 0 ? static_cast<int32_t>(grid_for(a.m_old, kCompRows)) : 0;
+  const int32_t tt = t.m_old > 0 ? static_cast<int32_t>(grid_for(t.m_old, kCompRows)) : 0;
+  if (ta + tt == 0) return hipSuccess;
+  if (from_counts && (ta > kBlock || tt > kBlock)) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_compact_write_pair, dim3(ta + tt), dim3(kBlock), 0, s, a, t, ta, tt, tile_off, from_counts);
   return hipGetLastError();
 }
 

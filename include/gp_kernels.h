@@ -121,6 +121,31 @@ hipError_t gpk_compact_write(const int32_t* ptr, const double* val, int32_t m_ol
 hipError_t gpk_compact_map_total(const int32_t* ptr, const int32_t* idx, int32_t m_old, const int32_t* row_map,
                                  const gpk_rank_word* col_rank, int32_t* jn, int32_t* out_ptr, int32_t* tile_cnt,
                                  int32_t* kept_total, hipStream_t stream);
+/* A CSR and its transpose compacted with the same two maps (A by row_map
+ * and the column rank words, AT by the column map and the row rank words)
+ * in one map launch and one write launch.  tile_cnt holds
+ * gpk_compact_tiles(a.m_old) + gpk_compact_tiles(t.m_old) + 1 words: A's tile
+ * counts, then AT's.  The write pass takes either those raw counts
+ * (from_counts, allowed when each side has at most 256 tiles)
+ * or one exclusive scan of them (gpk_scan_lb_i32 over both sides' tiles).
+ * kept_total (optional, zeroed by the caller) receives A's kept count. */
+typedef struct gpk_compact_side {
+  const int32_t* ptr;
+  const int32_t* idx;
+  const double* val;
+  int32_t m_old;
+  int32_t m_new;
+  const int32_t* row_map;
+  const gpk_rank_word* col_rank;
+  int32_t* jn;
+  int32_t* out_ptr;
+  int32_t* out_idx;
+  double* out_val;
+} gpk_compact_side;
+hipError_t gpk_compact_map_pair(gpk_compact_side a, gpk_compact_side t, int32_t* tile_cnt, int32_t* kept_total,
+                                hipStream_t stream);
+hipError_t gpk_compact_write_pair(gpk_compact_side a, gpk_compact_side t, const int32_t* tile_off,
+                                  int32_t from_counts, hipStream_t stream);
 /* The same without the scan, for at most 256 tiles (m_old <= 65,536): it
  * takes the raw tile counts of gpk_compact_map and derives the offsets. */
 hipError_t gpk_compact_write_counts(const int32_t* ptr, const double* val, int32_t m_old, const int32_t* row_map,
