@@ -101,6 +101,7 @@ def kernel_rooflines(gp_ffi, dlp, lp, launches):
     act["kernels"] = measure_activity_kernels(gp_ffi, dlp, lp, stream, launches, mn.numpy(), mx.numpy())
     comp = measure_compaction(gp_ffi, dlp, lp, stream, launches)
     log(f"compaction: {comp['us']:.1f} us/call, {comp['achieved']:.0f} GB/s")
+    comp["pair"] = measure_compaction_pair(gp_ffi, dlp, lp, stream, launches)
     tr = measure_transpose(gp_ffi, dlp, lp, stream, max(5, launches // 5))
     return act, comp, tr
 
@@ -301,6 +302,105 @@ def measure_compaction(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
             "compulsory_bytes": floor, "compulsory_achieved": floor / (us * 1e-6) / 1e9,
             "workload": f"A of the LP ({m}x{n}, {nnz} nnz) with 15% of rows and columns dropped "
                         f"-> {m_new} rows, {nnz_new} nnz"}
+
+
+def measure_compaction_pair(gp_ffi, dlp, lp, stream, launches, frac=0.15, seed=7):
+    """Times the compaction a presolve plan application runs: A and its
+    transpose by the same maps, with both rank-word arrays built, one map
+    launch over both matrices' tiles (gpk_compact_map_pair), one scan of both
+    sides' tile counts and one write launch (gpk_compact_write_pair), on the
+    full LP with 15% of rows and columns dropped; checks both outputs against
+    the host."""
+    import ctypes as C
+
+    import numpy as np
+
+    class Lookback(C.Structure):
+        _fields_ = [("status", C.c_void_p), ("ticket", C.c_void_p), ("error", C.c_void_p), ("epoch", C.c_uint32)]
+
+    vp = C.c_void_p
+
+    class Side(C.Structure):
+        _fields_ = [("ptr", vp), ("idx", vp), ("val", vp), ("m_old", C.c_int32), ("m_new", C.c_int32),
+                    ("row_map", vp), ("col_rank", vp), ("jn", vp), ("out_ptr", vp), ("out_idx", vp),
+                    ("out_val", vp)]
+
+    m, n = lp["m"], lp["n"]
+    nnz = len(lp["A_idx"])
+    rng = np.random.default_rng(seed)
+    keep_r = rng.random(m) >= frac
+    keep_c = rng.random(n) >= frac
+    row_map = np.where(keep_r, np.cumsum(keep_r) - 1, -1).astype(np.int32)
+    col_map = np.where(keep_c, np.cumsum(keep_c) - 1, -1).astype(np.int32)
+    rows = np.repeat(np.arange(m), np.diff(lp["A_ptr"]))
+    kept = keep_r[rows] & keep_c[lp["A_idx"]]
+    m_new, n_new, nnz_new = int(keep_r.sum()), int(keep_c.sum()), int(kept.sum())
+    L = gp_ffi.lib()
+    L.gpk_build_rank.argtypes = [vp, C.c_int32, vp, vp]
+    L.gpk_compact_map_pair.argtypes = [Side, Side, vp, vp, vp]
+    L.gpk_compact_write_pair.argtypes = [Side, Side, vp, C.c_int32, vp]
+    L.gpk_compact_tiles.restype = C.c_int32
+    L.gpk_compact_tiles.argtypes = [C.c_int32]
+    L.gpk_scan_lb_i32.argtypes = [vp, vp, C.c_int64, Lookback, vp]
+    L.gpk_lookback_tiles.restype = C.c_int64
+    L.gpk_lookback_tiles.argtypes = [C.c_int64]
+    d_rm, d_cm = gp_ffi.DeviceArray(row_map), gp_ffi.DeviceArray(col_map)
+    c_rank = gp_ffi.DeviceArray(size=2 * ((n + 63) // 64), dtype=np.uint64)
+    r_rank = gp_ffi.DeviceArray(size=2 * ((m + 63) // 64), dtype=np.uint64)
+    jn, t_jn = gp_ffi.DeviceArray(size=nnz, dtype=np.int32), gp_ffi.DeviceArray(size=nnz, dtype=np.int32)
+    o_ptr = gp_ffi.DeviceArray(size=m + 1, dtype=np.int32)
+    t_ptr = gp_ffi.DeviceArray(size=n + 1, dtype=np.int32)
+    o_idx, t_idx = gp_ffi.DeviceArray(size=nnz, dtype=np.int32), gp_ffi.DeviceArray(size=nnz, dtype=np.int32)
+    o_val, t_val = gp_ffi.DeviceArray(size=nnz), gp_ffi.DeviceArray(size=nnz)
+    ta, tt = L.gpk_compact_tiles(m), L.gpk_compact_tiles(n)
+    from_counts = ta <= 256 and tt <= 256
+    t_off = gp_ffi.DeviceArray(size=ta + tt + 1, dtype=np.int32)
+    status = gp_ffi.DeviceArray(np.zeros(max(1, L.gpk_lookback_tiles(ta + tt)), np.uint64))
+    tickets = gp_ffi.DeviceArray(np.zeros(4 * launches + 16, np.int32))
+    err = gp_ffi.DeviceArray(np.zeros(1, np.int32))
+    A, AT = dlp.A.g, dlp.AT.g
+    a = Side(A.row_ptr, A.col_idx, A.val, m, m_new, d_rm.ptr, c_rank.ptr, jn.ptr, o_ptr.ptr, o_idx.ptr, o_val.ptr)
+    t = Side(AT.row_ptr, AT.col_idx, AT.val, n, n_new, d_cm.ptr, r_rank.ptr, t_jn.ptr, t_ptr.ptr, t_idx.ptr,
+             t_val.ptr)
+    launch_no = [0]
+    ck = gp_ffi.hip_check
+
+    def whole():
+        ck(L.gpk_build_rank(d_cm.ptr, n, c_rank.ptr, stream.ptr), "build rank")
+        ck(L.gpk_build_rank(d_rm.ptr, m, r_rank.ptr, stream.ptr), "build rank")
+        ck(L.gpk_compact_map_pair(a, t, t_off.ptr, None, stream.ptr), "map pair")
+        if not from_counts:
+            k = launch_no[0]
+            launch_no[0] += 1
+            lb = Lookback(status.ptr, tickets.ptr + 4 * k, err.ptr, k + 1)
+            ck(L.gpk_scan_lb_i32(t_off.ptr, t_off.ptr, ta + tt, lb, stream.ptr), "scan")
+        ck(L.gpk_compact_write_pair(a, t, t_off.ptr, 1 if from_counts else 0, stream.ptr), "write pair")
+
+    for _ in range(3):
+        whole()
+    stream.sync()
+    e0, e1 = gp_ffi.Event(), gp_ffi.Event()
+    e0.record(stream)
+    for _ in range(launches):
+        whole()
+    e1.record(stream)
+    us = 1e3 * e0.elapsed_ms(e1) / launches
+    want_ptr = np.concatenate([[0], np.cumsum(np.bincount(rows[kept], minlength=m)[keep_r])]).astype(np.int32)
+    at_rows = np.repeat(np.arange(n), np.diff(lp["AT_ptr"]))
+    t_kept = keep_c[at_rows] & keep_r[lp["AT_idx"]]
+    want_tptr = np.concatenate([[0], np.cumsum(np.bincount(at_rows[t_kept], minlength=n)[keep_c])]).astype(np.int32)
+    if (err.numpy()[0] != 0 or not np.array_equal(o_ptr.numpy()[:m_new + 1], want_ptr)
+            or not np.array_equal(o_idx.numpy()[:nnz_new], col_map[lp["A_idx"][kept]])
+            or not np.array_equal(o_val.numpy()[:nnz_new], lp["A_val"][kept])
+            or not np.array_equal(t_ptr.numpy()[:n_new + 1], want_tptr)
+            or not np.array_equal(t_idx.numpy()[:nnz_new], row_map[lp["AT_idx"][t_kept]])
+            or not np.array_equal(t_val.numpy()[:nnz_new], lp["AT_val"][t_kept])):
+        raise SystemExit("bench.py: the paired compaction differs from the host")
+    nbytes = compaction_bytes(m, n, nnz, m_new, nnz_new) + compaction_bytes(n, m, nnz, n_new, nnz_new)
+    log(f"paired compaction (A and AT): {us:.1f} us/call, {nbytes / (us * 1e-6) / 1e9:.0f} GB/s")
+    return {"us": us, "bytes": nbytes, "achieved": nbytes / (us * 1e-6) / 1e9,
+            "workload": f"A and AT of the LP ({m}x{n}, {nnz} nnz) with 15% of rows and columns dropped "
+                        f"-> {m_new}x{n_new}, {nnz_new} nnz"}
 
 
 def transpose_bytes(m, n, nnz):
@@ -1149,6 +1249,14 @@ def main():
                      "compulsory_bytes": comp["compulsory_bytes"],
                      "compulsory_frac": comp["compulsory_achieved"] / HBM_PEAK_GBS,
                      "workload": comp["workload"], "kernels_us": comp["kernels_us"]},
+        "roofline_compaction_pair": {"bound": "hbm",
+                                     "kernel": "paired compaction of A and AT: 2 x k_build_rank + "
+                                               "k_compact_map_pair + tile-offset scan + k_compact_write_pair",
+                                     "achieved": comp["pair"]["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": comp["pair"]["achieved"] / HBM_PEAK_GBS, "traffic": None,
+                                     "us_per_call": comp["pair"]["us"],
+                                     "algorithmic_bytes": comp["pair"]["bytes"],
+                                     "workload": comp["pair"]["workload"]},
         "roofline_activity": {"bound": "hbm", "kernel": "k_pack_lu + k_row_activity_lu (one gp_row_activity call)",
                               "achieved": act["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                               "frac": act["achieved"] / HBM_PEAK_GBS,
